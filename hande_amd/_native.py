@@ -1,0 +1,128 @@
+"""ctypes view of the engine and host C-ABI (include/fqmc.h, include/fqmc_host.h).
+
+The shared library ``hande_amd/libfqmc.so`` is built in-tree by
+``hande_amd/build.py`` (hipcc, gfx950).  Loading fails loudly when it is
+missing: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfqmc.so")
+
+FQ_OK = 0
+FQ_ERR = {
+    -1: "FQ_ERR_ARG", -2: "FQ_ERR_NOMEM", -3: "FQ_ERR_DEVICE", -4: "FQ_ERR_RNG_EXHAUSTED",
+    -5: "FQ_ERR_CAPACITY", -6: "FQ_ERR_NO_DEVICE",
+}
+FQ_SYS_HUB_K, FQ_SYS_READ_IN = 1, 2
+FQ_EXCIT_RENORM, FQ_EXCIT_NO_RENORM = 0, 1
+FQ_RNG_PHILOX, FQ_RNG_REPLAY = 0, 1
+FQH_SYM_AUFBAU, FQH_SYM_TOT, FQH_SYM_INDEX = 0, 1, 2
+FQ_MAX_SPACES = 2
+FQ_MAX_WORDS = 2
+
+i32, i64, u64, f64 = C.c_int32, C.c_int64, C.c_uint64, C.c_double
+P = C.POINTER
+
+
+class FqSystem(C.Structure):
+    _fields_ = [
+        ("system", i32), ("nbasis", i32), ("nel", i32), ("nalpha", i32), ("nbeta", i32),
+        ("nvirt", i32), ("nvirt_alpha", i32), ("nvirt_beta", i32), ("nwords", i32),
+        ("f0", P(u64)), ("H00", f64), ("sp_eigv", P(f64)),
+        ("nsym", i32), ("sym_table", P(i32)), ("inv_sym", P(i32)), ("coulomb_k", f64),
+        ("norb", i32), ("nsym_pg", i32), ("orb_sym", P(i32)), ("nbasis_sym_spin", P(i32)),
+        ("max_sym_spin", i32), ("sym_spin_basis_fns", P(i32)), ("ecore", f64),
+        ("h1", P(f64)), ("eri", P(f64)), ("n_eri", i64),
+        ("pattempt_single", f64), ("pattempt_double", f64),
+    ]
+
+
+class FqParams(C.Structure):
+    _fields_ = [
+        ("nspaces", i32), ("initiator", i32), ("initiator_pop", f64), ("quadrature_initiator", i32),
+        ("real_amplitudes", i32), ("pop_bit_shift", i32), ("spawn_cutoff", i64), ("excit_gen", i32),
+        ("rng_mode", i32), ("philox_seed", u64), ("main_capacity", i64), ("spawn_capacity", i64),
+        ("device", i32),
+    ]
+
+
+class FqCycleOut(C.Structure):
+    _fields_ = [
+        ("proj_energy", f64 * FQ_MAX_SPACES), ("D0_population", f64 * FQ_MAX_SPACES),
+        ("nparticles", f64 * FQ_MAX_SPACES), ("nstates", i64), ("nspawn_events", i64),
+        ("ndeath", i64), ("nattempts", i64), ("rng_consumed", i64),
+        ("spawn_overflow", i32), ("main_overflow", i32),
+    ]
+
+
+class FqhQmcIn(C.Structure):
+    _fields_ = [
+        ("tau", f64), ("rng_seed", i64), ("init_pop", f64), ("mc_cycles", i32), ("nreports", i32),
+        ("target_population", f64), ("shift_damping", f64), ("initial_shift", f64),
+        ("vary_shift_from", f64), ("initiator", i32), ("initiator_threshold", f64),
+        ("real_amplitudes", i32), ("real_amplitude_force_32", i32), ("spawn_cutoff", f64),
+        ("excit_gen", i32), ("state_size", i64), ("spawned_state_size", i64),
+        ("replica_tricks", i32), ("quadrature_initiator", i32), ("rng_mode", i32), ("device", i32),
+    ]
+
+
+class FqhReportRow(C.Structure):
+    _fields_ = [
+        ("iterations", i64), ("shift", f64 * FQ_MAX_SPACES), ("proj_energy", f64 * FQ_MAX_SPACES),
+        ("D0_population", f64 * FQ_MAX_SPACES), ("nparticles", f64 * FQ_MAX_SPACES),
+        ("nstates", i64), ("nspawn_events", i64), ("rspawn", f64), ("time_per_cycle", f64),
+    ]
+
+
+def _declare(lib):
+    vp = C.c_void_p
+    sig = {
+        "fqh_hubbard_k": (C.c_int, [C.c_int, C.c_int, C.c_int, P(C.c_int), f64, f64, C.c_int, C.c_int, P(vp)]),
+        "fqh_read_in": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]),
+        "fqh_view": (P(FqSystem), [vp]),
+        "fqh_ref_occ": (C.c_int, [vp, P(C.c_int)]),
+        "fqh_basis_info": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(f64)]),
+        "fqh_diagonal": (f64, [vp, P(u64)]),
+        "fqh_free": (None, [vp]),
+        "fqh_fciqmc": (C.c_int, [vp, P(FqhQmcIn), P(FqhReportRow), C.c_int, P(C.c_int)]),
+        "fq_create": (C.c_int, [P(FqSystem), P(FqParams), P(vp)]),
+        "fq_load": (C.c_int, [vp, P(u64), P(i64), P(f64), i64]),
+        "fq_push_rng": (C.c_int, [vp, P(f64), i64]),
+        "fq_rng_available": (C.c_int, [vp, P(i64)]),
+        "fq_cycle": (C.c_int, [vp, P(f64), P(f64), f64, P(FqCycleOut)]),
+        "fq_annihilate": (C.c_int, [vp, P(u64), P(i64), P(i64), i64, P(FqCycleOut)]),
+        "fq_nstates": (C.c_int, [vp, P(i64)]),
+        "fq_download": (C.c_int, [vp, P(u64), P(i64), P(f64), i64, P(i64)]),
+        "fq_last_timing": (C.c_int, [vp, P(f64), P(f64), P(f64)]),
+        "fq_destroy": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        if hasattr(lib, name):
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return lib
+
+
+_LIB = None
+
+
+def lib(path=None):
+    """Load the engine library (built in-tree); raise if it is missing."""
+    global _LIB
+    if path is not None:
+        return _declare(C.CDLL(path))
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                "(hipcc, gfx950). There is no CPU fallback.")
+        _LIB = _declare(C.CDLL(LIB_PATH))
+    return _LIB
+
+
+def check(rc, what):
+    if rc != FQ_OK:
+        raise RuntimeError(f"{what} failed: {FQ_ERR.get(rc, rc)}")

@@ -1,0 +1,176 @@
+"""Python mirror of the reference's FCIQMC input surface.
+
+``hubbard_k{}``, ``read_in{}`` and ``fciqmc{}`` take the same keys and
+defaults as the reference's Lua functions (src/lua_hande_systems.f90:300-380,
+560-620; src/lua_hande_calc.f90:1244-1503; defaults src/qmc_data.f90:121-303)
+and run on the MI355X engine through the C-ABI in include/fqmc_host.h.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _native as N
+from .lua_input import LuaCall, parse
+
+_EXCIT = {"renorm": N.FQ_EXCIT_RENORM, "no_renorm": N.FQ_EXCIT_NO_RENORM}
+_RNG = {"philox": N.FQ_RNG_PHILOX, "replay": N.FQ_RNG_REPLAY}
+
+
+class System:
+    """A system set up on the host; owns the tables the engine uploads."""
+
+    def __init__(self, handle, lib):
+        self._h = handle
+        self._lib = lib
+        self.view = lib.fqh_view(handle).contents
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._lib is not None:
+            self._lib.fqh_free(self._h)
+            self._h = None
+
+    @property
+    def nbasis(self):
+        return self.view.nbasis
+
+    @property
+    def nel(self):
+        return self.view.nel
+
+    @property
+    def H00(self):
+        return self.view.H00
+
+    def reference_occ(self):
+        occ = (C.c_int * self.view.nel)()
+        N.check(self._lib.fqh_ref_occ(self._h, occ), "fqh_ref_occ")
+        return list(occ)
+
+    def f0(self):
+        return np.array([self.view.f0[i] for i in range(self.view.nwords)], dtype=np.uint64)
+
+    def basis(self):
+        n = self.view.nbasis
+        ms = (C.c_int * n)()
+        lab = (C.c_int * n)()
+        eps = (C.c_double * n)()
+        N.check(self._lib.fqh_basis_info(self._h, ms, lab, eps), "fqh_basis_info")
+        return list(ms), list(lab), list(eps)
+
+    def diagonal(self, f):
+        arr = np.ascontiguousarray(f, dtype=np.uint64)
+        return self._lib.fqh_diagonal(self._h, arr.ctypes.data_as(C.POINTER(C.c_uint64)))
+
+
+def hubbard_k(electrons=None, nel=None, lattice=None, ms=0, sym=None, U=1.0, t=1.0, lib=None, **_):
+    """hubbard_k{...}: Hubbard model in a momentum-space basis."""
+    lib = lib or N.lib()
+    nel = electrons if electrons is not None else nel
+    if nel is None or lattice is None:
+        raise ValueError("hubbard_k needs electrons/nel and lattice")
+    ndim = len(lattice)
+    flat = [int(x) for vec in lattice for x in vec]
+    mode, index = N.FQH_SYM_AUFBAU, 0
+    if isinstance(sym, str):
+        mode = N.FQH_SYM_TOT if sym == "tot_sym" else N.FQH_SYM_AUFBAU
+    elif sym is not None:
+        mode, index = N.FQH_SYM_INDEX, int(sym)
+    h = C.c_void_p()
+    arr = (C.c_int * len(flat))(*flat)
+    N.check(lib.fqh_hubbard_k(int(nel), int(ms), ndim, arr, float(U), float(t), mode, index, C.byref(h)),
+            "hubbard_k")
+    return System(h, lib)
+
+
+def read_in(int_file=None, nel=None, electrons=None, ms=0, sym=None, CAS=None, base_dir=".", lib=None, **_):
+    """read_in{...}: molecular Hamiltonian from an FCIDUMP-format integral file (RHF)."""
+    lib = lib or N.lib()
+    nel = electrons if electrons is not None else nel
+    path = int_file if os.path.isabs(int_file) else os.path.join(base_dir, int_file)
+    cas = CAS or (-1, -1)
+    symv = -1 if (sym is None or isinstance(sym, str)) else int(sym)
+    h = C.c_void_p()
+    N.check(lib.fqh_read_in(path.encode(), int(nel or 0), int(ms), symv, int(cas[0]), int(cas[1]), C.byref(h)),
+            "read_in")
+    return System(h, lib)
+
+
+def qmc_in(qmc, fciqmc_opts=None, rng_mode="replay", device=0):
+    """Build the qmc/fciqmc option block with the reference's defaults."""
+    q = dict(qmc)
+    f = dict(fciqmc_opts or {})
+    st = N.FqhQmcIn()
+    st.tau = float(q["tau"])
+    st.rng_seed = int(q.get("rng_seed", 7))
+    st.init_pop = float(q.get("init_pop", 10))
+    st.mc_cycles = int(q.get("mc_cycles", 1))
+    st.nreports = int(q.get("nreports", 1))
+    st.target_population = float(q.get("target_population", 1.0e300))
+    st.shift_damping = float(q.get("shift_damping", 0.05))
+    st.initial_shift = float(q.get("initial_shift", 0.0))
+    # vary_shift_from defaults to initial_shift (src/lua_hande_calc.f90:1483-1491)
+    st.vary_shift_from = float(q.get("vary_shift_from", st.initial_shift))
+    st.initiator = int(bool(q.get("initiator", False)))
+    st.initiator_threshold = float(q.get("initiator_threshold", 3.0))
+    st.real_amplitudes = int(bool(q.get("real_amplitudes", False)))
+    st.real_amplitude_force_32 = int(bool(q.get("real_amplitude_force_32", False)))
+    st.spawn_cutoff = float(q.get("spawn_cutoff", 0.01))
+    st.excit_gen = _EXCIT[q.get("excit_gen", "renorm")]
+    st.state_size = int(q.get("state_size", -1000))
+    st.spawned_state_size = int(q.get("spawned_state_size", -500))
+    st.replica_tricks = int(bool(f.get("replica_tricks", False)))
+    st.quadrature_initiator = int(bool(f.get("quadrature_initiator", True)))
+    st.rng_mode = _RNG[rng_mode]
+    st.device = int(device)
+    return st
+
+
+REPORT_COLUMNS = ("iterations", "Shift", "\\sum H_0j N_j", "N_0", "# H psips", "# states",
+                  "# spawn_events", "R_spawn")
+
+
+def rows_to_table(rows, nrows, nspaces=1):
+    """Report rows as a float array with the reference's column order."""
+    out = []
+    for r in rows[:nrows]:
+        line = [float(r.iterations)]
+        for s in range(nspaces):
+            line += [r.shift[s], r.proj_energy[s], r.D0_population[s], r.nparticles[s]]
+        line += [float(r.nstates), float(r.nspawn_events), r.rspawn]
+        out.append(line)
+    return np.array(out)
+
+
+def fciqmc(sys, qmc, fciqmc=None, rng_mode="replay", device=0, lib=None, **_):
+    """fciqmc{...}: run FCIQMC on the GPU engine and return the report table."""
+    lib = lib or sys._lib
+    st = qmc_in(qmc, fciqmc, rng_mode=rng_mode, device=device)
+    max_rows = st.nreports + 1
+    rows = (N.FqhReportRow * max_rows)()
+    nrows = C.c_int()
+    N.check(lib.fqh_fciqmc(sys.handle, C.byref(st), rows, max_rows, C.byref(nrows)), "fciqmc")
+    return rows_to_table(rows, nrows.value, 2 if st.replica_tricks else 1)
+
+
+def load_deck(text, base_dir=".", lib=None):
+    """Parse a deck; return (system, fciqmc options) for its fciqmc{} call."""
+    env, calls = parse(text)
+    lib = lib or N.lib()
+    call = next((c for c in calls if isinstance(c, LuaCall) and c.func == "fciqmc"), None)
+    if call is None:
+        raise ValueError("deck has no fciqmc{} call")
+    sysdef = call.args["sys"]
+    if not isinstance(sysdef, LuaCall):
+        raise ValueError("fciqmc{} needs sys = hubbard_k{...} or read_in{...}")
+    if sysdef.func == "hubbard_k":
+        system = hubbard_k(lib=lib, **sysdef.args)
+    elif sysdef.func == "read_in":
+        system = read_in(base_dir=base_dir, lib=lib, **sysdef.args)
+    else:
+        raise ValueError(f"system {sysdef.func} is outside the FCIQMC hot path")
+    return system, call.args

@@ -1,0 +1,161 @@
+/*
+ * fqmc.h -- C-ABI of the MI355X FCIQMC walker-propagation engine.
+ *
+ * The engine replaces the body of the reference's Monte Carlo cycle
+ * (src/fciqmc.f90:307-399): the loop over occupied determinants (decode,
+ * projected-energy accumulation, initiator flag, decide_nattempts, spawning,
+ * stochastic death) followed by direct_annihilation (sort, self-annihilation,
+ * merge with the main list, rounding of small real populations, insertion of
+ * new determinants).  The host keeps the calc/system setup, the report loop
+ * and the shift update (src/energy_evaluation.F90:126-711) and talks to the
+ * engine only through the functions below: plain C types, no exceptions, an
+ * int status code on every call.
+ *
+ * Determinant layout at the boundary follows the reference's particle_t
+ * (src/qmc_data.f90:615-682): states(nwords, n) column-major 64-bit words
+ * (DET_SIZE=64), pops(nspaces, n) int64 fixed point (value = pop / 2^shift,
+ * src/particle_t_utils.f90:55-82), dat(1, n) = <D|H|D> - H00.  Orbital i
+ * (1-based) lives in word (i-1)/64, bit (i-1)%64 (src/basis_types.f90:152-169);
+ * odd orbitals are alpha, even are beta.
+ */
+#ifndef FQMC_H
+#define FQMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FQ_OK 0
+#define FQ_ERR_ARG (-1)            /* invalid argument / unsupported combination */
+#define FQ_ERR_NOMEM (-2)          /* device allocation failed */
+#define FQ_ERR_DEVICE (-3)         /* HIP runtime error */
+#define FQ_ERR_RNG_EXHAUSTED (-4)  /* replay stream ran out of draws mid-cycle */
+#define FQ_ERR_CAPACITY (-5)       /* initial walker list larger than the main list */
+#define FQ_ERR_NO_DEVICE (-6)      /* no usable GPU */
+
+/* ---- enumerations ------------------------------------------------------ */
+enum fq_system_kind { FQ_SYS_HUB_K = 1, FQ_SYS_READ_IN = 2 };
+enum fq_excit_gen { FQ_EXCIT_RENORM = 0, FQ_EXCIT_NO_RENORM = 1 };
+enum fq_rng_mode {
+  FQ_RNG_PHILOX = 0, /* production: counter-based Philox4x32-10 per attempt */
+  FQ_RNG_REPLAY = 1  /* correctness: consume a host-supplied dSFMT stream in reference order */
+};
+
+#define FQ_MAX_SPACES 2
+#define FQ_MAX_WORDS 2
+
+/* Read-only system tables (host-computed once, uploaded by fq_create).
+ * Hubbard-k fields follow src/momentum_symmetry.f90:17-71 and
+ * src/system.f90:578; molecular fields follow src/molecular_integrals.F90
+ * (RHF store, 8-fold packed two-body index, :736-845) and
+ * src/pg_symmetry.f90 (XOR products of abelian irreps). */
+typedef struct fq_system {
+  int32_t system;      /* enum fq_system_kind */
+  int32_t nbasis;      /* spin orbitals */
+  int32_t nel, nalpha, nbeta;
+  int32_t nvirt, nvirt_alpha, nvirt_beta;
+  int32_t nwords;      /* 64-bit words per determinant, ceil(nbasis/64) */
+  const uint64_t *f0;  /* reference determinant, nwords words */
+  double H00;          /* <D0|H|D0> */
+  const double *sp_eigv; /* nbasis single-particle energies */
+  /* Hubbard model in momentum space */
+  int32_t nsym;              /* number of wavevectors = nbasis/2 */
+  const int32_t *sym_table;  /* nsym*nsym, entry [(i-1)*nsym+(j-1)] = label of k_i+k_j (1-based) */
+  const int32_t *inv_sym;    /* nsym, label of -k_i (1-based) */
+  double coulomb_k;          /* U / nsites */
+  /* molecular Hamiltonian from an integral dump (RHF) */
+  int32_t norb;                    /* spatial orbitals, nbasis/2 */
+  int32_t nsym_pg;                 /* irreps 0..nsym_pg-1, products by XOR */
+  const int32_t *orb_sym;          /* nbasis: irrep of each spin orbital */
+  const int32_t *nbasis_sym_spin;  /* 2*nsym_pg: [ims*nsym_pg + sym], ims 0 = beta, 1 = alpha */
+  int32_t max_sym_spin;            /* max(nbasis_sym_spin) */
+  const int32_t *sym_spin_basis_fns; /* 2*nsym_pg*max_sym_spin: [(ims*nsym_pg+sym)*max_sym_spin + k] = orbital (1-based) */
+  double ecore;
+  const double *h1;                /* norb*norb one-body integrals (spatial, row-major) */
+  const double *eri;               /* packed (ij|kl), index tri(tri(i,j),tri(k,l)) over spatial orbitals */
+  int64_t n_eri;
+  double pattempt_single, pattempt_double;
+} fq_system;
+
+/* Per-calculation parameters (src/qmc_data.f90:121-303 defaults). */
+typedef struct fq_params {
+  int32_t nspaces;              /* 1, or 2 for a replica pair */
+  int32_t initiator;            /* initiator approximation (src/ifciqmc.f90) */
+  double initiator_pop;         /* initiator threshold, decoded units (default 3.0) */
+  int32_t quadrature_initiator; /* combine pairs of spaces (default true) */
+  int32_t real_amplitudes;
+  int32_t pop_bit_shift;        /* pop_real_factor = 2^shift: 0, 11 (real_32) or 31 (real_64) */
+  int64_t spawn_cutoff;         /* encoded: ceiling(cutoff*real_factor); 0 for integer runs */
+  int32_t excit_gen;            /* enum fq_excit_gen */
+  int32_t rng_mode;             /* enum fq_rng_mode */
+  uint64_t philox_seed;         /* production-mode key */
+  int64_t main_capacity;        /* main walker list slots (state_size) */
+  int64_t spawn_capacity;       /* spawned list slots (spawned_state_size) */
+  int32_t device;               /* HIP device ordinal */
+} fq_params;
+
+/* Per-cycle results, the quantities the reference accumulates per cycle. */
+typedef struct fq_cycle_out {
+  double proj_energy[FQ_MAX_SPACES];   /* sum_j H_0j N_j over dets at cycle start */
+  double D0_population[FQ_MAX_SPACES]; /* reference population at cycle start */
+  double nparticles[FQ_MAX_SPACES];    /* sum |N_i| after annihilation (decoded) */
+  int64_t nstates;                     /* occupied determinants after annihilation */
+  int64_t nspawn_events;               /* spawned entries created this cycle */
+  int64_t ndeath;                      /* sum |kill| (encoded units) */
+  int64_t nattempts;                   /* spawning attempts made this cycle */
+  int64_t rng_consumed;                /* replay mode: draws taken from the stream */
+  int32_t spawn_overflow;              /* spawned list full (spawn%error) */
+  int32_t main_overflow;               /* main list full (psip_list%error) */
+} fq_cycle_out;
+
+typedef struct fq_engine fq_engine;
+
+/* Create an engine on params->device and upload the system tables.
+ * Replaces the allocation in init_psip_list/init_spawn_store
+ * (src/qmc.F90:773,1418) and the table set-up read by the procedure pointers
+ * bound in init_proc_pointers (src/qmc.F90:217-703). */
+int fq_create(const fq_system *sys, const fq_params *params, fq_engine **out);
+
+/* Load a sorted walker list (initial_distribution, src/qmc.F90:1507, or a
+ * restart).  states: nwords*n, pops: nspaces*n, dat: n (K_ii). */
+int fq_load(fq_engine *e, const uint64_t *states, const int64_t *pops, const double *dat, int64_t n);
+
+/* Append draws to the device replay stream (FQ_RNG_REPLAY only).  The host
+ * generates them with dSFMT exactly as get_rand_close_open hands them out
+ * (lib/dSFMT_F03_interface/dSFMT_interface.F90:387-409). */
+int fq_push_rng(fq_engine *e, const double *draws, int64_t n);
+/* Draws pushed but not yet consumed. */
+int fq_rng_available(fq_engine *e, int64_t *n);
+
+/* One Monte Carlo cycle (src/fciqmc.f90:307-399): spawning and death over
+ * the main list, then direct_annihilation (src/annihilation.f90:9).
+ * shift / eproj_old: per space (eproj_old = proj_energy_old used by death,
+ * src/fciqmc.f90:278,370).  In replay mode, returns FQ_ERR_RNG_EXHAUSTED if
+ * the stream ran dry; the engine state is then undefined. */
+int fq_cycle(fq_engine *e, const double *shift, const double *eproj_old, double tau, fq_cycle_out *out);
+
+/* Annihilate a caller-supplied spawned list into the main list
+ * (direct_annihilation without the spawning loop; src/annihilation.f90:9-214
+ * and src/spawn_data.F90:359-1101).  sstates: nwords*ns, spops: nspaces*ns,
+ * sflags: ns initiator flag words (ignored without the initiator approximation).
+ * Replay mode draws the real-amplitude rounding numbers from the stream. */
+int fq_annihilate(fq_engine *e, const uint64_t *sstates, const int64_t *spops, const int64_t *sflags,
+                  int64_t ns, fq_cycle_out *out);
+
+/* Current walker count and a copy of the main list to the host (report /
+ * restart boundaries only).  Arrays must hold capacity entries. */
+int fq_nstates(fq_engine *e, int64_t *n);
+int fq_download(fq_engine *e, uint64_t *states, int64_t *pops, double *dat, int64_t capacity, int64_t *n);
+
+/* Device-side timing of the last cycle's kernels (milliseconds). */
+int fq_last_timing(fq_engine *e, double *spawn_ms, double *annihilate_ms, double *total_ms);
+
+void fq_destroy(fq_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FQMC_H */

@@ -1,0 +1,1202 @@
+// CPU ORACLE -- test infrastructure only.
+//
+// A serial restatement of the reference's FCIQMC evolve loop, used by tests/
+// (and the CPU-baseline leg of bench.py) as the checker for the GPU engine.
+// Nothing in the product links or calls this file.
+//
+// It follows, statement by statement and in the same floating-point order:
+//   src/fciqmc.f90:276-464,635-769  report loop, MC cycle, spawning attempts
+//   src/qmc_common.F90:379-406,950-1017,1240-1304  decide_nattempts,
+//       init_mc_cycle, end_mc_cycle, spawning_rate
+//   src/ifciqmc.f90:14-57  set_parent_flag
+//   src/spawning.F90:215-315,680-766,907-1182  split/standard spawners,
+//       set_child_sign, attempt_to_spawn, add_(flagged_)spawned_particle
+//   src/stoch_utils.f90:51-136  stochastic rounding
+//   src/death.f90:11-130  stochastic_death
+//   src/spawn_data.F90:359-418,859-1101  qsort + (initiator) self-annihilation
+//   src/sort.f90:213-396  qsort_int_64_list (exact comparison/swap sequence)
+//   src/search.f90:47-135  binary_search
+//   src/annihilation.f90:214-486,537-675,677-901  merge with the main list,
+//       remove_unoccupied_dets, round_low_population_spawns, insert_new_walkers
+//   src/excitations.F90:75-200,247-406  get_excitation, permutations, create_excited_det
+//   src/excit_gen_hub_k.f90:29-159,469-770  Hubbard-k generators and pgen
+//   src/hamiltonian_hub_k.f90:91-232, src/hubbard_k.f90  Hubbard-k matrix elements
+//   src/excit_gen_mol.f90:16-101,384-946,1140-1327  molecular renorm generator
+//   src/hamiltonian_molecular.f90:73-346  molecular Slater--Condon rules
+//   src/energy_evaluation.F90:425-711,795-986,1433-1454  estimators, shift
+//   lib/dSFMT_F03_interface/dSFMT_interface.F90:387-409  RNG stream
+// The dSFMT stream generator itself is hande_amd/csrc/host/dsfmt.hpp, pinned
+// against the reference's C source in tests/golden.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "dsfmt.hpp"
+#include "fqmc.h"
+#include "fqmc_host.h"
+
+namespace {
+
+constexpr double kDepsilon = 1.e-12;
+
+inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+inline int64_t fsign64(int64_t a, int64_t b) {  // Fortran SIGN(a,b) for integers
+  int64_t m = a < 0 ? -a : a;
+  return b >= 0 ? m : -m;
+}
+inline int fmod2(int x) { return x % 2; }        // Fortran MOD (truncating)
+inline int fmodulo2(int x) { return ((x % 2) + 2) % 2; }  // Fortran MODULO
+
+struct Excit {
+  int nexcit = 0;
+  int from[2] = {0, 0};
+  int to[2] = {0, 0};
+  bool perm = false;
+};
+
+struct Det {
+  uint64_t f[FQ_MAX_WORDS];
+};
+
+struct Spawned {
+  uint64_t f[FQ_MAX_WORDS];
+  int64_t pop[FQ_MAX_SPACES];
+  int64_t flag;
+};
+
+class Oracle {
+ public:
+  Oracle(const fq_system* s, const fq_params* p, uint32_t seed) : rng_(seed) {
+    sys_ = *s;
+    par_ = *p;
+    W = s->nwords;
+    S = p->nspaces;
+    nbasis = s->nbasis;
+    nel = s->nel;
+    rf = int64_t{1} << p->pop_bit_shift;
+    f0.assign(s->f0, s->f0 + W);
+    sp_eigv.assign(s->sp_eigv, s->sp_eigv + nbasis);
+    if (s->system == FQ_SYS_HUB_K) {
+      sym_table.assign(s->sym_table, s->sym_table + s->nsym * s->nsym);
+      inv_sym.assign(s->inv_sym, s->inv_sym + s->nsym);
+    } else {
+      nsym = s->nsym_pg;
+      orb_sym.assign(s->orb_sym, s->orb_sym + nbasis);
+      nbss.assign(s->nbasis_sym_spin, s->nbasis_sym_spin + 2 * nsym);
+      ssbf.assign(s->sym_spin_basis_fns, s->sym_spin_basis_fns + 2 * nsym * s->max_sym_spin);
+      h1.assign(s->h1, s->h1 + s->norb * s->norb);
+      eri.assign(s->eri, s->eri + s->n_eri);
+    }
+    // excit_mask(:, orb) = bits of all orbitals after orb (src/excitations.F90:17-45).
+    excit_mask.assign(static_cast<size_t>(nbasis + 1) * W, 0);
+    for (int i = 1; i <= nbasis; ++i)
+      for (int j = i + 1; j <= nbasis; ++j) excit_mask[i * W + (j - 1) / 64] |= uint64_t{1} << ((j - 1) % 64);
+  }
+
+  // ---- state ----------------------------------------------------------------
+  std::vector<Det> states;
+  std::vector<int64_t> pops;  // S per det
+  std::vector<double> dat;    // K_ii
+  double nparticles[FQ_MAX_SPACES] = {0, 0};
+  bool main_error = false, spawn_error = false;
+
+  void load(const uint64_t* st, const int64_t* p, const double* d, int64_t n) {
+    states.resize(n);
+    pops.assign(p, p + n * S);
+    dat.assign(d, d + n);
+    for (int64_t i = 0; i < n; ++i)
+      for (int w = 0; w < W; ++w) states[i].f[w] = st[i * W + w];
+    for (int s = 0; s < S; ++s) {
+      // src/qmc.F90:864 nparticles = sum(abs(real(pops)/real_factor))
+      double acc = 0.0;
+      for (int64_t i = 0; i < n; ++i) acc = acc + std::fabs(static_cast<double>(pops[i * S + s]) / rf);
+      nparticles[s] = acc;
+    }
+  }
+
+  uint64_t rng_consumed() const { return rng_.consumed(); }
+
+  void download(uint64_t* st, int64_t* p, double* d) const {
+    for (size_t i = 0; i < states.size(); ++i) {
+      for (int w = 0; w < W; ++w) st[i * W + w] = states[i].f[w];
+      for (int s = 0; s < S; ++s) p[i * S + s] = pops[i * S + s];
+      d[i] = dat[i];
+    }
+  }
+
+  // ---- estimators over the current list (initial_ci_projected_energy) ------
+  void projected_energy(double* proj, double* d0) {
+    for (int s = 0; s < S; ++s) proj[s] = d0[s] = 0.0;
+    for (size_t i = 0; i < states.size(); ++i) {
+      std::vector<int> occ = decode_occ(states[i].f);
+      for (int s = 0; s < S; ++s) {
+        double pop = static_cast<double>(pops[i * S + s]) / rf;
+        Excit c = get_excitation(states[i].f, f0.data());
+        update_proj_energy(states[i].f, occ, pop, c, proj[s], d0[s]);
+      }
+    }
+  }
+
+  // ---- one MC cycle (src/fciqmc.f90:307-399) ---------------------------------
+  void cycle(const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
+    std::memset(out, 0, sizeof(*out));
+    uint64_t rng0 = rng_.consumed();
+    spawned.clear();
+    int64_t ndeath = 0;
+    int64_t attempts = 0;
+    for (size_t idet = 0; idet < states.size(); ++idet) {
+      Decoded dd = decode_full(states[idet].f);
+      double real_pop[FQ_MAX_SPACES];
+      for (int s = 0; s < S; ++s) real_pop[s] = static_cast<double>(pops[idet * S + s]) / rf;
+      int64_t flag = parent_flag(real_pop);
+      for (int s = 0; s < S; ++s) {
+        Excit c = get_excitation(states[idet].f, f0.data());
+        update_proj_energy(states[idet].f, dd.occ, real_pop[s], c, out->proj_energy[s], out->D0_population[s]);
+        int nat = decide_nattempts(real_pop[s]);
+        attempts += nat;
+        const int64_t parent = pops[idet * S + s];
+        for (int ip = 0; ip < nat; ++ip) {
+          Excit conn;
+          int64_t nspawn = spawn(dd, states[idet].f, parent, tau, conn);
+          if (nspawn != 0) create_spawned(states[idet].f, conn, nspawn, s, flag);
+        }
+        death(tau, dat[idet], shift[s], eproj_old[s], pops[idet * S + s], nparticles[s], ndeath);
+      }
+    }
+    out->nspawn_events = static_cast<int64_t>(spawned.size());
+    annihilate();
+    for (int s = 0; s < S; ++s) out->nparticles[s] = nparticles[s];
+    out->nstates = static_cast<int64_t>(states.size());
+    out->ndeath = ndeath;
+    out->nattempts = attempts;
+    out->rng_consumed = static_cast<int64_t>(rng_.consumed() - rng0);
+    out->spawn_overflow = spawn_error;
+    out->main_overflow = main_error;
+  }
+
+  // Annihilate an externally supplied spawned list (entries in the given order).
+  void annihilate_external(const uint64_t* sst, const int64_t* sp, const int64_t* sf, int64_t ns,
+                           fq_cycle_out* out) {
+    std::memset(out, 0, sizeof(*out));
+    uint64_t rng0 = rng_.consumed();
+    spawned.resize(ns);
+    for (int64_t i = 0; i < ns; ++i) {
+      std::memset(&spawned[i], 0, sizeof(Spawned));
+      for (int w = 0; w < W; ++w) spawned[i].f[w] = sst[i * W + w];
+      for (int s = 0; s < S; ++s) spawned[i].pop[s] = sp[i * S + s];
+      spawned[i].flag = sf ? sf[i] : 0;
+    }
+    out->nspawn_events = ns;
+    annihilate();
+    for (int s = 0; s < S; ++s) out->nparticles[s] = nparticles[s];
+    out->nstates = static_cast<int64_t>(states.size());
+    out->rng_consumed = static_cast<int64_t>(rng_.consumed() - rng0);
+  }
+
+ private:
+  fq_system sys_;
+  fq_params par_;
+  int W, S, nbasis, nel, nsym = 1;
+  int64_t rf;
+  fqmc::Dsfmt19937 rng_;
+  std::vector<uint64_t> f0;
+  std::vector<double> sp_eigv;
+  std::vector<int> sym_table, inv_sym, orb_sym, nbss, ssbf;
+  std::vector<double> h1, eri;
+  std::vector<uint64_t> excit_mask;
+  std::vector<Spawned> spawned;
+
+  double rand() { return rng_.next(); }
+  bool btest(const uint64_t* f, int orb) const { return (f[(orb - 1) / 64] >> ((orb - 1) % 64)) & 1; }
+  int ms_of(int orb) const { return (orb & 1) ? 1 : -1; }
+  int ims_of(int orb) const { return (ms_of(orb) + 3) / 2; }  // 1 = beta, 2 = alpha
+
+  struct Decoded {
+    std::vector<int> occ, occ_a, occ_b, unocc_a, unocc_b;
+    std::vector<int> symunocc;  // [(ims-1)*nsym + sym]
+  };
+
+  std::vector<int> decode_occ(const uint64_t* f) const {
+    std::vector<int> occ;
+    for (int i = 1; i <= nbasis; ++i)
+      if (btest(f, i)) occ.push_back(i);
+    return occ;
+  }
+
+  Decoded decode_full(const uint64_t* f) const {
+    Decoded d;
+    d.occ = decode_occ(f);
+    if (sys_.system == FQ_SYS_HUB_K) {
+      for (int i = 1; i <= nbasis; ++i) {
+        bool o = btest(f, i);
+        if (i & 1)
+          (o ? d.occ_a : d.unocc_a).push_back(i);
+        else
+          (o ? d.occ_b : d.unocc_b).push_back(i);
+      }
+    } else {
+      d.symunocc = nbss;
+      for (int o : d.occ) d.symunocc[(ims_of(o) - 1) * nsym + orb_sym[o - 1]] -= 1;
+    }
+    return d;
+  }
+  int symunocc(const Decoded& d, int ims, int sym) const { return d.symunocc[(ims - 1) * nsym + sym]; }
+
+  // ---- excitations ---------------------------------------------------------
+  Excit get_excitation(const uint64_t* f1, const uint64_t* f2) const {
+    Excit e;
+    bool differ = false;
+    for (int w = 0; w < W; ++w) differ = differ || f1[w] != f2[w];
+    if (!differ) return e;
+    int nx = 0;
+    for (int w = 0; w < W; ++w) nx += popc(f1[w] ^ f2[w]);
+    e.nexcit = nx / 2;
+    const int shift = nel - e.nexcit;
+    if (e.nexcit <= 2) {
+      int ie1 = 0, ie2 = 0, iel1 = 0, iel2 = 0, perm = 0;
+      for (int w = 0; w < W; ++w) {
+        if (f1[w] == f2[w]) {
+          if (fmodulo2(ie1 - ie2) == 1) {
+            int n = popc(f1[w]);
+            iel1 += n;
+            iel2 += n;
+          }
+          continue;
+        }
+        for (int j = 0; j < 64; ++j) {
+          bool t1 = (f1[w] >> j) & 1, t2 = (f2[w] >> j) & 1;
+          const int orb = w * 64 + j + 1;
+          if (t2) ++iel2;
+          if (t1) {
+            ++iel1;
+            if (!t2) {
+              ++ie1;
+              e.from[ie1 - 1] = orb;
+              perm += shift - iel1 + ie1;
+            }
+          } else if (t2) {
+            ++ie2;
+            e.to[ie2 - 1] = orb;
+            perm += shift - iel2 + ie2;
+          }
+        }
+      }
+      e.perm = fmod2(perm) == 1;
+    }
+    return e;
+  }
+  void perm1(const uint64_t* f, Excit& e) const {
+    int perm = 0;
+    for (int w = 0; w < W; ++w)
+      perm += popc(f[w] & (excit_mask[e.from[0] * W + w] ^ excit_mask[e.to[0] * W + w]));
+    if (e.from[0] > e.to[0]) perm -= 1;
+    e.perm = fmod2(perm) == 1;
+  }
+  void perm2(const uint64_t* f, Excit& e) const {
+    int perm = 0;
+    for (int w = 0; w < W; ++w) {
+      uint64_t ia = excit_mask[e.from[0] * W + w] ^ excit_mask[e.to[0] * W + w];
+      uint64_t jb = excit_mask[e.from[1] * W + w] ^ excit_mask[e.to[1] * W + w];
+      perm += popc((f[w] & ia) ^ (f[w] & jb));
+    }
+    if (e.from[0] > e.to[0]) perm += 1;
+    if (e.from[0] > e.to[1]) perm += 1;
+    if (e.from[1] > e.to[1] || e.from[1] < e.to[0]) perm += 1;
+    e.perm = fmod2(perm) == 1;
+  }
+  void excited_det(const uint64_t* f, const Excit& e, uint64_t* out) const {
+    for (int w = 0; w < W; ++w) out[w] = f[w];
+    for (int i = 0; i < e.nexcit; ++i) {
+      int o = e.from[i];
+      out[(o - 1) / 64] &= ~(uint64_t{1} << ((o - 1) % 64));
+      o = e.to[i];
+      out[(o - 1) / 64] |= uint64_t{1} << ((o - 1) % 64);
+    }
+  }
+
+  // ---- Hubbard-k -------------------------------------------------------------
+  int st(int a, int b) const { return sym_table[(a - 1) * sys_.nsym + (b - 1)]; }
+  double two_e_hub_k(int p1, int p2, int p3, int p4) const {
+    bool s12 = ms_of(p1) == ms_of(p2), s34 = ms_of(p3) == ms_of(p4);
+    int dk = st((p1 + 1) / 2, (p2 + 1) / 2);
+    dk = st(dk, inv_sym[(p3 + 1) / 2 - 1]);
+    bool conserved = dk == (p4 + 1) / 2;
+    if (!s12 && (s12 == s34) && conserved) return ms_of(p1) == ms_of(p3) ? sys_.coulomb_k : -sys_.coulomb_k;
+    return 0.0;
+  }
+
+  // ---- molecular integrals ---------------------------------------------------
+  static int64_t tri(int64_t i, int64_t j) { return (i * (i - 1)) / 2 + j; }
+  int sp(int i) const { return (i + 1) / 2; }
+  double h1g(int i, int j) const { return h1[(sp(i) - 1) * sys_.norb + (sp(j) - 1)]; }
+  double eri_nz(int i, int j, int a, int b) const {
+    int ii = std::max(i, a), aa = std::min(i, a), jj = std::max(j, b), bb = std::min(j, b);
+    int64_t ia = tri(sp(ii), sp(aa)), jb = tri(sp(jj), sp(bb));
+    return eri[(ia < jb ? tri(jb, ia) : tri(ia, jb)) - 1];
+  }
+  double sc1_excit(const std::vector<int>& occ, int i, int a, bool perm) const {
+    double h = h1g(i, a);
+    for (int o : occ) {
+      if (o != i) {
+        h = h + eri_nz(i, o, a, o);
+        if (ms_of(o) == ms_of(i)) h = h - eri_nz(i, o, o, a);
+      }
+    }
+    if (perm) h = -h;
+    return h;
+  }
+  double sc2_excit(int i, int j, int a, int b, bool perm) const {
+    double h = 0.0;
+    if (ms_of(i) == ms_of(a)) h = eri_nz(i, j, a, b);
+    if (ms_of(i) == ms_of(b)) h = h - eri_nz(i, j, b, a);
+    if (perm) h = -h;
+    return h;
+  }
+  double sc0(const uint64_t* f) const {
+    std::vector<int> occ = decode_occ(f);
+    if (sys_.system == FQ_SYS_HUB_K) {
+      double h = static_cast<double>(sys_.nalpha * sys_.nbeta) * sys_.coulomb_k;
+      for (int o : occ) h = h + sp_eigv[o - 1];
+      return h;
+    }
+    double h = sys_.ecore;
+    for (size_t ie = 0; ie < occ.size(); ++ie) {
+      int i = occ[ie];
+      h = h + h1g(i, i);
+      for (size_t je = ie + 1; je < occ.size(); ++je) {
+        int j = occ[je];
+        h = h + eri_nz(i, j, i, j);
+        if (ms_of(i) == ms_of(j)) h = h - eri_nz(i, j, j, i);
+      }
+    }
+    return h;
+  }
+
+  void update_proj_energy(const uint64_t* f, const std::vector<int>& occ, double pop, Excit& c, double& proj,
+                          double& d0) const {
+    if (c.nexcit == 0) {
+      d0 = d0 + pop;
+      return;
+    }
+    if (sys_.system == FQ_SYS_HUB_K) {
+      if (c.nexcit == 2) {
+        double h = two_e_hub_k(c.from[0], c.from[1], c.to[0], c.to[1]);
+        if (c.perm) h = -h;
+        proj = proj + h * pop;
+      }
+      return;
+    }
+    if (c.nexcit == 1) {
+      if (ms_of(c.from[0]) == ms_of(c.to[0]) && orb_sym[c.from[0] - 1] == orb_sym[c.to[0] - 1]) {
+        double h = sc1_excit(occ, c.from[0], c.to[0], c.perm);
+        proj = proj + h * pop;
+      }
+    } else if (c.nexcit == 2) {
+      if (ms_of(c.from[0]) + ms_of(c.from[1]) == ms_of(c.to[0]) + ms_of(c.to[1])) {
+        int ij = orb_sym[c.from[0] - 1] ^ orb_sym[c.from[1] - 1];
+        int ab = orb_sym[c.to[0] - 1] ^ orb_sym[c.to[1] - 1];
+        if (ij == ab) {
+          double h = sc2_excit(c.from[0], c.from[1], c.to[0], c.to[1], c.perm);
+          proj = proj + h * pop;
+        }
+      }
+    }
+  }
+
+  int64_t parent_flag(const double* pop) const {
+    int64_t flag = 0;
+    if (!par_.initiator) return 0;
+    const double ip = par_.initiator_pop;
+    if (par_.quadrature_initiator && S % 2 == 0) {
+      for (int i = 0; i < S; i += 2) {
+        double mag = pop[i] * pop[i] + pop[i + 1] * pop[i + 1];
+        if (!(mag > ip * ip)) flag |= (int64_t{1} << i) | (int64_t{1} << (i + 1));
+      }
+    } else {
+      for (int i = 0; i < S; ++i)
+        if (!(std::fabs(pop[i]) > ip)) flag |= int64_t{1} << i;
+    }
+    return flag;
+  }
+
+  int decide_nattempts(double population) {
+    int n = std::abs(static_cast<int>(population));
+    double pextra = std::fabs(population) - n;
+    if (std::fabs(pextra) > kDepsilon) {
+      if (pextra > rand()) n = n + 1;
+    }
+    return n;
+  }
+
+  int64_t stoch_round_spawn(double pspawn) {
+    const int64_t cutoff = par_.spawn_cutoff;
+    int64_t nspawn;
+    if (pspawn < static_cast<double>(cutoff)) {
+      nspawn = (pspawn > rand() * static_cast<double>(cutoff)) ? cutoff : 0;
+    } else {
+      nspawn = static_cast<int64_t>(pspawn);
+      double padd = pspawn - static_cast<double>(nspawn);
+      if (padd > rand()) nspawn = nspawn + 1;
+    }
+    return nspawn;
+  }
+
+  // ---- spawners --------------------------------------------------------------
+  int64_t spawn(const Decoded& d, const uint64_t* f, int64_t parent, double tau, Excit& conn) {
+    if (sys_.system == FQ_SYS_HUB_K) return spawn_hub_k(d, f, parent, tau, conn);
+    return spawn_mol(d, f, parent, tau, conn);
+  }
+
+  void choose_ij_hub_k(const Decoded& d, int& i, int& j, int& ij_sym) {
+    double r = rand();
+    const int na = sys_.nalpha, nb = sys_.nbeta;
+    int ind = static_cast<int>(r * na * nb) + 1;
+    i = static_cast<int>((ind - 1.0) / nb) + 1;
+    j = ind - (i - 1) * nb;
+    i = d.occ_a[i - 1];
+    j = d.occ_b[j - 1];
+    ij_sym = st((i + 1) / 2, (j + 1) / 2);
+  }
+  bool find_ab_hub_k(const Decoded& d, const uint64_t* f, int ij_sym, int& a, int& b) {
+    int r = static_cast<int>(rand() * sys_.nvirt_alpha) + 1;
+    a = d.unocc_a[r - 1];
+    int ka = (a + 1) / 2;
+    b = 2 * st(ij_sym, inv_sym[ka - 1]);
+    return !btest(f, b);
+  }
+  double pgen_hub_k(const Decoded& d, const uint64_t* f, int ij_sym) const {
+    int forbidden = 0;
+    for (int a = 0; a < sys_.nvirt_alpha; ++a) {
+      int ka = (d.unocc_a[a] + 1) / 2;
+      int b = 2 * st(ij_sym, inv_sym[ka - 1]);
+      if (btest(f, b)) ++forbidden;
+    }
+    return 1.0 / static_cast<double>(sys_.nalpha * sys_.nbeta * (sys_.nvirt_alpha - forbidden));
+  }
+  double sc2_hub_k_excit(const uint64_t* f, Excit& c) const {
+    if (c.from[0] > c.from[1]) std::swap(c.from[0], c.from[1]);
+    if (c.to[0] > c.to[1]) std::swap(c.to[0], c.to[1]);
+    double h = sys_.coulomb_k;
+    perm2(f, c);
+    if (c.perm) h = -h;
+    if (fmod2(c.from[0] - c.to[0]) != 0) h = -h;
+    return h;
+  }
+
+  int64_t spawn_hub_k(const Decoded& d, const uint64_t* f, int64_t parent, double tau, Excit& conn) {
+    int i, j, ij_sym;
+    double pgen, absh;
+    conn = Excit();
+    choose_ij_hub_k(d, i, j, ij_sym);
+    conn.from[0] = i;
+    conn.from[1] = j;
+    if (par_.excit_gen == FQ_EXCIT_NO_RENORM) {
+      int a, b;
+      bool allowed = find_ab_hub_k(d, f, ij_sym, a, b);
+      conn.to[0] = a;
+      conn.to[1] = b;
+      conn.nexcit = 2;
+      if (allowed) {
+        pgen = 1.0 / static_cast<double>(sys_.nalpha * sys_.nbeta * sys_.nvirt_alpha);
+        absh = std::fabs(sys_.coulomb_k);
+      } else {
+        pgen = 1.0;
+        absh = 0.0;
+      }
+    } else {
+      pgen = pgen_hub_k(d, f, ij_sym);
+      absh = std::fabs(sys_.coulomb_k);
+    }
+    int64_t nspawn = stoch_round_spawn(static_cast<double>(rf) * tau * absh / pgen);
+    if (nspawn != 0) {
+      double h;
+      if (par_.excit_gen == FQ_EXCIT_NO_RENORM) {
+        h = sc2_hub_k_excit(f, conn);
+      } else {
+        int sym2 = st((conn.from[0] + 1) / 2, (conn.from[1] + 1) / 2);
+        conn.nexcit = 2;
+        int a, b;
+        while (!find_ab_hub_k(d, f, sym2, a, b)) {
+        }
+        conn.to[0] = a;
+        conn.to[1] = b;
+        h = sc2_hub_k_excit(f, conn);
+      }
+      nspawn = (h > 0.0) ? -fsign64(nspawn, parent) : fsign64(nspawn, parent);
+    }
+    return nspawn;
+  }
+
+  // Molecular renormalised generator (src/excit_gen_mol.f90:16-101).
+  int64_t spawn_mol(const Decoded& d, const uint64_t* f, int64_t parent, double tau, Excit& conn) {
+    double pgen = 1.0, h = 0.0;
+    bool allowed = false;
+    conn = Excit();
+    if (rand() < sys_.pattempt_single) {
+      int i = 0, a = 0;
+      allowed = choose_ia(d, f, i, a);
+      conn.nexcit = 1;
+      conn.from[0] = i;
+      conn.to[0] = a;
+      if (allowed) {
+        pgen = sys_.pattempt_single * pgen_single(d, a);
+        perm1(f, conn);
+        h = sc1_excit(d.occ, i, a, conn.perm);
+      } else {
+        h = 0.0;
+        pgen = 1.0;
+      }
+    } else {
+      int i, j, ij_sym, ij_spin;
+      double pgen_ij;
+      choose_ij_mol(d, i, j, ij_sym, ij_spin, pgen_ij);
+      conn.from[0] = i;
+      conn.from[1] = j;
+      int a = 0, b = 0;
+      allowed = choose_ab_mol(d, f, ij_sym, ij_spin, a, b);
+      conn.to[0] = a;
+      conn.to[1] = b;
+      conn.nexcit = 2;
+      if (allowed) {
+        pgen = sys_.pattempt_double * pgen_ij * pgen_double(d, ij_sym, a, b, ij_spin);
+        perm2(f, conn);
+        h = sc2_excit(conn.from[0], conn.from[1], conn.to[0], conn.to[1], conn.perm);
+      } else {
+        h = 0.0;
+        pgen = 1.0;
+      }
+    }
+    // attempt_to_spawn (src/spawning.F90:711-766)
+    double pspawn = tau * std::fabs(h) / pgen;
+    pspawn = pspawn * static_cast<double>(rf);
+    int64_t nspawn = stoch_round_spawn(pspawn);
+    if (nspawn > 0) nspawn = (h > 0.0) ? -fsign64(nspawn, parent) : fsign64(nspawn, parent);
+    return nspawn;
+  }
+
+  bool choose_ia(const Decoded& d, const uint64_t* f, int& i, int& a) {
+    bool allowed = false;
+    for (int k = 0; k < nel; ++k) {
+      int o = d.occ[k];
+      if (symunocc(d, ims_of(o), orb_sym[o - 1]) != 0) {
+        allowed = true;
+        break;
+      }
+    }
+    if (!allowed) return false;
+    while (true) {
+      i = d.occ[static_cast<int>(rand() * nel)];
+      int imsa = ims_of(i), isyma = orb_sym[i - 1];
+      if (symunocc(d, imsa, isyma) != 0) {
+        const int g = (imsa - 1) * nsym + isyma;
+        while (true) {
+          int ind = static_cast<int>(nbss[g] * rand()) + 1;
+          a = ssbf[g * sys_.max_sym_spin + ind - 1];
+          if (!btest(f, a)) break;
+        }
+        break;
+      }
+    }
+    return true;
+  }
+  double pgen_single(const Decoded& d, int a) const {
+    int ni = nel;
+    for (int k = 0; k < nel; ++k) {
+      int o = d.occ[k];
+      if (symunocc(d, ims_of(o), orb_sym[o - 1]) == 0) ni = ni - 1;
+    }
+    return 1.0 / static_cast<double>(ni * symunocc(d, ims_of(a), orb_sym[a - 1]));
+  }
+  void choose_ij_mol(const Decoded& d, int& i, int& j, int& ij_sym, int& ij_spin, double& pgen_ij) {
+    int ind = static_cast<int>(rand() * nel * (nel - 1) / 2) + 1;
+    int j_ind = static_cast<int>(1.50 + std::sqrt(2 * ind - 1.750));
+    int i_ind = ind - ((j_ind - 1) * (j_ind - 2)) / 2;
+    i = d.occ[i_ind - 1];
+    j = d.occ[j_ind - 1];
+    ij_sym = orb_sym[i - 1] ^ orb_sym[j - 1];
+    ij_spin = ms_of(i) + ms_of(j);
+    pgen_ij = 2.0 / static_cast<double>(nel * (nel - 1));
+  }
+  bool choose_ab_mol(const Decoded& d, const uint64_t* f, int sym, int spin, int& a, int& b) {
+    bool allowed = false;
+    int fac = 1, shift = 0, na = nbasis;
+    for (int isyma = 0; isyma < nsym && !allowed; ++isyma) {
+      int isymb = isyma ^ sym;
+      if (spin == -2) {
+        if (symunocc(d, 1, isyma) > 0 &&
+            (symunocc(d, 1, isymb) > 1 || (symunocc(d, 1, isymb) == 1 && isyma != isymb)))
+          allowed = true;
+      } else if (spin == 0) {
+        if ((symunocc(d, 1, isyma) > 0 && symunocc(d, 2, isymb) > 0) ||
+            (symunocc(d, 2, isyma) > 0 && symunocc(d, 1, isymb) > 0))
+          allowed = true;
+      } else {
+        if (symunocc(d, 2, isyma) > 0 &&
+            (symunocc(d, 2, isymb) > 1 || (symunocc(d, 2, isymb) == 1 && isyma != isymb)))
+          allowed = true;
+      }
+    }
+    if (spin == -2) {
+      fac = 2;
+      shift = 0;
+      na = nbasis / 2;
+    } else if (spin == 0) {
+      fac = 1;
+      shift = 0;
+      na = nbasis;
+    } else {
+      fac = 2;
+      shift = 1;
+      na = nbasis / 2;
+    }
+    if (!allowed) return false;
+    while (true) {
+      a = static_cast<int>(rand() * na) + 1;
+      a = fac * a - shift;
+      if (!btest(f, a)) {
+        int imsb = (spin - ms_of(a) + 3) / 2;
+        int isymb = sym ^ orb_sym[a - 1];
+        int nu = symunocc(d, imsb, isymb);
+        if (nu > 1 || (nu == 1 && (isymb != orb_sym[a - 1] || spin == 0))) {
+          const int g = (imsb - 1) * nsym + isymb;
+          while (true) {
+            int ind = static_cast<int>(nbss[g] * rand()) + 1;
+            b = ssbf[g * sys_.max_sym_spin + ind - 1];
+            if (b != a && !btest(f, b)) break;
+          }
+          break;
+        }
+      }
+    }
+    if (a > b) std::swap(a, b);
+    return true;
+  }
+  double pgen_double(const Decoded& d, int ij_sym, int a, int b, int spin) const {
+    int imsa = ims_of(a), imsb = ims_of(b);
+    int n_aij;
+    double p_bija, p_aijb;
+    if (spin == -2 || spin == 2) {
+      const int ims = spin == -2 ? 1 : 2;
+      n_aij = spin == -2 ? sys_.nvirt_beta : sys_.nvirt_alpha;
+      for (int isyma = 0; isyma < nsym; ++isyma) {
+        int isymb = isyma ^ ij_sym;
+        if (symunocc(d, ims, isymb) == 0)
+          n_aij = n_aij - symunocc(d, ims, isyma);
+        else if (isyma == isymb && symunocc(d, ims, isymb) == 1)
+          n_aij = n_aij - symunocc(d, ims, isyma);
+      }
+      if (orb_sym[a - 1] == orb_sym[b - 1]) {
+        p_aijb = 1.0 / static_cast<double>(symunocc(d, imsa, orb_sym[a - 1]) - 1);
+        p_bija = 1.0 / static_cast<double>(symunocc(d, imsb, orb_sym[b - 1]) - 1);
+      } else {
+        p_aijb = 1.0 / static_cast<double>(symunocc(d, imsa, orb_sym[a - 1]));
+        p_bija = 1.0 / static_cast<double>(symunocc(d, imsb, orb_sym[b - 1]));
+      }
+    } else {
+      n_aij = sys_.nvirt;
+      for (int isyma = 0; isyma < nsym; ++isyma) {
+        int isymb = isyma ^ ij_sym;
+        if (symunocc(d, 1, isymb) == 0) n_aij = n_aij - symunocc(d, 2, isyma);
+        if (symunocc(d, 2, isymb) == 0) n_aij = n_aij - symunocc(d, 1, isyma);
+      }
+      p_aijb = 1.0 / static_cast<double>(symunocc(d, imsa, orb_sym[a - 1]));
+      p_bija = 1.0 / static_cast<double>(symunocc(d, imsb, orb_sym[b - 1]));
+    }
+    return (1.0 / static_cast<double>(n_aij)) * (p_bija + p_aijb);
+  }
+
+  void create_spawned(const uint64_t* f, const Excit& conn, int64_t nspawn, int space, int64_t flag) {
+    if (static_cast<int64_t>(spawned.size()) + 1 > par_.spawn_capacity) {
+      spawn_error = true;
+      return;
+    }
+    Spawned sp;
+    std::memset(&sp, 0, sizeof(sp));
+    excited_det(f, conn, sp.f);
+    sp.pop[space] = nspawn;
+    sp.flag = par_.initiator ? flag : 0;
+    spawned.push_back(sp);
+  }
+
+  void death(double tau, double Kii, double shift, double E, int64_t& population, double& tot, int64_t& ndeath) {
+    const double weight = 1.0, qnpc = 1.0, dmqmc = 1.0, cheby = 1.0;
+    double pd = tau * ((Kii - E) * weight + (E - shift) * qnpc) * dmqmc;
+    pd = pd * cheby;
+    pd = pd * static_cast<double>(population < 0 ? -population : population);
+    int64_t kill = static_cast<int64_t>(pd);
+    pd = pd - static_cast<double>(kill);
+    double r = rand();
+    if (std::fabs(pd) > r) kill = pd > 0.0 ? kill + 1 : kill - 1;
+    int64_t old = population;
+    if (population < 0)
+      population = population + kill;
+    else
+      population = population - kill;
+    tot = tot + static_cast<double>(std::llabs(population) - std::llabs(old)) / static_cast<double>(rf);
+    ndeath = ndeath + std::llabs(kill);
+  }
+
+  // ---- annihilation ----------------------------------------------------------
+  // Unsigned multi-word comparison, last word most significant
+  // (src/bit_utils.F90:341-480).
+  int cmp(const uint64_t* a, const uint64_t* b) const {  // bit_str_cmp: 1 if a<b, -1 if a>b
+    for (int w = W - 1; w >= 0; --w) {
+      if (a[w] < b[w]) return 1;
+      if (a[w] > b[w]) return -1;
+    }
+    return 0;
+  }
+  bool gt(const uint64_t* a, const uint64_t* b) const { return cmp(a, b) == -1; }
+  bool ge(const uint64_t* a, const uint64_t* b) const { return cmp(a, b) != 1; }
+
+  // qsort_int_64_list (src/sort.f90:213-396), element = Spawned.
+  void qsort_spawned(std::vector<Spawned>& list) const {
+    const int head = static_cast<int>(list.size());
+    auto L = [&](int k) -> Spawned& { return list[k - 1]; };  // 1-based view
+    int stack[2][64];
+    int nstack = 0, lo = 1, hi = head;
+    while (true) {
+      if (hi - lo < 7) {
+        for (int j = lo + 1; j <= hi; ++j) {
+          Spawned tmp = L(j);
+          int i;
+          for (i = j - 1; i >= 1; --i) {
+            if (ge(tmp.f, L(i).f)) break;
+            L(i + 1) = L(i);
+          }
+          L(i + 1) = tmp;
+        }
+        if (nstack == 0) break;
+        hi = stack[1][nstack - 1];
+        lo = stack[0][nstack - 1];
+        --nstack;
+      } else {
+        int pivot = (lo + hi) / 2;
+        std::swap(L(pivot), L(lo + 1));
+        if (gt(L(lo).f, L(hi).f)) std::swap(L(lo), L(hi));
+        if (gt(L(lo + 1).f, L(hi).f)) std::swap(L(lo + 1), L(hi));
+        if (gt(L(lo).f, L(lo + 1).f)) std::swap(L(lo), L(lo + 1));
+        int i = lo + 1, j = hi;
+        Spawned tmp = L(lo + 1);
+        while (true) {
+          ++i;
+          while (gt(tmp.f, L(i).f)) ++i;
+          --j;
+          while (gt(L(j).f, tmp.f)) --j;
+          if (j < i) break;
+          std::swap(L(i), L(j));
+        }
+        L(lo + 1) = L(j);
+        L(j) = tmp;
+        ++nstack;
+        if (hi - i + 1 >= j - lo) {
+          stack[1][nstack - 1] = hi;
+          stack[0][nstack - 1] = i;
+          hi = j - 1;
+        } else {
+          stack[1][nstack - 1] = j - 1;
+          stack[0][nstack - 1] = lo;
+          lo = i;
+        }
+      }
+    }
+  }
+
+  bool same(const uint64_t* a, const uint64_t* b) const {
+    for (int w = 0; w < W; ++w)
+      if (a[w] != b[w]) return false;
+    return true;
+  }
+  bool allzero(const int64_t* p) const {
+    for (int s = 0; s < S; ++s)
+      if (p[s] != 0) return false;
+    return true;
+  }
+
+  // annihilate_spawn_t (src/spawn_data.F90:859-935)
+  void self_annihilate_plain() {
+    int n = static_cast<int>(spawned.size());
+    if (n == 0) return;
+    std::vector<Spawned> out;
+    int k = 0;
+    while (k < n) {
+      Spawned acc = spawned[k];
+      int m = k + 1;
+      while (m < n && same(spawned[m].f, acc.f)) {
+        for (int s = 0; s < S; ++s) acc.pop[s] += spawned[m].pop[s];
+        ++m;
+      }
+      if (!allzero(acc.pop)) out.push_back(acc);
+      k = m;
+    }
+    spawned.swap(out);
+  }
+
+  // annihilate_spawn_t_initiator (src/spawn_data.F90:937-1101).  The first
+  // entry of each run sets the event count from the sign of its population
+  // converted to a default (32-bit) integer, as the reference does.
+  void self_annihilate_initiator() {
+    int n = static_cast<int>(spawned.size());
+    if (n == 0) return;
+    std::vector<Spawned> out;
+    int k = 0;
+    while (k < n) {
+      Spawned acc = spawned[k];
+      int64_t init_pop[FQ_MAX_SPACES];
+      int events[FQ_MAX_SPACES];
+      for (int s = 0; s < S; ++s) {
+        if (!((spawned[k].flag >> s) & 1)) {
+          init_pop[s] = spawned[k].pop[s];
+          events[s] = 0;
+        } else {
+          init_pop[s] = 0;
+          int32_t p32 = static_cast<int32_t>(static_cast<uint32_t>(spawned[k].pop[s] & 0xffffffffULL));
+          events[s] = p32 >= 0 ? 1 : -1;
+        }
+      }
+      int m = k + 1;
+      while (m < n && same(spawned[m].f, acc.f)) {
+        for (int s = 0; s < S; ++s) {
+          if (!((spawned[m].flag >> s) & 1))
+            init_pop[s] += spawned[m].pop[s];
+          else if (spawned[m].pop[s] < 0)
+            events[s] -= 1;
+          else if (spawned[m].pop[s] > 0)
+            events[s] += 1;
+        }
+        for (int s = 0; s < S; ++s) acc.pop[s] += spawned[m].pop[s];
+        ++m;
+      }
+      acc.flag = 0;
+      for (int s = 0; s < S; ++s) {
+        int sgn_tot = acc.pop[s] >= 0 ? 1 : -1;
+        int sgn_init = init_pop[s] >= 0 ? 1 : -1;
+        if (init_pop[s] != 0 && sgn_tot == sgn_init) {
+        } else if (std::abs(events[s]) > 1) {
+        } else {
+          acc.flag += int64_t{1} << s;
+        }
+      }
+      if (!allzero(acc.pop)) out.push_back(acc);
+      k = m;
+    }
+    spawned.swap(out);
+  }
+
+  // binary_search (src/search.f90:47-135), 0-based range [istart, iend].
+  void bsearch(const uint64_t* item, int istart, int iend, bool& hit, int& pos) const {
+    hit = false;
+    if (istart > iend) {
+      pos = istart;
+      return;
+    }
+    int lo = istart, hi = iend;
+    while (hi != lo) {
+      pos = (hi + lo) / 2;
+      int c = cmp(states[pos].f, item);
+      if (c == 0) {
+        hit = true;
+        break;
+      } else if (c == 1) {
+        lo = pos + 1;
+      } else {
+        hi = pos;
+      }
+    }
+    if (hi == lo) {
+      int c = cmp(states[hi].f, item);
+      if (c == 0) {
+        hit = true;
+        pos = hi;
+      } else if (c == 1) {
+        pos = hi + 1;
+      } else {
+        pos = hi;
+      }
+    }
+  }
+
+  void stoch_round(int64_t* p) {
+    for (int s = 0; s < S; ++s) {
+      if (std::llabs(p[s]) < rf && p[s] != 0) {
+        double r = rand() * static_cast<double>(rf);
+        p[s] = (static_cast<double>(std::llabs(p[s])) > r) ? fsign64(rf, p[s]) : 0;
+      }
+    }
+  }
+
+  void annihilate() {
+    if (!spawned.empty()) {
+      qsort_spawned(spawned);
+      if (par_.initiator)
+        self_annihilate_initiator();
+      else
+        self_annihilate_plain();
+    }
+    if (!spawned.empty()) {
+      annihilate_main_list();
+      remove_unoccupied();
+      if (par_.real_amplitudes) round_low_spawns();
+      insert_new();
+    } else {
+      remove_unoccupied();
+    }
+  }
+
+  void add_np(const int64_t* now, const int64_t* old) {
+    for (int s = 0; s < S; ++s)
+      nparticles[s] = nparticles[s] + static_cast<double>(std::llabs(now[s]) - std::llabs(old[s])) / rf;
+  }
+
+  void annihilate_main_list() {
+    int istart = 0, iend = static_cast<int>(states.size()) - 1;
+    std::vector<Spawned> keep;
+    for (size_t i = 0; i < spawned.size(); ++i) {
+      bool hit;
+      int pos;
+      bsearch(spawned[i].f, istart, iend, hit, pos);
+      if (hit) {
+        int64_t old[FQ_MAX_SPACES];
+        for (int s = 0; s < S; ++s) old[s] = pops[pos * S + s];
+        if (par_.initiator) {
+          for (int s = 0; s < S; ++s) {
+            if (pops[pos * S + s] != 0)
+              pops[pos * S + s] += spawned[i].pop[s];
+            else if (!((spawned[i].flag >> s) & 1))
+              pops[pos * S + s] = spawned[i].pop[s];
+          }
+        } else {
+          for (int s = 0; s < S; ++s) pops[pos * S + s] += spawned[i].pop[s];
+        }
+        add_np(&pops[pos * S], old);
+        istart = pos + 1;
+      } else {
+        if (par_.initiator) {
+          Spawned sp = spawned[i];
+          bool discard = true;
+          for (int s = 0; s < S; ++s) {
+            if ((spawned[i].flag >> s) & 1)
+              sp.pop[s] = 0;
+            else
+              discard = false;
+          }
+          if (!discard) keep.push_back(sp);
+        } else {
+          keep.push_back(spawned[i]);
+        }
+      }
+    }
+    spawned.swap(keep);
+  }
+
+  void remove_unoccupied() {
+    size_t k = 0;
+    for (size_t i = 0; i < states.size(); ++i) {
+      if (par_.real_amplitudes) {
+        int64_t old[FQ_MAX_SPACES];
+        for (int s = 0; s < S; ++s) old[s] = pops[i * S + s];
+        stoch_round(&pops[i * S]);
+        add_np(&pops[i * S], old);
+      }
+      if (!allzero(&pops[i * S])) {
+        states[k] = states[i];
+        for (int s = 0; s < S; ++s) pops[k * S + s] = pops[i * S + s];
+        dat[k] = dat[i];
+        ++k;
+      }
+    }
+    states.resize(k);
+    pops.resize(k * S);
+    dat.resize(k);
+  }
+
+  void round_low_spawns() {
+    std::vector<Spawned> keep;
+    for (Spawned& sp : spawned) {
+      stoch_round(sp.pop);
+      if (!allzero(sp.pop)) keep.push_back(sp);
+    }
+    spawned.swap(keep);
+  }
+
+  // insert_new_walkers: backward in-place merge (src/annihilation.f90:677-901).
+  void insert_new() {
+    const int64_t nnew = static_cast<int64_t>(spawned.size());
+    if (static_cast<int64_t>(states.size()) + nnew > par_.main_capacity) {
+      main_error = true;
+      return;
+    }
+    const int n0 = static_cast<int>(states.size());
+    states.resize(n0 + nnew);
+    pops.resize((n0 + nnew) * S);
+    dat.resize(n0 + nnew);
+    int iend = n0 - 1;
+    for (int64_t i = nnew - 1; i >= 0; --i) {
+      bool hit;
+      int pos;
+      bsearch(spawned[i].f, 0, iend, hit, pos);
+      for (int j = iend; j >= pos; --j) {
+        int k = j + static_cast<int>(i) + 1;
+        states[k] = states[j];
+        for (int s = 0; s < S; ++s) pops[k * S + s] = pops[j * S + s];
+        dat[k] = dat[j];
+      }
+      int k = pos + static_cast<int>(i);
+      for (int w = 0; w < W; ++w) states[k].f[w] = spawned[i].f[w];
+      for (int s = 0; s < S; ++s) pops[k * S + s] = spawned[i].pop[s];
+      dat[k] = sc0(spawned[i].f) - sys_.H00;
+      for (int s = 0; s < S; ++s)
+        nparticles[s] = nparticles[s] + std::fabs(static_cast<double>(spawned[i].pop[s]) / rf);
+      iend = pos - 1;
+    }
+  }
+};
+
+// Capacity in elements from a state_size / spawned_state_size input: a
+// positive value is an element count, a negative one is MB per core
+// (src/particle_t_utils.f90:84-101, src/qmc.F90:1445-1448).
+int64_t capacity_from(int64_t size, int64_t element_bytes, bool spawned) {
+  if (size >= 0) return size;
+  double mb = -static_cast<double>(size) * 1.0e6;
+  return static_cast<int64_t>(spawned ? mb / (2.0 * element_bytes) : mb / element_bytes);
+}
+
+// Report loop of do_fciqmc (src/fciqmc.f90:276-464) with the estimator and
+// shift bookkeeping of src/energy_evaluation.F90:425-711 and
+// src/qmc_common.F90:929-1017,1240-1304.
+int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows, int max_rows, int* nrows) {
+  fq_params p{};
+  p.nspaces = in->replica_tricks ? 2 : 1;
+  p.initiator = in->initiator;
+  p.initiator_pop = in->initiator_threshold;
+  p.quadrature_initiator = in->quadrature_initiator;
+  p.real_amplitudes = in->real_amplitudes;
+  p.pop_bit_shift = in->real_amplitudes ? (in->real_amplitude_force_32 ? 11 : 31) : 0;
+  const int64_t rf = int64_t{1} << p.pop_bit_shift;
+  p.spawn_cutoff = in->real_amplitudes ? static_cast<int64_t>(std::ceil(in->spawn_cutoff * static_cast<double>(rf))) : 0;
+  p.excit_gen = in->excit_gen;
+  p.rng_mode = FQ_RNG_REPLAY;
+  const int S = p.nspaces;
+  const int W = sys->nwords;
+  p.main_capacity = capacity_from(in->state_size, 8 * W + 8 * S + 8 * S, false);
+  p.spawn_capacity = capacity_from(in->spawned_state_size, 8 * (W + S + (in->initiator ? 1 : 0)), true);
+  Oracle o(sys, &p, static_cast<uint32_t>(static_cast<int32_t>(in->rng_seed)));
+  // initial_distribution (src/qmc.F90:1507-1560)
+  std::vector<int64_t> pop0(S, static_cast<int64_t>(std::llround(in->init_pop)) * rf);
+  std::vector<double> dat0(1, 0.0);
+  o.load(sys->f0, pop0.data(), dat0.data(), 1);
+  double shift[FQ_MAX_SPACES], proj[FQ_MAX_SPACES], d0[FQ_MAX_SPACES], ntot_old[FQ_MAX_SPACES];
+  bool vary[FQ_MAX_SPACES];
+  for (int s = 0; s < S; ++s) {
+    shift[s] = in->initial_shift;
+    vary[s] = false;
+    ntot_old[s] = o.nparticles[s];
+  }
+  o.projected_energy(proj, d0);
+  int nr = 0;
+  auto emit = [&](int64_t iter, int64_t nspawn, double rspawn) {
+    if (nr >= max_rows) return;
+    fqh_report_row& r = rows[nr++];
+    std::memset(&r, 0, sizeof(r));
+    r.iterations = iter;
+    for (int s = 0; s < S; ++s) {
+      r.shift[s] = shift[s];
+      r.proj_energy[s] = proj[s];
+      r.D0_population[s] = d0[s];
+      r.nparticles[s] = ntot_old[s];
+    }
+    r.nstates = static_cast<int64_t>(o.states.size());
+    r.nspawn_events = nspawn;
+    r.rspawn = rspawn;
+  };
+  emit(0, 0, 0.0);
+  for (int ireport = 1; ireport <= in->nreports; ++ireport) {
+    double eproj_old[FQ_MAX_SPACES];
+    for (int s = 0; s < S; ++s) eproj_old[s] = (std::fabs(d0[s]) < 2.2250738585072014e-308) ? 0.0 : proj[s] / d0[s];
+    double rspawn = 0.0;
+    for (int s = 0; s < S; ++s) proj[s] = d0[s] = 0.0;
+    int64_t nspawn_events = 0;
+    bool error = false;
+    for (int ic = 0; ic < in->mc_cycles; ++ic) {
+      const int64_t nattempts = static_cast<int64_t>(std::llround(2 * o.nparticles[0]));
+      fq_cycle_out out;
+      o.cycle(shift, eproj_old, in->tau, &out);
+      for (int s = 0; s < S; ++s) {
+        proj[s] = proj[s] + out.proj_energy[s];
+        d0[s] = d0[s] + out.D0_population[s];
+      }
+      nspawn_events = out.nspawn_events;
+      double rate = 0.0;
+      if (nattempts > 0) {
+        double ndeath_real = static_cast<double>(out.ndeath) / static_cast<double>(rf);
+        rate = (static_cast<double>(nspawn_events) + ndeath_real) / static_cast<double>(nattempts);
+      }
+      rspawn = rspawn + rate;
+      error = error || out.spawn_overflow || out.main_overflow;
+    }
+    double ntot[FQ_MAX_SPACES];
+    for (int s = 0; s < S; ++s) {
+      ntot[s] = o.nparticles[s];
+      proj[s] = proj[s] / in->mc_cycles;
+      d0[s] = d0[s] / in->mc_cycles;
+    }
+    rspawn = rspawn / in->mc_cycles;
+    for (int s = 0; s < S; ++s) {
+      if (vary[s]) {
+        const double denom = 1.0 * in->tau * in->mc_cycles;
+        shift[s] = shift[s] - std::log(ntot[s] / ntot_old[s]) * in->shift_damping / denom -
+                   std::log(ntot[s] / in->target_population) * 0.0 / denom;
+      }
+    }
+    for (int s = 0; s < S; ++s) ntot_old[s] = ntot[s];
+    for (int s = 0; s < S; ++s) {
+      if (!vary[s] && ntot[s] > in->target_population) {
+        vary[s] = true;
+        shift[s] = in->vary_shift_from;
+      }
+    }
+    emit(static_cast<int64_t>(ireport) * in->mc_cycles, nspawn_events, rspawn);
+    if (error) break;
+  }
+  *nrows = nr;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Oracle handle with the engine's cycle-level interface.
+void* orc_create(const fq_system* s, const fq_params* p, int64_t seed) {
+  return new Oracle(s, p, static_cast<uint32_t>(seed));
+}
+void orc_destroy(void* h) { delete static_cast<Oracle*>(h); }
+int orc_load(void* h, const uint64_t* st, const int64_t* p, const double* d, int64_t n) {
+  static_cast<Oracle*>(h)->load(st, p, d, n);
+  return 0;
+}
+int orc_cycle(void* h, const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
+  static_cast<Oracle*>(h)->cycle(shift, eproj_old, tau, out);
+  return 0;
+}
+int orc_annihilate(void* h, const uint64_t* sst, const int64_t* sp, const int64_t* sf, int64_t ns,
+                   fq_cycle_out* out) {
+  static_cast<Oracle*>(h)->annihilate_external(sst, sp, sf, ns, out);
+  return 0;
+}
+int64_t orc_nstates(void* h) { return static_cast<int64_t>(static_cast<Oracle*>(h)->states.size()); }
+int orc_download(void* h, uint64_t* st, int64_t* p, double* d) {
+  static_cast<Oracle*>(h)->download(st, p, d);
+  return 0;
+}
+uint64_t orc_rng_consumed(void* h) { return static_cast<Oracle*>(h)->rng_consumed(); }
+void orc_projected_energy(void* h, double* proj, double* d0) { static_cast<Oracle*>(h)->projected_energy(proj, d0); }
+double orc_nparticles(void* h, int s) { return static_cast<Oracle*>(h)->nparticles[s]; }
+
+int orc_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows, int max_rows, int* nrows) {
+  return run_fciqmc(sys, in, rows, max_rows, nrows);
+}
+
+}  // extern "C"
