@@ -87,6 +87,7 @@ def _declare(lib):
         "fqh_diagonal": (f64, [vp, P(u64)]),
         "fqh_free": (None, [vp]),
         "fqh_fciqmc": (C.c_int, [vp, P(FqhQmcIn), P(FqhReportRow), C.c_int, P(C.c_int)]),
+        "fqh_dsfmt_fill": (C.c_int, [i64, P(f64), i64]),
         "fq_create": (C.c_int, [P(FqSystem), P(FqParams), P(vp)]),
         "fq_load": (C.c_int, [vp, P(u64), P(i64), P(f64), i64]),
         "fq_push_rng": (C.c_int, [vp, P(f64), i64]),

@@ -1,0 +1,712 @@
+// Device-side arithmetic of one FCIQMC cycle for gfx950.
+//
+// Every routine here restates a reference routine in the same floating-point
+// expression order (the HIP sources are compiled with -ffp-contract=off), so
+// that a replayed random stream reproduces the reference's Markov chain:
+//   src/excitations.F90:75-200,247-406   excitation level, parity, child det
+//   src/hubbard_k.f90, src/hamiltonian_hub_k.f90:91-232   Hubbard-k elements
+//   src/excit_gen_hub_k.f90:29-159,469-770   Hubbard-k generators, pgen
+//   src/excit_gen_mol.f90:16-101,384-946,1140-1327   molecular renorm generator
+//   src/hamiltonian_molecular.f90:73-346   molecular Slater--Condon rules
+//   src/molecular_integrals.F90:736-845   8-fold packed two-body index
+//   src/spawning.F90:215-315,680-766, src/stoch_utils.f90:51-136
+//   src/death.f90:11-130, src/qmc_common.F90:379-406, src/ifciqmc.f90:14-57
+// Determinants are bit strings of up to FQ_MAX_WORDS 64-bit words; orbital o
+// (1-based) is bit (o-1)%64 of word (o-1)/64; odd orbitals are alpha.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fqmc.h"
+
+namespace fqd {
+
+constexpr double kDepsilon = 1.e-12;
+constexpr uint64_t kAlphaBits = 0x5555555555555555ULL;  // odd orbitals
+constexpr uint64_t kBetaBits = 0xAAAAAAAAAAAAAAAAULL;   // even orbitals
+constexpr int kMaxRejection = 1 << 20;                  // safety cap on rejection loops
+
+// System tables and run parameters, passed by value to every kernel.
+struct Sys {
+  int system, nbasis, nel, nalpha, nbeta, nvirt, nvirt_alpha, nvirt_beta, W, S;
+  uint64_t f0[FQ_MAX_WORDS];
+  uint64_t valid[FQ_MAX_WORDS];  // bits of existing orbitals
+  double H00, coulomb_k;
+  int nsym;
+  const int* sym_table;
+  const int* inv_sym;
+  const double* sp_eigv;
+  int norb, nsym_pg, max_sym_spin;
+  const int* orb_sym;
+  const int* nbss;
+  const int* ssbf;
+  double ecore;
+  const double* h1;
+  const double* eri;
+  double psingle, pdouble;
+  int initiator, quadrature, real_amps, excit_gen;
+  double initiator_pop;
+  int64_t rf, spawn_cutoff;
+};
+
+// ---------------------------------------------------------------------------
+// Random number sources
+// ---------------------------------------------------------------------------
+// Replay: consume a host-uploaded close-open stream in order.
+struct ReplayRng {
+  const double* buf;
+  int64_t pos;
+  int64_t avail;
+  bool failed;
+  __device__ double next() {
+    if (pos >= avail) {
+      failed = true;
+      return 0.5;
+    }
+    return buf[pos++];
+  }
+};
+
+// Philox4x32-10 (Salmon et al., SC'11), counter = (c0, c1, c2, draw-block).
+struct Philox {
+  uint32_t k0, k1, c0, c1, c2;
+  uint32_t block;
+  uint32_t out[4];
+  int have;
+  bool failed;
+  __device__ Philox(uint64_t key, uint32_t a, uint32_t b, uint32_t c)
+      : k0(static_cast<uint32_t>(key)), k1(static_cast<uint32_t>(key >> 32)), c0(a), c1(b), c2(c), block(0),
+        have(0), failed(false) {}
+  __device__ static void round(uint32_t* x, uint32_t ka, uint32_t kb) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    uint32_t hi0 = __umulhi(M0, x[0]), lo0 = M0 * x[0];
+    uint32_t hi1 = __umulhi(M1, x[2]), lo1 = M1 * x[2];
+    uint32_t y0 = hi1 ^ x[1] ^ ka, y1 = lo1, y2 = hi0 ^ x[3] ^ kb, y3 = lo0;
+    x[0] = y0;
+    x[1] = y1;
+    x[2] = y2;
+    x[3] = y3;
+  }
+  __device__ void generate() {
+    uint32_t x[4] = {c0, c1, c2, block++};
+    uint32_t ka = k0, kb = k1;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+      round(x, ka, kb);
+      ka += 0x9E3779B9u;
+      kb += 0xBB67AE85u;
+    }
+    out[0] = x[0];
+    out[1] = x[1];
+    out[2] = x[2];
+    out[3] = x[3];
+    have = 4;
+  }
+  // 53-bit uniform double in [0,1) from two 32-bit words.
+  __device__ double next() {
+    if (have < 2) generate();
+    uint32_t a = out[4 - have], b = out[5 - have];
+    have -= 2;
+    uint64_t v = (static_cast<uint64_t>(a >> 5) << 26) | (b >> 6);
+    return static_cast<double>(v) * (1.0 / 9007199254740992.0);
+  }
+};
+
+// Draw purposes: the third counter word of a Philox stream.
+enum : uint32_t { kDrawAttempt = 1, kDrawNattempts = 2, kDrawDeath = 3, kDrawRoundMain = 4, kDrawRoundSpawn = 5 };
+
+// ---------------------------------------------------------------------------
+// Bit-string helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool btest(const uint64_t* f, int orb) {
+  const int b = orb - 1;
+  return (f[b >> 6] >> (b & 63)) & 1;
+}
+__device__ __forceinline__ void bset(uint64_t* f, int orb) {
+  const int b = orb - 1;
+  f[b >> 6] |= uint64_t{1} << (b & 63);
+}
+__device__ __forceinline__ void bclr(uint64_t* f, int orb) {
+  const int b = orb - 1;
+  f[b >> 6] &= ~(uint64_t{1} << (b & 63));
+}
+// Word w of excit_mask(:, orb): bits of all orbitals after orb
+// (src/excitations.F90:17-45).
+__device__ __forceinline__ uint64_t after_mask(int orb, int w) {
+  const int b = orb - 1, bw = b >> 6, bb = b & 63;
+  if (w < bw) return 0;
+  if (w > bw) return ~uint64_t{0};
+  return bb == 63 ? 0 : (~uint64_t{0} << (bb + 1));
+}
+// k-th (0-based) set bit of x.
+__device__ __forceinline__ int kth_bit(uint64_t x, int k) {
+  for (int i = 0; i < k; ++i) x &= x - 1;
+  return __builtin_ctzll(x);
+}
+// Orbital (1-based) of the k-th set bit of (words & pattern) across W words.
+__device__ __forceinline__ int select_orb(const uint64_t* x, int W, int k) {
+  for (int w = 0; w < W; ++w) {
+    const int c = __popcll(x[w]);
+    if (k < c) return (w << 6) + kth_bit(x[w], k) + 1;
+    k -= c;
+  }
+  return 0;
+}
+__device__ __forceinline__ int64_t fsign64(int64_t a, int64_t b) {
+  const int64_t m = a < 0 ? -a : a;
+  return b >= 0 ? m : -m;
+}
+__device__ __forceinline__ int ms_of(int orb) { return (orb & 1) ? 1 : -1; }
+__device__ __forceinline__ int ims_of(int orb) { return (orb & 1) ? 2 : 1; }  // 1 = beta, 2 = alpha
+
+struct Excit {
+  int nexcit;
+  int from[2];
+  int to[2];
+  bool perm;
+};
+
+// get_excitation(f1, f2) (src/excitations.F90:75-200), bit-parallel form:
+// the running electron counts at a differing bit are prefix popcounts.
+__device__ inline Excit get_excitation(const uint64_t* f1, const uint64_t* f2, int W, int nel) {
+  Excit e;
+  e.nexcit = 0;
+  e.from[0] = e.from[1] = e.to[0] = e.to[1] = 0;
+  e.perm = false;
+  int nx = 0;
+  for (int w = 0; w < W; ++w) nx += __popcll(f1[w] ^ f2[w]);
+  if (nx == 0) return e;
+  e.nexcit = nx / 2;
+  if (e.nexcit > 2) return e;
+  const int shift = nel - e.nexcit;
+  int ie1 = 0, ie2 = 0, iel1 = 0, iel2 = 0, perm = 0;
+  for (int w = 0; w < W; ++w) {
+    const uint64_t a = f1[w], b = f2[w];
+    if (a == b) {
+      if ((((ie1 - ie2) % 2) + 2) % 2 == 1) {
+        const int n = __popcll(a);
+        iel1 += n;
+        iel2 += n;
+      }
+      continue;
+    }
+    uint64_t d = a ^ b;
+    while (d) {
+      const int j = __builtin_ctzll(d);
+      d &= d - 1;
+      const uint64_t le = (j == 63) ? ~uint64_t{0} : ((uint64_t{2} << j) - 1);
+      const int orb = (w << 6) + j + 1;
+      if ((a >> j) & 1) {
+        const int c1 = iel1 + __popcll(a & le);
+        ++ie1;
+        e.from[ie1 - 1] = orb;
+        perm += shift - c1 + ie1;
+      } else {
+        const int c2 = iel2 + __popcll(b & le);
+        ++ie2;
+        e.to[ie2 - 1] = orb;
+        perm += shift - c2 + ie2;
+      }
+    }
+    iel1 += __popcll(a);
+    iel2 += __popcll(b);
+  }
+  e.perm = (perm % 2) == 1;
+  return e;
+}
+
+__device__ inline bool perm1(const uint64_t* f, int W, int from, int to) {
+  int perm = 0;
+  for (int w = 0; w < W; ++w) perm += __popcll(f[w] & (after_mask(from, w) ^ after_mask(to, w)));
+  if (from > to) perm -= 1;
+  return (perm % 2) == 1;
+}
+__device__ inline bool perm2(const uint64_t* f, int W, const Excit& e) {
+  int perm = 0;
+  for (int w = 0; w < W; ++w) {
+    const uint64_t ia = after_mask(e.from[0], w) ^ after_mask(e.to[0], w);
+    const uint64_t jb = after_mask(e.from[1], w) ^ after_mask(e.to[1], w);
+    perm += __popcll((f[w] & ia) ^ (f[w] & jb));
+  }
+  if (e.from[0] > e.to[0]) perm += 1;
+  if (e.from[0] > e.to[1]) perm += 1;
+  if (e.from[1] > e.to[1] || e.from[1] < e.to[0]) perm += 1;
+  return (perm % 2) == 1;
+}
+
+// ---------------------------------------------------------------------------
+// Hubbard model, momentum space
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int st(const Sys& s, int a, int b) { return s.sym_table[(a - 1) * s.nsym + (b - 1)]; }
+
+__device__ inline double two_e_hub_k(const Sys& s, int p1, int p2, int p3, int p4) {
+  const bool s12 = ms_of(p1) == ms_of(p2), s34 = ms_of(p3) == ms_of(p4);
+  int dk = st(s, (p1 + 1) / 2, (p2 + 1) / 2);
+  dk = st(s, dk, s.inv_sym[(p3 + 1) / 2 - 1]);
+  if (!s12 && (s12 == s34) && dk == (p4 + 1) / 2) return ms_of(p1) == ms_of(p3) ? s.coulomb_k : -s.coulomb_k;
+  return 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Molecular integrals (RHF; spatial index (o+1)/2)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t tri(int64_t i, int64_t j) { return (i * (i - 1)) / 2 + j; }
+__device__ __forceinline__ double h1g(const Sys& s, int i, int j) {
+  return s.h1[((i + 1) / 2 - 1) * s.norb + ((j + 1) / 2 - 1)];
+}
+__device__ __forceinline__ double eri_nz(const Sys& s, int i, int j, int a, int b) {
+  const int ii = i > a ? i : a, aa = i > a ? a : i, jj = j > b ? j : b, bb = j > b ? b : j;
+  const int64_t ia = tri((ii + 1) / 2, (aa + 1) / 2), jb = tri((jj + 1) / 2, (bb + 1) / 2);
+  return s.eri[(ia < jb ? tri(jb, ia) : tri(ia, jb)) - 1];
+}
+__device__ inline double sc1_excit(const Sys& s, const uint64_t* f, int i, int a, bool perm) {
+  double h = h1g(s, i, a);
+  for (int w = 0; w < s.W; ++w) {
+    uint64_t x = f[w];
+    while (x) {
+      const int o = (w << 6) + __builtin_ctzll(x) + 1;
+      x &= x - 1;
+      if (o != i) {
+        h = h + eri_nz(s, i, o, a, o);
+        if (ms_of(o) == ms_of(i)) h = h - eri_nz(s, i, o, o, a);
+      }
+    }
+  }
+  if (perm) h = -h;
+  return h;
+}
+__device__ inline double sc2_excit(const Sys& s, int i, int j, int a, int b, bool perm) {
+  double h = 0.0;
+  if (ms_of(i) == ms_of(a)) h = eri_nz(s, i, j, a, b);
+  if (ms_of(i) == ms_of(b)) h = h - eri_nz(s, i, j, b, a);
+  if (perm) h = -h;
+  return h;
+}
+
+// <D|H|D> in the reference's summation order (slater_condon0_*).
+__device__ inline double diagonal(const Sys& s, const uint64_t* f) {
+  if (s.system == FQ_SYS_HUB_K) {
+    double h = static_cast<double>(s.nalpha * s.nbeta) * s.coulomb_k;
+    for (int w = 0; w < s.W; ++w) {
+      uint64_t x = f[w];
+      while (x) {
+        const int o = (w << 6) + __builtin_ctzll(x) + 1;
+        x &= x - 1;
+        h = h + s.sp_eigv[o - 1];
+      }
+    }
+    return h;
+  }
+  double h = s.ecore;
+  for (int w = 0; w < s.W; ++w) {
+    uint64_t x = f[w];
+    while (x) {
+      const int i = (w << 6) + __builtin_ctzll(x) + 1;
+      x &= x - 1;
+      h = h + h1g(s, i, i);
+      // pairs (i, j) with j after i
+      for (int w2 = w; w2 < s.W; ++w2) {
+        uint64_t y = (w2 == w) ? x : f[w2];
+        while (y) {
+          const int j = (w2 << 6) + __builtin_ctzll(y) + 1;
+          y &= y - 1;
+          h = h + eri_nz(s, i, j, i, j);
+          if (ms_of(i) == ms_of(j)) h = h - eri_nz(s, i, j, j, i);
+        }
+      }
+    }
+  }
+  return h;
+}
+
+// Projected-energy contribution of one determinant (update_proj_energy_*).
+__device__ inline void proj_energy_term(const Sys& s, const uint64_t* f, double pop, double& proj, double& d0) {
+  Excit c = get_excitation(f, s.f0, s.W, s.nel);
+  if (c.nexcit == 0) {
+    d0 = d0 + pop;
+    return;
+  }
+  if (s.system == FQ_SYS_HUB_K) {
+    if (c.nexcit == 2) {
+      double h = two_e_hub_k(s, c.from[0], c.from[1], c.to[0], c.to[1]);
+      if (c.perm) h = -h;
+      proj = proj + h * pop;
+    }
+    return;
+  }
+  if (c.nexcit == 1) {
+    if (ms_of(c.from[0]) == ms_of(c.to[0]) && s.orb_sym[c.from[0] - 1] == s.orb_sym[c.to[0] - 1]) {
+      const double h = sc1_excit(s, f, c.from[0], c.to[0], c.perm);
+      proj = proj + h * pop;
+    }
+  } else if (c.nexcit == 2) {
+    if (ms_of(c.from[0]) + ms_of(c.from[1]) == ms_of(c.to[0]) + ms_of(c.to[1]) &&
+        (s.orb_sym[c.from[0] - 1] ^ s.orb_sym[c.from[1] - 1]) == (s.orb_sym[c.to[0] - 1] ^ s.orb_sym[c.to[1] - 1])) {
+      const double h = sc2_excit(s, c.from[0], c.from[1], c.to[0], c.to[1], c.perm);
+      proj = proj + h * pop;
+    }
+  }
+}
+
+__device__ inline int64_t parent_flag(const Sys& s, const double* pop) {
+  int64_t flag = 0;
+  if (!s.initiator) return 0;
+  const double ip = s.initiator_pop;
+  if (s.quadrature && (s.S % 2) == 0) {
+    for (int i = 0; i < s.S; i += 2) {
+      const double mag = pop[i] * pop[i] + pop[i + 1] * pop[i + 1];
+      if (!(mag > ip * ip)) flag |= (int64_t{1} << i) | (int64_t{1} << (i + 1));
+    }
+  } else {
+    for (int i = 0; i < s.S; ++i)
+      if (!(fabs(pop[i]) > ip)) flag |= int64_t{1} << i;
+  }
+  return flag;
+}
+
+template <class Rng>
+__device__ inline int decide_nattempts(Rng& rng, double population) {
+  int n = abs(static_cast<int>(population));
+  const double pextra = fabs(population) - n;
+  if (fabs(pextra) > kDepsilon) {
+    if (pextra > rng.next()) n = n + 1;
+  }
+  return n;
+}
+
+template <class Rng>
+__device__ inline int64_t stoch_round_spawn(const Sys& s, Rng& rng, double pspawn) {
+  const int64_t cutoff = s.spawn_cutoff;
+  int64_t nspawn;
+  if (pspawn < static_cast<double>(cutoff)) {
+    nspawn = (pspawn > rng.next() * static_cast<double>(cutoff)) ? cutoff : 0;
+  } else {
+    nspawn = static_cast<int64_t>(pspawn);
+    const double padd = pspawn - static_cast<double>(nspawn);
+    if (padd > rng.next()) nspawn = nspawn + 1;
+  }
+  return nspawn;
+}
+
+// stochastic_death (src/death.f90:11-130) with the FCIQMC defaults
+// (quasi-Newton weight 1, pop control 1, dmqmc factor 1, Chebyshev weight 1).
+template <class Rng>
+__device__ inline int64_t death(Rng& rng, double tau, double Kii, double shift, double E, int64_t& population) {
+  const double weight = 1.0, qnpc = 1.0, dmqmc = 1.0, cheby = 1.0;
+  double pd = tau * ((Kii - E) * weight + (E - shift) * qnpc) * dmqmc;
+  pd = pd * cheby;
+  pd = pd * static_cast<double>(population < 0 ? -population : population);
+  int64_t kill = static_cast<int64_t>(pd);
+  pd = pd - static_cast<double>(kill);
+  const double r = rng.next();
+  if (fabs(pd) > r) kill = pd > 0.0 ? kill + 1 : kill - 1;
+  if (population < 0)
+    population = population + kill;
+  else
+    population = population - kill;
+  return kill < 0 ? -kill : kill;
+}
+
+// stochastic_round (src/stoch_utils.f90:51-85) of one population.
+template <class Rng>
+__device__ inline int64_t stoch_round_pop(Rng& rng, int64_t p, int64_t rf) {
+  const int64_t ap = p < 0 ? -p : p;
+  if (ap < rf && p != 0) {
+    const double r = rng.next() * static_cast<double>(rf);
+    return (static_cast<double>(ap) > r) ? fsign64(rf, p) : 0;
+  }
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// Spawning
+// ---------------------------------------------------------------------------
+// Result of one spawning attempt.
+struct SpawnResult {
+  int64_t nspawn;  // encoded, signed
+  uint64_t child[FQ_MAX_WORDS];
+};
+
+__device__ __forceinline__ void excited_det(const uint64_t* f, int W, const Excit& c, uint64_t* out) {
+  for (int w = 0; w < W; ++w) out[w] = f[w];
+  for (int i = 0; i < c.nexcit; ++i) {
+    bclr(out, c.from[i]);
+    bset(out, c.to[i]);
+  }
+}
+
+// Hubbard-k split generator (spawn_lattice_split_gen with gen_excit_init_hub_k /
+// gen_excit_finalise_hub_k, or the no_renorm pair).
+template <class Rng>
+__device__ inline int64_t spawn_hub_k(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
+                                      Excit& conn) {
+  uint64_t occa[FQ_MAX_WORDS], occb[FQ_MAX_WORDS], unoa[FQ_MAX_WORDS];
+  for (int w = 0; w < s.W; ++w) {
+    occa[w] = f[w] & kAlphaBits;
+    occb[w] = f[w] & kBetaBits;
+    unoa[w] = ~f[w] & kAlphaBits & s.valid[w];
+  }
+  // choose_ij_hub_k
+  const double r = rng.next();
+  const int na = s.nalpha, nb = s.nbeta;
+  const int ind = static_cast<int>(r * na * nb) + 1;
+  int i = static_cast<int>((ind - 1.0) / nb) + 1;
+  int j = ind - (i - 1) * nb;
+  i = select_orb(occa, s.W, i - 1);
+  j = select_orb(occb, s.W, j - 1);
+  const int ij_sym = st(s, (i + 1) / 2, (j + 1) / 2);
+  conn.nexcit = 2;
+  conn.from[0] = i;
+  conn.from[1] = j;
+  conn.to[0] = conn.to[1] = 0;
+  conn.perm = false;
+  double pgen, absh;
+  if (s.excit_gen == FQ_EXCIT_NO_RENORM) {
+    const int rr = static_cast<int>(rng.next() * s.nvirt_alpha) + 1;
+    const int a = select_orb(unoa, s.W, rr - 1);
+    const int b = 2 * st(s, ij_sym, s.inv_sym[(a + 1) / 2 - 1]);
+    conn.to[0] = a;
+    conn.to[1] = b;
+    if (!btest(f, b)) {
+      pgen = 1.0 / static_cast<double>(s.nalpha * s.nbeta * s.nvirt_alpha);
+      absh = fabs(s.coulomb_k);
+    } else {
+      pgen = 1.0;
+      absh = 0.0;
+    }
+  } else {
+    int forbidden = 0;
+    for (int w = 0; w < s.W; ++w) {
+      uint64_t x = unoa[w];
+      while (x) {
+        const int a = (w << 6) + __builtin_ctzll(x) + 1;
+        x &= x - 1;
+        const int b = 2 * st(s, ij_sym, s.inv_sym[(a + 1) / 2 - 1]);
+        if (btest(f, b)) ++forbidden;
+      }
+    }
+    pgen = 1.0 / static_cast<double>(s.nalpha * s.nbeta * (s.nvirt_alpha - forbidden));
+    absh = fabs(s.coulomb_k);
+  }
+  int64_t nspawn = stoch_round_spawn(s, rng, static_cast<double>(s.rf) * tau * absh / pgen);
+  if (nspawn != 0) {
+    if (s.excit_gen != FQ_EXCIT_NO_RENORM) {
+      const int sym2 = st(s, (conn.from[0] + 1) / 2, (conn.from[1] + 1) / 2);
+      int a = 0, b = 0;
+      for (int tries = 0; tries < kMaxRejection; ++tries) {
+        const int rr = static_cast<int>(rng.next() * s.nvirt_alpha) + 1;
+        a = select_orb(unoa, s.W, rr - 1);
+        b = 2 * st(s, sym2, s.inv_sym[(a + 1) / 2 - 1]);
+        if (!btest(f, b) || rng.failed) break;
+      }
+      conn.to[0] = a;
+      conn.to[1] = b;
+    }
+    // slater_condon2_hub_k_excit
+    if (conn.from[0] > conn.from[1]) {
+      const int t = conn.from[0];
+      conn.from[0] = conn.from[1];
+      conn.from[1] = t;
+    }
+    if (conn.to[0] > conn.to[1]) {
+      const int t = conn.to[0];
+      conn.to[0] = conn.to[1];
+      conn.to[1] = t;
+    }
+    double h = s.coulomb_k;
+    conn.perm = perm2(f, s.W, conn);
+    if (conn.perm) h = -h;
+    if (((conn.from[0] - conn.to[0]) % 2) != 0) h = -h;
+    nspawn = (h > 0.0) ? -fsign64(nspawn, parent) : fsign64(nspawn, parent);
+  }
+  return nspawn;
+}
+
+// Unoccupied counts per (spin, irrep), packed 8 bits per irrep (<= 8 irreps).
+struct SymUnocc {
+  uint64_t pk[2];  // [0] beta, [1] alpha
+  __device__ int get(int ims, int sym) const { return static_cast<int>((pk[ims - 1] >> (8 * sym)) & 0xff); }
+};
+__device__ inline SymUnocc sym_unocc(const Sys& s, const uint64_t* f) {
+  SymUnocc u;
+  u.pk[0] = u.pk[1] = 0;
+  for (int g = 0; g < s.nsym_pg; ++g) {
+    u.pk[0] |= static_cast<uint64_t>(s.nbss[g]) << (8 * g);
+    u.pk[1] |= static_cast<uint64_t>(s.nbss[s.nsym_pg + g]) << (8 * g);
+  }
+  for (int w = 0; w < s.W; ++w) {
+    uint64_t x = f[w];
+    while (x) {
+      const int o = (w << 6) + __builtin_ctzll(x) + 1;
+      x &= x - 1;
+      u.pk[ims_of(o) - 1] -= uint64_t{1} << (8 * s.orb_sym[o - 1]);
+    }
+  }
+  return u;
+}
+__device__ __forceinline__ int ssbf(const Sys& s, int ims, int sym, int k) {
+  return s.ssbf[((ims - 1) * s.nsym_pg + sym) * s.max_sym_spin + k];
+}
+__device__ __forceinline__ int nbss(const Sys& s, int ims, int sym) { return s.nbss[(ims - 1) * s.nsym_pg + sym]; }
+
+// gen_excit_mol + attempt_to_spawn (renormalised molecular generator).
+template <class Rng>
+__device__ inline int64_t spawn_mol(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
+                                    Excit& conn) {
+  const SymUnocc u = sym_unocc(s, f);
+  double pgen = 1.0, h = 0.0;
+  conn.from[0] = conn.from[1] = conn.to[0] = conn.to[1] = 0;
+  conn.perm = false;
+  const int nel = s.nel;
+  if (rng.next() < s.psingle) {
+    // gen_single_excit_mol / choose_ia_mol
+    conn.nexcit = 1;
+    bool allowed = false;
+    int ni = nel;
+    for (int w = 0; w < s.W; ++w) {
+      uint64_t x = f[w];
+      while (x) {
+        const int o = (w << 6) + __builtin_ctzll(x) + 1;
+        x &= x - 1;
+        if (u.get(ims_of(o), s.orb_sym[o - 1]) != 0)
+          allowed = true;
+        else
+          ni = ni - 1;
+      }
+    }
+    if (allowed) {
+      int i = 0, a = 0;
+      for (int tries = 0; tries < kMaxRejection; ++tries) {
+        i = select_orb(f, s.W, static_cast<int>(rng.next() * nel));
+        const int imsa = ims_of(i), isyma = s.orb_sym[i - 1];
+        if (u.get(imsa, isyma) != 0) {
+          for (int t2 = 0; t2 < kMaxRejection; ++t2) {
+            const int ind = static_cast<int>(nbss(s, imsa, isyma) * rng.next()) + 1;
+            a = ssbf(s, imsa, isyma, ind - 1);
+            if (!btest(f, a) || rng.failed) break;
+          }
+          break;
+        }
+        if (rng.failed) break;
+      }
+      conn.from[0] = i;
+      conn.to[0] = a;
+      pgen = s.psingle * (1.0 / static_cast<double>(ni * u.get(ims_of(a), s.orb_sym[a - 1])));
+      conn.perm = perm1(f, s.W, i, a);
+      h = sc1_excit(s, f, i, a, conn.perm);
+    } else {
+      h = 0.0;
+      pgen = 1.0;
+    }
+  } else {
+    // choose_ij_mol
+    const int ind = static_cast<int>(rng.next() * nel * (nel - 1) / 2) + 1;
+    const int j_ind = static_cast<int>(1.50 + sqrt(2 * ind - 1.750));
+    const int i_ind = ind - ((j_ind - 1) * (j_ind - 2)) / 2;
+    const int i = select_orb(f, s.W, i_ind - 1);
+    const int j = select_orb(f, s.W, j_ind - 1);
+    const int ij_sym = s.orb_sym[i - 1] ^ s.orb_sym[j - 1];
+    const int spin = ms_of(i) + ms_of(j);
+    const double pgen_ij = 2.0 / static_cast<double>(nel * (nel - 1));
+    conn.nexcit = 2;
+    conn.from[0] = i;
+    conn.from[1] = j;
+    // choose_ab_mol
+    bool allowed = false;
+    for (int isyma = 0; isyma < s.nsym_pg && !allowed; ++isyma) {
+      const int isymb = isyma ^ ij_sym;
+      if (spin == -2) {
+        allowed = u.get(1, isyma) > 0 && (u.get(1, isymb) > 1 || (u.get(1, isymb) == 1 && isyma != isymb));
+      } else if (spin == 0) {
+        allowed = (u.get(1, isyma) > 0 && u.get(2, isymb) > 0) || (u.get(2, isyma) > 0 && u.get(1, isymb) > 0);
+      } else {
+        allowed = u.get(2, isyma) > 0 && (u.get(2, isymb) > 1 || (u.get(2, isymb) == 1 && isyma != isymb));
+      }
+    }
+    const int fac = spin == 0 ? 1 : 2, shift = spin == 2 ? 1 : 0, na = spin == 0 ? s.nbasis : s.nbasis / 2;
+    if (allowed) {
+      int a = 0, b = 0;
+      for (int tries = 0; tries < kMaxRejection; ++tries) {
+        a = static_cast<int>(rng.next() * na) + 1;
+        a = fac * a - shift;
+        if (!btest(f, a)) {
+          const int imsb = (spin - ms_of(a) + 3) / 2;
+          const int isymb = ij_sym ^ s.orb_sym[a - 1];
+          const int nu = u.get(imsb, isymb);
+          if (nu > 1 || (nu == 1 && (isymb != s.orb_sym[a - 1] || spin == 0))) {
+            for (int t2 = 0; t2 < kMaxRejection; ++t2) {
+              const int indb = static_cast<int>(nbss(s, imsb, isymb) * rng.next()) + 1;
+              b = ssbf(s, imsb, isymb, indb - 1);
+              if ((b != a && !btest(f, b)) || rng.failed) break;
+            }
+            break;
+          }
+        }
+        if (rng.failed) break;
+      }
+      if (a > b) {
+        const int t = a;
+        a = b;
+        b = t;
+      }
+      conn.to[0] = a;
+      conn.to[1] = b;
+      // calc_pgen_double_mol
+      const int imsa = ims_of(a), imsb = ims_of(b);
+      int n_aij;
+      double p_bija, p_aijb;
+      if (spin != 0) {
+        const int ims = spin == -2 ? 1 : 2;
+        n_aij = spin == -2 ? s.nvirt_beta : s.nvirt_alpha;
+        for (int isyma = 0; isyma < s.nsym_pg; ++isyma) {
+          const int isymb = isyma ^ ij_sym;
+          if (u.get(ims, isymb) == 0)
+            n_aij = n_aij - u.get(ims, isyma);
+          else if (isyma == isymb && u.get(ims, isymb) == 1)
+            n_aij = n_aij - u.get(ims, isyma);
+        }
+        if (s.orb_sym[a - 1] == s.orb_sym[b - 1]) {
+          p_aijb = 1.0 / static_cast<double>(u.get(imsa, s.orb_sym[a - 1]) - 1);
+          p_bija = 1.0 / static_cast<double>(u.get(imsb, s.orb_sym[b - 1]) - 1);
+        } else {
+          p_aijb = 1.0 / static_cast<double>(u.get(imsa, s.orb_sym[a - 1]));
+          p_bija = 1.0 / static_cast<double>(u.get(imsb, s.orb_sym[b - 1]));
+        }
+      } else {
+        n_aij = s.nvirt;
+        for (int isyma = 0; isyma < s.nsym_pg; ++isyma) {
+          const int isymb = isyma ^ ij_sym;
+          if (u.get(1, isymb) == 0) n_aij = n_aij - u.get(2, isyma);
+          if (u.get(2, isymb) == 0) n_aij = n_aij - u.get(1, isyma);
+        }
+        p_aijb = 1.0 / static_cast<double>(u.get(imsa, s.orb_sym[a - 1]));
+        p_bija = 1.0 / static_cast<double>(u.get(imsb, s.orb_sym[b - 1]));
+      }
+      const double pd = (1.0 / static_cast<double>(n_aij)) * (p_bija + p_aijb);
+      pgen = s.pdouble * pgen_ij * pd;
+      conn.perm = perm2(f, s.W, conn);
+      h = sc2_excit(s, conn.from[0], conn.from[1], conn.to[0], conn.to[1], conn.perm);
+    } else {
+      h = 0.0;
+      pgen = 1.0;
+    }
+  }
+  // attempt_to_spawn (src/spawning.F90:711-766)
+  double pspawn = tau * fabs(h) / pgen;
+  pspawn = pspawn * static_cast<double>(s.rf);
+  int64_t nspawn = stoch_round_spawn(s, rng, pspawn);
+  if (nspawn > 0) nspawn = (h > 0.0) ? -fsign64(nspawn, parent) : fsign64(nspawn, parent);
+  return nspawn;
+}
+
+template <class Rng>
+__device__ inline int64_t spawn_attempt(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
+                                        uint64_t* child) {
+  Excit conn;
+  const int64_t n = (s.system == FQ_SYS_HUB_K) ? spawn_hub_k(s, rng, f, parent, tau, conn)
+                                               : spawn_mol(s, rng, f, parent, tau, conn);
+  if (n != 0) excited_det(f, s.W, conn, child);
+  return n;
+}
+
+}  // namespace fqd

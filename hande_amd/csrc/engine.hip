@@ -1,0 +1,1506 @@
+// MI355X (gfx950) FCIQMC walker-propagation engine: implementation of the
+// C-ABI in include/fqmc.h.
+//
+// One call to fq_cycle runs the body of the reference's Monte Carlo cycle
+// (src/fciqmc.f90:307-399) on the device:
+//   spawning + death      production: k_det_pass (one thread per determinant:
+//                         projected energy, initiator flag, decide_nattempts,
+//                         stochastic death) then k_spawn (one thread per
+//                         spawning attempt, parents found by binary search of
+//                         the attempt prefix sum, slots by wavefront ballot +
+//                         one atomic per wave).
+//                         replay: k_replay_walk, one thread walking the list
+//                         in the reference's order and consuming the uploaded
+//                         dSFMT stream exactly as get_rand_close_open does.
+//   sort                  production: LDS-staged LSD radix sort of the spawned
+//                         bit strings (8-bit digits, last word most significant).
+//                         replay: the reference's quicksort (src/sort.f90:213-396)
+//                         on one thread, so equal keys keep its order.
+//   self-annihilation     k_reduce_runs: one thread per run of equal keys,
+//                         plain and initiator rules (src/spawn_data.F90:859-1101).
+//   merge with main list  k_merge_runs: binary search of each unique key in the
+//                         sorted main list (src/annihilation.f90:294-486).
+//   rounding, compaction  k_round_main / k_round_new: stochastic rounding of small
+//                         real populations (src/annihilation.f90:537-675), draws
+//                         from Philox or from the replay stream at scan offsets.
+//   insertion             k_scatter_main / k_scatter_new: merge-path placement
+//                         of surviving main entries and new determinants into the
+//                         other buffer, new K_ii from the Slater--Condon diagonal
+//                         (src/annihilation.f90:677-901).
+// Memory layout (HBM, structure of arrays): determinant word w of entry i at
+// det[w*cap + i]; population of space s at pop[s*cap + i]; K_ii at kii[i].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "device_common.hpp"
+#include "fqmc.h"
+
+using fqd::Sys;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRadixItems = 8;
+constexpr int kRadixTile = kBlock * kRadixItems;
+
+#define FQ_HIP(call)                                  \
+  do {                                                \
+    hipError_t err_ = (call);                         \
+    if (err_ != hipSuccess) {                         \
+      fprintf(stderr, "fqmc: %s failed: %s (%s:%d)\n", #call, hipGetErrorString(err_), __FILE__, __LINE__); \
+      return FQ_ERR_DEVICE;                           \
+    }                                                 \
+  } while (0)
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline unsigned grid_for(int64_t n, int64_t per_block = kBlock) {
+  int64_t g = ceil_div(n, per_block);
+  if (g < 1) g = 1;
+  return static_cast<unsigned>(g);
+}
+
+// Per-cycle device counters.
+struct Stats {
+  double proj[FQ_MAX_SPACES];
+  double d0[FQ_MAX_SPACES];
+  int64_t ndeath;
+  int64_t nspawn;          // spawned entries requested (may exceed capacity)
+  int64_t spawn_overflow;
+  int64_t total_attempts;
+  int64_t nruns;
+  int64_t new_n;
+  int64_t main_overflow;
+  int64_t rng_pos;
+  int64_t rng_failed;
+  unsigned long long abs_pop[FQ_MAX_SPACES];  // sum |pop| after annihilation (encoded)
+};
+
+// ---------------------------------------------------------------------------
+// Prefix sums (reduce-then-scan, three kernels per level)
+// ---------------------------------------------------------------------------
+__global__ void k_block_reduce(const int64_t* __restrict__ in, int64_t n, int64_t* __restrict__ sums) {
+  __shared__ int64_t sh[kBlock];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kBlock * 4;
+  int64_t acc = 0;
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = base + k * kBlock + threadIdx.x;
+    if (i < n) acc += in[i];
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = sh[0];
+}
+
+// Exclusive scan of a 1024-element tile with an added per-block offset.
+__global__ void k_block_scan(const int64_t* __restrict__ in, int64_t n, const int64_t* __restrict__ offsets,
+                             int64_t* __restrict__ out) {
+  __shared__ int64_t sh[kBlock];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kBlock * 4;
+  int64_t v[4];
+  int64_t local = 0;
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = base + threadIdx.x * 4 + k;
+    v[k] = (i < n) ? in[i] : 0;
+    local += v[k];
+  }
+  sh[threadIdx.x] = local;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the per-thread sums.
+  for (int s = 1; s < kBlock; s <<= 1) {
+    int64_t add = (threadIdx.x >= s) ? sh[threadIdx.x - s] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  int64_t run = sh[threadIdx.x] - local + (offsets ? offsets[blockIdx.x] : 0);
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = base + threadIdx.x * 4 + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+  }
+}
+
+__global__ void k_set_total(const int64_t* __restrict__ in, int64_t n, int64_t* __restrict__ out) {
+  // out[n] = out[n-1] + in[n-1]
+  if (n == 0)
+    out[0] = 0;
+  else
+    out[n] = out[n - 1] + in[n - 1];
+}
+
+}  // namespace
+
+struct fq_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Sys sys{};
+  fq_params par{};
+  int W = 1, S = 1;
+  // device tables
+  int *d_sym_table = nullptr, *d_inv_sym = nullptr, *d_orb_sym = nullptr, *d_nbss = nullptr, *d_ssbf = nullptr;
+  double *d_sp_eigv = nullptr, *d_h1 = nullptr, *d_eri = nullptr;
+  // main list (double-buffered)
+  int64_t cap = 0, n = 0;
+  int cur = 0;
+  uint64_t* det[2] = {nullptr, nullptr};
+  int64_t* pop[2] = {nullptr, nullptr};
+  double* kii[2] = {nullptr, nullptr};
+  int64_t* pop_new = nullptr;   // S*cap, populations after death
+  int64_t* nat = nullptr;       // S*cap attempts per (det, space), then scanned
+  int64_t* att_off = nullptr;   // S*cap+1
+  int64_t* flags = nullptr;     // cap parent initiator flags
+  int64_t* msurv = nullptr;     // cap+1
+  int64_t* mrank = nullptr;     // cap+1
+  int64_t* ins = nullptr;       // cap+2
+  int64_t* ins_scan = nullptr;  // cap+2
+  int64_t* mdraw = nullptr;     // cap+1 (replay rounding draws)
+  int64_t* mdraw_off = nullptr;
+  // spawned list
+  int64_t scap = 0;
+  uint64_t* sdet = nullptr;  // W*scap
+  int64_t* spop = nullptr;   // S*scap
+  int64_t* sflag = nullptr;  // scap
+  uint64_t* sord = nullptr;  // scap attempt ordinal
+  // sort buffers
+  uint64_t* kbuf[2] = {nullptr, nullptr};  // W*scap
+  uint32_t* vbuf[2] = {nullptr, nullptr};  // scap
+  int64_t* hist = nullptr;                 // 256*nb+1
+  int64_t* hist_scan = nullptr;
+  int64_t hist_cap = 0;
+  // runs
+  int64_t* head = nullptr;      // scap+1
+  int64_t* head_scan = nullptr;  // scap+1
+  int64_t* run_start = nullptr;  // scap+1
+  uint64_t* rdet = nullptr;      // W*scap
+  int64_t* rpop = nullptr;       // S*scap
+  int64_t* rflag = nullptr;      // scap
+  int64_t* rpos = nullptr;       // scap
+  int64_t* rstate = nullptr;     // scap: 0 dead, 1 hit, 2 new candidate
+  int64_t* nsurv = nullptr;      // scap+1
+  int64_t* nrank = nullptr;      // scap+1
+  int64_t* ndraw = nullptr;      // scap+1
+  int64_t* ndraw_off = nullptr;
+  // scan scratch
+  std::vector<int64_t*> scan_tmp;
+  std::vector<int64_t> scan_tmp_cap;
+  // replay stream
+  double* rng_buf = nullptr;
+  int64_t rng_cap = 0, rng_pos = 0, rng_end = 0;
+  // per-cycle
+  Stats* d_stats = nullptr;
+  Stats h_stats{};
+  uint64_t cycle_index = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float t_spawn = 0, t_annih = 0, t_total = 0;
+};
+
+namespace {
+
+// ---- device allocation helpers ---------------------------------------------
+template <class T>
+int dalloc(T** p, int64_t count) {
+  if (count <= 0) count = 1;
+  if (hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * static_cast<size_t>(count)) != hipSuccess) {
+    *p = nullptr;
+    return FQ_ERR_NOMEM;
+  }
+  return FQ_OK;
+}
+template <class T>
+void dfree(T*& p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+// Scratch buffer `slot` of at least `need` elements (grown on demand).
+int64_t* scan_scratch(fq_engine* e, int slot, int64_t need) {
+  while (static_cast<int>(e->scan_tmp.size()) <= slot) {
+    e->scan_tmp.push_back(nullptr);
+    e->scan_tmp_cap.push_back(0);
+  }
+  if (e->scan_tmp_cap[slot] < need) {
+    dfree(e->scan_tmp[slot]);
+    if (dalloc(&e->scan_tmp[slot], need) != FQ_OK) {
+      e->scan_tmp_cap[slot] = 0;
+      return nullptr;
+    }
+    e->scan_tmp_cap[slot] = need;
+  }
+  return e->scan_tmp[slot];
+}
+
+// Exclusive scan: out[0..n] with out[n] = total.  Tiles of 1024 elements;
+// block totals are scanned recursively.
+int scan_exclusive(fq_engine* e, const int64_t* in, int64_t* out, int64_t n, int level = 0) {
+  if (n == 0) {
+    hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, e->stream, in, 0, out);
+    return FQ_OK;
+  }
+  const int64_t tile = kBlock * 4;
+  const int64_t nb = ceil_div(n, tile);
+  if (nb == 1) {
+    hipLaunchKernelGGL(k_block_scan, dim3(1), dim3(kBlock), 0, e->stream, in, n, nullptr, out);
+  } else {
+    int64_t* sums = scan_scratch(e, 2 * level, nb);
+    int64_t* sums_scan = scan_scratch(e, 2 * level + 1, nb + 1);
+    if (!sums || !sums_scan) return FQ_ERR_NOMEM;
+    hipLaunchKernelGGL(k_block_reduce, dim3(nb), dim3(kBlock), 0, e->stream, in, n, sums);
+    int rc = scan_exclusive(e, sums, sums_scan, nb, level + 1);
+    if (rc != FQ_OK) return rc;
+    hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(kBlock), 0, e->stream, in, n, sums_scan, out);
+  }
+  hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, e->stream, in, n, out);
+  return FQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels: spawning and death
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_det(const uint64_t* det, int64_t cap, int W, int64_t i, uint64_t* f) {
+  for (int w = 0; w < W; ++w) f[w] = det[w * cap + i];
+  for (int w = W; w < FQ_MAX_WORDS; ++w) f[w] = 0;
+}
+
+// Production: per determinant (loop body of src/fciqmc.f90:315-374 minus the
+// spawning attempts).  Projected-energy partials are reduced per block and
+// summed in block order by k_sum_partials (deterministic).
+__global__ __launch_bounds__(kBlock) void k_det_pass(Sys s, int64_t n, int64_t cap, const uint64_t* __restrict__ det,
+                                                     const int64_t* __restrict__ pop, const double* __restrict__ kii,
+                                                     int64_t* __restrict__ pop_new, int64_t* __restrict__ nat,
+                                                     int64_t* __restrict__ flags, double tau, double sh0, double sh1,
+                                                     double e0, double e1, uint64_t seed, uint32_t cyc,
+                                                     double* __restrict__ partials, Stats* __restrict__ st) {
+  __shared__ double sh_proj[FQ_MAX_SPACES][kBlock];
+  __shared__ double sh_d0[FQ_MAX_SPACES][kBlock];
+  __shared__ unsigned long long sh_death[kBlock];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  double proj[FQ_MAX_SPACES] = {0.0, 0.0}, d0[FQ_MAX_SPACES] = {0.0, 0.0};
+  unsigned long long ndeath = 0;
+  if (i < n) {
+    uint64_t f[FQ_MAX_WORDS];
+    load_det(det, cap, s.W, i, f);
+    double rp[FQ_MAX_SPACES] = {0.0, 0.0};
+    int64_t p[FQ_MAX_SPACES] = {0, 0};
+    for (int sp = 0; sp < s.S; ++sp) {
+      p[sp] = pop[sp * cap + i];
+      rp[sp] = static_cast<double>(p[sp]) / static_cast<double>(s.rf);
+    }
+    flags[i] = fqd::parent_flag(s, rp);
+    const double K = kii[i];
+    for (int sp = 0; sp < s.S; ++sp) {
+      fqd::proj_energy_term(s, f, rp[sp], proj[sp], d0[sp]);
+      fqd::Philox rng(seed, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32) ^ (sp << 28), cyc);
+      rng.c2 = cyc ^ (fqd::kDrawNattempts << 24);
+      nat[i * s.S + sp] = fqd::decide_nattempts(rng, rp[sp]);
+      fqd::Philox rd(seed, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32) ^ (sp << 28),
+                     cyc ^ (fqd::kDrawDeath << 24));
+      int64_t q = p[sp];
+      ndeath += static_cast<unsigned long long>(
+          fqd::death(rd, tau, K, sp == 0 ? sh0 : sh1, sp == 0 ? e0 : e1, q));
+      pop_new[sp * cap + i] = q;
+    }
+  }
+  for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
+    sh_proj[sp][threadIdx.x] = proj[sp];
+    sh_d0[sp][threadIdx.x] = d0[sp];
+  }
+  sh_death[threadIdx.x] = ndeath;
+  __syncthreads();
+  for (int k = kBlock / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
+        sh_proj[sp][threadIdx.x] += sh_proj[sp][threadIdx.x + k];
+        sh_d0[sp][threadIdx.x] += sh_d0[sp][threadIdx.x + k];
+      }
+      sh_death[threadIdx.x] += sh_death[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
+      partials[(2 * sp) * gridDim.x + blockIdx.x] = sh_proj[sp][0];
+      partials[(2 * sp + 1) * gridDim.x + blockIdx.x] = sh_d0[sp][0];
+    }
+    atomicAdd(reinterpret_cast<unsigned long long*>(&st->ndeath), sh_death[0]);
+  }
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partials, int nb, Stats* __restrict__ st) {
+  // One thread per (quantity, space): ordered sum over blocks.
+  const int q = threadIdx.x;  // 0..3: proj0, d00, proj1, d01
+  if (q >= 2 * FQ_MAX_SPACES) return;
+  double acc = 0.0;
+  for (int b = 0; b < nb; ++b) acc = acc + partials[q * nb + b];
+  const int sp = q / 2;
+  if (q % 2 == 0)
+    st->proj[sp] = acc;
+  else
+    st->d0[sp] = acc;
+}
+
+// Production: one thread per spawning attempt (grid-stride).  Parent entry =
+// last e with att_off[e] <= t (binary search over the n*S (det, space)
+// entries, det-major).  Successful children get slots by wavefront ballot and
+// one atomic per wave.
+__global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t cap, int64_t total,
+                                                  const int64_t* __restrict__ att_off,
+                                                  const uint64_t* __restrict__ det, const int64_t* __restrict__ pop,
+                                                  const int64_t* __restrict__ flags, double tau, uint64_t seed,
+                                                  uint32_t cyc, int64_t scap, uint64_t* __restrict__ sdet,
+                                                  int64_t* __restrict__ spop, int64_t* __restrict__ sflag,
+                                                  uint64_t* __restrict__ sord, Stats* __restrict__ st) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  // Every lane of a wave iterates the same number of times so ballots stay uniform.
+  const int64_t iters = (total + stride - 1) / stride;
+  const int lane = threadIdx.x & 63;
+  for (int64_t it = 0; it < iters; ++it) {
+    const int64_t t = t0 + it * stride;
+    int64_t nspawn = 0;
+    uint64_t child[FQ_MAX_WORDS] = {0, 0};
+    int64_t e = 0;
+    if (t < total) {
+      int64_t lo = 0, hi = nent - 1;
+      while (lo < hi) {  // last e with att_off[e] <= t
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (att_off[mid] <= t)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      e = lo;
+      const int sp = static_cast<int>(e % s.S);
+      const int64_t i = e / s.S;
+      uint64_t f[FQ_MAX_WORDS];
+      load_det(det, cap, s.W, i, f);
+      const int64_t parent = pop[sp * cap + i];
+      fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
+                      cyc ^ (fqd::kDrawAttempt << 24));
+      nspawn = fqd::spawn_attempt(s, rng, f, parent, tau, child);
+    }
+    const bool ok = nspawn != 0;
+    const unsigned long long mask = __ballot(ok);
+    if (mask == 0) continue;
+    const int cnt = __popcll(mask);
+    const int leader = __builtin_ctzll(mask);
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(reinterpret_cast<unsigned long long*>(&st->nspawn), cnt);
+    base = __shfl(base, leader);
+    if (ok) {
+      const int64_t slot = static_cast<int64_t>(base) + __popcll(mask & ((1ull << lane) - 1));
+      if (slot < scap) {
+        const int sp = static_cast<int>(e % s.S);
+        const int64_t i = e / s.S;
+        for (int w = 0; w < s.W; ++w) sdet[w * scap + slot] = child[w];
+        for (int q = 0; q < s.S; ++q) spop[q * scap + slot] = (q == sp) ? nspawn : 0;
+        sflag[slot] = s.initiator ? flags[i] : 0;
+        sord[slot] = static_cast<uint64_t>(t);
+      } else {
+        atomicExch(reinterpret_cast<unsigned long long*>(&st->spawn_overflow), 1ull);
+      }
+    }
+  }
+}
+
+// Replay: the whole spawning/death loop on one thread, in reference order.
+__global__ void k_replay_walk(Sys s, int64_t n, int64_t cap, const uint64_t* __restrict__ det,
+                              const int64_t* __restrict__ pop, const double* __restrict__ kii,
+                              int64_t* __restrict__ pop_new, double tau, double sh0, double sh1, double e0,
+                              double e1, const double* __restrict__ rbuf, int64_t rpos, int64_t rend, int64_t scap,
+                              uint64_t* __restrict__ sdet, int64_t* __restrict__ spop, int64_t* __restrict__ sflag,
+                              uint64_t* __restrict__ sord, Stats* __restrict__ st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  fqd::ReplayRng rng{rbuf, rpos, rend, false};
+  double proj[FQ_MAX_SPACES] = {0.0, 0.0}, d0[FQ_MAX_SPACES] = {0.0, 0.0};
+  int64_t ndeath = 0, nspawn_total = 0, attempts = 0;
+  bool overflow = false;
+  for (int64_t i = 0; i < n && !rng.failed; ++i) {
+    uint64_t f[FQ_MAX_WORDS];
+    load_det(det, cap, s.W, i, f);
+    double rp[FQ_MAX_SPACES] = {0.0, 0.0};
+    int64_t p[FQ_MAX_SPACES] = {0, 0};
+    for (int sp = 0; sp < s.S; ++sp) {
+      p[sp] = pop[sp * cap + i];
+      rp[sp] = static_cast<double>(p[sp]) / static_cast<double>(s.rf);
+    }
+    const int64_t flag = fqd::parent_flag(s, rp);
+    for (int sp = 0; sp < s.S; ++sp) {
+      fqd::proj_energy_term(s, f, rp[sp], proj[sp], d0[sp]);
+      const int nat = fqd::decide_nattempts(rng, rp[sp]);
+      attempts += nat;
+      for (int a = 0; a < nat && !rng.failed; ++a) {
+        uint64_t child[FQ_MAX_WORDS] = {0, 0};
+        const int64_t ns = fqd::spawn_attempt(s, rng, f, p[sp], tau, child);
+        if (ns != 0) {
+          if (nspawn_total + 1 > scap) {
+            overflow = true;
+          } else {
+            const int64_t slot = nspawn_total++;
+            for (int w = 0; w < s.W; ++w) sdet[w * scap + slot] = child[w];
+            for (int q = 0; q < s.S; ++q) spop[q * scap + slot] = (q == sp) ? ns : 0;
+            sflag[slot] = s.initiator ? flag : 0;
+            sord[slot] = static_cast<uint64_t>(slot);
+          }
+        }
+      }
+      int64_t q = p[sp];
+      ndeath += fqd::death(rng, tau, kii[i], sp == 0 ? sh0 : sh1, sp == 0 ? e0 : e1, q);
+      p[sp] = q;
+      pop_new[sp * cap + i] = q;
+    }
+  }
+  for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
+    st->proj[sp] = proj[sp];
+    st->d0[sp] = d0[sp];
+  }
+  st->ndeath = ndeath;
+  st->nspawn = nspawn_total;
+  st->spawn_overflow = overflow ? 1 : 0;
+  st->total_attempts = attempts;
+  st->rng_pos = rng.pos;
+  st->rng_failed = rng.failed ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels: sorting
+// ---------------------------------------------------------------------------
+// Unsigned multi-word comparison, last word most significant (bit_str_cmp).
+__device__ __forceinline__ int key_cmp(const uint64_t* keys, int64_t stride, int W, int64_t a, int64_t b) {
+  for (int w = W - 1; w >= 0; --w) {
+    const uint64_t x = keys[w * stride + a], y = keys[w * stride + b];
+    if (x < y) return -1;
+    if (x > y) return 1;
+  }
+  return 0;
+}
+
+// Radix pass histogram: hist[d*nb + block] = count of digit d in the block's tile.
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* __restrict__ kw, int64_t n, int shift,
+                                                       int64_t* __restrict__ hist) {
+  __shared__ int cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRadixTile;
+  for (int r = 0; r < kRadixItems; ++r) {
+    const int64_t i = base + r * kBlock + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[(kw[i] >> shift) & 0xff], 1);
+  }
+  __syncthreads();
+  hist[static_cast<int64_t>(threadIdx.x) * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Radix pass scatter, stable: tiles are processed in rounds of 256 elements in
+// input order; within a round, ranks come from 8 wavefront ballots (lanes with
+// equal digits) and per-wave digit counts in LDS.
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                          int64_t n, int W, int w_digit, int shift,
+                                                          const int64_t* __restrict__ hist_scan,
+                                                          uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                          int64_t stride) {
+  __shared__ int64_t base_d[256];
+  __shared__ int wcnt[kBlock / 64][256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  base_d[tid] = hist_scan[static_cast<int64_t>(tid) * gridDim.x + blockIdx.x];
+  const int64_t tbase = static_cast<int64_t>(blockIdx.x) * kRadixTile;
+  for (int r = 0; r < kRadixItems; ++r) {
+    for (int k = tid; k < (kBlock / 64) * 256; k += kBlock) (&wcnt[0][0])[k] = 0;
+    __syncthreads();
+    const int64_t i = tbase + r * kBlock + tid;
+    const bool valid = i < n;
+    uint64_t kv[FQ_MAX_WORDS] = {0, 0};
+    uint32_t v = 0;
+    int digit = 0;
+    if (valid) {
+      for (int w = 0; w < W; ++w) kv[w] = kin[w * stride + i];
+      v = vin[i];
+      digit = static_cast<int>((kv[w_digit] >> shift) & 0xff);
+    }
+    unsigned long long peers = __ballot(valid);
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (digit >> b) & 1;
+      const unsigned long long m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const int rank = __popcll(peers & ((1ull << lane) - 1));
+    if (valid && rank == 0) wcnt[wave][digit] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int64_t pos = base_d[digit] + rank;
+      for (int ww = 0; ww < wave; ++ww) pos += wcnt[ww][digit];
+      for (int w = 0; w < W; ++w) kout[w * stride + pos] = kv[w];
+      vout[pos] = v;
+    }
+    __syncthreads();
+    int add = 0;
+    for (int ww = 0; ww < kBlock / 64; ++ww) add += wcnt[ww][tid];
+    base_d[tid] += add;
+    __syncthreads();
+  }
+}
+
+__global__ void k_iota_copy_keys(const uint64_t* __restrict__ sdet, int64_t ns, int W, int64_t scap,
+                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  for (int w = 0; w < W; ++w) keys[w * scap + i] = sdet[w * scap + i];
+  vals[i] = static_cast<uint32_t>(i);
+}
+
+// Replay: the reference's quicksort (src/sort.f90:213-396) on an index array,
+// one thread; comparisons read the spawned bit strings.
+__global__ void k_replay_qsort(const uint64_t* __restrict__ sdet, int64_t ns, int W, int64_t scap,
+                               uint32_t* __restrict__ idx, uint64_t* __restrict__ keys_sorted) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int64_t i = 0; i < ns; ++i) idx[i] = static_cast<uint32_t>(i);
+  auto gt = [&](uint32_t a, uint32_t b) { return key_cmp(sdet, scap, W, a, b) > 0; };
+  auto ge = [&](uint32_t a, uint32_t b) { return key_cmp(sdet, scap, W, a, b) >= 0; };
+  // 1-based view of idx
+  auto L = [&](int64_t k) -> uint32_t& { return idx[k - 1]; };
+  int64_t stack_lo[64], stack_hi[64];
+  int nstack = 0;
+  int64_t lo = 1, hi = ns;
+  while (true) {
+    if (hi - lo < 7) {
+      for (int64_t j = lo + 1; j <= hi; ++j) {
+        const uint32_t tmp = L(j);
+        int64_t i;
+        for (i = j - 1; i >= 1; --i) {
+          if (ge(tmp, L(i))) break;
+          L(i + 1) = L(i);
+        }
+        L(i + 1) = tmp;
+      }
+      if (nstack == 0) break;
+      hi = stack_hi[nstack - 1];
+      lo = stack_lo[nstack - 1];
+      --nstack;
+    } else {
+      const int64_t pivot = (lo + hi) / 2;
+      uint32_t t = L(pivot);
+      L(pivot) = L(lo + 1);
+      L(lo + 1) = t;
+      if (gt(L(lo), L(hi))) {
+        t = L(lo);
+        L(lo) = L(hi);
+        L(hi) = t;
+      }
+      if (gt(L(lo + 1), L(hi))) {
+        t = L(lo + 1);
+        L(lo + 1) = L(hi);
+        L(hi) = t;
+      }
+      if (gt(L(lo), L(lo + 1))) {
+        t = L(lo);
+        L(lo) = L(lo + 1);
+        L(lo + 1) = t;
+      }
+      int64_t i = lo + 1, j = hi;
+      const uint32_t piv = L(lo + 1);
+      while (true) {
+        ++i;
+        while (gt(piv, L(i))) ++i;
+        --j;
+        while (gt(L(j), piv)) --j;
+        if (j < i) break;
+        t = L(i);
+        L(i) = L(j);
+        L(j) = t;
+      }
+      L(lo + 1) = L(j);
+      L(j) = piv;
+      ++nstack;
+      if (hi - i + 1 >= j - lo) {
+        stack_hi[nstack - 1] = hi;
+        stack_lo[nstack - 1] = i;
+        hi = j - 1;
+      } else {
+        stack_hi[nstack - 1] = j - 1;
+        stack_lo[nstack - 1] = lo;
+        lo = i;
+      }
+    }
+  }
+  for (int64_t i = 0; i < ns; ++i)
+    for (int w = 0; w < W; ++w) keys_sorted[w * scap + i] = sdet[w * scap + idx[i]];
+}
+
+// ---------------------------------------------------------------------------
+// Kernels: self-annihilation of the sorted spawned list
+// ---------------------------------------------------------------------------
+__global__ void k_heads(const uint64_t* __restrict__ keys, int64_t ns, int W, int64_t scap, int64_t* __restrict__ head) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  head[i] = (i == 0 || key_cmp(keys, scap, W, i, i - 1) != 0) ? 1 : 0;
+}
+
+__global__ void k_run_starts(const int64_t* __restrict__ head, const int64_t* __restrict__ head_scan, int64_t ns,
+                             int64_t* __restrict__ run_start) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < ns && head[i]) run_start[head_scan[i]] = i;
+  if (i == 0) run_start[head_scan[ns]] = ns;
+}
+
+// One thread per run of equal keys: annihilate_spawn_t(_initiator).  In replay
+// mode the first entry is the first in sorted order (the reference's); in
+// production it is the entry with the smallest attempt ordinal.
+__global__ void k_reduce_runs(Sys s, int64_t nruns, const int64_t* __restrict__ run_start,
+                              const uint64_t* __restrict__ keys, const uint32_t* __restrict__ perm,
+                              const int64_t* __restrict__ spop, const int64_t* __restrict__ sflag,
+                              const uint64_t* __restrict__ sord, int64_t scap, int first_by_order,
+                              uint64_t* __restrict__ rdet, int64_t* __restrict__ rpop, int64_t* __restrict__ rflag,
+                              int64_t* __restrict__ rstate) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  const int64_t a = run_start[r], b = run_start[r + 1];
+  int64_t first = a;
+  if (!first_by_order) {
+    uint64_t best = sord[perm[a]];
+    for (int64_t k = a + 1; k < b; ++k) {
+      const uint64_t o = sord[perm[k]];
+      if (o < best) {
+        best = o;
+        first = k;
+      }
+    }
+  }
+  int64_t tot[FQ_MAX_SPACES] = {0, 0}, init[FQ_MAX_SPACES] = {0, 0};
+  int events[FQ_MAX_SPACES] = {0, 0};
+  for (int sp = 0; sp < s.S; ++sp) {
+    const uint32_t j = perm[first];
+    const int64_t v = spop[sp * scap + j];
+    tot[sp] = v;
+    if (!((sflag[j] >> sp) & 1)) {
+      init[sp] = v;
+    } else {
+      const int32_t v32 = static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(v) & 0xffffffffULL));
+      events[sp] = v32 >= 0 ? 1 : -1;
+    }
+  }
+  for (int64_t k = a; k < b; ++k) {
+    if (k == first) continue;
+    const uint32_t j = perm[k];
+    for (int sp = 0; sp < s.S; ++sp) {
+      const int64_t v = spop[sp * scap + j];
+      if (!((sflag[j] >> sp) & 1))
+        init[sp] += v;
+      else if (v < 0)
+        events[sp] -= 1;
+      else if (v > 0)
+        events[sp] += 1;
+      tot[sp] += v;
+    }
+  }
+  int64_t fl = 0;
+  bool any = false;
+  for (int sp = 0; sp < s.S; ++sp) {
+    any = any || tot[sp] != 0;
+    if (s.initiator) {
+      const int st = tot[sp] >= 0 ? 1 : -1, si = init[sp] >= 0 ? 1 : -1;
+      if (init[sp] != 0 && st == si) {
+      } else if (abs(events[sp]) > 1) {
+      } else {
+        fl += int64_t{1} << sp;
+      }
+    }
+    rpop[sp * scap + r] = tot[sp];
+  }
+  for (int w = 0; w < s.W; ++w) rdet[w * scap + r] = keys[w * scap + a];
+  rflag[r] = fl;
+  rstate[r] = any ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels: merge with the main list, rounding, insertion
+// ---------------------------------------------------------------------------
+// Per surviving run: binary search of its key in the (old) main list; a hit
+// adds into the post-death populations (annihilate_main_list(_initiator)), a
+// miss becomes a new-determinant candidate with flagged spaces cleared.
+__global__ void k_merge_runs(Sys s, int64_t nruns, int64_t n, int64_t cap, const uint64_t* __restrict__ det,
+                             int64_t* __restrict__ pop_new, const uint64_t* __restrict__ rdet,
+                             int64_t* __restrict__ rpop, const int64_t* __restrict__ rflag, int64_t scap,
+                             int64_t* __restrict__ rpos, int64_t* __restrict__ rstate) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  if (rstate[r] == 0) {
+    rpos[r] = 0;
+    return;
+  }
+  uint64_t k[FQ_MAX_WORDS] = {0, 0};
+  for (int w = 0; w < s.W; ++w) k[w] = rdet[w * scap + r];
+  // lower bound in [0, n)
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    int c = 0;
+    for (int w = s.W - 1; w >= 0 && c == 0; --w) {
+      const uint64_t x = det[w * cap + mid];
+      c = (x < k[w]) ? -1 : (x > k[w] ? 1 : 0);
+    }
+    if (c < 0)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  bool hit = false;
+  if (lo < n) {
+    hit = true;
+    for (int w = 0; w < s.W; ++w) hit = hit && det[w * cap + lo] == k[w];
+  }
+  rpos[r] = lo;
+  const int64_t fl = rflag[r];
+  if (hit) {
+    for (int sp = 0; sp < s.S; ++sp) {
+      int64_t& m = pop_new[sp * cap + lo];
+      const int64_t v = rpop[sp * scap + r];
+      if (!s.initiator)
+        m += v;
+      else if (m != 0)
+        m += v;
+      else if (!((fl >> sp) & 1))
+        m = v;
+    }
+    rstate[r] = 3;  // annihilated into the main list
+  } else {
+    if (s.initiator) {
+      bool discard = true;
+      for (int sp = 0; sp < s.S; ++sp) {
+        if ((fl >> sp) & 1)
+          rpop[sp * scap + r] = 0;
+        else
+          discard = false;
+      }
+      if (discard) {
+        rstate[r] = 0;
+        return;
+      }
+    }
+    rstate[r] = 2;  // new determinant candidate
+  }
+}
+
+// Number of rounding draws an entry needs: spaces with 0 < |pop| < rf.
+__device__ __forceinline__ int round_draws(const int64_t* p, int64_t stride, int64_t i, int S, int64_t rf) {
+  int c = 0;
+  for (int sp = 0; sp < S; ++sp) {
+    const int64_t v = p[sp * stride + i];
+    const int64_t av = v < 0 ? -v : v;
+    if (av < rf && v != 0) ++c;
+  }
+  return c;
+}
+
+__global__ void k_count_round_main(Sys s, int64_t n, int64_t cap, const int64_t* __restrict__ pop_new,
+                                   int64_t* __restrict__ mdraw) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  mdraw[i] = s.real_amps ? round_draws(pop_new, cap, i, s.S, s.rf) : 0;
+}
+__global__ void k_count_round_new(Sys s, int64_t nruns, const int64_t* __restrict__ rpop,
+                                  const int64_t* __restrict__ rstate, int64_t scap, int64_t* __restrict__ ndraw) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  ndraw[r] = (s.real_amps && rstate[r] == 2) ? round_draws(rpop, scap, r, s.S, s.rf) : 0;
+}
+
+// remove_unoccupied_dets: round small real populations, mark survivors.
+__global__ void k_round_main(Sys s, int64_t n, int64_t cap, int64_t* __restrict__ pop_new, int replay,
+                             const double* __restrict__ rbuf, int64_t rbase, const int64_t* __restrict__ mdraw_off,
+                             uint64_t seed, uint32_t cyc, int64_t* __restrict__ msurv) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool alive = false;
+  if (s.real_amps) {
+    if (replay) {
+      fqd::ReplayRng rng{rbuf, rbase + mdraw_off[i], rbase + mdraw_off[i + 1], false};
+      for (int sp = 0; sp < s.S; ++sp) pop_new[sp * cap + i] = fqd::stoch_round_pop(rng, pop_new[sp * cap + i], s.rf);
+    } else {
+      fqd::Philox rng(seed, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32),
+                      cyc ^ (fqd::kDrawRoundMain << 24));
+      for (int sp = 0; sp < s.S; ++sp) pop_new[sp * cap + i] = fqd::stoch_round_pop(rng, pop_new[sp * cap + i], s.rf);
+    }
+  }
+  for (int sp = 0; sp < s.S; ++sp) alive = alive || pop_new[sp * cap + i] != 0;
+  msurv[i] = alive ? 1 : 0;
+}
+
+// round_low_population_spawns for new-determinant candidates.
+__global__ void k_round_new(Sys s, int64_t nruns, int64_t* __restrict__ rpop, const int64_t* __restrict__ rstate,
+                            int64_t scap, int replay, const double* __restrict__ rbuf, int64_t rbase,
+                            const int64_t* __restrict__ ndraw_off, uint64_t seed, uint32_t cyc,
+                            int64_t* __restrict__ nsurv) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  if (rstate[r] != 2) {
+    nsurv[r] = 0;
+    return;
+  }
+  if (s.real_amps) {
+    if (replay) {
+      fqd::ReplayRng rng{rbuf, rbase + ndraw_off[r], rbase + ndraw_off[r + 1], false};
+      for (int sp = 0; sp < s.S; ++sp) rpop[sp * scap + r] = fqd::stoch_round_pop(rng, rpop[sp * scap + r], s.rf);
+    } else {
+      fqd::Philox rng(seed, static_cast<uint32_t>(r), static_cast<uint32_t>(r >> 32),
+                      cyc ^ (fqd::kDrawRoundSpawn << 24));
+      for (int sp = 0; sp < s.S; ++sp) rpop[sp * scap + r] = fqd::stoch_round_pop(rng, rpop[sp * scap + r], s.rf);
+    }
+  }
+  bool alive = false;
+  for (int sp = 0; sp < s.S; ++sp) alive = alive || rpop[sp * scap + r] != 0;
+  nsurv[r] = alive ? 1 : 0;
+}
+
+__global__ void k_zero(int64_t* __restrict__ p, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_ins_count(int64_t nruns, const int64_t* __restrict__ nsurv, const int64_t* __restrict__ rpos,
+                            int64_t* __restrict__ ins) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= nruns || !nsurv[r]) return;
+  atomicAdd(reinterpret_cast<unsigned long long*>(&ins[rpos[r]]), 1ull);
+}
+
+// Decide the new list length; insertion is skipped when the main list would
+// overflow (insert_new_walkers' psip_list%error path).
+__global__ void k_finalize_counts(int64_t n, int64_t nruns, const int64_t* __restrict__ mrank,
+                                  const int64_t* __restrict__ nrank, int64_t cap, Stats* __restrict__ st) {
+  const int64_t nmain = mrank[n], nnew = nrank[nruns];
+  if (nmain + nnew > cap) {
+    st->main_overflow = 1;
+    st->new_n = nmain;
+  } else {
+    st->new_n = nmain + nnew;
+  }
+}
+
+__global__ void k_scatter_main(Sys s, int64_t n, int64_t cap, const uint64_t* __restrict__ det,
+                               const int64_t* __restrict__ pop_new, const double* __restrict__ kii,
+                               const int64_t* __restrict__ msurv, const int64_t* __restrict__ mrank,
+                               const int64_t* __restrict__ ins_scan, const Stats* __restrict__ st,
+                               uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
+                               double* __restrict__ kii_out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n || !msurv[i]) return;
+  const int64_t o = mrank[i] + (st->main_overflow ? 0 : ins_scan[i + 1]);
+  for (int w = 0; w < s.W; ++w) det_out[w * cap + o] = det[w * cap + i];
+  for (int sp = 0; sp < s.S; ++sp) pop_out[sp * cap + o] = pop_new[sp * cap + i];
+  kii_out[o] = kii[i];
+}
+
+__global__ void k_scatter_new(Sys s, int64_t nruns, int64_t cap, int64_t scap, const uint64_t* __restrict__ rdet,
+                              const int64_t* __restrict__ rpop, const int64_t* __restrict__ rpos,
+                              const int64_t* __restrict__ nsurv, const int64_t* __restrict__ nrank,
+                              const int64_t* __restrict__ mrank, const Stats* __restrict__ st,
+                              uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
+                              double* __restrict__ kii_out) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= nruns || !nsurv[r] || st->main_overflow) return;
+  const int64_t o = nrank[r] + mrank[rpos[r]];
+  uint64_t f[FQ_MAX_WORDS] = {0, 0};
+  for (int w = 0; w < s.W; ++w) {
+    f[w] = rdet[w * scap + r];
+    det_out[w * cap + o] = f[w];
+  }
+  for (int sp = 0; sp < s.S; ++sp) pop_out[sp * cap + o] = rpop[sp * scap + r];
+  kii_out[o] = fqd::diagonal(s, f) - s.H00;
+}
+
+__global__ __launch_bounds__(kBlock) void k_abs_pop(int64_t n, int64_t cap, int S, const int64_t* __restrict__ pop,
+                                                    Stats* __restrict__ st) {
+  __shared__ unsigned long long sh[FQ_MAX_SPACES][kBlock];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
+    unsigned long long v = 0;
+    if (i < n && sp < S) {
+      const int64_t p = pop[sp * cap + i];
+      v = static_cast<unsigned long long>(p < 0 ? -p : p);
+    }
+    sh[sp][threadIdx.x] = v;
+  }
+  __syncthreads();
+  for (int k = kBlock / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k)
+      for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) sh[sp][threadIdx.x] += sh[sp][threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int sp = 0; sp < S; ++sp) atomicAdd(&st->abs_pop[sp], sh[sp][0]);
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+// Make room for `extra` more draws after the live part [rng_pos, rng_end),
+// moving the live part to the front of a (possibly larger) buffer.
+int reserve_rng(fq_engine* e, int64_t extra) {
+  if (e->rng_end + extra <= e->rng_cap) return FQ_OK;
+  const int64_t live = e->rng_end - e->rng_pos;
+  const int64_t cap = std::max<int64_t>(live + extra, 2 * e->rng_cap);
+  double* nb = nullptr;
+  if (dalloc(&nb, cap) != FQ_OK) return FQ_ERR_NOMEM;
+  if (live > 0)
+    FQ_HIP(hipMemcpyAsync(nb, e->rng_buf + e->rng_pos, sizeof(double) * live, hipMemcpyDeviceToDevice, e->stream));
+  FQ_HIP(hipStreamSynchronize(e->stream));
+  dfree(e->rng_buf);
+  e->rng_buf = nb;
+  e->rng_cap = cap;
+  e->rng_end = live;
+  e->rng_pos = 0;
+  return FQ_OK;
+}
+
+int sync_stats(fq_engine* e) {
+  FQ_HIP(hipMemcpyAsync(&e->h_stats, e->d_stats, sizeof(Stats), hipMemcpyDeviceToHost, e->stream));
+  FQ_HIP(hipStreamSynchronize(e->stream));
+  return FQ_OK;
+}
+
+// Sort the spawned list (ns entries) and reduce runs; leaves nruns in h_stats.
+int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
+  const int W = e->W;
+  const int64_t scap = e->scap;
+  uint32_t* perm = nullptr;
+  uint64_t* keys = nullptr;
+  if (ns > 0) {
+    if (replay) {
+      hipLaunchKernelGGL(k_replay_qsort, dim3(1), dim3(1), 0, e->stream, e->sdet, ns, W, scap, e->vbuf[0],
+                         e->kbuf[0]);
+      perm = e->vbuf[0];
+      keys = e->kbuf[0];
+    } else {
+      hipLaunchKernelGGL(k_iota_copy_keys, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->sdet, ns, W, scap,
+                         e->kbuf[0], e->vbuf[0]);
+      const int64_t nb = ceil_div(ns, kRadixTile);
+      if (e->hist_cap < 256 * nb + 1) {
+        dfree(e->hist);
+        dfree(e->hist_scan);
+        if (dalloc(&e->hist, 256 * nb + 1) != FQ_OK || dalloc(&e->hist_scan, 256 * nb + 1) != FQ_OK)
+          return FQ_ERR_NOMEM;
+        e->hist_cap = 256 * nb + 1;
+      }
+      int src = 0;
+      for (int w = 0; w < W; ++w) {
+        const int bits = std::min(64, e->sys.nbasis - 64 * w);
+        for (int shift = 0; shift < bits; shift += 8) {
+          hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src] + w * scap, ns, shift,
+                             e->hist);
+          int rc = scan_exclusive(e, e->hist, e->hist_scan, 256 * nb);
+          if (rc != FQ_OK) return rc;
+          hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src], e->vbuf[src], ns,
+                             W, w, shift, e->hist_scan, e->kbuf[1 - src], e->vbuf[1 - src], scap);
+          src = 1 - src;
+        }
+      }
+      perm = e->vbuf[src];
+      keys = e->kbuf[src];
+    }
+    hipLaunchKernelGGL(k_heads, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, keys, ns, W, scap, e->head);
+    int rc = scan_exclusive(e, e->head, e->head_scan, ns);
+    if (rc != FQ_OK) return rc;
+    hipLaunchKernelGGL(k_run_starts, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->head, e->head_scan, ns,
+                       e->run_start);
+  }
+  int64_t nruns = 0;
+  if (ns > 0) {
+    FQ_HIP(hipMemcpyAsync(&nruns, e->head_scan + ns, sizeof(int64_t), hipMemcpyDeviceToHost, e->stream));
+    FQ_HIP(hipStreamSynchronize(e->stream));
+    hipLaunchKernelGGL(k_reduce_runs, dim3(grid_for(nruns)), dim3(kBlock), 0, e->stream, e->sys, nruns,
+                       e->run_start, keys, perm, e->spop, e->sflag, e->sord, scap, replay ? 1 : 0, e->rdet, e->rpop,
+                       e->rflag, e->rstate);
+  }
+  e->h_stats.nruns = nruns;
+  return FQ_OK;
+}
+
+// Annihilation proper: merge runs, round, compact, insert.  ns spawned
+// entries are in the spawned list; pop_new holds the post-death populations.
+int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng_pos) {
+  int rc = sort_and_reduce(e, ns, replay);
+  if (rc != FQ_OK) return rc;
+  const int64_t nruns = e->h_stats.nruns;
+  const int64_t n = e->n, cap = e->cap, scap = e->scap;
+  const int c = e->cur;
+  hipStream_t st = e->stream;
+  if (nruns > 0)
+    hipLaunchKernelGGL(k_merge_runs, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, n, cap, e->det[c],
+                       e->pop_new, e->rdet, e->rpop, e->rflag, scap, e->rpos, e->rstate);
+  // Rounding draws (replay: main list first, then new candidates, in order).
+  int64_t main_draws = 0, new_draws = 0;
+  if (replay && e->sys.real_amps) {
+    hipLaunchKernelGGL(k_count_round_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->pop_new,
+                       e->mdraw);
+    rc = scan_exclusive(e, e->mdraw, e->mdraw_off, n);
+    if (rc != FQ_OK) return rc;
+    if (nruns > 0) {
+      hipLaunchKernelGGL(k_count_round_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, e->rpop,
+                         e->rstate, scap, e->ndraw);
+      rc = scan_exclusive(e, e->ndraw, e->ndraw_off, nruns);
+      if (rc != FQ_OK) return rc;
+    }
+    FQ_HIP(hipMemcpyAsync(&main_draws, e->mdraw_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    if (nruns > 0) FQ_HIP(hipMemcpyAsync(&new_draws, e->ndraw_off + nruns, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    FQ_HIP(hipStreamSynchronize(st));
+    if (rng_pos + main_draws + new_draws > e->rng_end) return FQ_ERR_RNG_EXHAUSTED;
+  }
+  const uint64_t seed = e->par.philox_seed;
+  hipLaunchKernelGGL(k_round_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->pop_new,
+                     replay ? 1 : 0, e->rng_buf, rng_pos, e->mdraw_off, seed, cyc, e->msurv);
+  if (nruns > 0)
+    hipLaunchKernelGGL(k_round_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, e->rpop, e->rstate,
+                       scap, replay ? 1 : 0, e->rng_buf, rng_pos + main_draws, e->ndraw_off, seed, cyc, e->nsurv);
+  rng_pos += main_draws + new_draws;
+  rc = scan_exclusive(e, e->msurv, e->mrank, n);
+  if (rc != FQ_OK) return rc;
+  if (nruns > 0) {
+    rc = scan_exclusive(e, e->nsurv, e->nrank, nruns);
+    if (rc != FQ_OK) return rc;
+  } else {
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, e->nrank, 1);
+  }
+  hipLaunchKernelGGL(k_zero, dim3(grid_for(n + 1)), dim3(kBlock), 0, st, e->ins, n + 1);
+  if (nruns > 0)
+    hipLaunchKernelGGL(k_ins_count, dim3(grid_for(nruns)), dim3(kBlock), 0, st, nruns, e->nsurv, e->rpos, e->ins);
+  rc = scan_exclusive(e, e->ins, e->ins_scan, n + 1);
+  if (rc != FQ_OK) return rc;
+  hipLaunchKernelGGL(k_finalize_counts, dim3(1), dim3(1), 0, st, n, nruns, e->mrank, e->nrank, cap, e->d_stats);
+  const int o = 1 - c;
+  hipLaunchKernelGGL(k_scatter_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->det[c], e->pop_new,
+                     e->kii[c], e->msurv, e->mrank, e->ins_scan, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
+  if (nruns > 0)
+    hipLaunchKernelGGL(k_scatter_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, cap, scap, e->rdet,
+                       e->rpop, e->rpos, e->nsurv, e->nrank, e->mrank, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
+  rc = sync_stats(e);
+  if (rc != FQ_OK) return rc;
+  e->cur = o;
+  e->n = e->h_stats.new_n;
+  hipLaunchKernelGGL(k_abs_pop, dim3(grid_for(e->n)), dim3(kBlock), 0, st, e->n, cap, e->S, e->pop[o], e->d_stats);
+  return sync_stats(e);
+}
+
+void fill_out(fq_engine* e, fq_cycle_out* out, int64_t rng_used) {
+  std::memset(out, 0, sizeof(*out));
+  const Stats& s = e->h_stats;
+  const double rf = static_cast<double>(e->sys.rf);
+  for (int sp = 0; sp < e->S; ++sp) {
+    out->proj_energy[sp] = s.proj[sp];
+    out->D0_population[sp] = s.d0[sp];
+    out->nparticles[sp] = static_cast<double>(s.abs_pop[sp]) / rf;
+  }
+  out->nstates = e->n;
+  out->nspawn_events = std::min<int64_t>(s.nspawn, e->scap);
+  out->ndeath = s.ndeath;
+  out->nattempts = s.total_attempts;
+  out->rng_consumed = rng_used;
+  out->spawn_overflow = s.spawn_overflow ? 1 : 0;
+  out->main_overflow = s.main_overflow ? 1 : 0;
+}
+
+int alloc_engine(fq_engine* e) {
+  const int64_t cap = e->cap, scap = e->scap;
+  const int W = e->W, S = e->S;
+  int rc = FQ_OK;
+  for (int b = 0; b < 2; ++b) {
+    rc |= dalloc(&e->det[b], W * cap);
+    rc |= dalloc(&e->pop[b], S * cap);
+    rc |= dalloc(&e->kii[b], cap);
+  }
+  rc |= dalloc(&e->pop_new, S * cap);
+  rc |= dalloc(&e->nat, S * cap + 1);  // det-major (det, space) entries
+  rc |= dalloc(&e->att_off, S * cap + 1);
+  rc |= dalloc(&e->flags, cap);
+  rc |= dalloc(&e->msurv, cap + 1);
+  rc |= dalloc(&e->mrank, cap + 1);
+  rc |= dalloc(&e->ins, cap + 2);
+  rc |= dalloc(&e->ins_scan, cap + 2);
+  rc |= dalloc(&e->mdraw, cap + 1);
+  rc |= dalloc(&e->mdraw_off, cap + 1);
+  rc |= dalloc(&e->sdet, W * scap);
+  rc |= dalloc(&e->spop, S * scap);
+  rc |= dalloc(&e->sflag, scap);
+  rc |= dalloc(&e->sord, scap);
+  for (int b = 0; b < 2; ++b) {
+    rc |= dalloc(&e->kbuf[b], W * scap);
+    rc |= dalloc(&e->vbuf[b], scap);
+  }
+  rc |= dalloc(&e->head, scap + 1);
+  rc |= dalloc(&e->head_scan, scap + 1);
+  rc |= dalloc(&e->run_start, scap + 1);
+  rc |= dalloc(&e->rdet, W * scap);
+  rc |= dalloc(&e->rpop, S * scap);
+  rc |= dalloc(&e->rflag, scap);
+  rc |= dalloc(&e->rpos, scap);
+  rc |= dalloc(&e->rstate, scap);
+  rc |= dalloc(&e->nsurv, scap + 1);
+  rc |= dalloc(&e->nrank, scap + 1);
+  rc |= dalloc(&e->ndraw, scap + 1);
+  rc |= dalloc(&e->ndraw_off, scap + 1);
+  rc |= dalloc(&e->d_stats, 1);
+  return rc == FQ_OK ? FQ_OK : FQ_ERR_NOMEM;
+}
+
+template <class T>
+int upload(T** dst, const T* src, int64_t count, hipStream_t st) {
+  if (!src || count <= 0) {
+    *dst = nullptr;
+    return FQ_OK;
+  }
+  if (dalloc(dst, count) != FQ_OK) return FQ_ERR_NOMEM;
+  FQ_HIP(hipMemcpyAsync(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice, st));
+  return FQ_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
+  if (!sys || !params || !out) return FQ_ERR_ARG;
+  if (sys->nwords < 1 || sys->nwords > FQ_MAX_WORDS || params->nspaces < 1 || params->nspaces > FQ_MAX_SPACES)
+    return FQ_ERR_ARG;
+  if (sys->system == FQ_SYS_READ_IN && sys->nsym_pg > 8) return FQ_ERR_ARG;
+  if (params->main_capacity < 1 || params->spawn_capacity < 1) return FQ_ERR_ARG;
+  if (params->spawn_capacity > 0xffffffffLL) return FQ_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= params->device) return FQ_ERR_NO_DEVICE;
+  auto* e = new fq_engine();
+  e->device = params->device;
+  if (hipSetDevice(e->device) != hipSuccess) {
+    delete e;
+    return FQ_ERR_NO_DEVICE;
+  }
+  if (hipStreamCreate(&e->stream) != hipSuccess) {
+    delete e;
+    return FQ_ERR_DEVICE;
+  }
+  e->par = *params;
+  e->W = sys->nwords;
+  e->S = params->nspaces;
+  e->cap = params->main_capacity;
+  e->scap = params->spawn_capacity;
+  Sys& s = e->sys;
+  s.system = sys->system;
+  s.nbasis = sys->nbasis;
+  s.nel = sys->nel;
+  s.nalpha = sys->nalpha;
+  s.nbeta = sys->nbeta;
+  s.nvirt = sys->nvirt;
+  s.nvirt_alpha = sys->nvirt_alpha;
+  s.nvirt_beta = sys->nvirt_beta;
+  s.W = sys->nwords;
+  s.S = params->nspaces;
+  for (int w = 0; w < FQ_MAX_WORDS; ++w) {
+    s.f0[w] = w < sys->nwords ? sys->f0[w] : 0;
+    const int bits = std::max(0, std::min(64, sys->nbasis - 64 * w));
+    s.valid[w] = bits == 64 ? ~uint64_t{0} : ((uint64_t{1} << bits) - 1);
+  }
+  s.H00 = sys->H00;
+  s.coulomb_k = sys->coulomb_k;
+  s.nsym = sys->nsym;
+  s.norb = sys->norb;
+  s.nsym_pg = sys->nsym_pg;
+  s.max_sym_spin = sys->max_sym_spin;
+  s.ecore = sys->ecore;
+  s.psingle = sys->pattempt_single;
+  s.pdouble = sys->pattempt_double;
+  s.initiator = params->initiator;
+  s.quadrature = params->quadrature_initiator;
+  s.real_amps = params->real_amplitudes;
+  s.excit_gen = params->excit_gen;
+  s.initiator_pop = params->initiator_pop;
+  s.rf = int64_t{1} << params->pop_bit_shift;
+  s.spawn_cutoff = params->spawn_cutoff;
+  int rc = FQ_OK;
+  rc |= upload(&e->d_sp_eigv, sys->sp_eigv, sys->nbasis, e->stream);
+  if (sys->system == FQ_SYS_HUB_K) {
+    rc |= upload(&e->d_sym_table, sys->sym_table, static_cast<int64_t>(sys->nsym) * sys->nsym, e->stream);
+    rc |= upload(&e->d_inv_sym, sys->inv_sym, sys->nsym, e->stream);
+  } else {
+    rc |= upload(&e->d_orb_sym, sys->orb_sym, sys->nbasis, e->stream);
+    rc |= upload(&e->d_nbss, sys->nbasis_sym_spin, 2 * sys->nsym_pg, e->stream);
+    rc |= upload(&e->d_ssbf, sys->sym_spin_basis_fns, 2 * sys->nsym_pg * sys->max_sym_spin, e->stream);
+    rc |= upload(&e->d_h1, sys->h1, static_cast<int64_t>(sys->norb) * sys->norb, e->stream);
+    rc |= upload(&e->d_eri, sys->eri, sys->n_eri, e->stream);
+  }
+  s.sp_eigv = e->d_sp_eigv;
+  s.sym_table = e->d_sym_table;
+  s.inv_sym = e->d_inv_sym;
+  s.orb_sym = e->d_orb_sym;
+  s.nbss = e->d_nbss;
+  s.ssbf = e->d_ssbf;
+  s.h1 = e->d_h1;
+  s.eri = e->d_eri;
+  if (rc == FQ_OK) rc = alloc_engine(e);
+  for (int k = 0; k < 4 && rc == FQ_OK; ++k)
+    if (hipEventCreate(&e->ev[k]) != hipSuccess) rc = FQ_ERR_DEVICE;
+  if (rc != FQ_OK) {
+    fq_destroy(e);
+    return rc;
+  }
+  if (hipStreamSynchronize(e->stream) != hipSuccess) {
+    fq_destroy(e);
+    return FQ_ERR_DEVICE;
+  }
+  *out = e;
+  return FQ_OK;
+}
+
+int fq_load(fq_engine* e, const uint64_t* states, const int64_t* pops, const double* dat, int64_t n) {
+  if (!e || n < 0 || (n > 0 && (!states || !pops || !dat))) return FQ_ERR_ARG;
+  if (n > e->cap) return FQ_ERR_CAPACITY;
+  FQ_HIP(hipSetDevice(e->device));
+  const int W = e->W, S = e->S;
+  std::vector<uint64_t> dw(static_cast<size_t>(W) * n);
+  std::vector<int64_t> dp(static_cast<size_t>(S) * n);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int w = 0; w < W; ++w) dw[w * n + i] = states[i * W + w];
+    for (int s = 0; s < S; ++s) dp[s * n + i] = pops[i * S + s];
+  }
+  e->cur = 0;
+  for (int w = 0; w < W && n > 0; ++w)
+    FQ_HIP(hipMemcpyAsync(e->det[0] + w * e->cap, dw.data() + w * n, 8 * n, hipMemcpyHostToDevice, e->stream));
+  for (int s = 0; s < S && n > 0; ++s)
+    FQ_HIP(hipMemcpyAsync(e->pop[0] + s * e->cap, dp.data() + s * n, 8 * n, hipMemcpyHostToDevice, e->stream));
+  if (n > 0) FQ_HIP(hipMemcpyAsync(e->kii[0], dat, 8 * n, hipMemcpyHostToDevice, e->stream));
+  FQ_HIP(hipStreamSynchronize(e->stream));
+  e->n = n;
+  return FQ_OK;
+}
+
+int fq_push_rng(fq_engine* e, const double* draws, int64_t n) {
+  if (!e || n < 0 || (n > 0 && !draws)) return FQ_ERR_ARG;
+  if (n == 0) return FQ_OK;
+  FQ_HIP(hipSetDevice(e->device));
+  int rc = reserve_rng(e, n);
+  if (rc != FQ_OK) return rc;
+  FQ_HIP(hipMemcpyAsync(e->rng_buf + e->rng_end, draws, sizeof(double) * n, hipMemcpyHostToDevice, e->stream));
+  FQ_HIP(hipStreamSynchronize(e->stream));
+  e->rng_end += n;
+  return FQ_OK;
+}
+
+int fq_rng_available(fq_engine* e, int64_t* n) {
+  if (!e || !n) return FQ_ERR_ARG;
+  *n = e->rng_end - e->rng_pos;
+  return FQ_OK;
+}
+
+int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
+  if (!e || !shift || !eproj_old || !out) return FQ_ERR_ARG;
+  FQ_HIP(hipSetDevice(e->device));
+  const bool replay = e->par.rng_mode == FQ_RNG_REPLAY;
+  const int64_t n = e->n, cap = e->cap;
+  const int c = e->cur;
+  hipStream_t st = e->stream;
+  const uint32_t cyc = static_cast<uint32_t>(e->cycle_index++);
+  const double sh1 = e->S > 1 ? shift[1] : 0.0, e1 = e->S > 1 ? eproj_old[1] : 0.0;
+  FQ_HIP(hipMemsetAsync(e->d_stats, 0, sizeof(Stats), st));
+  FQ_HIP(hipEventRecord(e->ev[0], st));
+  int64_t rng_pos = e->rng_pos;
+  int64_t ns = 0;
+  if (replay) {
+    hipLaunchKernelGGL(k_replay_walk, dim3(1), dim3(1), 0, st, e->sys, n, cap, e->det[c], e->pop[c], e->kii[c],
+                       e->pop_new, tau, shift[0], sh1, eproj_old[0], e1, e->rng_buf, e->rng_pos, e->rng_end, e->scap,
+                       e->sdet, e->spop, e->sflag, e->sord, e->d_stats);
+    int rc = sync_stats(e);
+    if (rc != FQ_OK) return rc;
+    if (e->h_stats.rng_failed) return FQ_ERR_RNG_EXHAUSTED;
+    rng_pos = e->h_stats.rng_pos;
+    ns = e->h_stats.nspawn;
+  } else {
+    const unsigned nb = grid_for(n);
+    double* partials = reinterpret_cast<double*>(e->mdraw);  // scratch: 4*nb doubles <= cap+1 int64
+    if (4 * static_cast<int64_t>(nb) > cap + 1) return FQ_ERR_CAPACITY;
+    if (n > 0)
+      hipLaunchKernelGGL(k_det_pass, dim3(nb), dim3(kBlock), 0, st, e->sys, n, cap, e->det[c], e->pop[c], e->kii[c],
+                         e->pop_new, e->nat, e->flags, tau, shift[0], sh1, eproj_old[0], e1, e->par.philox_seed, cyc,
+                         partials, e->d_stats);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partials, n > 0 ? static_cast<int>(nb) : 0,
+                       e->d_stats);
+    // Attempt offsets over the n*S (det, space) entries.
+    const int64_t nent = e->S * n;
+    int rc = scan_exclusive(e, e->nat, e->att_off, nent);
+    if (rc != FQ_OK) return rc;
+    int64_t total = 0;
+    FQ_HIP(hipMemcpyAsync(&total, e->att_off + nent, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    FQ_HIP(hipStreamSynchronize(st));
+    if (total > 0) {
+      const int64_t want = ceil_div(total, kBlock);
+      const unsigned g = static_cast<unsigned>(std::min<int64_t>(want, 65536));
+      hipLaunchKernelGGL(k_spawn, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, total, e->att_off, e->det[c],
+                         e->pop[c], e->flags, tau, e->par.philox_seed, cyc, e->scap, e->sdet, e->spop, e->sflag,
+                         e->sord, e->d_stats);
+    }
+    rc = sync_stats(e);
+    if (rc != FQ_OK) return rc;
+    e->h_stats.total_attempts = total;
+    ns = std::min<int64_t>(e->h_stats.nspawn, e->scap);
+  }
+  FQ_HIP(hipEventRecord(e->ev[1], st));
+  const Stats keep = e->h_stats;
+  int rc = annihilate(e, ns, replay, cyc, rng_pos);
+  if (rc != FQ_OK) return rc;
+  FQ_HIP(hipEventRecord(e->ev[2], st));
+  FQ_HIP(hipEventSynchronize(e->ev[2]));
+  // restore spawn-phase quantities the annihilation's stat resets do not touch
+  e->h_stats.total_attempts = keep.total_attempts;
+  const int64_t used = rng_pos - e->rng_pos;
+  e->rng_pos = rng_pos;
+  hipEventElapsedTime(&e->t_spawn, e->ev[0], e->ev[1]);
+  hipEventElapsedTime(&e->t_annih, e->ev[1], e->ev[2]);
+  hipEventElapsedTime(&e->t_total, e->ev[0], e->ev[2]);
+  fill_out(e, out, used);
+  return FQ_OK;
+}
+
+int fq_annihilate(fq_engine* e, const uint64_t* sstates, const int64_t* spops, const int64_t* sflags, int64_t ns,
+                  fq_cycle_out* out) {
+  if (!e || !out || ns < 0 || (ns > 0 && (!sstates || !spops))) return FQ_ERR_ARG;
+  if (ns > e->scap) return FQ_ERR_CAPACITY;
+  FQ_HIP(hipSetDevice(e->device));
+  const bool replay = e->par.rng_mode == FQ_RNG_REPLAY;
+  const int W = e->W, S = e->S;
+  const int64_t n = e->n, cap = e->cap;
+  std::vector<uint64_t> dw(static_cast<size_t>(W) * std::max<int64_t>(ns, 1));
+  std::vector<int64_t> dp(static_cast<size_t>(S) * std::max<int64_t>(ns, 1)), df(std::max<int64_t>(ns, 1));
+  std::vector<uint64_t> dord(std::max<int64_t>(ns, 1));
+  for (int64_t i = 0; i < ns; ++i) {
+    for (int w = 0; w < W; ++w) dw[w * ns + i] = sstates[i * W + w];
+    for (int s = 0; s < S; ++s) dp[s * ns + i] = spops[i * S + s];
+    df[i] = (sflags && e->par.initiator) ? sflags[i] : 0;
+    dord[i] = static_cast<uint64_t>(i);
+  }
+  hipStream_t st = e->stream;
+  for (int w = 0; w < W && ns > 0; ++w)
+    FQ_HIP(hipMemcpyAsync(e->sdet + w * e->scap, dw.data() + w * ns, 8 * ns, hipMemcpyHostToDevice, st));
+  for (int s = 0; s < S && ns > 0; ++s)
+    FQ_HIP(hipMemcpyAsync(e->spop + s * e->scap, dp.data() + s * ns, 8 * ns, hipMemcpyHostToDevice, st));
+  if (ns > 0) {
+    FQ_HIP(hipMemcpyAsync(e->sflag, df.data(), 8 * ns, hipMemcpyHostToDevice, st));
+    FQ_HIP(hipMemcpyAsync(e->sord, dord.data(), 8 * ns, hipMemcpyHostToDevice, st));
+  }
+  // post-death populations = current populations
+  for (int s = 0; s < S && n > 0; ++s)
+    FQ_HIP(hipMemcpyAsync(e->pop_new + s * cap, e->pop[e->cur] + s * cap, 8 * n, hipMemcpyDeviceToDevice, st));
+  FQ_HIP(hipMemsetAsync(e->d_stats, 0, sizeof(Stats), st));
+  FQ_HIP(hipStreamSynchronize(st));
+  const uint32_t cyc = static_cast<uint32_t>(e->cycle_index++);
+  int64_t rng_pos = e->rng_pos;
+  int rc = annihilate(e, ns, replay, cyc, rng_pos);
+  if (rc != FQ_OK) return rc;
+  const int64_t used = rng_pos - e->rng_pos;
+  e->rng_pos = rng_pos;
+  e->h_stats.nspawn = ns;
+  fill_out(e, out, used);
+  return FQ_OK;
+}
+
+int fq_nstates(fq_engine* e, int64_t* n) {
+  if (!e || !n) return FQ_ERR_ARG;
+  *n = e->n;
+  return FQ_OK;
+}
+
+int fq_download(fq_engine* e, uint64_t* states, int64_t* pops, double* dat, int64_t capacity, int64_t* nout) {
+  if (!e || !nout) return FQ_ERR_ARG;
+  const int64_t n = e->n;
+  *nout = n;
+  if (capacity < n) return FQ_ERR_CAPACITY;
+  if (n == 0) return FQ_OK;
+  FQ_HIP(hipSetDevice(e->device));
+  const int W = e->W, S = e->S;
+  std::vector<uint64_t> dw(static_cast<size_t>(W) * n);
+  std::vector<int64_t> dp(static_cast<size_t>(S) * n);
+  const int c = e->cur;
+  for (int w = 0; w < W; ++w)
+    FQ_HIP(hipMemcpyAsync(dw.data() + w * n, e->det[c] + w * e->cap, 8 * n, hipMemcpyDeviceToHost, e->stream));
+  for (int s = 0; s < S; ++s)
+    FQ_HIP(hipMemcpyAsync(dp.data() + s * n, e->pop[c] + s * e->cap, 8 * n, hipMemcpyDeviceToHost, e->stream));
+  if (dat) FQ_HIP(hipMemcpyAsync(dat, e->kii[c], 8 * n, hipMemcpyDeviceToHost, e->stream));
+  FQ_HIP(hipStreamSynchronize(e->stream));
+  for (int64_t i = 0; i < n; ++i) {
+    if (states)
+      for (int w = 0; w < W; ++w) states[i * W + w] = dw[w * n + i];
+    if (pops)
+      for (int s = 0; s < S; ++s) pops[i * S + s] = dp[s * n + i];
+  }
+  return FQ_OK;
+}
+
+int fq_last_timing(fq_engine* e, double* spawn_ms, double* annihilate_ms, double* total_ms) {
+  if (!e) return FQ_ERR_ARG;
+  if (spawn_ms) *spawn_ms = e->t_spawn;
+  if (annihilate_ms) *annihilate_ms = e->t_annih;
+  if (total_ms) *total_ms = e->t_total;
+  return FQ_OK;
+}
+
+void fq_destroy(fq_engine* e) {
+  if (!e) return;
+  hipSetDevice(e->device);
+  if (e->stream) hipStreamSynchronize(e->stream);
+  for (int b = 0; b < 2; ++b) {
+    dfree(e->det[b]);
+    dfree(e->pop[b]);
+    dfree(e->kii[b]);
+    dfree(e->kbuf[b]);
+    dfree(e->vbuf[b]);
+  }
+  dfree(e->pop_new);
+  dfree(e->nat);
+  dfree(e->att_off);
+  dfree(e->flags);
+  dfree(e->msurv);
+  dfree(e->mrank);
+  dfree(e->ins);
+  dfree(e->ins_scan);
+  dfree(e->mdraw);
+  dfree(e->mdraw_off);
+  dfree(e->sdet);
+  dfree(e->spop);
+  dfree(e->sflag);
+  dfree(e->sord);
+  dfree(e->hist);
+  dfree(e->hist_scan);
+  dfree(e->head);
+  dfree(e->head_scan);
+  dfree(e->run_start);
+  dfree(e->rdet);
+  dfree(e->rpop);
+  dfree(e->rflag);
+  dfree(e->rpos);
+  dfree(e->rstate);
+  dfree(e->nsurv);
+  dfree(e->nrank);
+  dfree(e->ndraw);
+  dfree(e->ndraw_off);
+  for (auto*& p : e->scan_tmp) dfree(p);
+  dfree(e->rng_buf);
+  dfree(e->d_stats);
+  dfree(e->d_sp_eigv);
+  dfree(e->d_sym_table);
+  dfree(e->d_inv_sym);
+  dfree(e->d_orb_sym);
+  dfree(e->d_nbss);
+  dfree(e->d_ssbf);
+  dfree(e->d_h1);
+  dfree(e->d_eri);
+  for (int k = 0; k < 4; ++k)
+    if (e->ev[k]) hipEventDestroy(e->ev[k]);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+}
+
+}  // extern "C"

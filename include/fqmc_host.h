@@ -89,6 +89,11 @@ typedef struct fqh_report_row {
  * status code. */
 int fqh_fciqmc(const fqh_system *s, const fqh_qmc_in *in, fqh_report_row *rows, int max_rows, int *nrows);
 
+/* The replay stream: n close-open dSFMT-19937 draws for rng_seed, in the
+ * order get_rand_close_open returns them
+ * (lib/dSFMT_F03_interface/dSFMT_interface.F90:192-250,387-409). */
+int fqh_dsfmt_fill(int64_t seed, double *out, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif
