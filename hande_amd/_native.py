@@ -57,6 +57,14 @@ class FqCycleOut(C.Structure):
     ]
 
 
+class FqProfileOut(C.Structure):
+    _fields_ = [
+        ("det_pass_ms", f64), ("spawn_ms", f64), ("sort_ms", f64), ("reduce_ms", f64), ("merge_ms", f64),
+        ("total_ms", f64), ("nstates_in", i64), ("nattempts", i64), ("nspawned", i64), ("nruns", i64),
+        ("nstates_out", i64), ("radix_passes", i32),
+    ]
+
+
 class FqhQmcIn(C.Structure):
     _fields_ = [
         ("tau", f64), ("rng_seed", i64), ("init_pop", f64), ("mc_cycles", i32), ("nreports", i32),
@@ -97,6 +105,7 @@ def _declare(lib):
         "fq_nstates": (C.c_int, [vp, P(i64)]),
         "fq_download": (C.c_int, [vp, P(u64), P(i64), P(f64), i64, P(i64)]),
         "fq_last_timing": (C.c_int, [vp, P(f64), P(f64), P(f64)]),
+        "fq_last_profile": (C.c_int, [vp, P(FqProfileOut)]),
         "fq_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():

@@ -43,6 +43,8 @@ struct Sys {
   double ecore;
   const double* h1;
   const double* eri;
+  const double* jmat;  // norb*norb: (pp|qq) = eri(i,j,i,j) for spatial p,q
+  const double* kmat;  // norb*norb: (pq|qp) = eri(i,j,j,i)
   double psingle, pdouble;
   int initiator, quadrature, real_amps, excit_gen;
   double initiator_pop;
@@ -297,6 +299,8 @@ __device__ inline double diagonal(const Sys& s, const uint64_t* f) {
     }
     return h;
   }
+  // Coulomb and exchange elements come from dense per-pair tables holding the
+  // same values as the packed store, in the reference's summation order.
   double h = s.ecore;
   for (int w = 0; w < s.W; ++w) {
     uint64_t x = f[w];
@@ -304,14 +308,16 @@ __device__ inline double diagonal(const Sys& s, const uint64_t* f) {
       const int i = (w << 6) + __builtin_ctzll(x) + 1;
       x &= x - 1;
       h = h + h1g(s, i, i);
+      const int pi = ((i + 1) / 2 - 1) * s.norb;
       // pairs (i, j) with j after i
       for (int w2 = w; w2 < s.W; ++w2) {
         uint64_t y = (w2 == w) ? x : f[w2];
         while (y) {
           const int j = (w2 << 6) + __builtin_ctzll(y) + 1;
           y &= y - 1;
-          h = h + eri_nz(s, i, j, i, j);
-          if (ms_of(i) == ms_of(j)) h = h - eri_nz(s, i, j, j, i);
+          const int pj = (j + 1) / 2 - 1;
+          h = h + s.jmat[pi + pj];
+          if (ms_of(i) == ms_of(j)) h = h - s.kmat[pi + pj];
         }
       }
     }
@@ -530,6 +536,16 @@ struct SymUnocc {
 __device__ inline SymUnocc sym_unocc(const Sys& s, const uint64_t* f) {
   SymUnocc u;
   u.pk[0] = u.pk[1] = 0;
+  if (s.nsym_pg == 1) {  // one irrep: unoccupied counts per spin from popcounts
+    int na = 0, nb = 0;
+    for (int w = 0; w < s.W; ++w) {
+      na += __popcll(f[w] & kAlphaBits);
+      nb += __popcll(f[w] & kBetaBits);
+    }
+    u.pk[0] = static_cast<uint64_t>(s.nbss[0] - nb);
+    u.pk[1] = static_cast<uint64_t>(s.nbss[1] - na);
+    return u;
+  }
   for (int g = 0; g < s.nsym_pg; ++g) {
     u.pk[0] |= static_cast<uint64_t>(s.nbss[g]) << (8 * g);
     u.pk[1] |= static_cast<uint64_t>(s.nbss[s.nsym_pg + g]) << (8 * g);

@@ -147,7 +147,7 @@ struct fq_engine {
   int W = 1, S = 1;
   // device tables
   int *d_sym_table = nullptr, *d_inv_sym = nullptr, *d_orb_sym = nullptr, *d_nbss = nullptr, *d_ssbf = nullptr;
-  double *d_sp_eigv = nullptr, *d_h1 = nullptr, *d_eri = nullptr;
+  double *d_sp_eigv = nullptr, *d_h1 = nullptr, *d_eri = nullptr, *d_jmat = nullptr, *d_kmat = nullptr;
   // main list (double-buffered)
   int64_t cap = 0, n = 0;
   int cur = 0;
@@ -164,6 +164,15 @@ struct fq_engine {
   int64_t* ins_scan = nullptr;  // cap+2
   int64_t* mdraw = nullptr;     // cap+1 (replay rounding draws)
   int64_t* mdraw_off = nullptr;
+  double* part_d = nullptr;     // 4 * (cap/256+1) block partials
+  int64_t* part_i = nullptr;    // 2 * (cap/256+1)
+  // spawning-attempt staging (one slot per attempt of a batch)
+  int64_t bcap = 0;
+  uint64_t* st_det = nullptr;   // W*bcap
+  int64_t* st_pop = nullptr;    // bcap
+  int64_t* st_ent = nullptr;    // bcap
+  int64_t* st_flag = nullptr;   // bcap+1
+  int64_t* st_pos = nullptr;    // bcap+1
   // spawned list
   int64_t scap = 0;
   uint64_t* sdet = nullptr;  // W*scap
@@ -201,6 +210,11 @@ struct fq_engine {
   uint64_t cycle_index = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float t_spawn = 0, t_annih = 0, t_total = 0;
+  // phase events: 0 cycle start, 1 det pass end, 2 spawn start, 3 spawn end,
+  // 4 sort start, 5 sort end, 6 reduce end, 7 merge start, 8 merge end
+  hipEvent_t pev[9] = {};
+  fq_profile_out prof{};
+  int radix_passes = 0;
 };
 
 namespace {
@@ -217,7 +231,7 @@ int dalloc(T** p, int64_t count) {
 }
 template <class T>
 void dfree(T*& p) {
-  if (p) hipFree(p);
+  if (p) (void)hipFree(p);
   p = nullptr;
 }
 
@@ -271,20 +285,20 @@ __device__ __forceinline__ void load_det(const uint64_t* det, int64_t cap, int W
 }
 
 // Production: per determinant (loop body of src/fciqmc.f90:315-374 minus the
-// spawning attempts).  Projected-energy partials are reduced per block and
-// summed in block order by k_sum_partials (deterministic).
+// spawning attempts).  Block partial sums (projected energy, reference
+// population, deaths) are reduced in a fixed order by k_sum_partials, so the
+// estimators are deterministic and no global atomics are issued.
 __global__ __launch_bounds__(kBlock) void k_det_pass(Sys s, int64_t n, int64_t cap, const uint64_t* __restrict__ det,
                                                      const int64_t* __restrict__ pop, const double* __restrict__ kii,
                                                      int64_t* __restrict__ pop_new, int64_t* __restrict__ nat,
                                                      int64_t* __restrict__ flags, double tau, double sh0, double sh1,
                                                      double e0, double e1, uint64_t seed, uint32_t cyc,
-                                                     double* __restrict__ partials, Stats* __restrict__ st) {
-  __shared__ double sh_proj[FQ_MAX_SPACES][kBlock];
-  __shared__ double sh_d0[FQ_MAX_SPACES][kBlock];
-  __shared__ unsigned long long sh_death[kBlock];
+                                                     double* __restrict__ part_d, int64_t* __restrict__ part_i) {
+  __shared__ double sh_d[2 * FQ_MAX_SPACES][kBlock];
+  __shared__ int64_t sh_death[kBlock];
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   double proj[FQ_MAX_SPACES] = {0.0, 0.0}, d0[FQ_MAX_SPACES] = {0.0, 0.0};
-  unsigned long long ndeath = 0;
+  int64_t ndeath = 0;
   if (i < n) {
     uint64_t f[FQ_MAX_WORDS];
     load_det(det, cap, s.W, i, f);
@@ -298,86 +312,109 @@ __global__ __launch_bounds__(kBlock) void k_det_pass(Sys s, int64_t n, int64_t c
     const double K = kii[i];
     for (int sp = 0; sp < s.S; ++sp) {
       fqd::proj_energy_term(s, f, rp[sp], proj[sp], d0[sp]);
-      fqd::Philox rng(seed, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32) ^ (sp << 28), cyc);
-      rng.c2 = cyc ^ (fqd::kDrawNattempts << 24);
+      const uint32_t hi = static_cast<uint32_t>(i >> 32) ^ (static_cast<uint32_t>(sp) << 28);
+      fqd::Philox rng(seed, static_cast<uint32_t>(i), hi, cyc ^ (fqd::kDrawNattempts << 24));
       nat[i * s.S + sp] = fqd::decide_nattempts(rng, rp[sp]);
-      fqd::Philox rd(seed, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32) ^ (sp << 28),
-                     cyc ^ (fqd::kDrawDeath << 24));
+      fqd::Philox rd(seed, static_cast<uint32_t>(i), hi, cyc ^ (fqd::kDrawDeath << 24));
       int64_t q = p[sp];
-      ndeath += static_cast<unsigned long long>(
-          fqd::death(rd, tau, K, sp == 0 ? sh0 : sh1, sp == 0 ? e0 : e1, q));
+      ndeath += fqd::death(rd, tau, K, sp == 0 ? sh0 : sh1, sp == 0 ? e0 : e1, q);
       pop_new[sp * cap + i] = q;
     }
   }
   for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
-    sh_proj[sp][threadIdx.x] = proj[sp];
-    sh_d0[sp][threadIdx.x] = d0[sp];
+    sh_d[2 * sp][threadIdx.x] = proj[sp];
+    sh_d[2 * sp + 1][threadIdx.x] = d0[sp];
   }
   sh_death[threadIdx.x] = ndeath;
   __syncthreads();
   for (int k = kBlock / 2; k > 0; k >>= 1) {
     if (threadIdx.x < k) {
-      for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
-        sh_proj[sp][threadIdx.x] += sh_proj[sp][threadIdx.x + k];
-        sh_d0[sp][threadIdx.x] += sh_d0[sp][threadIdx.x + k];
-      }
+      for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) sh_d[q][threadIdx.x] += sh_d[q][threadIdx.x + k];
       sh_death[threadIdx.x] += sh_death[threadIdx.x + k];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
-      partials[(2 * sp) * gridDim.x + blockIdx.x] = sh_proj[sp][0];
-      partials[(2 * sp + 1) * gridDim.x + blockIdx.x] = sh_d0[sp][0];
-    }
-    atomicAdd(reinterpret_cast<unsigned long long*>(&st->ndeath), sh_death[0]);
+    for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) part_d[q * gridDim.x + blockIdx.x] = sh_d[q][0];
+    part_i[blockIdx.x] = sh_death[0];
   }
 }
 
-__global__ void k_sum_partials(const double* __restrict__ partials, int nb, Stats* __restrict__ st) {
-  // One thread per (quantity, space): ordered sum over blocks.
-  const int q = threadIdx.x;  // 0..3: proj0, d00, proj1, d01
-  if (q >= 2 * FQ_MAX_SPACES) return;
-  double acc = 0.0;
-  for (int b = 0; b < nb; ++b) acc = acc + partials[q * nb + b];
-  const int sp = q / 2;
-  if (q % 2 == 0)
-    st->proj[sp] = acc;
-  else
-    st->d0[sp] = acc;
+// Fixed-order reduction of the k_det_pass block partials: thread t sums
+// blocks t, t+256, ... in order, then a tree in LDS.
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ part_d,
+                                                         const int64_t* __restrict__ part_i, int nb,
+                                                         Stats* __restrict__ st) {
+  __shared__ double sh[2 * FQ_MAX_SPACES][kBlock];
+  __shared__ int64_t shi[kBlock];
+  double acc[2 * FQ_MAX_SPACES] = {0.0, 0.0, 0.0, 0.0};
+  int64_t ai = 0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+    for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) acc[q] = acc[q] + part_d[q * nb + b];
+    ai += part_i[b];
+  }
+  for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) sh[q][threadIdx.x] = acc[q];
+  shi[threadIdx.x] = ai;
+  __syncthreads();
+  for (int k = kBlock / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + k];
+      shi[threadIdx.x] += shi[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
+      st->proj[sp] = sh[2 * sp][0];
+      st->d0[sp] = sh[2 * sp + 1][0];
+    }
+    st->ndeath = shi[0];
+  }
 }
 
-// Production: one thread per spawning attempt (grid-stride).  Parent entry =
-// last e with att_off[e] <= t (binary search over the n*S (det, space)
-// entries, det-major).  Successful children get slots by wavefront ballot and
-// one atomic per wave.
-__global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t cap, int64_t total,
+// Last entry e in [lo, hi] with att_off[e] <= t.
+__device__ __forceinline__ int64_t parent_of(const int64_t* __restrict__ att_off, int64_t t, int64_t lo, int64_t hi) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (att_off[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+constexpr int kSpawnTile = 4 * kBlock;  // attempts per block tile
+
+// Production: one thread per spawning attempt.  Each block takes tiles of
+// kSpawnTile consecutive attempts of the batch [t_begin, t_end); the parents
+// of a tile's first and last attempts bound a short window of the attempt
+// prefix sum (n*S (det, space) entries, det-major) in which each thread
+// searches its own parent.  The result of attempt t goes to slot t - t_begin
+// of the staging arrays (coalesced, no atomics, attempt order); k_compact packs
+// the successful ones into the spawned list.
+__global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t cap, int64_t t_begin, int64_t t_end,
                                                   const int64_t* __restrict__ att_off,
                                                   const uint64_t* __restrict__ det, const int64_t* __restrict__ pop,
-                                                  const int64_t* __restrict__ flags, double tau, uint64_t seed,
-                                                  uint32_t cyc, int64_t scap, uint64_t* __restrict__ sdet,
-                                                  int64_t* __restrict__ spop, int64_t* __restrict__ sflag,
-                                                  uint64_t* __restrict__ sord, Stats* __restrict__ st) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  // Every lane of a wave iterates the same number of times so ballots stay uniform.
-  const int64_t iters = (total + stride - 1) / stride;
-  const int lane = threadIdx.x & 63;
-  for (int64_t it = 0; it < iters; ++it) {
-    const int64_t t = t0 + it * stride;
-    int64_t nspawn = 0;
-    uint64_t child[FQ_MAX_WORDS] = {0, 0};
-    int64_t e = 0;
-    if (t < total) {
-      int64_t lo = 0, hi = nent - 1;
-      while (lo < hi) {  // last e with att_off[e] <= t
-        const int64_t mid = (lo + hi + 1) >> 1;
-        if (att_off[mid] <= t)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      e = lo;
+                                                  double tau, uint64_t seed, uint32_t cyc, int64_t bcap,
+                                                  uint64_t* __restrict__ st_det, int64_t* __restrict__ st_pop,
+                                                  int64_t* __restrict__ st_ent, int64_t* __restrict__ st_flag) {
+  __shared__ int64_t win[2];
+  const int64_t ntiles = (t_end - t_begin + kSpawnTile - 1) / kSpawnTile;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t t0 = t_begin + tile * kSpawnTile;
+    const int64_t t1 = (t0 + kSpawnTile < t_end) ? t0 + kSpawnTile : t_end;
+    if (threadIdx.x == 0) {
+      const int64_t p0 = parent_of(att_off, t0, 0, nent - 1);
+      win[0] = p0;
+      win[1] = parent_of(att_off, t1 - 1, p0, nent - 1);
+    }
+    __syncthreads();
+    const int64_t lo = win[0], hi = win[1];
+    for (int k = 0; k < kSpawnTile / kBlock; ++k) {
+      const int64_t t = t0 + k * kBlock + threadIdx.x;
+      if (t >= t1) break;
+      const int64_t e = parent_of(att_off, t, lo, hi);
       const int sp = static_cast<int>(e % s.S);
       const int64_t i = e / s.S;
       uint64_t f[FQ_MAX_WORDS];
@@ -385,30 +422,44 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
       const int64_t parent = pop[sp * cap + i];
       fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
                       cyc ^ (fqd::kDrawAttempt << 24));
-      nspawn = fqd::spawn_attempt(s, rng, f, parent, tau, child);
+      uint64_t child[FQ_MAX_WORDS] = {0, 0};
+      const int64_t nspawn = fqd::spawn_attempt(s, rng, f, parent, tau, child);
+      const int64_t u = t - t_begin;
+      for (int w = 0; w < s.W; ++w) st_det[w * bcap + u] = child[w];
+      st_pop[u] = nspawn;
+      st_ent[u] = e;
+      st_flag[u] = nspawn != 0 ? 1 : 0;
     }
-    const bool ok = nspawn != 0;
-    const unsigned long long mask = __ballot(ok);
-    if (mask == 0) continue;
-    const int cnt = __popcll(mask);
-    const int leader = __builtin_ctzll(mask);
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(reinterpret_cast<unsigned long long*>(&st->nspawn), cnt);
-    base = __shfl(base, leader);
-    if (ok) {
-      const int64_t slot = static_cast<int64_t>(base) + __popcll(mask & ((1ull << lane) - 1));
-      if (slot < scap) {
-        const int sp = static_cast<int>(e % s.S);
-        const int64_t i = e / s.S;
-        for (int w = 0; w < s.W; ++w) sdet[w * scap + slot] = child[w];
-        for (int q = 0; q < s.S; ++q) spop[q * scap + slot] = (q == sp) ? nspawn : 0;
-        sflag[slot] = s.initiator ? flags[i] : 0;
-        sord[slot] = static_cast<uint64_t>(t);
-      } else {
-        atomicExch(reinterpret_cast<unsigned long long*>(&st->spawn_overflow), 1ull);
-      }
-    }
+    __syncthreads();
   }
+}
+
+// Pack the successful attempts of a batch into the spawned list, after the
+// entries of earlier batches (st->nspawn), in attempt order.
+__global__ void k_compact(Sys s, int64_t nb, int64_t t_begin, const uint64_t* __restrict__ st_det,
+                          const int64_t* __restrict__ st_pop, const int64_t* __restrict__ st_ent,
+                          const int64_t* __restrict__ st_flag, const int64_t* __restrict__ pos, int64_t bcap,
+                          const int64_t* __restrict__ flags, int64_t scap, uint64_t* __restrict__ sdet,
+                          int64_t* __restrict__ spop, int64_t* __restrict__ sflag, uint64_t* __restrict__ sord,
+                          Stats* __restrict__ st) {
+  const int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= nb || !st_flag[u]) return;
+  const int64_t slot = st->nspawn + pos[u];
+  if (slot >= scap) {
+    st->spawn_overflow = 1;
+    return;
+  }
+  const int64_t e = st_ent[u];
+  const int sp = static_cast<int>(e % s.S);
+  const int64_t i = e / s.S;
+  for (int w = 0; w < s.W; ++w) sdet[w * scap + slot] = st_det[w * bcap + u];
+  for (int q = 0; q < s.S; ++q) spop[q * scap + slot] = (q == sp) ? st_pop[u] : 0;
+  sflag[slot] = s.initiator ? flags[i] : 0;
+  sord[slot] = static_cast<uint64_t>(t_begin + u);
+}
+
+__global__ void k_add_spawned(const int64_t* __restrict__ pos, int64_t nb, Stats* __restrict__ st) {
+  st->nspawn += pos[nb];
 }
 
 // Replay: the whole spawning/death loop on one thread, in reference order.
@@ -916,7 +967,7 @@ __global__ void k_scatter_new(Sys s, int64_t nruns, int64_t cap, int64_t scap, c
 }
 
 __global__ __launch_bounds__(kBlock) void k_abs_pop(int64_t n, int64_t cap, int S, const int64_t* __restrict__ pop,
-                                                    Stats* __restrict__ st) {
+                                                    int64_t* __restrict__ part_i) {
   __shared__ unsigned long long sh[FQ_MAX_SPACES][kBlock];
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
@@ -934,7 +985,23 @@ __global__ __launch_bounds__(kBlock) void k_abs_pop(int64_t n, int64_t cap, int 
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int sp = 0; sp < S; ++sp) atomicAdd(&st->abs_pop[sp], sh[sp][0]);
+    for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) part_i[sp * gridDim.x + blockIdx.x] = static_cast<int64_t>(sh[sp][0]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_abs(const int64_t* __restrict__ part_i, int nb, Stats* __restrict__ st) {
+  __shared__ unsigned long long sh[FQ_MAX_SPACES][kBlock];
+  unsigned long long acc[FQ_MAX_SPACES] = {0, 0};
+  for (int b = threadIdx.x; b < nb; b += kBlock)
+    for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) acc[sp] += static_cast<unsigned long long>(part_i[sp * nb + b]);
+  for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) sh[sp][threadIdx.x] = acc[sp];
+  __syncthreads();
+  for (int k = kBlock / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k)
+      for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) sh[sp][threadIdx.x] += sh[sp][threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) st->abs_pop[sp] = sh[sp][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -971,6 +1038,8 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
   const int64_t scap = e->scap;
   uint32_t* perm = nullptr;
   uint64_t* keys = nullptr;
+  e->radix_passes = 0;
+  FQ_HIP(hipEventRecord(e->pev[4], e->stream));
   if (ns > 0) {
     if (replay) {
       hipLaunchKernelGGL(k_replay_qsort, dim3(1), dim3(1), 0, e->stream, e->sdet, ns, W, scap, e->vbuf[0],
@@ -999,11 +1068,15 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
           hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src], e->vbuf[src], ns,
                              W, w, shift, e->hist_scan, e->kbuf[1 - src], e->vbuf[1 - src], scap);
           src = 1 - src;
+          ++e->radix_passes;
         }
       }
       perm = e->vbuf[src];
       keys = e->kbuf[src];
     }
+  }
+  FQ_HIP(hipEventRecord(e->pev[5], e->stream));
+  if (ns > 0) {
     hipLaunchKernelGGL(k_heads, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, keys, ns, W, scap, e->head);
     int rc = scan_exclusive(e, e->head, e->head_scan, ns);
     if (rc != FQ_OK) return rc;
@@ -1015,9 +1088,10 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
     FQ_HIP(hipMemcpyAsync(&nruns, e->head_scan + ns, sizeof(int64_t), hipMemcpyDeviceToHost, e->stream));
     FQ_HIP(hipStreamSynchronize(e->stream));
     hipLaunchKernelGGL(k_reduce_runs, dim3(grid_for(nruns)), dim3(kBlock), 0, e->stream, e->sys, nruns,
-                       e->run_start, keys, perm, e->spop, e->sflag, e->sord, scap, replay ? 1 : 0, e->rdet, e->rpop,
+                       e->run_start, keys, perm, e->spop, e->sflag, e->sord, scap, 1, e->rdet, e->rpop,
                        e->rflag, e->rstate);
   }
+  FQ_HIP(hipEventRecord(e->pev[6], e->stream));
   e->h_stats.nruns = nruns;
   return FQ_OK;
 }
@@ -1031,6 +1105,7 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
   const int64_t n = e->n, cap = e->cap, scap = e->scap;
   const int c = e->cur;
   hipStream_t st = e->stream;
+  FQ_HIP(hipEventRecord(e->pev[7], st));
   if (nruns > 0)
     hipLaunchKernelGGL(k_merge_runs, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, n, cap, e->det[c],
                        e->pop_new, e->rdet, e->rpop, e->rflag, scap, e->rpos, e->rstate);
@@ -1079,11 +1154,14 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
   if (nruns > 0)
     hipLaunchKernelGGL(k_scatter_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, cap, scap, e->rdet,
                        e->rpop, e->rpos, e->nsurv, e->nrank, e->mrank, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
+  FQ_HIP(hipEventRecord(e->pev[8], st));
   rc = sync_stats(e);
   if (rc != FQ_OK) return rc;
   e->cur = o;
   e->n = e->h_stats.new_n;
-  hipLaunchKernelGGL(k_abs_pop, dim3(grid_for(e->n)), dim3(kBlock), 0, st, e->n, cap, e->S, e->pop[o], e->d_stats);
+  const unsigned nba = grid_for(e->n);
+  hipLaunchKernelGGL(k_abs_pop, dim3(nba), dim3(kBlock), 0, st, e->n, cap, e->S, e->pop[o], e->part_i);
+  hipLaunchKernelGGL(k_sum_abs, dim3(1), dim3(kBlock), 0, st, e->part_i, static_cast<int>(nba), e->d_stats);
   return sync_stats(e);
 }
 
@@ -1124,6 +1202,14 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->ins_scan, cap + 2);
   rc |= dalloc(&e->mdraw, cap + 1);
   rc |= dalloc(&e->mdraw_off, cap + 1);
+  rc |= dalloc(&e->part_d, 4 * (cap / kBlock + 2));
+  rc |= dalloc(&e->part_i, 2 * (cap / kBlock + 2));
+  e->bcap = scap;
+  rc |= dalloc(&e->st_det, W * e->bcap);
+  rc |= dalloc(&e->st_pop, e->bcap);
+  rc |= dalloc(&e->st_ent, e->bcap);
+  rc |= dalloc(&e->st_flag, e->bcap + 1);
+  rc |= dalloc(&e->st_pos, e->bcap + 1);
   rc |= dalloc(&e->sdet, W * scap);
   rc |= dalloc(&e->spop, S * scap);
   rc |= dalloc(&e->sflag, scap);
@@ -1233,6 +1319,23 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
     rc |= upload(&e->d_ssbf, sys->sym_spin_basis_fns, 2 * sys->nsym_pg * sys->max_sym_spin, e->stream);
     rc |= upload(&e->d_h1, sys->h1, static_cast<int64_t>(sys->norb) * sys->norb, e->stream);
     rc |= upload(&e->d_eri, sys->eri, sys->n_eri, e->stream);
+    // Dense Coulomb/exchange pair tables for the diagonal rule: entries copied
+    // from the packed store (same values, no re-computation).
+    const int no = sys->norb;
+    std::vector<double> J(static_cast<size_t>(no) * no), K(static_cast<size_t>(no) * no);
+    auto tri = [](int64_t i, int64_t j) { return (i * (i - 1)) / 2 + j; };
+    auto idx = [&](int p, int q, int r, int t) {  // (pq|rt) packed, 1-based spatial
+      const int64_t a = p >= q ? tri(p, q) : tri(q, p), b = r >= t ? tri(r, t) : tri(t, r);
+      return (a >= b ? tri(a, b) : tri(b, a)) - 1;
+    };
+    for (int pp = 1; pp <= no; ++pp)
+      for (int qq = 1; qq <= no; ++qq) {
+        J[(pp - 1) * no + (qq - 1)] = sys->eri[idx(pp, pp, qq, qq)];  // <ij|ij> = (ii|jj)
+        K[(pp - 1) * no + (qq - 1)] = sys->eri[idx(pp, qq, qq, pp)];  // <ij|ji> = (ij|ji)
+      }
+    if (rc == FQ_OK) rc = upload(&e->d_jmat, J.data(), static_cast<int64_t>(J.size()), e->stream);
+    if (rc == FQ_OK) rc = upload(&e->d_kmat, K.data(), static_cast<int64_t>(K.size()), e->stream);
+    if (rc == FQ_OK) FQ_HIP(hipStreamSynchronize(e->stream));
   }
   s.sp_eigv = e->d_sp_eigv;
   s.sym_table = e->d_sym_table;
@@ -1242,9 +1345,13 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   s.ssbf = e->d_ssbf;
   s.h1 = e->d_h1;
   s.eri = e->d_eri;
+  s.jmat = e->d_jmat;
+  s.kmat = e->d_kmat;
   if (rc == FQ_OK) rc = alloc_engine(e);
   for (int k = 0; k < 4 && rc == FQ_OK; ++k)
     if (hipEventCreate(&e->ev[k]) != hipSuccess) rc = FQ_ERR_DEVICE;
+  for (int k = 0; k < 9 && rc == FQ_OK; ++k)
+    if (hipEventCreate(&e->pev[k]) != hipSuccess) rc = FQ_ERR_DEVICE;
   if (rc != FQ_OK) {
     fq_destroy(e);
     return rc;
@@ -1308,12 +1415,16 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
   const double sh1 = e->S > 1 ? shift[1] : 0.0, e1 = e->S > 1 ? eproj_old[1] : 0.0;
   FQ_HIP(hipMemsetAsync(e->d_stats, 0, sizeof(Stats), st));
   FQ_HIP(hipEventRecord(e->ev[0], st));
+  FQ_HIP(hipEventRecord(e->pev[0], st));
   int64_t rng_pos = e->rng_pos;
   int64_t ns = 0;
   if (replay) {
     hipLaunchKernelGGL(k_replay_walk, dim3(1), dim3(1), 0, st, e->sys, n, cap, e->det[c], e->pop[c], e->kii[c],
                        e->pop_new, tau, shift[0], sh1, eproj_old[0], e1, e->rng_buf, e->rng_pos, e->rng_end, e->scap,
                        e->sdet, e->spop, e->sflag, e->sord, e->d_stats);
+    FQ_HIP(hipEventRecord(e->pev[1], st));
+    FQ_HIP(hipEventRecord(e->pev[2], st));
+    FQ_HIP(hipEventRecord(e->pev[3], st));
     int rc = sync_stats(e);
     if (rc != FQ_OK) return rc;
     if (e->h_stats.rng_failed) return FQ_ERR_RNG_EXHAUSTED;
@@ -1321,14 +1432,13 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
     ns = e->h_stats.nspawn;
   } else {
     const unsigned nb = grid_for(n);
-    double* partials = reinterpret_cast<double*>(e->mdraw);  // scratch: 4*nb doubles <= cap+1 int64
-    if (4 * static_cast<int64_t>(nb) > cap + 1) return FQ_ERR_CAPACITY;
     if (n > 0)
       hipLaunchKernelGGL(k_det_pass, dim3(nb), dim3(kBlock), 0, st, e->sys, n, cap, e->det[c], e->pop[c], e->kii[c],
                          e->pop_new, e->nat, e->flags, tau, shift[0], sh1, eproj_old[0], e1, e->par.philox_seed, cyc,
-                         partials, e->d_stats);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partials, n > 0 ? static_cast<int>(nb) : 0,
-                       e->d_stats);
+                         e->part_d, e->part_i);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, e->part_d, e->part_i,
+                       n > 0 ? static_cast<int>(nb) : 0, e->d_stats);
+    FQ_HIP(hipEventRecord(e->pev[1], st));
     // Attempt offsets over the n*S (det, space) entries.
     const int64_t nent = e->S * n;
     int rc = scan_exclusive(e, e->nat, e->att_off, nent);
@@ -1336,13 +1446,22 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
     int64_t total = 0;
     FQ_HIP(hipMemcpyAsync(&total, e->att_off + nent, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     FQ_HIP(hipStreamSynchronize(st));
-    if (total > 0) {
-      const int64_t want = ceil_div(total, kBlock);
-      const unsigned g = static_cast<unsigned>(std::min<int64_t>(want, 65536));
-      hipLaunchKernelGGL(k_spawn, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, total, e->att_off, e->det[c],
-                         e->pop[c], e->flags, tau, e->par.philox_seed, cyc, e->scap, e->sdet, e->spop, e->sflag,
+    FQ_HIP(hipEventRecord(e->pev[2], st));
+    for (int64_t tb = 0; tb < total; tb += e->bcap) {
+      const int64_t te = std::min(total, tb + e->bcap);
+      const int64_t nbt = te - tb;
+      const unsigned g = static_cast<unsigned>(std::min<int64_t>(ceil_div(nbt, kSpawnTile), 1 << 20));
+      hipLaunchKernelGGL(k_spawn, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
+                         e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
+                         e->st_flag);
+      rc = scan_exclusive(e, e->st_flag, e->st_pos, nbt);
+      if (rc != FQ_OK) return rc;
+      hipLaunchKernelGGL(k_compact, dim3(grid_for(nbt)), dim3(kBlock), 0, st, e->sys, nbt, tb, e->st_det, e->st_pop,
+                         e->st_ent, e->st_flag, e->st_pos, e->bcap, e->flags, e->scap, e->sdet, e->spop, e->sflag,
                          e->sord, e->d_stats);
+      hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, st, e->st_pos, nbt, e->d_stats);
     }
+    FQ_HIP(hipEventRecord(e->pev[3], st));
     rc = sync_stats(e);
     if (rc != FQ_OK) return rc;
     e->h_stats.total_attempts = total;
@@ -1358,9 +1477,31 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
   e->h_stats.total_attempts = keep.total_attempts;
   const int64_t used = rng_pos - e->rng_pos;
   e->rng_pos = rng_pos;
-  hipEventElapsedTime(&e->t_spawn, e->ev[0], e->ev[1]);
-  hipEventElapsedTime(&e->t_annih, e->ev[1], e->ev[2]);
-  hipEventElapsedTime(&e->t_total, e->ev[0], e->ev[2]);
+  (void)hipEventElapsedTime(&e->t_spawn, e->ev[0], e->ev[1]);
+  (void)hipEventElapsedTime(&e->t_annih, e->ev[1], e->ev[2]);
+  (void)hipEventElapsedTime(&e->t_total, e->ev[0], e->ev[2]);
+  {
+    float t[6] = {0, 0, 0, 0, 0, 0};
+    (void)hipEventElapsedTime(&t[0], e->pev[0], e->pev[1]);
+    (void)hipEventElapsedTime(&t[1], e->pev[2], e->pev[3]);
+    (void)hipEventElapsedTime(&t[2], e->pev[4], e->pev[5]);
+    (void)hipEventElapsedTime(&t[3], e->pev[5], e->pev[6]);
+    (void)hipEventElapsedTime(&t[4], e->pev[7], e->pev[8]);
+    (void)hipEventElapsedTime(&t[5], e->pev[0], e->pev[8]);
+    fq_profile_out& pr = e->prof;
+    pr.det_pass_ms = t[0];
+    pr.spawn_ms = t[1];
+    pr.sort_ms = t[2];
+    pr.reduce_ms = t[3];
+    pr.merge_ms = t[4];
+    pr.total_ms = t[5];
+    pr.nstates_in = n;
+    pr.nattempts = e->h_stats.total_attempts;
+    pr.nspawned = ns;
+    pr.nruns = e->h_stats.nruns;
+    pr.nstates_out = e->n;
+    pr.radix_passes = e->radix_passes;
+  }
   fill_out(e, out, used);
   return FQ_OK;
 }
@@ -1439,6 +1580,12 @@ int fq_download(fq_engine* e, uint64_t* states, int64_t* pops, double* dat, int6
   return FQ_OK;
 }
 
+int fq_last_profile(fq_engine* e, fq_profile_out* out) {
+  if (!e || !out) return FQ_ERR_ARG;
+  *out = e->prof;
+  return FQ_OK;
+}
+
 int fq_last_timing(fq_engine* e, double* spawn_ms, double* annihilate_ms, double* total_ms) {
   if (!e) return FQ_ERR_ARG;
   if (spawn_ms) *spawn_ms = e->t_spawn;
@@ -1449,8 +1596,8 @@ int fq_last_timing(fq_engine* e, double* spawn_ms, double* annihilate_ms, double
 
 void fq_destroy(fq_engine* e) {
   if (!e) return;
-  hipSetDevice(e->device);
-  if (e->stream) hipStreamSynchronize(e->stream);
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
   for (int b = 0; b < 2; ++b) {
     dfree(e->det[b]);
     dfree(e->pop[b]);
@@ -1468,6 +1615,13 @@ void fq_destroy(fq_engine* e) {
   dfree(e->ins_scan);
   dfree(e->mdraw);
   dfree(e->mdraw_off);
+  dfree(e->part_d);
+  dfree(e->part_i);
+  dfree(e->st_det);
+  dfree(e->st_pop);
+  dfree(e->st_ent);
+  dfree(e->st_flag);
+  dfree(e->st_pos);
   dfree(e->sdet);
   dfree(e->spop);
   dfree(e->sflag);
@@ -1497,9 +1651,13 @@ void fq_destroy(fq_engine* e) {
   dfree(e->d_ssbf);
   dfree(e->d_h1);
   dfree(e->d_eri);
+  dfree(e->d_jmat);
+  dfree(e->d_kmat);
   for (int k = 0; k < 4; ++k)
-    if (e->ev[k]) hipEventDestroy(e->ev[k]);
-  if (e->stream) hipStreamDestroy(e->stream);
+    if (e->ev[k]) (void)hipEventDestroy(e->ev[k]);
+  for (int k = 0; k < 9; ++k)
+    if (e->pev[k]) (void)hipEventDestroy(e->pev[k]);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }
 

@@ -153,6 +153,24 @@ int fq_download(fq_engine *e, uint64_t *states, int64_t *pops, double *dat, int6
 /* Device-side timing of the last cycle's kernels (milliseconds). */
 int fq_last_timing(fq_engine *e, double *spawn_ms, double *annihilate_ms, double *total_ms);
 
+/* Per-phase device times of the last fq_cycle, from HIP events recorded on
+ * the engine's stream around each phase, and the sizes that phase handled. */
+typedef struct fq_profile_out {
+  double det_pass_ms;    /* per-determinant pass: estimators, nattempts, death */
+  double spawn_ms;       /* spawning-attempt kernel */
+  double sort_ms;        /* radix sort (or reference quicksort) of the spawned list */
+  double reduce_ms;      /* run detection and self-annihilation */
+  double merge_ms;       /* merge with the main list, rounding, compaction, insertion */
+  double total_ms;       /* whole cycle, first to last kernel */
+  int64_t nstates_in;    /* occupied determinants at the start of the cycle */
+  int64_t nattempts;     /* spawning attempts */
+  int64_t nspawned;      /* spawned entries written */
+  int64_t nruns;         /* unique spawned determinants */
+  int64_t nstates_out;   /* occupied determinants after annihilation */
+  int32_t radix_passes;  /* 8-bit digit passes of the sort */
+} fq_profile_out;
+int fq_last_profile(fq_engine *e, fq_profile_out *out);
+
 void fq_destroy(fq_engine *e);
 
 #ifdef __cplusplus

@@ -101,6 +101,10 @@ class Oracle {
   std::vector<double> dat;    // K_ii
   double nparticles[FQ_MAX_SPACES] = {0, 0};
   bool main_error = false, spawn_error = false;
+  // Order of equal keys before self-annihilation: the reference's quicksort
+  // (default) or a stable sort (the first entry of a run is then the earliest
+  // spawned, the rule the GPU's production mode uses).
+  bool stable_order = false;
 
   void load(const uint64_t* st, const int64_t* p, const double* d, int64_t n) {
     states.resize(n);
@@ -929,7 +933,11 @@ class Oracle {
 
   void annihilate() {
     if (!spawned.empty()) {
-      qsort_spawned(spawned);
+      if (stable_order)
+        std::stable_sort(spawned.begin(), spawned.end(),
+                         [&](const Spawned& a, const Spawned& b) { return cmp(a.f, b.f) == 1; });
+      else
+        qsort_spawned(spawned);
       if (par_.initiator)
         self_annihilate_initiator();
       else
@@ -1194,6 +1202,7 @@ int orc_download(void* h, uint64_t* st, int64_t* p, double* d) {
 uint64_t orc_rng_consumed(void* h) { return static_cast<Oracle*>(h)->rng_consumed(); }
 void orc_projected_energy(void* h, double* proj, double* d0) { static_cast<Oracle*>(h)->projected_energy(proj, d0); }
 double orc_nparticles(void* h, int s) { return static_cast<Oracle*>(h)->nparticles[s]; }
+void orc_set_stable_order(void* h, int on) { static_cast<Oracle*>(h)->stable_order = on != 0; }
 
 int orc_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows, int max_rows, int* nrows) {
   return run_fciqmc(sys, in, rows, max_rows, nrows);

@@ -15,6 +15,7 @@ Outputs (small JSON data files, no reference source):
     reference determinant).  Row selection and tolerance follow
     tools/pyhande/pyhande/testcode.py:57-61 and test_suite/userconfig:14-15.
 """
+import gzip
 import hashlib
 import json
 import os
@@ -204,7 +205,14 @@ def main():
                     break
             entry["eigenvalues"] = eig
         # data files the test reads (integral dumps) are referenced, not copied
-        entry["data_files"] = [x for x in os.listdir(path) if x in ("FCIDUMP", "INTDUMP")]
+        entry["data_files"] = []
+        for x in os.listdir(path):
+            if x in ("FCIDUMP", "INTDUMP") and not name.startswith("fci_") and not name.endswith("-Lz"):
+                # the integral dump the deck reads: test input data, stored gzipped
+                dst = os.path.join(HERE, f"{name}.{x}.gz")
+                with open(os.path.join(path, x), "rb") as src, gzip.open(dst, "wb", compresslevel=9) as out:
+                    out.write(src.read())
+                entry["data_files"].append(os.path.basename(dst))
         goldens[name] = entry
     with open(os.path.join(HERE, "qmc_goldens.json"), "w") as f:
         json.dump(goldens, f, indent=1)

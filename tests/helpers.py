@@ -28,7 +28,8 @@ def goldens():
 
 
 def oracle():
-    if not os.path.exists(ORACLE_SO):
+    src = os.path.join(ROOT, "oracle", "fqmc_oracle.cpp")
+    if not os.path.exists(ORACLE_SO) or (os.path.exists(src) and os.path.getmtime(src) > os.path.getmtime(ORACLE_SO)):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), os.path.join(ROOT, "oracle", "liboracle.so")],
                        check=True)
     lib = C.CDLL(ORACLE_SO)
@@ -45,6 +46,7 @@ def oracle():
     lib.orc_download.argtypes = [vp, P(C.c_uint64), P(C.c_int64), P(C.c_double)]
     lib.orc_rng_consumed.restype = C.c_uint64
     lib.orc_rng_consumed.argtypes = [vp]
+    lib.orc_set_stable_order.argtypes = [vp, C.c_int]
     lib.orc_fciqmc.argtypes = [P(N.FqSystem), P(N.FqhQmcIn), P(N.FqhReportRow), C.c_int, P(C.c_int)]
     return lib
 
@@ -90,3 +92,20 @@ def engine_params(system_view, st):
     p.spawn_capacity = 1 << 16
     p.device = 0
     return p
+
+
+def materialize_data(name, directory):
+    """Write a golden entry's integral dump(s) (stored gzipped) into directory."""
+    import gzip
+    for fname in goldens()[name].get("data_files", []):
+        target = os.path.join(directory, fname.split(".", 1)[1][:-3])  # "<name>.INTDUMP.gz" -> INTDUMP
+        with gzip.open(os.path.join(GOLDEN, fname), "rb") as src, open(target, "wb") as dst:
+            dst.write(src.read())
+    return directory
+
+
+def deck(name, directory, lib=None):
+    """(system, fciqmc options) for a golden entry, its data files in directory."""
+    e = goldens()[name]
+    materialize_data(name, directory)
+    return api.load_deck(e["input"], base_dir=directory, lib=lib)

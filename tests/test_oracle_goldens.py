@@ -28,3 +28,18 @@ def test_oracle_reproduces_golden_trajectory(name):
         st.nreports = TRUNCATE[name]
         ref = ref[:TRUNCATE[name] + 1]
     compare_tables(oracle_report(oracle(), system, st), ref)
+
+
+# Molecular decks (integral dumps from the reference's test suite): H2O with a
+# CAS(8,13) and point-group symmetry; Ne with the initiator approximation.
+MOLECULAR = {"H2O-RHF-cc-pVTZ": 60, "Ne-RHF-aug-cc-pVDZ": 80}
+
+
+@pytest.mark.parametrize("name", sorted(MOLECULAR))
+def test_oracle_reproduces_molecular_golden(name, tmp_path):
+    from helpers import deck
+    e = goldens()[name]
+    system, opts = deck(name, str(tmp_path), lib=N.lib())
+    st = api.qmc_in(opts["qmc"], opts.get("fciqmc"))
+    st.nreports = MOLECULAR[name]
+    compare_tables(oracle_report(oracle(), system, st), e["report"][:MOLECULAR[name] + 1])

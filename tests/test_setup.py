@@ -53,3 +53,23 @@ def test_hubbard_closed_forms():
     assert abs(s.H00 - (25 * 0.25 + 2 * (-4.0 - 4 * 2.0))) < 1e-12
     # diagonal element of the reference from the host Slater-Condon rule
     assert s.diagonal(s.f0()) == s.H00
+
+
+def test_read_in_h2o_cas_matches_reference(tmp_path):
+    """FCIDUMP reader with a CAS(8,13): frozen-core energy folded into E_core
+    and the one-body integrals; H00 and pattempt_single as printed by the
+    reference (H00 -76.02403856, pattempt_single 0.04938272)."""
+    from helpers import deck
+    e = goldens()["H2O-RHF-cc-pVTZ"]
+    system, _ = deck("H2O-RHF-cc-pVTZ", str(tmp_path), lib=N.lib())
+    v = system.view
+    assert (v.nbasis, v.nel) == (26, 8)
+    assert abs(v.H00 - e["json"]["H00"]) < 1e-8
+    assert abs(v.pattempt_single - e["json"]["pattempt_single"]) < 1e-8
+    assert abs(v.pattempt_single + v.pattempt_double - 1.0) < 1e-15
+    # the host diagonal rule agrees with H00 and the eight-fold ERI symmetry holds
+    assert system.diagonal(system.f0()) == v.H00
+    ms, sym, eps = system.basis()
+    assert ms[:4] == [1, -1, 1, -1]
+    assert all(eps[2 * i] == eps[2 * i + 1] for i in range(13))
+    assert all(eps[2 * i] <= eps[2 * i + 2] + 1e-12 for i in range(12))
