@@ -8,19 +8,25 @@ population 1e7, tau 1e-3, renormalised excitation generator, FP64 integrals.
 
 A step is one Monte Carlo cycle (spawning + death + annihilation) of the whole
 walker population resident in HBM.  The walker distribution is prepared
-untimed: 2e5 walkers on the Hartree--Fock determinant are propagated at a larger time
-step (0.01) until the population passes the 1e7 target and the shift has held
-it for 150 cycles, then W warm-up cycles run at tau = 1e-3 before K timed
-cycles.
+untimed: target/1000 walkers on the Hartree--Fock determinant are propagated
+at a larger time step (0.01) until the population passes the target and the
+shift (started from the projected energy) has held it for 150 cycles, then W
+warm-up cycles run at tau = 1e-3 before K timed cycles.
 
 Prints one JSON line (rank 0).  Extra fields:
-  roofline      dominant kernel phase: algorithmic HBM bytes per launch / its
-                average device time (HIP events on the engine's stream) vs the
-                8 TB/s HBM3E peak;
+  roofline      the spawning-attempt kernel (the largest device-time share):
+                algorithmic HBM bytes per launch / its average launch time
+                (HIP event pair around each launch on the engine's stream)
+                vs the 8 TB/s HBM3E peak;
   cpu_baseline  the CPU oracle (a serial restatement of the reference loop)
-                timed on one cycle of the same state on one host core.
+                timed on the same walker state: one instance per host core on
+                a slice of the determinants each (as MPI ranks would hold), and
+                one instance on one core over the whole list.
+Under ``rocprofv3 --selected-regions`` only the timed cycles are profiled
+(roctxProfilerResume/Pause bracket them).
 """
 import argparse
+import concurrent.futures
 import ctypes as C
 import json
 import math
@@ -42,6 +48,31 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+class ProfilerRegion:
+    """roctxProfilerPause/Resume when the roctx library is present (no-ops otherwise)."""
+
+    def __init__(self):
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1"):
+            try:
+                self.lib = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            for fn in ("roctxProfilerPause", "roctxProfilerResume"):
+                getattr(self.lib, fn).argtypes = [C.c_uint64]
+                getattr(self.lib, fn).restype = C.c_int
+
+    def pause(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+
+    def resume(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
 
 
 def make_engine(lib, system, target, seed, main_cap, spawn_cap):
@@ -122,8 +153,30 @@ def phase_bytes(prof, W=1, S=1):
     }
 
 
-def cpu_baseline(lib, system, eng, p, shift, eold, tau, max_seconds=30.0):
-    """Time the CPU oracle on one cycle from the GPU's current walker list."""
+def spawn_kernel_bytes(prof, W=1, S=1):
+    """Algorithmic HBM bytes of the spawning-attempt kernel in one cycle: every
+    parent's determinant words and population read once, every spawned entry
+    (determinant words, population per space, initiator flag) written once."""
+    return prof.nstates_in * (8 * W + 8 * S) + prof.nspawned * (8 * W + 8 * S + 8)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(lib, system, eng, p, shift, eold, tau, budget_s=10.0):
+    """Time the CPU oracle on the GPU's current walker list.
+
+    Parallel form: one oracle instance per host core (threads; the oracle's
+    ctypes calls release the GIL), each owning a contiguous slice of the
+    determinants with its own seed, as the reference's MPI ranks each own a
+    share of the list (no inter-rank exchange is timed, which favours the CPU).
+    Serial form: one instance over the whole list on one core.  Each form runs
+    whole cycles until ``budget_s`` seconds have passed (at least one)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import helpers
     orc = helpers.oracle()
@@ -137,17 +190,58 @@ def cpu_baseline(lib, system, eng, p, shift, eold, tau, max_seconds=30.0):
     got = C.c_int64()
     N.check(lib.fq_download(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)), po.ctypes.data_as(C.POINTER(C.c_int64)),
                             da.ctypes.data_as(C.POINTER(C.c_double)), nd, C.byref(got)), "fq_download")
-    h = orc.orc_create(system.view, C.byref(p), 7)
-    orc.orc_load(h, st.ctypes.data_as(C.POINTER(C.c_uint64)), po.ctypes.data_as(C.POINTER(C.c_int64)),
-                 da.ctypes.data_as(C.POINTER(C.c_double)), nd)
-    out = N.FqCycleOut()
+    walkers = np.abs(po).sum() / 2**31
+
+    def run(lo, hi, seed, stop_at):
+        h = orc.orc_create(system.view, C.byref(p), seed)
+        sl = slice(lo, hi)
+        sst = np.ascontiguousarray(st.reshape(nd, W)[sl]).reshape(-1)
+        spo = np.ascontiguousarray(po[sl])
+        sda = np.ascontiguousarray(da[sl])
+        orc.orc_load(h, sst.ctypes.data_as(C.POINTER(C.c_uint64)), spo.ctypes.data_as(C.POINTER(C.c_int64)),
+                     sda.ctypes.data_as(C.POINTER(C.c_double)), hi - lo)
+        out = N.FqCycleOut()
+        att, cyc = 0, 0
+        t0 = time.perf_counter()
+        while True:
+            orc.orc_cycle(h, shift, eold, tau, C.byref(out))
+            att += out.nattempts
+            cyc += 1
+            now = time.perf_counter()
+            if now >= stop_at or cyc >= 50:
+                break
+        orc.orc_destroy(h)
+        return att, cyc, now - t0
+
+    P = host_cores()
+    bounds = [nd * k // P for k in range(P + 1)]
     t0 = time.perf_counter()
-    orc.orc_cycle(h, shift, eold, tau, C.byref(out))
-    dt = time.perf_counter() - t0
-    orc.orc_destroy(h)
-    return {"value": out.nattempts / dt, "unit": "spawn attempts/s", "cores": 1, "kind": "port",
-            "sample": f"one full cycle ({out.nattempts} attempts, {nd} determinants) of the same "
-                      f"{np.abs(po).sum() / 2**31:.3g}-walker state, serial C++ oracle, {dt:.1f} s"}
+    with concurrent.futures.ThreadPoolExecutor(max_workers=P) as ex:
+        futs = [ex.submit(run, bounds[k], bounds[k + 1], 7 + k, t0 + budget_s) for k in range(P)]
+        res = [f.result() for f in futs]
+    wall_p = time.perf_counter() - t0
+    att_p = sum(r[0] for r in res)
+    cyc_p = min(r[1] for r in res)
+    att_1, cyc_1, wall_1 = run(0, nd, 7, time.perf_counter() + budget_s)
+    return {"value": att_p / wall_p, "unit": "spawn attempts/s", "cores": P, "kind": "port",
+            "sample": f"{P} oracle instances (serial C++ restatement of the reference cycle), one per host core, "
+                      f"each on 1/{P} of the {nd} determinants of the same {walkers:.3g}-walker state, "
+                      f">= {cyc_p} whole cycles each, {att_p} attempts in {wall_p:.1f} s",
+            "single_core": {"value": att_1 / wall_1, "cores": 1,
+                            "sample": f"one instance over all {nd} determinants, {cyc_1} cycles, "
+                                      f"{att_1} attempts in {wall_1:.1f} s"}}
+
+
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(dst[-4:], src[-4:]))
+    print(json.dumps({"copy_probe_bytes": 8 * n, "repeats": 3, "ok": ok}), flush=True)
 
 
 def main():
@@ -160,8 +254,16 @@ def main():
     ap.add_argument("--equil-cycles", type=int, default=1200, help="max untimed preparation cycles")
     ap.add_argument("--hold-cycles", type=int, default=150, help="preparation cycles after the shift starts varying")
     ap.add_argument("--equil-tau", type=float, default=0.01)
+    ap.add_argument("--prep-damping", type=float, default=0.1, help="shift damping while preparing")
+    ap.add_argument("--start-frac", type=float, default=1e-3, help="initial HF population / target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB three times (calibrates the HBM "
+                         "byte counters FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the walker-propagation path runs on a single MI355X here "
               f"(--gpus {args.gpus} requested). Multi-GPU sharding is a design, not a measured path.",
@@ -177,11 +279,13 @@ def main():
     eng, p = make_engine(lib, system, args.target, 7, cap, cap)
     rf = 1 << 31
     f0 = system.f0()
-    pop0 = np.array([int(args.target / 50) * rf], dtype=np.int64)
+    pop0 = np.array([max(1, int(args.target * args.start_frac)) * rf], dtype=np.int64)
     dat0 = np.zeros(1)
     N.check(lib.fq_load(eng, f0.ctypes.data_as(C.POINTER(C.c_uint64)), pop0.ctypes.data_as(C.POINTER(C.c_int64)),
                         dat0.ctypes.data_as(C.POINTER(C.c_double)), 1), "fq_load")
-    prop = Propagator(lib, eng, args.target)
+    region = ProfilerRegion()
+    region.pause()
+    prop = Propagator(lib, eng, args.target, damping=args.prep_damping)
     tau = 1e-3
     t0 = time.perf_counter()
     vary_at = None
@@ -194,11 +298,13 @@ def main():
                 f"({time.perf_counter() - t0:.1f}s)")
         if vary_at is not None and i - vary_at >= args.hold_cycles:
             break
+    prop.damping = 0.05  # the workload's shift damping from here on
     for i in range(args.warmup):
         prop.cycle(tau)
     profs = []
     attempts = 0
     wall = 0.0
+    region.resume()
     for i in range(args.steps):
         t1 = time.perf_counter()
         out = prop.cycle(tau)
@@ -207,17 +313,26 @@ def main():
         pr = N.FqProfileOut()
         lib.fq_last_profile(eng, C.byref(pr))
         profs.append(pr)
+    region.pause()
     value = attempts / wall
     last = prop.last
     phases = ["det_pass", "spawn", "sort", "reduce", "merge"]
     avg_ms = {ph: float(np.mean([getattr(pr, ph + "_ms") for pr in profs])) for ph in phases}
     avg_bytes = {ph: float(np.mean([phase_bytes(pr)[ph] for pr in profs])) for ph in phases}
-    dom = max(phases, key=lambda ph: avg_ms[ph])
-    achieved = avg_bytes[dom] / (avg_ms[dom] * 1e-3) / 1e9
+    launches = sum(pr.spawn_launches for pr in profs)
+    k_ms = sum(pr.spawn_kernel_ms for pr in profs) / launches
+    k_bytes = sum(spawn_kernel_bytes(pr) for pr in profs) / launches
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    # the spawning kernel dominates when its time exceeds every other phase's total
+    dominant = all(k_ms * launches / len(profs) >= avg_ms[ph] for ph in phases if ph != "spawn")
+    annih_ms = sum(pr.sort_ms + pr.reduce_ms + pr.merge_ms for pr in profs)
+    annih_rate = sum(pr.nspawned for pr in profs) / (annih_ms * 1e-3)
     log("[bench] phase ms: " + ", ".join(f"{ph}={avg_ms[ph]:.3f}" for ph in phases) +
         f", cycle={np.mean([pr.total_ms for pr in profs]):.3f}")
     log("[bench] phase GB/s (algorithmic): " +
         ", ".join(f"{ph}={avg_bytes[ph] / (avg_ms[ph] * 1e-3) / 1e9:.1f}" for ph in phases if avg_ms[ph] > 0))
+    log(f"[bench] k_spawn: {launches} launches, {k_ms:.3f} ms/launch, {k_bytes / 1e6:.1f} MB/launch, "
+        f"{achieved:.1f} GB/s; annihilated entries/s {annih_rate:.4g}")
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -245,8 +360,10 @@ def main():
             "attempts_per_step": attempts / args.steps,
             "parallelism": "1 GPU",
         },
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "annihilated_entries_per_s": annih_rate,
+        "roofline": {"bound": "hbm", "kernel": "k_spawn", "dominant": dominant, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launch_ms": k_ms, "bytes_per_launch": k_bytes, "launches": launches,
                      "phase_ms": avg_ms, "phase_bytes": avg_bytes},
         "cpu_baseline": cpu,
     }

@@ -13,10 +13,10 @@ LIB_PATH = os.path.join(HERE, "libfqmc.so")
 FQ_OK = 0
 FQ_ERR = {
     -1: "FQ_ERR_ARG", -2: "FQ_ERR_NOMEM", -3: "FQ_ERR_DEVICE", -4: "FQ_ERR_RNG_EXHAUSTED",
-    -5: "FQ_ERR_CAPACITY", -6: "FQ_ERR_NO_DEVICE",
+    -5: "FQ_ERR_CAPACITY", -6: "FQ_ERR_NO_DEVICE", -7: "FQ_ERR_BIASED",
 }
 FQ_SYS_HUB_K, FQ_SYS_READ_IN = 1, 2
-FQ_EXCIT_RENORM, FQ_EXCIT_NO_RENORM = 0, 1
+FQ_EXCIT_RENORM, FQ_EXCIT_NO_RENORM, FQ_EXCIT_HEAT_BATH = 0, 1, 2
 FQ_RNG_PHILOX, FQ_RNG_REPLAY = 0, 1
 FQH_SYM_AUFBAU, FQH_SYM_TOT, FQH_SYM_INDEX = 0, 1, 2
 FQ_MAX_SPACES = 2
@@ -61,7 +61,7 @@ class FqProfileOut(C.Structure):
     _fields_ = [
         ("det_pass_ms", f64), ("spawn_ms", f64), ("sort_ms", f64), ("reduce_ms", f64), ("merge_ms", f64),
         ("total_ms", f64), ("nstates_in", i64), ("nattempts", i64), ("nspawned", i64), ("nruns", i64),
-        ("nstates_out", i64), ("radix_passes", i32),
+        ("nstates_out", i64), ("radix_passes", i32), ("spawn_launches", i32), ("spawn_kernel_ms", f64),
     ]
 
 

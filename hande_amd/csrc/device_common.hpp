@@ -7,6 +7,8 @@
 //   src/hubbard_k.f90, src/hamiltonian_hub_k.f90:91-232   Hubbard-k elements
 //   src/excit_gen_hub_k.f90:29-159,469-770   Hubbard-k generators, pgen
 //   src/excit_gen_mol.f90:16-101,384-946,1140-1327   molecular renorm generator
+//   src/excit_gen_heat_bath_mol.F90:252-548, src/excit_gen_utils.f90:9-66,142-167,
+//       lib/local/alias.f90:13-184   heat-bath generator and the alias method
 //   src/hamiltonian_molecular.f90:73-346   molecular Slater--Condon rules
 //   src/molecular_integrals.F90:736-845   8-fold packed two-body index
 //   src/spawning.F90:215-315,680-766, src/stoch_utils.f90:51-136
@@ -46,6 +48,19 @@ struct Sys {
   const double* jmat;  // norb*norb: (pp|qq) = eri(i,j,i,j) for spatial p,q
   const double* kmat;  // norb*norb: (pq|qp) = eri(i,j,j,i)
   double psingle, pdouble;
+  // Heat-bath tables (excit_gen FQ_EXCIT_HEAT_BATH), i-major over 1-based spin
+  // orbitals: hb_ija[hb3(i,j,a)] = hb_ija%weights(a,j,i), hb_ija_tot[hb2(i,j)] =
+  // hb_ija%weights_tot(j,i), hb_ijab[hb4(i,j,a,b)] = hb_ijab%weights(b,a,j,i), ...
+  const double* hb_i;
+  const double* hb_ij;
+  const double* hb_ija;
+  const double* hb_ija_tot;
+  const double* hb_ija_u;
+  const int* hb_ija_k;
+  const double* hb_ijab;
+  const double* hb_ijab_tot;
+  const double* hb_ijab_u;
+  const int* hb_ijab_k;
   int initiator, quadrature, real_amps, excit_gen;
   double initiator_pop;
   int64_t rf, spawn_cutoff;
@@ -104,13 +119,15 @@ struct Philox {
     out[3] = x[3];
     have = 4;
   }
-  // 53-bit uniform double in [0,1) from two 32-bit words.
+  // Uniform double in [0,1) on the 2^-52 grid (as dSFMT's close-open values),
+  // from two 32-bit words.  On that grid r*n < n rounds below n for every
+  // integer n < 2^52, so int(r*n) is always a valid index in [0, n).
   __device__ double next() {
     if (have < 2) generate();
     uint32_t a = out[4 - have], b = out[5 - have];
     have -= 2;
-    uint64_t v = (static_cast<uint64_t>(a >> 5) << 26) | (b >> 6);
-    return static_cast<double>(v) * (1.0 / 9007199254740992.0);
+    uint64_t v = (static_cast<uint64_t>(a >> 6) << 26) | (b >> 6);
+    return static_cast<double>(v) * (1.0 / 4503599627370496.0);
   }
 };
 
@@ -715,12 +732,212 @@ __device__ inline int64_t spawn_mol(const Sys& s, Rng& rng, const uint64_t* f, i
   return nspawn;
 }
 
+
+// ---------------------------------------------------------------------------
+// Heat-bath generator (Holmes et al.; src/excit_gen_heat_bath_mol.F90:252-548)
+// ---------------------------------------------------------------------------
+constexpr int kHbMaxNel = 32;  // electrons a heat-bath determinant may hold here
+
+__device__ __forceinline__ int64_t hb2(const Sys& s, int i, int j) {
+  return static_cast<int64_t>(i - 1) * s.nbasis + (j - 1);
+}
+__device__ __forceinline__ int64_t hb3(const Sys& s, int i, int j, int a) { return hb2(s, i, j) * s.nbasis + (a - 1); }
+__device__ __forceinline__ int64_t hb4(const Sys& s, int i, int j, int a, int b) {
+  return hb3(s, i, j, a) * s.nbasis + (b - 1);
+}
+
+// select_weighted_value (lib/local/alias.f90:64-99): the alias tables of
+// generate_alias_tables built over n <= kHbMaxNel weights, then one draw r.
+__device__ inline int alias_pick(int n, const double* w, double tot, double r) {
+  double u[kHbMaxNel];
+  int8_t k[kHbMaxNel], under[kHbMaxNel], over[kHbMaxNel];
+  int nunder = 0, nover = 0;
+  const double scale = static_cast<double>(n) / tot;
+  for (int i = 1; i <= n; ++i) {
+    u[i - 1] = w[i - 1] * scale;
+    if (u[i - 1] <= 1.0)
+      under[nunder++] = static_cast<int8_t>(i);
+    else
+      over[nover++] = static_cast<int8_t>(i);
+    k[i - 1] = static_cast<int8_t>(i);
+  }
+  while (nover > 0 && nunder > 0) {
+    const int ov = over[nover - 1], un = under[nunder - 1];
+    k[un - 1] = static_cast<int8_t>(ov);
+    nunder = nunder - 1;
+    u[ov - 1] = u[ov - 1] - (1.0 - u[un - 1]);
+    if (u[ov - 1] < 1.0) {
+      under[nunder++] = over[nover - 1];
+      nover = nover - 1;
+    }
+  }
+  double x = r * n;
+  int kk = static_cast<int>(floor(x));
+  x = x - kk;
+  kk = kk + 1;
+  return x < u[kk - 1] ? kk : k[kk - 1];
+}
+// select_weighted_value_precalc (lib/local/alias.f90:13-62) on a stored table.
+__device__ __forceinline__ int alias_draw(int n, const double* u, const int* k, double r) {
+  double x = r * n;
+  int kk = static_cast<int>(floor(x));
+  x = x - kk;
+  kk = kk + 1;
+  return x < u[kk - 1] ? kk : k[kk - 1];
+}
+
+// Probability of a single i -> a given j (src/excit_gen_heat_bath_mol.F90:367-380).
+__device__ inline double hb_psingle(const Sys& s, const uint64_t* f, int i, int j, int a) {
+  if (!(s.orb_sym[a - 1] == s.orb_sym[i - 1] && ms_of(a) == ms_of(i))) return 0.0;
+  const double hm = fabs(sc1_excit(s, f, i, a, false));
+  const double t = s.hb_ijab_tot[hb3(s, i, j, a)];
+  return hm < t ? hm / (t + hm) : 0.5;
+}
+
+// gen_excit_mol_heat_bath + attempt_to_spawn.
 template <class Rng>
+__device__ inline int64_t spawn_heat_bath(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
+                                          Excit& conn) {
+  const int nel = s.nel;
+  int occ[kHbMaxNel];
+  {
+    int k = 0;
+    for (int w = 0; w < s.W; ++w) {
+      uint64_t x = f[w];
+      while (x) {
+        occ[k++] = (w << 6) + __builtin_ctzll(x) + 1;
+        x &= x - 1;
+      }
+    }
+  }
+  conn.nexcit = 0;
+  conn.from[0] = conn.from[1] = conn.to[0] = conn.to[1] = 0;
+  conn.perm = false;
+  // find_i_d_weights + select_ij_heat_bath
+  double iw[kHbMaxNel], ijw[kHbMaxNel];
+  double iw_tot = 0.0;
+  for (int k = 0; k < nel; ++k) {
+    iw[k] = s.hb_i[occ[k] - 1];
+    iw_tot = iw_tot + iw[k];
+  }
+  const int i_ind = alias_pick(nel, iw, iw_tot, rng.next());
+  const int i = occ[i_ind - 1];
+  double ij_tot = 0.0;
+  for (int k = 0; k < nel; ++k) {
+    ijw[k] = s.hb_ij[hb2(s, i, occ[k])];
+    ij_tot = ij_tot + ijw[k];
+  }
+  bool allowed = false;
+  int j = 0, j_ind = 0;
+  if (ij_tot > 0.0) {
+    j_ind = alias_pick(nel, ijw, ij_tot, rng.next());
+    j = occ[j_ind - 1];
+    allowed = true;
+  }
+  allowed = allowed && fabs(s.hb_ija_tot[hb2(s, i, j)]) > 0.0;
+  int a = 0;
+  bool dbl = false;
+  double psingle = 0.0, hm_ia = 0.0;
+  if (allowed) {
+    a = alias_draw(s.nbasis, s.hb_ija_u + hb3(s, i, j, 1), s.hb_ija_k + hb3(s, i, j, 1), rng.next());
+    if (!btest(f, a)) {
+      if (s.orb_sym[a - 1] == s.orb_sym[i - 1] && ms_of(a) == ms_of(i)) {
+        conn.nexcit = 1;
+        conn.from[0] = i;
+        conn.to[0] = a;
+        conn.perm = perm1(f, s.W, i, a);
+        hm_ia = fabs(sc1_excit(s, f, i, a, conn.perm));
+        const double x = rng.next();
+        const double t = s.hb_ijab_tot[hb3(s, i, j, a)];
+        psingle = hm_ia < t ? hm_ia / (t + hm_ia) : 0.5;
+        dbl = !(x < psingle);
+      } else {
+        dbl = true;
+        psingle = 0.0;
+      }
+    } else {
+      allowed = false;
+    }
+  }
+  double pgen = 1.0, h = 0.0;
+  if (allowed) {
+    if (dbl) {
+      const int b = alias_draw(s.nbasis, s.hb_ijab_u + hb4(s, i, j, a, 1), s.hb_ijab_k + hb4(s, i, j, a, 1),
+                               rng.next());
+      if (!btest(f, b)) {
+        double ji_tot = 0.0;
+        for (int k = 0; k < nel; ++k) ji_tot = ji_tot + s.hb_ij[hb2(s, j, occ[k])];
+        const double pi = iw[i_ind - 1] / iw_tot, pj = ijw[j_ind - 1] / ij_tot;
+        const double pj2 = iw[j_ind - 1] / iw_tot, pi2 = s.hb_ij[hb2(s, j, i)] / ji_tot;
+        const double pgen_ija = (pi * pj) * (s.hb_ija[hb3(s, i, j, a)] / s.hb_ija_tot[hb2(s, i, j)]) *
+                                (1.0 - psingle) * (s.hb_ijab[hb4(s, i, j, a, b)] / s.hb_ijab_tot[hb3(s, i, j, a)]);
+        double ps = hb_psingle(s, f, i, j, b);
+        const double pgen_ijb = (pi * pj) * (s.hb_ija[hb3(s, i, j, b)] / s.hb_ija_tot[hb2(s, i, j)]) * (1.0 - ps) *
+                                (s.hb_ijab[hb4(s, i, j, b, a)] / s.hb_ijab_tot[hb3(s, i, j, b)]);
+        ps = hb_psingle(s, f, j, i, a);
+        const double pgen_jia = (pj2 * pi2) * (s.hb_ija[hb3(s, j, i, a)] / s.hb_ija_tot[hb2(s, j, i)]) *
+                                (1.0 - ps) * (s.hb_ijab[hb4(s, j, i, a, b)] / s.hb_ijab_tot[hb3(s, j, i, a)]);
+        ps = hb_psingle(s, f, j, i, b);
+        const double pgen_jib = (pj2 * pi2) * (s.hb_ija[hb3(s, j, i, b)] / s.hb_ija_tot[hb2(s, j, i)]) *
+                                (1.0 - ps) * (s.hb_ijab[hb4(s, j, i, b, a)] / s.hb_ijab_tot[hb3(s, j, i, b)]);
+        pgen = pgen_ija + pgen_ijb + pgen_jia + pgen_jib;
+        conn.nexcit = 2;
+        conn.from[0] = i < j ? i : j;
+        conn.from[1] = i < j ? j : i;
+        conn.to[0] = a < b ? a : b;
+        conn.to[1] = a < b ? b : a;
+        conn.perm = perm2(f, s.W, conn);
+        h = sc2_excit(s, conn.from[0], conn.from[1], conn.to[0], conn.to[1], conn.perm);
+      } else {
+        allowed = false;
+      }
+    } else {
+      h = sc1_excit(s, f, i, a, conn.perm);
+      pgen = 0.0;
+      for (int q = 0; q < nel; ++q) {
+        const int oq = occ[q];
+        if (i != oq && a != oq) {
+          const double t = s.hb_ijab_tot[hb3(s, i, oq, a)];
+          const double psq = hm_ia < t ? hm_ia / (t + hm_ia) : 0.5;
+          pgen = pgen + (psq * (ijw[q] / ij_tot) * (s.hb_ija[hb3(s, i, oq, a)] / s.hb_ija_tot[hb2(s, i, oq)]));
+        }
+      }
+      pgen = pgen * (iw[i_ind - 1] / iw_tot);
+    }
+  }
+  if (!allowed) {
+    h = 0.0;
+    pgen = 1.0;
+    conn.nexcit = 0;
+  }
+  // attempt_to_spawn (src/spawning.F90:711-766)
+  double pspawn = tau * fabs(h) / pgen;
+  pspawn = pspawn * static_cast<double>(s.rf);
+  int64_t nspawn = stoch_round_spawn(s, rng, pspawn);
+  if (nspawn > 0) nspawn = (h > 0.0) ? -fsign64(nspawn, parent) : fsign64(nspawn, parent);
+  return nspawn;
+}
+
+// Generator families; the production spawning kernel is compiled once per
+// family so each carries only its own generator's registers.
+enum Gen : int { kGenHubK = 0, kGenMolRenorm = 1, kGenHeatBath = 2, kGenAny = 3 };
+__host__ __device__ inline int gen_of(int system, int excit_gen) {
+  if (system == FQ_SYS_HUB_K) return kGenHubK;
+  return excit_gen == FQ_EXCIT_HEAT_BATH ? kGenHeatBath : kGenMolRenorm;
+}
+
+template <int G = kGenAny, class Rng>
 __device__ inline int64_t spawn_attempt(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
                                         uint64_t* child) {
   Excit conn;
-  const int64_t n = (s.system == FQ_SYS_HUB_K) ? spawn_hub_k(s, rng, f, parent, tau, conn)
-                                               : spawn_mol(s, rng, f, parent, tau, conn);
+  const int g = G == kGenAny ? gen_of(s.system, s.excit_gen) : G;
+  int64_t n;
+  if ((G == kGenAny || G == kGenHubK) && g == kGenHubK)
+    n = spawn_hub_k(s, rng, f, parent, tau, conn);
+  else if ((G == kGenAny || G == kGenHeatBath) && g == kGenHeatBath)
+    n = spawn_heat_bath(s, rng, f, parent, tau, conn);
+  else
+    n = spawn_mol(s, rng, f, parent, tau, conn);
   if (n != 0) excited_det(f, s.W, conn, child);
   return n;
 }

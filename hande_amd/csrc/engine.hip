@@ -148,6 +148,10 @@ struct fq_engine {
   // device tables
   int *d_sym_table = nullptr, *d_inv_sym = nullptr, *d_orb_sym = nullptr, *d_nbss = nullptr, *d_ssbf = nullptr;
   double *d_sp_eigv = nullptr, *d_h1 = nullptr, *d_eri = nullptr, *d_jmat = nullptr, *d_kmat = nullptr;
+  // heat-bath tables (excit_gen FQ_EXCIT_HEAT_BATH)
+  double *d_hb_i = nullptr, *d_hb_ij = nullptr, *d_hb_ija = nullptr, *d_hb_ija_tot = nullptr, *d_hb_ija_u = nullptr;
+  double *d_hb_ijab = nullptr, *d_hb_ijab_tot = nullptr, *d_hb_ijab_u = nullptr;
+  int *d_hb_ija_k = nullptr, *d_hb_ijab_k = nullptr, *d_hb_bias = nullptr;
   // main list (double-buffered)
   int64_t cap = 0, n = 0;
   int cur = 0;
@@ -213,6 +217,10 @@ struct fq_engine {
   // phase events: 0 cycle start, 1 det pass end, 2 spawn start, 3 spawn end,
   // 4 sort start, 5 sort end, 6 reduce end, 7 merge start, 8 merge end
   hipEvent_t pev[9] = {};
+  // an event pair around each spawning-kernel launch of a cycle (the first kSpawnEvents)
+  static constexpr int kSpawnEvents = 16;
+  hipEvent_t sev[2 * kSpawnEvents] = {};
+  int spawn_launches = 0;
   fq_profile_out prof{};
   int radix_passes = 0;
 };
@@ -393,6 +401,7 @@ constexpr int kSpawnTile = 4 * kBlock;  // attempts per block tile
 // searches its own parent.  The result of attempt t goes to slot t - t_begin
 // of the staging arrays (coalesced, no atomics, attempt order); k_compact packs
 // the successful ones into the spawned list.
+template <int G>
 __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t cap, int64_t t_begin, int64_t t_end,
                                                   const int64_t* __restrict__ att_off,
                                                   const uint64_t* __restrict__ det, const int64_t* __restrict__ pop,
@@ -423,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
       fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
                       cyc ^ (fqd::kDrawAttempt << 24));
       uint64_t child[FQ_MAX_WORDS] = {0, 0};
-      const int64_t nspawn = fqd::spawn_attempt(s, rng, f, parent, tau, child);
+      const int64_t nspawn = fqd::spawn_attempt<G>(s, rng, f, parent, tau, child);
       const int64_t u = t - t_begin;
       for (int w = 0; w < s.W; ++w) st_det[w * bcap + u] = child[w];
       st_pop[u] = nspawn;
@@ -1183,6 +1192,166 @@ void fill_out(fq_engine* e, fq_cycle_out* out, int64_t rng_used) {
   out->main_overflow = s.main_overflow ? 1 : 0;
 }
 
+// ---- heat-bath tables: init_excit_mol_heat_bath (src/excit_gen_heat_bath_mol.F90:14-248)
+// The reference's loops (i, j, a, b in that nesting) run in serial order per
+// table entry: every sum below adds its terms in the reference's order.
+
+// One thread per (i, j, a): the b loop; weights(b,a,j,i), weights_tot(a,j,i)
+// and hb_ija%weights(a,j,i) (the same sum: invalid b contribute +0.0).
+__global__ void k_hb_ijab(Sys s, double* __restrict__ ijab_w, double* __restrict__ ijab_tot,
+                          double* __restrict__ ija_w) {
+  const int nb = s.nbasis;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<int64_t>(nb) * nb * nb) return;
+  const int i = static_cast<int>(t / (static_cast<int64_t>(nb) * nb)) + 1;
+  const int j = static_cast<int>((t / nb) % nb) + 1;
+  const int a = static_cast<int>(t % nb) + 1;
+  double* w = ijab_w + t * nb;
+  double tot = 0.0;
+  if (i != j && a != i && a != j) {
+    const int it = i < j ? i : j, jt = i < j ? j : i;
+    const int isymb = s.orb_sym[it - 1] ^ s.orb_sym[jt - 1] ^ s.orb_sym[a - 1];
+    for (int b = 1; b <= nb; ++b) {
+      double wb = 0.0;
+      const bool spin_ok = (fqd::ms_of(it) == fqd::ms_of(a) && fqd::ms_of(jt) == fqd::ms_of(b)) ||
+                           (fqd::ms_of(it) == fqd::ms_of(b) && fqd::ms_of(jt) == fqd::ms_of(a));
+      if (spin_ok && s.orb_sym[b - 1] == isymb && b != a && b != i && b != j)
+        wb = wb + fabs(fqd::sc2_excit(s, it, jt, a < b ? a : b, a < b ? b : a, false));
+      w[b - 1] = wb;
+      tot = tot + wb;
+    }
+  } else {
+    for (int b = 0; b < nb; ++b) w[b] = 0.0;
+  }
+  ijab_tot[t] = tot;
+  ija_w[t] = tot;
+}
+
+// One thread per (i, j): ij_weights(j,i) as the flat (a, b) sum and
+// hb_ija%weights_tot(j,i) as the sum over a of the per-a totals.
+__global__ void k_hb_ij(Sys s, const double* __restrict__ ijab_w, const double* __restrict__ ija_w,
+                        double* __restrict__ ij_w, double* __restrict__ ija_tot) {
+  const int nb = s.nbasis;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<int64_t>(nb) * nb) return;
+  double flat = 0.0, tot = 0.0;
+  if (t / nb != t % nb) {
+    const double* w = ijab_w + t * nb * nb;
+    for (int64_t ab = 0; ab < static_cast<int64_t>(nb) * nb; ++ab) flat = flat + w[ab];
+    for (int a = 0; a < nb; ++a) tot = tot + ija_w[t * nb + a];
+  }
+  ij_w[t] = flat;
+  ija_tot[t] = tot;
+}
+
+// One thread per i: i_weights(i) = sum over j of ij_weights(j,i), and the
+// reference's bias test (src/excit_gen_heat_bath_mol.F90:185-202) per i.
+__global__ void k_hb_i(Sys s, const double* __restrict__ ij_w, const double* __restrict__ ija_w,
+                       double* __restrict__ i_w, int* __restrict__ bias) {
+  const int nb = s.nbasis;
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + 1;
+  if (i > nb) return;
+  double iw = 0.0;
+  for (int j = 1; j <= nb; ++j) iw = iw + ij_w[static_cast<int64_t>(i - 1) * nb + (j - 1)];
+  i_w[i - 1] = iw;
+  int biased = 0;
+  for (int a = 1; a <= nb; ++a) {
+    int nz = 0;
+    for (int j = 1; j <= nb; ++j)
+      if (ija_w[(static_cast<int64_t>(i - 1) * nb + (j - 1)) * nb + (a - 1)] > fqd::kDepsilon) ++nz;
+    if (nz < nb - s.nel && i != a && s.orb_sym[a - 1] == s.orb_sym[i - 1] && fqd::ms_of(a) == fqd::ms_of(i))
+      biased = 1;
+  }
+  bias[i - 1] = biased;
+}
+
+// generate_alias_tables (lib/local/alias.f90:101-184) for ntab tables of n
+// entries, one thread per table; table t is built when gate[t / gate_div] and
+// tot[t] are nonzero (the reference builds no table otherwise).
+__global__ void k_hb_alias(int64_t ntab, int n, const double* __restrict__ w, const double* __restrict__ tot,
+                           const double* __restrict__ gate, int64_t gate_div, double* __restrict__ u,
+                           int* __restrict__ k) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= ntab) return;
+  if (!(fabs(gate[t / gate_div]) > 0.0) || !(fabs(tot[t]) > 0.0)) return;
+  const double* wt = w + t * n;
+  double* ut = u + t * n;
+  int* kt = k + t * n;
+  uint8_t under[128], over[128];
+  int nunder = 0, nover = 0;
+  const double scale = static_cast<double>(n) / tot[t];
+  for (int i = 1; i <= n; ++i) {
+    ut[i - 1] = wt[i - 1] * scale;
+    if (ut[i - 1] <= 1.0)
+      under[nunder++] = static_cast<uint8_t>(i);
+    else
+      over[nover++] = static_cast<uint8_t>(i);
+    kt[i - 1] = i;
+  }
+  while (nover > 0 && nunder > 0) {
+    const int ov = over[nover - 1], un = under[nunder - 1];
+    kt[un - 1] = ov;
+    nunder = nunder - 1;
+    ut[ov - 1] = ut[ov - 1] - (1.0 - ut[un - 1]);
+    if (ut[ov - 1] < 1.0) {
+      under[nunder++] = over[nover - 1];
+      nover = nover - 1;
+    }
+  }
+}
+
+int build_heat_bath(fq_engine* e) {
+  Sys& s = e->sys;
+  const int64_t nb = s.nbasis, n2 = nb * nb, n3 = n2 * nb, n4 = n3 * nb;
+  int rc = FQ_OK;
+  rc |= dalloc(&e->d_hb_i, nb);
+  rc |= dalloc(&e->d_hb_ij, n2);
+  rc |= dalloc(&e->d_hb_ija_tot, n2);
+  rc |= dalloc(&e->d_hb_ija, n3);
+  rc |= dalloc(&e->d_hb_ija_u, n3);
+  rc |= dalloc(&e->d_hb_ija_k, n3);
+  rc |= dalloc(&e->d_hb_ijab_tot, n3);
+  rc |= dalloc(&e->d_hb_ijab, n4);
+  rc |= dalloc(&e->d_hb_ijab_u, n4);
+  rc |= dalloc(&e->d_hb_ijab_k, n4);
+  rc |= dalloc(&e->d_hb_bias, nb);
+  if (rc != FQ_OK) return FQ_ERR_NOMEM;
+  hipStream_t st = e->stream;
+  FQ_HIP(hipMemsetAsync(e->d_hb_ija_u, 0, sizeof(double) * n3, st));
+  FQ_HIP(hipMemsetAsync(e->d_hb_ija_k, 0, sizeof(int) * n3, st));
+  FQ_HIP(hipMemsetAsync(e->d_hb_ijab_u, 0, sizeof(double) * n4, st));
+  FQ_HIP(hipMemsetAsync(e->d_hb_ijab_k, 0, sizeof(int) * n4, st));
+  hipLaunchKernelGGL(k_hb_ijab, dim3(static_cast<unsigned>(ceil_div(n3, kBlock))), dim3(kBlock), 0, st, s,
+                     e->d_hb_ijab, e->d_hb_ijab_tot, e->d_hb_ija);
+  hipLaunchKernelGGL(k_hb_ij, dim3(static_cast<unsigned>(ceil_div(n2, kBlock))), dim3(kBlock), 0, st, s,
+                     e->d_hb_ijab, e->d_hb_ija, e->d_hb_ij, e->d_hb_ija_tot);
+  hipLaunchKernelGGL(k_hb_i, dim3(static_cast<unsigned>(ceil_div(nb, kBlock))), dim3(kBlock), 0, st, s, e->d_hb_ij,
+                     e->d_hb_ija, e->d_hb_i, e->d_hb_bias);
+  hipLaunchKernelGGL(k_hb_alias, dim3(static_cast<unsigned>(ceil_div(n2, kBlock))), dim3(kBlock), 0, st, n2,
+                     static_cast<int>(nb), e->d_hb_ija, e->d_hb_ija_tot, e->d_hb_ija_tot, int64_t{1}, e->d_hb_ija_u,
+                     e->d_hb_ija_k);
+  hipLaunchKernelGGL(k_hb_alias, dim3(static_cast<unsigned>(ceil_div(n3, kBlock))), dim3(kBlock), 0, st, n3,
+                     static_cast<int>(nb), e->d_hb_ijab, e->d_hb_ijab_tot, e->d_hb_ija_tot, nb, e->d_hb_ijab_u,
+                     e->d_hb_ijab_k);
+  FQ_HIP(hipGetLastError());
+  std::vector<int> bias(static_cast<size_t>(nb));
+  FQ_HIP(hipMemcpyAsync(bias.data(), e->d_hb_bias, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+  FQ_HIP(hipStreamSynchronize(st));
+  for (int v : bias)
+    if (v) return FQ_ERR_BIASED;
+  s.hb_i = e->d_hb_i;
+  s.hb_ij = e->d_hb_ij;
+  s.hb_ija = e->d_hb_ija;
+  s.hb_ija_tot = e->d_hb_ija_tot;
+  s.hb_ija_u = e->d_hb_ija_u;
+  s.hb_ija_k = e->d_hb_ija_k;
+  s.hb_ijab = e->d_hb_ijab;
+  s.hb_ijab_tot = e->d_hb_ijab_tot;
+  s.hb_ijab_u = e->d_hb_ijab_u;
+  s.hb_ijab_k = e->d_hb_ijab_k;
+  return FQ_OK;
+}
+
 int alloc_engine(fq_engine* e) {
   const int64_t cap = e->cap, scap = e->scap;
   const int W = e->W, S = e->S;
@@ -1257,6 +1426,10 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   if (sys->nwords < 1 || sys->nwords > FQ_MAX_WORDS || params->nspaces < 1 || params->nspaces > FQ_MAX_SPACES)
     return FQ_ERR_ARG;
   if (sys->system == FQ_SYS_READ_IN && sys->nsym_pg > 8) return FQ_ERR_ARG;
+  if (params->excit_gen < FQ_EXCIT_RENORM || params->excit_gen > FQ_EXCIT_HEAT_BATH) return FQ_ERR_ARG;
+  if (params->excit_gen == FQ_EXCIT_HEAT_BATH && (sys->system != FQ_SYS_READ_IN || sys->nel > fqd::kHbMaxNel))
+    return FQ_ERR_ARG;
+  if (sys->system == FQ_SYS_READ_IN && params->excit_gen == FQ_EXCIT_NO_RENORM) return FQ_ERR_ARG;
   if (params->main_capacity < 1 || params->spawn_capacity < 1) return FQ_ERR_ARG;
   if (params->spawn_capacity > 0xffffffffLL) return FQ_ERR_ARG;
   int ndev = 0;
@@ -1347,11 +1520,14 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   s.eri = e->d_eri;
   s.jmat = e->d_jmat;
   s.kmat = e->d_kmat;
+  if (rc == FQ_OK && params->excit_gen == FQ_EXCIT_HEAT_BATH) rc = build_heat_bath(e);
   if (rc == FQ_OK) rc = alloc_engine(e);
   for (int k = 0; k < 4 && rc == FQ_OK; ++k)
     if (hipEventCreate(&e->ev[k]) != hipSuccess) rc = FQ_ERR_DEVICE;
   for (int k = 0; k < 9 && rc == FQ_OK; ++k)
     if (hipEventCreate(&e->pev[k]) != hipSuccess) rc = FQ_ERR_DEVICE;
+  for (int k = 0; k < 2 * fq_engine::kSpawnEvents && rc == FQ_OK; ++k)
+    if (hipEventCreate(&e->sev[k]) != hipSuccess) rc = FQ_ERR_DEVICE;
   if (rc != FQ_OK) {
     fq_destroy(e);
     return rc;
@@ -1447,13 +1623,21 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
     FQ_HIP(hipMemcpyAsync(&total, e->att_off + nent, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     FQ_HIP(hipStreamSynchronize(st));
     FQ_HIP(hipEventRecord(e->pev[2], st));
+    e->spawn_launches = 0;
     for (int64_t tb = 0; tb < total; tb += e->bcap) {
       const int64_t te = std::min(total, tb + e->bcap);
       const int64_t nbt = te - tb;
       const unsigned g = static_cast<unsigned>(std::min<int64_t>(ceil_div(nbt, kSpawnTile), 1 << 20));
-      hipLaunchKernelGGL(k_spawn, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
+      const int li = e->spawn_launches++;
+      if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li], st));
+      const int gen = fqd::gen_of(e->sys.system, e->sys.excit_gen);
+      auto kern = gen == fqd::kGenHubK ? k_spawn<fqd::kGenHubK>
+                  : gen == fqd::kGenHeatBath ? k_spawn<fqd::kGenHeatBath>
+                                             : k_spawn<fqd::kGenMolRenorm>;
+      hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
                          e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
                          e->st_flag);
+      if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
       rc = scan_exclusive(e, e->st_flag, e->st_pos, nbt);
       if (rc != FQ_OK) return rc;
       hipLaunchKernelGGL(k_compact, dim3(grid_for(nbt)), dim3(kBlock), 0, st, e->sys, nbt, tb, e->st_det, e->st_pop,
@@ -1501,6 +1685,13 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
     pr.nruns = e->h_stats.nruns;
     pr.nstates_out = e->n;
     pr.radix_passes = e->radix_passes;
+    pr.spawn_launches = replay ? 0 : std::min(e->spawn_launches, fq_engine::kSpawnEvents);
+    pr.spawn_kernel_ms = 0.0;
+    for (int li = 0; li < pr.spawn_launches; ++li) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e->sev[2 * li], e->sev[2 * li + 1]);
+      pr.spawn_kernel_ms += ms;
+    }
   }
   fill_out(e, out, used);
   return FQ_OK;
@@ -1653,10 +1844,23 @@ void fq_destroy(fq_engine* e) {
   dfree(e->d_eri);
   dfree(e->d_jmat);
   dfree(e->d_kmat);
+  dfree(e->d_hb_i);
+  dfree(e->d_hb_ij);
+  dfree(e->d_hb_ija);
+  dfree(e->d_hb_ija_tot);
+  dfree(e->d_hb_ija_u);
+  dfree(e->d_hb_ija_k);
+  dfree(e->d_hb_ijab);
+  dfree(e->d_hb_ijab_tot);
+  dfree(e->d_hb_ijab_u);
+  dfree(e->d_hb_ijab_k);
+  dfree(e->d_hb_bias);
   for (int k = 0; k < 4; ++k)
     if (e->ev[k]) (void)hipEventDestroy(e->ev[k]);
   for (int k = 0; k < 9; ++k)
     if (e->pev[k]) (void)hipEventDestroy(e->pev[k]);
+  for (int k = 0; k < 2 * fq_engine::kSpawnEvents; ++k)
+    if (e->sev[k]) (void)hipEventDestroy(e->sev[k]);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }

@@ -35,10 +35,16 @@ extern "C" {
 #define FQ_ERR_RNG_EXHAUSTED (-4)  /* replay stream ran out of draws mid-cycle */
 #define FQ_ERR_CAPACITY (-5)       /* initial walker list larger than the main list */
 #define FQ_ERR_NO_DEVICE (-6)      /* no usable GPU */
+#define FQ_ERR_BIASED (-7)         /* heat-bath tables cannot reach every allowed single excitation
+                                      (the stop_all of src/excit_gen_heat_bath_mol.F90:185-202) */
 
 /* ---- enumerations ------------------------------------------------------ */
 enum fq_system_kind { FQ_SYS_HUB_K = 1, FQ_SYS_READ_IN = 2 };
-enum fq_excit_gen { FQ_EXCIT_RENORM = 0, FQ_EXCIT_NO_RENORM = 1 };
+enum fq_excit_gen {
+  FQ_EXCIT_RENORM = 0,     /* excit_gen 'renorm' (Hubbard-k split generator / gen_excit_mol) */
+  FQ_EXCIT_NO_RENORM = 1,  /* excit_gen 'no_renorm' (Hubbard-k only) */
+  FQ_EXCIT_HEAT_BATH = 2   /* excit_gen 'heat_bath': gen_excit_mol_heat_bath, src/excit_gen_heat_bath_mol.F90:252-548 (read_in only) */
+};
 enum fq_rng_mode {
   FQ_RNG_PHILOX = 0, /* production: counter-based Philox4x32-10 per attempt */
   FQ_RNG_REPLAY = 1  /* correctness: consume a host-supplied dSFMT stream in reference order */
@@ -168,6 +174,8 @@ typedef struct fq_profile_out {
   int64_t nruns;         /* unique spawned determinants */
   int64_t nstates_out;   /* occupied determinants after annihilation */
   int32_t radix_passes;  /* 8-bit digit passes of the sort */
+  int32_t spawn_launches;    /* launches of the spawning-attempt kernel this cycle */
+  double spawn_kernel_ms;    /* summed device time of those launches (event pair around each) */
 } fq_profile_out;
 int fq_last_profile(fq_engine *e, fq_profile_out *out);
 

@@ -22,6 +22,8 @@
 //   src/excit_gen_hub_k.f90:29-159,469-770  Hubbard-k generators and pgen
 //   src/hamiltonian_hub_k.f90:91-232, src/hubbard_k.f90  Hubbard-k matrix elements
 //   src/excit_gen_mol.f90:16-101,384-946,1140-1327  molecular renorm generator
+//   src/excit_gen_heat_bath_mol.F90:14-548, src/excit_gen_utils.f90:9-66,142-167,
+//       lib/local/alias.f90:13-184  heat-bath tables, alias method, generator
 //   src/hamiltonian_molecular.f90:73-346  molecular Slater--Condon rules
 //   src/energy_evaluation.F90:425-711,795-986,1433-1454  estimators, shift
 //   lib/dSFMT_F03_interface/dSFMT_interface.F90:387-409  RNG stream
@@ -31,6 +33,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "dsfmt.hpp"
@@ -93,6 +96,7 @@ class Oracle {
     excit_mask.assign(static_cast<size_t>(nbasis + 1) * W, 0);
     for (int i = 1; i <= nbasis; ++i)
       for (int j = i + 1; j <= nbasis; ++j) excit_mask[i * W + (j - 1) / 64] |= uint64_t{1} << ((j - 1) % 64);
+    if (s->system == FQ_SYS_READ_IN && p->excit_gen == FQ_EXCIT_HEAT_BATH) build_heat_bath();
   }
 
   // ---- state ----------------------------------------------------------------
@@ -212,7 +216,16 @@ class Oracle {
   std::vector<uint64_t> excit_mask;
   std::vector<Spawned> spawned;
 
-  double rand() { return rng_.next(); }
+
+ public:
+  // Draws come from the dSFMT stream, or from a caller-supplied list (the
+  // generator probe used by the tests).
+  const double* forced_ = nullptr;
+  int64_t nforced_ = 0, iforced_ = 0;
+  double rand() {
+    if (forced_) return iforced_ < nforced_ ? forced_[iforced_++] : (++iforced_, 0.5);
+    return rng_.next();
+  }
   bool btest(const uint64_t* f, int orb) const { return (f[(orb - 1) / 64] >> ((orb - 1) % 64)) & 1; }
   int ms_of(int orb) const { return (orb & 1) ? 1 : -1; }
   int ims_of(int orb) const { return (ms_of(orb) + 3) / 2; }  // 1 = beta, 2 = alpha
@@ -450,6 +463,7 @@ class Oracle {
   // ---- spawners --------------------------------------------------------------
   int64_t spawn(const Decoded& d, const uint64_t* f, int64_t parent, double tau, Excit& conn) {
     if (sys_.system == FQ_SYS_HUB_K) return spawn_hub_k(d, f, parent, tau, conn);
+    if (par_.excit_gen == FQ_EXCIT_HEAT_BATH) return spawn_heat_bath(d, f, parent, tau, conn);
     return spawn_mol(d, f, parent, tau, conn);
   }
 
@@ -577,6 +591,273 @@ class Oracle {
     pspawn = pspawn * static_cast<double>(rf);
     int64_t nspawn = stoch_round_spawn(pspawn);
     if (nspawn > 0) nspawn = (h > 0.0) ? -fsign64(nspawn, parent) : fsign64(nspawn, parent);
+    return nspawn;
+  }
+
+  // ---- heat-bath generator ---------------------------------------------------
+  // Tables of init_excit_mol_heat_bath (src/excit_gen_heat_bath_mol.F90:14-248)
+  // with the loops in serial order (one OpenMP thread): i outermost, then j,
+  // a, b.  Stored i-major: ija(i,j,a) = hb_ija%weights(a,j,i), etc.
+ public:
+  struct HeatBath {
+    int nb = 0;
+    bool biased = false;
+    std::vector<double> i_w, ij_w, ija_w, ija_tot, ija_u, ijab_w, ijab_tot, ijab_u;
+    std::vector<int> ija_k, ijab_k;
+    size_t ij(int i, int j) const { return static_cast<size_t>(i - 1) * nb + (j - 1); }
+    size_t ija(int i, int j, int a) const { return ij(i, j) * nb + (a - 1); }
+    size_t ijab(int i, int j, int a, int b) const { return ija(i, j, a) * nb + (b - 1); }
+  } hb;
+
+  // generate_alias_tables (lib/local/alias.f90:101-184), 1-based outcomes.
+  static void alias_tables(int n, const double* w, double tot, double* u, int* k) {
+    std::vector<int> under(n), over(n);
+    int nunder = 0, nover = 0;
+    const double scale = static_cast<double>(n) / tot;
+    for (int i = 1; i <= n; ++i) {
+      u[i - 1] = w[i - 1] * scale;
+      if (u[i - 1] <= 1.0)
+        under[nunder++] = i;
+      else
+        over[nover++] = i;
+      k[i - 1] = i;
+    }
+    while (nover > 0 && nunder > 0) {
+      const int ov = over[nover - 1], un = under[nunder - 1];
+      k[un - 1] = ov;
+      nunder = nunder - 1;
+      u[ov - 1] = u[ov - 1] - (1.0 - u[un - 1]);
+      if (u[ov - 1] < 1.0) {
+        under[nunder++] = over[nover - 1];
+        nover = nover - 1;
+      }
+    }
+  }
+  // select_weighted_value_precalc (lib/local/alias.f90:13-62).
+  int alias_draw(int n, const double* u, const int* k) {
+    double x = rand() * n;
+    int kk = static_cast<int>(std::floor(x));
+    x = x - kk;
+    kk = kk + 1;
+    return x < u[kk - 1] ? kk : k[kk - 1];
+  }
+  // select_weighted_value (lib/local/alias.f90:64-99).
+  int weighted_draw(int n, const double* w, double tot) {
+    std::vector<double> u(n);
+    std::vector<int> k(n);
+    alias_tables(n, w, tot, u.data(), k.data());
+    return alias_draw(n, u.data(), k.data());
+  }
+
+ private:
+  void build_heat_bath() {
+    const int nb = nbasis;
+    hb.nb = nb;
+    const size_t n2 = static_cast<size_t>(nb) * nb, n3 = n2 * nb, n4 = n3 * nb;
+    hb.i_w.assign(nb, 0.0);
+    hb.ij_w.assign(n2, 0.0);
+    hb.ija_tot.assign(n2, 0.0);
+    hb.ija_w.assign(n3, 0.0);
+    hb.ijab_tot.assign(n3, 0.0);
+    hb.ijab_w.assign(n4, 0.0);
+    hb.ija_u.assign(n3, 0.0);
+    hb.ija_k.assign(n3, 0);
+    hb.ijab_u.assign(n4, 0.0);
+    hb.ijab_k.assign(n4, 0);
+    std::vector<int> j_nonzero(n2, 0);  // (i, a)
+    for (int i = 1; i <= nb; ++i) {
+      double i_weight = 0.0;
+      for (int j = 1; j <= nb; ++j) {
+        double ij_weight = 0.0;
+        if (i != j) {
+          const int it = std::min(i, j), jt = std::max(i, j);
+          const int ij_sym = orb_sym[it - 1] ^ orb_sym[jt - 1];
+          double ija_weights_tot = 0.0, i_weight_extra = 0.0;
+          for (int a = 1; a <= nb; ++a) {
+            double ija_weight = 0.0;
+            if (a != i && a != j) {
+              const int isymb = ij_sym ^ orb_sym[a - 1];
+              for (int b = 1; b <= nb; ++b) {
+                double ijab_weight = 0.0;
+                const bool spin_ok = (ms_of(it) == ms_of(a) && ms_of(jt) == ms_of(b)) ||
+                                     (ms_of(it) == ms_of(b) && ms_of(jt) == ms_of(a));
+                if (spin_ok && orb_sym[b - 1] == isymb && b != a && b != i && b != j) {
+                  const int at = std::min(a, b), bt = std::max(a, b);
+                  const double h = std::fabs(sc2_excit(it, jt, at, bt, false));
+                  i_weight_extra = i_weight_extra + h;
+                  ij_weight = ij_weight + h;
+                  ija_weight = ija_weight + h;
+                  ijab_weight = ijab_weight + h;
+                }
+                hb.ijab_w[hb.ijab(i, j, a, b)] = ijab_weight;
+                hb.ijab_tot[hb.ija(i, j, a)] = hb.ijab_tot[hb.ija(i, j, a)] + ijab_weight;
+              }
+            }
+            hb.ija_w[hb.ija(i, j, a)] = ija_weight;
+            ija_weights_tot = ija_weights_tot + ija_weight;
+            if (ija_weight > kDepsilon) j_nonzero[hb.ij(i, a)] += 1;
+          }
+          i_weight = i_weight + i_weight_extra;
+          hb.ija_tot[hb.ij(i, j)] = hb.ija_tot[hb.ij(i, j)] + ija_weights_tot;
+        }
+        hb.ij_w[hb.ij(i, j)] = ij_weight;
+      }
+      hb.i_w[i - 1] = i_weight;
+      // bias test (src/excit_gen_heat_bath_mol.F90:185-202), gamma_sym = 0
+      for (int a = 1; a <= nb; ++a)
+        if (j_nonzero[hb.ij(i, a)] < nb - nel && i != a && orb_sym[a - 1] == orb_sym[i - 1] &&
+            ms_of(a) == ms_of(i))
+          hb.biased = true;
+    }
+    for (int i = 1; i <= nb; ++i)
+      for (int j = 1; j <= nb; ++j) {
+        if (!(std::fabs(hb.ija_tot[hb.ij(i, j)]) > 0.0)) continue;
+        alias_tables(nb, &hb.ija_w[hb.ija(i, j, 1)], hb.ija_tot[hb.ij(i, j)], &hb.ija_u[hb.ija(i, j, 1)],
+                     &hb.ija_k[hb.ija(i, j, 1)]);
+        for (int a = 1; a <= nb; ++a)
+          if (std::fabs(hb.ijab_tot[hb.ija(i, j, a)]) > 0.0)
+            alias_tables(nb, &hb.ijab_w[hb.ijab(i, j, a, 1)], hb.ijab_tot[hb.ija(i, j, a)],
+                         &hb.ijab_u[hb.ijab(i, j, a, 1)], &hb.ijab_k[hb.ijab(i, j, a, 1)]);
+      }
+  }
+
+  // psingle of the single/double decision for i -> a given j
+  // (src/excit_gen_heat_bath_mol.F90:367-380).
+  double hb_psingle(const std::vector<int>& occ, int i, int j, int a) const {
+    if (!(orb_sym[a - 1] == orb_sym[i - 1] && ms_of(a) == ms_of(i))) return 0.0;
+    const double hm = std::fabs(sc1_excit(occ, i, a, false));
+    const double t = hb.ijab_tot[hb.ija(i, j, a)];
+    return hm < t ? hm / (t + hm) : 0.5;
+  }
+
+ public:
+  // Result of one call of the heat-bath generator (for the generator probe).
+  struct HbResult {
+    int allowed = 0, nexcit = 0, from[2] = {0, 0}, to[2] = {0, 0};
+    double pgen = 1.0, hmatel = 0.0;
+  };
+
+  // gen_excit_mol_heat_bath (src/excit_gen_heat_bath_mol.F90:252-548) with
+  // select_ij_heat_bath / find_i_d_weights (src/excit_gen_utils.f90:9-66,142-167).
+  HbResult gen_heat_bath(const std::vector<int>& occ, const uint64_t* f, Excit& conn) {
+    HbResult r;
+    conn = Excit();
+    std::vector<double> iw(nel), ijw(nel), jiw(nel);
+    double iw_tot = 0.0;
+    for (int k = 0; k < nel; ++k) {
+      iw[k] = hb.i_w[occ[k] - 1];
+      iw_tot = iw_tot + iw[k];
+    }
+    const int i_ind = weighted_draw(nel, iw.data(), iw_tot);
+    const int i = occ[i_ind - 1];
+    double ij_tot = 0.0;
+    for (int k = 0; k < nel; ++k) {
+      ijw[k] = hb.ij_w[hb.ij(i, occ[k])];
+      ij_tot = ij_tot + ijw[k];
+    }
+    bool allowed = false;
+    int j = 0, j_ind = 0;
+    double ji_tot = 0.0;
+    if (ij_tot > 0.0) {
+      j_ind = weighted_draw(nel, ijw.data(), ij_tot);
+      j = occ[j_ind - 1];
+      for (int k = 0; k < nel; ++k) {
+        jiw[k] = hb.ij_w[hb.ij(j, occ[k])];
+        ji_tot = ji_tot + jiw[k];
+      }
+      allowed = true;
+    }
+    allowed = allowed && std::fabs(hb.ija_tot[hb.ij(i, j)]) > 0.0;
+    int a = 0;
+    bool dbl = false;
+    double psingle = 0.0, hm_ia = 0.0;
+    if (allowed) {
+      a = alias_draw(nbasis, &hb.ija_u[hb.ija(i, j, 1)], &hb.ija_k[hb.ija(i, j, 1)]);
+      if (!btest(f, a)) {
+        if (orb_sym[a - 1] == orb_sym[i - 1] && ms_of(a) == ms_of(i)) {
+          conn.nexcit = 1;
+          conn.from[0] = i;
+          conn.to[0] = a;
+          perm1(f, conn);
+          hm_ia = std::fabs(sc1_excit(occ, i, a, conn.perm));
+          const double x = rand();
+          const double t = hb.ijab_tot[hb.ija(i, j, a)];
+          psingle = hm_ia < t ? hm_ia / (t + hm_ia) : 0.5;
+          dbl = !(x < psingle);
+        } else {
+          dbl = true;
+          psingle = 0.0;
+        }
+      } else {
+        allowed = false;
+      }
+    }
+    double pgen = 1.0, h = 0.0;
+    if (allowed) {
+      if (dbl) {
+        const int b = alias_draw(nbasis, &hb.ijab_u[hb.ijab(i, j, a, 1)], &hb.ijab_k[hb.ijab(i, j, a, 1)]);
+        if (!btest(f, b)) {
+          const double pi = iw[i_ind - 1] / iw_tot, pj = ijw[j_ind - 1] / ij_tot;
+          const double pj2 = iw[j_ind - 1] / iw_tot, pi2 = jiw[i_ind - 1] / ji_tot;
+          const double pgen_ija = (pi * pj) * (hb.ija_w[hb.ija(i, j, a)] / hb.ija_tot[hb.ij(i, j)]) *
+                                  (1.0 - psingle) * (hb.ijab_w[hb.ijab(i, j, a, b)] / hb.ijab_tot[hb.ija(i, j, a)]);
+          double ps = hb_psingle(occ, i, j, b);
+          const double pgen_ijb = (pi * pj) * (hb.ija_w[hb.ija(i, j, b)] / hb.ija_tot[hb.ij(i, j)]) * (1.0 - ps) *
+                                  (hb.ijab_w[hb.ijab(i, j, b, a)] / hb.ijab_tot[hb.ija(i, j, b)]);
+          ps = hb_psingle(occ, j, i, a);
+          const double pgen_jia = (pj2 * pi2) * (hb.ija_w[hb.ija(j, i, a)] / hb.ija_tot[hb.ij(j, i)]) * (1.0 - ps) *
+                                  (hb.ijab_w[hb.ijab(j, i, a, b)] / hb.ijab_tot[hb.ija(j, i, a)]);
+          ps = hb_psingle(occ, j, i, b);
+          const double pgen_jib = (pj2 * pi2) * (hb.ija_w[hb.ija(j, i, b)] / hb.ija_tot[hb.ij(j, i)]) * (1.0 - ps) *
+                                  (hb.ijab_w[hb.ijab(j, i, b, a)] / hb.ijab_tot[hb.ija(j, i, b)]);
+          pgen = pgen_ija + pgen_ijb + pgen_jia + pgen_jib;
+          conn = Excit();
+          conn.nexcit = 2;
+          conn.from[0] = std::min(i, j);
+          conn.from[1] = std::max(i, j);
+          conn.to[0] = std::min(a, b);
+          conn.to[1] = std::max(a, b);
+          perm2(f, conn);
+          h = sc2_excit(conn.from[0], conn.from[1], conn.to[0], conn.to[1], conn.perm);
+        } else {
+          allowed = false;
+        }
+      } else {
+        h = sc1_excit(occ, i, a, conn.perm);
+        pgen = 0.0;
+        for (int q = 0; q < nel; ++q) {
+          const int oq = occ[q];
+          if (i != oq && a != oq) {
+            const double t = hb.ijab_tot[hb.ija(i, oq, a)];
+            const double psq = hm_ia < t ? hm_ia / (t + hm_ia) : 0.5;
+            pgen = pgen + (psq * (ijw[q] / ij_tot) * (hb.ija_w[hb.ija(i, oq, a)] / hb.ija_tot[hb.ij(i, oq)]));
+          }
+        }
+        pgen = pgen * (iw[i_ind - 1] / iw_tot);
+      }
+    }
+    if (!allowed) {
+      h = 0.0;
+      pgen = 1.0;
+    }
+    r.allowed = allowed;
+    r.nexcit = allowed ? conn.nexcit : 0;
+    for (int k = 0; k < 2; ++k) {
+      r.from[k] = conn.from[k];
+      r.to[k] = conn.to[k];
+    }
+    r.pgen = pgen;
+    r.hmatel = h;
+    return r;
+  }
+
+  // spawn_standard with the heat-bath generator (src/spawning.F90:34-124).
+  int64_t spawn_heat_bath(const Decoded& d, const uint64_t* f, int64_t parent, double tau, Excit& conn) {
+    const HbResult r = gen_heat_bath(d.occ, f, conn);
+    double pspawn = tau * std::fabs(r.hmatel) / r.pgen;
+    pspawn = pspawn * static_cast<double>(rf);
+    int64_t nspawn = stoch_round_spawn(pspawn);
+    if (nspawn > 0) nspawn = (r.hmatel > 0.0) ? -fsign64(nspawn, parent) : fsign64(nspawn, parent);
     return nspawn;
   }
 
@@ -1203,6 +1484,51 @@ uint64_t orc_rng_consumed(void* h) { return static_cast<Oracle*>(h)->rng_consume
 void orc_projected_energy(void* h, double* proj, double* d0) { static_cast<Oracle*>(h)->projected_energy(proj, d0); }
 double orc_nparticles(void* h, int s) { return static_cast<Oracle*>(h)->nparticles[s]; }
 void orc_set_stable_order(void* h, int on) { static_cast<Oracle*>(h)->stable_order = on != 0; }
+
+// Heat-bath probes for the tests.
+int orc_hb_biased(void* h) { return static_cast<Oracle*>(h)->hb.biased ? 1 : 0; }
+// Copy a heat-bath table ("i_w", "ij_w", "ija_w", "ija_tot", "ija_u", "ijab_w", "ijab_tot", "ijab_u")
+// into out[0..n); returns its length (or -1 for an unknown name).
+int64_t orc_hb_table(void* h, const char* name, double* out, int64_t n) {
+  const auto& hb = static_cast<Oracle*>(h)->hb;
+  const std::vector<double>* t = nullptr;
+  const std::string nm(name);
+  if (nm == "i_w") t = &hb.i_w;
+  if (nm == "ij_w") t = &hb.ij_w;
+  if (nm == "ija_w") t = &hb.ija_w;
+  if (nm == "ija_tot") t = &hb.ija_tot;
+  if (nm == "ija_u") t = &hb.ija_u;
+  if (nm == "ijab_w") t = &hb.ijab_w;
+  if (nm == "ijab_tot") t = &hb.ijab_tot;
+  if (nm == "ijab_u") t = &hb.ijab_u;
+  if (!t) return -1;
+  const int64_t len = static_cast<int64_t>(t->size());
+  if (out) std::memcpy(out, t->data(), sizeof(double) * static_cast<size_t>(std::min(n, len)));
+  return len;
+}
+void orc_alias_tables(int n, const double* w, double tot, double* u, int* k) { Oracle::alias_tables(n, w, tot, u, k); }
+// One heat-bath generator call from determinant f with the given uniform draws.
+// ires = {allowed, nexcit, from1, from2, to1, to2}, dres = {pgen, hmatel};
+// returns the number of draws the call consumed.
+int64_t orc_hb_generate(void* h, const uint64_t* f, const double* draws, int64_t ndraws, int* ires, double* dres) {
+  Oracle* o = static_cast<Oracle*>(h);
+  o->forced_ = draws;
+  o->nforced_ = ndraws;
+  o->iforced_ = 0;
+  Excit conn;
+  const Oracle::HbResult r = o->gen_heat_bath(o->decode_occ(f), f, conn);
+  const int64_t used = o->iforced_;
+  o->forced_ = nullptr;
+  ires[0] = r.allowed;
+  ires[1] = r.nexcit;
+  ires[2] = r.from[0];
+  ires[3] = r.from[1];
+  ires[4] = r.to[0];
+  ires[5] = r.to[1];
+  dres[0] = r.pgen;
+  dres[1] = r.hmatel;
+  return used;
+}
 
 int orc_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows, int max_rows, int* nrows) {
   return run_fciqmc(sys, in, rows, max_rows, nrows);

@@ -1,0 +1,85 @@
+"""Exact ground state of a small synthetic molecular Hamiltonian.
+
+Dense diagonalisation over every Sz = 0 determinant, with matrix elements from
+the Slater--Condon rules written here directly from the integrals (spin
+orbital o: spatial (o+1)//2, odd = alpha) -- a brute-force check, independent
+of the engine and the oracle, for systems of a few hundred determinants."""
+import itertools
+
+import numpy as np
+
+from hande_amd import synthetic
+
+
+def _sp(o):
+    return (o - 1) // 2
+
+
+def _ms(o):
+    return 1 if o % 2 == 1 else -1
+
+
+def _phys(eri, p, q, r, s):
+    return eri[_sp(p), _sp(r), _sp(q), _sp(s)] if (_ms(p) == _ms(r) and _ms(q) == _ms(s)) else 0.0
+
+
+def _anti(eri, p, q, r, s):
+    return _phys(eri, p, q, r, s) - _phys(eri, p, q, s, r)
+
+
+def _element(h1, eri, ecore, bra, ket):
+    """<bra|H|ket> for sorted occupation tuples."""
+    sb, sk = set(bra), set(ket)
+    diff_b = sorted(sb - sk)
+    diff_k = sorted(sk - sb)
+    n = len(diff_b)
+    if n > 2:
+        return 0.0
+    if n == 0:
+        e = ecore
+        for i in ket:
+            e += h1[_sp(i), _sp(i)]
+        for a, b in itertools.combinations(ket, 2):
+            e += _anti(eri, a, b, a, b)
+        return e
+    # permutation sign bringing ket into maximal coincidence with bra
+    cur = list(ket)
+    sign = 1
+    for f, t in zip(diff_k, diff_b):
+        k = cur.index(f)
+        sign *= (-1) ** k
+        cur.pop(k)
+        pos = sum(1 for o in cur if o < t)
+        sign *= (-1) ** pos
+        cur.insert(pos, t)
+    if n == 1:
+        i, a = diff_k[0], diff_b[0]
+        v = h1[_sp(a), _sp(i)] if _ms(a) == _ms(i) else 0.0
+        for o in ket:
+            if o != i:
+                v += _anti(eri, a, o, i, o)
+        return sign * v
+    i, j = diff_k
+    a, b = diff_b
+    return sign * _anti(eri, a, b, i, j)
+
+
+def ground_state(norb, nel, seed):
+    """(E0, H[hf, hf]) of the synthetic system with nel electrons, Sz = 0."""
+    _eps, h1, eri = synthetic.synthetic_integrals(norb, seed)
+    alphas = [2 * k + 1 for k in range(norb)]
+    betas = [2 * k + 2 for k in range(norb)]
+    dets = []
+    for al in itertools.combinations(alphas, nel // 2):
+        for be in itertools.combinations(betas, nel // 2):
+            dets.append(tuple(sorted(al + be)))
+    n = len(dets)
+    H = np.zeros((n, n))
+    for x in range(n):
+        for y in range(x, n):
+            v = _element(h1, eri, 0.0, dets[x], dets[y])
+            H[x, y] = H[y, x] = v
+    w = np.linalg.eigvalsh(H)
+    hf = tuple(sorted(alphas[:nel // 2] + betas[:nel // 2]))
+    k = dets.index(hf)
+    return float(w[0]), float(H[k, k])

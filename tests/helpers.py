@@ -48,6 +48,12 @@ def oracle():
     lib.orc_rng_consumed.argtypes = [vp]
     lib.orc_set_stable_order.argtypes = [vp, C.c_int]
     lib.orc_fciqmc.argtypes = [P(N.FqSystem), P(N.FqhQmcIn), P(N.FqhReportRow), C.c_int, P(C.c_int)]
+    lib.orc_hb_biased.argtypes = [vp]
+    lib.orc_hb_table.restype = C.c_int64
+    lib.orc_hb_table.argtypes = [vp, C.c_char_p, P(C.c_double), C.c_int64]
+    lib.orc_alias_tables.argtypes = [C.c_int, P(C.c_double), C.c_double, P(C.c_double), P(C.c_int)]
+    lib.orc_hb_generate.restype = C.c_int64
+    lib.orc_hb_generate.argtypes = [vp, P(C.c_uint64), P(C.c_double), C.c_int64, P(C.c_int), P(C.c_double)]
     return lib
 
 
@@ -109,3 +115,82 @@ def deck(name, directory, lib=None):
     e = goldens()[name]
     materialize_data(name, directory)
     return api.load_deck(e["input"], base_dir=directory, lib=lib)
+
+
+def download(lib, eng, W, S):
+    """The engine's walker list (states, pops, K_ii) as numpy arrays."""
+    n = C.c_int64()
+    lib.fq_nstates(eng, C.byref(n))
+    cap = max(n.value, 1)
+    st = np.zeros(cap * W, dtype=np.uint64)
+    po = np.zeros(cap * S, dtype=np.int64)
+    da = np.zeros(cap)
+    out = C.c_int64()
+    N.check(lib.fq_download(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)), po.ctypes.data_as(C.POINTER(C.c_int64)),
+                            da.ctypes.data_as(C.POINTER(C.c_double)), cap, C.byref(out)), "fq_download")
+    k = out.value
+    return st[:k * W], po[:k * S], da[:k]
+
+
+def odownload(orc, h, W, S):
+    """The oracle's walker list (states, pops, K_ii) as numpy arrays."""
+    k = orc.orc_nstates(h)
+    st = np.zeros(max(k, 1) * W, dtype=np.uint64)
+    po = np.zeros(max(k, 1) * S, dtype=np.int64)
+    da = np.zeros(max(k, 1))
+    orc.orc_download(h, st.ctypes.data_as(C.POINTER(C.c_uint64)), po.ctypes.data_as(C.POINTER(C.c_int64)),
+                     da.ctypes.data_as(C.POINTER(C.c_double)))
+    return st[:k * W], po[:k * S], da[:k]
+
+
+def lockstep(system, st, ncycles, init_pop=None, shift=0.0, label=""):
+    """Run the engine (replay mode) and the oracle side by side from init_pop
+    walkers on the reference determinant, with the same dSFMT stream and shift;
+    assert bit-identical cycle outputs and walker lists after every cycle.
+    Returns the last cycle's output."""
+    lib = N.lib()
+    orc = oracle()
+    p = engine_params(system.view, st)
+    W, S = system.view.nwords, p.nspaces
+    eng = C.c_void_p()
+    N.check(lib.fq_create(system.view, C.byref(p), C.byref(eng)), "fq_create")
+    h = orc.orc_create(system.view, C.byref(p), st.rng_seed)
+    try:
+        rf = 1 << p.pop_bit_shift
+        f0 = np.array([system.view.f0[w] for w in range(W)], dtype=np.uint64)
+        ip = st.init_pop if init_pop is None else init_pop
+        pop0 = np.full(S, int(round(ip)) * rf, dtype=np.int64)
+        dat0 = np.zeros(1)
+        args = (f0.ctypes.data_as(C.POINTER(C.c_uint64)), pop0.ctypes.data_as(C.POINTER(C.c_int64)),
+                dat0.ctypes.data_as(C.POINTER(C.c_double)), 1)
+        N.check(lib.fq_load(eng, *args), "fq_load")
+        orc.orc_load(h, *args)
+        stream = np.zeros(1 << 22)
+        lib.fqh_dsfmt_fill(st.rng_seed, stream.ctypes.data_as(C.POINTER(C.c_double)), len(stream))
+        N.check(lib.fq_push_rng(eng, stream.ctypes.data_as(C.POINTER(C.c_double)), len(stream)), "fq_push_rng")
+        sh = (C.c_double * 2)(shift, shift)
+        eold = (C.c_double * 2)(0.0, 0.0)
+        consumed = 0
+        o1 = None
+        for cyc in range(ncycles):
+            o1, o2 = N.FqCycleOut(), N.FqCycleOut()
+            N.check(lib.fq_cycle(eng, sh, eold, st.tau, C.byref(o1)), "fq_cycle")
+            orc.orc_cycle(h, sh, eold, st.tau, C.byref(o2))
+            consumed += o1.rng_consumed
+            assert o1.rng_consumed == o2.rng_consumed, cyc
+            assert o1.nspawn_events == o2.nspawn_events and o1.ndeath == o2.ndeath, cyc
+            assert o1.nstates == o2.nstates, cyc
+            for s in range(S):
+                assert o1.nparticles[s] == o2.nparticles[s], cyc
+                assert o1.D0_population[s] == o2.D0_population[s], cyc
+                assert o1.proj_energy[s] == o2.proj_energy[s], cyc
+            a, b = download(lib, eng, W, S), odownload(orc, h, W, S)
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y), cyc
+            if cyc % 50 == 0:
+                print(f"  {label} cycle {cyc}: nstates={o1.nstates} N={o1.nparticles[0]}", flush=True)
+        assert consumed == orc.orc_rng_consumed(h)
+        return o1
+    finally:
+        lib.fq_destroy(eng)
+        orc.orc_destroy(h)
