@@ -16,7 +16,7 @@ FQ_ERR = {
     -5: "FQ_ERR_CAPACITY", -6: "FQ_ERR_NO_DEVICE", -7: "FQ_ERR_BIASED",
 }
 FQ_SYS_HUB_K, FQ_SYS_READ_IN = 1, 2
-FQ_EXCIT_RENORM, FQ_EXCIT_NO_RENORM, FQ_EXCIT_HEAT_BATH = 0, 1, 2
+FQ_EXCIT_RENORM, FQ_EXCIT_NO_RENORM, FQ_EXCIT_HEAT_BATH, FQ_EXCIT_LEARNED = 0, 1, 2, 3
 FQ_RNG_PHILOX, FQ_RNG_REPLAY = 0, 1
 FQH_SYM_AUFBAU, FQH_SYM_TOT, FQH_SYM_INDEX = 0, 1, 2
 FQ_MAX_SPACES = 2
@@ -44,7 +44,7 @@ class FqParams(C.Structure):
         ("nspaces", i32), ("initiator", i32), ("initiator_pop", f64), ("quadrature_initiator", i32),
         ("real_amplitudes", i32), ("pop_bit_shift", i32), ("spawn_cutoff", i64), ("excit_gen", i32),
         ("rng_mode", i32), ("philox_seed", u64), ("main_capacity", i64), ("spawn_capacity", i64),
-        ("device", i32),
+        ("device", i32), ("proposal_mix", f64),
     ]
 
 
@@ -106,6 +106,13 @@ def _declare(lib):
         "fq_download": (C.c_int, [vp, P(u64), P(i64), P(f64), i64, P(i64)]),
         "fq_last_timing": (C.c_int, [vp, P(f64), P(f64), P(f64)]),
         "fq_last_profile": (C.c_int, [vp, P(FqProfileOut)]),
+        "fq_mlp_nparams": (C.c_int, [vp, P(i64)]),
+        "fq_mlp_set_params": (C.c_int, [vp, P(C.c_float), i64]),
+        "fq_mlp_get_params": (C.c_int, [vp, P(C.c_float), i64]),
+        "fq_mlp_forward": (C.c_int, [vp, P(u64), i64, P(C.c_float)]),
+        "fq_mlp_loss_grad": (C.c_int, [vp, P(u64), P(i32), P(f64), i64, P(f64), P(C.c_float)]),
+        "fq_mlp_train": (C.c_int, [vp, i64, i32, f64, u64, P(f64)]),
+        "fq_probe_excitations": (C.c_int, [vp, P(u64), i64, u64, P(i32), P(f64), P(f64)]),
         "fq_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():

@@ -13,7 +13,8 @@ import numpy as np
 from . import _native as N
 from .lua_input import LuaCall, parse
 
-_EXCIT = {"renorm": N.FQ_EXCIT_RENORM, "no_renorm": N.FQ_EXCIT_NO_RENORM, "heat_bath": N.FQ_EXCIT_HEAT_BATH}
+_EXCIT = {"renorm": N.FQ_EXCIT_RENORM, "no_renorm": N.FQ_EXCIT_NO_RENORM, "heat_bath": N.FQ_EXCIT_HEAT_BATH,
+          "learned": N.FQ_EXCIT_LEARNED}
 _RNG = {"philox": N.FQ_RNG_PHILOX, "replay": N.FQ_RNG_REPLAY}
 
 

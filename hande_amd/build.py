@@ -38,9 +38,10 @@ def build(verbose=True):
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
         run(["g++", *COMMON, "-Wall", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    obj = os.path.join(BUILD, "engine.hip.o")
-    run([HIPCC, *COMMON, *HIP_FLAGS, "-c", os.path.join(CSRC, "engine.hip"), "-o", obj])
-    objs.append(obj)
+    for src in ("engine.hip", "mlp.hip"):
+        obj = os.path.join(BUILD, src + ".o")
+        run([HIPCC, *COMMON, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     tmp = OUT + ".tmp"
     run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs])
     shutil.move(tmp, OUT)

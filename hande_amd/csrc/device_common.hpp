@@ -61,6 +61,12 @@ struct Sys {
   const double* hb_ijab_tot;
   const double* hb_ijab_u;
   const int* hb_ijab_k;
+  // Learned proposal (excit_gen FQ_EXCIT_LEARNED): row d of mlp_tab holds the
+  // hole weights (nbasis) then the particle weights (nbasis) of determinant d
+  // of the current main list; mix is the uniform-fallback fraction of doubles.
+  const float* mlp_tab;
+  int64_t mlp_ld;
+  double mix;
   int initiator, quadrature, real_amps, excit_gen;
   double initiator_pop;
   int64_t rf, spawn_cutoff;
@@ -132,7 +138,15 @@ struct Philox {
 };
 
 // Draw purposes: the third counter word of a Philox stream.
-enum : uint32_t { kDrawAttempt = 1, kDrawNattempts = 2, kDrawDeath = 3, kDrawRoundMain = 4, kDrawRoundSpawn = 5 };
+enum : uint32_t {
+  kDrawAttempt = 1,
+  kDrawNattempts = 2,
+  kDrawDeath = 3,
+  kDrawRoundMain = 4,
+  kDrawRoundSpawn = 5,
+  kDrawTrain = 6,
+  kDrawProbe = 7
+};
 
 // ---------------------------------------------------------------------------
 // Bit-string helpers
@@ -582,55 +596,78 @@ __device__ __forceinline__ int ssbf(const Sys& s, int ims, int sym, int k) {
 }
 __device__ __forceinline__ int nbss(const Sys& s, int ims, int sym) { return s.nbss[(ims - 1) * s.nsym_pg + sym]; }
 
-// gen_excit_mol + attempt_to_spawn (renormalised molecular generator).
+// gen_single_excit_mol / choose_ia_mol (src/excit_gen_mol.f90:384-448,521-608):
+// renormalised single excitation; sets h and pgen (pgen includes psingle).
 template <class Rng>
-__device__ inline int64_t spawn_mol(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
-                                    Excit& conn) {
+__device__ inline bool single_mol(const Sys& s, Rng& rng, const uint64_t* f, const SymUnocc& u, Excit& conn,
+                                  double& h, double& pgen) {
+  const int nel = s.nel;
+  conn.nexcit = 1;
+  bool allowed = false;
+  int ni = nel;
+  for (int w = 0; w < s.W; ++w) {
+    uint64_t x = f[w];
+    while (x) {
+      const int o = (w << 6) + __builtin_ctzll(x) + 1;
+      x &= x - 1;
+      if (u.get(ims_of(o), s.orb_sym[o - 1]) != 0)
+        allowed = true;
+      else
+        ni = ni - 1;
+    }
+  }
+  if (allowed) {
+    int i = 0, a = 0;
+    for (int tries = 0; tries < kMaxRejection; ++tries) {
+      i = select_orb(f, s.W, static_cast<int>(rng.next() * nel));
+      const int imsa = ims_of(i), isyma = s.orb_sym[i - 1];
+      if (u.get(imsa, isyma) != 0) {
+        for (int t2 = 0; t2 < kMaxRejection; ++t2) {
+          const int ind = static_cast<int>(nbss(s, imsa, isyma) * rng.next()) + 1;
+          a = ssbf(s, imsa, isyma, ind - 1);
+          if (!btest(f, a) || rng.failed) break;
+        }
+        break;
+      }
+      if (rng.failed) break;
+    }
+    conn.from[0] = i;
+    conn.to[0] = a;
+    pgen = s.psingle * (1.0 / static_cast<double>(ni * u.get(ims_of(a), s.orb_sym[a - 1])));
+    conn.perm = perm1(f, s.W, i, a);
+    h = sc1_excit(s, f, i, a, conn.perm);
+  } else {
+    h = 0.0;
+    pgen = 1.0;
+  }
+  return allowed;
+}
+
+// attempt_to_spawn (src/spawning.F90:711-766): one draw, sign opposite to the
+// parent's when h > 0.
+template <class Rng>
+__device__ inline int64_t attempt_to_spawn(const Sys& s, Rng& rng, double tau, double h, double pgen,
+                                           int64_t parent) {
+  double pspawn = tau * fabs(h) / pgen;
+  pspawn = pspawn * static_cast<double>(s.rf);
+  int64_t nspawn = stoch_round_spawn(s, rng, pspawn);
+  if (nspawn > 0) nspawn = (h > 0.0) ? -fsign64(nspawn, parent) : fsign64(nspawn, parent);
+  return nspawn;
+}
+
+// gen_excit_mol (renormalised molecular generator, src/excit_gen_mol.f90:16-101):
+// the excitation, h and pgen; returns whether an allowed excitation was made.
+template <class Rng>
+__device__ inline bool gen_mol(const Sys& s, Rng& rng, const uint64_t* f, Excit& conn, double& h, double& pgen) {
   const SymUnocc u = sym_unocc(s, f);
-  double pgen = 1.0, h = 0.0;
+  pgen = 1.0;
+  h = 0.0;
+  bool ok = false;
   conn.from[0] = conn.from[1] = conn.to[0] = conn.to[1] = 0;
   conn.perm = false;
   const int nel = s.nel;
   if (rng.next() < s.psingle) {
-    // gen_single_excit_mol / choose_ia_mol
-    conn.nexcit = 1;
-    bool allowed = false;
-    int ni = nel;
-    for (int w = 0; w < s.W; ++w) {
-      uint64_t x = f[w];
-      while (x) {
-        const int o = (w << 6) + __builtin_ctzll(x) + 1;
-        x &= x - 1;
-        if (u.get(ims_of(o), s.orb_sym[o - 1]) != 0)
-          allowed = true;
-        else
-          ni = ni - 1;
-      }
-    }
-    if (allowed) {
-      int i = 0, a = 0;
-      for (int tries = 0; tries < kMaxRejection; ++tries) {
-        i = select_orb(f, s.W, static_cast<int>(rng.next() * nel));
-        const int imsa = ims_of(i), isyma = s.orb_sym[i - 1];
-        if (u.get(imsa, isyma) != 0) {
-          for (int t2 = 0; t2 < kMaxRejection; ++t2) {
-            const int ind = static_cast<int>(nbss(s, imsa, isyma) * rng.next()) + 1;
-            a = ssbf(s, imsa, isyma, ind - 1);
-            if (!btest(f, a) || rng.failed) break;
-          }
-          break;
-        }
-        if (rng.failed) break;
-      }
-      conn.from[0] = i;
-      conn.to[0] = a;
-      pgen = s.psingle * (1.0 / static_cast<double>(ni * u.get(ims_of(a), s.orb_sym[a - 1])));
-      conn.perm = perm1(f, s.W, i, a);
-      h = sc1_excit(s, f, i, a, conn.perm);
-    } else {
-      h = 0.0;
-      pgen = 1.0;
-    }
+    ok = single_mol(s, rng, f, u, conn, h, pgen);
   } else {
     // choose_ij_mol
     const int ind = static_cast<int>(rng.next() * nel * (nel - 1) / 2) + 1;
@@ -719,17 +756,22 @@ __device__ inline int64_t spawn_mol(const Sys& s, Rng& rng, const uint64_t* f, i
       pgen = s.pdouble * pgen_ij * pd;
       conn.perm = perm2(f, s.W, conn);
       h = sc2_excit(s, conn.from[0], conn.from[1], conn.to[0], conn.to[1], conn.perm);
+      ok = true;
     } else {
       h = 0.0;
       pgen = 1.0;
     }
   }
-  // attempt_to_spawn (src/spawning.F90:711-766)
-  double pspawn = tau * fabs(h) / pgen;
-  pspawn = pspawn * static_cast<double>(s.rf);
-  int64_t nspawn = stoch_round_spawn(s, rng, pspawn);
-  if (nspawn > 0) nspawn = (h > 0.0) ? -fsign64(nspawn, parent) : fsign64(nspawn, parent);
-  return nspawn;
+  return ok;
+}
+
+// spawn_standard with the renormalised generator (src/spawning.F90:34-124).
+template <class Rng>
+__device__ inline int64_t spawn_mol(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
+                                    Excit& conn) {
+  double h = 0.0, pgen = 1.0;
+  gen_mol(s, rng, f, conn, h, pgen);
+  return attempt_to_spawn(s, rng, tau, h, pgen, parent);
 }
 
 
@@ -794,10 +836,10 @@ __device__ inline double hb_psingle(const Sys& s, const uint64_t* f, int i, int 
   return hm < t ? hm / (t + hm) : 0.5;
 }
 
-// gen_excit_mol_heat_bath + attempt_to_spawn.
+// gen_excit_mol_heat_bath: the excitation, its matrix element h and pgen.
 template <class Rng>
-__device__ inline int64_t spawn_heat_bath(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
-                                          Excit& conn) {
+__device__ inline bool gen_heat_bath(const Sys& s, Rng& rng, const uint64_t* f, Excit& conn, double& h,
+                                     double& pgen) {
   const int nel = s.nel;
   int occ[kHbMaxNel];
   {
@@ -859,7 +901,8 @@ __device__ inline int64_t spawn_heat_bath(const Sys& s, Rng& rng, const uint64_t
       allowed = false;
     }
   }
-  double pgen = 1.0, h = 0.0;
+  pgen = 1.0;
+  h = 0.0;
   if (allowed) {
     if (dbl) {
       const int b = alias_draw(s.nbasis, s.hb_ijab_u + hb4(s, i, j, a, 1), s.hb_ijab_k + hb4(s, i, j, a, 1),
@@ -910,25 +953,227 @@ __device__ inline int64_t spawn_heat_bath(const Sys& s, Rng& rng, const uint64_t
     pgen = 1.0;
     conn.nexcit = 0;
   }
-  // attempt_to_spawn (src/spawning.F90:711-766)
-  double pspawn = tau * fabs(h) / pgen;
-  pspawn = pspawn * static_cast<double>(s.rf);
-  int64_t nspawn = stoch_round_spawn(s, rng, pspawn);
-  if (nspawn > 0) nspawn = (h > 0.0) ? -fsign64(nspawn, parent) : fsign64(nspawn, parent);
-  return nspawn;
+  return allowed;
+}
+
+// spawn_standard with the heat-bath generator (src/spawning.F90:34-124).
+template <class Rng>
+__device__ inline int64_t spawn_heat_bath(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
+                                          Excit& conn) {
+  double h = 0.0, pgen = 1.0;
+  gen_heat_bath(s, rng, f, conn, h, pgen);
+  return attempt_to_spawn(s, rng, tau, h, pgen, parent);
+}
+
+
+// ---------------------------------------------------------------------------
+// Learned double-excitation proposal (no reference counterpart; an addition
+// beside the heat-bath generator).  Doubles come from a mixture: with
+// probability 1 - mix the network's per-determinant weights (hole pair by
+// weight among occupied orbitals, first particle by weight among unoccupied
+// ones, second particle by weight within the first one's spin/irrep partner
+// class), with probability mix a uniform choice (pair, particle, partner).
+// pgen is the exact mixture probability of the unordered excitation, so the
+// sampler is unbiased whatever the weights; singles use the renormalised
+// generator.
+// ---------------------------------------------------------------------------
+struct PartnerClass {
+  int ims, sym;
+  bool ok;
+};
+// Class the partner of particle x must come from for the hole pair (i, j).
+__device__ inline PartnerClass partner_class(const Sys& s, int i, int j, int x) {
+  const int msb = ms_of(i) + ms_of(j) - ms_of(x);
+  PartnerClass c;
+  c.ok = (msb == 1 || msb == -1);
+  c.ims = (msb + 3) / 2;
+  c.sym = s.orb_sym[i - 1] ^ s.orb_sym[j - 1] ^ s.orb_sym[x - 1];
+  return c;
+}
+// Unoccupied members of class c other than x: count and particle-weight sum.
+__device__ inline void partner_stats(const Sys& s, const uint64_t* f, const float* fw, const PartnerClass& c, int x,
+                                     int& count, double& z) {
+  count = 0;
+  z = 0.0;
+  if (!c.ok) return;
+  const int nm = nbss(s, c.ims, c.sym);
+  for (int k = 0; k < nm; ++k) {
+    const int o = ssbf(s, c.ims, c.sym, k);
+    if (o != x && !btest(f, o)) {
+      ++count;
+      z = z + static_cast<double>(fw[o - 1]);
+    }
+  }
+}
+// Member of class c (unoccupied, not x) by index (uniform) or by weight target.
+__device__ inline int partner_pick(const Sys& s, const uint64_t* f, const float* fw, const PartnerClass& c, int x,
+                                   int kth, double target, bool by_weight) {
+  const int nm = nbss(s, c.ims, c.sym);
+  int last = 0, seen = 0;
+  double cum = 0.0;
+  for (int k = 0; k < nm; ++k) {
+    const int o = ssbf(s, c.ims, c.sym, k);
+    if (o == x || btest(f, o)) continue;
+    last = o;
+    if (by_weight) {
+      cum = cum + static_cast<double>(fw[o - 1]);
+      if (target < cum) return o;
+    } else if (seen++ == kth) {
+      return o;
+    }
+  }
+  return last;
+}
+
+template <class Rng>
+__device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, const float* wrow, Excit& conn,
+                                   double& h, double& pgen) {
+  const SymUnocc u = sym_unocc(s, f);
+  pgen = 1.0;
+  h = 0.0;
+  bool allowed = false;
+  conn.nexcit = 0;
+  conn.from[0] = conn.from[1] = conn.to[0] = conn.to[1] = 0;
+  conn.perm = false;
+  const int nel = s.nel, nb = s.nbasis;
+  if (rng.next() < s.psingle) {
+    allowed = single_mol(s, rng, f, u, conn, h, pgen);
+  } else {
+    const float* ew = wrow;
+    const float* fw = wrow + nb;
+    uint64_t un[FQ_MAX_WORDS];
+    for (int w = 0; w < s.W; ++w) un[w] = ~f[w] & s.valid[w];
+    double zh = 0.0, zp = 0.0;
+    for (int w = 0; w < s.W; ++w) {
+      for (uint64_t x = f[w]; x; x &= x - 1) zh = zh + static_cast<double>(ew[(w << 6) + __builtin_ctzll(x)]);
+      for (uint64_t x = un[w]; x; x &= x - 1) zp = zp + static_cast<double>(fw[(w << 6) + __builtin_ctzll(x)]);
+    }
+    const int nvirt = nb - nel;
+    int i = 0, j = 0, a = 0, b = 0;
+    bool ok = true;
+    PartnerClass cb;
+    int nbc = 0;
+    double zb = 0.0;
+    if (rng.next() < s.mix) {
+      // uniform component: unordered hole pair (choose_ij_mol's index map), a, partner
+      const int ind = static_cast<int>(rng.next() * nel * (nel - 1) / 2) + 1;
+      const int j_ind = static_cast<int>(1.50 + sqrt(2 * ind - 1.750));
+      const int i_ind = ind - ((j_ind - 1) * (j_ind - 2)) / 2;
+      i = select_orb(f, s.W, i_ind - 1);
+      j = select_orb(f, s.W, j_ind - 1);
+      a = select_orb(un, s.W, static_cast<int>(rng.next() * nvirt));
+      cb = partner_class(s, i, j, a);
+      partner_stats(s, f, fw, cb, a, nbc, zb);
+      if (nbc == 0)
+        ok = false;
+      else
+        b = partner_pick(s, f, fw, cb, a, static_cast<int>(rng.next() * nbc), 0.0, false);
+    } else {
+      // network component: holes by weight (i, then j != i), a by weight, partner by weight
+      double x = rng.next() * zh, cum = 0.0;
+      for (int w = 0; w < s.W && !i; ++w)
+        for (uint64_t y = f[w]; y; y &= y - 1) {
+          const int o = (w << 6) + __builtin_ctzll(y) + 1;
+          cum = cum + static_cast<double>(ew[o - 1]);
+          if (x < cum) {
+            i = o;
+            break;
+          }
+        }
+      if (!i) i = select_orb(f, s.W, nel - 1);
+      x = rng.next() * (zh - static_cast<double>(ew[i - 1]));
+      cum = 0.0;
+      int lastj = 0;
+      for (int w = 0; w < s.W && !j; ++w)
+        for (uint64_t y = f[w]; y; y &= y - 1) {
+          const int o = (w << 6) + __builtin_ctzll(y) + 1;
+          if (o == i) continue;
+          lastj = o;
+          cum = cum + static_cast<double>(ew[o - 1]);
+          if (x < cum) {
+            j = o;
+            break;
+          }
+        }
+      if (!j) j = lastj;
+      x = rng.next() * zp;
+      cum = 0.0;
+      int lasta = 0;
+      for (int w = 0; w < s.W && !a; ++w)
+        for (uint64_t y = un[w]; y; y &= y - 1) {
+          const int o = (w << 6) + __builtin_ctzll(y) + 1;
+          lasta = o;
+          cum = cum + static_cast<double>(fw[o - 1]);
+          if (x < cum) {
+            a = o;
+            break;
+          }
+        }
+      if (!a) a = lasta;
+      cb = partner_class(s, i, j, a);
+      partner_stats(s, f, fw, cb, a, nbc, zb);
+      if (nbc == 0)
+        ok = false;
+      else
+        b = partner_pick(s, f, fw, cb, a, 0, rng.next() * zb, true);
+    }
+    if (ok) {
+      int nac = 0;
+      double za = 0.0;
+      partner_stats(s, f, fw, partner_class(s, i, j, b), b, nac, za);
+      const double ei = ew[i - 1], ej = ew[j - 1], fa = fw[a - 1], fb = fw[b - 1];
+      const double hp = ei / zh * ej / (zh - ei) + ej / zh * ei / (zh - ej);
+      const double pp = fa / zp * fb / zb + fb / zp * fa / za;
+      const double q_unif = 2.0 / static_cast<double>(nel * (nel - 1)) / nvirt * (1.0 / nbc + 1.0 / nac);
+      pgen = s.pdouble * ((1.0 - s.mix) * (hp * pp) + s.mix * q_unif);
+      conn.nexcit = 2;
+      conn.from[0] = i < j ? i : j;
+      conn.from[1] = i < j ? j : i;
+      conn.to[0] = a < b ? a : b;
+      conn.to[1] = a < b ? b : a;
+      conn.perm = perm2(f, s.W, conn);
+      h = sc2_excit(s, conn.from[0], conn.from[1], conn.to[0], conn.to[1], conn.perm);
+      allowed = true;
+    } else {
+      h = 0.0;
+      pgen = 1.0;
+    }
+  }
+  return allowed;
+}
+
+template <class Rng>
+__device__ inline int64_t spawn_learned(const Sys& s, Rng& rng, const uint64_t* f, const float* wrow, int64_t parent,
+                                        double tau, Excit& conn) {
+  double h = 0.0, pgen = 1.0;
+  gen_learned(s, rng, f, wrow, conn, h, pgen);
+  return attempt_to_spawn(s, rng, tau, h, pgen, parent);
+}
+
+// One generator call without the spawning step (the excitation probe): the
+// family is chosen at run time; returns whether an allowed excitation was made.
+template <class Rng>
+__device__ inline bool generate_any(const Sys& s, Rng& rng, const uint64_t* f, const float* wrow, Excit& conn,
+                                    double& h, double& pgen) {
+  if (s.excit_gen == FQ_EXCIT_HEAT_BATH) return gen_heat_bath(s, rng, f, conn, h, pgen);
+  if (s.excit_gen == FQ_EXCIT_LEARNED) return gen_learned(s, rng, f, wrow, conn, h, pgen);
+  return gen_mol(s, rng, f, conn, h, pgen);
 }
 
 // Generator families; the production spawning kernel is compiled once per
 // family so each carries only its own generator's registers.
-enum Gen : int { kGenHubK = 0, kGenMolRenorm = 1, kGenHeatBath = 2, kGenAny = 3 };
+enum Gen : int { kGenHubK = 0, kGenMolRenorm = 1, kGenHeatBath = 2, kGenLearned = 3, kGenAny = 4 };
 __host__ __device__ inline int gen_of(int system, int excit_gen) {
   if (system == FQ_SYS_HUB_K) return kGenHubK;
+  if (excit_gen == FQ_EXCIT_LEARNED) return kGenLearned;
   return excit_gen == FQ_EXCIT_HEAT_BATH ? kGenHeatBath : kGenMolRenorm;
 }
 
+// One spawning attempt from determinant f; wrow is the determinant's row of
+// the learned proposal's weights (read only by that generator).
 template <int G = kGenAny, class Rng>
-__device__ inline int64_t spawn_attempt(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
-                                        uint64_t* child) {
+__device__ inline int64_t spawn_attempt(const Sys& s, Rng& rng, const uint64_t* f, const float* wrow,
+                                        int64_t parent, double tau, uint64_t* child) {
   Excit conn;
   const int g = G == kGenAny ? gen_of(s.system, s.excit_gen) : G;
   int64_t n;
@@ -936,6 +1181,8 @@ __device__ inline int64_t spawn_attempt(const Sys& s, Rng& rng, const uint64_t* 
     n = spawn_hub_k(s, rng, f, parent, tau, conn);
   else if ((G == kGenAny || G == kGenHeatBath) && g == kGenHeatBath)
     n = spawn_heat_bath(s, rng, f, parent, tau, conn);
+  else if ((G == kGenAny || G == kGenLearned) && g == kGenLearned)
+    n = spawn_learned(s, rng, f, wrow, parent, tau, conn);
   else
     n = spawn_mol(s, rng, f, parent, tau, conn);
   if (n != 0) excited_det(f, s.W, conn, child);

@@ -39,6 +39,7 @@
 
 #include "device_common.hpp"
 #include "fqmc.h"
+#include "mlp.hpp"
 
 using fqd::Sys;
 
@@ -152,6 +153,9 @@ struct fq_engine {
   double *d_hb_i = nullptr, *d_hb_ij = nullptr, *d_hb_ija = nullptr, *d_hb_ija_tot = nullptr, *d_hb_ija_u = nullptr;
   double *d_hb_ijab = nullptr, *d_hb_ijab_tot = nullptr, *d_hb_ijab_u = nullptr;
   int *d_hb_ija_k = nullptr, *d_hb_ijab_k = nullptr, *d_hb_bias = nullptr;
+  // learned proposal (excit_gen FQ_EXCIT_LEARNED): network, per-determinant weights
+  fqm::Mlp mlp;
+  float* mlp_tab = nullptr;  // cap x 2*nbasis
   // main list (double-buffered)
   int64_t cap = 0, n = 0;
   int cur = 0;
@@ -432,7 +436,8 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
       fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
                       cyc ^ (fqd::kDrawAttempt << 24));
       uint64_t child[FQ_MAX_WORDS] = {0, 0};
-      const int64_t nspawn = fqd::spawn_attempt<G>(s, rng, f, parent, tau, child);
+      const float* wrow = G == fqd::kGenLearned ? s.mlp_tab + i * s.mlp_ld : nullptr;
+      const int64_t nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child);
       const int64_t u = t - t_begin;
       for (int w = 0; w < s.W; ++w) st_det[w * bcap + u] = child[w];
       st_pop[u] = nspawn;
@@ -499,7 +504,7 @@ __global__ void k_replay_walk(Sys s, int64_t n, int64_t cap, const uint64_t* __r
       attempts += nat;
       for (int a = 0; a < nat && !rng.failed; ++a) {
         uint64_t child[FQ_MAX_WORDS] = {0, 0};
-        const int64_t ns = fqd::spawn_attempt(s, rng, f, p[sp], tau, child);
+        const int64_t ns = fqd::spawn_attempt(s, rng, f, nullptr, p[sp], tau, child);
         if (ns != 0) {
           if (nspawn_total + 1 > scap) {
             overflow = true;
@@ -1352,6 +1357,46 @@ int build_heat_bath(fq_engine* e) {
   return FQ_OK;
 }
 
+constexpr int64_t kTrainBatchMax = 16384;  // heat-bath samples per training step
+
+int setup_learned(fq_engine* e) {
+  Sys& s = e->sys;
+  int rc = fqm::mlp_alloc(e->mlp, s.nbasis, s.W, kTrainBatchMax);
+  if (rc != FQ_OK) return rc;
+  rc = fqm::mlp_init(e->mlp, e->par.philox_seed, e->stream);
+  if (rc != FQ_OK) return rc;
+  if (dalloc(&e->mlp_tab, e->cap * e->mlp.d_out) != FQ_OK) return FQ_ERR_NOMEM;
+  s.mlp_tab = e->mlp_tab;
+  s.mlp_ld = e->mlp.d_out;
+  s.mix = e->par.proposal_mix > 0.0 ? e->par.proposal_mix : 0.1;
+  return FQ_OK;
+}
+
+int scan_cb(void* ctx, const int64_t* in, int64_t* out, int64_t n) {
+  return scan_exclusive(static_cast<fq_engine*>(ctx), in, out, n);
+}
+
+// One generator call per thread from determinant f (the excitation probe).
+__global__ void k_probe(Sys s, uint64_t f0, uint64_t f1, const float* __restrict__ wrow, int64_t n, uint64_t seed,
+                        int32_t* __restrict__ exc, double* __restrict__ pgen, double* __restrict__ hmat) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint64_t f[FQ_MAX_WORDS] = {f0, f1};
+  fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32), fqd::kDrawProbe << 24);
+  fqd::Excit conn;
+  double h = 0.0, pg = 1.0;
+  const bool ok = fqd::generate_any(s, rng, f, wrow, conn, h, pg);
+  exc[6 * t] = ok ? 1 : 0;
+  exc[6 * t + 1] = ok ? conn.nexcit : 0;
+  exc[6 * t + 2] = conn.from[0];
+  exc[6 * t + 3] = conn.from[1];
+  exc[6 * t + 4] = conn.to[0];
+  exc[6 * t + 5] = conn.to[1];
+  pgen[t] = pg;
+  hmat[t] = h;
+}
+
+
 int alloc_engine(fq_engine* e) {
   const int64_t cap = e->cap, scap = e->scap;
   const int W = e->W, S = e->S;
@@ -1426,8 +1471,12 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   if (sys->nwords < 1 || sys->nwords > FQ_MAX_WORDS || params->nspaces < 1 || params->nspaces > FQ_MAX_SPACES)
     return FQ_ERR_ARG;
   if (sys->system == FQ_SYS_READ_IN && sys->nsym_pg > 8) return FQ_ERR_ARG;
-  if (params->excit_gen < FQ_EXCIT_RENORM || params->excit_gen > FQ_EXCIT_HEAT_BATH) return FQ_ERR_ARG;
-  if (params->excit_gen == FQ_EXCIT_HEAT_BATH && (sys->system != FQ_SYS_READ_IN || sys->nel > fqd::kHbMaxNel))
+  if (params->excit_gen < FQ_EXCIT_RENORM || params->excit_gen > FQ_EXCIT_LEARNED) return FQ_ERR_ARG;
+  if ((params->excit_gen == FQ_EXCIT_HEAT_BATH || params->excit_gen == FQ_EXCIT_LEARNED) &&
+      (sys->system != FQ_SYS_READ_IN || sys->nel > fqd::kHbMaxNel))
+    return FQ_ERR_ARG;
+  if (params->excit_gen == FQ_EXCIT_LEARNED &&
+      (params->rng_mode != FQ_RNG_PHILOX || params->proposal_mix < 0.0 || params->proposal_mix > 1.0))
     return FQ_ERR_ARG;
   if (sys->system == FQ_SYS_READ_IN && params->excit_gen == FQ_EXCIT_NO_RENORM) return FQ_ERR_ARG;
   if (params->main_capacity < 1 || params->spawn_capacity < 1) return FQ_ERR_ARG;
@@ -1520,7 +1569,9 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   s.eri = e->d_eri;
   s.jmat = e->d_jmat;
   s.kmat = e->d_kmat;
-  if (rc == FQ_OK && params->excit_gen == FQ_EXCIT_HEAT_BATH) rc = build_heat_bath(e);
+  if (rc == FQ_OK && (params->excit_gen == FQ_EXCIT_HEAT_BATH || params->excit_gen == FQ_EXCIT_LEARNED))
+    rc = build_heat_bath(e);
+  if (rc == FQ_OK && params->excit_gen == FQ_EXCIT_LEARNED) rc = setup_learned(e);
   if (rc == FQ_OK) rc = alloc_engine(e);
   for (int k = 0; k < 4 && rc == FQ_OK; ++k)
     if (hipEventCreate(&e->ev[k]) != hipSuccess) rc = FQ_ERR_DEVICE;
@@ -1615,6 +1666,10 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, e->part_d, e->part_i,
                        n > 0 ? static_cast<int>(nb) : 0, e->d_stats);
     FQ_HIP(hipEventRecord(e->pev[1], st));
+    if (e->par.excit_gen == FQ_EXCIT_LEARNED) {
+      int mrc = fqm::mlp_infer(e->mlp, e->sys, e->det[c], cap, n, e->mlp_tab, st);
+      if (mrc != FQ_OK) return mrc;
+    }
     // Attempt offsets over the n*S (det, space) entries.
     const int64_t nent = e->S * n;
     int rc = scan_exclusive(e, e->nat, e->att_off, nent);
@@ -1785,6 +1840,162 @@ int fq_last_timing(fq_engine* e, double* spawn_ms, double* annihilate_ms, double
   return FQ_OK;
 }
 
+// ---- learned proposal ---------------------------------------------------------
+int fq_mlp_nparams(fq_engine* e, int64_t* n) {
+  if (!e || !n) return FQ_ERR_ARG;
+  if (e->par.excit_gen != FQ_EXCIT_LEARNED) return FQ_ERR_ARG;
+  *n = e->mlp.nparams;
+  return FQ_OK;
+}
+
+int fq_mlp_set_params(fq_engine* e, const float* p, int64_t n) {
+  if (!e || !p || e->par.excit_gen != FQ_EXCIT_LEARNED || n != e->mlp.nparams) return FQ_ERR_ARG;
+  FQ_HIP(hipSetDevice(e->device));
+  return fqm::mlp_set(e->mlp, p, e->stream);
+}
+
+int fq_mlp_get_params(fq_engine* e, float* p, int64_t n) {
+  if (!e || !p || e->par.excit_gen != FQ_EXCIT_LEARNED || n != e->mlp.nparams) return FQ_ERR_ARG;
+  FQ_HIP(hipSetDevice(e->device));
+  FQ_HIP(hipMemcpyAsync(p, e->mlp.p, sizeof(float) * n, hipMemcpyDeviceToHost, e->stream));
+  FQ_HIP(hipStreamSynchronize(e->stream));
+  return FQ_OK;
+}
+
+int fq_mlp_forward(fq_engine* e, const uint64_t* dets, int64_t n, float* out) {
+  if (!e || !dets || !out || n < 1 || e->par.excit_gen != FQ_EXCIT_LEARNED) return FQ_ERR_ARG;
+  FQ_HIP(hipSetDevice(e->device));
+  const int W = e->W;
+  std::vector<uint64_t> soa(static_cast<size_t>(W) * n);
+  for (int64_t i = 0; i < n; ++i)
+    for (int w = 0; w < W; ++w) soa[w * n + i] = dets[i * W + w];
+  uint64_t* d_det = nullptr;
+  float* d_out = nullptr;
+  int rc = dalloc(&d_det, W * n);
+  rc |= dalloc(&d_out, n * e->mlp.d_out);
+  if (rc == FQ_OK) {
+    hipStream_t st = e->stream;
+    if (hipMemcpyAsync(d_det, soa.data(), sizeof(uint64_t) * W * n, hipMemcpyHostToDevice, st) != hipSuccess)
+      rc = FQ_ERR_DEVICE;
+    if (rc == FQ_OK) rc = fqm::mlp_infer(e->mlp, e->sys, d_det, n, n, d_out, st);
+    if (rc == FQ_OK &&
+        hipMemcpyAsync(out, d_out, sizeof(float) * n * e->mlp.d_out, hipMemcpyDeviceToHost, st) != hipSuccess)
+      rc = FQ_ERR_DEVICE;
+    if (hipStreamSynchronize(st) != hipSuccess) rc = FQ_ERR_DEVICE;
+  } else {
+    rc = FQ_ERR_NOMEM;
+  }
+  dfree(d_det);
+  dfree(d_out);
+  return rc;
+}
+
+int fq_mlp_loss_grad(fq_engine* e, const uint64_t* dets, const int32_t* exc, const double* w, int64_t n,
+                     double* loss, float* grad) {
+  if (!e || !dets || !exc || !w || !loss || n < 1 || e->par.excit_gen != FQ_EXCIT_LEARNED) return FQ_ERR_ARG;
+  fqm::Mlp& m = e->mlp;
+  if (n > m.bcap) return FQ_ERR_CAPACITY;
+  const int W = e->W;
+  // every weighted sample: a double i<j (occupied) -> a<b (unoccupied), orbitals in 1..nbasis
+  for (int64_t k = 0; k < n; ++k) {
+    if (!(w[k] > 0.0)) continue;
+    const int32_t* x = exc + 4 * k;
+    auto occ = [&](int o) { return (dets[k * W + (o - 1) / 64] >> ((o - 1) % 64)) & 1; };
+    for (int q = 0; q < 4; ++q)
+      if (x[q] < 1 || x[q] > e->sys.nbasis) return FQ_ERR_ARG;
+    if (!(x[0] < x[1] && x[2] < x[3]) || !occ(x[0]) || !occ(x[1]) || occ(x[2]) || occ(x[3])) return FQ_ERR_ARG;
+  }
+  FQ_HIP(hipSetDevice(e->device));
+  hipStream_t st = e->stream;
+  std::vector<uint64_t> soa(static_cast<size_t>(W) * n);
+  for (int64_t i = 0; i < n; ++i)
+    for (int q = 0; q < W; ++q) soa[q * n + i] = dets[i * W + q];
+  for (int q = 0; q < W; ++q)
+    FQ_HIP(hipMemcpyAsync(m.sdet + q * m.bcap, soa.data() + q * n, sizeof(uint64_t) * n, hipMemcpyHostToDevice, st));
+  FQ_HIP(hipMemcpyAsync(m.sexc, exc, sizeof(int32_t) * 4 * n, hipMemcpyHostToDevice, st));
+  FQ_HIP(hipMemcpyAsync(m.sw, w, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  int rc = fqm::mlp_train_step(m, e->sys, n, 0.0, false, st);
+  if (rc != FQ_OK) return rc;
+  double sums[2] = {0.0, 0.0};
+  FQ_HIP(hipMemcpyAsync(sums, m.dsum, sizeof(sums), hipMemcpyDeviceToHost, st));
+  if (grad) FQ_HIP(hipMemcpyAsync(grad, m.g, sizeof(float) * m.nparams, hipMemcpyDeviceToHost, st));
+  FQ_HIP(hipStreamSynchronize(st));
+  *loss = sums[1];
+  return FQ_OK;
+}
+
+int fq_mlp_train(fq_engine* e, int64_t batch, int32_t steps, double lr, uint64_t seed, double* losses) {
+  if (!e || batch < 1 || steps < 0 || !(lr > 0.0) || e->par.excit_gen != FQ_EXCIT_LEARNED) return FQ_ERR_ARG;
+  fqm::Mlp& m = e->mlp;
+  if (batch > m.bcap) return FQ_ERR_CAPACITY;
+  if (e->n < 1) return FQ_ERR_ARG;
+  FQ_HIP(hipSetDevice(e->device));
+  hipStream_t st = e->stream;
+  const int c = e->cur;
+  for (int32_t k = 0; k < steps; ++k) {
+    int rc = fqm::mlp_sample(m, e->sys, e->det[c], e->pop[c], e->cap, e->n, batch, seed,
+                             static_cast<uint32_t>(m.adam_t), st, scan_cb, e);
+    if (rc != FQ_OK) return rc;
+    rc = fqm::mlp_train_step(m, e->sys, batch, lr, true, st);
+    if (rc != FQ_OK) return rc;
+    if (losses) {
+      double sums[2] = {0.0, 0.0};
+      FQ_HIP(hipMemcpyAsync(sums, m.dsum, sizeof(sums), hipMemcpyDeviceToHost, st));
+      FQ_HIP(hipStreamSynchronize(st));
+      losses[k] = sums[1];
+    }
+  }
+  FQ_HIP(hipStreamSynchronize(st));
+  return FQ_OK;
+}
+
+int fq_probe_excitations(fq_engine* e, const uint64_t* det, int64_t n, uint64_t seed, int32_t* exc, double* pgen,
+                         double* hmatel) {
+  if (!e || !det || !exc || !pgen || !hmatel || n < 1) return FQ_ERR_ARG;
+  if (e->sys.system != FQ_SYS_READ_IN) return FQ_ERR_ARG;
+  const int W = e->W;
+  {  // the determinant must hold nel electrons in orbitals 1..nbasis
+    int nel = 0;
+    for (int w = 0; w < W; ++w) {
+      if (det[w] & ~e->sys.valid[w]) return FQ_ERR_ARG;
+      nel += __builtin_popcountll(det[w]);
+    }
+    if (nel != e->sys.nel) return FQ_ERR_ARG;
+  }
+  FQ_HIP(hipSetDevice(e->device));
+  hipStream_t st = e->stream;
+  int32_t* d_exc = nullptr;
+  double *d_pg = nullptr, *d_h = nullptr;
+  float* d_row = nullptr;
+  uint64_t* d_det = nullptr;
+  int rc = dalloc(&d_exc, 6 * n);
+  rc |= dalloc(&d_pg, n);
+  rc |= dalloc(&d_h, n);
+  rc |= dalloc(&d_det, W);
+  rc |= dalloc(&d_row, 2 * static_cast<int64_t>(e->sys.nbasis));
+  if (rc == FQ_OK && hipMemcpyAsync(d_det, det, sizeof(uint64_t) * W, hipMemcpyHostToDevice, st) != hipSuccess)
+    rc = FQ_ERR_DEVICE;
+  if (rc == FQ_OK && e->par.excit_gen == FQ_EXCIT_LEARNED) rc = fqm::mlp_infer(e->mlp, e->sys, d_det, 1, 1, d_row, st);
+  if (rc == FQ_OK) {
+    hipLaunchKernelGGL(k_probe, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, det[0], W > 1 ? det[1] : 0,
+                       d_row, n, seed, d_exc, d_pg, d_h);
+    if (hipGetLastError() != hipSuccess) rc = FQ_ERR_DEVICE;
+  }
+  if (rc == FQ_OK) {
+    if (hipMemcpyAsync(exc, d_exc, sizeof(int32_t) * 6 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(pgen, d_pg, sizeof(double) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(hmatel, d_h, sizeof(double) * n, hipMemcpyDeviceToHost, st) != hipSuccess)
+      rc = FQ_ERR_DEVICE;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) rc = FQ_ERR_DEVICE;
+  dfree(d_exc);
+  dfree(d_pg);
+  dfree(d_h);
+  dfree(d_det);
+  dfree(d_row);
+  return rc;
+}
+
 void fq_destroy(fq_engine* e) {
   if (!e) return;
   (void)hipSetDevice(e->device);
@@ -1855,6 +2066,8 @@ void fq_destroy(fq_engine* e) {
   dfree(e->d_hb_ijab_u);
   dfree(e->d_hb_ijab_k);
   dfree(e->d_hb_bias);
+  fqm::mlp_free(e->mlp);
+  dfree(e->mlp_tab);
   for (int k = 0; k < 4; ++k)
     if (e->ev[k]) (void)hipEventDestroy(e->ev[k]);
   for (int k = 0; k < 9; ++k)

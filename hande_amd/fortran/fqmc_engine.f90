@@ -12,7 +12,8 @@ module fqmc_engine
 
     integer(c_int32_t), parameter :: FQ_OK = 0
     integer(c_int32_t), parameter :: FQ_SYS_HUB_K = 1, FQ_SYS_READ_IN = 2
-    integer(c_int32_t), parameter :: FQ_EXCIT_RENORM = 0, FQ_EXCIT_NO_RENORM = 1
+    integer(c_int32_t), parameter :: FQ_EXCIT_RENORM = 0, FQ_EXCIT_NO_RENORM = 1, FQ_EXCIT_HEAT_BATH = 2
+    integer(c_int32_t), parameter :: FQ_EXCIT_LEARNED = 3
     integer(c_int32_t), parameter :: FQ_RNG_PHILOX = 0, FQ_RNG_REPLAY = 1
     integer, parameter :: FQ_MAX_SPACES = 2
 
@@ -45,6 +46,7 @@ module fqmc_engine
         integer(c_int64_t) :: philox_seed
         integer(c_int64_t) :: main_capacity, spawn_capacity
         integer(c_int32_t) :: device
+        real(c_double) :: proposal_mix
     end type fq_params
 
     type, bind(c) :: fq_cycle_out

@@ -43,7 +43,9 @@ enum fq_system_kind { FQ_SYS_HUB_K = 1, FQ_SYS_READ_IN = 2 };
 enum fq_excit_gen {
   FQ_EXCIT_RENORM = 0,     /* excit_gen 'renorm' (Hubbard-k split generator / gen_excit_mol) */
   FQ_EXCIT_NO_RENORM = 1,  /* excit_gen 'no_renorm' (Hubbard-k only) */
-  FQ_EXCIT_HEAT_BATH = 2   /* excit_gen 'heat_bath': gen_excit_mol_heat_bath, src/excit_gen_heat_bath_mol.F90:252-548 (read_in only) */
+  FQ_EXCIT_HEAT_BATH = 2,  /* excit_gen 'heat_bath': gen_excit_mol_heat_bath, src/excit_gen_heat_bath_mol.F90:252-548 (read_in only) */
+  FQ_EXCIT_LEARNED = 3     /* learned double proposal mixed with a uniform fallback; singles as 'renorm'
+                              (read_in only, production RNG only; no reference counterpart) */
 };
 enum fq_rng_mode {
   FQ_RNG_PHILOX = 0, /* production: counter-based Philox4x32-10 per attempt */
@@ -101,6 +103,8 @@ typedef struct fq_params {
   int64_t main_capacity;        /* main walker list slots (state_size) */
   int64_t spawn_capacity;       /* spawned list slots (spawned_state_size) */
   int32_t device;               /* HIP device ordinal */
+  double proposal_mix;          /* FQ_EXCIT_LEARNED: fraction of doubles drawn from the uniform
+                                   fallback, in (0, 1]; 0 selects 0.1 */
 } fq_params;
 
 /* Per-cycle results, the quantities the reference accumulates per cycle. */
@@ -178,6 +182,39 @@ typedef struct fq_profile_out {
   double spawn_kernel_ms;    /* summed device time of those launches (event pair around each) */
 } fq_profile_out;
 int fq_last_profile(fq_engine *e, fq_profile_out *out);
+
+/* ---- Learned double-excitation proposal (excit_gen FQ_EXCIT_LEARNED) ---------
+ * No counterpart in the reference: an addition beside its heat-bath generator
+ * (src/excit_gen_heat_bath_mol.F90), which produces the training samples.
+ * The network maps a determinant's occupations (and occupied orbital
+ * energies) through two 256-wide GELU layers to hole and particle weights;
+ * parameters are float32, laid out W1[2n][256], b1[256], W2[256][256],
+ * b2[256], W3[256][2n], b3[2n] (n = nbasis).  fq_create initialises them from
+ * philox_seed; inference runs in bf16 with fp32 accumulation every cycle. */
+int fq_mlp_nparams(fq_engine *e, int64_t *n);
+int fq_mlp_set_params(fq_engine *e, const float *params, int64_t n);  /* also resets the optimiser */
+int fq_mlp_get_params(fq_engine *e, float *params, int64_t n);
+/* Proposal weights exp(clamp(logit, -60, 60)) of n determinants (n x W words,
+ * determinant-major as in fq_load) into out (n x 2*nbasis: holes, then particles). */
+int fq_mlp_forward(fq_engine *e, const uint64_t *dets, int64_t n, float *out);
+/* Training loss sum_s w_s (-log q(ex_s | D_s)) / sum_s w_s of n samples
+ * (exc: n x 4 = i, j, a, b with i<j, a<b) and its gradient (nparams floats;
+ * grad may be NULL); fp32 forward and backward. */
+int fq_mlp_loss_grad(fq_engine *e, const uint64_t *dets, const int32_t *exc, const double *w, int64_t n,
+                     double *loss, float *grad);
+/* steps Adam steps (learning rate lr), each on batch heat-bath samples drawn on
+ * the device from the current walker list (determinant chosen with weight
+ * 1 + floor|N_i|); losses (may be NULL) receives each step's loss. */
+int fq_mlp_train(fq_engine *e, int64_t batch, int32_t steps, double lr, uint64_t seed, double *losses);
+
+/* ---- Excitation probe (molecular systems) --------------------------------------
+ * n independent calls of the engine's excitation generator from one
+ * determinant (W words), Philox stream keyed by seed: exc (n x 6: allowed,
+ * nexcit, from1, from2, to1, to2), pgen and the matrix element of each.  The
+ * generator is the one the spawning kernel uses (gen_excit_mol,
+ * gen_excit_mol_heat_bath or the learned proposal), without attempt_to_spawn. */
+int fq_probe_excitations(fq_engine *e, const uint64_t *det, int64_t n, uint64_t seed, int32_t *exc, double *pgen,
+                         double *hmatel);
 
 void fq_destroy(fq_engine *e);
 
