@@ -9,9 +9,12 @@ population 1e7, tau 1e-3, renormalised excitation generator, FP64 integrals.
 A step is one Monte Carlo cycle (spawning + death + annihilation) of the whole
 walker population resident in HBM.  The walker distribution is prepared
 untimed: target/1000 walkers on the Hartree--Fock determinant are propagated
-at a larger time step (0.01) until the population passes the target and the
-shift (started from the projected energy) has held it for 150 cycles, then W
-warm-up cycles run at tau = 1e-3 before K timed cycles.
+at a larger time step (0.01); once the population passes the target the shift
+varies with a proportional-integral update that pulls the population to the
+target, for 250 cycles.  Then W warm-up cycles run at tau = 1e-3 with the
+reference's shift update (damping 0.05) before K timed cycles.  A prepared
+list can be saved (--save-state) and reloaded (--load-state), so a profiled
+run contains only cycles of the measured shape.
 
 Prints one JSON line (rank 0).  Extra fields:
   roofline      the spawning-attempt kernel (the largest device-time share):
@@ -96,15 +99,16 @@ def make_engine(lib, system, target, seed, main_cap, spawn_cap):
 
 
 class Propagator:
-    """Cycle driver with the reference's shift update (src/energy_evaluation.F90:659-711).
+    """Cycle driver with the reference's shift update (src/energy_evaluation.F90:659-711):
+    every `update_every` cycles S -= damping/(tau*every) * ln(N/N_old) once the
+    population has passed the target.  With pull > 0 (preparation only) an
+    integral term -pull*damping^2/(tau*every)*ln(N/target) also pulls the
+    population to the target."""
 
-    When the population first exceeds the target the shift starts from the
-    projected energy of the last update window (vary_shift_from = 'proje',
-    src/energy_evaluation.F90:638-648)."""
-
-    def __init__(self, lib, eng, target, damping=0.05, update_every=5):
+    def __init__(self, lib, eng, target, damping=0.05, update_every=5, pull=0.0):
         self.lib, self.eng = lib, eng
         self.target, self.damping, self.every = target, damping, update_every
+        self.pull = pull  # integral gain towards the target (preparation only)
         self.shift = (C.c_double * 2)(0.0, 0.0)
         self.eold = (C.c_double * 2)(0.0, 0.0)
         self.vary = False
@@ -122,12 +126,14 @@ class Propagator:
         n = out.nparticles[0]
         if self.count % self.every == 0:
             if self.vary and self.n_old:
-                self.shift[0] = self.shift[0] - math.log(n / self.n_old) * self.damping / (tau * self.every)
+                dt = tau * self.every
+                self.shift[0] = self.shift[0] - math.log(n / self.n_old) * self.damping / dt
+                if self.pull:
+                    self.shift[0] -= self.pull * self.damping ** 2 / dt * math.log(n / self.target)
             if abs(self.d0) > 0:
                 self.eold[0] = self.proj / self.d0
             if not self.vary and n > self.target:
                 self.vary = True
-                self.shift[0] = self.eold[0]
             self.n_old = n
             self.proj = self.d0 = 0.0
         self.last = out
@@ -151,6 +157,31 @@ def phase_bytes(prof, W=1, S=1):
         # main list read twice (merge search ~ touched lines ignored) and written once, runs read
         "merge": nd * (2 * bm + 3 * 8) + nout * bm + nr * (8 * W + 8 * S + 8 + 8 * 3),
     }
+
+
+def download_list(lib, eng, W):
+    n = C.c_int64()
+    lib.fq_nstates(eng, C.byref(n))
+    nd = n.value
+    st = np.zeros(nd * W, dtype=np.uint64)
+    po = np.zeros(nd, dtype=np.int64)
+    da = np.zeros(nd)
+    got = C.c_int64()
+    N.check(lib.fq_download(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)), po.ctypes.data_as(C.POINTER(C.c_int64)),
+                            da.ctypes.data_as(C.POINTER(C.c_double)), nd, C.byref(got)), "fq_download")
+    return st, po, da
+
+
+def traffic_summary(workload):
+    """HBM bytes per launch of the spawning kernel measured by the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes for this workload (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", "k_spawn_mol26_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t if t.get("workload") == workload else None
 
 
 def spawn_kernel_bytes(prof, W=1, S=1):
@@ -180,16 +211,9 @@ def cpu_baseline(lib, system, eng, p, shift, eold, tau, budget_s=10.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import helpers
     orc = helpers.oracle()
-    n = C.c_int64()
-    lib.fq_nstates(eng, C.byref(n))
-    nd = n.value
     W = system.view.nwords
-    st = np.zeros(nd * W, dtype=np.uint64)
-    po = np.zeros(nd, dtype=np.int64)
-    da = np.zeros(nd)
-    got = C.c_int64()
-    N.check(lib.fq_download(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)), po.ctypes.data_as(C.POINTER(C.c_int64)),
-                            da.ctypes.data_as(C.POINTER(C.c_double)), nd, C.byref(got)), "fq_download")
+    st, po, da = download_list(lib, eng, W)
+    nd = len(da)
     walkers = np.abs(po).sum() / 2**31
 
     def run(lo, hi, seed, stop_at):
@@ -252,9 +276,11 @@ def main():
     ap.add_argument("--config", default="mol26", choices=sorted(synthetic.MOLECULAR_CONFIGS))
     ap.add_argument("--target", type=float, default=1e7)
     ap.add_argument("--equil-cycles", type=int, default=1200, help="max untimed preparation cycles")
-    ap.add_argument("--hold-cycles", type=int, default=150, help="preparation cycles after the shift starts varying")
+    ap.add_argument("--hold-cycles", type=int, default=250, help="preparation cycles after the shift starts varying")
     ap.add_argument("--equil-tau", type=float, default=0.01)
-    ap.add_argument("--prep-damping", type=float, default=0.1, help="shift damping while preparing")
+    ap.add_argument("--prep-damping", type=float, default=0.2, help="shift damping while preparing")
+    ap.add_argument("--save-state", default=None, help="write the prepared walker list and shift to this .npz")
+    ap.add_argument("--load-state", default=None, help="start from a list written by --save-state (no preparation)")
     ap.add_argument("--start-frac", type=float, default=1e-3, help="initial HF population / target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -279,26 +305,41 @@ def main():
     eng, p = make_engine(lib, system, args.target, 7, cap, cap)
     rf = 1 << 31
     f0 = system.f0()
-    pop0 = np.array([max(1, int(args.target * args.start_frac)) * rf], dtype=np.int64)
-    dat0 = np.zeros(1)
-    N.check(lib.fq_load(eng, f0.ctypes.data_as(C.POINTER(C.c_uint64)), pop0.ctypes.data_as(C.POINTER(C.c_int64)),
-                        dat0.ctypes.data_as(C.POINTER(C.c_double)), 1), "fq_load")
     region = ProfilerRegion()
     region.pause()
-    prop = Propagator(lib, eng, args.target, damping=args.prep_damping)
+    prop = Propagator(lib, eng, args.target, damping=args.prep_damping, pull=1.0)
     tau = 1e-3
-    t0 = time.perf_counter()
-    vary_at = None
-    for i in range(args.equil_cycles):
-        out = prop.cycle(args.equil_tau)
-        if prop.vary and vary_at is None:
-            vary_at = i
-        if i % 50 == 0:
-            log(f"[bench] prepare {i}: N={out.nparticles[0]:.4g} dets={out.nstates} shift={prop.shift[0]:.4f} "
-                f"({time.perf_counter() - t0:.1f}s)")
-        if vary_at is not None and i - vary_at >= args.hold_cycles:
-            break
+    if args.load_state:
+        z = np.load(args.load_state)
+        st, po, da = (np.ascontiguousarray(z[k]) for k in ("states", "pops", "kii"))
+        N.check(lib.fq_load(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)), po.ctypes.data_as(C.POINTER(C.c_int64)),
+                            da.ctypes.data_as(C.POINTER(C.c_double)), len(da)), "fq_load")
+        prop.shift[0], prop.eold[0] = float(z["shift"]), float(z["eold"])
+        prop.vary = True
+        log(f"[bench] loaded {len(da)} determinants from {args.load_state}")
+    else:
+        pop0 = np.array([max(1, int(args.target * args.start_frac)) * rf], dtype=np.int64)
+        dat0 = np.zeros(1)
+        N.check(lib.fq_load(eng, f0.ctypes.data_as(C.POINTER(C.c_uint64)), pop0.ctypes.data_as(C.POINTER(C.c_int64)),
+                            dat0.ctypes.data_as(C.POINTER(C.c_double)), 1), "fq_load")
+        t0 = time.perf_counter()
+        vary_at = None
+        for i in range(args.equil_cycles):
+            out = prop.cycle(args.equil_tau)
+            if prop.vary and vary_at is None:
+                vary_at = i
+            if i % 50 == 0:
+                log(f"[bench] prepare {i}: N={out.nparticles[0]:.4g} dets={out.nstates} shift={prop.shift[0]:.4f} "
+                    f"({time.perf_counter() - t0:.1f}s)")
+            if vary_at is not None and i - vary_at >= args.hold_cycles:
+                break
+        log(f"[bench] prepared: N={prop.last.nparticles[0]:.4g} dets={prop.last.nstates} shift={prop.shift[0]:.4f}")
+    if args.save_state:
+        st, po, da = download_list(lib, eng, system.view.nwords)
+        np.savez(args.save_state, states=st, pops=po, kii=da, shift=prop.shift[0], eold=prop.eold[0])
+        log(f"[bench] saved {len(da)} determinants to {args.save_state}")
     prop.damping = 0.05  # the workload's shift damping from here on
+    prop.pull = 0.0
     for i in range(args.warmup):
         prop.cycle(tau)
     profs = []
@@ -339,6 +380,9 @@ def main():
             cpu = cpu_baseline(lib, system, eng, p, prop.shift, prop.eold, tau)
         except Exception as ex:  # the baseline is reported, never the target
             log(f"[bench] cpu baseline failed: {ex}")
+    workload = (f"i-FCIQMC {args.config}: synthetic molecular integrals {norb} spatial orbitals, "
+                f"{nel} electrons, C1, real_64, initiator 3, tau 1e-3, renorm, target {args.target:g}")
+    traffic = traffic_summary(workload)
     result = {
         "metric": "spawn attempts/s",
         "value": value,
@@ -353,8 +397,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"i-FCIQMC {args.config}: synthetic molecular integrals {norb} spatial orbitals, "
-                        f"{nel} electrons, C1, real_64, initiator 3, tau 1e-3, renorm, target {args.target:g}",
+            "workload": workload,
             "walkers": last.nparticles[0],
             "determinants": int(last.nstates),
             "attempts_per_step": attempts / args.steps,
@@ -362,7 +405,8 @@ def main():
         },
         "annihilated_entries_per_s": annih_rate,
         "roofline": {"bound": "hbm", "kernel": "k_spawn", "dominant": dominant, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "launch_ms": k_ms, "bytes_per_launch": k_bytes, "launches": launches,
                      "phase_ms": avg_ms, "phase_bytes": avg_bytes},
         "cpu_baseline": cpu,

@@ -106,6 +106,9 @@ def _declare(lib):
         "fq_download": (C.c_int, [vp, P(u64), P(i64), P(f64), i64, P(i64)]),
         "fq_last_timing": (C.c_int, [vp, P(f64), P(f64), P(f64)]),
         "fq_last_profile": (C.c_int, [vp, P(FqProfileOut)]),
+        "fq_create_batch": (C.c_int, [P(FqSystem), i32, P(FqParams), P(vp)]),
+        "fq_load_batch": (C.c_int, [vp, P(i32), P(u64), P(i64), P(f64), i64]),
+        "fq_cycle_batch": (C.c_int, [vp, P(f64), P(f64), P(f64), P(FqCycleOut)]),
         "fq_mlp_nparams": (C.c_int, [vp, P(i64)]),
         "fq_mlp_set_params": (C.c_int, [vp, P(C.c_float), i64]),
         "fq_mlp_get_params": (C.c_int, [vp, P(C.c_float), i64]),

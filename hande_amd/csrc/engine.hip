@@ -46,6 +46,11 @@ using fqd::Sys;
 namespace {
 
 constexpr int kBlock = 256;
+// Batched instances share one list: the instance index sits in bits 48..63 of
+// word 0 above the orbital bits (systems of at most 48 spin orbitals).
+constexpr int kInstShift = 48;
+constexpr uint64_t kOrbMask = (uint64_t{1} << kInstShift) - 1;
+constexpr int kMaxInst = 1 << 16;
 constexpr int kRadixItems = 8;
 constexpr int kRadixTile = kBlock * kRadixItems;
 
@@ -153,6 +158,21 @@ struct fq_engine {
   double *d_hb_i = nullptr, *d_hb_ij = nullptr, *d_hb_ija = nullptr, *d_hb_ija_tot = nullptr, *d_hb_ija_u = nullptr;
   double *d_hb_ijab = nullptr, *d_hb_ijab_tot = nullptr, *d_hb_ijab_u = nullptr;
   int *d_hb_ija_k = nullptr, *d_hb_ijab_k = nullptr, *d_hb_bias = nullptr;
+  // batched independent instances (fq_create_batch): instance index in bits
+  // 48..63 of word 0; per-instance tables held by sub-engines
+  bool batched = false;
+  int ninst = 1;
+  int key_bits = 0;                  // key bits the radix sort orders (nbasis, or 48 + instance bits)
+  std::vector<fq_engine*> subs;
+  Sys* d_sys_arr = nullptr;          // [ninst]
+  double* d_ipar = nullptr;          // tau[ninst], shift[S][ninst], eproj_old[S][ninst]
+  int64_t* d_ibound = nullptr;       // [2][ninst + 1]: instance bounds of the old and the new list
+  int64_t* d_isb = nullptr;          // [ninst + 1]: instance bounds of the spawned list
+  int64_t* d_iatt = nullptr;         // [ninst]: first attempt of each instance
+  double* d_cterm = nullptr;         // [2S][cap]: per-determinant projected-energy and N_0 terms
+  int64_t* d_cdeath = nullptr;       // [cap]
+  double* d_iout_d = nullptr;        // [ninst][2S]
+  int64_t* d_iout_i = nullptr;       // [ninst][4 + S]
   // learned proposal (excit_gen FQ_EXCIT_LEARNED): network, per-determinant weights
   fqm::Mlp mlp;
   float* mlp_tab = nullptr;  // cap x 2*nbasis
@@ -405,13 +425,15 @@ constexpr int kSpawnTile = 4 * kBlock;  // attempts per block tile
 // searches its own parent.  The result of attempt t goes to slot t - t_begin
 // of the staging arrays (coalesced, no atomics, attempt order); k_compact packs
 // the successful ones into the spawned list.
-template <int G>
+template <int G, bool B>
 __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t cap, int64_t t_begin, int64_t t_end,
                                                   const int64_t* __restrict__ att_off,
                                                   const uint64_t* __restrict__ det, const int64_t* __restrict__ pop,
                                                   double tau, uint64_t seed, uint32_t cyc, int64_t bcap,
                                                   uint64_t* __restrict__ st_det, int64_t* __restrict__ st_pop,
-                                                  int64_t* __restrict__ st_ent, int64_t* __restrict__ st_flag) {
+                                                  int64_t* __restrict__ st_ent, int64_t* __restrict__ st_flag,
+                                                  const Sys* __restrict__ sys_arr, const int64_t* __restrict__ iatt,
+                                                  const double* __restrict__ itau) {
   __shared__ int64_t win[2];
   const int64_t ntiles = (t_end - t_begin + kSpawnTile - 1) / kSpawnTile;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -433,11 +455,23 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
       uint64_t f[FQ_MAX_WORDS];
       load_det(det, cap, s.W, i, f);
       const int64_t parent = pop[sp * cap + i];
-      fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
-                      cyc ^ (fqd::kDrawAttempt << 24));
       uint64_t child[FQ_MAX_WORDS] = {0, 0};
-      const float* wrow = G == fqd::kGenLearned ? s.mlp_tab + i * s.mlp_ld : nullptr;
-      const int64_t nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child);
+      int64_t nspawn;
+      if constexpr (B) {
+        // batched instances: the instance's tables, time step, key and attempt numbering
+        const int inst = static_cast<int>(f[0] >> kInstShift);
+        f[0] &= kOrbMask;
+        const int64_t trel = t - iatt[inst];
+        fqd::Philox rng(seed + static_cast<uint64_t>(inst), static_cast<uint32_t>(trel),
+                        static_cast<uint32_t>(trel >> 32), cyc ^ (fqd::kDrawAttempt << 24));
+        nspawn = fqd::spawn_attempt<G>(sys_arr[inst], rng, f, nullptr, parent, itau[inst], child);
+        if (nspawn != 0) child[0] |= static_cast<uint64_t>(inst) << kInstShift;
+      } else {
+        fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
+                        cyc ^ (fqd::kDrawAttempt << 24));
+        const float* wrow = G == fqd::kGenLearned ? s.mlp_tab + i * s.mlp_ld : nullptr;
+        nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child);
+      }
       const int64_t u = t - t_begin;
       for (int w = 0; w < s.W; ++w) st_det[w * bcap + u] = child[w];
       st_pop[u] = nspawn;
@@ -474,6 +508,132 @@ __global__ void k_compact(Sys s, int64_t nb, int64_t t_begin, const uint64_t* __
 
 __global__ void k_add_spawned(const int64_t* __restrict__ pos, int64_t nb, Stats* __restrict__ st) {
   st->nspawn += pos[nb];
+}
+
+// ---------------------------------------------------------------------------
+// Batched instances
+// ---------------------------------------------------------------------------
+// bound[k] = first position in the sorted list whose instance index is >= k.
+__global__ void k_inst_bounds(const uint64_t* __restrict__ w0, int64_t n, int ninst, int64_t* __restrict__ bound) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > ninst) return;
+  if (k == ninst) {
+    bound[k] = n;
+    return;
+  }
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (static_cast<int>(w0[mid] >> kInstShift) < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  bound[k] = lo;
+}
+
+// k_det_pass for a batched list: each determinant uses its instance's tables,
+// time step, shift and key; Philox counters count determinants from the
+// instance's first, so an instance follows the same chain as alone.  The
+// estimator terms go to per-determinant arrays for per-instance sums.
+__global__ __launch_bounds__(kBlock) void k_det_pass_batch(Sys s, const Sys* __restrict__ sys_arr,
+                                                           const int64_t* __restrict__ bound,
+                                                           const double* __restrict__ ipar, int ninst, int64_t n,
+                                                           int64_t cap, const uint64_t* __restrict__ det,
+                                                           const int64_t* __restrict__ pop,
+                                                           const double* __restrict__ kii,
+                                                           int64_t* __restrict__ pop_new, int64_t* __restrict__ nat,
+                                                           int64_t* __restrict__ flags, uint64_t seed, uint32_t cyc,
+                                                           double* __restrict__ cterm,
+                                                           int64_t* __restrict__ cdeath) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint64_t f[FQ_MAX_WORDS];
+  load_det(det, cap, s.W, i, f);
+  const int inst = static_cast<int>(f[0] >> kInstShift);
+  f[0] &= kOrbMask;
+  const Sys& si = sys_arr[inst];
+  const int64_t irel = i - bound[inst];
+  const double tau = ipar[inst];
+  double rp[FQ_MAX_SPACES] = {0.0, 0.0};
+  int64_t p[FQ_MAX_SPACES] = {0, 0};
+  for (int sp = 0; sp < s.S; ++sp) {
+    p[sp] = pop[sp * cap + i];
+    rp[sp] = static_cast<double>(p[sp]) / static_cast<double>(s.rf);
+  }
+  flags[i] = fqd::parent_flag(s, rp);
+  const double K = kii[i];
+  int64_t ndeath = 0;
+  for (int sp = 0; sp < s.S; ++sp) {
+    double proj = 0.0, d0 = 0.0;
+    fqd::proj_energy_term(si, f, rp[sp], proj, d0);
+    cterm[(2 * sp) * cap + i] = proj;
+    cterm[(2 * sp + 1) * cap + i] = d0;
+    const uint32_t hi = static_cast<uint32_t>(irel >> 32) ^ (static_cast<uint32_t>(sp) << 28);
+    const uint64_t key = seed + static_cast<uint64_t>(inst);
+    fqd::Philox rng(key, static_cast<uint32_t>(irel), hi, cyc ^ (fqd::kDrawNattempts << 24));
+    nat[i * s.S + sp] = fqd::decide_nattempts(rng, rp[sp]);
+    fqd::Philox rd(key, static_cast<uint32_t>(irel), hi, cyc ^ (fqd::kDrawDeath << 24));
+    int64_t q = p[sp];
+    const double sh = ipar[ninst + sp * ninst + inst], eo = ipar[(1 + s.S) * ninst + sp * ninst + inst];
+    ndeath += fqd::death(rd, tau, K, sh, eo, q);
+    pop_new[sp * cap + i] = q;
+  }
+  cdeath[i] = ndeath;
+}
+
+// First attempt index of each instance: attempts are numbered over the
+// (determinant, space) entries in list order.
+__global__ void k_inst_att(int ninst, int S, const int64_t* __restrict__ bound, const int64_t* __restrict__ att_off,
+                           int64_t* __restrict__ iatt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ninst) iatt[k] = att_off[S * bound[k]];
+}
+
+// Per-instance cycle results, one block per instance, fixed-order sums:
+// projected-energy terms, N_0 and deaths over the old list, attempts,
+// spawned entries, occupied determinants and sum |N| over the new list.
+__global__ __launch_bounds__(kBlock) void k_inst_stats(int ninst, int S, int64_t cap, const int64_t* __restrict__ ob,
+                                                       const int64_t* __restrict__ nbd, const int64_t* __restrict__ sb,
+                                                       const double* __restrict__ cterm,
+                                                       const int64_t* __restrict__ cdeath,
+                                                       const int64_t* __restrict__ att_off,
+                                                       const int64_t* __restrict__ pop,
+                                                       double* __restrict__ out_d, int64_t* __restrict__ out_i) {
+  __shared__ double shd[2 * FQ_MAX_SPACES][kBlock];
+  __shared__ int64_t shi[1 + FQ_MAX_SPACES][kBlock];
+  const int k = blockIdx.x;
+  const int64_t a = ob[k], b = ob[k + 1], na = nbd[k], nbb = nbd[k + 1];
+  double accd[2 * FQ_MAX_SPACES] = {0.0, 0.0, 0.0, 0.0};
+  int64_t acci[1 + FQ_MAX_SPACES] = {0, 0, 0};
+  for (int64_t i = a + threadIdx.x; i < b; i += kBlock) {
+    for (int q = 0; q < 2 * S; ++q) accd[q] = accd[q] + cterm[q * cap + i];
+    acci[0] += cdeath[i];
+  }
+  for (int64_t i = na + threadIdx.x; i < nbb; i += kBlock)
+    for (int sp = 0; sp < S; ++sp) {
+      const int64_t p = pop[sp * cap + i];
+      acci[1 + sp] += p < 0 ? -p : p;
+    }
+  for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][threadIdx.x] = accd[q];
+  for (int q = 0; q < 1 + FQ_MAX_SPACES; ++q) shi[q][threadIdx.x] = acci[q];
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][threadIdx.x] = shd[q][threadIdx.x] + shd[q][threadIdx.x + h];
+      for (int q = 0; q < 1 + FQ_MAX_SPACES; ++q) shi[q][threadIdx.x] += shi[q][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 2 * S; ++q) out_d[k * 2 * S + q] = shd[q][0];
+    int64_t* o = out_i + k * (4 + S);
+    o[0] = shi[0][0];                          // deaths
+    o[1] = att_off[S * b] - att_off[S * a];    // attempts
+    o[2] = sb[k + 1] - sb[k];                  // spawned entries
+    o[3] = nbb - na;                           // occupied determinants
+    for (int sp = 0; sp < S; ++sp) o[4 + sp] = shi[1 + sp][0];
+  }
 }
 
 // Replay: the whole spawning/death loop on one thread, in reference order.
@@ -967,7 +1127,7 @@ __global__ void k_scatter_new(Sys s, int64_t nruns, int64_t cap, int64_t scap, c
                               const int64_t* __restrict__ nsurv, const int64_t* __restrict__ nrank,
                               const int64_t* __restrict__ mrank, const Stats* __restrict__ st,
                               uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
-                              double* __restrict__ kii_out) {
+                              double* __restrict__ kii_out, const Sys* __restrict__ sys_arr) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= nruns || !nsurv[r] || st->main_overflow) return;
   const int64_t o = nrank[r] + mrank[rpos[r]];
@@ -977,7 +1137,13 @@ __global__ void k_scatter_new(Sys s, int64_t nruns, int64_t cap, int64_t scap, c
     det_out[w * cap + o] = f[w];
   }
   for (int sp = 0; sp < s.S; ++sp) pop_out[sp * cap + o] = rpop[sp * scap + r];
-  kii_out[o] = fqd::diagonal(s, f) - s.H00;
+  if (sys_arr) {  // batched: the instance's diagonal
+    const Sys& si = sys_arr[f[0] >> kInstShift];
+    f[0] &= kOrbMask;
+    kii_out[o] = fqd::diagonal(si, f) - si.H00;
+  } else {
+    kii_out[o] = fqd::diagonal(s, f) - s.H00;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_abs_pop(int64_t n, int64_t cap, int S, const int64_t* __restrict__ pop,
@@ -1073,7 +1239,7 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
       }
       int src = 0;
       for (int w = 0; w < W; ++w) {
-        const int bits = std::min(64, e->sys.nbasis - 64 * w);
+        const int bits = std::min(64, e->key_bits - 64 * w);
         for (int shift = 0; shift < bits; shift += 8) {
           hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src] + w * scap, ns, shift,
                              e->hist);
@@ -1167,7 +1333,8 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
                      e->kii[c], e->msurv, e->mrank, e->ins_scan, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
   if (nruns > 0)
     hipLaunchKernelGGL(k_scatter_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, cap, scap, e->rdet,
-                       e->rpop, e->rpos, e->nsurv, e->nrank, e->mrank, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
+                       e->rpop, e->rpos, e->nsurv, e->nrank, e->mrank, e->d_stats, e->det[o], e->pop[o], e->kii[o],
+                       e->batched ? e->d_sys_arr : nullptr);
   FQ_HIP(hipEventRecord(e->pev[8], st));
   rc = sync_stats(e);
   if (rc != FQ_OK) return rc;
@@ -1496,6 +1663,7 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   e->par = *params;
   e->W = sys->nwords;
   e->S = params->nspaces;
+  e->key_bits = sys->nbasis;
   e->cap = params->main_capacity;
   e->scap = params->spawn_capacity;
   Sys& s = e->sys;
@@ -1632,7 +1800,7 @@ int fq_rng_available(fq_engine* e, int64_t* n) {
 }
 
 int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
-  if (!e || !shift || !eproj_old || !out) return FQ_ERR_ARG;
+  if (!e || !shift || !eproj_old || !out || e->batched) return FQ_ERR_ARG;
   FQ_HIP(hipSetDevice(e->device));
   const bool replay = e->par.rng_mode == FQ_RNG_REPLAY;
   const int64_t n = e->n, cap = e->cap;
@@ -1686,12 +1854,13 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
       const int li = e->spawn_launches++;
       if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li], st));
       const int gen = fqd::gen_of(e->sys.system, e->sys.excit_gen);
-      auto kern = gen == fqd::kGenHubK ? k_spawn<fqd::kGenHubK>
-                  : gen == fqd::kGenHeatBath ? k_spawn<fqd::kGenHeatBath>
-                                             : k_spawn<fqd::kGenMolRenorm>;
+      auto kern = gen == fqd::kGenHubK       ? k_spawn<fqd::kGenHubK, false>
+                  : gen == fqd::kGenHeatBath ? k_spawn<fqd::kGenHeatBath, false>
+                  : gen == fqd::kGenLearned  ? k_spawn<fqd::kGenLearned, false>
+                                             : k_spawn<fqd::kGenMolRenorm, false>;
       hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
                          e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
-                         e->st_flag);
+                         e->st_flag, nullptr, nullptr, nullptr);
       if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
       rc = scan_exclusive(e, e->st_flag, e->st_pos, nbt);
       if (rc != FQ_OK) return rc;
@@ -1837,6 +2006,206 @@ int fq_last_timing(fq_engine* e, double* spawn_ms, double* annihilate_ms, double
   if (spawn_ms) *spawn_ms = e->t_spawn;
   if (annihilate_ms) *annihilate_ms = e->t_annih;
   if (total_ms) *total_ms = e->t_total;
+  return FQ_OK;
+}
+
+// ---- batched independent instances ---------------------------------------------
+int fq_create_batch(const fq_system* systems, int32_t ninst, const fq_params* params, fq_engine** out) {
+  if (!systems || !params || !out || ninst < 1 || ninst > kMaxInst) return FQ_ERR_ARG;
+  if (params->rng_mode != FQ_RNG_PHILOX || params->real_amplitudes) return FQ_ERR_ARG;
+  if (params->excit_gen == FQ_EXCIT_HEAT_BATH || params->excit_gen == FQ_EXCIT_LEARNED) return FQ_ERR_ARG;
+  for (int k = 0; k < ninst; ++k)
+    if (systems[k].nwords != 1 || systems[k].nbasis > kInstShift || systems[k].system != systems[0].system)
+      return FQ_ERR_ARG;
+  fq_engine* e = nullptr;
+  int rc = fq_create(&systems[0], params, &e);
+  if (rc != FQ_OK) return rc;
+  e->batched = true;
+  e->ninst = ninst;
+  int ib = 0;
+  while ((1 << ib) < ninst) ++ib;
+  e->key_bits = kInstShift + std::max(ib, 1);
+  fq_params small = *params;
+  small.main_capacity = small.spawn_capacity = 1;
+  std::vector<Sys> hs(static_cast<size_t>(ninst));
+  for (int k = 0; k < ninst && rc == FQ_OK; ++k) {
+    fq_engine* sub = nullptr;
+    rc = fq_create(&systems[k], &small, &sub);
+    if (rc == FQ_OK) {
+      e->subs.push_back(sub);
+      hs[k] = sub->sys;
+    }
+  }
+  const int64_t S = e->S;
+  if (rc == FQ_OK) rc = upload(&e->d_sys_arr, hs.data(), ninst, e->stream);
+  if (rc == FQ_OK) {
+    rc |= dalloc(&e->d_ipar, (1 + 2 * S) * ninst);
+    rc |= dalloc(&e->d_ibound, 2 * (ninst + 1));
+    rc |= dalloc(&e->d_isb, ninst + 1);
+    rc |= dalloc(&e->d_iatt, ninst);
+    rc |= dalloc(&e->d_cterm, 2 * S * e->cap);
+    rc |= dalloc(&e->d_cdeath, e->cap);
+    rc |= dalloc(&e->d_iout_d, 2 * S * ninst);
+    rc |= dalloc(&e->d_iout_i, (4 + S) * ninst);
+    if (rc != FQ_OK) rc = FQ_ERR_NOMEM;
+  }
+  if (rc == FQ_OK && hipStreamSynchronize(e->stream) != hipSuccess) rc = FQ_ERR_DEVICE;
+  if (rc != FQ_OK) {
+    fq_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return FQ_OK;
+}
+
+int fq_load_batch(fq_engine* e, const int32_t* inst, const uint64_t* states, const int64_t* pops, const double* dat,
+                  int64_t n) {
+  if (!e || !e->batched || n < 0 || (n > 0 && (!inst || !states || !pops || !dat))) return FQ_ERR_ARG;
+  const int S = e->S;
+  std::vector<int64_t> order(static_cast<size_t>(n));
+  std::vector<uint64_t> key(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; ++i) {
+    if (inst[i] < 0 || inst[i] >= e->ninst || (states[i] >> kInstShift) != 0) return FQ_ERR_ARG;
+    key[i] = states[i] | (static_cast<uint64_t>(inst[i]) << kInstShift);
+    order[i] = i;
+  }
+  std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+  for (int64_t i = 1; i < n; ++i)
+    if (key[order[i]] == key[order[i - 1]]) return FQ_ERR_ARG;
+  std::vector<uint64_t> st(static_cast<size_t>(n));
+  std::vector<int64_t> po(static_cast<size_t>(S) * n);
+  std::vector<double> da(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; ++i) {
+    st[i] = key[order[i]];
+    for (int q = 0; q < S; ++q) po[i * S + q] = pops[order[i] * S + q];
+    da[i] = dat[order[i]];
+  }
+  return fq_load(e, st.data(), po.data(), da.data(), n);  // the encoded, sorted list
+}
+
+int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, const double* tau,
+                   fq_cycle_out* out) {
+  if (!e || !e->batched || !shift || !eproj_old || !tau || !out) return FQ_ERR_ARG;
+  FQ_HIP(hipSetDevice(e->device));
+  const int ninst = e->ninst, S = e->S;
+  const int64_t n = e->n, cap = e->cap;
+  const int c = e->cur;
+  hipStream_t st = e->stream;
+  const uint32_t cyc = static_cast<uint32_t>(e->cycle_index++);
+  std::vector<double> par(static_cast<size_t>((1 + 2 * S) * ninst));
+  for (int k = 0; k < ninst; ++k) {
+    par[k] = tau[k];
+    for (int sp = 0; sp < S; ++sp) {
+      par[ninst + sp * ninst + k] = shift[k * S + sp];
+      par[(1 + S) * ninst + sp * ninst + k] = eproj_old[k * S + sp];
+    }
+  }
+  FQ_HIP(hipMemcpyAsync(e->d_ipar, par.data(), sizeof(double) * par.size(), hipMemcpyHostToDevice, st));
+  FQ_HIP(hipMemsetAsync(e->d_stats, 0, sizeof(Stats), st));
+  FQ_HIP(hipEventRecord(e->ev[0], st));
+  FQ_HIP(hipEventRecord(e->pev[0], st));
+  int64_t* ob = e->d_ibound;
+  int64_t* nbd = e->d_ibound + (ninst + 1);
+  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[c], n, ninst, ob);
+  if (n > 0)
+    hipLaunchKernelGGL(k_det_pass_batch, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, e->d_sys_arr, ob, e->d_ipar,
+                       ninst, n, cap, e->det[c], e->pop[c], e->kii[c], e->pop_new, e->nat, e->flags,
+                       e->par.philox_seed, cyc, e->d_cterm, e->d_cdeath);
+  FQ_HIP(hipEventRecord(e->pev[1], st));
+  const int64_t nent = S * n;
+  int rc = scan_exclusive(e, e->nat, e->att_off, nent);
+  if (rc != FQ_OK) return rc;
+  hipLaunchKernelGGL(k_inst_att, dim3(grid_for(ninst)), dim3(kBlock), 0, st, ninst, S, ob, e->att_off, e->d_iatt);
+  int64_t total = 0;
+  FQ_HIP(hipMemcpyAsync(&total, e->att_off + nent, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  FQ_HIP(hipStreamSynchronize(st));
+  FQ_HIP(hipEventRecord(e->pev[2], st));
+  e->spawn_launches = 0;
+  for (int64_t tb = 0; tb < total; tb += e->bcap) {
+    const int64_t te = std::min(total, tb + e->bcap);
+    const int64_t nbt = te - tb;
+    const unsigned g = static_cast<unsigned>(std::min<int64_t>(ceil_div(nbt, kSpawnTile), 1 << 20));
+    const int li = e->spawn_launches++;
+    if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li], st));
+    auto kern = e->sys.system == FQ_SYS_HUB_K ? k_spawn<fqd::kGenHubK, true> : k_spawn<fqd::kGenMolRenorm, true>;
+    hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
+                       e->pop[c], 0.0, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
+                       e->st_flag, e->d_sys_arr, e->d_iatt, e->d_ipar);
+    if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
+    rc = scan_exclusive(e, e->st_flag, e->st_pos, nbt);
+    if (rc != FQ_OK) return rc;
+    hipLaunchKernelGGL(k_compact, dim3(grid_for(nbt)), dim3(kBlock), 0, st, e->sys, nbt, tb, e->st_det, e->st_pop,
+                       e->st_ent, e->st_flag, e->st_pos, e->bcap, e->flags, e->scap, e->sdet, e->spop, e->sflag,
+                       e->sord, e->d_stats);
+    hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, st, e->st_pos, nbt, e->d_stats);
+  }
+  FQ_HIP(hipEventRecord(e->pev[3], st));
+  rc = sync_stats(e);
+  if (rc != FQ_OK) return rc;
+  const int64_t ns = std::min<int64_t>(e->h_stats.nspawn, e->scap);
+  // spawned entries are in attempt order, hence grouped by instance
+  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->sdet, ns, ninst, e->d_isb);
+  const Stats keep = e->h_stats;
+  int64_t rng_pos = e->rng_pos;
+  rc = annihilate(e, ns, false, cyc, rng_pos);
+  if (rc != FQ_OK) return rc;
+  const int o = e->cur;
+  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[o], e->n, ninst, nbd);
+  hipLaunchKernelGGL(k_inst_stats, dim3(ninst), dim3(kBlock), 0, st, ninst, S, cap, ob, nbd, e->d_isb, e->d_cterm,
+                     e->d_cdeath, e->att_off, e->pop[o], e->d_iout_d, e->d_iout_i);
+  std::vector<double> od(static_cast<size_t>(2 * S * ninst));
+  std::vector<int64_t> oi(static_cast<size_t>((4 + S) * ninst));
+  FQ_HIP(hipMemcpyAsync(od.data(), e->d_iout_d, sizeof(double) * od.size(), hipMemcpyDeviceToHost, st));
+  FQ_HIP(hipMemcpyAsync(oi.data(), e->d_iout_i, sizeof(int64_t) * oi.size(), hipMemcpyDeviceToHost, st));
+  FQ_HIP(hipEventRecord(e->ev[2], st));
+  FQ_HIP(hipStreamSynchronize(st));
+  (void)hipEventElapsedTime(&e->t_total, e->ev[0], e->ev[2]);
+  {
+    fq_profile_out& pr = e->prof;
+    float t[6] = {0, 0, 0, 0, 0, 0};
+    (void)hipEventElapsedTime(&t[0], e->pev[0], e->pev[1]);
+    (void)hipEventElapsedTime(&t[1], e->pev[2], e->pev[3]);
+    (void)hipEventElapsedTime(&t[2], e->pev[4], e->pev[5]);
+    (void)hipEventElapsedTime(&t[3], e->pev[5], e->pev[6]);
+    (void)hipEventElapsedTime(&t[4], e->pev[7], e->pev[8]);
+    (void)hipEventElapsedTime(&t[5], e->pev[0], e->pev[8]);
+    pr.det_pass_ms = t[0];
+    pr.spawn_ms = t[1];
+    pr.sort_ms = t[2];
+    pr.reduce_ms = t[3];
+    pr.merge_ms = t[4];
+    pr.total_ms = t[5];
+    pr.nstates_in = n;
+    pr.nattempts = total;
+    pr.nspawned = ns;
+    pr.nruns = e->h_stats.nruns;
+    pr.nstates_out = e->n;
+    pr.radix_passes = e->radix_passes;
+    pr.spawn_launches = std::min(e->spawn_launches, fq_engine::kSpawnEvents);
+    pr.spawn_kernel_ms = 0.0;
+    for (int li = 0; li < pr.spawn_launches; ++li) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e->sev[2 * li], e->sev[2 * li + 1]);
+      pr.spawn_kernel_ms += ms;
+    }
+  }
+  const double rf = static_cast<double>(e->sys.rf);
+  for (int k = 0; k < ninst; ++k) {
+    fq_cycle_out& r = out[k];
+    std::memset(&r, 0, sizeof(r));
+    const int64_t* q = oi.data() + k * (4 + S);
+    for (int sp = 0; sp < S; ++sp) {
+      r.proj_energy[sp] = od[k * 2 * S + 2 * sp];
+      r.D0_population[sp] = od[k * 2 * S + 2 * sp + 1];
+      r.nparticles[sp] = static_cast<double>(q[4 + sp]) / rf;
+    }
+    r.ndeath = q[0];
+    r.nattempts = q[1];
+    r.nspawn_events = q[2];
+    r.nstates = q[3];
+    r.spawn_overflow = keep.spawn_overflow ? 1 : 0;
+    r.main_overflow = e->h_stats.main_overflow ? 1 : 0;
+  }
   return FQ_OK;
 }
 
@@ -2068,6 +2437,17 @@ void fq_destroy(fq_engine* e) {
   dfree(e->d_hb_bias);
   fqm::mlp_free(e->mlp);
   dfree(e->mlp_tab);
+  dfree(e->d_sys_arr);
+  dfree(e->d_ipar);
+  dfree(e->d_ibound);
+  dfree(e->d_isb);
+  dfree(e->d_iatt);
+  dfree(e->d_cterm);
+  dfree(e->d_cdeath);
+  dfree(e->d_iout_d);
+  dfree(e->d_iout_i);
+  for (fq_engine* sub : e->subs) fq_destroy(sub);
+  e->subs.clear();
   for (int k = 0; k < 4; ++k)
     if (e->ev[k]) (void)hipEventDestroy(e->ev[k]);
   for (int k = 0; k < 9; ++k)

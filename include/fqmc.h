@@ -183,6 +183,25 @@ typedef struct fq_profile_out {
 } fq_profile_out;
 int fq_last_profile(fq_engine *e, fq_profile_out *out);
 
+/* ---- Batched independent instances ------------------------------------------
+ * Many small systems (or independent replicas) propagated by one engine: one
+ * walker list, one set of kernel launches per cycle for all instances.  No
+ * counterpart in the reference (each HANDE run is one system).  Instance k's
+ * tables come from systems[k]; all share params (integer populations,
+ * production RNG, one 64-bit word per determinant, nbasis <= 48).  The
+ * instance index occupies bits 48..63 of a determinant's word (fq_download
+ * returns it there).  Instance k draws from Philox key philox_seed + k with
+ * counters numbered from its own first determinant and attempt, so each
+ * instance follows exactly the chain it would follow alone in an engine
+ * created with philox_seed + k. */
+int fq_create_batch(const fq_system *systems, int32_t ninst, const fq_params *params, fq_engine **engine);
+/* n walkers: inst[i] is the instance of states[i] (bits 48..63 clear). */
+int fq_load_batch(fq_engine *e, const int32_t *inst, const uint64_t *states, const int64_t *pops, const double *dat,
+                  int64_t n);
+/* One cycle of every instance: shift and eproj_old are ninst x nspaces, tau is
+ * per instance; out[k] receives instance k's results. */
+int fq_cycle_batch(fq_engine *e, const double *shift, const double *eproj_old, const double *tau, fq_cycle_out *out);
+
 /* ---- Learned double-excitation proposal (excit_gen FQ_EXCIT_LEARNED) ---------
  * No counterpart in the reference: an addition beside its heat-bath generator
  * (src/excit_gen_heat_bath_mol.F90), which produces the training samples.

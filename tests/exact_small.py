@@ -64,8 +64,10 @@ def _element(h1, eri, ecore, bra, ket):
     return sign * _anti(eri, a, b, i, j)
 
 
-def ground_state(norb, nel, seed):
-    """(E0, H[hf, hf]) of the synthetic system with nel electrons, Sz = 0."""
+def ground_state(norb, nel, seed, with_weight=False):
+    """(E0, H[hf, hf]) of the synthetic system with nel electrons, Sz = 0
+    (and the ground state's weight |c_hf|^2 on the Hartree--Fock determinant
+    when with_weight: the projected energy needs it well away from zero)."""
     _eps, h1, eri = synthetic.synthetic_integrals(norb, seed)
     alphas = [2 * k + 1 for k in range(norb)]
     betas = [2 * k + 2 for k in range(norb)]
@@ -79,7 +81,9 @@ def ground_state(norb, nel, seed):
         for y in range(x, n):
             v = _element(h1, eri, 0.0, dets[x], dets[y])
             H[x, y] = H[y, x] = v
-    w = np.linalg.eigvalsh(H)
+    w, v = np.linalg.eigh(H)
     hf = tuple(sorted(alphas[:nel // 2] + betas[:nel // 2]))
     k = dets.index(hf)
+    if with_weight:
+        return float(w[0]), float(H[k, k]), float(v[k, 0] ** 2)
     return float(w[0]), float(H[k, k])
