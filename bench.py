@@ -78,7 +78,9 @@ class ProfilerRegion:
             self.lib.roctxProfilerResume(0)
 
 
-def make_engine(lib, system, target, seed, main_cap, spawn_cap):
+def molecular_params(seed, main_cap, spawn_cap, excit_gen=N.FQ_EXCIT_RENORM, mix=0.1, min_attempts=0):
+    """i-FCIQMC engine options of the molecular workloads: real_64 populations,
+    initiator threshold 3, spawn cutoff 0.01, production Philox stream."""
     p = N.FqParams()
     p.nspaces = 1
     p.initiator = 1
@@ -87,12 +89,19 @@ def make_engine(lib, system, target, seed, main_cap, spawn_cap):
     p.real_amplitudes = 1
     p.pop_bit_shift = 31
     p.spawn_cutoff = int(math.ceil(0.01 * (1 << 31)))
-    p.excit_gen = N.FQ_EXCIT_RENORM
+    p.excit_gen = excit_gen
     p.rng_mode = N.FQ_RNG_PHILOX
     p.philox_seed = seed
     p.main_capacity = main_cap
     p.spawn_capacity = spawn_cap
     p.device = 0
+    p.proposal_mix = mix
+    p.proposal_min_attempts = min_attempts
+    return p
+
+
+def make_engine(lib, system, target, seed, main_cap, spawn_cap, **kw):
+    p = molecular_params(seed, main_cap, spawn_cap, **kw)
     eng = C.c_void_p()
     N.check(lib.fq_create(system.view, C.byref(p), C.byref(eng)), "fq_create")
     return eng, p
@@ -268,6 +277,221 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": 8 * n, "repeats": 3, "ok": ok}), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# E_proj statistical efficiency 1/(sigma^2 t)
+# ---------------------------------------------------------------------------
+def prepare(lib, system, target, seed, cap, equil_tau=0.01, hold=250, max_cycles=1500):
+    """Untimed preparation (renormalised generator): returns the walker list and shift."""
+    eng, _p = make_engine(lib, system, target, seed, cap, cap)
+    f0 = system.f0()
+    pop0 = np.array([max(1, int(target * 1e-3)) << 31], dtype=np.int64)
+    dat0 = np.zeros(1)
+    N.check(lib.fq_load(eng, f0.ctypes.data_as(C.POINTER(C.c_uint64)), pop0.ctypes.data_as(C.POINTER(C.c_int64)),
+                        dat0.ctypes.data_as(C.POINTER(C.c_double)), 1), "fq_load")
+    prop = Propagator(lib, eng, target, damping=0.2, pull=1.0)
+    vary_at = None
+    for i in range(max_cycles):
+        prop.cycle(equil_tau)
+        if prop.vary and vary_at is None:
+            vary_at = i
+        if vary_at is not None and i - vary_at >= hold:
+            break
+    st, po, da = download_list(lib, eng, system.view.nwords)
+    log(f"[eff] prepared N={prop.last.nparticles[0]:.4g} on {len(da)} determinants, shift {prop.shift[0]:.4f}")
+    lib.fq_destroy(eng)
+    return st, po, da, prop.shift[0], prop.eold[0]
+
+
+def production(lib, eng, target, state, tau, cycles, warmup, report=10):
+    """Warm-up then timed production cycles from a prepared state; returns the
+    per-report (sum H_0j N_j, N_0) series, wall time and attempts."""
+    st, po, da, shift, eold = state
+    N.check(lib.fq_load(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)), po.ctypes.data_as(C.POINTER(C.c_int64)),
+                        da.ctypes.data_as(C.POINTER(C.c_double)), len(da)), "fq_load")
+    prop = Propagator(lib, eng, target)
+    prop.shift[0], prop.eold[0], prop.vary = shift, eold, True
+    for _ in range(warmup):
+        prop.cycle(tau)
+    proj, d0, attempts = [], [], 0
+    accp = accd = 0.0
+    t0 = time.perf_counter()
+    for i in range(cycles):
+        out = prop.cycle(tau)
+        attempts += out.nattempts
+        accp += out.proj_energy[0]
+        accd += out.D0_population[0]
+        if (i + 1) % report == 0:
+            proj.append(accp)
+            d0.append(accd)
+            accp = accd = 0.0
+    wall = time.perf_counter() - t0
+    return np.array(proj), np.array(d0), wall, attempts, prop.last
+
+
+def efficiency_learned(args):
+    from hande_amd import reblock
+    lib = N.lib()
+    norb, nel, seed = synthetic.MOLECULAR_CONFIGS["mol40"]
+    tmp = tempfile.mkdtemp(prefix="fqmc_eff_")
+    system = api.read_in(int_file=synthetic.write_fcidump(os.path.join(tmp, "FCIDUMP"), norb, nel, seed), nel=nel, ms=0)
+    cap = int(max(6 * args.target, 1 << 20))
+    state = prepare(lib, system, args.target, 11, cap)
+    res = {}
+    for gen in ("heat_bath", "learned"):
+        code = N.FQ_EXCIT_HEAT_BATH if gen == "heat_bath" else N.FQ_EXCIT_LEARNED
+        eng, _p = make_engine(lib, system, args.target, 23, cap, cap, excit_gen=code, mix=0.1,
+                              min_attempts=args.min_attempts)
+        t_train = 0.0
+        losses = None
+        if gen == "learned":
+            st, po, da = state[:3]
+            N.check(lib.fq_load(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                po.ctypes.data_as(C.POINTER(C.c_int64)), da.ctypes.data_as(C.POINTER(C.c_double)),
+                                len(da)), "fq_load")
+            losses = np.zeros(args.train_steps)
+            t1 = time.perf_counter()
+            N.check(lib.fq_mlp_train(eng, 16384, args.train_steps, 1e-3, 5,
+                                     losses.ctypes.data_as(C.POINTER(C.c_double))), "fq_mlp_train")
+            t_train = time.perf_counter() - t1
+            log(f"[eff] trained {args.train_steps} steps in {t_train:.1f}s: loss {losses[:10].mean():.4f} -> "
+                f"{losses[-10:].mean():.4f}")
+        proj, d0, wall, att, last = production(lib, eng, args.target, state, 1e-3, args.eff_cycles, 50)
+        lib.fq_destroy(eng)
+        rb = reblock.projected_energy(proj, d0)
+        t = wall + t_train
+        res[gen] = {"E_proj": system.H00 + rb["energy"], "sigma": rb["error"], "wall_s": wall, "train_s": t_train,
+                    "efficiency": reblock.efficiency(rb["error"], t), "attempts_per_s": att / wall,
+                    "reblock_level": rb["level"], "converged": rb["converged"],
+                    "walkers": last.nparticles[0], "determinants": int(last.nstates)}
+        log(f"[eff] {gen}: E_proj {res[gen]['E_proj']:.6f} +- {rb['error']:.6f}, {wall:.1f}s, "
+            f"efficiency {res[gen]['efficiency']:.4g}")
+    a, b = res["learned"], res["heat_bath"]
+    agree = abs(a["E_proj"] - b["E_proj"]) < 2 * math.hypot(a["sigma"], b["sigma"])
+    print(json.dumps({"metric": "E_proj efficiency 1/(sigma^2 t), learned vs heat_bath", "unit": "1/(Ha^2 s)",
+                      "value": a["efficiency"] / b["efficiency"], "higher_is_better": True,
+                      "energies_agree_2sigma": bool(agree), "n_gpus": 1, "data": "synthetic",
+                      "config": {"workload": f"i-FCIQMC mol40: synthetic molecular integrals {norb} spatial orbitals, "
+                                             f"{nel} electrons, C1, real_64, initiator 3, tau 1e-3, target "
+                                             f"{args.target:g}, {args.eff_cycles} production cycles",
+                                 "proposal_mix": 0.1, "proposal_min_attempts": args.min_attempts,
+                                 "train_steps": args.train_steps},
+                      "learned": a, "heat_bath": b}), flush=True)
+
+
+def c3_instances(n=64, seed=2024):
+    """The batched workload: n Hubbard-k rings of 10-18 sites, fillings near
+    half, U/t uniform on [1, 8], targets log-uniform on [1e4, 1e6]."""
+    rng = np.random.default_rng(seed)
+    decks, targets = [], []
+    for k in range(n):
+        L = int(rng.choice([10, 12, 14, 16, 18]))
+        nel = L + int(rng.choice([-2, 0, 0, 2]))
+        U = float(rng.uniform(1.0, 8.0))
+        decks.append(dict(electrons=nel, lattice=[[L]], ms=0, sym=1, U=U, t=1.0))
+        targets.append(float(10 ** rng.uniform(4, 6)))
+    return decks, np.array(targets)
+
+
+def integer_params(seed, cap):
+    p = N.FqParams()
+    p.nspaces = 1
+    p.initiator = 0
+    p.real_amplitudes = 0
+    p.pop_bit_shift = 0
+    p.spawn_cutoff = 0
+    p.excit_gen = N.FQ_EXCIT_RENORM
+    p.rng_mode = N.FQ_RNG_PHILOX
+    p.philox_seed = seed
+    p.main_capacity = cap
+    p.spawn_capacity = cap
+    p.device = 0
+    return p
+
+
+def efficiency_batched(args):
+    """64 instances: one batched engine against the same instances propagated
+    one engine at a time.  Batched chains equal the solo chains bit for bit, so
+    sigma is the same and the efficiency ratio is the wall-time ratio."""
+    lib = N.lib()
+    decks, targets = c3_instances(args.instances)
+    K = len(decks)
+    systems = [api.hubbard_k(**d) for d in decks]
+    tau = 0.005
+    cap = int(6 * targets.sum()) + (1 << 20)
+    arr = (N.FqSystem * K)(*[s.view for s in systems])
+    beng = C.c_void_p()
+    N.check(lib.fq_create_batch(arr, K, C.byref(integer_params(77, cap)), C.byref(beng)), "fq_create_batch")
+    inst = np.arange(K, dtype=np.int32)
+    f0 = np.array([int(s.f0()[0]) for s in systems], dtype=np.uint64)
+    pops = np.full(K, 10, dtype=np.int64)
+    N.check(lib.fq_load_batch(beng, inst.ctypes.data_as(C.POINTER(C.c_int32)), f0.ctypes.data_as(C.POINTER(C.c_uint64)),
+                              pops.ctypes.data_as(C.POINTER(C.c_int64)), np.zeros(K).ctypes.data_as(C.POINTER(C.c_double)),
+                              K), "load")
+    shift = np.zeros(K)
+    eold = np.zeros(K)
+    taus = np.full(K, tau)
+    vary = np.zeros(K, bool)
+    n_old = np.zeros(K)
+    outs = (N.FqCycleOut * K)()
+
+    def batch_cycle(c):
+        N.check(lib.fq_cycle_batch(beng, shift.ctypes.data_as(C.POINTER(C.c_double)),
+                                   eold.ctypes.data_as(C.POINTER(C.c_double)), taus.ctypes.data_as(C.POINTER(C.c_double)),
+                                   outs), "fq_cycle_batch")
+        if (c + 1) % 5 == 0:
+            n = np.array([outs[k].nparticles[0] for k in range(K)])
+            upd = vary & (n_old > 0)
+            shift[upd] -= np.log(n[upd] / n_old[upd]) * 0.05 / (tau * 5)
+            vary[:] |= n > targets
+            n_old[:] = n
+
+    c = 0
+    t0 = time.perf_counter()
+    while c < args.prep_cycles and not (vary.all() and c > 400):
+        batch_cycle(c)
+        c += 1
+    log(f"[batch] prepared {K} instances in {c} cycles ({time.perf_counter() - t0:.1f}s): "
+        f"{sum(outs[k].nparticles[0] for k in range(K)):.4g} walkers")
+    # timed batched cycles
+    attempts = 0
+    t1 = time.perf_counter()
+    for i in range(args.eff_cycles):
+        batch_cycle(c + i)
+        attempts += sum(outs[k].nattempts for k in range(K))
+    t_batch = time.perf_counter() - t1
+    # the same instances, one engine each, from the batched list
+    st, po, da = download_list(lib, beng, 1)
+    lib.fq_destroy(beng)
+    t_solo = 0.0
+    att_solo = 0
+    for k in range(K):
+        sel = (st >> np.uint64(48)) == np.uint64(k)
+        sk = np.ascontiguousarray(st[sel] & np.uint64((1 << 48) - 1))
+        pk = np.ascontiguousarray(po[sel])
+        dk = np.ascontiguousarray(da[sel])
+        eng = C.c_void_p()
+        N.check(lib.fq_create(systems[k].view, C.byref(integer_params(77 + k, max(8 * len(dk), 1 << 16))),
+                              C.byref(eng)), "fq_create")
+        N.check(lib.fq_load(eng, sk.ctypes.data_as(C.POINTER(C.c_uint64)), pk.ctypes.data_as(C.POINTER(C.c_int64)),
+                            dk.ctypes.data_as(C.POINTER(C.c_double)), len(dk)), "fq_load")
+        sh = (C.c_double * 2)(shift[k], 0.0)
+        eo = (C.c_double * 2)(0.0, 0.0)
+        o = N.FqCycleOut()
+        t2 = time.perf_counter()
+        for i in range(args.eff_cycles):
+            N.check(lib.fq_cycle(eng, sh, eo, tau, C.byref(o)), "fq_cycle")
+            att_solo += o.nattempts
+        t_solo += time.perf_counter() - t2
+        lib.fq_destroy(eng)
+    print(json.dumps({"metric": "E_proj efficiency 1/(sigma^2 t), batched vs one engine per instance",
+                      "value": t_solo / t_batch, "unit": "ratio", "higher_is_better": True, "n_gpus": 1,
+                      "data": "synthetic",
+                      "config": {"workload": f"{K} Hubbard-k rings of 10-18 sites, U/t 1-8, targets 1e4-1e6, "
+                                             f"integer populations, tau {tau}, {args.eff_cycles} cycles"},
+                      "batched": {"wall_s": t_batch, "attempts_per_s": attempts / t_batch},
+                      "one_engine_per_instance": {"wall_s": t_solo, "attempts_per_s": att_solo / t_solo}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,12 +507,26 @@ def main():
     ap.add_argument("--load-state", default=None, help="start from a list written by --save-state (no preparation)")
     ap.add_argument("--start-frac", type=float, default=1e-3, help="initial HF population / target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--efficiency", choices=["learned", "batched"], default=None,
+                    help="measure the E_proj statistical efficiency of the learned proposal (against heat-bath, "
+                         "mol40) or of batched instances (against one engine per instance) instead")
+    ap.add_argument("--eff-cycles", type=int, default=1000)
+    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--min-attempts", type=int, default=4)
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--prep-cycles", type=int, default=3000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (calibrates the HBM "
                          "byte counters FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.efficiency == "learned":
+        efficiency_learned(args)
+        return
+    if args.efficiency == "batched":
+        efficiency_batched(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the walker-propagation path runs on a single MI355X here "

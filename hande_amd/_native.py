@@ -44,7 +44,7 @@ class FqParams(C.Structure):
         ("nspaces", i32), ("initiator", i32), ("initiator_pop", f64), ("quadrature_initiator", i32),
         ("real_amplitudes", i32), ("pop_bit_shift", i32), ("spawn_cutoff", i64), ("excit_gen", i32),
         ("rng_mode", i32), ("philox_seed", u64), ("main_capacity", i64), ("spawn_capacity", i64),
-        ("device", i32), ("proposal_mix", f64),
+        ("device", i32), ("proposal_mix", f64), ("proposal_min_attempts", i32),
     ]
 
 

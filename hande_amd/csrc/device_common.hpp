@@ -66,6 +66,8 @@ struct Sys {
   // of the current main list; mix is the uniform-fallback fraction of doubles.
   const float* mlp_tab;
   int64_t mlp_ld;
+  const int64_t* mlp_on;   // per determinant: 1 if the network ran on it this cycle
+  const int64_t* mlp_row;  // its row of mlp_tab (exclusive prefix of mlp_on)
   double mix;
   int initiator, quadrature, real_amps, excit_gen;
   double initiator_pop;
@@ -1001,7 +1003,7 @@ __device__ inline void partner_stats(const Sys& s, const uint64_t* f, const floa
     const int o = ssbf(s, c.ims, c.sym, k);
     if (o != x && !btest(f, o)) {
       ++count;
-      z = z + static_cast<double>(fw[o - 1]);
+      if (fw) z = z + static_cast<double>(fw[o - 1]);
     }
   }
 }
@@ -1039,14 +1041,18 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
   if (rng.next() < s.psingle) {
     allowed = single_mol(s, rng, f, u, conn, h, pgen);
   } else {
+    // without a network row the mixture is the uniform component alone
+    const double mix = wrow ? s.mix : 1.0;
     const float* ew = wrow;
-    const float* fw = wrow + nb;
+    const float* fw = wrow ? wrow + nb : nullptr;
     uint64_t un[FQ_MAX_WORDS];
     for (int w = 0; w < s.W; ++w) un[w] = ~f[w] & s.valid[w];
     double zh = 0.0, zp = 0.0;
-    for (int w = 0; w < s.W; ++w) {
-      for (uint64_t x = f[w]; x; x &= x - 1) zh = zh + static_cast<double>(ew[(w << 6) + __builtin_ctzll(x)]);
-      for (uint64_t x = un[w]; x; x &= x - 1) zp = zp + static_cast<double>(fw[(w << 6) + __builtin_ctzll(x)]);
+    if (wrow) {
+      for (int w = 0; w < s.W; ++w) {
+        for (uint64_t x = f[w]; x; x &= x - 1) zh = zh + static_cast<double>(ew[(w << 6) + __builtin_ctzll(x)]);
+        for (uint64_t x = un[w]; x; x &= x - 1) zp = zp + static_cast<double>(fw[(w << 6) + __builtin_ctzll(x)]);
+      }
     }
     const int nvirt = nb - nel;
     int i = 0, j = 0, a = 0, b = 0;
@@ -1054,7 +1060,7 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
     PartnerClass cb;
     int nbc = 0;
     double zb = 0.0;
-    if (rng.next() < s.mix) {
+    if (rng.next() < mix) {
       // uniform component: unordered hole pair (choose_ij_mol's index map), a, partner
       const int ind = static_cast<int>(rng.next() * nel * (nel - 1) / 2) + 1;
       const int j_ind = static_cast<int>(1.50 + sqrt(2 * ind - 1.750));
@@ -1121,11 +1127,15 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
       int nac = 0;
       double za = 0.0;
       partner_stats(s, f, fw, partner_class(s, i, j, b), b, nac, za);
-      const double ei = ew[i - 1], ej = ew[j - 1], fa = fw[a - 1], fb = fw[b - 1];
-      const double hp = ei / zh * ej / (zh - ei) + ej / zh * ei / (zh - ej);
-      const double pp = fa / zp * fb / zb + fb / zp * fa / za;
       const double q_unif = 2.0 / static_cast<double>(nel * (nel - 1)) / nvirt * (1.0 / nbc + 1.0 / nac);
-      pgen = s.pdouble * ((1.0 - s.mix) * (hp * pp) + s.mix * q_unif);
+      if (wrow) {
+        const double ei = ew[i - 1], ej = ew[j - 1], fa = fw[a - 1], fb = fw[b - 1];
+        const double hp = ei / zh * ej / (zh - ei) + ej / zh * ei / (zh - ej);
+        const double pp = fa / zp * fb / zb + fb / zp * fa / za;
+        pgen = s.pdouble * ((1.0 - mix) * (hp * pp) + mix * q_unif);
+      } else {
+        pgen = s.pdouble * q_unif;
+      }
       conn.nexcit = 2;
       conn.from[0] = i < j ? i : j;
       conn.from[1] = i < j ? j : i;

@@ -87,61 +87,83 @@ struct Stats {
 };
 
 // ---------------------------------------------------------------------------
-// Prefix sums (reduce-then-scan, three kernels per level)
+// Prefix sums: tile sums, one-block scan of the tile sums, tile scans (three
+// launches at most; one when the input fits a tile).  out[n] = total.
 // ---------------------------------------------------------------------------
-__global__ void k_block_reduce(const int64_t* __restrict__ in, int64_t n, int64_t* __restrict__ sums) {
+constexpr int kScanItems = 16;
+constexpr int64_t kScanTile = static_cast<int64_t>(kBlock) * kScanItems;
+
+__global__ __launch_bounds__(kBlock) void k_tile_sums(const int64_t* __restrict__ in, int64_t n,
+                                                      int64_t* __restrict__ sums) {
   __shared__ int64_t sh[kBlock];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kBlock * 4;
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
   int64_t acc = 0;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < kScanItems; ++k) {
     const int64_t i = base + k * kBlock + threadIdx.x;
     if (i < n) acc += in[i];
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
     __syncthreads();
   }
   if (threadIdx.x == 0) sums[blockIdx.x] = sh[0];
 }
 
-// Exclusive scan of a 1024-element tile with an added per-block offset.
-__global__ void k_block_scan(const int64_t* __restrict__ in, int64_t n, const int64_t* __restrict__ offsets,
-                             int64_t* __restrict__ out) {
-  __shared__ int64_t sh[kBlock];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kBlock * 4;
-  int64_t v[4];
-  int64_t local = 0;
-  for (int k = 0; k < 4; ++k) {
-    const int64_t i = base + threadIdx.x * 4 + k;
-    v[k] = (i < n) ? in[i] : 0;
-    local += v[k];
-  }
-  sh[threadIdx.x] = local;
-  __syncthreads();
-  // Hillis-Steele inclusive scan of the per-thread sums.
-  for (int s = 1; s < kBlock; s <<= 1) {
-    int64_t add = (threadIdx.x >= s) ? sh[threadIdx.x - s] : 0;
+// Inclusive Hillis-Steele scan of sh[0..kBlock) in place.
+__device__ __forceinline__ void block_inclusive_scan(int64_t* sh) {
+  for (int h = 1; h < kBlock; h <<= 1) {
+    const int64_t add = threadIdx.x >= h ? sh[threadIdx.x - h] : 0;
     __syncthreads();
     sh[threadIdx.x] += add;
     __syncthreads();
   }
-  int64_t run = sh[threadIdx.x] - local + (offsets ? offsets[blockIdx.x] : 0);
-  for (int k = 0; k < 4; ++k) {
-    const int64_t i = base + threadIdx.x * 4 + k;
-    if (i < n) out[i] = run;
-    run += v[k];
+}
+
+// Exclusive scan of nb tile sums on one block, in chunks with a running carry.
+__global__ __launch_bounds__(kBlock) void k_scan_sums(const int64_t* __restrict__ sums, int64_t nb,
+                                                      int64_t* __restrict__ off) {
+  __shared__ int64_t sh[kBlock];
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < nb; c0 += kBlock) {
+    const int64_t i = c0 + threadIdx.x;
+    const int64_t v = i < nb ? sums[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    block_inclusive_scan(sh);
+    if (i < nb) off[i] = carry + sh[threadIdx.x] - v;
+    carry += sh[kBlock - 1];
+    __syncthreads();
   }
 }
 
-__global__ void k_set_total(const int64_t* __restrict__ in, int64_t n, int64_t* __restrict__ out) {
-  // out[n] = out[n-1] + in[n-1]
-  if (n == 0)
-    out[0] = 0;
-  else
-    out[n] = out[n - 1] + in[n - 1];
+// Exclusive scan of one tile plus its offset; the tile holding element n-1
+// also writes out[n] = total.
+__global__ __launch_bounds__(kBlock) void k_tile_scan(const int64_t* __restrict__ in, int64_t n,
+                                                      const int64_t* __restrict__ off, int64_t* __restrict__ out) {
+  __shared__ int64_t sh[kBlock];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+  int64_t v[kScanItems];
+  int64_t local = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    v[k] = i < n ? in[i] : 0;
+    local += v[k];
+  }
+  sh[threadIdx.x] = local;
+  __syncthreads();
+  block_inclusive_scan(sh);
+  int64_t run = sh[threadIdx.x] - local + (off ? off[blockIdx.x] : 0);
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+    if (i == n - 1) out[n] = run;
+  }
 }
+
+__global__ void k_set_zero_total(int64_t* __restrict__ out) { out[0] = 0; }
 
 }  // namespace
 
@@ -175,7 +197,11 @@ struct fq_engine {
   int64_t* d_iout_i = nullptr;       // [ninst][4 + S]
   // learned proposal (excit_gen FQ_EXCIT_LEARNED): network, per-determinant weights
   fqm::Mlp mlp;
-  float* mlp_tab = nullptr;  // cap x 2*nbasis
+  float* mlp_tab = nullptr;  // rows x 2*nbasis, one row per determinant the network runs on
+  int64_t mlp_tab_rows = 0;
+  int64_t* mlp_on = nullptr;   // [cap] network runs on this determinant this cycle
+  int64_t* mlp_row = nullptr;  // [cap + 1] exclusive prefix of mlp_on
+  int64_t* mlp_idx = nullptr;  // [cap] determinants the network runs on
   // main list (double-buffered)
   int64_t cap = 0, n = 0;
   int cur = 0;
@@ -284,27 +310,23 @@ int64_t* scan_scratch(fq_engine* e, int slot, int64_t need) {
   return e->scan_tmp[slot];
 }
 
-// Exclusive scan: out[0..n] with out[n] = total.  Tiles of 1024 elements;
-// block totals are scanned recursively.
-int scan_exclusive(fq_engine* e, const int64_t* in, int64_t* out, int64_t n, int level = 0) {
+// Exclusive scan: out[0..n] with out[n] = total (in and out distinct).
+int scan_exclusive(fq_engine* e, const int64_t* in, int64_t* out, int64_t n) {
   if (n == 0) {
-    hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, e->stream, in, 0, out);
+    hipLaunchKernelGGL(k_set_zero_total, dim3(1), dim3(1), 0, e->stream, out);
     return FQ_OK;
   }
-  const int64_t tile = kBlock * 4;
-  const int64_t nb = ceil_div(n, tile);
+  const int64_t nb = ceil_div(n, kScanTile);
   if (nb == 1) {
-    hipLaunchKernelGGL(k_block_scan, dim3(1), dim3(kBlock), 0, e->stream, in, n, nullptr, out);
-  } else {
-    int64_t* sums = scan_scratch(e, 2 * level, nb);
-    int64_t* sums_scan = scan_scratch(e, 2 * level + 1, nb + 1);
-    if (!sums || !sums_scan) return FQ_ERR_NOMEM;
-    hipLaunchKernelGGL(k_block_reduce, dim3(nb), dim3(kBlock), 0, e->stream, in, n, sums);
-    int rc = scan_exclusive(e, sums, sums_scan, nb, level + 1);
-    if (rc != FQ_OK) return rc;
-    hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(kBlock), 0, e->stream, in, n, sums_scan, out);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kBlock), 0, e->stream, in, n, nullptr, out);
+    return FQ_OK;
   }
-  hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, e->stream, in, n, out);
+  int64_t* sums = scan_scratch(e, 0, nb);
+  int64_t* offs = scan_scratch(e, 1, nb);
+  if (!sums || !offs) return FQ_ERR_NOMEM;
+  hipLaunchKernelGGL(k_tile_sums, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, e->stream, in, n, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, e->stream, sums, nb, offs);
+  hipLaunchKernelGGL(k_tile_scan, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, e->stream, in, n, offs, out);
   return FQ_OK;
 }
 
@@ -469,7 +491,7 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
       } else {
         fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
                         cyc ^ (fqd::kDrawAttempt << 24));
-        const float* wrow = G == fqd::kGenLearned ? s.mlp_tab + i * s.mlp_ld : nullptr;
+        const float* wrow = (G == fqd::kGenLearned && s.mlp_on[i]) ? s.mlp_tab + s.mlp_row[i] * s.mlp_ld : nullptr;
         nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child);
       }
       const int64_t u = t - t_begin;
@@ -590,9 +612,13 @@ __global__ void k_inst_att(int ninst, int S, const int64_t* __restrict__ bound, 
   if (k < ninst) iatt[k] = att_off[S * bound[k]];
 }
 
-// Per-instance cycle results, one block per instance, fixed-order sums:
-// projected-energy terms, N_0 and deaths over the old list, attempts,
-// spawned entries, occupied determinants and sum |N| over the new list.
+// Per-instance cycle results, one block per instance: projected-energy terms,
+// N_0 and deaths over the old list, attempts, spawned entries, occupied
+// determinants and sum |N| over the new list.  The floating-point sums repeat
+// the single-instance reduction exactly (blocks of 256 determinants counted
+// from the instance's first, each reduced by the k_det_pass tree, then block b
+// added to partial b mod 256 in order and the k_sum_partials tree), so a
+// batched instance reports the same estimators, bit for bit, as it alone.
 __global__ __launch_bounds__(kBlock) void k_inst_stats(int ninst, int S, int64_t cap, const int64_t* __restrict__ ob,
                                                        const int64_t* __restrict__ nbd, const int64_t* __restrict__ sb,
                                                        const double* __restrict__ cterm,
@@ -602,31 +628,43 @@ __global__ __launch_bounds__(kBlock) void k_inst_stats(int ninst, int S, int64_t
                                                        double* __restrict__ out_d, int64_t* __restrict__ out_i) {
   __shared__ double shd[2 * FQ_MAX_SPACES][kBlock];
   __shared__ int64_t shi[1 + FQ_MAX_SPACES][kBlock];
-  const int k = blockIdx.x;
+  const int k = blockIdx.x, tid = threadIdx.x;
   const int64_t a = ob[k], b = ob[k + 1], na = nbd[k], nbb = nbd[k + 1];
-  double accd[2 * FQ_MAX_SPACES] = {0.0, 0.0, 0.0, 0.0};
-  int64_t acci[1 + FQ_MAX_SPACES] = {0, 0, 0};
-  for (int64_t i = a + threadIdx.x; i < b; i += kBlock) {
-    for (int q = 0; q < 2 * S; ++q) accd[q] = accd[q] + cterm[q * cap + i];
-    acci[0] += cdeath[i];
+  const int nq = 2 * S;
+  double acc[2 * FQ_MAX_SPACES] = {0.0, 0.0, 0.0, 0.0};
+  const int64_t nvb = (b - a + kBlock - 1) / kBlock;
+  for (int64_t vb = 0; vb < nvb; ++vb) {
+    const int64_t i = a + vb * kBlock + tid;
+    for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][tid] = (q < nq && i < b) ? cterm[q * cap + i] : 0.0;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+      if (tid < h)
+        for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][tid] += shd[q][tid + h];
+      __syncthreads();
+    }
+    if (tid == static_cast<int>(vb % kBlock))
+      for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) acc[q] = acc[q] + shd[q][0];
+    __syncthreads();
   }
-  for (int64_t i = na + threadIdx.x; i < nbb; i += kBlock)
+  int64_t acci[1 + FQ_MAX_SPACES] = {0, 0, 0};
+  for (int64_t i = a + tid; i < b; i += kBlock) acci[0] += cdeath[i];
+  for (int64_t i = na + tid; i < nbb; i += kBlock)
     for (int sp = 0; sp < S; ++sp) {
       const int64_t p = pop[sp * cap + i];
       acci[1 + sp] += p < 0 ? -p : p;
     }
-  for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][threadIdx.x] = accd[q];
-  for (int q = 0; q < 1 + FQ_MAX_SPACES; ++q) shi[q][threadIdx.x] = acci[q];
+  for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][tid] = acc[q];
+  for (int q = 0; q < 1 + FQ_MAX_SPACES; ++q) shi[q][tid] = acci[q];
   __syncthreads();
   for (int h = kBlock / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) {
-      for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][threadIdx.x] = shd[q][threadIdx.x] + shd[q][threadIdx.x + h];
-      for (int q = 0; q < 1 + FQ_MAX_SPACES; ++q) shi[q][threadIdx.x] += shi[q][threadIdx.x + h];
+    if (tid < h) {
+      for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][tid] += shd[q][tid + h];
+      for (int q = 0; q < 1 + FQ_MAX_SPACES; ++q) shi[q][tid] += shi[q][tid + h];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < 2 * S; ++q) out_d[k * 2 * S + q] = shd[q][0];
+  if (tid == 0) {
+    for (int q = 0; q < nq; ++q) out_d[k * nq + q] = shd[q][0];
     int64_t* o = out_i + k * (4 + S);
     o[0] = shi[0][0];                          // deaths
     o[1] = att_off[S * b] - att_off[S * a];    // attempts
@@ -949,24 +987,13 @@ __global__ void k_reduce_runs(Sys s, int64_t nruns, const int64_t* __restrict__ 
 // Per surviving run: binary search of its key in the (old) main list; a hit
 // adds into the post-death populations (annihilate_main_list(_initiator)), a
 // miss becomes a new-determinant candidate with flagged spaces cleared.
-__global__ void k_merge_runs(Sys s, int64_t nruns, int64_t n, int64_t cap, const uint64_t* __restrict__ det,
-                             int64_t* __restrict__ pop_new, const uint64_t* __restrict__ rdet,
-                             int64_t* __restrict__ rpop, const int64_t* __restrict__ rflag, int64_t scap,
-                             int64_t* __restrict__ rpos, int64_t* __restrict__ rstate) {
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= nruns) return;
-  if (rstate[r] == 0) {
-    rpos[r] = 0;
-    return;
-  }
-  uint64_t k[FQ_MAX_WORDS] = {0, 0};
-  for (int w = 0; w < s.W; ++w) k[w] = rdet[w * scap + r];
-  // lower bound in [0, n)
-  int64_t lo = 0, hi = n;
+// Lower bound of key k in the sorted main list positions [lo, hi).
+__device__ __forceinline__ int64_t main_lower_bound(const uint64_t* __restrict__ det, int64_t cap, int W,
+                                                    const uint64_t* k, int64_t lo, int64_t hi) {
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
     int c = 0;
-    for (int w = s.W - 1; w >= 0 && c == 0; --w) {
+    for (int w = W - 1; w >= 0 && c == 0; --w) {
       const uint64_t x = det[w * cap + mid];
       c = (x < k[w]) ? -1 : (x > k[w] ? 1 : 0);
     }
@@ -975,6 +1002,39 @@ __global__ void k_merge_runs(Sys s, int64_t nruns, int64_t n, int64_t cap, const
     else
       hi = mid;
   }
+  return lo;
+}
+
+// Runs are sorted, so a block's runs land between the lower bounds of its
+// first and last keys: two searches of the whole list per block, then each
+// thread searches only that window (merge-path partition of the two lists).
+__global__ __launch_bounds__(kBlock) void k_merge_runs(Sys s, int64_t nruns, int64_t n, int64_t cap,
+                                                       const uint64_t* __restrict__ det,
+                                                       int64_t* __restrict__ pop_new, const uint64_t* __restrict__ rdet,
+                                                       int64_t* __restrict__ rpop, const int64_t* __restrict__ rflag,
+                                                       int64_t scap, int64_t* __restrict__ rpos,
+                                                       int64_t* __restrict__ rstate) {
+  __shared__ int64_t win[2];
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
+  const int64_t r1 = (r0 + blockDim.x < nruns ? r0 + blockDim.x : nruns) - 1;
+  if (threadIdx.x < 2) {
+    const int64_t rr = threadIdx.x == 0 ? r0 : r1;
+    uint64_t k[FQ_MAX_WORDS] = {0, 0};
+    for (int w = 0; w < s.W; ++w) k[w] = rdet[w * scap + rr];
+    win[threadIdx.x] = main_lower_bound(det, cap, s.W, k, 0, n);
+  }
+  __syncthreads();
+  const int64_t r = r0 + threadIdx.x;
+  if (r >= nruns) return;
+  if (rstate[r] == 0) {
+    rpos[r] = 0;
+    return;
+  }
+  uint64_t k[FQ_MAX_WORDS] = {0, 0};
+  for (int w = 0; w < s.W; ++w) k[w] = rdet[w * scap + r];
+  // lower bound in [win0, min(win1 + 1, n)): every key of the block lies in
+  // [key(r0), key(r1)], so its bound lies between theirs
+  const int64_t lo = main_lower_bound(det, cap, s.W, k, win[0], win[1] < n ? win[1] + 1 : n);
   bool hit = false;
   if (lo < n) {
     hit = true;
@@ -1532,11 +1592,32 @@ int setup_learned(fq_engine* e) {
   if (rc != FQ_OK) return rc;
   rc = fqm::mlp_init(e->mlp, e->par.philox_seed, e->stream);
   if (rc != FQ_OK) return rc;
-  if (dalloc(&e->mlp_tab, e->cap * e->mlp.d_out) != FQ_OK) return FQ_ERR_NOMEM;
+  e->mlp_tab_rows = std::min<int64_t>(e->cap, int64_t{1} << 16);
+  if (dalloc(&e->mlp_tab, e->mlp_tab_rows * e->mlp.d_out) != FQ_OK || dalloc(&e->mlp_on, e->cap) != FQ_OK ||
+      dalloc(&e->mlp_row, e->cap + 1) != FQ_OK || dalloc(&e->mlp_idx, e->cap) != FQ_OK)
+    return FQ_ERR_NOMEM;
   s.mlp_tab = e->mlp_tab;
   s.mlp_ld = e->mlp.d_out;
+  s.mlp_on = e->mlp_on;
+  s.mlp_row = e->mlp_row;
   s.mix = e->par.proposal_mix > 0.0 ? e->par.proposal_mix : 0.1;
   return FQ_OK;
+}
+
+// Determinants with at least min_att spawning attempts this cycle get a
+// network row; mlp_gather lists them in order.
+__global__ void k_mlp_mark(int64_t n, int S, const int64_t* __restrict__ nat, int64_t min_att,
+                           int64_t* __restrict__ on) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t a = 0;
+  for (int sp = 0; sp < S; ++sp) a += nat[i * S + sp];
+  on[i] = a >= min_att ? 1 : 0;
+}
+__global__ void k_mlp_gather(int64_t n, const int64_t* __restrict__ on, const int64_t* __restrict__ row,
+                             int64_t* __restrict__ idx) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && on[i]) idx[row[i]] = i;
 }
 
 int scan_cb(void* ctx, const int64_t* in, int64_t* out, int64_t n) {
@@ -1642,8 +1723,8 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   if ((params->excit_gen == FQ_EXCIT_HEAT_BATH || params->excit_gen == FQ_EXCIT_LEARNED) &&
       (sys->system != FQ_SYS_READ_IN || sys->nel > fqd::kHbMaxNel))
     return FQ_ERR_ARG;
-  if (params->excit_gen == FQ_EXCIT_LEARNED &&
-      (params->rng_mode != FQ_RNG_PHILOX || params->proposal_mix < 0.0 || params->proposal_mix > 1.0))
+  if (params->excit_gen == FQ_EXCIT_LEARNED && (params->rng_mode != FQ_RNG_PHILOX || params->proposal_mix < 0.0 ||
+                                                 params->proposal_mix > 1.0 || params->proposal_min_attempts < 0))
     return FQ_ERR_ARG;
   if (sys->system == FQ_SYS_READ_IN && params->excit_gen == FQ_EXCIT_NO_RENORM) return FQ_ERR_ARG;
   if (params->main_capacity < 1 || params->spawn_capacity < 1) return FQ_ERR_ARG;
@@ -1834,8 +1915,25 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, e->part_d, e->part_i,
                        n > 0 ? static_cast<int>(nb) : 0, e->d_stats);
     FQ_HIP(hipEventRecord(e->pev[1], st));
-    if (e->par.excit_gen == FQ_EXCIT_LEARNED) {
-      int mrc = fqm::mlp_infer(e->mlp, e->sys, e->det[c], cap, n, e->mlp_tab, st);
+    if (e->par.excit_gen == FQ_EXCIT_LEARNED && n > 0) {
+      hipLaunchKernelGGL(k_mlp_mark, dim3(grid_for(n)), dim3(kBlock), 0, st, n, e->S, e->nat,
+                         static_cast<int64_t>(e->par.proposal_min_attempts), e->mlp_on);
+      int mrc = scan_exclusive(e, e->mlp_on, e->mlp_row, n);
+      if (mrc != FQ_OK) return mrc;
+      hipLaunchKernelGGL(k_mlp_gather, dim3(grid_for(n)), dim3(kBlock), 0, st, n, e->mlp_on, e->mlp_row,
+                         e->mlp_idx);
+      int64_t ninf = 0;
+      FQ_HIP(hipMemcpyAsync(&ninf, e->mlp_row + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+      FQ_HIP(hipStreamSynchronize(st));
+      if (ninf > e->mlp_tab_rows) {  // grow the weight table (rare: rows follow the inferred determinants)
+        const int64_t rows = std::min<int64_t>(e->cap, std::max<int64_t>(ninf, 2 * e->mlp_tab_rows));
+        dfree(e->mlp_tab);
+        e->mlp_tab_rows = 0;
+        if (dalloc(&e->mlp_tab, rows * e->mlp.d_out) != FQ_OK) return FQ_ERR_NOMEM;
+        e->mlp_tab_rows = rows;
+        e->sys.mlp_tab = e->mlp_tab;
+      }
+      mrc = fqm::mlp_infer(e->mlp, e->sys, e->det[c], cap, ninf, e->mlp_tab, st, e->mlp_idx);
       if (mrc != FQ_OK) return mrc;
     }
     // Attempt offsets over the n*S (det, space) entries.
@@ -2437,6 +2535,9 @@ void fq_destroy(fq_engine* e) {
   dfree(e->d_hb_bias);
   fqm::mlp_free(e->mlp);
   dfree(e->mlp_tab);
+  dfree(e->mlp_on);
+  dfree(e->mlp_row);
+  dfree(e->mlp_idx);
   dfree(e->d_sys_arr);
   dfree(e->d_ipar);
   dfree(e->d_ibound);

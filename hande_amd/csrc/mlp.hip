@@ -66,6 +66,7 @@ __device__ inline void mfma_rows16(const uint16_t* A, int lda, int K, const uint
 
 __global__ __launch_bounds__(kBlock) void k_mlp_infer(int nb, int k1, int n3, int d_out,
                                                       const uint64_t* __restrict__ det, int64_t cap, int64_t n,
+                                                      const int64_t* __restrict__ idx,
                                                       const double* __restrict__ eps,
                                                       const uint16_t* __restrict__ w1t, const float* __restrict__ b1,
                                                       const uint16_t* __restrict__ w2t, const float* __restrict__ b2,
@@ -74,13 +75,14 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(int nb, int k1, int n3, in
   __shared__ __attribute__((aligned(16))) uint16_t bufA[kInferRows * kHidden];
   __shared__ __attribute__((aligned(16))) uint16_t bufB[kInferRows * kHidden];
   const int64_t row0 = static_cast<int64_t>(blockIdx.x) * kInferRows;
-  for (int idx = threadIdx.x; idx < kInferRows * k1; idx += blockDim.x) {
-    const int r = idx / k1, c = idx - r * k1;
+  for (int el = threadIdx.x; el < kInferRows * k1; el += blockDim.x) {
+    const int r = el / k1, c = el - r * k1;
     const int64_t d = row0 + r;
     float x = 0.0f;
     if (d < n && c < 2 * nb) {
       const int o = c < nb ? c : c - nb;
-      const bool occ = (det[static_cast<int64_t>(o >> 6) * cap + d] >> (o & 63)) & 1;
+      const int64_t src = idx ? idx[d] : d;
+      const bool occ = (det[static_cast<int64_t>(o >> 6) * cap + src] >> (o & 63)) & 1;
       x = occ ? (c < nb ? 1.0f : static_cast<float>(eps[o])) : 0.0f;
     }
     bufA[r * k1 + c] = f2bf(x);
@@ -472,10 +474,11 @@ int mlp_init(Mlp& m, uint64_t seed, hipStream_t st) {
 }
 
 int mlp_infer(const Mlp& m, const fqd::Sys& s, const uint64_t* det, int64_t cap, int64_t n, float* tab,
-              hipStream_t st) {
+              hipStream_t st, const int64_t* idx) {
   if (n <= 0) return FQ_OK;
   const unsigned g = static_cast<unsigned>(ceil_div(n, kInferRows));
-  hipLaunchKernelGGL(k_mlp_infer, dim3(g), dim3(kBlock), 0, st, m.nb, m.k1, m.n3, m.d_out, det, cap, n, s.sp_eigv,
+  hipLaunchKernelGGL(k_mlp_infer, dim3(g), dim3(kBlock), 0, st, m.nb, m.k1, m.n3, m.d_out, det, cap, n, idx,
+                     s.sp_eigv,
                      m.w1t, m.p + m.o_b1, m.w2t, m.p + m.o_b2, m.w3t, m.p + m.o_b3, tab,
                      static_cast<int64_t>(m.d_out));
   MLP_HIP(hipGetLastError());

@@ -52,9 +52,10 @@ int mlp_init(Mlp& m, uint64_t seed, hipStream_t st);
 int mlp_set(Mlp& m, const float* host, hipStream_t st);
 // bf16 inference copies from the master parameters.
 int mlp_refresh(Mlp& m, hipStream_t st);
-// Proposal weights of determinants [0, n) of a structure-of-arrays list.
+// Proposal weights of determinants [0, n) of a structure-of-arrays list (row
+// r of tab for determinant r), or of determinants idx[0..n) when idx is given.
 int mlp_infer(const Mlp& m, const fqd::Sys& s, const uint64_t* det, int64_t cap, int64_t n, float* tab,
-              hipStream_t st);
+              hipStream_t st, const int64_t* idx = nullptr);
 // Forward/backward over the first ns stored samples (sdet/sexc/sw): fills g,
 // leaves sum(w) and the weighted loss in dsum.  apply: one Adam step (lr).
 int mlp_train_step(Mlp& m, const fqd::Sys& s, int64_t ns, double lr, bool apply, hipStream_t st);

@@ -47,6 +47,7 @@ module fqmc_engine
         integer(c_int64_t) :: main_capacity, spawn_capacity
         integer(c_int32_t) :: device
         real(c_double) :: proposal_mix
+        integer(c_int32_t) :: proposal_min_attempts
     end type fq_params
 
     type, bind(c) :: fq_cycle_out

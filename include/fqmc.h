@@ -105,6 +105,9 @@ typedef struct fq_params {
   int32_t device;               /* HIP device ordinal */
   double proposal_mix;          /* FQ_EXCIT_LEARNED: fraction of doubles drawn from the uniform
                                    fallback, in (0, 1]; 0 selects 0.1 */
+  int32_t proposal_min_attempts; /* FQ_EXCIT_LEARNED: determinants with fewer spawning attempts in a
+                                    cycle skip the network and draw doubles from the fallback alone
+                                    (still exact); 0 runs the network on every determinant */
 } fq_params;
 
 /* Per-cycle results, the quantities the reference accumulates per cycle. */

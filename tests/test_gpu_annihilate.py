@@ -126,3 +126,61 @@ def test_annihilate_molecular_new_kii(tmp_path):
     rng = np.random.default_rng(9)
     main, pops, sk, sp, sf = _random_state(rng, system, 200, 600, 1, False, big=False)
     assert _run(lib, orc, system, st, p, main, pops, sk, sp, sf, stable=False) > 0
+
+
+def _random_keys(rng, nbits, nel, count):
+    """count distinct random nbits-bit keys with nel bits set (vectorised rejection)."""
+    table = np.array([bin(b).count("1") for b in range(256)], dtype=np.int64)
+    out = np.zeros(0, dtype=np.uint64)
+    while len(out) < count:
+        x = rng.integers(0, 1 << nbits, 8 * count, dtype=np.uint64)
+        c = table[x.view(np.uint8).reshape(-1, 8)].sum(axis=1)
+        out = np.unique(np.concatenate([out, x[c == nel]]))
+    return rng.permutation(out)[:count]
+
+
+def test_annihilate_production_large_lists():
+    """Lists of 1e6 entries: every prefix sum spans many tiles and several
+    chunks of the tile-sum scan; radix sort over 32-bit keys in 4 passes."""
+    print("[annihilate] production, 6e5 main + 1.2e6 spawned", flush=True)
+    lib, orc = N.lib(), oracle()
+    system = api.hubbard_k(electrons=10, lattice=[[4, 0], [0, 4]], ms=0, sym="tot_sym", U=4)
+    rng = np.random.default_rng(21)
+    st, p = _mk(system, True, True, "philox")
+    p.main_capacity = 1 << 22
+    p.spawn_capacity = 1 << 22
+    rf = 1 << p.pop_bit_shift
+    nmain, nspawn = 600_000, 1_200_000
+    keys = _random_keys(rng, 32, 10, nmain + nspawn // 2)
+    main = np.sort(keys[:nmain])
+    pops = (rng.integers(1, 20, nmain) * rng.choice([-1, 1], nmain) * (3 * rf)).astype(np.int64)
+    sk = rng.choice(keys, nspawn, replace=True)
+    sp = (rng.integers(1, 4, nspawn) * rng.choice([-1, 1], nspawn) * (3 * rf // 2)).astype(np.int64)
+    sf = rng.integers(0, 2, nspawn).astype(np.int64)
+    eng = C.c_void_p()
+    N.check(lib.fq_create(system.view, C.byref(p), C.byref(eng)), "fq_create")
+    h = orc.orc_create(system.view, C.byref(p), st.rng_seed)
+    orc.orc_set_stable_order(h, 1)
+    ptr = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+    kii = np.zeros(nmain)
+    try:
+        N.check(lib.fq_load(eng, ptr(main, C.c_uint64), ptr(pops, C.c_int64), ptr(kii, C.c_double), nmain), "load")
+        orc.orc_load(h, ptr(main, C.c_uint64), ptr(pops, C.c_int64), ptr(kii, C.c_double), nmain)
+        o1, o2 = N.FqCycleOut(), N.FqCycleOut()
+        N.check(lib.fq_annihilate(eng, ptr(sk, C.c_uint64), ptr(sp, C.c_int64), ptr(sf, C.c_int64), nspawn,
+                                  C.byref(o1)), "fq_annihilate")
+        orc.orc_annihilate(h, ptr(sk, C.c_uint64), ptr(sp, C.c_int64), ptr(sf, C.c_int64), nspawn, C.byref(o2))
+        assert o1.nstates == o2.nstates and o1.nparticles[0] == o2.nparticles[0]
+        n = o1.nstates
+        a = [np.zeros(n, np.uint64), np.zeros(n, np.int64), np.zeros(n)]
+        got = C.c_int64()
+        N.check(lib.fq_download(eng, ptr(a[0], C.c_uint64), ptr(a[1], C.c_int64), ptr(a[2], C.c_double), n,
+                                C.byref(got)), "download")
+        b = [np.zeros(n, np.uint64), np.zeros(n, np.int64), np.zeros(n)]
+        orc.orc_download(h, ptr(b[0], C.c_uint64), ptr(b[1], C.c_int64), ptr(b[2], C.c_double))
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+        print(f"  {n} determinants after annihilation", flush=True)
+    finally:
+        lib.fq_destroy(eng)
+        orc.orc_destroy(h)
