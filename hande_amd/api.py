@@ -114,8 +114,11 @@ def qmc_in(qmc, fciqmc_opts=None, rng_mode="replay", device=0):
     st.target_population = float(q.get("target_population", 1.0e300))
     st.shift_damping = float(q.get("shift_damping", 0.05))
     st.initial_shift = float(q.get("initial_shift", 0.0))
-    # vary_shift_from defaults to initial_shift (src/lua_hande_calc.f90:1483-1491)
-    st.vary_shift_from = float(q.get("vary_shift_from", st.initial_shift))
+    # vary_shift_from defaults to initial_shift; 'proje' starts from the projected
+    # energy (src/lua_hande_calc.f90:1483-1491)
+    vsf = q.get("vary_shift_from", st.initial_shift)
+    st.vary_shift_from_proje = int(isinstance(vsf, str) and vsf == "proje")
+    st.vary_shift_from = 0.0 if st.vary_shift_from_proje else float(vsf)
     st.initiator = int(bool(q.get("initiator", False)))
     st.initiator_threshold = float(q.get("initiator_threshold", 3.0))
     st.real_amplitudes = int(bool(q.get("real_amplitudes", False)))

@@ -185,6 +185,8 @@ struct fq_engine {
   bool batched = false;
   int ninst = 1;
   int key_bits = 0;                  // key bits the radix sort orders (nbasis, or 48 + instance bits)
+  int rank_words = 0, rank_bits = 0;  // production sort on combinatorial ranks (0: on the bit strings)
+  uint64_t* d_binom = nullptr;        // C(p, i), p < nbasis, 1 <= i <= nel, as (low, high) halves
   std::vector<fq_engine*> subs;
   Sys* d_sys_arr = nullptr;          // [ninst]
   double* d_ipar = nullptr;          // tau[ninst], shift[S][ninst], eproj_old[S][ninst]
@@ -810,6 +812,33 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __rest
   }
 }
 
+// Sort keys as combinatorial ranks: for a bit string with set bits
+// p_1 < p_2 < ... < p_k (0-based), rank = sum_i C(p_i, i).  For a fixed number
+// of set bits this is a bijection onto [0, C(nbasis, k)) that preserves the
+// numeric order of the strings, so sorting ranks sorts determinants in the
+// main list's order with ceil(log2 C(nbasis, nel)) key bits instead of nbasis
+// (34 instead of 52 for 10 electrons in 52 spin orbitals).  binom holds
+// C(p, i) for p < nbasis, 1 <= i <= nel as (low, high) 64-bit halves.
+__global__ void k_rank_keys(const uint64_t* __restrict__ sdet, int64_t ns, int W, int64_t scap,
+                            const uint64_t* __restrict__ binom, int nel, int rank_words,
+                            uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  unsigned __int128 rank = 0;
+  int k = 0;
+  for (int w = 0; w < W; ++w) {
+    for (uint64_t x = sdet[w * scap + i]; x; x &= x - 1) {
+      const int p = (w << 6) + __builtin_ctzll(x);
+      const uint64_t* c = binom + 2 * (static_cast<int64_t>(p) * nel + k);
+      rank += (static_cast<unsigned __int128>(c[1]) << 64) | c[0];
+      ++k;
+    }
+  }
+  keys[i] = static_cast<uint64_t>(rank);
+  if (rank_words > 1) keys[scap + i] = static_cast<uint64_t>(rank >> 64);
+  vals[i] = static_cast<uint32_t>(i);
+}
+
 __global__ void k_iota_copy_keys(const uint64_t* __restrict__ sdet, int64_t ns, int W, int64_t scap,
                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -916,7 +945,7 @@ __global__ void k_run_starts(const int64_t* __restrict__ head, const int64_t* __
 // mode the first entry is the first in sorted order (the reference's); in
 // production it is the entry with the smallest attempt ordinal.
 __global__ void k_reduce_runs(Sys s, int64_t nruns, const int64_t* __restrict__ run_start,
-                              const uint64_t* __restrict__ keys, const uint32_t* __restrict__ perm,
+                              const uint64_t* __restrict__ sdet, const uint32_t* __restrict__ perm,
                               const int64_t* __restrict__ spop, const int64_t* __restrict__ sflag,
                               const uint64_t* __restrict__ sord, int64_t scap, int first_by_order,
                               uint64_t* __restrict__ rdet, int64_t* __restrict__ rpop, int64_t* __restrict__ rflag,
@@ -976,7 +1005,7 @@ __global__ void k_reduce_runs(Sys s, int64_t nruns, const int64_t* __restrict__ 
     }
     rpop[sp * scap + r] = tot[sp];
   }
-  for (int w = 0; w < s.W; ++w) rdet[w * scap + r] = keys[w * scap + a];
+  for (int w = 0; w < s.W; ++w) rdet[w * scap + r] = sdet[w * scap + perm[a]];
   rflag[r] = fl;
   rstate[r] = any ? 1 : 0;
 }
@@ -1276,6 +1305,7 @@ int sync_stats(fq_engine* e) {
 int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
   const int W = e->W;
   const int64_t scap = e->scap;
+  int kw = W, kbits = e->key_bits;  // key words and significant bits the sort orders
   uint32_t* perm = nullptr;
   uint64_t* keys = nullptr;
   e->radix_passes = 0;
@@ -1287,8 +1317,15 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
       perm = e->vbuf[0];
       keys = e->kbuf[0];
     } else {
-      hipLaunchKernelGGL(k_iota_copy_keys, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->sdet, ns, W, scap,
-                         e->kbuf[0], e->vbuf[0]);
+      if (e->rank_words) {
+        kw = e->rank_words;
+        kbits = e->rank_bits;
+        hipLaunchKernelGGL(k_rank_keys, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->sdet, ns, W, scap,
+                           e->d_binom, e->sys.nel, e->rank_words, e->kbuf[0], e->vbuf[0]);
+      } else {
+        hipLaunchKernelGGL(k_iota_copy_keys, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->sdet, ns, W, scap,
+                           e->kbuf[0], e->vbuf[0]);
+      }
       const int64_t nb = ceil_div(ns, kRadixTile);
       if (e->hist_cap < 256 * nb + 1) {
         dfree(e->hist);
@@ -1298,15 +1335,15 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
         e->hist_cap = 256 * nb + 1;
       }
       int src = 0;
-      for (int w = 0; w < W; ++w) {
-        const int bits = std::min(64, e->key_bits - 64 * w);
+      for (int w = 0; w < kw; ++w) {
+        const int bits = std::min(64, kbits - 64 * w);
         for (int shift = 0; shift < bits; shift += 8) {
           hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src] + w * scap, ns, shift,
                              e->hist);
           int rc = scan_exclusive(e, e->hist, e->hist_scan, 256 * nb);
           if (rc != FQ_OK) return rc;
           hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src], e->vbuf[src], ns,
-                             W, w, shift, e->hist_scan, e->kbuf[1 - src], e->vbuf[1 - src], scap);
+                             kw, w, shift, e->hist_scan, e->kbuf[1 - src], e->vbuf[1 - src], scap);
           src = 1 - src;
           ++e->radix_passes;
         }
@@ -1317,7 +1354,7 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
   }
   FQ_HIP(hipEventRecord(e->pev[5], e->stream));
   if (ns > 0) {
-    hipLaunchKernelGGL(k_heads, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, keys, ns, W, scap, e->head);
+    hipLaunchKernelGGL(k_heads, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, keys, ns, kw, scap, e->head);
     int rc = scan_exclusive(e, e->head, e->head_scan, ns);
     if (rc != FQ_OK) return rc;
     hipLaunchKernelGGL(k_run_starts, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->head, e->head_scan, ns,
@@ -1328,7 +1365,7 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
     FQ_HIP(hipMemcpyAsync(&nruns, e->head_scan + ns, sizeof(int64_t), hipMemcpyDeviceToHost, e->stream));
     FQ_HIP(hipStreamSynchronize(e->stream));
     hipLaunchKernelGGL(k_reduce_runs, dim3(grid_for(nruns)), dim3(kBlock), 0, e->stream, e->sys, nruns,
-                       e->run_start, keys, perm, e->spop, e->sflag, e->sord, scap, 1, e->rdet, e->rpop,
+                       e->run_start, e->sdet, perm, e->spop, e->sflag, e->sord, scap, 1, e->rdet, e->rpop,
                        e->rflag, e->rstate);
   }
   FQ_HIP(hipEventRecord(e->pev[6], e->stream));
@@ -1584,6 +1621,38 @@ int build_heat_bath(fq_engine* e) {
   return FQ_OK;
 }
 
+template <class T>
+int upload(T** dst, const T* src, int64_t count, hipStream_t st);
+
+// Binomial table and key width for the production sort on combinatorial ranks
+// (used when the rank needs fewer bits than the bit string).
+int setup_rank_keys(fq_engine* e) {
+  const int nb = e->sys.nbasis, nel = e->sys.nel;
+  if (nel < 1 || nel > nb) return FQ_OK;
+  // Pascal's triangle up to row nb in 128-bit integers (C(128, 64) < 2^128)
+  std::vector<unsigned __int128> row(static_cast<size_t>(nel) + 1, 0), prev;
+  std::vector<uint64_t> tab(static_cast<size_t>(2) * nb * nel, 0);
+  row[0] = 1;
+  for (int p = 0; p <= nb; ++p) {
+    if (p > 0) {
+      prev = row;
+      for (int i = 1; i <= nel; ++i) row[i] = prev[i] + prev[i - 1];
+    }
+    if (p < nb)
+      for (int i = 1; i <= nel; ++i) {
+        tab[2 * (static_cast<size_t>(p) * nel + (i - 1))] = static_cast<uint64_t>(row[i]);
+        tab[2 * (static_cast<size_t>(p) * nel + (i - 1)) + 1] = static_cast<uint64_t>(row[i] >> 64);
+      }
+  }
+  unsigned __int128 top = row[nel] - 1;  // largest rank: C(nb, nel) - 1
+  int bits = 1;
+  while (bits < 128 && (top >> bits) != 0) ++bits;
+  if (bits >= nb) return FQ_OK;  // no gain over the bit strings
+  e->rank_bits = bits;
+  e->rank_words = bits > 64 ? 2 : 1;
+  return upload(&e->d_binom, tab.data(), static_cast<int64_t>(tab.size()), e->stream);
+}
+
 constexpr int64_t kTrainBatchMax = 16384;  // heat-bath samples per training step
 
 int setup_learned(fq_engine* e) {
@@ -1818,6 +1887,7 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   s.eri = e->d_eri;
   s.jmat = e->d_jmat;
   s.kmat = e->d_kmat;
+  if (rc == FQ_OK && params->rng_mode == FQ_RNG_PHILOX) rc = setup_rank_keys(e);
   if (rc == FQ_OK && (params->excit_gen == FQ_EXCIT_HEAT_BATH || params->excit_gen == FQ_EXCIT_LEARNED))
     rc = build_heat_bath(e);
   if (rc == FQ_OK && params->excit_gen == FQ_EXCIT_LEARNED) rc = setup_learned(e);
@@ -2120,6 +2190,7 @@ int fq_create_batch(const fq_system* systems, int32_t ninst, const fq_params* pa
   if (rc != FQ_OK) return rc;
   e->batched = true;
   e->ninst = ninst;
+  e->rank_words = 0;  // the instance index is part of the key: sort the encoded words
   int ib = 0;
   while ((1 << ib) < ninst) ++ib;
   e->key_bits = kInstShift + std::max(ib, 1);
@@ -2538,6 +2609,7 @@ void fq_destroy(fq_engine* e) {
   dfree(e->mlp_on);
   dfree(e->mlp_row);
   dfree(e->mlp_idx);
+  dfree(e->d_binom);
   dfree(e->d_sys_arr);
   dfree(e->d_ipar);
   dfree(e->d_ibound);

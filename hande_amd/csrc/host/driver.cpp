@@ -174,7 +174,7 @@ extern "C" int fqh_fciqmc(const fqh_system* hs, const fqh_qmc_in* in, fqh_report
     for (int s = 0; s < S; ++s) {
       if (!vary[s] && ntot[s] > in->target_population) {
         vary[s] = true;
-        shift[s] = in->vary_shift_from;
+        shift[s] = in->vary_shift_from_proje ? proj[s] / d0[s] : in->vary_shift_from;
       }
     }
     emit(static_cast<int64_t>(ireport) * in->mc_cycles, nspawn_events, rspawn, tcyc);

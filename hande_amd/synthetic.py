@@ -14,7 +14,11 @@ symmetry with
 import numpy as np
 
 
-def synthetic_integrals(norb, seed, nchol=80):
+def synthetic_integrals(norb, seed, nchol=80, chol_scale=0.1):
+    """(eps, h, eri) of the recipe above; chol_scale is the standard deviation of
+    the Cholesky vectors (0.1 for the benchmark systems; smaller values give
+    weakly correlated test systems whose ground state is close to the
+    reference determinant)."""
     rng = np.random.default_rng(seed)
     eps = np.linspace(-2.0, 3.0, norb)
     idx = np.arange(norb)
@@ -22,14 +26,14 @@ def synthetic_integrals(norb, seed, nchol=80):
     h = rng.normal(0.0, 0.05, (norb, norb)) * damp
     h = np.triu(h, 1)
     h = h + h.T + np.diag(eps)
-    L = rng.normal(0.0, 0.1, (nchol, norb, norb)) * damp[None]
+    L = rng.normal(0.0, chol_scale, (nchol, norb, norb)) * damp[None]
     L = 0.5 * (L + L.transpose(0, 2, 1))
     eri = np.einsum("Ppq,Prs->pqrs", L, L, optimize=True)
     return eps, h, eri
 
 
-def write_fcidump(path, norb, nelec, seed, nchol=80, ecore=0.0):
-    eps, h, eri = synthetic_integrals(norb, seed, nchol)
+def write_fcidump(path, norb, nelec, seed, nchol=80, ecore=0.0, chol_scale=0.1):
+    eps, h, eri = synthetic_integrals(norb, seed, nchol, chol_scale)
     lines = [f" &FCI NORB={norb},NELEC={nelec},MS2=0,",
              "  ORBSYM=" + ",".join(["1"] * norb) + ",",
              "  ISYM=1,", " &END"]

@@ -69,6 +69,8 @@ typedef struct fqh_qmc_in {
   int32_t quadrature_initiator;
   int32_t rng_mode;             /* enum fq_rng_mode */
   int32_t device;
+  int32_t vary_shift_from_proje; /* vary_shift_from = 'proje': the shift starts from the report's projected
+                                    energy (src/energy_evaluation.F90:638-648) */
 } fqh_qmc_in;
 
 /* One row of the QMC report table (src/qmc_io.f90:412-508). */

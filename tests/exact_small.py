@@ -64,11 +64,11 @@ def _element(h1, eri, ecore, bra, ket):
     return sign * _anti(eri, a, b, i, j)
 
 
-def ground_state(norb, nel, seed, with_weight=False):
+def ground_state(norb, nel, seed, with_weight=False, chol_scale=0.1):
     """(E0, H[hf, hf]) of the synthetic system with nel electrons, Sz = 0
     (and the ground state's weight |c_hf|^2 on the Hartree--Fock determinant
     when with_weight: the projected energy needs it well away from zero)."""
-    _eps, h1, eri = synthetic.synthetic_integrals(norb, seed)
+    _eps, h1, eri = synthetic.synthetic_integrals(norb, seed, chol_scale=chol_scale)
     alphas = [2 * k + 1 for k in range(norb)]
     betas = [2 * k + 2 for k in range(norb)]
     dets = []

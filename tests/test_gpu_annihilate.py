@@ -184,3 +184,60 @@ def test_annihilate_production_large_lists():
     finally:
         lib.fq_destroy(eng)
         orc.orc_destroy(h)
+
+
+def test_annihilate_production_two_word_rank_keys():
+    """128 spin orbitals and 20 electrons: the production sort orders 78-bit
+    combinatorial ranks held in two words."""
+    print("[annihilate] production, 128 spin orbitals, two-word rank keys", flush=True)
+    lib, orc = N.lib(), oracle()
+    system = api.hubbard_k(electrons=20, lattice=[[64]], ms=0, sym="tot_sym", U=2)
+    assert system.view.nwords == 2 and system.view.nbasis == 128
+    rng = np.random.default_rng(33)
+    st, p = _mk(system, True, True, "philox")
+    p.main_capacity = 1 << 18
+    p.spawn_capacity = 1 << 18
+    rf = 1 << p.pop_bit_shift
+    nmain, nspawn = 20_000, 60_000
+    keys = set()
+    while len(keys) < nmain + nspawn // 2:
+        occ = rng.choice(128, 20, replace=False)
+        keys.add(sum(1 << int(o) for o in occ))
+    keys = sorted(keys)
+    main_keys = sorted(rng.choice(len(keys), nmain, replace=False))
+    main = np.array([[keys[i] & ((1 << 64) - 1), keys[i] >> 64] for i in main_keys], dtype=np.uint64)
+    pops = (rng.integers(1, 20, nmain) * rng.choice([-1, 1], nmain) * (3 * rf)).astype(np.int64)
+    pick = rng.integers(0, len(keys), nspawn)
+    sk = np.array([[keys[i] & ((1 << 64) - 1), keys[i] >> 64] for i in pick], dtype=np.uint64)
+    sp = (rng.integers(1, 4, nspawn) * rng.choice([-1, 1], nspawn) * (3 * rf // 2)).astype(np.int64)
+    sf = rng.integers(0, 2, nspawn).astype(np.int64)
+    eng = C.c_void_p()
+    N.check(lib.fq_create(system.view, C.byref(p), C.byref(eng)), "fq_create")
+    h = orc.orc_create(system.view, C.byref(p), st.rng_seed)
+    orc.orc_set_stable_order(h, 1)
+    ptr = lambda a, t: np.ascontiguousarray(a).ctypes.data_as(C.POINTER(t))
+    main = np.ascontiguousarray(main)
+    sk = np.ascontiguousarray(sk)
+    kii = np.zeros(nmain)
+    try:
+        N.check(lib.fq_load(eng, ptr(main, C.c_uint64), ptr(pops, C.c_int64), ptr(kii, C.c_double), nmain), "load")
+        orc.orc_load(h, ptr(main, C.c_uint64), ptr(pops, C.c_int64), ptr(kii, C.c_double), nmain)
+        o1, o2 = N.FqCycleOut(), N.FqCycleOut()
+        N.check(lib.fq_annihilate(eng, ptr(sk, C.c_uint64), ptr(sp, C.c_int64), ptr(sf, C.c_int64), nspawn,
+                                  C.byref(o1)), "fq_annihilate")
+        orc.orc_annihilate(h, ptr(sk, C.c_uint64), ptr(sp, C.c_int64), ptr(sf, C.c_int64), nspawn, C.byref(o2))
+        assert o1.nstates == o2.nstates and o1.nparticles[0] == o2.nparticles[0]
+        n = o1.nstates
+        a = [np.zeros(2 * n, np.uint64), np.zeros(n, np.int64), np.zeros(n)]
+        got = C.c_int64()
+        N.check(lib.fq_download(eng, a[0].ctypes.data_as(C.POINTER(C.c_uint64)),
+                                a[1].ctypes.data_as(C.POINTER(C.c_int64)), a[2].ctypes.data_as(C.POINTER(C.c_double)),
+                                n, C.byref(got)), "download")
+        b = [np.zeros(2 * n, np.uint64), np.zeros(n, np.int64), np.zeros(n)]
+        orc.orc_download(h, b[0].ctypes.data_as(C.POINTER(C.c_uint64)), b[1].ctypes.data_as(C.POINTER(C.c_int64)),
+                         b[2].ctypes.data_as(C.POINTER(C.c_double)))
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    finally:
+        lib.fq_destroy(eng)
+        orc.orc_destroy(h)

@@ -38,14 +38,14 @@ def test_replay_walker_lists_match_oracle_every_cycle(name, ncycles):
     lockstep(system, st, ncycles, label=name)
 
 
-MOLECULAR = {"H2O-RHF-cc-pVTZ": 30, "Ne-RHF-aug-cc-pVDZ": 40}
+MOLECULAR = {"H2O-RHF-cc-pVTZ": 30, "Ne-RHF-aug-cc-pVDZ": 40, "H2-sto-3g-replica": 1500}
 
 
 @pytest.mark.parametrize("name", sorted(MOLECULAR))
 def test_replay_reproduces_molecular_golden(name, tmp_path):
-    """Molecular renorm generator, Slater--Condon rules and (Ne) the initiator
-    rules on the GPU, replaying the reference's stream: the first report rows of
-    the golden table, bit-exact at 1e-10."""
+    """Molecular renorm generator, Slater--Condon rules, (Ne) the initiator
+    rules and (H2) a replica pair on the GPU, replaying the reference's stream:
+    the first report rows of the golden table, bit-exact at 1e-10."""
     from helpers import deck
     e = goldens()[name]
     print(f"[replay molecular] {name}", flush=True)
@@ -54,3 +54,19 @@ def test_replay_reproduces_molecular_golden(name, tmp_path):
     q["nreports"] = MOLECULAR[name]
     got = api.fciqmc(system, q, opts.get("fciqmc"), rng_mode="replay")
     compare_tables(got, e["report"][:MOLECULAR[name] + 1])
+
+
+@pytest.mark.parametrize("excit_gen", ["renorm", "heat_bath"])
+def test_replay_lockstep_two_word_determinants(tmp_path, excit_gen):
+    """68 spin orbitals: determinants span two 64-bit words (orbitals 65-68 in
+    the second); real_64 initiator run, walker lists bit-identical each cycle."""
+    import os
+    from hande_amd import synthetic
+    print(f"[replay lockstep] two words, {excit_gen}", flush=True)
+    path = synthetic.write_fcidump(os.path.join(str(tmp_path), "FCIDUMP"), 34, 6, 3406)
+    system = api.read_in(int_file=path, nel=6, ms=0)
+    assert system.view.nwords == 2
+    st = api.qmc_in(dict(tau=0.01, rng_seed=19, init_pop=30, real_amplitudes=True, initiator=True,
+                         excit_gen=excit_gen), rng_mode="replay")
+    out = lockstep(system, st, 80, label=f"W=2 {excit_gen}")
+    assert out.nstates > 50

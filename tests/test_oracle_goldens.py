@@ -31,8 +31,9 @@ def test_oracle_reproduces_golden_trajectory(name):
 
 
 # Molecular decks (integral dumps from the reference's test suite): H2O with a
-# CAS(8,13) and point-group symmetry; Ne with the initiator approximation.
-MOLECULAR = {"H2O-RHF-cc-pVTZ": 60, "Ne-RHF-aug-cc-pVDZ": 80}
+# CAS(8,13) and point-group symmetry; Ne with the initiator approximation; H2
+# with a replica pair (two walker spaces, vary_shift_from = 'proje').
+MOLECULAR = {"H2O-RHF-cc-pVTZ": 60, "Ne-RHF-aug-cc-pVDZ": 80, "H2-sto-3g-replica": 2000}
 
 
 @pytest.mark.parametrize("name", sorted(MOLECULAR))
