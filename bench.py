@@ -78,11 +78,11 @@ class ProfilerRegion:
             self.lib.roctxProfilerResume(0)
 
 
-def molecular_params(seed, main_cap, spawn_cap, excit_gen=N.FQ_EXCIT_RENORM, mix=0.1, min_attempts=0):
+def molecular_params(seed, main_cap, spawn_cap, excit_gen=N.FQ_EXCIT_RENORM, mix=0.1, min_attempts=0, spaces=1):
     """i-FCIQMC engine options of the molecular workloads: real_64 populations,
     initiator threshold 3, spawn cutoff 0.01, production Philox stream."""
     p = N.FqParams()
-    p.nspaces = 1
+    p.nspaces = spaces
     p.initiator = 1
     p.initiator_pop = 3.0
     p.quadrature_initiator = 1
@@ -114,37 +114,49 @@ class Propagator:
     integral term -pull*damping^2/(tau*every)*ln(N/target) also pulls the
     population to the target."""
 
-    def __init__(self, lib, eng, target, damping=0.05, update_every=5, pull=0.0):
+    def __init__(self, lib, eng, target, damping=0.05, update_every=5, pull=0.0, spaces=1):
         self.lib, self.eng = lib, eng
         self.target, self.damping, self.every = target, damping, update_every
         self.pull = pull  # integral gain towards the target (preparation only)
+        self.S = spaces   # walker spaces (2 for a replica pair), each with its own shift
         self.shift = (C.c_double * 2)(0.0, 0.0)
         self.eold = (C.c_double * 2)(0.0, 0.0)
-        self.vary = False
-        self.n_old = None
+        self.vary_s = [False, False]
+        self.n_old = [None, None]
         self.count = 0
-        self.proj = self.d0 = 0.0
+        self.proj = [0.0, 0.0]
+        self.d0 = [0.0, 0.0]
         self.last = None
+
+    @property
+    def vary(self):
+        return all(self.vary_s[:self.S])
+
+    @vary.setter
+    def vary(self, value):
+        self.vary_s = [bool(value)] * 2
 
     def cycle(self, tau):
         out = N.FqCycleOut()
         N.check(self.lib.fq_cycle(self.eng, self.shift, self.eold, tau, C.byref(out)), "fq_cycle")
         self.count += 1
-        self.proj += out.proj_energy[0]
-        self.d0 += out.D0_population[0]
-        n = out.nparticles[0]
+        for s in range(self.S):
+            self.proj[s] += out.proj_energy[s]
+            self.d0[s] += out.D0_population[s]
         if self.count % self.every == 0:
-            if self.vary and self.n_old:
-                dt = tau * self.every
-                self.shift[0] = self.shift[0] - math.log(n / self.n_old) * self.damping / dt
-                if self.pull:
-                    self.shift[0] -= self.pull * self.damping ** 2 / dt * math.log(n / self.target)
-            if abs(self.d0) > 0:
-                self.eold[0] = self.proj / self.d0
-            if not self.vary and n > self.target:
-                self.vary = True
-            self.n_old = n
-            self.proj = self.d0 = 0.0
+            dt = tau * self.every
+            for s in range(self.S):
+                n = out.nparticles[s]
+                if self.vary_s[s] and self.n_old[s]:
+                    self.shift[s] = self.shift[s] - math.log(n / self.n_old[s]) * self.damping / dt
+                    if self.pull:
+                        self.shift[s] -= self.pull * self.damping ** 2 / dt * math.log(n / self.target)
+                if abs(self.d0[s]) > 0:
+                    self.eold[s] = self.proj[s] / self.d0[s]
+                if not self.vary_s[s] and n > self.target:
+                    self.vary_s[s] = True
+                self.n_old[s] = n
+                self.proj[s] = self.d0[s] = 0.0
         self.last = out
         return out
 
@@ -168,12 +180,12 @@ def phase_bytes(prof, W=1, S=1):
     }
 
 
-def download_list(lib, eng, W):
+def download_list(lib, eng, W, S=1):
     n = C.c_int64()
     lib.fq_nstates(eng, C.byref(n))
     nd = n.value
     st = np.zeros(nd * W, dtype=np.uint64)
-    po = np.zeros(nd, dtype=np.int64)
+    po = np.zeros(nd * S, dtype=np.int64)
     da = np.zeros(nd)
     got = C.c_int64()
     N.check(lib.fq_download(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)), po.ctypes.data_as(C.POINTER(C.c_int64)),
@@ -221,7 +233,8 @@ def cpu_baseline(lib, system, eng, p, shift, eold, tau, budget_s=10.0):
     import helpers
     orc = helpers.oracle()
     W = system.view.nwords
-    st, po, da = download_list(lib, eng, W)
+    S = p.nspaces
+    st, po, da = download_list(lib, eng, W, S)
     nd = len(da)
     walkers = np.abs(po).sum() / 2**31
 
@@ -229,7 +242,7 @@ def cpu_baseline(lib, system, eng, p, shift, eold, tau, budget_s=10.0):
         h = orc.orc_create(system.view, C.byref(p), seed)
         sl = slice(lo, hi)
         sst = np.ascontiguousarray(st.reshape(nd, W)[sl]).reshape(-1)
-        spo = np.ascontiguousarray(po[sl])
+        spo = np.ascontiguousarray(po.reshape(nd, S)[sl]).reshape(-1)
         sda = np.ascontiguousarray(da[sl])
         orc.orc_load(h, sst.ctypes.data_as(C.POINTER(C.c_uint64)), spo.ctypes.data_as(C.POINTER(C.c_int64)),
                      sda.ctypes.data_as(C.POINTER(C.c_double)), hi - lo)
@@ -362,7 +375,7 @@ def efficiency_learned(args):
         res[gen] = {"E_proj": system.H00 + rb["energy"], "sigma": rb["error"], "wall_s": wall, "train_s": t_train,
                     "efficiency": reblock.efficiency(rb["error"], t), "attempts_per_s": att / wall,
                     "reblock_level": rb["level"], "converged": rb["converged"],
-                    "walkers": last.nparticles[0], "determinants": int(last.nstates)}
+                    "walkers": sum(last.nparticles[k] for k in range(S)), "determinants": int(last.nstates)}
         log(f"[eff] {gen}: E_proj {res[gen]['E_proj']:.6f} +- {rb['error']:.6f}, {wall:.1f}s, "
             f"efficiency {res[gen]['efficiency']:.4g}")
     a, b = res["learned"], res["heat_bath"]
@@ -434,22 +447,32 @@ def efficiency_batched(args):
     n_old = np.zeros(K)
     outs = (N.FqCycleOut * K)()
 
-    def batch_cycle(c):
+    def batch_cycle(c, prep=False):
         N.check(lib.fq_cycle_batch(beng, shift.ctypes.data_as(C.POINTER(C.c_double)),
                                    eold.ctypes.data_as(C.POINTER(C.c_double)), taus.ctypes.data_as(C.POINTER(C.c_double)),
                                    outs), "fq_cycle_batch")
         if (c + 1) % 5 == 0:
             n = np.array([outs[k].nparticles[0] for k in range(K)])
             upd = vary & (n_old > 0)
-            shift[upd] -= np.log(n[upd] / n_old[upd]) * 0.05 / (tau * 5)
-            vary[:] |= n > targets
+            if prep:
+                # preparation: stronger damping plus an integral pull to the
+                # target, from a twentieth of it, so that strongly correlated
+                # rings (growth ~10% per cycle at zero shift) do not overshoot
+                shift[upd] -= (np.log(n[upd] / n_old[upd]) * 0.2 + 0.04 * np.log(n[upd] / targets[upd])) / (tau * 5)
+                vary[:] |= n > targets / 20
+            else:
+                shift[upd] -= np.log(n[upd] / n_old[upd]) * 0.05 / (tau * 5)
+                vary[:] |= n > targets
             n_old[:] = n
 
     c = 0
     t0 = time.perf_counter()
     while c < args.prep_cycles and not (vary.all() and c > 400):
-        batch_cycle(c)
+        batch_cycle(c, prep=True)
         c += 1
+        if c % 200 == 0:
+            log(f"[batch] prep cycle {c}: {sum(outs[k].nparticles[0] for k in range(K)):.4g} walkers, "
+                f"{int(vary.sum())}/{K} instances controlled, {time.perf_counter() - t0:.1f}s")
     log(f"[batch] prepared {K} instances in {c} cycles ({time.perf_counter() - t0:.1f}s): "
         f"{sum(outs[k].nparticles[0] for k in range(K)):.4g} walkers")
     # timed batched cycles
@@ -458,6 +481,8 @@ def efficiency_batched(args):
     for i in range(args.eff_cycles):
         batch_cycle(c + i)
         attempts += sum(outs[k].nattempts for k in range(K))
+        if (i + 1) % 200 == 0:
+            log(f"[batch] timed cycle {i + 1}: {time.perf_counter() - t1:.1f}s")
     t_batch = time.perf_counter() - t1
     # the same instances, one engine each, from the batched list
     st, po, da = download_list(lib, beng, 1)
@@ -483,6 +508,8 @@ def efficiency_batched(args):
             att_solo += o.nattempts
         t_solo += time.perf_counter() - t2
         lib.fq_destroy(eng)
+        if (k + 1) % 8 == 0:
+            log(f"[batch] one engine per instance: {k + 1}/{K} done, {t_solo:.1f}s")
     print(json.dumps({"metric": "E_proj efficiency 1/(sigma^2 t), batched vs one engine per instance",
                       "value": t_solo / t_batch, "unit": "ratio", "higher_is_better": True, "n_gpus": 1,
                       "data": "synthetic",
@@ -499,6 +526,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="mol26", choices=sorted(synthetic.MOLECULAR_CONFIGS))
     ap.add_argument("--target", type=float, default=1e7)
+    ap.add_argument("--replicas", type=int, default=1, choices=[1, 2], help="walker spaces (2: replica pair)")
     ap.add_argument("--equil-cycles", type=int, default=1200, help="max untimed preparation cycles")
     ap.add_argument("--hold-cycles", type=int, default=250, help="preparation cycles after the shift starts varying")
     ap.add_argument("--equil-tau", type=float, default=0.01)
@@ -540,12 +568,14 @@ def main():
     system = api.read_in(int_file=path, nel=nel, ms=0)
     log(f"[bench] {args.config}: nbasis={system.view.nbasis} nel={nel} H00={system.view.H00:.8f}")
     cap = int(max(4 * args.target, 1 << 20))
-    eng, p = make_engine(lib, system, args.target, 7, cap, cap)
+    S = args.replicas
+    W = system.view.nwords
+    eng, p = make_engine(lib, system, args.target, 7, cap, cap, spaces=S)
     rf = 1 << 31
     f0 = system.f0()
     region = ProfilerRegion()
     region.pause()
-    prop = Propagator(lib, eng, args.target, damping=args.prep_damping, pull=1.0)
+    prop = Propagator(lib, eng, args.target, damping=args.prep_damping, pull=1.0, spaces=S)
     tau = 1e-3
     if args.load_state:
         z = np.load(args.load_state)
@@ -556,7 +586,7 @@ def main():
         prop.vary = True
         log(f"[bench] loaded {len(da)} determinants from {args.load_state}")
     else:
-        pop0 = np.array([max(1, int(args.target * args.start_frac)) * rf], dtype=np.int64)
+        pop0 = np.full(S, max(1, int(args.target * args.start_frac)) * rf, dtype=np.int64)
         dat0 = np.zeros(1)
         N.check(lib.fq_load(eng, f0.ctypes.data_as(C.POINTER(C.c_uint64)), pop0.ctypes.data_as(C.POINTER(C.c_int64)),
                             dat0.ctypes.data_as(C.POINTER(C.c_double)), 1), "fq_load")
@@ -573,7 +603,7 @@ def main():
                 break
         log(f"[bench] prepared: N={prop.last.nparticles[0]:.4g} dets={prop.last.nstates} shift={prop.shift[0]:.4f}")
     if args.save_state:
-        st, po, da = download_list(lib, eng, system.view.nwords)
+        st, po, da = download_list(lib, eng, W, S)
         np.savez(args.save_state, states=st, pops=po, kii=da, shift=prop.shift[0], eold=prop.eold[0])
         log(f"[bench] saved {len(da)} determinants to {args.save_state}")
     prop.damping = 0.05  # the workload's shift damping from here on
@@ -597,10 +627,10 @@ def main():
     last = prop.last
     phases = ["det_pass", "spawn", "sort", "reduce", "merge"]
     avg_ms = {ph: float(np.mean([getattr(pr, ph + "_ms") for pr in profs])) for ph in phases}
-    avg_bytes = {ph: float(np.mean([phase_bytes(pr)[ph] for pr in profs])) for ph in phases}
+    avg_bytes = {ph: float(np.mean([phase_bytes(pr, W, S)[ph] for pr in profs])) for ph in phases}
     launches = sum(pr.spawn_launches for pr in profs)
     k_ms = sum(pr.spawn_kernel_ms for pr in profs) / launches
-    k_bytes = sum(spawn_kernel_bytes(pr) for pr in profs) / launches
+    k_bytes = sum(spawn_kernel_bytes(pr, W, S) for pr in profs) / launches
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     # the spawning kernel dominates when its time exceeds every other phase's total
     dominant = all(k_ms * launches / len(profs) >= avg_ms[ph] for ph in phases if ph != "spawn")
@@ -619,7 +649,8 @@ def main():
         except Exception as ex:  # the baseline is reported, never the target
             log(f"[bench] cpu baseline failed: {ex}")
     workload = (f"i-FCIQMC {args.config}: synthetic molecular integrals {norb} spatial orbitals, "
-                f"{nel} electrons, C1, real_64, initiator 3, tau 1e-3, renorm, target {args.target:g}")
+                f"{nel} electrons, C1, real_64, initiator 3, tau 1e-3, renorm, target {args.target:g}"
+                + (", replica pair" if S == 2 else ""))
     traffic = traffic_summary(workload)
     result = {
         "metric": "spawn attempts/s",
@@ -636,7 +667,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": workload,
-            "walkers": last.nparticles[0],
+            "walkers": sum(last.nparticles[k] for k in range(S)),
             "determinants": int(last.nstates),
             "attempts_per_step": attempts / args.steps,
             "parallelism": "1 GPU",

@@ -6,20 +6,24 @@
 //   spawning + death      production: k_det_pass (one thread per determinant:
 //                         projected energy, initiator flag, decide_nattempts,
 //                         stochastic death) then k_spawn (one thread per
-//                         spawning attempt, parents found by binary search of
-//                         the attempt prefix sum, slots by wavefront ballot +
-//                         one atomic per wave).
+//                         spawning attempt; parents found in a per-tile window
+//                         of the attempt prefix sum; results written to the
+//                         attempt's staging slot, then packed in attempt order
+//                         by a prefix-sum compaction: no atomics).
 //                         replay: k_replay_walk, one thread walking the list
 //                         in the reference's order and consuming the uploaded
 //                         dSFMT stream exactly as get_rand_close_open does.
-//   sort                  production: LDS-staged LSD radix sort of the spawned
-//                         bit strings (8-bit digits, last word most significant).
+//   sort                  production: LSD radix sort (8-bit digits, stable
+//                         per-round wavefront ranking) of the spawned list's
+//                         combinatorial ranks, which order determinants as
+//                         their bit strings do in fewer bits.
 //                         replay: the reference's quicksort (src/sort.f90:213-396)
 //                         on one thread, so equal keys keep its order.
 //   self-annihilation     k_reduce_runs: one thread per run of equal keys,
 //                         plain and initiator rules (src/spawn_data.F90:859-1101).
-//   merge with main list  k_merge_runs: binary search of each unique key in the
-//                         sorted main list (src/annihilation.f90:294-486).
+//   merge with main list  k_merge_runs: per block, the main-list window of its
+//                         runs' key range, then a search within that window
+//                         (src/annihilation.f90:294-486).
 //   rounding, compaction  k_round_main / k_round_new: stochastic rounding of small
 //                         real populations (src/annihilation.f90:537-675), draws
 //                         from Philox or from the replay stream at scan offsets.
@@ -27,6 +31,9 @@
 //                         of surviving main entries and new determinants into the
 //                         other buffer, new K_ii from the Slater--Condon diagonal
 //                         (src/annihilation.f90:677-901).
+// fq_cycle_batch runs the same kernels over many independent instances held in
+// one list; the heat-bath tables and the learned proposal are built here and
+// in mlp.hip.
 // Memory layout (HBM, structure of arrays): determinant word w of entry i at
 // det[w*cap + i]; population of space s at pop[s*cap + i]; K_ii at kii[i].
 #include <hip/hip_runtime.h>
@@ -227,8 +234,9 @@ struct fq_engine {
   uint64_t* st_det = nullptr;   // W*bcap
   int64_t* st_pop = nullptr;    // bcap
   int64_t* st_ent = nullptr;    // bcap
-  int64_t* st_flag = nullptr;   // bcap+1
-  int64_t* st_pos = nullptr;    // bcap+1
+  int64_t* st_att = nullptr;    // bcap: attempt index of each compacted success
+  int64_t* tile_cnt = nullptr;  // successes per spawning tile (bcap / kSpawnTile + 2)
+  int64_t* tile_off = nullptr;  // their exclusive prefix
   // spawned list
   int64_t scap = 0;
   uint64_t* sdet = nullptr;  // W*scap
@@ -441,24 +449,31 @@ __device__ __forceinline__ int64_t parent_of(const int64_t* __restrict__ att_off
 }
 
 constexpr int kSpawnTile = 4 * kBlock;  // attempts per block tile
+constexpr int kSpawnWin = 2048;          // parent-window entries staged in LDS
 
 // Production: one thread per spawning attempt.  Each block takes tiles of
 // kSpawnTile consecutive attempts of the batch [t_begin, t_end); the parents
 // of a tile's first and last attempts bound a short window of the attempt
 // prefix sum (n*S (det, space) entries, det-major) in which each thread
-// searches its own parent.  The result of attempt t goes to slot t - t_begin
-// of the staging arrays (coalesced, no atomics, attempt order); k_compact packs
-// the successful ones into the spawned list.
+// searches its own parent.  The successful attempts of a tile are ranked in
+// attempt order (wavefront ballots plus per-wave totals) and written
+// contiguously from the tile's first staging slot, with the tile's count in
+// tile_cnt; k_pack_tiles then copies the tiles, in order, into the spawned list.
 template <int G, bool B>
 __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t cap, int64_t t_begin, int64_t t_end,
                                                   const int64_t* __restrict__ att_off,
                                                   const uint64_t* __restrict__ det, const int64_t* __restrict__ pop,
                                                   double tau, uint64_t seed, uint32_t cyc, int64_t bcap,
                                                   uint64_t* __restrict__ st_det, int64_t* __restrict__ st_pop,
-                                                  int64_t* __restrict__ st_ent, int64_t* __restrict__ st_flag,
+                                                  int64_t* __restrict__ st_ent, int64_t* __restrict__ st_att,
+                                                  int64_t* __restrict__ tile_cnt,
                                                   const Sys* __restrict__ sys_arr, const int64_t* __restrict__ iatt,
                                                   const double* __restrict__ itau) {
   __shared__ int64_t win[2];
+  __shared__ int32_t wcount[kBlock / 64];
+  // att_off[lo + k] - t0 (clamped below at -1: every such entry starts at or
+  // before the tile's first attempt) when the window fits
+  __shared__ int32_t watt[kSpawnWin];
   const int64_t ntiles = (t_end - t_begin + kSpawnTile - 1) / kSpawnTile;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t t0 = t_begin + tile * kSpawnTile;
@@ -470,64 +485,113 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
     }
     __syncthreads();
     const int64_t lo = win[0], hi = win[1];
+    const bool staged = hi - lo < kSpawnWin;
+    if (staged)
+      for (int64_t k = threadIdx.x; k <= hi - lo; k += kBlock) {
+        const int64_t v = att_off[lo + k] - t0;
+        watt[k] = static_cast<int32_t>(v < -1 ? -1 : v);
+      }
+    __syncthreads();
+    int32_t run = 0;  // successes of this tile so far (all threads hold the same value)
     for (int k = 0; k < kSpawnTile / kBlock; ++k) {
       const int64_t t = t0 + k * kBlock + threadIdx.x;
-      if (t >= t1) break;
-      const int64_t e = parent_of(att_off, t, lo, hi);
-      const int sp = static_cast<int>(e % s.S);
-      const int64_t i = e / s.S;
-      uint64_t f[FQ_MAX_WORDS];
-      load_det(det, cap, s.W, i, f);
-      const int64_t parent = pop[sp * cap + i];
+      const bool valid = t < t1;
+      int64_t e = 0, nspawn = 0;
       uint64_t child[FQ_MAX_WORDS] = {0, 0};
-      int64_t nspawn;
-      if constexpr (B) {
-        // batched instances: the instance's tables, time step, key and attempt numbering
-        const int inst = static_cast<int>(f[0] >> kInstShift);
-        f[0] &= kOrbMask;
-        const int64_t trel = t - iatt[inst];
-        fqd::Philox rng(seed + static_cast<uint64_t>(inst), static_cast<uint32_t>(trel),
-                        static_cast<uint32_t>(trel >> 32), cyc ^ (fqd::kDrawAttempt << 24));
-        nspawn = fqd::spawn_attempt<G>(sys_arr[inst], rng, f, nullptr, parent, itau[inst], child);
-        if (nspawn != 0) child[0] |= static_cast<uint64_t>(inst) << kInstShift;
-      } else {
-        fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
-                        cyc ^ (fqd::kDrawAttempt << 24));
-        const float* wrow = (G == fqd::kGenLearned && s.mlp_on[i]) ? s.mlp_tab + s.mlp_row[i] * s.mlp_ld : nullptr;
-        nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child);
+      if (valid) {
+        if (staged) {  // last k in [0, hi - lo] with att_off[lo + k] <= t
+          const int32_t tr = static_cast<int32_t>(t - t0);
+          int a = 0, b = static_cast<int>(hi - lo);
+          while (a < b) {
+            const int mid = (a + b + 1) >> 1;
+            if (watt[mid] <= tr)
+              a = mid;
+            else
+              b = mid - 1;
+          }
+          e = lo + a;
+        } else {
+          e = parent_of(att_off, t, lo, hi);
+        }
+        const int sp = static_cast<int>(e % s.S);
+        const int64_t i = e / s.S;
+        uint64_t f[FQ_MAX_WORDS];
+        load_det(det, cap, s.W, i, f);
+        const int64_t parent = pop[sp * cap + i];
+        if constexpr (B) {
+          // batched instances: the instance's tables, time step, key and attempt numbering
+          const int inst = static_cast<int>(f[0] >> kInstShift);
+          f[0] &= kOrbMask;
+          const int64_t trel = t - iatt[inst];
+          fqd::Philox rng(seed + static_cast<uint64_t>(inst), static_cast<uint32_t>(trel),
+                          static_cast<uint32_t>(trel >> 32), cyc ^ (fqd::kDrawAttempt << 24));
+          nspawn = fqd::spawn_attempt<G>(sys_arr[inst], rng, f, nullptr, parent, itau[inst], child);
+          if (nspawn != 0) child[0] |= static_cast<uint64_t>(inst) << kInstShift;
+        } else {
+          fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
+                          cyc ^ (fqd::kDrawAttempt << 24));
+          const float* wrow = (G == fqd::kGenLearned && s.mlp_on[i]) ? s.mlp_tab + s.mlp_row[i] * s.mlp_ld : nullptr;
+          nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child);
+        }
+      }  // valid
+      // rank this round's successes in attempt order (wave ballot + per-wave totals)
+      const bool ok = valid && nspawn != 0;
+      const unsigned long long bal = __ballot(ok);
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      if (lane == 0) wcount[wave] = __popcll(bal);
+      __syncthreads();
+      int32_t pos = run + __popcll(bal & ((1ull << lane) - 1));
+      int32_t round_total = 0;
+      for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wave) pos += wcount[w];
+        round_total += wcount[w];
       }
-      const int64_t u = t - t_begin;
-      for (int w = 0; w < s.W; ++w) st_det[w * bcap + u] = child[w];
-      st_pop[u] = nspawn;
-      st_ent[u] = e;
-      st_flag[u] = nspawn != 0 ? 1 : 0;
+      if (ok) {  // compacted into the tile's region of the staging arrays
+        const int64_t u = (t0 - t_begin) + pos;
+        for (int w = 0; w < s.W; ++w) st_det[w * bcap + u] = child[w];
+        st_pop[u] = nspawn;
+        st_ent[u] = e;
+        st_att[u] = t;
+      }
+      run += round_total;
+      __syncthreads();
     }
+    if (threadIdx.x == 0) tile_cnt[tile] = run;
     __syncthreads();
   }
 }
 
-// Pack the successful attempts of a batch into the spawned list, after the
-// entries of earlier batches (st->nspawn), in attempt order.
-__global__ void k_compact(Sys s, int64_t nb, int64_t t_begin, const uint64_t* __restrict__ st_det,
-                          const int64_t* __restrict__ st_pop, const int64_t* __restrict__ st_ent,
-                          const int64_t* __restrict__ st_flag, const int64_t* __restrict__ pos, int64_t bcap,
-                          const int64_t* __restrict__ flags, int64_t scap, uint64_t* __restrict__ sdet,
-                          int64_t* __restrict__ spop, int64_t* __restrict__ sflag, uint64_t* __restrict__ sord,
-                          Stats* __restrict__ st) {
-  const int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (u >= nb || !st_flag[u]) return;
-  const int64_t slot = st->nspawn + pos[u];
-  if (slot >= scap) {
-    st->spawn_overflow = 1;
-    return;
+// Pack each tile's compacted successes into the spawned list after the entries
+// of earlier batches (st->nspawn), in attempt order; one block per tile.
+__global__ __launch_bounds__(kBlock) void k_pack_tiles(Sys s, int64_t ntiles, const int64_t* __restrict__ tile_cnt,
+                                                       const int64_t* __restrict__ tile_off,
+                                                       const uint64_t* __restrict__ st_det,
+                                                       const int64_t* __restrict__ st_pop,
+                                                       const int64_t* __restrict__ st_ent,
+                                                       const int64_t* __restrict__ st_att, int64_t bcap,
+                                                       int64_t t_begin, const int64_t* __restrict__ flags,
+                                                       int64_t scap, uint64_t* __restrict__ sdet,
+                                                       int64_t* __restrict__ spop, int64_t* __restrict__ sflag,
+                                                       uint64_t* __restrict__ sord, Stats* __restrict__ st) {
+  const int64_t tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  const int64_t cnt = tile_cnt[tile];
+  const int64_t base = st->nspawn + tile_off[tile];
+  for (int64_t j = threadIdx.x; j < cnt; j += kBlock) {
+    const int64_t u = tile * kSpawnTile + j;
+    const int64_t slot = base + j;
+    if (slot >= scap) {
+      st->spawn_overflow = 1;
+      continue;
+    }
+    const int64_t e = st_ent[u];
+    const int sp = static_cast<int>(e % s.S);
+    const int64_t i = e / s.S;
+    for (int w = 0; w < s.W; ++w) sdet[w * scap + slot] = st_det[w * bcap + u];
+    for (int q = 0; q < s.S; ++q) spop[q * scap + slot] = (q == sp) ? st_pop[u] : 0;
+    sflag[slot] = s.initiator ? flags[i] : 0;
+    sord[slot] = static_cast<uint64_t>(st_att[u]);
   }
-  const int64_t e = st_ent[u];
-  const int sp = static_cast<int>(e % s.S);
-  const int64_t i = e / s.S;
-  for (int w = 0; w < s.W; ++w) sdet[w * scap + slot] = st_det[w * bcap + u];
-  for (int q = 0; q < s.S; ++q) spop[q * scap + slot] = (q == sp) ? st_pop[u] : 0;
-  sflag[slot] = s.initiator ? flags[i] : 0;
-  sord[slot] = static_cast<uint64_t>(t_begin + u);
 }
 
 __global__ void k_add_spawned(const int64_t* __restrict__ pos, int64_t nb, Stats* __restrict__ st) {
@@ -1714,6 +1778,19 @@ __global__ void k_probe(Sys s, uint64_t f0, uint64_t f1, const float* __restrict
 }
 
 
+// Spawned entries of a batch of nbt attempts starting at t_begin: scan the
+// per-tile success counts, pack the tiles in order, advance the spawned count.
+int pack_spawned(fq_engine* e, int64_t nbt, int64_t t_begin) {
+  const int64_t nt = ceil_div(nbt, kSpawnTile);
+  int rc = scan_exclusive(e, e->tile_cnt, e->tile_off, nt);
+  if (rc != FQ_OK) return rc;
+  hipLaunchKernelGGL(k_pack_tiles, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0, e->stream, e->sys, nt,
+                     e->tile_cnt, e->tile_off, e->st_det, e->st_pop, e->st_ent, e->st_att, e->bcap, t_begin,
+                     e->flags, e->scap, e->sdet, e->spop, e->sflag, e->sord, e->d_stats);
+  hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, e->stream, e->tile_off, nt, e->d_stats);
+  return FQ_OK;
+}
+
 int alloc_engine(fq_engine* e) {
   const int64_t cap = e->cap, scap = e->scap;
   const int W = e->W, S = e->S;
@@ -1739,8 +1816,9 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->st_det, W * e->bcap);
   rc |= dalloc(&e->st_pop, e->bcap);
   rc |= dalloc(&e->st_ent, e->bcap);
-  rc |= dalloc(&e->st_flag, e->bcap + 1);
-  rc |= dalloc(&e->st_pos, e->bcap + 1);
+  rc |= dalloc(&e->st_att, e->bcap);
+  rc |= dalloc(&e->tile_cnt, e->bcap / kSpawnTile + 2);
+  rc |= dalloc(&e->tile_off, e->bcap / kSpawnTile + 2);
   rc |= dalloc(&e->sdet, W * scap);
   rc |= dalloc(&e->spop, S * scap);
   rc |= dalloc(&e->sflag, scap);
@@ -2028,14 +2106,10 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
                                              : k_spawn<fqd::kGenMolRenorm, false>;
       hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
                          e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
-                         e->st_flag, nullptr, nullptr, nullptr);
+                         e->st_att, e->tile_cnt, nullptr, nullptr, nullptr);
       if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
-      rc = scan_exclusive(e, e->st_flag, e->st_pos, nbt);
+      rc = pack_spawned(e, nbt, tb);
       if (rc != FQ_OK) return rc;
-      hipLaunchKernelGGL(k_compact, dim3(grid_for(nbt)), dim3(kBlock), 0, st, e->sys, nbt, tb, e->st_det, e->st_pop,
-                         e->st_ent, e->st_flag, e->st_pos, e->bcap, e->flags, e->scap, e->sdet, e->spop, e->sflag,
-                         e->sord, e->d_stats);
-      hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, st, e->st_pos, nbt, e->d_stats);
     }
     FQ_HIP(hipEventRecord(e->pev[3], st));
     rc = sync_stats(e);
@@ -2299,14 +2373,10 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
     auto kern = e->sys.system == FQ_SYS_HUB_K ? k_spawn<fqd::kGenHubK, true> : k_spawn<fqd::kGenMolRenorm, true>;
     hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
                        e->pop[c], 0.0, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
-                       e->st_flag, e->d_sys_arr, e->d_iatt, e->d_ipar);
+                       e->st_att, e->tile_cnt, e->d_sys_arr, e->d_iatt, e->d_ipar);
     if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
-    rc = scan_exclusive(e, e->st_flag, e->st_pos, nbt);
+    rc = pack_spawned(e, nbt, tb);
     if (rc != FQ_OK) return rc;
-    hipLaunchKernelGGL(k_compact, dim3(grid_for(nbt)), dim3(kBlock), 0, st, e->sys, nbt, tb, e->st_det, e->st_pop,
-                       e->st_ent, e->st_flag, e->st_pos, e->bcap, e->flags, e->scap, e->sdet, e->spop, e->sflag,
-                       e->sord, e->d_stats);
-    hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, st, e->st_pos, nbt, e->d_stats);
   }
   FQ_HIP(hipEventRecord(e->pev[3], st));
   rc = sync_stats(e);
@@ -2560,8 +2630,9 @@ void fq_destroy(fq_engine* e) {
   dfree(e->st_det);
   dfree(e->st_pop);
   dfree(e->st_ent);
-  dfree(e->st_flag);
-  dfree(e->st_pos);
+  dfree(e->st_att);
+  dfree(e->tile_cnt);
+  dfree(e->tile_off);
   dfree(e->sdet);
   dfree(e->spop);
   dfree(e->sflag);

@@ -29,6 +29,17 @@
 //   lib/dSFMT_F03_interface/dSFMT_interface.F90:387-409  RNG stream
 // The dSFMT stream generator itself is hande_amd/csrc/host/dsfmt.hpp, pinned
 // against the reference's C source in tests/golden.
+//
+// Production mode (orc_set_production) runs the same cycle with the GPU
+// engine's counter-based draws instead of one dSFMT stream: every random
+// decision reads its own Philox4x32-10 stream (Salmon, Moraes, Dror, Shaw,
+// SC'11; restated here and pinned by the Random123 known-answer vectors),
+// keyed by the seed and by (index, purpose, cycle):
+//   attempt count, death   (det i, space s)      purposes 2, 3
+//   spawning attempt       (global attempt t)    purpose 1 (det-major order)
+//   main-list rounding     (list position i)     purpose 4
+//   new-spawn rounding     (unique-key run r)    purpose 5
+// and equal spawned keys keep their spawning order (stable sort).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -67,7 +78,59 @@ struct Spawned {
   uint64_t f[FQ_MAX_WORDS];
   int64_t pop[FQ_MAX_SPACES];
   int64_t flag;
+  int64_t run;  // index of its unique key among all keys of the sorted list
 };
+
+// Philox4x32-10: counter (c0, c1, c2, block), key (k0, k1); ten rounds of two
+// 32x32->64 multiplications with the Weyl key schedule.
+void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = uint64_t{0xD2511F53u} * x0;
+    const uint64_t p1 = uint64_t{0xCD9E8D57u} * x2;
+    const uint32_t y0 = static_cast<uint32_t>(p1 >> 32) ^ x1 ^ k0;
+    const uint32_t y2 = static_cast<uint32_t>(p0 >> 32) ^ x3 ^ k1;
+    x1 = static_cast<uint32_t>(p1);
+    x3 = static_cast<uint32_t>(p0);
+    x0 = y0;
+    x2 = y2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = x0;
+  out[1] = x1;
+  out[2] = x2;
+  out[3] = x3;
+}
+
+// One keyed stream: blocks 0, 1, ... of the counter, two 32-bit words per
+// draw, each draw (w0 >> 6) * 2^26 + (w1 >> 6) times 2^-52.
+struct PhiloxStream {
+  uint32_t key[2] = {0, 0}, ctr[4] = {0, 0, 0, 0}, buf[4] = {0, 0, 0, 0};
+  int left = 0;
+  void reset(uint64_t k, uint32_t a, uint32_t b, uint32_t c) {
+    key[0] = static_cast<uint32_t>(k);
+    key[1] = static_cast<uint32_t>(k >> 32);
+    ctr[0] = a;
+    ctr[1] = b;
+    ctr[2] = c;
+    ctr[3] = 0;
+    left = 0;
+  }
+  double next() {
+    if (left < 2) {
+      philox4x32_10(ctr, key, buf);
+      ++ctr[3];
+      left = 4;
+    }
+    const uint32_t w0 = buf[4 - left], w1 = buf[5 - left];
+    left -= 2;
+    const uint64_t v = (static_cast<uint64_t>(w0 >> 6) << 26) | (w1 >> 6);
+    return static_cast<double>(v) * (1.0 / 4503599627370496.0);
+  }
+};
+enum : uint32_t { kDrawAttempt = 1, kDrawNattempts = 2, kDrawDeath = 3, kDrawRoundMain = 4, kDrawRoundSpawn = 5 };
 
 class Oracle {
  public:
@@ -154,6 +217,7 @@ class Oracle {
     spawned.clear();
     int64_t ndeath = 0;
     int64_t attempts = 0;
+    int64_t t = 0;  // production: global attempt index, det-major
     for (size_t idet = 0; idet < states.size(); ++idet) {
       Decoded dd = decode_full(states[idet].f);
       double real_pop[FQ_MAX_SPACES];
@@ -162,14 +226,19 @@ class Oracle {
       for (int s = 0; s < S; ++s) {
         Excit c = get_excitation(states[idet].f, f0.data());
         update_proj_energy(states[idet].f, dd.occ, real_pop[s], c, out->proj_energy[s], out->D0_population[s]);
+        const int64_t i64 = static_cast<int64_t>(idet);
+        const uint32_t hi = static_cast<uint32_t>(i64 >> 32) ^ (static_cast<uint32_t>(s) << 28);
+        draw_stream(static_cast<uint32_t>(i64), hi, kDrawNattempts);
         int nat = decide_nattempts(real_pop[s]);
         attempts += nat;
         const int64_t parent = pops[idet * S + s];
-        for (int ip = 0; ip < nat; ++ip) {
+        for (int ip = 0; ip < nat; ++ip, ++t) {
+          draw_stream(static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32), kDrawAttempt);
           Excit conn;
           int64_t nspawn = spawn(dd, states[idet].f, parent, tau, conn);
           if (nspawn != 0) create_spawned(states[idet].f, conn, nspawn, s, flag);
         }
+        draw_stream(static_cast<uint32_t>(i64), hi, kDrawDeath);
         death(tau, dat[idet], shift[s], eproj_old[s], pops[idet * S + s], nparticles[s], ndeath);
       }
     }
@@ -182,6 +251,7 @@ class Oracle {
     out->rng_consumed = static_cast<int64_t>(rng_.consumed() - rng0);
     out->spawn_overflow = spawn_error;
     out->main_overflow = main_error;
+    ++cyc_;
   }
 
   // Annihilate an externally supplied spawned list (entries in the given order).
@@ -224,7 +294,21 @@ class Oracle {
   int64_t nforced_ = 0, iforced_ = 0;
   double rand() {
     if (forced_) return iforced_ < nforced_ ? forced_[iforced_++] : (++iforced_, 0.5);
+    if (prod_) return ph_.next();
     return rng_.next();
+  }
+  // Production mode: counter-based Philox streams (see the file header).
+  bool prod_ = false;
+  uint64_t pseed_ = 0;
+  uint32_t cyc_ = 0;  // cycles run so far
+  PhiloxStream ph_;
+  void set_production(uint64_t seed) {
+    prod_ = true;
+    pseed_ = seed;
+    stable_order = true;
+  }
+  void draw_stream(uint32_t a, uint32_t b, uint32_t purpose) {
+    if (prod_) ph_.reset(pseed_, a, b, cyc_ ^ (purpose << 24));
   }
   bool btest(const uint64_t* f, int orb) const { return (f[(orb - 1) / 64] >> ((orb - 1) % 64)) & 1; }
   int ms_of(int orb) const { return (orb & 1) ? 1 : -1; }
@@ -1105,9 +1189,11 @@ class Oracle {
     int n = static_cast<int>(spawned.size());
     if (n == 0) return;
     std::vector<Spawned> out;
+    int64_t nrun = 0;
     int k = 0;
     while (k < n) {
       Spawned acc = spawned[k];
+      acc.run = nrun++;
       int m = k + 1;
       while (m < n && same(spawned[m].f, acc.f)) {
         for (int s = 0; s < S; ++s) acc.pop[s] += spawned[m].pop[s];
@@ -1126,9 +1212,11 @@ class Oracle {
     int n = static_cast<int>(spawned.size());
     if (n == 0) return;
     std::vector<Spawned> out;
+    int64_t nrun = 0;
     int k = 0;
     while (k < n) {
       Spawned acc = spawned[k];
+      acc.run = nrun++;
       int64_t init_pop[FQ_MAX_SPACES];
       int events[FQ_MAX_SPACES];
       for (int s = 0; s < S; ++s) {
@@ -1286,6 +1374,7 @@ class Oracle {
       if (par_.real_amplitudes) {
         int64_t old[FQ_MAX_SPACES];
         for (int s = 0; s < S; ++s) old[s] = pops[i * S + s];
+        draw_stream(static_cast<uint32_t>(i), static_cast<uint32_t>(static_cast<uint64_t>(i) >> 32), kDrawRoundMain);
         stoch_round(&pops[i * S]);
         add_np(&pops[i * S], old);
       }
@@ -1304,6 +1393,7 @@ class Oracle {
   void round_low_spawns() {
     std::vector<Spawned> keep;
     for (Spawned& sp : spawned) {
+      draw_stream(static_cast<uint32_t>(sp.run), static_cast<uint32_t>(sp.run >> 32), kDrawRoundSpawn);
       stoch_round(sp.pop);
       if (!allzero(sp.pop)) keep.push_back(sp);
     }
@@ -1462,6 +1552,12 @@ void* orc_create(const fq_system* s, const fq_params* p, int64_t seed) {
   return new Oracle(s, p, static_cast<uint32_t>(seed));
 }
 void orc_destroy(void* h) { delete static_cast<Oracle*>(h); }
+// Switch an oracle to production mode (Philox streams keyed by seed; cycle 0 next).
+void orc_set_production(void* h, uint64_t philox_seed) { static_cast<Oracle*>(h)->set_production(philox_seed); }
+
+// One Philox4x32-10 block (for the known-answer vectors).
+void orc_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox4x32_10(ctr, key, out); }
+
 int orc_load(void* h, const uint64_t* st, const int64_t* p, const double* d, int64_t n) {
   static_cast<Oracle*>(h)->load(st, p, d, n);
   return 0;

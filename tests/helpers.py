@@ -47,6 +47,8 @@ def oracle():
     lib.orc_rng_consumed.restype = C.c_uint64
     lib.orc_rng_consumed.argtypes = [vp]
     lib.orc_set_stable_order.argtypes = [vp, C.c_int]
+    lib.orc_set_production.argtypes = [vp, C.c_uint64]
+    lib.orc_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     lib.orc_fciqmc.argtypes = [P(N.FqSystem), P(N.FqhQmcIn), P(N.FqhReportRow), C.c_int, P(C.c_int)]
     lib.orc_hb_biased.argtypes = [vp]
     lib.orc_hb_table.restype = C.c_int64
@@ -191,6 +193,57 @@ def lockstep(system, st, ncycles, init_pop=None, shift=0.0, label=""):
                 print(f"  {label} cycle {cyc}: nstates={o1.nstates} N={o1.nparticles[0]}", flush=True)
         assert consumed == orc.orc_rng_consumed(h)
         return o1
+    finally:
+        lib.fq_destroy(eng)
+        orc.orc_destroy(h)
+
+
+def production_lockstep(system, p, ncycles, init_pop, shift=0.0, tau=0.01, label="", every=50):
+    """Run the engine in production mode (Philox) and the oracle in its
+    production mode with the same seed side by side from init_pop walkers on
+    the reference determinant.  Per cycle: spawn/death/attempt counts, list
+    length, the walker lists (determinants, populations, K_ii) and the
+    reference population must be identical; the projected energy and the
+    particle totals (summed in a different order on the GPU) agree to 1e-12
+    relative.  Returns the list of cycle outputs."""
+    lib = N.lib()
+    orc = oracle()
+    W, S = system.view.nwords, p.nspaces
+    eng = C.c_void_p()
+    N.check(lib.fq_create(system.view, C.byref(p), C.byref(eng)), "fq_create")
+    h = orc.orc_create(system.view, C.byref(p), 1)
+    orc.orc_set_production(h, p.philox_seed)
+    outs = []
+    try:
+        rf = 1 << p.pop_bit_shift
+        f0 = np.array([system.view.f0[w] for w in range(W)], dtype=np.uint64)
+        pop0 = np.full(S, int(round(init_pop)) * rf, dtype=np.int64)
+        dat0 = np.zeros(1)
+        args = (f0.ctypes.data_as(C.POINTER(C.c_uint64)), pop0.ctypes.data_as(C.POINTER(C.c_int64)),
+                dat0.ctypes.data_as(C.POINTER(C.c_double)), 1)
+        N.check(lib.fq_load(eng, *args), "fq_load")
+        orc.orc_load(h, *args)
+        sh = (C.c_double * 2)(shift, shift)
+        eold = (C.c_double * 2)(0.0, 0.0)
+        for cyc in range(ncycles):
+            o1, o2 = N.FqCycleOut(), N.FqCycleOut()
+            N.check(lib.fq_cycle(eng, sh, eold, tau, C.byref(o1)), "fq_cycle")
+            orc.orc_cycle(h, sh, eold, tau, C.byref(o2))
+            assert (o1.nspawn_events, o1.ndeath, o1.nattempts, o1.nstates) == \
+                   (o2.nspawn_events, o2.ndeath, o2.nattempts, o2.nstates), cyc
+            assert not o1.spawn_overflow and not o1.main_overflow
+            for s in range(S):
+                assert o1.D0_population[s] == o2.D0_population[s], cyc
+                np.testing.assert_allclose(o1.proj_energy[s], o2.proj_energy[s], rtol=1e-12, atol=1e-12)
+                np.testing.assert_allclose(o1.nparticles[s], o2.nparticles[s], rtol=1e-12)
+            a, b = download(lib, eng, W, S), odownload(orc, h, W, S)
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y), cyc
+            outs.append(o1)
+            if cyc % every == 0:
+                print(f"  {label} cycle {cyc}: nstates={o1.nstates} spawned={o1.nspawn_events} "
+                      f"N={o1.nparticles[0]:.6g}", flush=True)
+        return outs
     finally:
         lib.fq_destroy(eng)
         orc.orc_destroy(h)
