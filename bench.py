@@ -538,7 +538,9 @@ def main():
     ap.add_argument("--efficiency", choices=["learned", "batched"], default=None,
                     help="measure the E_proj statistical efficiency of the learned proposal (against heat-bath, "
                          "mol40) or of batched instances (against one engine per instance) instead")
-    ap.add_argument("--eff-cycles", type=int, default=1000)
+    ap.add_argument("--eff-cycles", type=int, default=None,
+                    help="timed cycles of an efficiency mode (default 3000 learned: 300 reports for the "
+                         "reblocking; 1000 batched)")
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--min-attempts", type=int, default=4)
     ap.add_argument("--instances", type=int, default=64)
@@ -547,6 +549,8 @@ def main():
                     help="only run a plain device-to-device copy of MIB MiB three times (calibrates the HBM "
                          "byte counters FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.eff_cycles is None:
+        args.eff_cycles = 3000 if args.efficiency == "learned" else 1000
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return

@@ -93,15 +93,18 @@ struct ReplayRng {
 };
 
 // Philox4x32-10 (Salmon et al., SC'11), counter = (c0, c1, c2, draw-block).
+// The four words of a block are held in named registers and consumed in two
+// halves (a private array indexed by a run-time position would live in
+// scratch memory).
 struct Philox {
   uint32_t k0, k1, c0, c1, c2;
   uint32_t block;
-  uint32_t out[4];
-  int have;
+  uint32_t o0, o1, o2, o3;
+  bool second;  // the block's second half is next
   bool failed;
   __device__ Philox(uint64_t key, uint32_t a, uint32_t b, uint32_t c)
       : k0(static_cast<uint32_t>(key)), k1(static_cast<uint32_t>(key >> 32)), c0(a), c1(b), c2(c), block(0),
-        have(0), failed(false) {}
+        o0(0), o1(0), o2(0), o3(0), second(false), failed(false) {}
   __device__ static void round(uint32_t* x, uint32_t ka, uint32_t kb) {
     const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
     uint32_t hi0 = __umulhi(M0, x[0]), lo0 = M0 * x[0];
@@ -121,19 +124,25 @@ struct Philox {
       ka += 0x9E3779B9u;
       kb += 0xBB67AE85u;
     }
-    out[0] = x[0];
-    out[1] = x[1];
-    out[2] = x[2];
-    out[3] = x[3];
-    have = 4;
+    o0 = x[0];
+    o1 = x[1];
+    o2 = x[2];
+    o3 = x[3];
   }
   // Uniform double in [0,1) on the 2^-52 grid (as dSFMT's close-open values),
   // from two 32-bit words.  On that grid r*n < n rounds below n for every
   // integer n < 2^52, so int(r*n) is always a valid index in [0, n).
   __device__ double next() {
-    if (have < 2) generate();
-    uint32_t a = out[4 - have], b = out[5 - have];
-    have -= 2;
+    uint32_t a, b;
+    if (second) {
+      a = o2;
+      b = o3;
+    } else {
+      generate();
+      a = o0;
+      b = o1;
+    }
+    second = !second;
     uint64_t v = (static_cast<uint64_t>(a >> 6) << 26) | (b >> 6);
     return static_cast<double>(v) * (1.0 / 4503599627370496.0);
   }
@@ -153,17 +162,30 @@ enum : uint32_t {
 // ---------------------------------------------------------------------------
 // Bit-string helpers
 // ---------------------------------------------------------------------------
+// Determinant words are addressed with compile-time indices only (selects
+// and loops unrolled to FQ_MAX_WORDS), so they stay in registers.
+static_assert(FQ_MAX_WORDS == 2, "bit helpers select between two words");
+#define FQ_FOR_WORDS(w, W) _Pragma("unroll") for (int w = 0; w < FQ_MAX_WORDS; ++w) if (w < (W))
 __device__ __forceinline__ bool btest(const uint64_t* f, int orb) {
   const int b = orb - 1;
-  return (f[b >> 6] >> (b & 63)) & 1;
+  const uint64_t x = (b >> 6) == 0 ? f[0] : f[1];
+  return (x >> (b & 63)) & 1;
 }
 __device__ __forceinline__ void bset(uint64_t* f, int orb) {
   const int b = orb - 1;
-  f[b >> 6] |= uint64_t{1} << (b & 63);
+  const uint64_t m = uint64_t{1} << (b & 63);
+  if ((b >> 6) == 0)
+    f[0] |= m;
+  else
+    f[1] |= m;
 }
 __device__ __forceinline__ void bclr(uint64_t* f, int orb) {
   const int b = orb - 1;
-  f[b >> 6] &= ~(uint64_t{1} << (b & 63));
+  const uint64_t m = ~(uint64_t{1} << (b & 63));
+  if ((b >> 6) == 0)
+    f[0] &= m;
+  else
+    f[1] &= m;
 }
 // Word w of excit_mask(:, orb): bits of all orbitals after orb
 // (src/excitations.F90:17-45).
@@ -180,7 +202,7 @@ __device__ __forceinline__ int kth_bit(uint64_t x, int k) {
 }
 // Orbital (1-based) of the k-th set bit of (words & pattern) across W words.
 __device__ __forceinline__ int select_orb(const uint64_t* x, int W, int k) {
-  for (int w = 0; w < W; ++w) {
+  FQ_FOR_WORDS(w, W) {
     const int c = __popcll(x[w]);
     if (k < c) return (w << 6) + kth_bit(x[w], k) + 1;
     k -= c;
@@ -209,13 +231,13 @@ __device__ inline Excit get_excitation(const uint64_t* f1, const uint64_t* f2, i
   e.from[0] = e.from[1] = e.to[0] = e.to[1] = 0;
   e.perm = false;
   int nx = 0;
-  for (int w = 0; w < W; ++w) nx += __popcll(f1[w] ^ f2[w]);
+  FQ_FOR_WORDS(w, W) nx += __popcll(f1[w] ^ f2[w]);
   if (nx == 0) return e;
   e.nexcit = nx / 2;
   if (e.nexcit > 2) return e;
   const int shift = nel - e.nexcit;
   int ie1 = 0, ie2 = 0, iel1 = 0, iel2 = 0, perm = 0;
-  for (int w = 0; w < W; ++w) {
+  FQ_FOR_WORDS(w, W) {
     const uint64_t a = f1[w], b = f2[w];
     if (a == b) {
       if ((((ie1 - ie2) % 2) + 2) % 2 == 1) {
@@ -252,13 +274,13 @@ __device__ inline Excit get_excitation(const uint64_t* f1, const uint64_t* f2, i
 
 __device__ inline bool perm1(const uint64_t* f, int W, int from, int to) {
   int perm = 0;
-  for (int w = 0; w < W; ++w) perm += __popcll(f[w] & (after_mask(from, w) ^ after_mask(to, w)));
+  FQ_FOR_WORDS(w, W) perm += __popcll(f[w] & (after_mask(from, w) ^ after_mask(to, w)));
   if (from > to) perm -= 1;
   return (perm % 2) == 1;
 }
 __device__ inline bool perm2(const uint64_t* f, int W, const Excit& e) {
   int perm = 0;
-  for (int w = 0; w < W; ++w) {
+  FQ_FOR_WORDS(w, W) {
     const uint64_t ia = after_mask(e.from[0], w) ^ after_mask(e.to[0], w);
     const uint64_t jb = after_mask(e.from[1], w) ^ after_mask(e.to[1], w);
     perm += __popcll((f[w] & ia) ^ (f[w] & jb));
@@ -296,7 +318,7 @@ __device__ __forceinline__ double eri_nz(const Sys& s, int i, int j, int a, int 
 }
 __device__ inline double sc1_excit(const Sys& s, const uint64_t* f, int i, int a, bool perm) {
   double h = h1g(s, i, a);
-  for (int w = 0; w < s.W; ++w) {
+  FQ_FOR_WORDS(w, s.W) {
     uint64_t x = f[w];
     while (x) {
       const int o = (w << 6) + __builtin_ctzll(x) + 1;
@@ -322,7 +344,7 @@ __device__ inline double sc2_excit(const Sys& s, int i, int j, int a, int b, boo
 __device__ inline double diagonal(const Sys& s, const uint64_t* f) {
   if (s.system == FQ_SYS_HUB_K) {
     double h = static_cast<double>(s.nalpha * s.nbeta) * s.coulomb_k;
-    for (int w = 0; w < s.W; ++w) {
+    FQ_FOR_WORDS(w, s.W) {
       uint64_t x = f[w];
       while (x) {
         const int o = (w << 6) + __builtin_ctzll(x) + 1;
@@ -335,7 +357,7 @@ __device__ inline double diagonal(const Sys& s, const uint64_t* f) {
   // Coulomb and exchange elements come from dense per-pair tables holding the
   // same values as the packed store, in the reference's summation order.
   double h = s.ecore;
-  for (int w = 0; w < s.W; ++w) {
+  FQ_FOR_WORDS(w, s.W) {
     uint64_t x = f[w];
     while (x) {
       const int i = (w << 6) + __builtin_ctzll(x) + 1;
@@ -343,7 +365,8 @@ __device__ inline double diagonal(const Sys& s, const uint64_t* f) {
       h = h + h1g(s, i, i);
       const int pi = ((i + 1) / 2 - 1) * s.norb;
       // pairs (i, j) with j after i
-      for (int w2 = w; w2 < s.W; ++w2) {
+      FQ_FOR_WORDS(w2, s.W) {
+        if (w2 < w) continue;
         uint64_t y = (w2 == w) ? x : f[w2];
         while (y) {
           const int j = (w2 << 6) + __builtin_ctzll(y) + 1;
@@ -467,10 +490,13 @@ struct SpawnResult {
 };
 
 __device__ __forceinline__ void excited_det(const uint64_t* f, int W, const Excit& c, uint64_t* out) {
-  for (int w = 0; w < W; ++w) out[w] = f[w];
-  for (int i = 0; i < c.nexcit; ++i) {
-    bclr(out, c.from[i]);
-    bset(out, c.to[i]);
+  FQ_FOR_WORDS(w, W) out[w] = f[w];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    if (i < c.nexcit) {
+      bclr(out, c.from[i]);
+      bset(out, c.to[i]);
+    }
   }
 }
 
@@ -480,7 +506,7 @@ template <class Rng>
 __device__ inline int64_t spawn_hub_k(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
                                       Excit& conn) {
   uint64_t occa[FQ_MAX_WORDS], occb[FQ_MAX_WORDS], unoa[FQ_MAX_WORDS];
-  for (int w = 0; w < s.W; ++w) {
+  FQ_FOR_WORDS(w, s.W) {
     occa[w] = f[w] & kAlphaBits;
     occb[w] = f[w] & kBetaBits;
     unoa[w] = ~f[w] & kAlphaBits & s.valid[w];
@@ -515,7 +541,7 @@ __device__ inline int64_t spawn_hub_k(const Sys& s, Rng& rng, const uint64_t* f,
     }
   } else {
     int forbidden = 0;
-    for (int w = 0; w < s.W; ++w) {
+    FQ_FOR_WORDS(w, s.W) {
       uint64_t x = unoa[w];
       while (x) {
         const int a = (w << 6) + __builtin_ctzll(x) + 1;
@@ -564,14 +590,16 @@ __device__ inline int64_t spawn_hub_k(const Sys& s, Rng& rng, const uint64_t* f,
 // Unoccupied counts per (spin, irrep), packed 8 bits per irrep (<= 8 irreps).
 struct SymUnocc {
   uint64_t pk[2];  // [0] beta, [1] alpha
-  __device__ int get(int ims, int sym) const { return static_cast<int>((pk[ims - 1] >> (8 * sym)) & 0xff); }
+  __device__ int get(int ims, int sym) const {
+    return static_cast<int>(((ims == 1 ? pk[0] : pk[1]) >> (8 * sym)) & 0xff);
+  }
 };
 __device__ inline SymUnocc sym_unocc(const Sys& s, const uint64_t* f) {
   SymUnocc u;
   u.pk[0] = u.pk[1] = 0;
   if (s.nsym_pg == 1) {  // one irrep: unoccupied counts per spin from popcounts
     int na = 0, nb = 0;
-    for (int w = 0; w < s.W; ++w) {
+    FQ_FOR_WORDS(w, s.W) {
       na += __popcll(f[w] & kAlphaBits);
       nb += __popcll(f[w] & kBetaBits);
     }
@@ -583,12 +611,16 @@ __device__ inline SymUnocc sym_unocc(const Sys& s, const uint64_t* f) {
     u.pk[0] |= static_cast<uint64_t>(s.nbss[g]) << (8 * g);
     u.pk[1] |= static_cast<uint64_t>(s.nbss[s.nsym_pg + g]) << (8 * g);
   }
-  for (int w = 0; w < s.W; ++w) {
+  FQ_FOR_WORDS(w, s.W) {
     uint64_t x = f[w];
     while (x) {
       const int o = (w << 6) + __builtin_ctzll(x) + 1;
       x &= x - 1;
-      u.pk[ims_of(o) - 1] -= uint64_t{1} << (8 * s.orb_sym[o - 1]);
+      const uint64_t d = uint64_t{1} << (8 * s.orb_sym[o - 1]);
+      if (ims_of(o) == 1)
+        u.pk[0] -= d;
+      else
+        u.pk[1] -= d;
     }
   }
   return u;
@@ -601,13 +633,13 @@ __device__ __forceinline__ int nbss(const Sys& s, int ims, int sym) { return s.n
 // gen_single_excit_mol / choose_ia_mol (src/excit_gen_mol.f90:384-448,521-608):
 // renormalised single excitation; sets h and pgen (pgen includes psingle).
 template <class Rng>
-__device__ inline bool single_mol(const Sys& s, Rng& rng, const uint64_t* f, const SymUnocc& u, Excit& conn,
+__device__ inline bool single_mol(const Sys& s, Rng& rng, const uint64_t* f, const SymUnocc u, Excit& conn,
                                   double& h, double& pgen) {
   const int nel = s.nel;
   conn.nexcit = 1;
   bool allowed = false;
   int ni = nel;
-  for (int w = 0; w < s.W; ++w) {
+  FQ_FOR_WORDS(w, s.W) {
     uint64_t x = f[w];
     while (x) {
       const int o = (w << 6) + __builtin_ctzll(x) + 1;
@@ -790,36 +822,67 @@ __device__ __forceinline__ int64_t hb4(const Sys& s, int i, int j, int a, int b)
   return hb3(s, i, j, a) * s.nbasis + (b - 1);
 }
 
-// select_weighted_value (lib/local/alias.f90:64-99): the alias tables of
-// generate_alias_tables built over n <= kHbMaxNel weights, then one draw r.
-__device__ inline int alias_pick(int n, const double* w, double tot, double r) {
-  double u[kHbMaxNel];
-  int8_t k[kHbMaxNel], under[kHbMaxNel], over[kHbMaxNel];
-  int nunder = 0, nover = 0;
+// Per-thread storage for the on-the-fly weights of a heat-bath attempt: entry
+// k at w[k * stride] (a column of LDS in the production kernel).
+struct HbBuf {
+  double* w = nullptr;
+  int stride = 1;
+  __host__ __device__ double& operator[](int k) const { return w[k * stride]; }
+};
+
+// select_weighted_value (lib/local/alias.f90:64-99): generate_alias_tables
+// over the n <= 32 weights wt[0..n), then one draw r.  The construction is
+// replayed without its tables: the "over" stack only ever loses its top (the
+// highest remaining index whose scaled weight exceeds 1), and an index moved
+// from it to the "under" stack is popped again by the very next step, so two
+// index masks, that one pending index and the running value of the current
+// top reproduce every step (same operations, same order).  The drawn column
+// kk keeps itself as alias unless it is popped from "under", when its alias
+// is the top of "over" at that step and its threshold its value then.
+__host__ __device__ inline int alias_pick(int n, const HbBuf& wt, double tot, double r) {
   const double scale = static_cast<double>(n) / tot;
-  for (int i = 1; i <= n; ++i) {
-    u[i - 1] = w[i - 1] * scale;
-    if (u[i - 1] <= 1.0)
-      under[nunder++] = static_cast<int8_t>(i);
-    else
-      over[nover++] = static_cast<int8_t>(i);
-    k[i - 1] = static_cast<int8_t>(i);
-  }
-  while (nover > 0 && nunder > 0) {
-    const int ov = over[nover - 1], un = under[nunder - 1];
-    k[un - 1] = static_cast<int8_t>(ov);
-    nunder = nunder - 1;
-    u[ov - 1] = u[ov - 1] - (1.0 - u[un - 1]);
-    if (u[ov - 1] < 1.0) {
-      under[nunder++] = over[nover - 1];
-      nover = nover - 1;
-    }
-  }
   double x = r * n;
   int kk = static_cast<int>(floor(x));
   x = x - kk;
   kk = kk + 1;
-  return x < u[kk - 1] ? kk : k[kk - 1];
+  uint32_t under = 0, over = 0;
+  for (int i = 1; i <= n; ++i) {
+    if (wt[i - 1] * scale <= 1.0)
+      under |= 1u << (i - 1);
+    else
+      over |= 1u << (i - 1);
+  }
+  int ov = 0, pend = 0;
+  double u_ov = 0.0, u_pend = 0.0;
+  if (over) {
+    ov = 32 - __builtin_clz(over);
+    u_ov = wt[ov - 1] * scale;
+  }
+  while (over && (pend || under)) {
+    int un;
+    double u_un;
+    if (pend) {
+      un = pend;
+      u_un = u_pend;
+      pend = 0;
+    } else {
+      un = 32 - __builtin_clz(under);
+      under &= ~(1u << (un - 1));
+      u_un = wt[un - 1] * scale;
+    }
+    if (un == kk) return x < u_un ? kk : ov;
+    u_ov = u_ov - (1.0 - u_un);
+    if (u_ov < 1.0) {
+      pend = ov;
+      u_pend = u_ov;
+      over &= ~(1u << (ov - 1));
+      if (over) {
+        ov = 32 - __builtin_clz(over);
+        u_ov = wt[ov - 1] * scale;
+      }
+    }
+  }
+  return kk;
 }
 // select_weighted_value_precalc (lib/local/alias.f90:13-62) on a stored table.
 __device__ __forceinline__ int alias_draw(int n, const double* u, const int* k, double r) {
@@ -839,43 +902,46 @@ __device__ inline double hb_psingle(const Sys& s, const uint64_t* f, int i, int 
 }
 
 // gen_excit_mol_heat_bath: the excitation, its matrix element h and pgen.
+// The nel on-the-fly weights (first P(i), then P(j|i)) live in wb; occupied
+// orbitals are visited in ascending order straight from the bit string.
 template <class Rng>
 __device__ inline bool gen_heat_bath(const Sys& s, Rng& rng, const uint64_t* f, Excit& conn, double& h,
-                                     double& pgen) {
+                                     double& pgen, const HbBuf& wb) {
   const int nel = s.nel;
-  int occ[kHbMaxNel];
-  {
-    int k = 0;
-    for (int w = 0; w < s.W; ++w) {
-      uint64_t x = f[w];
-      while (x) {
-        occ[k++] = (w << 6) + __builtin_ctzll(x) + 1;
-        x &= x - 1;
-      }
-    }
-  }
   conn.nexcit = 0;
   conn.from[0] = conn.from[1] = conn.to[0] = conn.to[1] = 0;
   conn.perm = false;
   // find_i_d_weights + select_ij_heat_bath
-  double iw[kHbMaxNel], ijw[kHbMaxNel];
   double iw_tot = 0.0;
-  for (int k = 0; k < nel; ++k) {
-    iw[k] = s.hb_i[occ[k] - 1];
-    iw_tot = iw_tot + iw[k];
+  {
+    int k = 0;
+    FQ_FOR_WORDS(w, s.W) {
+      for (uint64_t x = f[w]; x; x &= x - 1) {
+        const double v = s.hb_i[(w << 6) + __builtin_ctzll(x)];
+        wb[k++] = v;
+        iw_tot = iw_tot + v;
+      }
+    }
   }
-  const int i_ind = alias_pick(nel, iw, iw_tot, rng.next());
-  const int i = occ[i_ind - 1];
+  const int i_ind = alias_pick(nel, wb, iw_tot, rng.next());
+  const int i = select_orb(f, s.W, i_ind - 1);
+  const double iw_i = wb[i_ind - 1];
   double ij_tot = 0.0;
-  for (int k = 0; k < nel; ++k) {
-    ijw[k] = s.hb_ij[hb2(s, i, occ[k])];
-    ij_tot = ij_tot + ijw[k];
+  {
+    int k = 0;
+    FQ_FOR_WORDS(w, s.W) {
+      for (uint64_t x = f[w]; x; x &= x - 1) {
+        const double v = s.hb_ij[hb2(s, i, (w << 6) + __builtin_ctzll(x) + 1)];
+        wb[k++] = v;
+        ij_tot = ij_tot + v;
+      }
+    }
   }
   bool allowed = false;
   int j = 0, j_ind = 0;
   if (ij_tot > 0.0) {
-    j_ind = alias_pick(nel, ijw, ij_tot, rng.next());
-    j = occ[j_ind - 1];
+    j_ind = alias_pick(nel, wb, ij_tot, rng.next());
+    j = select_orb(f, s.W, j_ind - 1);
     allowed = true;
   }
   allowed = allowed && fabs(s.hb_ija_tot[hb2(s, i, j)]) > 0.0;
@@ -911,9 +977,12 @@ __device__ inline bool gen_heat_bath(const Sys& s, Rng& rng, const uint64_t* f, 
                                rng.next());
       if (!btest(f, b)) {
         double ji_tot = 0.0;
-        for (int k = 0; k < nel; ++k) ji_tot = ji_tot + s.hb_ij[hb2(s, j, occ[k])];
-        const double pi = iw[i_ind - 1] / iw_tot, pj = ijw[j_ind - 1] / ij_tot;
-        const double pj2 = iw[j_ind - 1] / iw_tot, pi2 = s.hb_ij[hb2(s, j, i)] / ji_tot;
+        FQ_FOR_WORDS(w, s.W) {
+          for (uint64_t x = f[w]; x; x &= x - 1) ji_tot = ji_tot + s.hb_ij[hb2(s, j, (w << 6) + __builtin_ctzll(x) + 1)];
+        }
+        // P(i) and P(j|i) of the drawn path; P(j) from the i weights (hb_i)
+        const double pi = iw_i / iw_tot, pj = wb[j_ind - 1] / ij_tot;
+        const double pj2 = s.hb_i[j - 1] / iw_tot, pi2 = s.hb_ij[hb2(s, j, i)] / ji_tot;
         const double pgen_ija = (pi * pj) * (s.hb_ija[hb3(s, i, j, a)] / s.hb_ija_tot[hb2(s, i, j)]) *
                                 (1.0 - psingle) * (s.hb_ijab[hb4(s, i, j, a, b)] / s.hb_ijab_tot[hb3(s, i, j, a)]);
         double ps = hb_psingle(s, f, i, j, b);
@@ -939,15 +1008,18 @@ __device__ inline bool gen_heat_bath(const Sys& s, Rng& rng, const uint64_t* f, 
     } else {
       h = sc1_excit(s, f, i, a, conn.perm);
       pgen = 0.0;
-      for (int q = 0; q < nel; ++q) {
-        const int oq = occ[q];
-        if (i != oq && a != oq) {
-          const double t = s.hb_ijab_tot[hb3(s, i, oq, a)];
-          const double psq = hm_ia < t ? hm_ia / (t + hm_ia) : 0.5;
-          pgen = pgen + (psq * (ijw[q] / ij_tot) * (s.hb_ija[hb3(s, i, oq, a)] / s.hb_ija_tot[hb2(s, i, oq)]));
+      int q = 0;  // wb still holds P(j|i) weights, in occupied order
+      FQ_FOR_WORDS(w, s.W) {
+        for (uint64_t x = f[w]; x; x &= x - 1, ++q) {
+          const int oq = (w << 6) + __builtin_ctzll(x) + 1;
+          if (i != oq && a != oq) {
+            const double t = s.hb_ijab_tot[hb3(s, i, oq, a)];
+            const double psq = hm_ia < t ? hm_ia / (t + hm_ia) : 0.5;
+            pgen = pgen + (psq * (wb[q] / ij_tot) * (s.hb_ija[hb3(s, i, oq, a)] / s.hb_ija_tot[hb2(s, i, oq)]));
+          }
         }
       }
-      pgen = pgen * (iw[i_ind - 1] / iw_tot);
+      pgen = pgen * (iw_i / iw_tot);
     }
   }
   if (!allowed) {
@@ -961,9 +1033,9 @@ __device__ inline bool gen_heat_bath(const Sys& s, Rng& rng, const uint64_t* f, 
 // spawn_standard with the heat-bath generator (src/spawning.F90:34-124).
 template <class Rng>
 __device__ inline int64_t spawn_heat_bath(const Sys& s, Rng& rng, const uint64_t* f, int64_t parent, double tau,
-                                          Excit& conn) {
+                                          Excit& conn, const HbBuf& wb) {
   double h = 0.0, pgen = 1.0;
-  gen_heat_bath(s, rng, f, conn, h, pgen);
+  gen_heat_bath(s, rng, f, conn, h, pgen, wb);
   return attempt_to_spawn(s, rng, tau, h, pgen, parent);
 }
 
@@ -1046,10 +1118,10 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
     const float* ew = wrow;
     const float* fw = wrow ? wrow + nb : nullptr;
     uint64_t un[FQ_MAX_WORDS];
-    for (int w = 0; w < s.W; ++w) un[w] = ~f[w] & s.valid[w];
+    FQ_FOR_WORDS(w, s.W) un[w] = ~f[w] & s.valid[w];
     double zh = 0.0, zp = 0.0;
     if (wrow) {
-      for (int w = 0; w < s.W; ++w) {
+      FQ_FOR_WORDS(w, s.W) {
         for (uint64_t x = f[w]; x; x &= x - 1) zh = zh + static_cast<double>(ew[(w << 6) + __builtin_ctzll(x)]);
         for (uint64_t x = un[w]; x; x &= x - 1) zp = zp + static_cast<double>(fw[(w << 6) + __builtin_ctzll(x)]);
       }
@@ -1077,7 +1149,7 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
     } else {
       // network component: holes by weight (i, then j != i), a by weight, partner by weight
       double x = rng.next() * zh, cum = 0.0;
-      for (int w = 0; w < s.W && !i; ++w)
+      FQ_FOR_WORDS(w, s.W) if (!i)
         for (uint64_t y = f[w]; y; y &= y - 1) {
           const int o = (w << 6) + __builtin_ctzll(y) + 1;
           cum = cum + static_cast<double>(ew[o - 1]);
@@ -1090,7 +1162,7 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
       x = rng.next() * (zh - static_cast<double>(ew[i - 1]));
       cum = 0.0;
       int lastj = 0;
-      for (int w = 0; w < s.W && !j; ++w)
+      FQ_FOR_WORDS(w, s.W) if (!j)
         for (uint64_t y = f[w]; y; y &= y - 1) {
           const int o = (w << 6) + __builtin_ctzll(y) + 1;
           if (o == i) continue;
@@ -1105,7 +1177,7 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
       x = rng.next() * zp;
       cum = 0.0;
       int lasta = 0;
-      for (int w = 0; w < s.W && !a; ++w)
+      FQ_FOR_WORDS(w, s.W) if (!a)
         for (uint64_t y = un[w]; y; y &= y - 1) {
           const int o = (w << 6) + __builtin_ctzll(y) + 1;
           lasta = o;
@@ -1165,7 +1237,10 @@ __device__ inline int64_t spawn_learned(const Sys& s, Rng& rng, const uint64_t* 
 template <class Rng>
 __device__ inline bool generate_any(const Sys& s, Rng& rng, const uint64_t* f, const float* wrow, Excit& conn,
                                     double& h, double& pgen) {
-  if (s.excit_gen == FQ_EXCIT_HEAT_BATH) return gen_heat_bath(s, rng, f, conn, h, pgen);
+  if (s.excit_gen == FQ_EXCIT_HEAT_BATH) {
+    double wloc[kHbMaxNel];
+    return gen_heat_bath(s, rng, f, conn, h, pgen, HbBuf{wloc, 1});
+  }
   if (s.excit_gen == FQ_EXCIT_LEARNED) return gen_learned(s, rng, f, wrow, conn, h, pgen);
   return gen_mol(s, rng, f, conn, h, pgen);
 }
@@ -1180,21 +1255,29 @@ __host__ __device__ inline int gen_of(int system, int excit_gen) {
 }
 
 // One spawning attempt from determinant f; wrow is the determinant's row of
-// the learned proposal's weights (read only by that generator).
+// the learned proposal's weights (read only by that generator); hb is the
+// heat-bath generator's weight storage (a local array when not given).
 template <int G = kGenAny, class Rng>
 __device__ inline int64_t spawn_attempt(const Sys& s, Rng& rng, const uint64_t* f, const float* wrow,
-                                        int64_t parent, double tau, uint64_t* child) {
+                                        int64_t parent, double tau, uint64_t* child, HbBuf hb = HbBuf{}) {
   Excit conn;
   const int g = G == kGenAny ? gen_of(s.system, s.excit_gen) : G;
   int64_t n;
-  if ((G == kGenAny || G == kGenHubK) && g == kGenHubK)
+  if ((G == kGenAny || G == kGenHubK) && g == kGenHubK) {
     n = spawn_hub_k(s, rng, f, parent, tau, conn);
-  else if ((G == kGenAny || G == kGenHeatBath) && g == kGenHeatBath)
-    n = spawn_heat_bath(s, rng, f, parent, tau, conn);
-  else if ((G == kGenAny || G == kGenLearned) && g == kGenLearned)
+  } else if ((G == kGenAny || G == kGenHeatBath) && g == kGenHeatBath) {
+    if constexpr (G == kGenAny) {
+      double wloc[kHbMaxNel];
+      n = spawn_heat_bath(s, rng, f, parent, tau, conn, HbBuf{wloc, 1});
+    } else {
+      n = spawn_heat_bath(s, rng, f, parent, tau, conn, hb);
+    }
+  }
+  else if ((G == kGenAny || G == kGenLearned) && g == kGenLearned) {
     n = spawn_learned(s, rng, f, wrow, parent, tau, conn);
-  else
+  } else {
     n = spawn_mol(s, rng, f, parent, tau, conn);
+  }
   if (n != 0) excited_det(f, s.W, conn, child);
   return n;
 }

@@ -344,8 +344,8 @@ int scan_exclusive(fq_engine* e, const int64_t* in, int64_t* out, int64_t n) {
 // Kernels: spawning and death
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void load_det(const uint64_t* det, int64_t cap, int W, int64_t i, uint64_t* f) {
-  for (int w = 0; w < W; ++w) f[w] = det[w * cap + i];
-  for (int w = W; w < FQ_MAX_WORDS; ++w) f[w] = 0;
+#pragma unroll
+  for (int w = 0; w < FQ_MAX_WORDS; ++w) f[w] = w < W ? det[w * cap + i] : 0;
 }
 
 // Production: per determinant (loop body of src/fciqmc.f90:315-374 minus the
@@ -474,6 +474,12 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
   // att_off[lo + k] - t0 (clamped below at -1: every such entry starts at or
   // before the tile's first attempt) when the window fits
   __shared__ int32_t watt[kSpawnWin];
+  // heat-bath: each thread's nel on-the-fly weights in a column of dynamic LDS
+  fqd::HbBuf hb;
+  if constexpr (G == fqd::kGenHeatBath) {
+    extern __shared__ double hb_lds[];
+    hb = fqd::HbBuf{hb_lds + threadIdx.x, kBlock};
+  }
   const int64_t ntiles = (t_end - t_begin + kSpawnTile - 1) / kSpawnTile;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t t0 = t_begin + tile * kSpawnTile;
@@ -531,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
           fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
                           cyc ^ (fqd::kDrawAttempt << 24));
           const float* wrow = (G == fqd::kGenLearned && s.mlp_on[i]) ? s.mlp_tab + s.mlp_row[i] * s.mlp_ld : nullptr;
-          nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child);
+          nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child, hb);
         }
       }  // valid
       // rank this round's successes in attempt order (wave ballot + per-wave totals)
@@ -548,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
       }
       if (ok) {  // compacted into the tile's region of the staging arrays
         const int64_t u = (t0 - t_begin) + pos;
-        for (int w = 0; w < s.W; ++w) st_det[w * bcap + u] = child[w];
+        FQ_FOR_WORDS(w, s.W) st_det[w * bcap + u] = child[w];
         st_pop[u] = nspawn;
         st_ent[u] = e;
         st_att[u] = t;
@@ -774,7 +780,7 @@ __global__ void k_replay_walk(Sys s, int64_t n, int64_t cap, const uint64_t* __r
             overflow = true;
           } else {
             const int64_t slot = nspawn_total++;
-            for (int w = 0; w < s.W; ++w) sdet[w * scap + slot] = child[w];
+            FQ_FOR_WORDS(w, s.W) sdet[w * scap + slot] = child[w];
             for (int q = 0; q < s.S; ++q) spop[q * scap + slot] = (q == sp) ? ns : 0;
             sflag[slot] = s.initiator ? flag : 0;
             sord[slot] = static_cast<uint64_t>(slot);
@@ -849,9 +855,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __rest
     uint32_t v = 0;
     int digit = 0;
     if (valid) {
-      for (int w = 0; w < W; ++w) kv[w] = kin[w * stride + i];
+      FQ_FOR_WORDS(w, W) kv[w] = kin[w * stride + i];
       v = vin[i];
-      digit = static_cast<int>((kv[w_digit] >> shift) & 0xff);
+      digit = static_cast<int>(((w_digit == 0 ? kv[0] : kv[1]) >> shift) & 0xff);
     }
     unsigned long long peers = __ballot(valid);
     for (int b = 0; b < 8; ++b) {
@@ -1086,10 +1092,12 @@ __device__ __forceinline__ int64_t main_lower_bound(const uint64_t* __restrict__
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
     int c = 0;
-    for (int w = W - 1; w >= 0 && c == 0; --w) {
-      const uint64_t x = det[w * cap + mid];
-      c = (x < k[w]) ? -1 : (x > k[w] ? 1 : 0);
-    }
+#pragma unroll
+    for (int w = FQ_MAX_WORDS - 1; w >= 0; --w)
+      if (w < W && c == 0) {
+        const uint64_t x = det[w * cap + mid];
+        c = (x < k[w]) ? -1 : (x > k[w] ? 1 : 0);
+      }
     if (c < 0)
       lo = mid + 1;
     else
@@ -1113,7 +1121,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_runs(Sys s, int64_t nruns, int
   if (threadIdx.x < 2) {
     const int64_t rr = threadIdx.x == 0 ? r0 : r1;
     uint64_t k[FQ_MAX_WORDS] = {0, 0};
-    for (int w = 0; w < s.W; ++w) k[w] = rdet[w * scap + rr];
+    FQ_FOR_WORDS(w, s.W) k[w] = rdet[w * scap + rr];
     win[threadIdx.x] = main_lower_bound(det, cap, s.W, k, 0, n);
   }
   __syncthreads();
@@ -1124,14 +1132,14 @@ __global__ __launch_bounds__(kBlock) void k_merge_runs(Sys s, int64_t nruns, int
     return;
   }
   uint64_t k[FQ_MAX_WORDS] = {0, 0};
-  for (int w = 0; w < s.W; ++w) k[w] = rdet[w * scap + r];
+  FQ_FOR_WORDS(w, s.W) k[w] = rdet[w * scap + r];
   // lower bound in [win0, min(win1 + 1, n)): every key of the block lies in
   // [key(r0), key(r1)], so its bound lies between theirs
   const int64_t lo = main_lower_bound(det, cap, s.W, k, win[0], win[1] < n ? win[1] + 1 : n);
   bool hit = false;
   if (lo < n) {
     hit = true;
-    for (int w = 0; w < s.W; ++w) hit = hit && det[w * cap + lo] == k[w];
+    FQ_FOR_WORDS(w, s.W) hit = hit && det[w * cap + lo] == k[w];
   }
   rpos[r] = lo;
   const int64_t fl = rflag[r];
@@ -1285,7 +1293,7 @@ __global__ void k_scatter_new(Sys s, int64_t nruns, int64_t cap, int64_t scap, c
   if (r >= nruns || !nsurv[r] || st->main_overflow) return;
   const int64_t o = nrank[r] + mrank[rpos[r]];
   uint64_t f[FQ_MAX_WORDS] = {0, 0};
-  for (int w = 0; w < s.W; ++w) {
+  FQ_FOR_WORDS(w, s.W) {
     f[w] = rdet[w * scap + r];
     det_out[w * cap + o] = f[w];
   }
@@ -1778,6 +1786,18 @@ __global__ void k_probe(Sys s, uint64_t f0, uint64_t f1, const float* __restrict
 }
 
 
+// Dynamic LDS of the heat-bath spawning kernel: nel weights per thread.
+size_t hb_lds_bytes(int nel) { return static_cast<size_t>(nel) * kBlock * sizeof(double); }
+// Whether that LDS, with the kernel's static LDS, fits a workgroup here.
+bool hb_lds_fits(int device, int nel) {
+  hipFuncAttributes fa;
+  int maxlds = 0;
+  if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_spawn<fqd::kGenHeatBath, false>)) != hipSuccess ||
+      hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+    return false;
+  return fa.sharedSizeBytes + hb_lds_bytes(nel) <= static_cast<size_t>(maxlds);
+}
+
 // Spawned entries of a batch of nbt attempts starting at t_begin: scan the
 // per-tile success counts, pack the tiles in order, advance the spawned count.
 int pack_spawned(fq_engine* e, int64_t nbt, int64_t t_begin) {
@@ -1883,6 +1903,10 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   if (hipSetDevice(e->device) != hipSuccess) {
     delete e;
     return FQ_ERR_NO_DEVICE;
+  }
+  if (params->excit_gen == FQ_EXCIT_HEAT_BATH && !hb_lds_fits(e->device, sys->nel)) {
+    delete e;
+    return FQ_ERR_ARG;
   }
   if (hipStreamCreate(&e->stream) != hipSuccess) {
     delete e;
@@ -2104,7 +2128,8 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
                   : gen == fqd::kGenHeatBath ? k_spawn<fqd::kGenHeatBath, false>
                   : gen == fqd::kGenLearned  ? k_spawn<fqd::kGenLearned, false>
                                              : k_spawn<fqd::kGenMolRenorm, false>;
-      hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
+      const size_t lds = gen == fqd::kGenHeatBath ? hb_lds_bytes(e->sys.nel) : 0;
+      hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), lds, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
                          e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
                          e->st_att, e->tile_cnt, nullptr, nullptr, nullptr);
       if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));

@@ -327,11 +327,12 @@ __global__ void k_hb_sample(fqd::Sys s, const uint64_t* __restrict__ det, int64_
       hi = mid - 1;
   }
   uint64_t f[FQ_MAX_WORDS] = {0, 0};
-  for (int w = 0; w < s.W; ++w) f[w] = det[static_cast<int64_t>(w) * cap + lo];
+  FQ_FOR_WORDS(w, s.W) f[w] = det[static_cast<int64_t>(w) * cap + lo];
   fqd::Excit conn;
   double h = 0.0, pgen = 1.0;
-  fqd::gen_heat_bath(s, rng, f, conn, h, pgen);
-  for (int w = 0; w < s.W; ++w) sdet[static_cast<int64_t>(w) * bcap + t] = f[w];
+  double wloc[fqd::kHbMaxNel];
+  fqd::gen_heat_bath(s, rng, f, conn, h, pgen, fqd::HbBuf{wloc, 1});
+  FQ_FOR_WORDS(w, s.W) sdet[static_cast<int64_t>(w) * bcap + t] = f[w];
   const bool use = conn.nexcit == 2 && h != 0.0;
   sexc[4 * t] = conn.from[0];
   sexc[4 * t + 1] = conn.from[1];

@@ -6,7 +6,10 @@ Restates the analysis the reference's tools apply to FCIQMC output:
     standard error and its error at each level (the last point is dropped
     when the length is odd; covariance normalised by N - ddof, ddof = 1);
   * the optimal block of Wolff and Lee et al. (blocking.py:171-259): the
-    smallest level with B^3 > 2 n (SE_B/SE_0)^4, B = 2^level;
+    smallest level with B^3 > 2 n (SE_B/SE_0)^4, B = 2^level; when no level
+    satisfies it the series is too short for its correlation time and the
+    error is reported as NaN (pyblock likewise finds no optimal block), not
+    read off the last level's two or three blocks;
   * the ratio estimator with covariance (tools/pyblock/pyblock/error.py:ratio)
     used for E_proj = <sum H_0j N_j> / <N_0>
     (tools/pyhande/pyhande/analysis.py:14-62);
@@ -60,18 +63,17 @@ def ratio(mean_a, mean_b, se_a, se_b, cov_ab, ndata):
 
 def projected_energy(sum_h0j_nj, n0):
     """E_proj = <sum H_0j N_j>/<N_0> and its error at the optimal block of the
-    pair (the larger of the two variables' optimal levels)."""
+    pair (the larger of the two variables' optimal levels); the error is NaN
+    when either variable has no optimal level."""
     data = np.vstack([np.asarray(sum_h0j_nj, float), np.asarray(n0, float)])
     stats = reblock(data)
     opt = find_optimal_block(data.shape[1], stats)
-    if any(math.isnan(o) for o in opt):
-        level = len(stats) - 1
-        converged = False
-    else:
-        level = int(max(opt))
-        converged = True
+    converged = not any(math.isnan(o) for o in opt)
+    level = int(max(opt)) if converged else len(stats) - 1
     _lvl, n, mean, cov, se, _e = stats[level]
     e, err = ratio(mean[0], mean[1], se[0], se[1], cov[0, 1], n)
+    if not converged:
+        e, err = stats[0][2][0] / stats[0][2][1], float("nan")
     return {"energy": e, "error": err, "level": level, "converged": converged, "ndata": n}
 
 
@@ -79,9 +81,12 @@ def mean_error(series):
     """Reblocked mean and standard error of one column."""
     stats = reblock(np.asarray(series, float))
     opt = find_optimal_block(len(series), stats)[0]
-    level = len(stats) - 1 if math.isnan(opt) else int(opt)
+    converged = not math.isnan(opt)
+    level = int(opt) if converged else len(stats) - 1
     _lvl, n, mean, _cov, se, _e = stats[level]
-    return {"mean": float(mean[0]), "error": float(se[0]), "level": level, "converged": not math.isnan(opt)}
+    if not converged:
+        return {"mean": float(stats[0][2][0]), "error": float("nan"), "level": level, "converged": False}
+    return {"mean": float(mean[0]), "error": float(se[0]), "level": level, "converged": True}
 
 
 def efficiency(error, seconds):

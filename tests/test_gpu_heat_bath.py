@@ -76,6 +76,7 @@ def test_molecular_projected_energy_matches_exact(tmp_path, excit_gen):
     sh = reblock.mean_error(prod[:, 1])
     print(f"  E_proj={e:.6f} +- {out['error']:.6f} (level {out['level']}), "
           f"shift={system.H00 + sh['mean']:.6f} +- {sh['error']:.6f}, exact {exact:.6f}", flush=True)
+    assert out["converged"] and sh["converged"], "reblocking found no optimal block"
     assert abs(e - exact) < 3 * out["error"] + 1e-4
     assert abs(system.H00 + sh["mean"] - exact) < 3 * sh["error"] + 5e-3
 
@@ -92,11 +93,12 @@ def test_two_word_projected_energy_matches_exact(tmp_path, excit_gen):
     system = api.read_in(int_file=path, nel=nel, ms=0)
     assert system.view.nwords == 2
     print(f"[production] synthetic {norb}o{nel}e W=2 {excit_gen}", flush=True)
-    q = dict(tau=0.01, rng_seed=31, init_pop=10, mc_cycles=20, nreports=500, target_population=3000,
+    q = dict(tau=0.01, rng_seed=31, init_pop=10, mc_cycles=20, nreports=1500, target_population=3000,
              shift_damping=0.05, state_size=100000, spawned_state_size=100000, excit_gen=excit_gen)
     tab = api.fciqmc(system, q, rng_mode="philox")
     prod = tab[150:]
     out = reblock.projected_energy(prod[:, 2], prod[:, 3])
     e = system.H00 + out["energy"]
-    print(f"  E_proj={e:.6f} +- {out['error']:.6f}, exact {exact:.6f}", flush=True)
+    print(f"  E_proj={e:.6f} +- {out['error']:.6f} (level {out['level']}), exact {exact:.6f}", flush=True)
+    assert out["converged"], "reblocking found no optimal block"
     assert abs(e - exact) < 3 * out["error"] + 1e-4

@@ -333,5 +333,6 @@ def test_learned_projected_energy_matches_exact(tmp_path):
     prod = tab[150:]
     out = reblock.projected_energy(prod[:, 2], prod[:, 3])
     e = system.H00 + out["energy"]
-    print(f"  E_proj={e:.6f} +- {out['error']:.6f}, exact {exact:.6f}", flush=True)
+    print(f"  E_proj={e:.6f} +- {out['error']:.6f} (level {out['level']}), exact {exact:.6f}", flush=True)
+    assert out["converged"], "reblocking found no optimal block"
     assert abs(e - exact) < 3 * out["error"] + 1e-4

@@ -44,5 +44,6 @@ def test_projected_energy_matches_fci(fci, deck, initiator):
     sh = reblock.mean_error(prod[:, 1])
     print(f"{fci} initiator={initiator}: E_proj={e:.6f} +- {out['error']:.6f} (level {out['level']}), "
           f"shift={system.H00 + sh['mean']:.6f} +- {sh['error']:.6f}, exact {exact:.6f}")
+    assert out["converged"] and sh["converged"], "reblocking found no optimal block"
     assert abs(e - exact) < 3 * out["error"] + 1e-4
     assert abs(system.H00 + sh["mean"] - exact) < 3 * sh["error"] + 5e-3

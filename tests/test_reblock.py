@@ -59,3 +59,15 @@ def test_ratio_closed_form():
     b = 5 + rng.normal(size=4096)
     out = reblock.projected_energy(2 * b, b)
     assert math.isclose(out["energy"], 2.0) and out["error"] < 1e-6
+
+
+def test_unconverged_series_reports_nan_error():
+    """A series much shorter than its correlation time has no optimal block;
+    its error is NaN rather than the last level's (two or three blocks)."""
+    rng = np.random.default_rng(4)
+    x = np.cumsum(rng.normal(size=64)) * 0.01 + 1.0  # random walk: correlated throughout
+    out = reblock.mean_error(x)
+    assert not out["converged"] and math.isnan(out["error"])
+    assert math.isclose(out["mean"], x.mean())
+    pe = reblock.projected_energy(x * 2.0 + rng.normal(size=64) * 1e-3, np.full(64, 1.0) + x * 0.1)
+    assert not pe["converged"] and math.isnan(pe["error"])
