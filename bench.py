@@ -303,10 +303,13 @@ def prepare(lib, system, target, seed, cap, equil_tau=0.01, hold=250, max_cycles
                         dat0.ctypes.data_as(C.POINTER(C.c_double)), 1), "fq_load")
     prop = Propagator(lib, eng, target, damping=0.2, pull=1.0)
     vary_at = None
+    t0 = time.perf_counter()
     for i in range(max_cycles):
         prop.cycle(equil_tau)
         if prop.vary and vary_at is None:
             vary_at = i
+        if (i + 1) % 200 == 0:
+            log(f"[eff] prep cycle {i + 1}: N={prop.last.nparticles[0]:.4g}, {time.perf_counter() - t0:.1f}s")
         if vary_at is not None and i - vary_at >= hold:
             break
     st, po, da = download_list(lib, eng, system.view.nwords)
@@ -337,6 +340,8 @@ def production(lib, eng, target, state, tau, cycles, warmup, report=10):
             proj.append(accp)
             d0.append(accd)
             accp = accd = 0.0
+        if (i + 1) % 500 == 0:
+            log(f"[eff] production cycle {i + 1}: N={out.nparticles[0]:.4g}, {time.perf_counter() - t0:.1f}s")
     wall = time.perf_counter() - t0
     return np.array(proj), np.array(d0), wall, attempts, prop.last
 
