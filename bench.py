@@ -161,20 +161,32 @@ class Propagator:
         return out
 
 
-def phase_bytes(prof, W=1, S=1):
-    """Algorithmic HBM bytes of each phase of one cycle (compulsory traffic)."""
+def sort_key_words(nbasis, nel):
+    """64-bit words of the production sort key: the combinatorial rank of a
+    determinant needs ceil(log2 C(nbasis, nel)) bits."""
+    return 1 if math.comb(nbasis, nel) <= (1 << 64) else 2
+
+
+def phase_bytes(prof, W=1, S=1, kw=None):
+    """Algorithmic HBM bytes of each phase of one cycle (compulsory traffic);
+    kw: words of the sort key (W when not given)."""
+    kw = W if kw is None else kw
     nd, ns, nr, nout = prof.nstates_in, prof.nspawned, prof.nruns, prof.nstates_out
     bm = 8 * W + 8 * S + 8          # main-list entry: det words, pops, K_ii
-    bs = 8 * W + 8 * S + 8 + 8      # spawned entry: det words, pops, flag, attempt ordinal
+    bs = 8 * W + 8 * S + 8          # spawned entry: det words, pops, flag
     return {
         # read det/pop/K_ii, write post-death pops, attempt counts, flags
         "det_pass": nd * (bm + 8 * S + 8 * S + 8),
         # parents (det, pop, flag) and attempt offsets once, spawned entries written
         "spawn": nd * (8 * W + 8 * S + 8 + 8 * S) + ns * bs,
-        # each 8-bit LSD pass reads keys+indices twice (histogram, scatter) and writes them once
-        "sort": prof.radix_passes * ns * (3 * (8 * W + 4)),
-        # run starts, gather of pops/flags/ordinals, unique runs written
-        "reduce": ns * (8 * W + 8 + 4 + 8 * S + 8 + 8) + nr * (8 * W + 8 * S + 8 + 8),
+        # rank keys written once; each 8-bit LSD pass reads keys+indices twice
+        # (histogram, scatter) and writes them once
+        "sort": ns * (8 * W + 8 * kw + 4) + prof.radix_passes * ns * (3 * (8 * kw + 4)),
+        # run heads: sorted keys and permutation read, pops/flags gathered and
+        # written in sorted order; reduction: those read back, run starts, the
+        # run's determinant, unique runs written
+        "reduce": ns * (8 * kw + 4 + 2 * (8 * S + 8)) + ns * (8 * S + 8) + nr * (16 + 4 + 8 * W)
+                  + nr * (8 * W + 8 * S + 8 + 8),
         # main list read twice (merge search ~ touched lines ignored) and written once, runs read
         "merge": nd * (2 * bm + 3 * 8) + nout * bm + nr * (8 * W + 8 * S + 8 + 8 * 3),
     }
@@ -346,6 +358,19 @@ def production(lib, eng, target, state, tau, cycles, warmup, report=10):
     return np.array(proj), np.array(d0), wall, attempts, prop.last
 
 
+EFF_BATCHES = 6
+
+
+def batch_means_ratio(num, den, nbatch):
+    """Ratio of means sum(num)/sum(den) and its standard error from the spread
+    of the ratios of nbatch contiguous batches (batch means; the same batches
+    for every generator compared)."""
+    n = len(num) // nbatch * nbatch
+    a = np.asarray(num[:n], float).reshape(nbatch, -1).sum(axis=1)
+    b = np.asarray(den[:n], float).reshape(nbatch, -1).sum(axis=1)
+    return a.sum() / b.sum(), float(np.std(a / b, ddof=1) / math.sqrt(nbatch))
+
+
 def efficiency_learned(args):
     from hande_amd import reblock
     lib = N.lib()
@@ -376,13 +401,16 @@ def efficiency_learned(args):
         proj, d0, wall, att, last = production(lib, eng, args.target, state, 1e-3, args.eff_cycles, 50)
         lib.fq_destroy(eng)
         rb = reblock.projected_energy(proj, d0)
+        e_bm, sigma = batch_means_ratio(proj, d0, EFF_BATCHES)
         t = wall + t_train
-        res[gen] = {"E_proj": system.H00 + rb["energy"], "sigma": rb["error"], "wall_s": wall, "train_s": t_train,
-                    "efficiency": reblock.efficiency(rb["error"], t), "attempts_per_s": att / wall,
-                    "reblock_level": rb["level"], "converged": rb["converged"],
-                    "walkers": sum(last.nparticles[k] for k in range(S)), "determinants": int(last.nstates)}
-        log(f"[eff] {gen}: E_proj {res[gen]['E_proj']:.6f} +- {rb['error']:.6f}, {wall:.1f}s, "
-            f"efficiency {res[gen]['efficiency']:.4g}")
+        res[gen] = {"E_proj": system.H00 + e_bm, "sigma": sigma, "wall_s": wall, "train_s": t_train,
+                    "efficiency": reblock.efficiency(sigma, t), "attempts_per_s": att / wall,
+                    "sigma_method": f"batch means: {EFF_BATCHES} contiguous batches of the production cycles",
+                    "reblock": {"E_proj": system.H00 + rb["energy"], "sigma": rb["error"], "level": rb["level"],
+                                "converged": rb["converged"]},
+                    "walkers": last.nparticles[0], "determinants": int(last.nstates)}
+        log(f"[eff] {gen}: E_proj {res[gen]['E_proj']:.6f} +- {sigma:.6f}, {wall:.1f}s, "
+            f"efficiency {res[gen]['efficiency']:.4g} (reblocking converged: {rb['converged']})")
     a, b = res["learned"], res["heat_bath"]
     agree = abs(a["E_proj"] - b["E_proj"]) < 2 * math.hypot(a["sigma"], b["sigma"])
     print(json.dumps({"metric": "E_proj efficiency 1/(sigma^2 t), learned vs heat_bath", "unit": "1/(Ha^2 s)",
@@ -636,7 +664,8 @@ def main():
     last = prop.last
     phases = ["det_pass", "spawn", "sort", "reduce", "merge"]
     avg_ms = {ph: float(np.mean([getattr(pr, ph + "_ms") for pr in profs])) for ph in phases}
-    avg_bytes = {ph: float(np.mean([phase_bytes(pr, W, S)[ph] for pr in profs])) for ph in phases}
+    kw = sort_key_words(system.view.nbasis, system.view.nel)
+    avg_bytes = {ph: float(np.mean([phase_bytes(pr, W, S, kw)[ph] for pr in profs])) for ph in phases}
     launches = sum(pr.spawn_launches for pr in profs)
     k_ms = sum(pr.spawn_kernel_ms for pr in profs) / launches
     k_bytes = sum(spawn_kernel_bytes(pr, W, S) for pr in profs) / launches

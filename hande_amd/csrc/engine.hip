@@ -60,6 +60,8 @@ constexpr uint64_t kOrbMask = (uint64_t{1} << kInstShift) - 1;
 constexpr int kMaxInst = 1 << 16;
 constexpr int kRadixItems = 8;
 constexpr int kRadixTile = kBlock * kRadixItems;
+constexpr int kRunTile = 4 * kBlock;  // sorted entries per run-detection tile
+constexpr int kMergeWin = 1024;       // main-list window entries a merge block stages in LDS
 
 #define FQ_HIP(call)                                  \
   do {                                                \
@@ -234,7 +236,6 @@ struct fq_engine {
   uint64_t* st_det = nullptr;   // W*bcap
   int64_t* st_pop = nullptr;    // bcap
   int64_t* st_ent = nullptr;    // bcap
-  int64_t* st_att = nullptr;    // bcap: attempt index of each compacted success
   int64_t* tile_cnt = nullptr;  // successes per spawning tile (bcap / kSpawnTile + 2)
   int64_t* tile_off = nullptr;  // their exclusive prefix
   // spawned list
@@ -242,7 +243,6 @@ struct fq_engine {
   uint64_t* sdet = nullptr;  // W*scap
   int64_t* spop = nullptr;   // S*scap
   int64_t* sflag = nullptr;  // scap
-  uint64_t* sord = nullptr;  // scap attempt ordinal
   // sort buffers
   uint64_t* kbuf[2] = {nullptr, nullptr};  // W*scap
   uint32_t* vbuf[2] = {nullptr, nullptr};  // scap
@@ -250,8 +250,11 @@ struct fq_engine {
   int64_t* hist_scan = nullptr;
   int64_t hist_cap = 0;
   // runs
-  int64_t* head = nullptr;      // scap+1
-  int64_t* head_scan = nullptr;  // scap+1
+  int64_t* run_stage = nullptr;  // scap: run starts compacted per tile
+  int64_t* run_tcnt = nullptr;   // runs per tile (scap / kRunTile + 2)
+  int64_t* run_toff = nullptr;   // their exclusive prefix
+  int64_t* gpop = nullptr;       // S*scap: spawned populations in sorted order
+  int64_t* gflag = nullptr;      // scap: spawned flags in sorted order
   int64_t* run_start = nullptr;  // scap+1
   uint64_t* rdet = nullptr;      // W*scap
   int64_t* rpop = nullptr;       // S*scap
@@ -413,9 +416,26 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restric
   __shared__ int64_t shi[kBlock];
   double acc[2 * FQ_MAX_SPACES] = {0.0, 0.0, 0.0, 0.0};
   int64_t ai = 0;
-  for (int b = threadIdx.x; b < nb; b += kBlock) {
-    for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) acc[q] = acc[q] + part_d[q * nb + b];
-    ai += part_i[b];
+  // blocks t, t + 256, ... added in that order; eight are loaded ahead at a time
+  constexpr int kAhead = 8;
+  for (int b0 = threadIdx.x; b0 < nb; b0 += kAhead * kBlock) {
+    double v[kAhead][2 * FQ_MAX_SPACES];
+    int64_t vi[kAhead];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      const int b = b0 + k * kBlock;
+#pragma unroll
+      for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) v[k][q] = b < nb ? part_d[q * nb + b] : 0.0;
+      vi[k] = b < nb ? part_i[b] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      if (b0 + k * kBlock < nb) {
+#pragma unroll
+        for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) acc[q] = acc[q] + v[k][q];
+        ai += vi[k];
+      }
+    }
   }
   for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) sh[q][threadIdx.x] = acc[q];
   shi[threadIdx.x] = ai;
@@ -465,8 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
                                                   const uint64_t* __restrict__ det, const int64_t* __restrict__ pop,
                                                   double tau, uint64_t seed, uint32_t cyc, int64_t bcap,
                                                   uint64_t* __restrict__ st_det, int64_t* __restrict__ st_pop,
-                                                  int64_t* __restrict__ st_ent, int64_t* __restrict__ st_att,
-                                                  int64_t* __restrict__ tile_cnt,
+                                                  int64_t* __restrict__ st_ent, int64_t* __restrict__ tile_cnt,
                                                   const Sys* __restrict__ sys_arr, const int64_t* __restrict__ iatt,
                                                   const double* __restrict__ itau) {
   __shared__ int64_t win[2];
@@ -557,7 +576,6 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
         FQ_FOR_WORDS(w, s.W) st_det[w * bcap + u] = child[w];
         st_pop[u] = nspawn;
         st_ent[u] = e;
-        st_att[u] = t;
       }
       run += round_total;
       __syncthreads();
@@ -573,12 +591,11 @@ __global__ __launch_bounds__(kBlock) void k_pack_tiles(Sys s, int64_t ntiles, co
                                                        const int64_t* __restrict__ tile_off,
                                                        const uint64_t* __restrict__ st_det,
                                                        const int64_t* __restrict__ st_pop,
-                                                       const int64_t* __restrict__ st_ent,
-                                                       const int64_t* __restrict__ st_att, int64_t bcap,
+                                                       const int64_t* __restrict__ st_ent, int64_t bcap,
                                                        int64_t t_begin, const int64_t* __restrict__ flags,
                                                        int64_t scap, uint64_t* __restrict__ sdet,
                                                        int64_t* __restrict__ spop, int64_t* __restrict__ sflag,
-                                                       uint64_t* __restrict__ sord, Stats* __restrict__ st) {
+                                                       Stats* __restrict__ st) {
   const int64_t tile = blockIdx.x;
   if (tile >= ntiles) return;
   const int64_t cnt = tile_cnt[tile];
@@ -596,7 +613,6 @@ __global__ __launch_bounds__(kBlock) void k_pack_tiles(Sys s, int64_t ntiles, co
     for (int w = 0; w < s.W; ++w) sdet[w * scap + slot] = st_det[w * bcap + u];
     for (int q = 0; q < s.S; ++q) spop[q * scap + slot] = (q == sp) ? st_pop[u] : 0;
     sflag[slot] = s.initiator ? flags[i] : 0;
-    sord[slot] = static_cast<uint64_t>(st_att[u]);
   }
 }
 
@@ -752,7 +768,7 @@ __global__ void k_replay_walk(Sys s, int64_t n, int64_t cap, const uint64_t* __r
                               int64_t* __restrict__ pop_new, double tau, double sh0, double sh1, double e0,
                               double e1, const double* __restrict__ rbuf, int64_t rpos, int64_t rend, int64_t scap,
                               uint64_t* __restrict__ sdet, int64_t* __restrict__ spop, int64_t* __restrict__ sflag,
-                              uint64_t* __restrict__ sord, Stats* __restrict__ st) {
+                              Stats* __restrict__ st) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   fqd::ReplayRng rng{rbuf, rpos, rend, false};
   double proj[FQ_MAX_SPACES] = {0.0, 0.0}, d0[FQ_MAX_SPACES] = {0.0, 0.0};
@@ -783,7 +799,6 @@ __global__ void k_replay_walk(Sys s, int64_t n, int64_t cap, const uint64_t* __r
             FQ_FOR_WORDS(w, s.W) sdet[w * scap + slot] = child[w];
             for (int q = 0; q < s.S; ++q) spop[q * scap + slot] = (q == sp) ? ns : 0;
             sflag[slot] = s.initiator ? flag : 0;
-            sord[slot] = static_cast<uint64_t>(slot);
           }
         }
       }
@@ -833,31 +848,51 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* __restric
   hist[static_cast<int64_t>(threadIdx.x) * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Radix pass scatter, stable: tiles are processed in rounds of 256 elements in
-// input order; within a round, ranks come from 8 wavefront ballots (lanes with
-// equal digits) and per-wave digit counts in LDS.
+// Radix pass scatter, stable, staged in LDS.  The tile's elements are ranked
+// in input order (rounds of 256; per round 8 wavefront ballots give each
+// lane's rank among equal digits of its wave, per-wave digit counts in LDS
+// give the rest) and placed in LDS sorted by digit at the tile's digit
+// offsets (an exclusive scan of the tile's histogram); the tile is then
+// written out in that order, so consecutive threads store consecutive
+// addresses of one digit's run (global base of digit d for this tile:
+// hist_scan[d * nb + tile]).  KW key words per element.
+template <int KW>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                          int64_t n, int W, int w_digit, int shift,
+                                                          int64_t n, int w_digit, int shift,
+                                                          const int64_t* __restrict__ hist,
                                                           const int64_t* __restrict__ hist_scan,
                                                           uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                           int64_t stride) {
-  __shared__ int64_t base_d[256];
+  __shared__ uint64_t sk[KW][kRadixTile];
+  __shared__ uint32_t sv[kRadixTile];
+  __shared__ int64_t gbase[256];
+  __shared__ int64_t loff[256];
+  __shared__ int run[256];
   __shared__ int wcnt[kBlock / 64][256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  base_d[tid] = hist_scan[static_cast<int64_t>(tid) * gridDim.x + blockIdx.x];
+  const int64_t nb = gridDim.x;
+  const int64_t cnt = hist[static_cast<int64_t>(tid) * nb + blockIdx.x];
+  gbase[tid] = hist_scan[static_cast<int64_t>(tid) * nb + blockIdx.x];
+  loff[tid] = cnt;
+  run[tid] = 0;
+  __syncthreads();
+  block_inclusive_scan(loff);
+  loff[tid] -= cnt;
   const int64_t tbase = static_cast<int64_t>(blockIdx.x) * kRadixTile;
+  const int tn = static_cast<int>(n - tbase < kRadixTile ? n - tbase : kRadixTile);
   for (int r = 0; r < kRadixItems; ++r) {
     for (int k = tid; k < (kBlock / 64) * 256; k += kBlock) (&wcnt[0][0])[k] = 0;
     __syncthreads();
-    const int64_t i = tbase + r * kBlock + tid;
-    const bool valid = i < n;
-    uint64_t kv[FQ_MAX_WORDS] = {0, 0};
+    const int li = r * kBlock + tid;
+    const bool valid = li < tn;
+    uint64_t kv[KW] = {};
     uint32_t v = 0;
     int digit = 0;
     if (valid) {
-      FQ_FOR_WORDS(w, W) kv[w] = kin[w * stride + i];
-      v = vin[i];
-      digit = static_cast<int>(((w_digit == 0 ? kv[0] : kv[1]) >> shift) & 0xff);
+#pragma unroll
+      for (int w = 0; w < KW; ++w) kv[w] = kin[w * stride + tbase + li];
+      v = vin[tbase + li];
+      digit = static_cast<int>(((w_digit == 0 ? kv[0] : kv[KW - 1]) >> shift) & 0xff);
     }
     unsigned long long peers = __ballot(valid);
     for (int b = 0; b < 8; ++b) {
@@ -869,16 +904,25 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __rest
     if (valid && rank == 0) wcnt[wave][digit] = __popcll(peers);
     __syncthreads();
     if (valid) {
-      int64_t pos = base_d[digit] + rank;
+      int pos = static_cast<int>(loff[digit]) + run[digit] + rank;
       for (int ww = 0; ww < wave; ++ww) pos += wcnt[ww][digit];
-      for (int w = 0; w < W; ++w) kout[w * stride + pos] = kv[w];
-      vout[pos] = v;
+#pragma unroll
+      for (int w = 0; w < KW; ++w) sk[w][pos] = kv[w];
+      sv[pos] = v;
     }
     __syncthreads();
     int add = 0;
     for (int ww = 0; ww < kBlock / 64; ++ww) add += wcnt[ww][tid];
-    base_d[tid] += add;
-    __syncthreads();
+    run[tid] += add;
+    __syncthreads();  // wcnt is cleared by the next round
+  }
+  for (int p = tid; p < tn; p += kBlock) {
+    const uint64_t dw = w_digit == 0 ? sk[0][p] : sk[KW - 1][p];
+    const int d = static_cast<int>((dw >> shift) & 0xff);
+    const int64_t g = gbase[d] + (p - loff[d]);
+#pragma unroll
+    for (int w = 0; w < KW; ++w) kout[w * stride + g] = sk[w][p];
+    vout[g] = sv[p];
   }
 }
 
@@ -998,61 +1042,97 @@ __global__ void k_replay_qsort(const uint64_t* __restrict__ sdet, int64_t ns, in
 // ---------------------------------------------------------------------------
 // Kernels: self-annihilation of the sorted spawned list
 // ---------------------------------------------------------------------------
-__global__ void k_heads(const uint64_t* __restrict__ keys, int64_t ns, int W, int64_t scap, int64_t* __restrict__ head) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= ns) return;
-  head[i] = (i == 0 || key_cmp(keys, scap, W, i, i - 1) != 0) ? 1 : 0;
+// Runs of equal keys in the sorted list, per tile of kRunTile entries: the
+// tile's run heads (first entry, or key differing from the previous one) are
+// ranked in order by wavefront ballots and written contiguously from the
+// tile's first slot of run_stage, with the tile's count in tile_cnt; each
+// entry's populations and flag are gathered through the sort permutation into
+// sorted order (gpop, gflag) for the run reduction.
+__global__ __launch_bounds__(kBlock) void k_run_heads(const uint64_t* __restrict__ keys, int64_t ns, int kw,
+                                                      int64_t scap, int S, const uint32_t* __restrict__ perm,
+                                                      const int64_t* __restrict__ spop,
+                                                      const int64_t* __restrict__ sflag,
+                                                      int64_t* __restrict__ gpop, int64_t* __restrict__ gflag,
+                                                      int64_t* __restrict__ run_stage,
+                                                      int64_t* __restrict__ tile_cnt) {
+  __shared__ int32_t wcount[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t ntiles = (ns + kRunTile - 1) / kRunTile;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    int32_t run = 0;
+    for (int k = 0; k < kRunTile / kBlock; ++k) {
+      const int64_t i = tile * kRunTile + k * kBlock + threadIdx.x;
+      bool head = false;
+      if (i < ns) {
+        head = i == 0;
+#pragma unroll
+        for (int w = 0; w < FQ_MAX_WORDS; ++w)
+          if (w < kw && i > 0) head = head || keys[w * scap + i] != keys[w * scap + i - 1];
+        const uint32_t j = perm[i];
+        for (int sp = 0; sp < S; ++sp) gpop[sp * scap + i] = spop[sp * scap + j];
+        gflag[i] = sflag[j];
+      }
+      const unsigned long long bal = __ballot(head);
+      if (lane == 0) wcount[wave] = __popcll(bal);
+      __syncthreads();
+      int32_t pos = run + __popcll(bal & ((1ull << lane) - 1));
+      int32_t round_total = 0;
+      for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wave) pos += wcount[w];
+        round_total += wcount[w];
+      }
+      if (head) run_stage[tile * kRunTile + pos] = i;
+      run += round_total;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) tile_cnt[tile] = run;
+    __syncthreads();
+  }
 }
 
-__global__ void k_run_starts(const int64_t* __restrict__ head, const int64_t* __restrict__ head_scan, int64_t ns,
-                             int64_t* __restrict__ run_start) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < ns && head[i]) run_start[head_scan[i]] = i;
-  if (i == 0) run_start[head_scan[ns]] = ns;
+// run_start[tile_off[t] + j] for the tile's j-th run; run_start[nruns] = ns.
+__global__ __launch_bounds__(kBlock) void k_pack_runs(int64_t ntiles, int64_t ns,
+                                                      const int64_t* __restrict__ tile_cnt,
+                                                      const int64_t* __restrict__ tile_off,
+                                                      const int64_t* __restrict__ run_stage,
+                                                      int64_t* __restrict__ run_start) {
+  const int64_t tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  const int64_t cnt = tile_cnt[tile], base = tile_off[tile];
+  for (int64_t j = threadIdx.x; j < cnt; j += kBlock) run_start[base + j] = run_stage[tile * kRunTile + j];
+  if (tile == ntiles - 1 && threadIdx.x == 0) run_start[tile_off[ntiles]] = ns;
 }
 
-// One thread per run of equal keys: annihilate_spawn_t(_initiator).  In replay
-// mode the first entry is the first in sorted order (the reference's); in
-// production it is the entry with the smallest attempt ordinal.
+// One thread per run of equal keys: annihilate_spawn_t(_initiator)
+// (src/spawn_data.F90:859-1101).  The run's first entry is its first in
+// sorted order: the reference's quicksort order in replay mode, the earliest
+// spawned in production (stable sort of an attempt-ordered list).
 __global__ void k_reduce_runs(Sys s, int64_t nruns, const int64_t* __restrict__ run_start,
                               const uint64_t* __restrict__ sdet, const uint32_t* __restrict__ perm,
-                              const int64_t* __restrict__ spop, const int64_t* __restrict__ sflag,
-                              const uint64_t* __restrict__ sord, int64_t scap, int first_by_order,
+                              const int64_t* __restrict__ gpop, const int64_t* __restrict__ gflag, int64_t scap,
                               uint64_t* __restrict__ rdet, int64_t* __restrict__ rpop, int64_t* __restrict__ rflag,
                               int64_t* __restrict__ rstate) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= nruns) return;
   const int64_t a = run_start[r], b = run_start[r + 1];
-  int64_t first = a;
-  if (!first_by_order) {
-    uint64_t best = sord[perm[a]];
-    for (int64_t k = a + 1; k < b; ++k) {
-      const uint64_t o = sord[perm[k]];
-      if (o < best) {
-        best = o;
-        first = k;
-      }
-    }
-  }
   int64_t tot[FQ_MAX_SPACES] = {0, 0}, init[FQ_MAX_SPACES] = {0, 0};
   int events[FQ_MAX_SPACES] = {0, 0};
+  const int64_t f0 = gflag[a];
   for (int sp = 0; sp < s.S; ++sp) {
-    const uint32_t j = perm[first];
-    const int64_t v = spop[sp * scap + j];
+    const int64_t v = gpop[sp * scap + a];
     tot[sp] = v;
-    if (!((sflag[j] >> sp) & 1)) {
+    if (!((f0 >> sp) & 1)) {
       init[sp] = v;
     } else {
       const int32_t v32 = static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(v) & 0xffffffffULL));
       events[sp] = v32 >= 0 ? 1 : -1;
     }
   }
-  for (int64_t k = a; k < b; ++k) {
-    if (k == first) continue;
-    const uint32_t j = perm[k];
+  for (int64_t k = a + 1; k < b; ++k) {
+    const int64_t fk = gflag[k];
     for (int sp = 0; sp < s.S; ++sp) {
-      const int64_t v = spop[sp * scap + j];
-      if (!((sflag[j] >> sp) & 1))
+      const int64_t v = gpop[sp * scap + k];
+      if (!((fk >> sp) & 1))
         init[sp] += v;
       else if (v < 0)
         events[sp] -= 1;
@@ -1075,7 +1155,8 @@ __global__ void k_reduce_runs(Sys s, int64_t nruns, const int64_t* __restrict__ 
     }
     rpop[sp * scap + r] = tot[sp];
   }
-  for (int w = 0; w < s.W; ++w) rdet[w * scap + r] = sdet[w * scap + perm[a]];
+  const uint32_t j = perm[a];
+  FQ_FOR_WORDS(w, s.W) rdet[w * scap + r] = sdet[w * scap + j];
   rflag[r] = fl;
   rstate[r] = any ? 1 : 0;
 }
@@ -1116,6 +1197,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_runs(Sys s, int64_t nruns, int
                                                        int64_t scap, int64_t* __restrict__ rpos,
                                                        int64_t* __restrict__ rstate) {
   __shared__ int64_t win[2];
+  __shared__ uint64_t wdet[FQ_MAX_WORDS][kMergeWin];  // the window's determinants when it fits
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
   const int64_t r1 = (r0 + blockDim.x < nruns ? r0 + blockDim.x : nruns) - 1;
   if (threadIdx.x < 2) {
@@ -1125,6 +1207,13 @@ __global__ __launch_bounds__(kBlock) void k_merge_runs(Sys s, int64_t nruns, int
     win[threadIdx.x] = main_lower_bound(det, cap, s.W, k, 0, n);
   }
   __syncthreads();
+  // every key of the block lies in [key(r0), key(r1)], so its lower bound
+  // lies in [wlo, whi): whi = min(bound(r1) + 1, n)
+  const int64_t wlo = win[0], whi = win[1] < n ? win[1] + 1 : n;
+  const bool staged = whi - wlo <= kMergeWin;
+  if (staged)
+    for (int64_t q = threadIdx.x; q < whi - wlo; q += kBlock) FQ_FOR_WORDS(w, s.W) wdet[w][q] = det[w * cap + wlo + q];
+  __syncthreads();
   const int64_t r = r0 + threadIdx.x;
   if (r >= nruns) return;
   if (rstate[r] == 0) {
@@ -1133,13 +1222,35 @@ __global__ __launch_bounds__(kBlock) void k_merge_runs(Sys s, int64_t nruns, int
   }
   uint64_t k[FQ_MAX_WORDS] = {0, 0};
   FQ_FOR_WORDS(w, s.W) k[w] = rdet[w * scap + r];
-  // lower bound in [win0, min(win1 + 1, n)): every key of the block lies in
-  // [key(r0), key(r1)], so its bound lies between theirs
-  const int64_t lo = main_lower_bound(det, cap, s.W, k, win[0], win[1] < n ? win[1] + 1 : n);
+  int64_t lo;
   bool hit = false;
-  if (lo < n) {
-    hit = true;
-    FQ_FOR_WORDS(w, s.W) hit = hit && det[w * cap + lo] == k[w];
+  if (staged) {
+    int a = 0, b = static_cast<int>(whi - wlo);
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      int c = 0;
+#pragma unroll
+      for (int w = FQ_MAX_WORDS - 1; w >= 0; --w)
+        if (w < s.W && c == 0) {
+          const uint64_t x = wdet[w][mid];
+          c = (x < k[w]) ? -1 : (x > k[w] ? 1 : 0);
+        }
+      if (c < 0)
+        a = mid + 1;
+      else
+        b = mid;
+    }
+    lo = wlo + a;
+    if (a < whi - wlo) {
+      hit = true;
+      FQ_FOR_WORDS(w, s.W) hit = hit && wdet[w][a] == k[w];
+    }
+  } else {
+    lo = main_lower_bound(det, cap, s.W, k, wlo, whi);
+    if (lo < n) {
+      hit = true;
+      FQ_FOR_WORDS(w, s.W) hit = hit && det[w * cap + lo] == k[w];
+    }
   }
   rpos[r] = lo;
   const int64_t fl = rflag[r];
@@ -1310,13 +1421,15 @@ __global__ void k_scatter_new(Sys s, int64_t nruns, int64_t cap, int64_t scap, c
 __global__ __launch_bounds__(kBlock) void k_abs_pop(int64_t n, int64_t cap, int S, const int64_t* __restrict__ pop,
                                                     int64_t* __restrict__ part_i) {
   __shared__ unsigned long long sh[FQ_MAX_SPACES][kBlock];
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  // exact integer sums: any grid and order give the same totals
   for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
     unsigned long long v = 0;
-    if (i < n && sp < S) {
-      const int64_t p = pop[sp * cap + i];
-      v = static_cast<unsigned long long>(p < 0 ? -p : p);
-    }
+    if (sp < S)
+      for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+           i += static_cast<int64_t>(gridDim.x) * kBlock) {
+        const int64_t p = pop[sp * cap + i];
+        v += static_cast<unsigned long long>(p < 0 ? -p : p);
+      }
     sh[sp][threadIdx.x] = v;
   }
   __syncthreads();
@@ -1414,8 +1527,9 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
                              e->hist);
           int rc = scan_exclusive(e, e->hist, e->hist_scan, 256 * nb);
           if (rc != FQ_OK) return rc;
-          hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src], e->vbuf[src], ns,
-                             kw, w, shift, e->hist_scan, e->kbuf[1 - src], e->vbuf[1 - src], scap);
+          auto scatter = kw == 1 ? k_radix_scatter<1> : k_radix_scatter<2>;
+          hipLaunchKernelGGL(scatter, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src], e->vbuf[src], ns, w, shift,
+                             e->hist, e->hist_scan, e->kbuf[1 - src], e->vbuf[1 - src], scap);
           src = 1 - src;
           ++e->radix_passes;
         }
@@ -1426,19 +1540,22 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
   }
   FQ_HIP(hipEventRecord(e->pev[5], e->stream));
   if (ns > 0) {
-    hipLaunchKernelGGL(k_heads, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, keys, ns, kw, scap, e->head);
-    int rc = scan_exclusive(e, e->head, e->head_scan, ns);
+    const int64_t nt = ceil_div(ns, kRunTile);
+    hipLaunchKernelGGL(k_run_heads, dim3(static_cast<unsigned>(std::min<int64_t>(nt, 1 << 20))), dim3(kBlock), 0,
+                       e->stream, keys, ns, kw, scap, e->S, perm, e->spop, e->sflag, e->gpop, e->gflag, e->run_stage,
+                       e->run_tcnt);
+    int rc = scan_exclusive(e, e->run_tcnt, e->run_toff, nt);
     if (rc != FQ_OK) return rc;
-    hipLaunchKernelGGL(k_run_starts, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->head, e->head_scan, ns,
-                       e->run_start);
+    hipLaunchKernelGGL(k_pack_runs, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0, e->stream, nt, ns, e->run_tcnt,
+                       e->run_toff, e->run_stage, e->run_start);
   }
   int64_t nruns = 0;
   if (ns > 0) {
-    FQ_HIP(hipMemcpyAsync(&nruns, e->head_scan + ns, sizeof(int64_t), hipMemcpyDeviceToHost, e->stream));
+    FQ_HIP(hipMemcpyAsync(&nruns, e->run_toff + ceil_div(ns, kRunTile), sizeof(int64_t), hipMemcpyDeviceToHost,
+                          e->stream));
     FQ_HIP(hipStreamSynchronize(e->stream));
     hipLaunchKernelGGL(k_reduce_runs, dim3(grid_for(nruns)), dim3(kBlock), 0, e->stream, e->sys, nruns,
-                       e->run_start, e->sdet, perm, e->spop, e->sflag, e->sord, scap, 1, e->rdet, e->rpop,
-                       e->rflag, e->rstate);
+                       e->run_start, e->sdet, perm, e->gpop, e->gflag, scap, e->rdet, e->rpop, e->rflag, e->rstate);
   }
   FQ_HIP(hipEventRecord(e->pev[6], e->stream));
   e->h_stats.nruns = nruns;
@@ -1509,7 +1626,7 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
   if (rc != FQ_OK) return rc;
   e->cur = o;
   e->n = e->h_stats.new_n;
-  const unsigned nba = grid_for(e->n);
+  const unsigned nba = std::min<unsigned>(grid_for(e->n), 2048);
   hipLaunchKernelGGL(k_abs_pop, dim3(nba), dim3(kBlock), 0, st, e->n, cap, e->S, e->pop[o], e->part_i);
   hipLaunchKernelGGL(k_sum_abs, dim3(1), dim3(kBlock), 0, st, e->part_i, static_cast<int>(nba), e->d_stats);
   return sync_stats(e);
@@ -1805,8 +1922,8 @@ int pack_spawned(fq_engine* e, int64_t nbt, int64_t t_begin) {
   int rc = scan_exclusive(e, e->tile_cnt, e->tile_off, nt);
   if (rc != FQ_OK) return rc;
   hipLaunchKernelGGL(k_pack_tiles, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0, e->stream, e->sys, nt,
-                     e->tile_cnt, e->tile_off, e->st_det, e->st_pop, e->st_ent, e->st_att, e->bcap, t_begin,
-                     e->flags, e->scap, e->sdet, e->spop, e->sflag, e->sord, e->d_stats);
+                     e->tile_cnt, e->tile_off, e->st_det, e->st_pop, e->st_ent, e->bcap, t_begin, e->flags,
+                     e->scap, e->sdet, e->spop, e->sflag, e->d_stats);
   hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, e->stream, e->tile_off, nt, e->d_stats);
   return FQ_OK;
 }
@@ -1836,19 +1953,20 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->st_det, W * e->bcap);
   rc |= dalloc(&e->st_pop, e->bcap);
   rc |= dalloc(&e->st_ent, e->bcap);
-  rc |= dalloc(&e->st_att, e->bcap);
   rc |= dalloc(&e->tile_cnt, e->bcap / kSpawnTile + 2);
   rc |= dalloc(&e->tile_off, e->bcap / kSpawnTile + 2);
   rc |= dalloc(&e->sdet, W * scap);
   rc |= dalloc(&e->spop, S * scap);
   rc |= dalloc(&e->sflag, scap);
-  rc |= dalloc(&e->sord, scap);
   for (int b = 0; b < 2; ++b) {
     rc |= dalloc(&e->kbuf[b], W * scap);
     rc |= dalloc(&e->vbuf[b], scap);
   }
-  rc |= dalloc(&e->head, scap + 1);
-  rc |= dalloc(&e->head_scan, scap + 1);
+  rc |= dalloc(&e->run_stage, scap);
+  rc |= dalloc(&e->run_tcnt, scap / kRunTile + 2);
+  rc |= dalloc(&e->run_toff, scap / kRunTile + 2);
+  rc |= dalloc(&e->gpop, S * scap);
+  rc |= dalloc(&e->gflag, scap);
   rc |= dalloc(&e->run_start, scap + 1);
   rc |= dalloc(&e->rdet, W * scap);
   rc |= dalloc(&e->rpop, S * scap);
@@ -2069,7 +2187,7 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
   if (replay) {
     hipLaunchKernelGGL(k_replay_walk, dim3(1), dim3(1), 0, st, e->sys, n, cap, e->det[c], e->pop[c], e->kii[c],
                        e->pop_new, tau, shift[0], sh1, eproj_old[0], e1, e->rng_buf, e->rng_pos, e->rng_end, e->scap,
-                       e->sdet, e->spop, e->sflag, e->sord, e->d_stats);
+                       e->sdet, e->spop, e->sflag, e->d_stats);
     FQ_HIP(hipEventRecord(e->pev[1], st));
     FQ_HIP(hipEventRecord(e->pev[2], st));
     FQ_HIP(hipEventRecord(e->pev[3], st));
@@ -2131,7 +2249,7 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
       const size_t lds = gen == fqd::kGenHeatBath ? hb_lds_bytes(e->sys.nel) : 0;
       hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), lds, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
                          e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
-                         e->st_att, e->tile_cnt, nullptr, nullptr, nullptr);
+                         e->tile_cnt, nullptr, nullptr, nullptr);
       if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
       rc = pack_spawned(e, nbt, tb);
       if (rc != FQ_OK) return rc;
@@ -2198,22 +2316,17 @@ int fq_annihilate(fq_engine* e, const uint64_t* sstates, const int64_t* spops, c
   const int64_t n = e->n, cap = e->cap;
   std::vector<uint64_t> dw(static_cast<size_t>(W) * std::max<int64_t>(ns, 1));
   std::vector<int64_t> dp(static_cast<size_t>(S) * std::max<int64_t>(ns, 1)), df(std::max<int64_t>(ns, 1));
-  std::vector<uint64_t> dord(std::max<int64_t>(ns, 1));
   for (int64_t i = 0; i < ns; ++i) {
     for (int w = 0; w < W; ++w) dw[w * ns + i] = sstates[i * W + w];
     for (int s = 0; s < S; ++s) dp[s * ns + i] = spops[i * S + s];
     df[i] = (sflags && e->par.initiator) ? sflags[i] : 0;
-    dord[i] = static_cast<uint64_t>(i);
   }
   hipStream_t st = e->stream;
   for (int w = 0; w < W && ns > 0; ++w)
     FQ_HIP(hipMemcpyAsync(e->sdet + w * e->scap, dw.data() + w * ns, 8 * ns, hipMemcpyHostToDevice, st));
   for (int s = 0; s < S && ns > 0; ++s)
     FQ_HIP(hipMemcpyAsync(e->spop + s * e->scap, dp.data() + s * ns, 8 * ns, hipMemcpyHostToDevice, st));
-  if (ns > 0) {
-    FQ_HIP(hipMemcpyAsync(e->sflag, df.data(), 8 * ns, hipMemcpyHostToDevice, st));
-    FQ_HIP(hipMemcpyAsync(e->sord, dord.data(), 8 * ns, hipMemcpyHostToDevice, st));
-  }
+  if (ns > 0) FQ_HIP(hipMemcpyAsync(e->sflag, df.data(), 8 * ns, hipMemcpyHostToDevice, st));
   // post-death populations = current populations
   for (int s = 0; s < S && n > 0; ++s)
     FQ_HIP(hipMemcpyAsync(e->pop_new + s * cap, e->pop[e->cur] + s * cap, 8 * n, hipMemcpyDeviceToDevice, st));
@@ -2398,7 +2511,7 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
     auto kern = e->sys.system == FQ_SYS_HUB_K ? k_spawn<fqd::kGenHubK, true> : k_spawn<fqd::kGenMolRenorm, true>;
     hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
                        e->pop[c], 0.0, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
-                       e->st_att, e->tile_cnt, e->d_sys_arr, e->d_iatt, e->d_ipar);
+                       e->tile_cnt, e->d_sys_arr, e->d_iatt, e->d_ipar);
     if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
     rc = pack_spawned(e, nbt, tb);
     if (rc != FQ_OK) return rc;
@@ -2655,17 +2768,18 @@ void fq_destroy(fq_engine* e) {
   dfree(e->st_det);
   dfree(e->st_pop);
   dfree(e->st_ent);
-  dfree(e->st_att);
   dfree(e->tile_cnt);
   dfree(e->tile_off);
   dfree(e->sdet);
   dfree(e->spop);
   dfree(e->sflag);
-  dfree(e->sord);
   dfree(e->hist);
   dfree(e->hist_scan);
-  dfree(e->head);
-  dfree(e->head_scan);
+  dfree(e->run_stage);
+  dfree(e->run_tcnt);
+  dfree(e->run_toff);
+  dfree(e->gpop);
+  dfree(e->gflag);
   dfree(e->run_start);
   dfree(e->rdet);
   dfree(e->rpop);

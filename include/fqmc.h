@@ -129,7 +129,13 @@ typedef struct fq_engine fq_engine;
 /* Create an engine on params->device and upload the system tables.
  * Replaces the allocation in init_psip_list/init_spawn_store
  * (src/qmc.F90:773,1418) and the table set-up read by the procedure pointers
- * bound in init_proc_pointers (src/qmc.F90:217-703). */
+ * bound in init_proc_pointers (src/qmc.F90:217-703).
+ * FQ_ERR_ARG for an inconsistent request, including the heat-bath or learned
+ * generator on a system other than read_in or with more than 32 electrons,
+ * and the heat-bath generator when its per-attempt weights (8 bytes per
+ * electron per thread of a 256-thread workgroup) do not fit the device's
+ * workgroup LDS; FQ_ERR_BIASED when the heat-bath tables cannot reach some
+ * single excitation (the reference stops there). */
 int fq_create(const fq_system *sys, const fq_params *params, fq_engine **out);
 
 /* Load a sorted walker list (initial_distribution, src/qmc.F90:1507, or a

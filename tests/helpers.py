@@ -247,3 +247,25 @@ def production_lockstep(system, p, ncycles, init_pop, shift=0.0, tau=0.01, label
     finally:
         lib.fq_destroy(eng)
         orc.orc_destroy(h)
+
+
+def replicate_estimates(system, q, seeds, skip=150):
+    """Independent production runs (one per Philox seed) of the same deck:
+    each gives the projected energy as the ratio of the production window's
+    means of sum H_0j N_j and N_0, and the mean shift; the runs are
+    independent, so the spread of these per-run values gives an honest
+    standard error however long the population-control correlation time is
+    (a single short run often has no reblocking plateau).  Returns
+    (E mean, its standard error, shift mean, its standard error), energies
+    including H00."""
+    es, ss = [], []
+    for sd in seeds:
+        qq = dict(q)
+        qq["rng_seed"] = sd
+        tab = api.fciqmc(system, qq, rng_mode="philox")
+        prod = tab[skip:]
+        es.append(system.H00 + prod[:, 2].mean() / prod[:, 3].mean())
+        ss.append(system.H00 + prod[:, 1].mean())
+    es, ss = np.array(es), np.array(ss)
+    k = len(seeds)
+    return es.mean(), es.std(ddof=1) / np.sqrt(k), ss.mean(), ss.std(ddof=1) / np.sqrt(k)

@@ -139,10 +139,13 @@ def _random_keys(rng, nbits, nel, count):
     return rng.permutation(out)[:count]
 
 
-def test_annihilate_production_large_lists():
+@pytest.mark.parametrize("nmain,nspawn", [(600_000, 1_200_000), (600_000, 3_000)])
+def test_annihilate_production_large_lists(nmain, nspawn):
     """Lists of 1e6 entries: every prefix sum spans many tiles and several
-    chunks of the tile-sum scan; radix sort over 32-bit keys in 4 passes."""
-    print("[annihilate] production, 6e5 main + 1.2e6 spawned", flush=True)
+    chunks of the tile-sum scan; radix sort over 32-bit keys in 4 passes.
+    With few spawns the main-list window of a merge block is too wide to stage
+    in LDS (searched in global memory); with many it is staged."""
+    print(f"[annihilate] production, {nmain} main + {nspawn} spawned", flush=True)
     lib, orc = N.lib(), oracle()
     system = api.hubbard_k(electrons=10, lattice=[[4, 0], [0, 4]], ms=0, sym="tot_sym", U=4)
     rng = np.random.default_rng(21)
@@ -150,7 +153,6 @@ def test_annihilate_production_large_lists():
     p.main_capacity = 1 << 22
     p.spawn_capacity = 1 << 22
     rf = 1 << p.pop_bit_shift
-    nmain, nspawn = 600_000, 1_200_000
     keys = _random_keys(rng, 32, 10, nmain + nspawn // 2)
     main = np.sort(keys[:nmain])
     pops = (rng.integers(1, 20, nmain) * rng.choice([-1, 1], nmain) * (3 * rf)).astype(np.int64)

@@ -7,9 +7,10 @@ src/excit_gen_heat_bath_mol.F90).
   system with real_64 populations and initiators.
 * A system whose heat-bath tables cannot reach every allowed single
   excitation is rejected at creation, as the reference stops there.
-* Production mode: the reblocked projected energy of a synthetic system agrees
-  with its exact ground-state energy (dense diagonalisation) within 3 sigma,
-  for the heat-bath and the renormalised generators.  The system is chosen
+* Production mode: the mean projected energy of eight independent runs of a
+  synthetic system agrees with its exact ground-state energy (dense
+  diagonalisation) within 4 standard errors, for the heat-bath and the
+  renormalised generators.  The system is chosen
   with its ground state 95% on the reference determinant: the synthetic
   integrals are not Hartree--Fock canonical, so single excitations enter the
   projected energy, and a ground state orthogonal to the reference (which
@@ -20,9 +21,9 @@ import os
 import pytest
 
 import exact_small
-from hande_amd import api, reblock, synthetic
+from hande_amd import api, synthetic
 from hande_amd import _native as N
-from helpers import deck, engine_params, lockstep
+from helpers import deck, engine_params, lockstep, replicate_estimates
 
 pytestmark = pytest.mark.gpu
 
@@ -61,31 +62,29 @@ def test_heat_bath_biased_system_rejected(tmp_path):
 
 @pytest.mark.parametrize("excit_gen", ["heat_bath", "renorm"])
 def test_molecular_projected_energy_matches_exact(tmp_path, excit_gen):
+    """Production mode: eight independent runs (Philox seeds); their mean
+    projected energy and shift within 4 standard errors (t, 7 degrees of
+    freedom) of the dense diagonalisation."""
     norb, nel, seed = 8, 4, 600
     exact, _hf, weight = exact_small.ground_state(norb, nel, seed, with_weight=True)
     assert weight > 0.5  # E_proj needs a ground state with a large reference weight
     path = synthetic.write_fcidump(os.path.join(tmp_path, "FCIDUMP"), norb, nel, seed)
     system = api.read_in(int_file=path, nel=nel, ms=0)
     print(f"[production] synthetic {norb}o{nel}e {excit_gen}", flush=True)
-    q = dict(tau=0.01, rng_seed=99, init_pop=10, mc_cycles=20, nreports=600, target_population=2000,
+    q = dict(tau=0.01, init_pop=10, mc_cycles=20, nreports=400, target_population=2000,
              shift_damping=0.05, state_size=100000, spawned_state_size=100000, excit_gen=excit_gen)
-    tab = api.fciqmc(system, q, rng_mode="philox")
-    prod = tab[150:]
-    out = reblock.projected_energy(prod[:, 2], prod[:, 3])
-    e = system.H00 + out["energy"]
-    sh = reblock.mean_error(prod[:, 1])
-    print(f"  E_proj={e:.6f} +- {out['error']:.6f} (level {out['level']}), "
-          f"shift={system.H00 + sh['mean']:.6f} +- {sh['error']:.6f}, exact {exact:.6f}", flush=True)
-    assert out["converged"] and sh["converged"], "reblocking found no optimal block"
-    assert abs(e - exact) < 3 * out["error"] + 1e-4
-    assert abs(system.H00 + sh["mean"] - exact) < 3 * sh["error"] + 5e-3
+    e, se, sh, sse = replicate_estimates(system, q, range(91, 99))
+    print(f"  E_proj={e:.6f} +- {se:.6f}, shift={sh:.6f} +- {sse:.6f}, exact {exact:.6f}", flush=True)
+    assert abs(e - exact) < 4 * se + 1e-4
+    assert abs(sh - exact) < 4 * sse + 5e-3
 
 
 @pytest.mark.parametrize("excit_gen", ["renorm", "heat_bath"])
 def test_two_word_projected_energy_matches_exact(tmp_path, excit_gen):
     """Production mode with determinants of two 64-bit words (34 spatial
     orbitals, 2 electrons; weakly correlated integrals so the ground state is
-    89% on the reference): E_proj within 3 sigma of the dense diagonalisation."""
+    89% on the reference): the mean E_proj of eight independent runs within
+    4 standard errors of the dense diagonalisation."""
     norb, nel, seed, cs = 34, 2, 3400, 0.03
     exact, _hf, weight = exact_small.ground_state(norb, nel, seed, with_weight=True, chol_scale=cs)
     assert weight > 0.5
@@ -93,12 +92,8 @@ def test_two_word_projected_energy_matches_exact(tmp_path, excit_gen):
     system = api.read_in(int_file=path, nel=nel, ms=0)
     assert system.view.nwords == 2
     print(f"[production] synthetic {norb}o{nel}e W=2 {excit_gen}", flush=True)
-    q = dict(tau=0.01, rng_seed=31, init_pop=10, mc_cycles=20, nreports=1500, target_population=3000,
+    q = dict(tau=0.01, init_pop=10, mc_cycles=20, nreports=400, target_population=3000,
              shift_damping=0.05, state_size=100000, spawned_state_size=100000, excit_gen=excit_gen)
-    tab = api.fciqmc(system, q, rng_mode="philox")
-    prod = tab[150:]
-    out = reblock.projected_energy(prod[:, 2], prod[:, 3])
-    e = system.H00 + out["energy"]
-    print(f"  E_proj={e:.6f} +- {out['error']:.6f} (level {out['level']}), exact {exact:.6f}", flush=True)
-    assert out["converged"], "reblocking found no optimal block"
-    assert abs(e - exact) < 3 * out["error"] + 1e-4
+    e, se, _sh, _sse = replicate_estimates(system, q, range(31, 39))
+    print(f"  E_proj={e:.6f} +- {se:.6f} (8 runs), exact {exact:.6f}", flush=True)
+    assert abs(e - exact) < 4 * se + 1e-4
