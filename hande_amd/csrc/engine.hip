@@ -223,8 +223,9 @@ struct fq_engine {
   int64_t* nat = nullptr;       // S*cap attempts per (det, space), then scanned
   int64_t* att_off = nullptr;   // S*cap+1
   int64_t* flags = nullptr;     // cap parent initiator flags
-  int64_t* msurv = nullptr;     // cap+1
-  int64_t* mrank = nullptr;     // cap+1
+  uint64_t* mmask = nullptr;    // cap/64+2: surviving main entries, one bit each
+  int64_t* mcnt = nullptr;      // survivors per 64-entry word
+  int64_t* moff = nullptr;      // their exclusive prefix
   int64_t* ins = nullptr;       // cap+2
   int64_t* ins_scan = nullptr;  // cap+2
   int64_t* mdraw = nullptr;     // cap+1 (replay rounding draws)
@@ -261,8 +262,9 @@ struct fq_engine {
   int64_t* rflag = nullptr;      // scap
   int64_t* rpos = nullptr;       // scap
   int64_t* rstate = nullptr;     // scap: 0 dead, 1 hit, 2 new candidate
-  int64_t* nsurv = nullptr;      // scap+1
-  int64_t* nrank = nullptr;      // scap+1
+  uint64_t* nmask = nullptr;     // scap/64+2: surviving new runs, one bit each
+  int64_t* ncnt = nullptr;       // survivors per 64-run word
+  int64_t* noff = nullptr;       // their exclusive prefix
   int64_t* ndraw = nullptr;      // scap+1
   int64_t* ndraw_off = nullptr;
   // scan scratch
@@ -1308,51 +1310,76 @@ __global__ void k_count_round_new(Sys s, int64_t nruns, const int64_t* __restric
   ndraw[r] = (s.real_amps && rstate[r] == 2) ? round_draws(rpop, scap, r, s.S, s.rf) : 0;
 }
 
-// remove_unoccupied_dets: round small real populations, mark survivors.
-__global__ void k_round_main(Sys s, int64_t n, int64_t cap, int64_t* __restrict__ pop_new, int replay,
-                             const double* __restrict__ rbuf, int64_t rbase, const int64_t* __restrict__ mdraw_off,
-                             uint64_t seed, uint32_t cyc, int64_t* __restrict__ msurv) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  bool alive = false;
-  if (s.real_amps) {
-    if (replay) {
-      fqd::ReplayRng rng{rbuf, rbase + mdraw_off[i], rbase + mdraw_off[i + 1], false};
-      for (int sp = 0; sp < s.S; ++sp) pop_new[sp * cap + i] = fqd::stoch_round_pop(rng, pop_new[sp * cap + i], s.rf);
-    } else {
-      fqd::Philox rng(seed, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32),
-                      cyc ^ (fqd::kDrawRoundMain << 24));
-      for (int sp = 0; sp < s.S; ++sp) pop_new[sp * cap + i] = fqd::stoch_round_pop(rng, pop_new[sp * cap + i], s.rf);
-    }
+// Survivor flags are kept as bits, one ballot word per 64 consecutive
+// entries, with each word's count; the rank of entry p among survivors is
+// off[p >> 6] + the word's bits below p (off: exclusive prefix of the counts).
+__device__ __forceinline__ bool bit_get(const uint64_t* __restrict__ mask, int64_t p) {
+  return (mask[p >> 6] >> (p & 63)) & 1;
+}
+__device__ __forceinline__ int64_t bit_rank(const int64_t* __restrict__ off, const uint64_t* __restrict__ mask,
+                                            int64_t p) {
+  const int b = static_cast<int>(p & 63);
+  return off[p >> 6] + (b ? __popcll(mask[p >> 6] & ((1ull << b) - 1)) : 0);
+}
+// All lanes of a wavefront call this; its 64 entries start at a multiple of 64.
+__device__ __forceinline__ void store_ballot(bool alive, int64_t i, int64_t n, uint64_t* __restrict__ mask,
+                                             int64_t* __restrict__ cnt) {
+  const unsigned long long bal = __ballot(alive);
+  if ((threadIdx.x & 63) == 0 && i < n) {
+    mask[i >> 6] = bal;
+    cnt[i >> 6] = __popcll(bal);
   }
-  for (int sp = 0; sp < s.S; ++sp) alive = alive || pop_new[sp * cap + i] != 0;
-  msurv[i] = alive ? 1 : 0;
+}
+
+// remove_unoccupied_dets: round small real populations, mark survivors.
+__global__ __launch_bounds__(kBlock) void k_round_main(Sys s, int64_t n, int64_t cap, int64_t* __restrict__ pop_new,
+                                                       int replay, const double* __restrict__ rbuf, int64_t rbase,
+                                                       const int64_t* __restrict__ mdraw_off, uint64_t seed,
+                                                       uint32_t cyc, uint64_t* __restrict__ mmask,
+                                                       int64_t* __restrict__ mcnt) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool alive = false;
+  if (i < n) {
+    if (s.real_amps) {
+      if (replay) {
+        fqd::ReplayRng rng{rbuf, rbase + mdraw_off[i], rbase + mdraw_off[i + 1], false};
+        for (int sp = 0; sp < s.S; ++sp)
+          pop_new[sp * cap + i] = fqd::stoch_round_pop(rng, pop_new[sp * cap + i], s.rf);
+      } else {
+        fqd::Philox rng(seed, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32),
+                        cyc ^ (fqd::kDrawRoundMain << 24));
+        for (int sp = 0; sp < s.S; ++sp)
+          pop_new[sp * cap + i] = fqd::stoch_round_pop(rng, pop_new[sp * cap + i], s.rf);
+      }
+    }
+    for (int sp = 0; sp < s.S; ++sp) alive = alive || pop_new[sp * cap + i] != 0;
+  }
+  store_ballot(alive, i, n, mmask, mcnt);
 }
 
 // round_low_population_spawns for new-determinant candidates.
-__global__ void k_round_new(Sys s, int64_t nruns, int64_t* __restrict__ rpop, const int64_t* __restrict__ rstate,
-                            int64_t scap, int replay, const double* __restrict__ rbuf, int64_t rbase,
-                            const int64_t* __restrict__ ndraw_off, uint64_t seed, uint32_t cyc,
-                            int64_t* __restrict__ nsurv) {
+__global__ __launch_bounds__(kBlock) void k_round_new(Sys s, int64_t nruns, int64_t* __restrict__ rpop,
+                                                      const int64_t* __restrict__ rstate, int64_t scap, int replay,
+                                                      const double* __restrict__ rbuf, int64_t rbase,
+                                                      const int64_t* __restrict__ ndraw_off, uint64_t seed,
+                                                      uint32_t cyc, uint64_t* __restrict__ nmask,
+                                                      int64_t* __restrict__ ncnt) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= nruns) return;
-  if (rstate[r] != 2) {
-    nsurv[r] = 0;
-    return;
-  }
-  if (s.real_amps) {
-    if (replay) {
-      fqd::ReplayRng rng{rbuf, rbase + ndraw_off[r], rbase + ndraw_off[r + 1], false};
-      for (int sp = 0; sp < s.S; ++sp) rpop[sp * scap + r] = fqd::stoch_round_pop(rng, rpop[sp * scap + r], s.rf);
-    } else {
-      fqd::Philox rng(seed, static_cast<uint32_t>(r), static_cast<uint32_t>(r >> 32),
-                      cyc ^ (fqd::kDrawRoundSpawn << 24));
-      for (int sp = 0; sp < s.S; ++sp) rpop[sp * scap + r] = fqd::stoch_round_pop(rng, rpop[sp * scap + r], s.rf);
-    }
-  }
   bool alive = false;
-  for (int sp = 0; sp < s.S; ++sp) alive = alive || rpop[sp * scap + r] != 0;
-  nsurv[r] = alive ? 1 : 0;
+  if (r < nruns && rstate[r] == 2) {
+    if (s.real_amps) {
+      if (replay) {
+        fqd::ReplayRng rng{rbuf, rbase + ndraw_off[r], rbase + ndraw_off[r + 1], false};
+        for (int sp = 0; sp < s.S; ++sp) rpop[sp * scap + r] = fqd::stoch_round_pop(rng, rpop[sp * scap + r], s.rf);
+      } else {
+        fqd::Philox rng(seed, static_cast<uint32_t>(r), static_cast<uint32_t>(r >> 32),
+                        cyc ^ (fqd::kDrawRoundSpawn << 24));
+        for (int sp = 0; sp < s.S; ++sp) rpop[sp * scap + r] = fqd::stoch_round_pop(rng, rpop[sp * scap + r], s.rf);
+      }
+    }
+    for (int sp = 0; sp < s.S; ++sp) alive = alive || rpop[sp * scap + r] != 0;
+  }
+  store_ballot(alive, r, nruns, nmask, ncnt);
 }
 
 __global__ void k_zero(int64_t* __restrict__ p, int64_t n) {
@@ -1360,18 +1387,19 @@ __global__ void k_zero(int64_t* __restrict__ p, int64_t n) {
   if (i < n) p[i] = 0;
 }
 
-__global__ void k_ins_count(int64_t nruns, const int64_t* __restrict__ nsurv, const int64_t* __restrict__ rpos,
+__global__ void k_ins_count(int64_t nruns, const uint64_t* __restrict__ nmask, const int64_t* __restrict__ rpos,
                             int64_t* __restrict__ ins) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= nruns || !nsurv[r]) return;
+  if (r >= nruns || !bit_get(nmask, r)) return;
   atomicAdd(reinterpret_cast<unsigned long long*>(&ins[rpos[r]]), 1ull);
 }
 
 // Decide the new list length; insertion is skipped when the main list would
 // overflow (insert_new_walkers' psip_list%error path).
-__global__ void k_finalize_counts(int64_t n, int64_t nruns, const int64_t* __restrict__ mrank,
-                                  const int64_t* __restrict__ nrank, int64_t cap, Stats* __restrict__ st) {
-  const int64_t nmain = mrank[n], nnew = nrank[nruns];
+__global__ void k_finalize_counts(int64_t n, int64_t nruns, const int64_t* __restrict__ moff,
+                                  const uint64_t* __restrict__ mmask, const int64_t* __restrict__ noff,
+                                  const uint64_t* __restrict__ nmask, int64_t cap, Stats* __restrict__ st) {
+  const int64_t nmain = bit_rank(moff, mmask, n), nnew = bit_rank(noff, nmask, nruns);
   if (nmain + nnew > cap) {
     st->main_overflow = 1;
     st->new_n = nmain;
@@ -1382,13 +1410,13 @@ __global__ void k_finalize_counts(int64_t n, int64_t nruns, const int64_t* __res
 
 __global__ void k_scatter_main(Sys s, int64_t n, int64_t cap, const uint64_t* __restrict__ det,
                                const int64_t* __restrict__ pop_new, const double* __restrict__ kii,
-                               const int64_t* __restrict__ msurv, const int64_t* __restrict__ mrank,
+                               const uint64_t* __restrict__ mmask, const int64_t* __restrict__ moff,
                                const int64_t* __restrict__ ins_scan, const Stats* __restrict__ st,
                                uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
                                double* __restrict__ kii_out) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n || !msurv[i]) return;
-  const int64_t o = mrank[i] + (st->main_overflow ? 0 : ins_scan[i + 1]);
+  if (i >= n || !bit_get(mmask, i)) return;
+  const int64_t o = bit_rank(moff, mmask, i) + (st->main_overflow ? 0 : ins_scan[i + 1]);
   for (int w = 0; w < s.W; ++w) det_out[w * cap + o] = det[w * cap + i];
   for (int sp = 0; sp < s.S; ++sp) pop_out[sp * cap + o] = pop_new[sp * cap + i];
   kii_out[o] = kii[i];
@@ -1396,13 +1424,14 @@ __global__ void k_scatter_main(Sys s, int64_t n, int64_t cap, const uint64_t* __
 
 __global__ void k_scatter_new(Sys s, int64_t nruns, int64_t cap, int64_t scap, const uint64_t* __restrict__ rdet,
                               const int64_t* __restrict__ rpop, const int64_t* __restrict__ rpos,
-                              const int64_t* __restrict__ nsurv, const int64_t* __restrict__ nrank,
-                              const int64_t* __restrict__ mrank, const Stats* __restrict__ st,
+                              const uint64_t* __restrict__ nmask, const int64_t* __restrict__ noff,
+                              const uint64_t* __restrict__ mmask, const int64_t* __restrict__ moff,
+                              const Stats* __restrict__ st,
                               uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
                               double* __restrict__ kii_out, const Sys* __restrict__ sys_arr) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= nruns || !nsurv[r] || st->main_overflow) return;
-  const int64_t o = nrank[r] + mrank[rpos[r]];
+  if (r >= nruns || !bit_get(nmask, r) || st->main_overflow) return;
+  const int64_t o = bit_rank(noff, nmask, r) + bit_rank(moff, mmask, rpos[r]);
   uint64_t f[FQ_MAX_WORDS] = {0, 0};
   FQ_FOR_WORDS(w, s.W) {
     f[w] = rdet[w * scap + r];
@@ -1595,31 +1624,34 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
   }
   const uint64_t seed = e->par.philox_seed;
   hipLaunchKernelGGL(k_round_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->pop_new,
-                     replay ? 1 : 0, e->rng_buf, rng_pos, e->mdraw_off, seed, cyc, e->msurv);
+                     replay ? 1 : 0, e->rng_buf, rng_pos, e->mdraw_off, seed, cyc, e->mmask, e->mcnt);
   if (nruns > 0)
     hipLaunchKernelGGL(k_round_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, e->rpop, e->rstate,
-                       scap, replay ? 1 : 0, e->rng_buf, rng_pos + main_draws, e->ndraw_off, seed, cyc, e->nsurv);
+                       scap, replay ? 1 : 0, e->rng_buf, rng_pos + main_draws, e->ndraw_off, seed, cyc, e->nmask,
+                       e->ncnt);
   rng_pos += main_draws + new_draws;
-  rc = scan_exclusive(e, e->msurv, e->mrank, n);
+  rc = scan_exclusive(e, e->mcnt, e->moff, ceil_div(n, 64));
   if (rc != FQ_OK) return rc;
   if (nruns > 0) {
-    rc = scan_exclusive(e, e->nsurv, e->nrank, nruns);
+    rc = scan_exclusive(e, e->ncnt, e->noff, ceil_div(nruns, 64));
     if (rc != FQ_OK) return rc;
   } else {
-    hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, e->nrank, 1);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, e->noff, 1);
   }
   hipLaunchKernelGGL(k_zero, dim3(grid_for(n + 1)), dim3(kBlock), 0, st, e->ins, n + 1);
   if (nruns > 0)
-    hipLaunchKernelGGL(k_ins_count, dim3(grid_for(nruns)), dim3(kBlock), 0, st, nruns, e->nsurv, e->rpos, e->ins);
+    hipLaunchKernelGGL(k_ins_count, dim3(grid_for(nruns)), dim3(kBlock), 0, st, nruns, e->nmask, e->rpos, e->ins);
   rc = scan_exclusive(e, e->ins, e->ins_scan, n + 1);
   if (rc != FQ_OK) return rc;
-  hipLaunchKernelGGL(k_finalize_counts, dim3(1), dim3(1), 0, st, n, nruns, e->mrank, e->nrank, cap, e->d_stats);
+  hipLaunchKernelGGL(k_finalize_counts, dim3(1), dim3(1), 0, st, n, nruns, e->moff, e->mmask, e->noff, e->nmask, cap,
+                     e->d_stats);
   const int o = 1 - c;
   hipLaunchKernelGGL(k_scatter_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->det[c], e->pop_new,
-                     e->kii[c], e->msurv, e->mrank, e->ins_scan, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
+                     e->kii[c], e->mmask, e->moff, e->ins_scan, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
   if (nruns > 0)
     hipLaunchKernelGGL(k_scatter_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, cap, scap, e->rdet,
-                       e->rpop, e->rpos, e->nsurv, e->nrank, e->mrank, e->d_stats, e->det[o], e->pop[o], e->kii[o],
+                       e->rpop, e->rpos, e->nmask, e->noff, e->mmask, e->moff, e->d_stats, e->det[o], e->pop[o],
+                       e->kii[o],
                        e->batched ? e->d_sys_arr : nullptr);
   FQ_HIP(hipEventRecord(e->pev[8], st));
   rc = sync_stats(e);
@@ -1941,8 +1973,9 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->nat, S * cap + 1);  // det-major (det, space) entries
   rc |= dalloc(&e->att_off, S * cap + 1);
   rc |= dalloc(&e->flags, cap);
-  rc |= dalloc(&e->msurv, cap + 1);
-  rc |= dalloc(&e->mrank, cap + 1);
+  rc |= dalloc(&e->mmask, cap / 64 + 2);
+  rc |= dalloc(&e->mcnt, cap / 64 + 2);
+  rc |= dalloc(&e->moff, cap / 64 + 2);
   rc |= dalloc(&e->ins, cap + 2);
   rc |= dalloc(&e->ins_scan, cap + 2);
   rc |= dalloc(&e->mdraw, cap + 1);
@@ -1973,8 +2006,9 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->rflag, scap);
   rc |= dalloc(&e->rpos, scap);
   rc |= dalloc(&e->rstate, scap);
-  rc |= dalloc(&e->nsurv, scap + 1);
-  rc |= dalloc(&e->nrank, scap + 1);
+  rc |= dalloc(&e->nmask, scap / 64 + 2);
+  rc |= dalloc(&e->ncnt, scap / 64 + 2);
+  rc |= dalloc(&e->noff, scap / 64 + 2);
   rc |= dalloc(&e->ndraw, scap + 1);
   rc |= dalloc(&e->ndraw_off, scap + 1);
   rc |= dalloc(&e->d_stats, 1);
@@ -2757,8 +2791,9 @@ void fq_destroy(fq_engine* e) {
   dfree(e->nat);
   dfree(e->att_off);
   dfree(e->flags);
-  dfree(e->msurv);
-  dfree(e->mrank);
+  dfree(e->mmask);
+  dfree(e->mcnt);
+  dfree(e->moff);
   dfree(e->ins);
   dfree(e->ins_scan);
   dfree(e->mdraw);
@@ -2786,8 +2821,9 @@ void fq_destroy(fq_engine* e) {
   dfree(e->rflag);
   dfree(e->rpos);
   dfree(e->rstate);
-  dfree(e->nsurv);
-  dfree(e->nrank);
+  dfree(e->nmask);
+  dfree(e->ncnt);
+  dfree(e->noff);
   dfree(e->ndraw);
   dfree(e->ndraw_off);
   for (auto*& p : e->scan_tmp) dfree(p);
