@@ -47,6 +47,8 @@ struct Sys {
   const double* eri;
   const double* jmat;  // norb*norb: (pp|qq) = eri(i,j,i,j) for spatial p,q
   const double* kmat;  // norb*norb: (pq|qp) = eri(i,j,j,i)
+  const double* s1j;   // norb^3: [p][a][o] = (pa|oo), the Coulomb terms of a single p -> a
+  const double* s1k;   // norb^3: [p][a][o] = (po|oa), its exchange terms
   double psingle, pdouble;
   // Heat-bath tables (excit_gen FQ_EXCIT_HEAT_BATH), i-major over 1-based spin
   // orbitals: hb_ija[hb3(i,j,a)] = hb_ija%weights(a,j,i), hb_ija_tot[hb2(i,j)] =
@@ -316,16 +318,22 @@ __device__ __forceinline__ double eri_nz(const Sys& s, int i, int j, int a, int 
   const int64_t ia = tri((ii + 1) / 2, (aa + 1) / 2), jb = tri((jj + 1) / 2, (bb + 1) / 2);
   return s.eri[(ia < jb ? tri(jb, ia) : tri(ia, jb)) - 1];
 }
+// slater_condon1_mol_excit: the terms <io|ao> and <io|oa> are read from the
+// (i, a) rows of the single-excitation tables (the packed store's values).
 __device__ inline double sc1_excit(const Sys& s, const uint64_t* f, int i, int a, bool perm) {
   double h = h1g(s, i, a);
+  const int64_t row = (static_cast<int64_t>((i + 1) / 2 - 1) * s.norb + ((a + 1) / 2 - 1)) * s.norb;
+  const double* __restrict__ rj = s.s1j + row;
+  const double* __restrict__ rk = s.s1k + row;
   FQ_FOR_WORDS(w, s.W) {
     uint64_t x = f[w];
     while (x) {
       const int o = (w << 6) + __builtin_ctzll(x) + 1;
       x &= x - 1;
       if (o != i) {
-        h = h + eri_nz(s, i, o, a, o);
-        if (ms_of(o) == ms_of(i)) h = h - eri_nz(s, i, o, o, a);
+        const int po = (o + 1) / 2 - 1;
+        h = h + rj[po];
+        if (ms_of(o) == ms_of(i)) h = h - rk[po];
       }
     }
   }

@@ -185,6 +185,7 @@ struct fq_engine {
   // device tables
   int *d_sym_table = nullptr, *d_inv_sym = nullptr, *d_orb_sym = nullptr, *d_nbss = nullptr, *d_ssbf = nullptr;
   double *d_sp_eigv = nullptr, *d_h1 = nullptr, *d_eri = nullptr, *d_jmat = nullptr, *d_kmat = nullptr;
+  double *d_s1j = nullptr, *d_s1k = nullptr;
   // heat-bath tables (excit_gen FQ_EXCIT_HEAT_BATH)
   double *d_hb_i = nullptr, *d_hb_ij = nullptr, *d_hb_ija = nullptr, *d_hb_ija_tot = nullptr, *d_hb_ija_u = nullptr;
   double *d_hb_ijab = nullptr, *d_hb_ijab_tot = nullptr, *d_hb_ijab_u = nullptr;
@@ -2127,8 +2128,19 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
         J[(pp - 1) * no + (qq - 1)] = sys->eri[idx(pp, pp, qq, qq)];  // <ij|ij> = (ii|jj)
         K[(pp - 1) * no + (qq - 1)] = sys->eri[idx(pp, qq, qq, pp)];  // <ij|ji> = (ij|ji)
       }
+    // Single-excitation tables: (pa|oo) and (po|oa) for every spatial p, a, o.
+    std::vector<double> S1J(static_cast<size_t>(no) * no * no), S1K(S1J.size());
+    for (int pp = 1; pp <= no; ++pp)
+      for (int aa = 1; aa <= no; ++aa)
+        for (int oo = 1; oo <= no; ++oo) {
+          const size_t t = (static_cast<size_t>(pp - 1) * no + (aa - 1)) * no + (oo - 1);
+          S1J[t] = sys->eri[idx(pp, aa, oo, oo)];
+          S1K[t] = sys->eri[idx(pp, oo, oo, aa)];
+        }
     if (rc == FQ_OK) rc = upload(&e->d_jmat, J.data(), static_cast<int64_t>(J.size()), e->stream);
     if (rc == FQ_OK) rc = upload(&e->d_kmat, K.data(), static_cast<int64_t>(K.size()), e->stream);
+    if (rc == FQ_OK) rc = upload(&e->d_s1j, S1J.data(), static_cast<int64_t>(S1J.size()), e->stream);
+    if (rc == FQ_OK) rc = upload(&e->d_s1k, S1K.data(), static_cast<int64_t>(S1K.size()), e->stream);
     if (rc == FQ_OK) FQ_HIP(hipStreamSynchronize(e->stream));
   }
   s.sp_eigv = e->d_sp_eigv;
@@ -2141,6 +2153,8 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   s.eri = e->d_eri;
   s.jmat = e->d_jmat;
   s.kmat = e->d_kmat;
+  s.s1j = e->d_s1j;
+  s.s1k = e->d_s1k;
   if (rc == FQ_OK && params->rng_mode == FQ_RNG_PHILOX) rc = setup_rank_keys(e);
   if (rc == FQ_OK && (params->excit_gen == FQ_EXCIT_HEAT_BATH || params->excit_gen == FQ_EXCIT_LEARNED))
     rc = build_heat_bath(e);
@@ -2839,6 +2853,8 @@ void fq_destroy(fq_engine* e) {
   dfree(e->d_eri);
   dfree(e->d_jmat);
   dfree(e->d_kmat);
+  dfree(e->d_s1j);
+  dfree(e->d_s1k);
   dfree(e->d_hb_i);
   dfree(e->d_hb_ij);
   dfree(e->d_hb_ija);
