@@ -309,14 +309,16 @@ __device__ inline double two_e_hub_k(const Sys& s, int p1, int p2, int p3, int p
 // ---------------------------------------------------------------------------
 // Molecular integrals (RHF; spatial index (o+1)/2)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int64_t tri(int64_t i, int64_t j) { return (i * (i - 1)) / 2 + j; }
 __device__ __forceinline__ double h1g(const Sys& s, int i, int j) {
   return s.h1[((i + 1) / 2 - 1) * s.norb + ((j + 1) / 2 - 1)];
 }
+// Packed 8-fold index in 32-bit arithmetic: with at most 128 spin orbitals
+// (64 spatial) pair indices stay below 2081 and packed indices below 2.2e6.
+__device__ __forceinline__ uint32_t tri32(uint32_t i, uint32_t j) { return ((i * (i - 1)) >> 1) + j; }
 __device__ __forceinline__ double eri_nz(const Sys& s, int i, int j, int a, int b) {
   const int ii = i > a ? i : a, aa = i > a ? a : i, jj = j > b ? j : b, bb = j > b ? b : j;
-  const int64_t ia = tri((ii + 1) / 2, (aa + 1) / 2), jb = tri((jj + 1) / 2, (bb + 1) / 2);
-  return s.eri[(ia < jb ? tri(jb, ia) : tri(ia, jb)) - 1];
+  const uint32_t ia = tri32((ii + 1) >> 1, (aa + 1) >> 1), jb = tri32((jj + 1) >> 1, (bb + 1) >> 1);
+  return s.eri[(ia < jb ? tri32(jb, ia) : tri32(ia, jb)) - 1];
 }
 // slater_condon1_mol_excit: the terms <io|ao> and <io|oa> are read from the
 // (i, a) rows of the single-excitation tables (the packed store's values).
@@ -349,6 +351,7 @@ __device__ inline double sc2_excit(const Sys& s, int i, int j, int a, int b, boo
 }
 
 // <D|H|D> in the reference's summation order (slater_condon0_*).
+__device__ inline double diagonal_mol(const Sys& s, const uint64_t* f, const double* J, const double* K);
 __device__ inline double diagonal(const Sys& s, const uint64_t* f) {
   if (s.system == FQ_SYS_HUB_K) {
     double h = static_cast<double>(s.nalpha * s.nbeta) * s.coulomb_k;
@@ -362,6 +365,12 @@ __device__ inline double diagonal(const Sys& s, const uint64_t* f) {
     }
     return h;
   }
+  return diagonal_mol(s, f, s.jmat, s.kmat);
+}
+
+// Molecular <D|H|D> with the dense Coulomb/exchange pair tables J, K (the
+// packed store's values) wherever they are held (global memory or LDS).
+__device__ inline double diagonal_mol(const Sys& s, const uint64_t* f, const double* J, const double* K) {
   // Coulomb and exchange elements come from dense per-pair tables holding the
   // same values as the packed store, in the reference's summation order.
   double h = s.ecore;
@@ -380,8 +389,8 @@ __device__ inline double diagonal(const Sys& s, const uint64_t* f) {
           const int j = (w2 << 6) + __builtin_ctzll(y) + 1;
           y &= y - 1;
           const int pj = (j + 1) / 2 - 1;
-          h = h + s.jmat[pi + pj];
-          if (ms_of(i) == ms_of(j)) h = h - s.kmat[pi + pj];
+          h = h + J[pi + pj];
+          if (ms_of(i) == ms_of(j)) h = h - K[pi + pj];
         }
       }
     }

@@ -263,6 +263,7 @@ struct fq_engine {
   int64_t* rflag = nullptr;      // scap
   int64_t* rpos = nullptr;       // scap
   int64_t* rstate = nullptr;     // scap: 0 dead, 1 hit, 2 new candidate
+  int64_t* nlist = nullptr;      // scap: surviving new runs in order
   uint64_t* nmask = nullptr;     // scap/64+2: surviving new runs, one bit each
   int64_t* ncnt = nullptr;       // survivors per 64-run word
   int64_t* noff = nullptr;       // their exclusive prefix
@@ -1423,16 +1424,45 @@ __global__ void k_scatter_main(Sys s, int64_t n, int64_t cap, const uint64_t* __
   kii_out[o] = kii[i];
 }
 
-__global__ void k_scatter_new(Sys s, int64_t nruns, int64_t cap, int64_t scap, const uint64_t* __restrict__ rdet,
-                              const int64_t* __restrict__ rpop, const int64_t* __restrict__ rpos,
-                              const uint64_t* __restrict__ nmask, const int64_t* __restrict__ noff,
-                              const uint64_t* __restrict__ mmask, const int64_t* __restrict__ moff,
-                              const Stats* __restrict__ st,
-                              uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
-                              double* __restrict__ kii_out, const Sys* __restrict__ sys_arr) {
+// Surviving new runs in order: list[rank among survivors] = run index.
+__global__ void k_new_list(int64_t nruns, const uint64_t* __restrict__ nmask, const int64_t* __restrict__ noff,
+                           int64_t* __restrict__ list) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= nruns || !bit_get(nmask, r) || st->main_overflow) return;
-  const int64_t o = bit_rank(noff, nmask, r) + bit_rank(moff, mmask, rpos[r]);
+  if (r < nruns && bit_get(nmask, r)) list[bit_rank(noff, nmask, r)] = r;
+}
+
+// Insert the surviving new runs, one thread each (list order = rank order),
+// with K_ii from the diagonal rule; a molecular system's dense pair tables are
+// staged in LDS when jk_lds (2 * norb^2 doubles) is nonzero.
+__global__ __launch_bounds__(kBlock) void k_scatter_new(Sys s, int64_t nruns, int64_t cap, int64_t scap,
+                                                        const uint64_t* __restrict__ rdet,
+                                                        const int64_t* __restrict__ rpop,
+                                                        const int64_t* __restrict__ rpos,
+                                                        const uint64_t* __restrict__ nmask,
+                                                        const int64_t* __restrict__ noff,
+                                                        const int64_t* __restrict__ list,
+                                                        const uint64_t* __restrict__ mmask,
+                                                        const int64_t* __restrict__ moff,
+                                                        const Stats* __restrict__ st,
+                                                        uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
+                                                        double* __restrict__ kii_out, const Sys* __restrict__ sys_arr,
+                                                        int jk_lds) {
+  extern __shared__ double jk[];
+  const int64_t nnew = bit_rank(noff, nmask, nruns);
+  const int64_t k0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
+  if (k0 >= nnew || st->main_overflow) return;  // whole block: no surviving run left
+  const int nn = s.norb * s.norb;
+  if (jk_lds) {
+    for (int q = threadIdx.x; q < nn; q += kBlock) {
+      jk[q] = s.jmat[q];
+      jk[nn + q] = s.kmat[q];
+    }
+    __syncthreads();
+  }
+  const int64_t k = k0 + threadIdx.x;
+  if (k >= nnew) return;
+  const int64_t r = list[k];
+  const int64_t o = k + bit_rank(moff, mmask, rpos[r]);
   uint64_t f[FQ_MAX_WORDS] = {0, 0};
   FQ_FOR_WORDS(w, s.W) {
     f[w] = rdet[w * scap + r];
@@ -1443,6 +1473,8 @@ __global__ void k_scatter_new(Sys s, int64_t nruns, int64_t cap, int64_t scap, c
     const Sys& si = sys_arr[f[0] >> kInstShift];
     f[0] &= kOrbMask;
     kii_out[o] = fqd::diagonal(si, f) - si.H00;
+  } else if (jk_lds) {
+    kii_out[o] = fqd::diagonal_mol(s, f, jk, jk + nn) - s.H00;
   } else {
     kii_out[o] = fqd::diagonal(s, f) - s.H00;
   }
@@ -1649,11 +1681,17 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
   const int o = 1 - c;
   hipLaunchKernelGGL(k_scatter_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->det[c], e->pop_new,
                      e->kii[c], e->mmask, e->moff, e->ins_scan, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
-  if (nruns > 0)
-    hipLaunchKernelGGL(k_scatter_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, cap, scap, e->rdet,
-                       e->rpop, e->rpos, e->nmask, e->noff, e->mmask, e->moff, e->d_stats, e->det[o], e->pop[o],
-                       e->kii[o],
-                       e->batched ? e->d_sys_arr : nullptr);
+  if (nruns > 0) {
+    hipLaunchKernelGGL(k_new_list, dim3(grid_for(nruns)), dim3(kBlock), 0, st, nruns, e->nmask, e->noff, e->nlist);
+    // dense pair tables in LDS for the diagonal when they fit in 32 KB
+    const size_t jk = e->sys.system == FQ_SYS_READ_IN && !e->batched
+                          ? 2 * sizeof(double) * static_cast<size_t>(e->sys.norb) * e->sys.norb
+                          : 0;
+    const bool use_lds = jk > 0 && jk <= 32768;
+    hipLaunchKernelGGL(k_scatter_new, dim3(grid_for(nruns)), dim3(kBlock), use_lds ? jk : 0, st, e->sys, nruns, cap,
+                       scap, e->rdet, e->rpop, e->rpos, e->nmask, e->noff, e->nlist, e->mmask, e->moff, e->d_stats,
+                       e->det[o], e->pop[o], e->kii[o], e->batched ? e->d_sys_arr : nullptr, use_lds ? 1 : 0);
+  }
   FQ_HIP(hipEventRecord(e->pev[8], st));
   rc = sync_stats(e);
   if (rc != FQ_OK) return rc;
@@ -2007,6 +2045,7 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->rflag, scap);
   rc |= dalloc(&e->rpos, scap);
   rc |= dalloc(&e->rstate, scap);
+  rc |= dalloc(&e->nlist, scap);
   rc |= dalloc(&e->nmask, scap / 64 + 2);
   rc |= dalloc(&e->ncnt, scap / 64 + 2);
   rc |= dalloc(&e->noff, scap / 64 + 2);
@@ -2835,6 +2874,7 @@ void fq_destroy(fq_engine* e) {
   dfree(e->rflag);
   dfree(e->rpos);
   dfree(e->rstate);
+  dfree(e->nlist);
   dfree(e->nmask);
   dfree(e->ncnt);
   dfree(e->noff);
