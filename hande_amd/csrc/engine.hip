@@ -62,6 +62,9 @@ constexpr int kRadixItems = 8;
 constexpr int kRadixTile = kBlock * kRadixItems;
 constexpr int kRunTile = 4 * kBlock;  // sorted entries per run-detection tile
 constexpr int kMergeWin = 1024;       // main-list window entries a merge block stages in LDS
+// Words per spawned-list record (populations, flag), padded to 16-byte
+// alignment so one record is one aligned read after the sort's permutation.
+__host__ __device__ constexpr int rec_stride(int S) { return S == 1 ? 2 : 4; }
 
 #define FQ_HIP(call)                                  \
   do {                                                \
@@ -243,8 +246,7 @@ struct fq_engine {
   // spawned list
   int64_t scap = 0;
   uint64_t* sdet = nullptr;  // W*scap
-  int64_t* spop = nullptr;   // S*scap
-  int64_t* sflag = nullptr;  // scap
+  int64_t* srec = nullptr;   // scap records: S populations then the flag (rec_stride(S) words each)
   // sort buffers
   uint64_t* kbuf[2] = {nullptr, nullptr};  // W*scap
   uint32_t* vbuf[2] = {nullptr, nullptr};  // scap
@@ -598,8 +600,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_tiles(Sys s, int64_t ntiles, co
                                                        const int64_t* __restrict__ st_ent, int64_t bcap,
                                                        int64_t t_begin, const int64_t* __restrict__ flags,
                                                        int64_t scap, uint64_t* __restrict__ sdet,
-                                                       int64_t* __restrict__ spop, int64_t* __restrict__ sflag,
-                                                       Stats* __restrict__ st) {
+                                                       int64_t* __restrict__ srec, Stats* __restrict__ st) {
   const int64_t tile = blockIdx.x;
   if (tile >= ntiles) return;
   const int64_t cnt = tile_cnt[tile];
@@ -615,8 +616,9 @@ __global__ __launch_bounds__(kBlock) void k_pack_tiles(Sys s, int64_t ntiles, co
     const int sp = static_cast<int>(e % s.S);
     const int64_t i = e / s.S;
     for (int w = 0; w < s.W; ++w) sdet[w * scap + slot] = st_det[w * bcap + u];
-    for (int q = 0; q < s.S; ++q) spop[q * scap + slot] = (q == sp) ? st_pop[u] : 0;
-    sflag[slot] = s.initiator ? flags[i] : 0;
+    int64_t* rec = srec + slot * rec_stride(s.S);
+    for (int q = 0; q < s.S; ++q) rec[q] = (q == sp) ? st_pop[u] : 0;
+    rec[s.S] = s.initiator ? flags[i] : 0;
   }
 }
 
@@ -771,8 +773,7 @@ __global__ void k_replay_walk(Sys s, int64_t n, int64_t cap, const uint64_t* __r
                               const int64_t* __restrict__ pop, const double* __restrict__ kii,
                               int64_t* __restrict__ pop_new, double tau, double sh0, double sh1, double e0,
                               double e1, const double* __restrict__ rbuf, int64_t rpos, int64_t rend, int64_t scap,
-                              uint64_t* __restrict__ sdet, int64_t* __restrict__ spop, int64_t* __restrict__ sflag,
-                              Stats* __restrict__ st) {
+                              uint64_t* __restrict__ sdet, int64_t* __restrict__ srec, Stats* __restrict__ st) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   fqd::ReplayRng rng{rbuf, rpos, rend, false};
   double proj[FQ_MAX_SPACES] = {0.0, 0.0}, d0[FQ_MAX_SPACES] = {0.0, 0.0};
@@ -801,8 +802,9 @@ __global__ void k_replay_walk(Sys s, int64_t n, int64_t cap, const uint64_t* __r
           } else {
             const int64_t slot = nspawn_total++;
             FQ_FOR_WORDS(w, s.W) sdet[w * scap + slot] = child[w];
-            for (int q = 0; q < s.S; ++q) spop[q * scap + slot] = (q == sp) ? ns : 0;
-            sflag[slot] = s.initiator ? flag : 0;
+            int64_t* rec = srec + slot * rec_stride(s.S);
+            for (int q = 0; q < s.S; ++q) rec[q] = (q == sp) ? ns : 0;
+            rec[s.S] = s.initiator ? flag : 0;
           }
         }
       }
@@ -1054,8 +1056,7 @@ __global__ void k_replay_qsort(const uint64_t* __restrict__ sdet, int64_t ns, in
 // sorted order (gpop, gflag) for the run reduction.
 __global__ __launch_bounds__(kBlock) void k_run_heads(const uint64_t* __restrict__ keys, int64_t ns, int kw,
                                                       int64_t scap, int S, const uint32_t* __restrict__ perm,
-                                                      const int64_t* __restrict__ spop,
-                                                      const int64_t* __restrict__ sflag,
+                                                      const int64_t* __restrict__ srec,
                                                       int64_t* __restrict__ gpop, int64_t* __restrict__ gflag,
                                                       int64_t* __restrict__ run_stage,
                                                       int64_t* __restrict__ tile_cnt) {
@@ -1072,9 +1073,9 @@ __global__ __launch_bounds__(kBlock) void k_run_heads(const uint64_t* __restrict
 #pragma unroll
         for (int w = 0; w < FQ_MAX_WORDS; ++w)
           if (w < kw && i > 0) head = head || keys[w * scap + i] != keys[w * scap + i - 1];
-        const uint32_t j = perm[i];
-        for (int sp = 0; sp < S; ++sp) gpop[sp * scap + i] = spop[sp * scap + j];
-        gflag[i] = sflag[j];
+        const int64_t* rec = srec + static_cast<int64_t>(perm[i]) * rec_stride(S);
+        for (int sp = 0; sp < S; ++sp) gpop[sp * scap + i] = rec[sp];
+        gflag[i] = rec[S];
       }
       const unsigned long long bal = __ballot(head);
       if (lane == 0) wcount[wave] = __popcll(bal);
@@ -1604,7 +1605,7 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
   if (ns > 0) {
     const int64_t nt = ceil_div(ns, kRunTile);
     hipLaunchKernelGGL(k_run_heads, dim3(static_cast<unsigned>(std::min<int64_t>(nt, 1 << 20))), dim3(kBlock), 0,
-                       e->stream, keys, ns, kw, scap, e->S, perm, e->spop, e->sflag, e->gpop, e->gflag, e->run_stage,
+                       e->stream, keys, ns, kw, scap, e->S, perm, e->srec, e->gpop, e->gflag, e->run_stage,
                        e->run_tcnt);
     int rc = scan_exclusive(e, e->run_tcnt, e->run_toff, nt);
     if (rc != FQ_OK) return rc;
@@ -1994,7 +1995,7 @@ int pack_spawned(fq_engine* e, int64_t nbt, int64_t t_begin) {
   if (rc != FQ_OK) return rc;
   hipLaunchKernelGGL(k_pack_tiles, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0, e->stream, e->sys, nt,
                      e->tile_cnt, e->tile_off, e->st_det, e->st_pop, e->st_ent, e->bcap, t_begin, e->flags,
-                     e->scap, e->sdet, e->spop, e->sflag, e->d_stats);
+                     e->scap, e->sdet, e->srec, e->d_stats);
   hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, e->stream, e->tile_off, nt, e->d_stats);
   return FQ_OK;
 }
@@ -2028,8 +2029,7 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->tile_cnt, e->bcap / kSpawnTile + 2);
   rc |= dalloc(&e->tile_off, e->bcap / kSpawnTile + 2);
   rc |= dalloc(&e->sdet, W * scap);
-  rc |= dalloc(&e->spop, S * scap);
-  rc |= dalloc(&e->sflag, scap);
+  rc |= dalloc(&e->srec, rec_stride(S) * scap);
   for (int b = 0; b < 2; ++b) {
     rc |= dalloc(&e->kbuf[b], W * scap);
     rc |= dalloc(&e->vbuf[b], scap);
@@ -2274,7 +2274,7 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
   if (replay) {
     hipLaunchKernelGGL(k_replay_walk, dim3(1), dim3(1), 0, st, e->sys, n, cap, e->det[c], e->pop[c], e->kii[c],
                        e->pop_new, tau, shift[0], sh1, eproj_old[0], e1, e->rng_buf, e->rng_pos, e->rng_end, e->scap,
-                       e->sdet, e->spop, e->sflag, e->d_stats);
+                       e->sdet, e->srec, e->d_stats);
     FQ_HIP(hipEventRecord(e->pev[1], st));
     FQ_HIP(hipEventRecord(e->pev[2], st));
     FQ_HIP(hipEventRecord(e->pev[3], st));
@@ -2402,18 +2402,17 @@ int fq_annihilate(fq_engine* e, const uint64_t* sstates, const int64_t* spops, c
   const int W = e->W, S = e->S;
   const int64_t n = e->n, cap = e->cap;
   std::vector<uint64_t> dw(static_cast<size_t>(W) * std::max<int64_t>(ns, 1));
-  std::vector<int64_t> dp(static_cast<size_t>(S) * std::max<int64_t>(ns, 1)), df(std::max<int64_t>(ns, 1));
+  const int RS = rec_stride(S);
+  std::vector<int64_t> rec(static_cast<size_t>(RS) * std::max<int64_t>(ns, 1), 0);
   for (int64_t i = 0; i < ns; ++i) {
     for (int w = 0; w < W; ++w) dw[w * ns + i] = sstates[i * W + w];
-    for (int s = 0; s < S; ++s) dp[s * ns + i] = spops[i * S + s];
-    df[i] = (sflags && e->par.initiator) ? sflags[i] : 0;
+    for (int s = 0; s < S; ++s) rec[i * RS + s] = spops[i * S + s];
+    rec[i * RS + S] = (sflags && e->par.initiator) ? sflags[i] : 0;
   }
   hipStream_t st = e->stream;
   for (int w = 0; w < W && ns > 0; ++w)
     FQ_HIP(hipMemcpyAsync(e->sdet + w * e->scap, dw.data() + w * ns, 8 * ns, hipMemcpyHostToDevice, st));
-  for (int s = 0; s < S && ns > 0; ++s)
-    FQ_HIP(hipMemcpyAsync(e->spop + s * e->scap, dp.data() + s * ns, 8 * ns, hipMemcpyHostToDevice, st));
-  if (ns > 0) FQ_HIP(hipMemcpyAsync(e->sflag, df.data(), 8 * ns, hipMemcpyHostToDevice, st));
+  if (ns > 0) FQ_HIP(hipMemcpyAsync(e->srec, rec.data(), 8 * RS * ns, hipMemcpyHostToDevice, st));
   // post-death populations = current populations
   for (int s = 0; s < S && n > 0; ++s)
     FQ_HIP(hipMemcpyAsync(e->pop_new + s * cap, e->pop[e->cur] + s * cap, 8 * n, hipMemcpyDeviceToDevice, st));
@@ -2859,8 +2858,7 @@ void fq_destroy(fq_engine* e) {
   dfree(e->tile_cnt);
   dfree(e->tile_off);
   dfree(e->sdet);
-  dfree(e->spop);
-  dfree(e->sflag);
+  dfree(e->srec);
   dfree(e->hist);
   dfree(e->hist_scan);
   dfree(e->run_stage);
