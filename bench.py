@@ -9,10 +9,12 @@ population 1e7, tau 1e-3, renormalised excitation generator, FP64 integrals.
 A step is one Monte Carlo cycle (spawning + death + annihilation) of the whole
 walker population resident in HBM.  The walker distribution is prepared
 untimed: target/1000 walkers on the Hartree--Fock determinant are propagated
-at a larger time step (0.01); once the population passes the target the shift
-varies with a proportional-integral update that pulls the population to the
-target, for 250 cycles.  Then W warm-up cycles run at tau = 1e-3 with the
-reference's shift update (damping 0.05) before K timed cycles.  A prepared
+at the production time step (1e-3; a larger step moves the stationary state of
+these systems, whose diagonal elements spread over tens of Hartree); once the
+population passes the target the shift varies with a proportional-integral
+update that pulls the population to the target, for 250 cycles.  Then W
+warm-up cycles run at tau = 1e-3 with the reference's shift update (damping
+0.05) before K timed cycles.  A prepared
 list can be saved (--save-state) and reloaded (--load-state), so a profiled
 run contains only cycles of the measured shape.
 
@@ -305,7 +307,7 @@ def copy_probe(mib):
 # ---------------------------------------------------------------------------
 # E_proj statistical efficiency 1/(sigma^2 t)
 # ---------------------------------------------------------------------------
-def prepare(lib, system, target, seed, cap, equil_tau=0.01, hold=250, max_cycles=1500):
+def prepare(lib, system, target, seed, cap, equil_tau=1e-3, hold=300, max_cycles=4000):
     """Untimed preparation (renormalised generator): returns the walker list and shift."""
     eng, _p = make_engine(lib, system, target, seed, cap, cap)
     f0 = system.f0()
@@ -560,9 +562,11 @@ def main():
     ap.add_argument("--config", default="mol26", choices=sorted(synthetic.MOLECULAR_CONFIGS))
     ap.add_argument("--target", type=float, default=1e7)
     ap.add_argument("--replicas", type=int, default=1, choices=[1, 2], help="walker spaces (2: replica pair)")
-    ap.add_argument("--equil-cycles", type=int, default=1200, help="max untimed preparation cycles")
+    ap.add_argument("--equil-cycles", type=int, default=4000, help="max untimed preparation cycles")
     ap.add_argument("--hold-cycles", type=int, default=250, help="preparation cycles after the shift starts varying")
-    ap.add_argument("--equil-tau", type=float, default=0.01)
+    ap.add_argument("--equil-tau", type=float, default=1e-3,
+                    help="preparation time step (the production one: a larger step changes the stationary state "
+                         "of systems with a wide diagonal spread)")
     ap.add_argument("--prep-damping", type=float, default=0.2, help="shift damping while preparing")
     ap.add_argument("--save-state", default=None, help="write the prepared walker list and shift to this .npz")
     ap.add_argument("--load-state", default=None, help="start from a list written by --save-state (no preparation)")

@@ -71,9 +71,11 @@ def test_molecular_projected_energy_matches_exact(tmp_path, excit_gen):
     path = synthetic.write_fcidump(os.path.join(tmp_path, "FCIDUMP"), norb, nel, seed)
     system = api.read_in(int_file=path, nel=nel, ms=0)
     print(f"[production] synthetic {norb}o{nel}e {excit_gen}", flush=True)
-    q = dict(tau=0.01, init_pop=10, mc_cycles=20, nreports=400, target_population=2000,
+    q = dict(tau=0.01, init_pop=10, mc_cycles=20, nreports=800, target_population=2000,
              shift_damping=0.05, state_size=100000, spawned_state_size=100000, excit_gen=excit_gen)
-    e, se, sh, sse = replicate_estimates(system, q, range(91, 99))
+    # the shift settles more slowly than E_proj after the population passes
+    # the target: the first 300 reports are left out
+    e, se, sh, sse = replicate_estimates(system, q, range(91, 99), skip=300)
     print(f"  E_proj={e:.6f} +- {se:.6f}, shift={sh:.6f} +- {sse:.6f}, exact {exact:.6f}", flush=True)
     assert abs(e - exact) < 4 * se + 1e-4
     assert abs(sh - exact) < 4 * sse + 5e-3
