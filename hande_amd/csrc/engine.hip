@@ -243,6 +243,7 @@ struct fq_engine {
   int64_t* st_ent = nullptr;    // bcap
   int64_t* tile_cnt = nullptr;  // successes per spawning tile (bcap / kSpawnTile + 2)
   int64_t* tile_off = nullptr;  // their exclusive prefix
+  int64_t* tile_win = nullptr;  // parents of each tile's first and last attempt
   // spawned list
   int64_t scap = 0;
   uint64_t* sdet = nullptr;  // W*scap
@@ -266,6 +267,7 @@ struct fq_engine {
   int64_t* rpos = nullptr;       // scap
   int64_t* rstate = nullptr;     // scap: 0 dead, 1 hit, 2 new candidate
   int64_t* nlist = nullptr;      // scap: surviving new runs in order
+  int64_t* mwin = nullptr;       // main-list bounds of each merge block's first and last run
   uint64_t* nmask = nullptr;     // scap/64+2: surviving new runs, one bit each
   int64_t* ncnt = nullptr;       // survivors per 64-run word
   int64_t* noff = nullptr;       // their exclusive prefix
@@ -475,6 +477,20 @@ __device__ __forceinline__ int64_t parent_of(const int64_t* __restrict__ att_off
 }
 
 constexpr int kSpawnTile = 4 * kBlock;  // attempts per block tile
+
+// Parents of each spawning tile's first and last attempt (one thread per
+// tile, so the searches' latencies overlap instead of opening every block).
+__global__ void k_tile_bounds(int64_t nent, const int64_t* __restrict__ att_off, int64_t t_begin, int64_t t_end,
+                              int64_t* __restrict__ tile_win) {
+  const int64_t ntiles = (t_end - t_begin + kSpawnTile - 1) / kSpawnTile;
+  const int64_t tile = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tile >= ntiles) return;
+  const int64_t t0 = t_begin + tile * kSpawnTile;
+  const int64_t t1 = (t0 + kSpawnTile < t_end) ? t0 + kSpawnTile : t_end;
+  const int64_t p0 = parent_of(att_off, t0, 0, nent - 1);
+  tile_win[2 * tile] = p0;
+  tile_win[2 * tile + 1] = parent_of(att_off, t1 - 1, p0, nent - 1);
+}
 constexpr int kSpawnWin = 2048;          // parent-window entries staged in LDS
 
 // Production: one thread per spawning attempt.  Each block takes tiles of
@@ -492,9 +508,9 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
                                                   double tau, uint64_t seed, uint32_t cyc, int64_t bcap,
                                                   uint64_t* __restrict__ st_det, int64_t* __restrict__ st_pop,
                                                   int64_t* __restrict__ st_ent, int64_t* __restrict__ tile_cnt,
+                                                  const int64_t* __restrict__ tile_win,
                                                   const Sys* __restrict__ sys_arr, const int64_t* __restrict__ iatt,
                                                   const double* __restrict__ itau) {
-  __shared__ int64_t win[2];
   __shared__ int32_t wcount[kBlock / 64];
   // att_off[lo + k] - t0 (clamped below at -1: every such entry starts at or
   // before the tile's first attempt) when the window fits
@@ -509,13 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t t0 = t_begin + tile * kSpawnTile;
     const int64_t t1 = (t0 + kSpawnTile < t_end) ? t0 + kSpawnTile : t_end;
-    if (threadIdx.x == 0) {
-      const int64_t p0 = parent_of(att_off, t0, 0, nent - 1);
-      win[0] = p0;
-      win[1] = parent_of(att_off, t1 - 1, p0, nent - 1);
-    }
-    __syncthreads();
-    const int64_t lo = win[0], hi = win[1];
+    const int64_t lo = tile_win[2 * tile], hi = tile_win[2 * tile + 1];
     const bool staged = hi - lo < kSpawnWin;
     if (staged)
       for (int64_t k = threadIdx.x; k <= hi - lo; k += kBlock) {
@@ -1192,6 +1202,22 @@ __device__ __forceinline__ int64_t main_lower_bound(const uint64_t* __restrict__
   return lo;
 }
 
+// Lower bounds in the main list of the first and last key of each merge
+// block's runs (one thread per block of kBlock runs).
+__global__ void k_merge_bounds(int64_t nruns, int64_t n, int64_t cap, int W, const uint64_t* __restrict__ det,
+                               const uint64_t* __restrict__ rdet, int64_t scap, int64_t* __restrict__ mwin) {
+  const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t r0 = b * kBlock;
+  if (r0 >= nruns) return;
+  const int64_t r1 = (r0 + kBlock < nruns ? r0 + kBlock : nruns) - 1;
+  uint64_t k[FQ_MAX_WORDS] = {0, 0};
+  FQ_FOR_WORDS(w, W) k[w] = rdet[w * scap + r0];
+  const int64_t lo = main_lower_bound(det, cap, W, k, 0, n);
+  FQ_FOR_WORDS(w, W) k[w] = rdet[w * scap + r1];
+  mwin[2 * b] = lo;
+  mwin[2 * b + 1] = main_lower_bound(det, cap, W, k, lo, n);
+}
+
 // Runs are sorted, so a block's runs land between the lower bounds of its
 // first and last keys: two searches of the whole list per block, then each
 // thread searches only that window (merge-path partition of the two lists).
@@ -1199,22 +1225,14 @@ __global__ __launch_bounds__(kBlock) void k_merge_runs(Sys s, int64_t nruns, int
                                                        const uint64_t* __restrict__ det,
                                                        int64_t* __restrict__ pop_new, const uint64_t* __restrict__ rdet,
                                                        int64_t* __restrict__ rpop, const int64_t* __restrict__ rflag,
-                                                       int64_t scap, int64_t* __restrict__ rpos,
-                                                       int64_t* __restrict__ rstate) {
-  __shared__ int64_t win[2];
+                                                       int64_t scap, const int64_t* __restrict__ mwin,
+                                                       int64_t* __restrict__ rpos, int64_t* __restrict__ rstate) {
   __shared__ uint64_t wdet[FQ_MAX_WORDS][kMergeWin];  // the window's determinants when it fits
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
-  const int64_t r1 = (r0 + blockDim.x < nruns ? r0 + blockDim.x : nruns) - 1;
-  if (threadIdx.x < 2) {
-    const int64_t rr = threadIdx.x == 0 ? r0 : r1;
-    uint64_t k[FQ_MAX_WORDS] = {0, 0};
-    FQ_FOR_WORDS(w, s.W) k[w] = rdet[w * scap + rr];
-    win[threadIdx.x] = main_lower_bound(det, cap, s.W, k, 0, n);
-  }
-  __syncthreads();
   // every key of the block lies in [key(r0), key(r1)], so its lower bound
   // lies in [wlo, whi): whi = min(bound(r1) + 1, n)
-  const int64_t wlo = win[0], whi = win[1] < n ? win[1] + 1 : n;
+  const int64_t b0 = mwin[2 * blockIdx.x], b1 = mwin[2 * blockIdx.x + 1];
+  const int64_t wlo = b0, whi = b1 < n ? b1 + 1 : n;
   const bool staged = whi - wlo <= kMergeWin;
   if (staged)
     for (int64_t q = threadIdx.x; q < whi - wlo; q += kBlock) FQ_FOR_WORDS(w, s.W) wdet[w][q] = det[w * cap + wlo + q];
@@ -1635,9 +1653,12 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
   const int c = e->cur;
   hipStream_t st = e->stream;
   FQ_HIP(hipEventRecord(e->pev[7], st));
-  if (nruns > 0)
+  if (nruns > 0) {
+    hipLaunchKernelGGL(k_merge_bounds, dim3(grid_for(grid_for(nruns))), dim3(kBlock), 0, st, nruns, n, cap, e->W,
+                       e->det[c], e->rdet, scap, e->mwin);
     hipLaunchKernelGGL(k_merge_runs, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, n, cap, e->det[c],
-                       e->pop_new, e->rdet, e->rpop, e->rflag, scap, e->rpos, e->rstate);
+                       e->pop_new, e->rdet, e->rpop, e->rflag, scap, e->mwin, e->rpos, e->rstate);
+  }
   // Rounding draws (replay: main list first, then new candidates, in order).
   int64_t main_draws = 0, new_draws = 0;
   if (replay && e->sys.real_amps) {
@@ -2028,6 +2049,7 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->st_ent, e->bcap);
   rc |= dalloc(&e->tile_cnt, e->bcap / kSpawnTile + 2);
   rc |= dalloc(&e->tile_off, e->bcap / kSpawnTile + 2);
+  rc |= dalloc(&e->tile_win, 2 * (e->bcap / kSpawnTile + 2));
   rc |= dalloc(&e->sdet, W * scap);
   rc |= dalloc(&e->srec, rec_stride(S) * scap);
   for (int b = 0; b < 2; ++b) {
@@ -2046,6 +2068,7 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->rpos, scap);
   rc |= dalloc(&e->rstate, scap);
   rc |= dalloc(&e->nlist, scap);
+  rc |= dalloc(&e->mwin, 2 * (scap / kBlock + 2));
   rc |= dalloc(&e->nmask, scap / 64 + 2);
   rc |= dalloc(&e->ncnt, scap / 64 + 2);
   rc |= dalloc(&e->noff, scap / 64 + 2);
@@ -2327,6 +2350,8 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
       const int64_t nbt = te - tb;
       const unsigned g = static_cast<unsigned>(std::min<int64_t>(ceil_div(nbt, kSpawnTile), 1 << 20));
       const int li = e->spawn_launches++;
+      hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(ceil_div(nbt, kSpawnTile))), dim3(kBlock), 0, st, nent,
+                         e->att_off, tb, te, e->tile_win);
       if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li], st));
       const int gen = fqd::gen_of(e->sys.system, e->sys.excit_gen);
       auto kern = gen == fqd::kGenHubK       ? k_spawn<fqd::kGenHubK, false>
@@ -2336,7 +2361,7 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
       const size_t lds = gen == fqd::kGenHeatBath ? hb_lds_bytes(e->sys.nel) : 0;
       hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), lds, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
                          e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
-                         e->tile_cnt, nullptr, nullptr, nullptr);
+                         e->tile_cnt, e->tile_win, nullptr, nullptr, nullptr);
       if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
       rc = pack_spawned(e, nbt, tb);
       if (rc != FQ_OK) return rc;
@@ -2593,11 +2618,13 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
     const int64_t nbt = te - tb;
     const unsigned g = static_cast<unsigned>(std::min<int64_t>(ceil_div(nbt, kSpawnTile), 1 << 20));
     const int li = e->spawn_launches++;
+    hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(ceil_div(nbt, kSpawnTile))), dim3(kBlock), 0, st, nent,
+                       e->att_off, tb, te, e->tile_win);
     if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li], st));
     auto kern = e->sys.system == FQ_SYS_HUB_K ? k_spawn<fqd::kGenHubK, true> : k_spawn<fqd::kGenMolRenorm, true>;
     hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
                        e->pop[c], 0.0, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
-                       e->tile_cnt, e->d_sys_arr, e->d_iatt, e->d_ipar);
+                       e->tile_cnt, e->tile_win, e->d_sys_arr, e->d_iatt, e->d_ipar);
     if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
     rc = pack_spawned(e, nbt, tb);
     if (rc != FQ_OK) return rc;
@@ -2857,6 +2884,7 @@ void fq_destroy(fq_engine* e) {
   dfree(e->st_ent);
   dfree(e->tile_cnt);
   dfree(e->tile_off);
+  dfree(e->tile_win);
   dfree(e->sdet);
   dfree(e->srec);
   dfree(e->hist);
@@ -2873,6 +2901,7 @@ void fq_destroy(fq_engine* e) {
   dfree(e->rpos);
   dfree(e->rstate);
   dfree(e->nlist);
+  dfree(e->mwin);
   dfree(e->nmask);
   dfree(e->ncnt);
   dfree(e->noff);
