@@ -521,10 +521,28 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
     extern __shared__ double hb_lds[];
     hb = fqd::HbBuf{hb_lds + threadIdx.x, kBlock};
   }
+  // Renormalised and learned generators decide single vs double on an
+  // attempt's first draw: those tiles are processed doubles first and singles
+  // last (their code paths differ), results parked in LDS by attempt, so a
+  // wavefront seldom runs both paths; successes are still compacted in
+  // attempt order.
+  constexpr bool kPart = (G == fqd::kGenMolRenorm || G == fqd::kGenLearned) && !B;
+  __shared__ int16_t order[kPart ? kSpawnTile : 1];   // processing slot -> attempt in tile
+  __shared__ int64_t rpop[kPart ? kSpawnTile : 1];    // results by attempt in tile
+  __shared__ int32_t rent[kPart ? kSpawnTile : 1];    // parent entry - lo
+  __shared__ int32_t wcs[kBlock / 64];
+  uint64_t* rdet = nullptr;                            // [w * kSpawnTile + r], dynamic LDS
+  if constexpr (kPart) {
+    extern __shared__ uint64_t part_lds[];
+    rdet = part_lds;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1;
   const int64_t ntiles = (t_end - t_begin + kSpawnTile - 1) / kSpawnTile;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t t0 = t_begin + tile * kSpawnTile;
     const int64_t t1 = (t0 + kSpawnTile < t_end) ? t0 + kSpawnTile : t_end;
+    const int tn = static_cast<int>(t1 - t0);
     const int64_t lo = tile_win[2 * tile], hi = tile_win[2 * tile + 1];
     const bool staged = hi - lo < kSpawnWin;
     if (staged)
@@ -533,55 +551,108 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
         watt[k] = static_cast<int32_t>(v < -1 ? -1 : v);
       }
     __syncthreads();
+    // one spawning attempt t (tile-relative r): its parent entry e, the
+    // spawned population and the child determinant
+    auto attempt = [&](int64_t t, int r, int64_t& e, int64_t& nspawn, uint64_t* child) {
+      if (staged) {  // last k in [0, hi - lo] with att_off[lo + k] <= t
+        int a = 0, b = static_cast<int>(hi - lo);
+        while (a < b) {
+          const int mid = (a + b + 1) >> 1;
+          if (watt[mid] <= r)
+            a = mid;
+          else
+            b = mid - 1;
+        }
+        e = lo + a;
+      } else {
+        e = parent_of(att_off, t, lo, hi);
+      }
+      const int sp = static_cast<int>(e % s.S);
+      const int64_t i = e / s.S;
+      uint64_t f[FQ_MAX_WORDS];
+      load_det(det, cap, s.W, i, f);
+      const int64_t parent = pop[sp * cap + i];
+      if constexpr (B) {
+        // batched instances: the instance's tables, time step, key and attempt numbering
+        const int inst = static_cast<int>(f[0] >> kInstShift);
+        f[0] &= kOrbMask;
+        const int64_t trel = t - iatt[inst];
+        fqd::Philox rng(seed + static_cast<uint64_t>(inst), static_cast<uint32_t>(trel),
+                        static_cast<uint32_t>(trel >> 32), cyc ^ (fqd::kDrawAttempt << 24));
+        nspawn = fqd::spawn_attempt<G>(sys_arr[inst], rng, f, nullptr, parent, itau[inst], child);
+        if (nspawn != 0) child[0] |= static_cast<uint64_t>(inst) << kInstShift;
+      } else {
+        fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
+                        cyc ^ (fqd::kDrawAttempt << 24));
+        const float* wrow = (G == fqd::kGenLearned && s.mlp_on[i]) ? s.mlp_tab + s.mlp_row[i] * s.mlp_ld : nullptr;
+        nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child, hb);
+      }
+    };
+    if constexpr (kPart) {
+      // processing order: doubles from the front, singles from the back
+      int32_t nd = 0, ns = 0;
+      for (int k = 0; k < kSpawnTile / kBlock; ++k) {
+        const int r = k * kBlock + threadIdx.x;
+        bool single = false;
+        if (r < tn) {
+          const int64_t t = t0 + r;
+          fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
+                          cyc ^ (fqd::kDrawAttempt << 24));
+          single = rng.next() < s.psingle;
+        }
+        const unsigned long long bd = __ballot(r < tn && !single), bs = __ballot(r < tn && single);
+        if (lane == 0) {
+          wcount[wave] = __popcll(bd);
+          wcs[wave] = __popcll(bs);
+        }
+        __syncthreads();
+        int32_t pd = nd + __popcll(bd & below), ps = ns + __popcll(bs & below);
+        for (int w = 0; w < kBlock / 64; ++w) {
+          if (w < wave) {
+            pd += wcount[w];
+            ps += wcs[w];
+          }
+          nd += wcount[w];
+          ns += wcs[w];
+        }
+        if (r < tn) order[single ? tn - 1 - ps : pd] = static_cast<int16_t>(r);
+        __syncthreads();
+      }
+      for (int k = 0; k < kSpawnTile / kBlock; ++k) {
+        const int q = k * kBlock + threadIdx.x;
+        if (q < tn) {
+          const int r = order[q];
+          int64_t e = 0, nspawn = 0;
+          uint64_t child[FQ_MAX_WORDS] = {0, 0};
+          attempt(t0 + r, r, e, nspawn, child);
+          FQ_FOR_WORDS(w, s.W) rdet[w * kSpawnTile + r] = child[w];
+          rpop[r] = nspawn;
+          rent[r] = static_cast<int32_t>(e - lo);
+        }
+      }
+      __syncthreads();
+    }
     int32_t run = 0;  // successes of this tile so far (all threads hold the same value)
     for (int k = 0; k < kSpawnTile / kBlock; ++k) {
-      const int64_t t = t0 + k * kBlock + threadIdx.x;
-      const bool valid = t < t1;
+      const int r = k * kBlock + threadIdx.x;
+      const bool valid = r < tn;
       int64_t e = 0, nspawn = 0;
       uint64_t child[FQ_MAX_WORDS] = {0, 0};
       if (valid) {
-        if (staged) {  // last k in [0, hi - lo] with att_off[lo + k] <= t
-          const int32_t tr = static_cast<int32_t>(t - t0);
-          int a = 0, b = static_cast<int>(hi - lo);
-          while (a < b) {
-            const int mid = (a + b + 1) >> 1;
-            if (watt[mid] <= tr)
-              a = mid;
-            else
-              b = mid - 1;
-          }
-          e = lo + a;
+        if constexpr (kPart) {
+          nspawn = rpop[r];
+          e = lo + rent[r];
+          FQ_FOR_WORDS(w, s.W) child[w] = rdet[w * kSpawnTile + r];
         } else {
-          e = parent_of(att_off, t, lo, hi);
+          attempt(t0 + r, r, e, nspawn, child);
         }
-        const int sp = static_cast<int>(e % s.S);
-        const int64_t i = e / s.S;
-        uint64_t f[FQ_MAX_WORDS];
-        load_det(det, cap, s.W, i, f);
-        const int64_t parent = pop[sp * cap + i];
-        if constexpr (B) {
-          // batched instances: the instance's tables, time step, key and attempt numbering
-          const int inst = static_cast<int>(f[0] >> kInstShift);
-          f[0] &= kOrbMask;
-          const int64_t trel = t - iatt[inst];
-          fqd::Philox rng(seed + static_cast<uint64_t>(inst), static_cast<uint32_t>(trel),
-                          static_cast<uint32_t>(trel >> 32), cyc ^ (fqd::kDrawAttempt << 24));
-          nspawn = fqd::spawn_attempt<G>(sys_arr[inst], rng, f, nullptr, parent, itau[inst], child);
-          if (nspawn != 0) child[0] |= static_cast<uint64_t>(inst) << kInstShift;
-        } else {
-          fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
-                          cyc ^ (fqd::kDrawAttempt << 24));
-          const float* wrow = (G == fqd::kGenLearned && s.mlp_on[i]) ? s.mlp_tab + s.mlp_row[i] * s.mlp_ld : nullptr;
-          nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child, hb);
-        }
-      }  // valid
+      }
       // rank this round's successes in attempt order (wave ballot + per-wave totals)
       const bool ok = valid && nspawn != 0;
       const unsigned long long bal = __ballot(ok);
-      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
       if (lane == 0) wcount[wave] = __popcll(bal);
       __syncthreads();
-      int32_t pos = run + __popcll(bal & ((1ull << lane) - 1));
+      int32_t pos = run + __popcll(bal & below);
       int32_t round_total = 0;
       for (int w = 0; w < kBlock / 64; ++w) {
         if (w < wave) pos += wcount[w];
@@ -2110,6 +2181,8 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
     return FQ_ERR_ARG;
   if (sys->system == FQ_SYS_READ_IN && params->excit_gen == FQ_EXCIT_NO_RENORM) return FQ_ERR_ARG;
   if (params->main_capacity < 1 || params->spawn_capacity < 1) return FQ_ERR_ARG;
+  // (determinant, space) entries are addressed relative to a window in 32 bits
+  if (params->main_capacity * params->nspaces >= (int64_t{1} << 31)) return FQ_ERR_ARG;
   if (params->spawn_capacity > 0xffffffffLL) return FQ_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= params->device) return FQ_ERR_NO_DEVICE;
@@ -2358,7 +2431,10 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
                   : gen == fqd::kGenHeatBath ? k_spawn<fqd::kGenHeatBath, false>
                   : gen == fqd::kGenLearned  ? k_spawn<fqd::kGenLearned, false>
                                              : k_spawn<fqd::kGenMolRenorm, false>;
-      const size_t lds = gen == fqd::kGenHeatBath ? hb_lds_bytes(e->sys.nel) : 0;
+      const size_t lds = gen == fqd::kGenHeatBath ? hb_lds_bytes(e->sys.nel)
+                         : (gen == fqd::kGenMolRenorm || gen == fqd::kGenLearned)
+                             ? sizeof(uint64_t) * e->W * kSpawnTile
+                             : 0;
       hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), lds, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
                          e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
                          e->tile_cnt, e->tile_win, nullptr, nullptr, nullptr);
