@@ -920,55 +920,71 @@ __device__ __forceinline__ int key_cmp(const uint64_t* keys, int64_t stride, int
   return 0;
 }
 
-// Radix pass histogram: hist[d*nb + block] = count of digit d in the block's tile.
+// Radix passes use DB-bit digits (8 or 9; 1 << DB buckets): the host mixes
+// the two widths to cover a key word in the fewest passes (34 bits: 9+9+8+8).
+// Histogram: hist[d * nb + block] = count of digit d in the block's tile.
+template <int DB>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* __restrict__ kw, int64_t n, int shift,
                                                        int64_t* __restrict__ hist) {
-  __shared__ int cnt[256];
-  cnt[threadIdx.x] = 0;
+  constexpr int NB = 1 << DB;
+  __shared__ int cnt[NB];
+  for (int d = threadIdx.x; d < NB; d += kBlock) cnt[d] = 0;
   __syncthreads();
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kRadixTile;
   for (int r = 0; r < kRadixItems; ++r) {
     const int64_t i = base + r * kBlock + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[(kw[i] >> shift) & 0xff], 1);
+    if (i < n) atomicAdd(&cnt[(kw[i] >> shift) & (NB - 1)], 1);
   }
   __syncthreads();
-  hist[static_cast<int64_t>(threadIdx.x) * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
+  for (int d = threadIdx.x; d < NB; d += kBlock) hist[static_cast<int64_t>(d) * gridDim.x + blockIdx.x] = cnt[d];
 }
 
 // Radix pass scatter, stable, staged in LDS.  The tile's elements are ranked
-// in input order (rounds of 256; per round 8 wavefront ballots give each
+// in input order (rounds of 256; per round DB wavefront ballots give each
 // lane's rank among equal digits of its wave, per-wave digit counts in LDS
 // give the rest) and placed in LDS sorted by digit at the tile's digit
 // offsets (an exclusive scan of the tile's histogram); the tile is then
 // written out in that order, so consecutive threads store consecutive
 // addresses of one digit's run (global base of digit d for this tile:
 // hist_scan[d * nb + tile]).  KW key words per element.
-template <int KW>
+template <int KW, int DB>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                           int64_t n, int w_digit, int shift,
                                                           const int64_t* __restrict__ hist,
                                                           const int64_t* __restrict__ hist_scan,
                                                           uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                           int64_t stride) {
+  constexpr int NB = 1 << DB, DPT = NB / kBlock;  // buckets, buckets per thread
   __shared__ uint64_t sk[KW][kRadixTile];
   __shared__ uint32_t sv[kRadixTile];
-  __shared__ int64_t gbase[256];
-  __shared__ int64_t loff[256];
-  __shared__ int run[256];
-  __shared__ int wcnt[kBlock / 64][256];
+  __shared__ int64_t gbase[NB];
+  __shared__ int32_t loff[NB];
+  __shared__ int32_t run[NB];
+  __shared__ int16_t wcnt[kBlock / 64][NB];
+  __shared__ int64_t part[kBlock];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t nb = gridDim.x;
-  const int64_t cnt = hist[static_cast<int64_t>(tid) * nb + blockIdx.x];
-  gbase[tid] = hist_scan[static_cast<int64_t>(tid) * nb + blockIdx.x];
-  loff[tid] = cnt;
-  run[tid] = 0;
+  // tile digit offsets: exclusive scan over the NB counts, DPT consecutive per thread
+  int64_t c[DPT], tsum = 0;
+  for (int j = 0; j < DPT; ++j) {
+    const int d = tid * DPT + j;
+    c[j] = hist[static_cast<int64_t>(d) * nb + blockIdx.x];
+    gbase[d] = hist_scan[static_cast<int64_t>(d) * nb + blockIdx.x];
+    run[d] = 0;
+    tsum += c[j];
+  }
+  part[tid] = tsum;
   __syncthreads();
-  block_inclusive_scan(loff);
-  loff[tid] -= cnt;
+  block_inclusive_scan(part);
+  int64_t acc = part[tid] - tsum;
+  for (int j = 0; j < DPT; ++j) {
+    loff[tid * DPT + j] = static_cast<int32_t>(acc);
+    acc += c[j];
+  }
   const int64_t tbase = static_cast<int64_t>(blockIdx.x) * kRadixTile;
   const int tn = static_cast<int>(n - tbase < kRadixTile ? n - tbase : kRadixTile);
   for (int r = 0; r < kRadixItems; ++r) {
-    for (int k = tid; k < (kBlock / 64) * 256; k += kBlock) (&wcnt[0][0])[k] = 0;
+    for (int k = tid; k < (kBlock / 64) * NB; k += kBlock) (&wcnt[0][0])[k] = 0;
     __syncthreads();
     const int li = r * kBlock + tid;
     const bool valid = li < tn;
@@ -979,33 +995,35 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __rest
 #pragma unroll
       for (int w = 0; w < KW; ++w) kv[w] = kin[w * stride + tbase + li];
       v = vin[tbase + li];
-      digit = static_cast<int>(((w_digit == 0 ? kv[0] : kv[KW - 1]) >> shift) & 0xff);
+      digit = static_cast<int>(((w_digit == 0 ? kv[0] : kv[KW - 1]) >> shift) & (NB - 1));
     }
     unsigned long long peers = __ballot(valid);
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < DB; ++b) {
       const bool bit = (digit >> b) & 1;
       const unsigned long long m = __ballot(bit);
       peers &= bit ? m : ~m;
     }
     const int rank = __popcll(peers & ((1ull << lane) - 1));
-    if (valid && rank == 0) wcnt[wave][digit] = __popcll(peers);
+    if (valid && rank == 0) wcnt[wave][digit] = static_cast<int16_t>(__popcll(peers));
     __syncthreads();
     if (valid) {
-      int pos = static_cast<int>(loff[digit]) + run[digit] + rank;
+      int pos = loff[digit] + run[digit] + rank;
       for (int ww = 0; ww < wave; ++ww) pos += wcnt[ww][digit];
 #pragma unroll
       for (int w = 0; w < KW; ++w) sk[w][pos] = kv[w];
       sv[pos] = v;
     }
     __syncthreads();
-    int add = 0;
-    for (int ww = 0; ww < kBlock / 64; ++ww) add += wcnt[ww][tid];
-    run[tid] += add;
+    for (int d = tid; d < NB; d += kBlock) {
+      int add = 0;
+      for (int ww = 0; ww < kBlock / 64; ++ww) add += wcnt[ww][d];
+      run[d] += add;
+    }
     __syncthreads();  // wcnt is cleared by the next round
   }
   for (int p = tid; p < tn; p += kBlock) {
     const uint64_t dw = w_digit == 0 ? sk[0][p] : sk[KW - 1][p];
-    const int d = static_cast<int>((dw >> shift) & 0xff);
+    const int d = static_cast<int>((dw >> shift) & (NB - 1));
     const int64_t g = gbase[d] + (p - loff[d]);
 #pragma unroll
     for (int w = 0; w < KW; ++w) kout[w * stride + g] = sk[w][p];
@@ -1664,25 +1682,30 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
                            e->kbuf[0], e->vbuf[0]);
       }
       const int64_t nb = ceil_div(ns, kRadixTile);
-      if (e->hist_cap < 256 * nb + 1) {
+      if (e->hist_cap < 512 * nb + 1) {
         dfree(e->hist);
         dfree(e->hist_scan);
-        if (dalloc(&e->hist, 256 * nb + 1) != FQ_OK || dalloc(&e->hist_scan, 256 * nb + 1) != FQ_OK)
+        if (dalloc(&e->hist, 512 * nb + 1) != FQ_OK || dalloc(&e->hist_scan, 512 * nb + 1) != FQ_OK)
           return FQ_ERR_NOMEM;
-        e->hist_cap = 256 * nb + 1;
+        e->hist_cap = 512 * nb + 1;
       }
       int src = 0;
       for (int w = 0; w < kw; ++w) {
         const int bits = std::min(64, kbits - 64 * w);
-        for (int shift = 0; shift < bits; shift += 8) {
-          hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src] + w * scap, ns, shift,
-                             e->hist);
-          int rc = scan_exclusive(e, e->hist, e->hist_scan, 256 * nb);
+        // fewest passes of 8- or 9-bit digits covering the word's bits
+        const int passes = std::max(1, (bits + 8) / 9), n9 = std::max(0, std::min(passes, bits - 8 * passes));
+        for (int pidx = 0, shift = 0; pidx < passes; ++pidx) {
+          const int db = pidx < n9 ? 9 : 8;
+          auto hist = db == 9 ? k_radix_hist<9> : k_radix_hist<8>;
+          hipLaunchKernelGGL(hist, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src] + w * scap, ns, shift, e->hist);
+          int rc = scan_exclusive(e, e->hist, e->hist_scan, (int64_t{1} << db) * nb);
           if (rc != FQ_OK) return rc;
-          auto scatter = kw == 1 ? k_radix_scatter<1> : k_radix_scatter<2>;
+          auto scatter = kw == 1 ? (db == 9 ? k_radix_scatter<1, 9> : k_radix_scatter<1, 8>)
+                                 : (db == 9 ? k_radix_scatter<2, 9> : k_radix_scatter<2, 8>);
           hipLaunchKernelGGL(scatter, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src], e->vbuf[src], ns, w, shift,
                              e->hist, e->hist_scan, e->kbuf[1 - src], e->vbuf[1 - src], scap);
           src = 1 - src;
+          shift += db;
           ++e->radix_passes;
         }
       }
