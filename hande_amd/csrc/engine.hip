@@ -183,6 +183,8 @@ struct fq_engine {
   int device = 0;
   hipStream_t stream = nullptr;
   Sys sys{};
+  Sys* d_sysp = nullptr;    // device copy of sys (read by the spawning kernel)
+  bool sys_dirty = true;    // sys changed since the last upload
   fq_params par{};
   int W = 1, S = 1;
   // device tables
@@ -491,7 +493,7 @@ __global__ void k_tile_bounds(int64_t nent, const int64_t* __restrict__ att_off,
   tile_win[2 * tile] = p0;
   tile_win[2 * tile + 1] = parent_of(att_off, t1 - 1, p0, nent - 1);
 }
-constexpr int kSpawnWin = 2048;          // parent-window entries staged in LDS
+constexpr int kSpawnWin = 1024;          // parent-window entries staged in LDS
 
 // Production: one thread per spawning attempt.  Each block takes tiles of
 // kSpawnTile consecutive attempts of the batch [t_begin, t_end); the parents
@@ -502,7 +504,7 @@ constexpr int kSpawnWin = 2048;          // parent-window entries staged in LDS
 // contiguously from the tile's first staging slot, with the tile's count in
 // tile_cnt; k_pack_tiles then copies the tiles, in order, into the spawned list.
 template <int G, bool B>
-__global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t cap, int64_t t_begin, int64_t t_end,
+__global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, int64_t nent, int64_t cap, int64_t t_begin, int64_t t_end,
                                                   const int64_t* __restrict__ att_off,
                                                   const uint64_t* __restrict__ det, const int64_t* __restrict__ pop,
                                                   double tau, uint64_t seed, uint32_t cyc, int64_t bcap,
@@ -511,6 +513,10 @@ __global__ __launch_bounds__(kBlock) void k_spawn(Sys s, int64_t nent, int64_t c
                                                   const int64_t* __restrict__ tile_win,
                                                   const Sys* __restrict__ sys_arr, const int64_t* __restrict__ iatt,
                                                   const double* __restrict__ itau) {
+  // the system descriptor is read from device memory (scalar loads on use)
+  // rather than passed by value: its many table pointers would otherwise sit
+  // in scalar registers for the whole kernel and spill into vector lanes
+  const Sys& s = *sdev;
   __shared__ int32_t wcount[kBlock / 64];
   // att_off[lo + k] - t0 (clamped below at -1: every such entry starts at or
   // before the tile's first attempt) when the window fits
@@ -2090,6 +2096,14 @@ __global__ void k_probe(Sys s, uint64_t f0, uint64_t f1, const float* __restrict
 }
 
 
+// Upload the system descriptor to its device copy when it has changed.
+int sync_sysp(fq_engine* e) {
+  if (!e->sys_dirty) return FQ_OK;
+  FQ_HIP(hipMemcpyAsync(e->d_sysp, &e->sys, sizeof(Sys), hipMemcpyHostToDevice, e->stream));
+  e->sys_dirty = false;
+  return FQ_OK;
+}
+
 // Dynamic LDS of the heat-bath spawning kernel: nel weights per thread.
 size_t hb_lds_bytes(int nel) { return static_cast<size_t>(nel) * kBlock * sizeof(double); }
 // Whether that LDS, with the kernel's static LDS, fits a workgroup here.
@@ -2162,6 +2176,7 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->rpos, scap);
   rc |= dalloc(&e->rstate, scap);
   rc |= dalloc(&e->nlist, scap);
+  rc |= dalloc(&e->d_sysp, 1);
   rc |= dalloc(&e->mwin, 2 * (scap / kBlock + 2));
   rc |= dalloc(&e->nmask, scap / 64 + 2);
   rc |= dalloc(&e->ncnt, scap / 64 + 2);
@@ -2428,6 +2443,7 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
         if (dalloc(&e->mlp_tab, rows * e->mlp.d_out) != FQ_OK) return FQ_ERR_NOMEM;
         e->mlp_tab_rows = rows;
         e->sys.mlp_tab = e->mlp_tab;
+        e->sys_dirty = true;
       }
       mrc = fqm::mlp_infer(e->mlp, e->sys, e->det[c], cap, ninf, e->mlp_tab, st, e->mlp_idx);
       if (mrc != FQ_OK) return mrc;
@@ -2441,6 +2457,7 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
     FQ_HIP(hipStreamSynchronize(st));
     FQ_HIP(hipEventRecord(e->pev[2], st));
     e->spawn_launches = 0;
+    if (int rs = sync_sysp(e); rs != FQ_OK) return rs;
     for (int64_t tb = 0; tb < total; tb += e->bcap) {
       const int64_t te = std::min(total, tb + e->bcap);
       const int64_t nbt = te - tb;
@@ -2458,7 +2475,7 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
                          : (gen == fqd::kGenMolRenorm || gen == fqd::kGenLearned)
                              ? sizeof(uint64_t) * e->W * kSpawnTile
                              : 0;
-      hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), lds, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
+      hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), lds, st, e->d_sysp, nent, cap, tb, te, e->att_off, e->det[c],
                          e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
                          e->tile_cnt, e->tile_win, nullptr, nullptr, nullptr);
       if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
@@ -2712,6 +2729,7 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
   FQ_HIP(hipStreamSynchronize(st));
   FQ_HIP(hipEventRecord(e->pev[2], st));
   e->spawn_launches = 0;
+  if (int rs = sync_sysp(e); rs != FQ_OK) return rs;
   for (int64_t tb = 0; tb < total; tb += e->bcap) {
     const int64_t te = std::min(total, tb + e->bcap);
     const int64_t nbt = te - tb;
@@ -2721,7 +2739,7 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
                        e->att_off, tb, te, e->tile_win);
     if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li], st));
     auto kern = e->sys.system == FQ_SYS_HUB_K ? k_spawn<fqd::kGenHubK, true> : k_spawn<fqd::kGenMolRenorm, true>;
-    hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->sys, nent, cap, tb, te, e->att_off, e->det[c],
+    hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->d_sysp, nent, cap, tb, te, e->att_off, e->det[c],
                        e->pop[c], 0.0, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
                        e->tile_cnt, e->tile_win, e->d_sys_arr, e->d_iatt, e->d_ipar);
     if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
@@ -3000,6 +3018,7 @@ void fq_destroy(fq_engine* e) {
   dfree(e->rpos);
   dfree(e->rstate);
   dfree(e->nlist);
+  dfree(e->d_sysp);
   dfree(e->mwin);
   dfree(e->nmask);
   dfree(e->ncnt);
