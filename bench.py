@@ -219,6 +219,28 @@ def traffic_summary(workload):
     return t if t.get("workload") == workload else None
 
 
+def issue_summary(workload):
+    """VALU issue of the spawning kernel from the committed rocprofv3 counter
+    pass (profiles/k_spawn_mol26_valu_raw.json), or None.  busy: fraction of
+    each SIMD's cycles with a VALU instruction in flight, SQ_ACTIVE_INST_VALU x
+    4 cycles / (1024 SIMDs x GRBM_GUI_ACTIVE / 8), the GRBM counter being summed
+    over the 8 XCDs (it equals 8 x kernel time x clock); lanes: active lanes
+    per VALU instruction / 64, SQ_THREAD_CYCLES_VALU / (64 x
+    SQ_ACTIVE_INST_VALU)."""
+    path = os.path.join(ROOT, "profiles", "k_spawn_mol26_valu_raw.json")
+    try:
+        with open(path) as f:
+            v = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if v.get("workload") != workload:
+        return None
+    c = v["raw_per_launch"]
+    return {"valu_busy": c["SQ_ACTIVE_INST_VALU"] * 4 / (1024 * c["GRBM_GUI_ACTIVE"] / 8),
+            "valu_lane_utilization": c["SQ_THREAD_CYCLES_VALU"] / (64 * c["SQ_ACTIVE_INST_VALU"]),
+            "valu_insts_per_launch": c["SQ_INSTS_VALU"], "source": "profiles/k_spawn_mol26_valu_raw.json"}
+
+
 def spawn_kernel_bytes(prof, W=1, S=1):
     """Algorithmic HBM bytes of the spawning-attempt kernel in one cycle: every
     parent's determinant words and population read once, every spawned entry
@@ -719,7 +741,9 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "launch_ms": k_ms, "bytes_per_launch": k_bytes, "launches": launches,
-                     "phase_ms": avg_ms, "phase_bytes": avg_bytes},
+                     "phase_ms": avg_ms, "phase_bytes": avg_bytes,
+                     # the kernel is bound by VALU issue, not HBM: counters of the committed profile
+                     "issue": issue_summary(workload)},
         "cpu_baseline": cpu,
     }
     lib.fq_destroy(eng)
