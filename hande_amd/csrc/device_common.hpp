@@ -197,10 +197,43 @@ __device__ __forceinline__ uint64_t after_mask(int orb, int w) {
   if (w > bw) return ~uint64_t{0};
   return bb == 63 ? 0 : (~uint64_t{0} << (bb + 1));
 }
-// k-th (0-based) set bit of x.
-__device__ __forceinline__ int kth_bit(uint64_t x, int k) {
-  for (int i = 0; i < k; ++i) x &= x - 1;
-  return __builtin_ctzll(x);
+// k-th (0-based) set bit of x (k < popcount(x)): a fixed descent over the
+// halves of the word by popcount (no loop whose trip count differs by lane).
+__host__ __device__ __forceinline__ int kth_bit(uint64_t x, int k) {
+  int pos = 0;
+  int c = __builtin_popcount(static_cast<uint32_t>(x));
+  if (k >= c) {
+    k -= c;
+    x >>= 32;
+    pos = 32;
+  }
+  uint32_t v = static_cast<uint32_t>(x);
+  c = __builtin_popcount(v & 0xffffu);
+  if (k >= c) {
+    k -= c;
+    v >>= 16;
+    pos += 16;
+  }
+  c = __builtin_popcount(v & 0xffu);
+  if (k >= c) {
+    k -= c;
+    v >>= 8;
+    pos += 8;
+  }
+  c = __builtin_popcount(v & 0xfu);
+  if (k >= c) {
+    k -= c;
+    v >>= 4;
+    pos += 4;
+  }
+  c = __builtin_popcount(v & 0x3u);
+  if (k >= c) {
+    k -= c;
+    v >>= 2;
+    pos += 2;
+  }
+  if (k >= static_cast<int>(v & 1u)) pos += 1;
+  return pos;
 }
 // Orbital (1-based) of the k-th set bit of (words & pattern) across W words.
 __device__ __forceinline__ int select_orb(const uint64_t* x, int W, int k) {
@@ -668,19 +701,25 @@ __device__ inline bool single_mol(const Sys& s, Rng& rng, const uint64_t* f, con
     }
   }
   if (allowed) {
+    // i until its spin/irrep has an unoccupied orbital, then a within it (two
+    // loops in sequence: the same draws as the nested form, one pass each)
     int i = 0, a = 0;
+    bool found = false;
     for (int tries = 0; tries < kMaxRejection; ++tries) {
       i = select_orb(f, s.W, static_cast<int>(rng.next() * nel));
-      const int imsa = ims_of(i), isyma = s.orb_sym[i - 1];
-      if (u.get(imsa, isyma) != 0) {
-        for (int t2 = 0; t2 < kMaxRejection; ++t2) {
-          const int ind = static_cast<int>(nbss(s, imsa, isyma) * rng.next()) + 1;
-          a = ssbf(s, imsa, isyma, ind - 1);
-          if (!btest(f, a) || rng.failed) break;
-        }
+      if (u.get(ims_of(i), s.orb_sym[i - 1]) != 0) {
+        found = true;
         break;
       }
       if (rng.failed) break;
+    }
+    if (found) {
+      const int imsa = ims_of(i), isyma = s.orb_sym[i - 1];
+      for (int t2 = 0; t2 < kMaxRejection; ++t2) {
+        const int ind = static_cast<int>(nbss(s, imsa, isyma) * rng.next()) + 1;
+        a = ssbf(s, imsa, isyma, ind - 1);
+        if (!btest(f, a) || rng.failed) break;
+      }
     }
     conn.from[0] = i;
     conn.to[0] = a;
@@ -746,24 +785,30 @@ __device__ inline bool gen_mol(const Sys& s, Rng& rng, const uint64_t* f, Excit&
     }
     const int fac = spin == 0 ? 1 : 2, shift = spin == 2 ? 1 : 0, na = spin == 0 ? s.nbasis : s.nbasis / 2;
     if (allowed) {
+      // a until unoccupied, then (when a partner can exist) b: two loops in
+      // sequence drawing exactly as the nested form
       int a = 0, b = 0;
+      bool found = false;
       for (int tries = 0; tries < kMaxRejection; ++tries) {
         a = static_cast<int>(rng.next() * na) + 1;
         a = fac * a - shift;
         if (!btest(f, a)) {
-          const int imsb = (spin - ms_of(a) + 3) / 2;
-          const int isymb = ij_sym ^ s.orb_sym[a - 1];
-          const int nu = u.get(imsb, isymb);
-          if (nu > 1 || (nu == 1 && (isymb != s.orb_sym[a - 1] || spin == 0))) {
-            for (int t2 = 0; t2 < kMaxRejection; ++t2) {
-              const int indb = static_cast<int>(nbss(s, imsb, isymb) * rng.next()) + 1;
-              b = ssbf(s, imsb, isymb, indb - 1);
-              if ((b != a && !btest(f, b)) || rng.failed) break;
-            }
-            break;
-          }
+          found = true;
+          break;
         }
         if (rng.failed) break;
+      }
+      if (found) {
+        const int imsb = (spin - ms_of(a) + 3) / 2;
+        const int isymb = ij_sym ^ s.orb_sym[a - 1];
+        const int nu = u.get(imsb, isymb);
+        if (nu > 1 || (nu == 1 && (isymb != s.orb_sym[a - 1] || spin == 0))) {
+          for (int t2 = 0; t2 < kMaxRejection; ++t2) {
+            const int indb = static_cast<int>(nbss(s, imsb, isymb) * rng.next()) + 1;
+            b = ssbf(s, imsb, isymb, indb - 1);
+            if ((b != a && !btest(f, b)) || rng.failed) break;
+          }
+        }
       }
       if (a > b) {
         const int t = a;
