@@ -70,3 +70,26 @@ def test_replay_lockstep_two_word_determinants(tmp_path, excit_gen):
                          excit_gen=excit_gen), rng_mode="replay")
     out = lockstep(system, st, 80, label=f"W=2 {excit_gen}")
     assert out.nstates > 50
+
+
+def test_hubbard_4x4_configuration_lockstep():
+    """The reference-runnable Hubbard configuration: 4x4 periodic lattice in
+    momentum space, 5 up + 5 down electrons, U/t = 4, totally symmetric
+    k-sector, integer populations, tau = 0.005, no initiator.  The GPU replays
+    the dSFMT stream with walker lists bit-identical to the oracle's after
+    every cycle (60 cycles from 1000 walkers on the Hartree--Fock determinant,
+    shift held at -18 relative to it), then the production path (Philox) does
+    the same against the oracle's production mode."""
+    from hande_amd import _native as N
+    from helpers import production_lockstep
+    system = api.hubbard_k(electrons=10, lattice=[[4, 0], [0, 4]], ms=0, sym="tot_sym", U=4, t=1)
+    print("[replay lockstep] Hubbard 4x4, 10 electrons", flush=True)
+    st = api.qmc_in(dict(tau=0.005, rng_seed=7, init_pop=1000), rng_mode="replay")
+    out = lockstep(system, st, 60, shift=-18.0, label="4x4 replay")
+    assert out.nstates > 5000
+    p = N.FqParams()
+    p.nspaces, p.initiator, p.real_amplitudes, p.pop_bit_shift, p.spawn_cutoff = 1, 0, 0, 0, 0
+    p.excit_gen, p.rng_mode, p.philox_seed = N.FQ_EXCIT_RENORM, N.FQ_RNG_PHILOX, 7
+    p.main_capacity = p.spawn_capacity = 1 << 18
+    outs = production_lockstep(system, p, 60, 1000, shift=-18.0, tau=0.005, label="4x4 production")
+    assert outs[-1].nstates > 5000
