@@ -720,12 +720,16 @@ __device__ inline bool single_mol(const Sys& s, Rng& rng, const uint64_t* f, con
         a = ssbf(s, imsa, isyma, ind - 1);
         if (!btest(f, a) || rng.failed) break;
       }
+      conn.from[0] = i;
+      conn.to[0] = a;
+      pgen = s.psingle * (1.0 / static_cast<double>(ni * u.get(ims_of(a), s.orb_sym[a - 1])));
+      conn.perm = perm1(f, s.W, i, a);
+      h = sc1_excit(s, f, i, a, conn.perm);
+    } else {  // draws ran out (replay stream exhausted) before an i was found: no excitation
+      h = 0.0;
+      pgen = 1.0;
+      allowed = false;
     }
-    conn.from[0] = i;
-    conn.to[0] = a;
-    pgen = s.psingle * (1.0 / static_cast<double>(ni * u.get(ims_of(a), s.orb_sym[a - 1])));
-    conn.perm = perm1(f, s.W, i, a);
-    h = sc1_excit(s, f, i, a, conn.perm);
   } else {
     h = 0.0;
     pgen = 1.0;
@@ -784,32 +788,38 @@ __device__ inline bool gen_mol(const Sys& s, Rng& rng, const uint64_t* f, Excit&
       }
     }
     const int fac = spin == 0 ? 1 : 2, shift = spin == 2 ? 1 : 0, na = spin == 0 ? s.nbasis : s.nbasis / 2;
+    int a = 0, b = 0;
+    bool found = false;
     if (allowed) {
-      // a until unoccupied, then (when a partner can exist) b: two loops in
-      // sequence drawing exactly as the nested form
-      int a = 0, b = 0;
-      bool found = false;
+      // a until it is unoccupied and has a possible partner (a free a without
+      // one is redrawn), then b: two loops in sequence drawing exactly as the
+      // reference's nested form
       for (int tries = 0; tries < kMaxRejection; ++tries) {
         a = static_cast<int>(rng.next() * na) + 1;
         a = fac * a - shift;
         if (!btest(f, a)) {
-          found = true;
-          break;
+          const int imsb = (spin - ms_of(a) + 3) / 2;
+          const int isymb = ij_sym ^ s.orb_sym[a - 1];
+          const int nu = u.get(imsb, isymb);
+          if (nu > 1 || (nu == 1 && (isymb != s.orb_sym[a - 1] || spin == 0))) {
+            found = true;
+            break;
+          }
         }
         if (rng.failed) break;
       }
       if (found) {
         const int imsb = (spin - ms_of(a) + 3) / 2;
         const int isymb = ij_sym ^ s.orb_sym[a - 1];
-        const int nu = u.get(imsb, isymb);
-        if (nu > 1 || (nu == 1 && (isymb != s.orb_sym[a - 1] || spin == 0))) {
-          for (int t2 = 0; t2 < kMaxRejection; ++t2) {
-            const int indb = static_cast<int>(nbss(s, imsb, isymb) * rng.next()) + 1;
-            b = ssbf(s, imsb, isymb, indb - 1);
-            if ((b != a && !btest(f, b)) || rng.failed) break;
-          }
+        for (int t2 = 0; t2 < kMaxRejection; ++t2) {
+          const int indb = static_cast<int>(nbss(s, imsb, isymb) * rng.next()) + 1;
+          b = ssbf(s, imsb, isymb, indb - 1);
+          if ((b != a && !btest(f, b)) || rng.failed) break;
         }
       }
+    }
+    if (allowed && !found) allowed = false;  // draws ran out before an a was found: no excitation
+    if (allowed) {
       if (a > b) {
         const int t = a;
         a = b;
