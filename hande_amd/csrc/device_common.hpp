@@ -65,8 +65,11 @@ struct Sys {
   const int* hb_ijab_k;
   // Learned proposal (excit_gen FQ_EXCIT_LEARNED): row d of mlp_tab holds the
   // hole weights (nbasis) then the particle weights (nbasis) of determinant d
-  // of the current main list; mix is the uniform-fallback fraction of doubles.
+  // of the current main list; mlp_aff[(h-1)*nbasis + p-1] is the learned
+  // hole-particle affinity of hole h and particle p; mix is the
+  // uniform-fallback fraction of doubles.
   const float* mlp_tab;
+  const float* mlp_aff;
   int64_t mlp_ld;
   const int64_t* mlp_on;   // per determinant: 1 if the network ran on it this cycle
   const int64_t* mlp_row;  // its row of mlp_tab (exclusive prefix of mlp_on)
@@ -1115,11 +1118,14 @@ __device__ inline int64_t spawn_heat_bath(const Sys& s, Rng& rng, const uint64_t
 // ---------------------------------------------------------------------------
 // Learned double-excitation proposal (no reference counterpart; an addition
 // beside the heat-bath generator).  Doubles come from a mixture: with
-// probability 1 - mix the network's per-determinant weights (hole pair by
-// weight among occupied orbitals, first particle by weight among unoccupied
-// ones, second particle by weight within the first one's spin/irrep partner
-// class), with probability mix a uniform choice (pair, particle, partner).
-// pgen is the exact mixture probability of the unordered excitation, so the
+// probability 1 - mix the network component, with probability mix a uniform
+// choice (pair, particle, partner).  The network component draws an ordered
+// hole pair (h1, h2) by the determinant's hole weights e (h1 by e, then h2 != h1
+// by e), the first particle p1 among the unoccupied orbitals by f_p * A[h1][p]
+// and its partner p2 within p1's spin/irrep partner class by f_p * A[h2][p]
+// (f: the determinant's particle weights, A: the learned hole-particle
+// affinity).  pgen is the exact mixture probability of the unordered
+// excitation, summed over the four paths (h1, p1) in {i, j} x {a, b}, so the
 // sampler is unbiased whatever the weights; singles use the renormalised
 // generator.
 // ---------------------------------------------------------------------------
@@ -1136,24 +1142,31 @@ __device__ inline PartnerClass partner_class(const Sys& s, int i, int j, int x) 
   c.sym = s.orb_sym[i - 1] ^ s.orb_sym[j - 1] ^ s.orb_sym[x - 1];
   return c;
 }
-// Unoccupied members of class c other than x: count and particle-weight sum.
-__device__ inline void partner_stats(const Sys& s, const uint64_t* f, const float* fw, const PartnerClass& c, int x,
-                                     int& count, double& z) {
+// Unoccupied members of class c other than x: their count and, when fw is
+// given, the sums of f_o * A[hi][o] (zi) and f_o * A[hj][o] (zj).
+__device__ inline void partner_stats(const Sys& s, const uint64_t* f, const float* fw, const float* ai,
+                                     const float* aj, const PartnerClass& c, int x, int& count, double& zi,
+                                     double& zj) {
   count = 0;
-  z = 0.0;
+  zi = zj = 0.0;
   if (!c.ok) return;
   const int nm = nbss(s, c.ims, c.sym);
   for (int k = 0; k < nm; ++k) {
     const int o = ssbf(s, c.ims, c.sym, k);
     if (o != x && !btest(f, o)) {
       ++count;
-      if (fw) z = z + static_cast<double>(fw[o - 1]);
+      if (fw) {
+        const double v = static_cast<double>(fw[o - 1]);
+        zi = zi + v * static_cast<double>(ai[o - 1]);
+        zj = zj + v * static_cast<double>(aj[o - 1]);
+      }
     }
   }
 }
-// Member of class c (unoccupied, not x) by index (uniform) or by weight target.
-__device__ inline int partner_pick(const Sys& s, const uint64_t* f, const float* fw, const PartnerClass& c, int x,
-                                   int kth, double target, bool by_weight) {
+// Member of class c (unoccupied, not x) by index (uniform) or by the weight
+// f_o * arow[o] reaching target.
+__device__ inline int partner_pick(const Sys& s, const uint64_t* f, const float* fw, const float* arow,
+                                   const PartnerClass& c, int x, int kth, double target, bool by_weight) {
   const int nm = nbss(s, c.ims, c.sym);
   int last = 0, seen = 0;
   double cum = 0.0;
@@ -1162,7 +1175,7 @@ __device__ inline int partner_pick(const Sys& s, const uint64_t* f, const float*
     if (o == x || btest(f, o)) continue;
     last = o;
     if (by_weight) {
-      cum = cum + static_cast<double>(fw[o - 1]);
+      cum = cum + static_cast<double>(fw[o - 1]) * static_cast<double>(arow[o - 1]);
       if (target < cum) return o;
     } else if (seen++ == kth) {
       return o;
@@ -1191,19 +1204,15 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
     const float* fw = wrow ? wrow + nb : nullptr;
     uint64_t un[FQ_MAX_WORDS];
     FQ_FOR_WORDS(w, s.W) un[w] = ~f[w] & s.valid[w];
-    double zh = 0.0, zp = 0.0;
+    double zh = 0.0;
     if (wrow) {
       FQ_FOR_WORDS(w, s.W) {
         for (uint64_t x = f[w]; x; x &= x - 1) zh = zh + static_cast<double>(ew[(w << 6) + __builtin_ctzll(x)]);
-        for (uint64_t x = un[w]; x; x &= x - 1) zp = zp + static_cast<double>(fw[(w << 6) + __builtin_ctzll(x)]);
       }
     }
     const int nvirt = nb - nel;
     int i = 0, j = 0, a = 0, b = 0;
     bool ok = true;
-    PartnerClass cb;
-    int nbc = 0;
-    double zb = 0.0;
     if (rng.next() < mix) {
       // uniform component: unordered hole pair (choose_ij_mol's index map), a, partner
       const int ind = static_cast<int>(rng.next() * nel * (nel - 1) / 2) + 1;
@@ -1212,14 +1221,16 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
       i = select_orb(f, s.W, i_ind - 1);
       j = select_orb(f, s.W, j_ind - 1);
       a = select_orb(un, s.W, static_cast<int>(rng.next() * nvirt));
-      cb = partner_class(s, i, j, a);
-      partner_stats(s, f, fw, cb, a, nbc, zb);
+      const PartnerClass cb = partner_class(s, i, j, a);
+      int nbc = 0;
+      double zi = 0.0, zj = 0.0;
+      partner_stats(s, f, nullptr, nullptr, nullptr, cb, a, nbc, zi, zj);
       if (nbc == 0)
         ok = false;
       else
-        b = partner_pick(s, f, fw, cb, a, static_cast<int>(rng.next() * nbc), 0.0, false);
+        b = partner_pick(s, f, nullptr, nullptr, cb, a, static_cast<int>(rng.next() * nbc), 0.0, false);
     } else {
-      // network component: holes by weight (i, then j != i), a by weight, partner by weight
+      // network component: holes by weight (i, then j != i), a by f A[i], partner by f A[j]
       double x = rng.next() * zh, cum = 0.0;
       FQ_FOR_WORDS(w, s.W) if (!i)
         for (uint64_t y = f[w]; y; y &= y - 1) {
@@ -1246,37 +1257,62 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
           }
         }
       if (!j) j = lastj;
-      x = rng.next() * zp;
+      const float* ai = s.mlp_aff + static_cast<int64_t>(i - 1) * nb;
+      const float* aj = s.mlp_aff + static_cast<int64_t>(j - 1) * nb;
+      double z1 = 0.0;
+      FQ_FOR_WORDS(w, s.W) {
+        for (uint64_t y = un[w]; y; y &= y - 1) {
+          const int o = (w << 6) + __builtin_ctzll(y);
+          z1 = z1 + static_cast<double>(fw[o]) * static_cast<double>(ai[o]);
+        }
+      }
+      x = rng.next() * z1;
       cum = 0.0;
       int lasta = 0;
       FQ_FOR_WORDS(w, s.W) if (!a)
         for (uint64_t y = un[w]; y; y &= y - 1) {
           const int o = (w << 6) + __builtin_ctzll(y) + 1;
           lasta = o;
-          cum = cum + static_cast<double>(fw[o - 1]);
+          cum = cum + static_cast<double>(fw[o - 1]) * static_cast<double>(ai[o - 1]);
           if (x < cum) {
             a = o;
             break;
           }
         }
       if (!a) a = lasta;
-      cb = partner_class(s, i, j, a);
-      partner_stats(s, f, fw, cb, a, nbc, zb);
+      const PartnerClass cb = partner_class(s, i, j, a);
+      int nbc = 0;
+      double zbi = 0.0, zbj = 0.0;
+      partner_stats(s, f, fw, ai, aj, cb, a, nbc, zbi, zbj);
       if (nbc == 0)
         ok = false;
       else
-        b = partner_pick(s, f, fw, cb, a, 0, rng.next() * zb, true);
+        b = partner_pick(s, f, fw, aj, cb, a, 0, rng.next() * zbj, true);
     }
     if (ok) {
-      int nac = 0;
-      double za = 0.0;
-      partner_stats(s, f, fw, partner_class(s, i, j, b), b, nac, za);
+      const float* ai = wrow ? s.mlp_aff + static_cast<int64_t>(i - 1) * nb : nullptr;
+      const float* aj = wrow ? s.mlp_aff + static_cast<int64_t>(j - 1) * nb : nullptr;
+      int nbc = 0, nac = 0;
+      double zbi = 0.0, zbj = 0.0, zai = 0.0, zaj = 0.0;  // Z2 of holes i, j over a's (b's) partner class
+      partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, a), a, nbc, zbi, zbj);
+      partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, b), b, nac, zai, zaj);
       const double q_unif = 2.0 / static_cast<double>(nel * (nel - 1)) / nvirt * (1.0 / nbc + 1.0 / nac);
       if (wrow) {
+        double z1i = 0.0, z1j = 0.0;
+        FQ_FOR_WORDS(w, s.W) {
+          for (uint64_t y = un[w]; y; y &= y - 1) {
+            const int o = (w << 6) + __builtin_ctzll(y);
+            const double v = static_cast<double>(fw[o]);
+            z1i = z1i + v * static_cast<double>(ai[o]);
+            z1j = z1j + v * static_cast<double>(aj[o]);
+          }
+        }
         const double ei = ew[i - 1], ej = ew[j - 1], fa = fw[a - 1], fb = fw[b - 1];
-        const double hp = ei / zh * ej / (zh - ei) + ej / zh * ei / (zh - ej);
-        const double pp = fa / zp * fb / zb + fb / zp * fa / za;
-        pgen = s.pdouble * ((1.0 - mix) * (hp * pp) + mix * q_unif);
+        const double aia = ai[a - 1], aib = ai[b - 1], aja = aj[a - 1], ajb = aj[b - 1];
+        const double pij = ei / zh * ej / (zh - ei), pji = ej / zh * ei / (zh - ej);
+        const double q_mlp = pij * (fa * aia / z1i) * (fb * ajb / zbj) + pij * (fb * aib / z1i) * (fa * aja / zaj) +
+                             pji * (fa * aja / z1j) * (fb * aib / zbi) + pji * (fb * ajb / z1j) * (fa * aia / zai);
+        pgen = s.pdouble * ((1.0 - mix) * q_mlp + mix * q_unif);
       } else {
         pgen = s.pdouble * q_unif;
       }

@@ -2048,6 +2048,7 @@ int setup_learned(fq_engine* e) {
       dalloc(&e->mlp_row, e->cap + 1) != FQ_OK || dalloc(&e->mlp_idx, e->cap) != FQ_OK)
     return FQ_ERR_NOMEM;
   s.mlp_tab = e->mlp_tab;
+  s.mlp_aff = e->mlp.aff;
   s.mlp_ld = e->mlp.d_out;
   s.mlp_on = e->mlp_on;
   s.mlp_row = e->mlp_row;
@@ -2834,6 +2835,15 @@ int fq_mlp_get_params(fq_engine* e, float* p, int64_t n) {
   if (!e || !p || e->par.excit_gen != FQ_EXCIT_LEARNED || n != e->mlp.nparams) return FQ_ERR_ARG;
   FQ_HIP(hipSetDevice(e->device));
   FQ_HIP(hipMemcpyAsync(p, e->mlp.p, sizeof(float) * n, hipMemcpyDeviceToHost, e->stream));
+  FQ_HIP(hipStreamSynchronize(e->stream));
+  return FQ_OK;
+}
+
+int fq_mlp_affinity(fq_engine* e, float* out, int64_t n) {
+  if (!e || !out || e->par.excit_gen != FQ_EXCIT_LEARNED) return FQ_ERR_ARG;
+  if (n != static_cast<int64_t>(e->mlp.nb) * e->mlp.nb) return FQ_ERR_ARG;
+  FQ_HIP(hipSetDevice(e->device));
+  FQ_HIP(hipMemcpyAsync(out, e->mlp.aff, sizeof(float) * n, hipMemcpyDeviceToHost, e->stream));
   FQ_HIP(hipStreamSynchronize(e->stream));
   return FQ_OK;
 }

@@ -216,18 +216,25 @@ __global__ void k_dsum(int64_t n, const double* __restrict__ x, double* __restri
 }
 
 // Weighted NLL of each sample's double under the network distribution and its
-// gradient with respect to the logits (one thread per sample).
+// gradient with respect to the logits (d3) and to the affinity rows of the
+// sample's two holes (dth: row i then row j, nb each), one thread per sample.
+// The network component sums four paths (h1, h2, p1, p2): the first particle
+// p1 is drawn with h1's affinity over the unoccupied orbitals (normaliser
+// Z1(h1)), its partner p2 with h2's affinity over p1's partner class (Z2).
 __global__ void k_mlp_loss_grad(fqd::Sys s, int64_t ns, int nb, int d_out, const uint64_t* __restrict__ sdet,
                                 int64_t bcap, const int32_t* __restrict__ sexc, const double* __restrict__ sw,
                                 const double* __restrict__ wsum, const float* __restrict__ z3,
-                                float* __restrict__ d3, double* __restrict__ sloss) {
+                                const float* __restrict__ th, float* __restrict__ d3, float* __restrict__ dth,
+                                double* __restrict__ sloss) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= ns) return;
   const float* z = z3 + t * d_out;
   float* d = d3 + t * d_out;
+  float* gth = dth + t * 2 * nb;
   const double wt = sw[t] / *wsum;
   if (!(sw[t] > 0.0)) {
     for (int c = 0; c < d_out; ++c) d[c] = 0.0f;
+    for (int c = 0; c < 2 * nb; ++c) gth[c] = 0.0f;
     sloss[t] = 0.0;
     return;
   }
@@ -236,27 +243,44 @@ __global__ void k_mlp_loss_grad(fqd::Sys s, int64_t ns, int nb, int d_out, const
   const int i = sexc[4 * t], j = sexc[4 * t + 1], a = sexc[4 * t + 2], b = sexc[4 * t + 3];
   auto ew = [&](int o) { return exp(static_cast<double>(fminf(fmaxf(z[o - 1], -kClip), kClip))); };
   auto fw = [&](int o) { return exp(static_cast<double>(fminf(fmaxf(z[nb + o - 1], -kClip), kClip))); };
+  auto aff = [&](int hole, int o) {
+    return exp(static_cast<double>(fminf(fmaxf(th[(hole - 1) * nb + o - 1], -kClip), kClip)));
+  };
   const fqd::PartnerClass cb = fqd::partner_class(s, i, j, a), ca = fqd::partner_class(s, i, j, b);
   auto in_class = [&](const fqd::PartnerClass& c, int o, int x) {
     return c.ok && o != x && !fqd::btest(f, o) && fqd::ims_of(o) == c.ims && s.orb_sym[o - 1] == c.sym;
   };
-  double zh = 0.0, zp = 0.0, zb = 0.0, za = 0.0;
+  // Z1(i), Z1(j); Z2 of holes i, j over b's class given a first (zbi, zbj) and a's given b (zai, zaj)
+  double zh = 0.0, z1i = 0.0, z1j = 0.0, zbi = 0.0, zbj = 0.0, zai = 0.0, zaj = 0.0;
   for (int o = 1; o <= nb; ++o) {
     if (fqd::btest(f, o)) {
       zh = zh + ew(o);
     } else {
-      const double v = fw(o);
-      zp = zp + v;
-      if (in_class(cb, o, a)) zb = zb + v;
-      if (in_class(ca, o, b)) za = za + v;
+      const double v = fw(o), wi = v * aff(i, o), wj = v * aff(j, o);
+      z1i = z1i + wi;
+      z1j = z1j + wj;
+      if (in_class(cb, o, a)) {
+        zbi = zbi + wi;
+        zbj = zbj + wj;
+      }
+      if (in_class(ca, o, b)) {
+        zai = zai + wi;
+        zaj = zaj + wj;
+      }
     }
   }
   const double ei = ew(i), ej = ew(j), fa = fw(a), fb = fw(b);
-  const double pij = ei / zh * ej / (zh - ei), pji = ej / zh * ei / (zh - ej), hp = pij + pji;
-  const double pab = fa / zp * fb / zb, pba = fb / zp * fa / za, pp = pab + pba;
-  sloss[t] = wt * (-log(hp) - log(pp));
+  const double aia = aff(i, a), aib = aff(i, b), aja = aff(j, a), ajb = aff(j, b);
+  const double pij = ei / zh * ej / (zh - ei), pji = ej / zh * ei / (zh - ej);
+  const double t1 = pij * (fa * aia / z1i) * (fb * ajb / zbj);  // (i, j, a, b)
+  const double t2 = pij * (fb * aib / z1i) * (fa * aja / zaj);  // (i, j, b, a)
+  const double t3 = pji * (fa * aja / z1j) * (fb * aib / zbi);  // (j, i, a, b)
+  const double t4 = pji * (fb * ajb / z1j) * (fa * aia / zai);  // (j, i, b, a)
+  const double q = t1 + t2 + t3 + t4;
+  const double r1 = t1 / q, r2 = t2 / q, r3 = t3 / q, r4 = t4 / q;
+  sloss[t] = wt * -log(q);
   for (int o = 1; o <= nb; ++o) {
-    double gu = 0.0, gv = 0.0;
+    double gu = 0.0, gi = 0.0, gj = 0.0;
     if (fqd::btest(f, o)) {
       const double e = ew(o);
       double gij = -e / zh - (o != i ? e / (zh - ei) : 0.0);
@@ -265,22 +289,56 @@ __global__ void k_mlp_loss_grad(fqd::Sys s, int64_t ns, int nb, int d_out, const
         gij += 1.0;
         gji += 1.0;
       }
-      gu = (pij * gij + pji * gji) / hp;
+      gu = (r1 + r2) * gij + (r3 + r4) * gji;
     } else {
-      const double v = fw(o);
-      double gab = -v / zp - (in_class(cb, o, a) ? v / zb : 0.0);
-      double gba = -v / zp - (in_class(ca, o, b) ? v / za : 0.0);
-      if (o == a || o == b) {
-        gab += 1.0;
-        gba += 1.0;
-      }
-      gv = (pab * gab + pba * gba) / pp;
+      const double v = fw(o), wi = v * aff(i, o), wj = v * aff(j, o);
+      const double da = o == a ? 1.0 : 0.0, db = o == b ? 1.0 : 0.0;
+      const bool inb = in_class(cb, o, a), ina = in_class(ca, o, b);
+      gi = r1 * (da - wi / z1i) + r2 * (db - wi / z1i) + r3 * (db - (inb ? wi / zbi : 0.0)) +
+           r4 * (da - (ina ? wi / zai : 0.0));
+      gj = r1 * (db - (inb ? wj / zbj : 0.0)) + r2 * (da - (ina ? wj / zaj : 0.0)) + r3 * (da - wj / z1j) +
+           r4 * (db - wj / z1j);
     }
     const bool in_h = fabsf(z[o - 1]) < kClip, in_p = fabsf(z[nb + o - 1]) < kClip;
     d[o - 1] = in_h ? static_cast<float>(-wt * gu) : 0.0f;
-    d[nb + o - 1] = in_p ? static_cast<float>(-wt * gv) : 0.0f;
+    d[nb + o - 1] = in_p ? static_cast<float>(-wt * (gi + gj)) : 0.0f;
+    gth[o - 1] = fabsf(th[(i - 1) * nb + o - 1]) < kClip ? static_cast<float>(-wt * gi) : 0.0f;
+    gth[nb + o - 1] = fabsf(th[(j - 1) * nb + o - 1]) < kClip ? static_cast<float>(-wt * gj) : 0.0f;
   }
   for (int c = 2 * nb; c < d_out; ++c) d[c] = 0.0f;
+}
+
+// Affinity gradient, deterministic: block (h, c) sums, in sample order, the
+// row gradients of the samples [c*chunk, (c+1)*chunk) whose hole i or j is
+// h + 1 into part[c][h][.]; k_aff_grad_sum adds the chunks in order.
+__global__ void k_aff_grad_part(int64_t ns, int nb, int64_t chunk, const int32_t* __restrict__ sexc,
+                                const double* __restrict__ sw, const float* __restrict__ dth,
+                                float* __restrict__ part) {
+  const int h = blockIdx.x + 1, c = blockIdx.y, o = threadIdx.x;
+  if (o >= nb) return;
+  const int64_t s0 = c * chunk, s1 = s0 + chunk < ns ? s0 + chunk : ns;
+  float acc = 0.0f;
+  for (int64_t t = s0; t < s1; ++t) {
+    if (!(sw[t] > 0.0)) continue;
+    if (sexc[4 * t] == h) acc += dth[t * 2 * nb + o];
+    if (sexc[4 * t + 1] == h) acc += dth[t * 2 * nb + nb + o];
+  }
+  part[(static_cast<int64_t>(c) * nb + (h - 1)) * nb + o] = acc;
+}
+
+__global__ void k_aff_grad_sum(int nb, int nchunk, const float* __restrict__ part, float* __restrict__ g) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n = static_cast<int64_t>(nb) * nb;
+  if (k >= n) return;
+  float acc = 0.0f;
+  for (int c = 0; c < nchunk; ++c) acc += part[c * n + k];
+  g[k] = acc;
+}
+
+// The stored affinity table: exp(clamp(theta, -60, 60)) rounded to float.
+__global__ void k_aff_table(int64_t n, const float* __restrict__ th, float* __restrict__ aff) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < n) aff[k] = static_cast<float>(exp(static_cast<double>(fminf(fmaxf(th[k], -kClip), kClip))));
 }
 
 __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -387,8 +445,10 @@ int mlp_alloc(Mlp& m, int nb, int W, int64_t bcap) {
   m.o_b2 = m.o_w2 + static_cast<int64_t>(kHidden) * kHidden;
   m.o_w3 = m.o_b2 + kHidden;
   m.o_b3 = m.o_w3 + static_cast<int64_t>(kHidden) * m.d_out;
-  m.nparams = m.o_b3 + m.d_out;
+  m.o_th = m.o_b3 + m.d_out;
+  m.nparams = m.o_th + static_cast<int64_t>(nb) * nb;
   m.bcap = bcap;
+  m.nchunk = static_cast<int>(ceil_div(bcap, kAffChunk));
   int rc = FQ_OK;
   rc |= dalloc(&m.p, m.nparams);
   rc |= dalloc(&m.g, m.nparams);
@@ -411,6 +471,9 @@ int mlp_alloc(Mlp& m, int nb, int W, int64_t bcap) {
   rc |= dalloc(&m.sw, bcap);
   rc |= dalloc(&m.sloss, bcap);
   rc |= dalloc(&m.dsum, 2);
+  rc |= dalloc(&m.aff, static_cast<int64_t>(nb) * nb);
+  rc |= dalloc(&m.dth, bcap * 2 * nb);
+  rc |= dalloc(&m.aff_part, static_cast<int64_t>(m.nchunk) * nb * nb);
   return rc == FQ_OK ? FQ_OK : FQ_ERR_NOMEM;
 }
 
@@ -436,6 +499,9 @@ void mlp_free(Mlp& m) {
   dfree(m.sw);
   dfree(m.sloss);
   dfree(m.dsum);
+  dfree(m.aff);
+  dfree(m.dth);
+  dfree(m.aff_part);
   dfree(m.scum);
   dfree(m.swts);
   m.scum_cap = 0;
@@ -446,6 +512,8 @@ int mlp_refresh(Mlp& m, hipStream_t st) {
                         static_cast<int64_t>(m.n3) * kHidden;
   hipLaunchKernelGGL(k_mlp_pack, dim3(blocks(total)), dim3(kBlock), 0, st, m.d_in, m.k1, m.d_out, m.n3, m.p, m.o_w1,
                      m.o_w2, m.o_w3, m.w1t, m.w2t, m.w3t);
+  const int64_t na = static_cast<int64_t>(m.nb) * m.nb;
+  hipLaunchKernelGGL(k_aff_table, dim3(blocks(na)), dim3(kBlock), 0, st, na, m.p + m.o_th, m.aff);
   MLP_HIP(hipGetLastError());
   return FQ_OK;
 }
@@ -498,8 +566,15 @@ int mlp_train_step(Mlp& m, const fqd::Sys& s, int64_t ns, double lr, bool apply,
   gemm(B, dout, H, m.a2, H, 0, m.p + m.o_w3, dout, 0, m.p + m.o_b3, m.z3, dout, st);
   hipLaunchKernelGGL(k_dsum, dim3(1), dim3(kBlock), 0, st, ns, m.sw, m.dsum);
   hipLaunchKernelGGL(k_mlp_loss_grad, dim3(blocks(ns)), dim3(kBlock), 0, st, s, ns, m.nb, dout, m.sdet, m.bcap,
-                     m.sexc, m.sw, m.dsum, m.z3, m.d3, m.sloss);
+                     m.sexc, m.sw, m.dsum, m.z3, m.p + m.o_th, m.d3, m.dth, m.sloss);
   hipLaunchKernelGGL(k_dsum, dim3(1), dim3(kBlock), 0, st, ns, m.sloss, m.dsum + 1);
+  const int nchunk = static_cast<int>(ceil_div(ns, kAffChunk));
+  hipLaunchKernelGGL(k_aff_grad_part, dim3(static_cast<unsigned>(m.nb), static_cast<unsigned>(nchunk)),
+                     dim3(kHidden / 2), 0, st, ns, m.nb, static_cast<int64_t>(kAffChunk), m.sexc, m.sw, m.dth,
+                     m.aff_part);
+  const int64_t na = static_cast<int64_t>(m.nb) * m.nb;
+  hipLaunchKernelGGL(k_aff_grad_sum, dim3(blocks(na)), dim3(kBlock), 0, st, m.nb, nchunk, m.aff_part,
+                     m.g + m.o_th);
   // backward
   gemm(H, dout, B, m.a2, H, 1, m.d3, dout, 0, nullptr, m.g + m.o_w3, dout, st);
   hipLaunchKernelGGL(k_colsum, dim3(blocks(dout)), dim3(kBlock), 0, st, B, dout, m.d3, dout, m.g + m.o_b3);
@@ -516,8 +591,11 @@ int mlp_train_step(Mlp& m, const fqd::Sys& s, int64_t ns, double lr, bool apply,
     m.adam_t += 1;
     const float c1 = static_cast<float>(1.0 - std::pow(0.9, static_cast<double>(m.adam_t)));
     const float c2 = static_cast<float>(1.0 - std::pow(0.999, static_cast<double>(m.adam_t)));
-    hipLaunchKernelGGL(k_adam, dim3(blocks(m.nparams)), dim3(kBlock), 0, st, m.nparams, m.p, m.g, m.m, m.v,
+    hipLaunchKernelGGL(k_adam, dim3(blocks(m.o_th)), dim3(kBlock), 0, st, m.o_th, m.p, m.g, m.m, m.v,
                        static_cast<float>(lr), c1, c2);
+    const int64_t na = m.nparams - m.o_th;
+    hipLaunchKernelGGL(k_adam, dim3(blocks(na)), dim3(kBlock), 0, st, na, m.p + m.o_th, m.g + m.o_th,
+                       m.m + m.o_th, m.v + m.o_th, static_cast<float>(lr * kAffLrScale), c1, c2);
     MLP_HIP(hipGetLastError());
     return mlp_refresh(m, st);
   }

@@ -2,8 +2,12 @@
 // from a determinant's occupations to per-orbital hole and particle weights.
 //
 // Parameters (float32 master copy, one flat vector):
-//   W1[d_in][256], b1[256], W2[256][256], b2[256], W3[256][d_out], b3[d_out]
+//   W1[d_in][256], b1[256], W2[256][256], b2[256], W3[256][d_out], b3[d_out],
+//   theta[nbasis][nbasis]
 // with d_in = d_out = 2*nbasis; input x_k = n_k, x_{nb+k} = n_k * eps_k.
+// theta is the determinant-independent hole-particle affinity (logits; zero
+// at initialisation, where the proposal is the product of hole and particle
+// weights); the generator reads aff = exp(clamp(theta, -60, 60)) as float.
 // Inference uses bf16 copies of W1..W3 (transposed to [N][K], K padded to a
 // multiple of 32) on the bf16 MFMA with fp32 accumulation; the stored
 // proposal weights are exp(clamp(z3, -60, 60)).  Training (fp32) minimises
@@ -19,10 +23,12 @@ namespace fqm {
 
 constexpr int kHidden = 256;
 constexpr float kClip = 60.0f;
+constexpr int kAffChunk = 512;        // samples per block of the affinity-gradient reduction
+constexpr double kAffLrScale = 10.0;  // Adam step of theta relative to the network's
 
 struct Mlp {
   int nb = 0, d_in = 0, d_out = 0, k1 = 0, n3 = 0;  // k1: d_in padded to 32; n3: d_out padded to 16
-  int64_t nparams = 0, o_w1 = 0, o_b1 = 0, o_w2 = 0, o_b2 = 0, o_w3 = 0, o_b3 = 0;
+  int64_t nparams = 0, o_w1 = 0, o_b1 = 0, o_w2 = 0, o_b2 = 0, o_w3 = 0, o_b3 = 0, o_th = 0;
   float* p = nullptr;  // master parameters
   float* g = nullptr;  // gradient
   float* m = nullptr;  // Adam moments
@@ -31,10 +37,14 @@ struct Mlp {
   uint16_t* w1t = nullptr;  // bf16 bits [256][k1]
   uint16_t* w2t = nullptr;  // [256][256]
   uint16_t* w3t = nullptr;  // [n3][256]
+  float* aff = nullptr;     // [nb][nb] exp(clamp(theta)), read by the generator
   // training batch storage (capacity bcap samples)
   int64_t bcap = 0;
   float *x = nullptr, *z1 = nullptr, *a1 = nullptr, *z2 = nullptr, *a2 = nullptr, *z3 = nullptr;
   float *d3 = nullptr, *d2 = nullptr, *d1 = nullptr;
+  float* dth = nullptr;       // [bcap][2][nb]: per-sample gradients of the affinity rows of holes i, j
+  float* aff_part = nullptr;  // [nchunk][nb][nb]: chunk sums of dth
+  int nchunk = 0;
   uint64_t* sdet = nullptr;  // [W][bcap]
   int32_t* sexc = nullptr;   // [bcap][4]: i, j, a, b (i<j, a<b)
   double* sw = nullptr;      // sample weights

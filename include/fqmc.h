@@ -215,16 +215,21 @@ int fq_cycle_batch(fq_engine *e, const double *shift, const double *eproj_old, c
  * No counterpart in the reference: an addition beside its heat-bath generator
  * (src/excit_gen_heat_bath_mol.F90), which produces the training samples.
  * The network maps a determinant's occupations (and occupied orbital
- * energies) through two 256-wide GELU layers to hole and particle weights;
- * parameters are float32, laid out W1[2n][256], b1[256], W2[256][256],
- * b2[256], W3[256][2n], b3[2n] (n = nbasis).  fq_create initialises them from
- * philox_seed; inference runs in bf16 with fp32 accumulation every cycle. */
+ * energies) through two 256-wide GELU layers to hole and particle weights; a
+ * learned determinant-independent hole-particle affinity table weights the
+ * particle paired with each hole.  Parameters are float32, laid out
+ * W1[2n][256], b1[256], W2[256][256], b2[256], W3[256][2n], b3[2n],
+ * theta[n][n] (n = nbasis; affinity exp(clamp(theta, -60, 60))).  fq_create
+ * initialises the network from philox_seed and theta to zero; inference runs
+ * in bf16 with fp32 accumulation every cycle. */
 int fq_mlp_nparams(fq_engine *e, int64_t *n);
 int fq_mlp_set_params(fq_engine *e, const float *params, int64_t n);  /* also resets the optimiser */
 int fq_mlp_get_params(fq_engine *e, float *params, int64_t n);
 /* Proposal weights exp(clamp(logit, -60, 60)) of n determinants (n x W words,
  * determinant-major as in fq_load) into out (n x 2*nbasis: holes, then particles). */
 int fq_mlp_forward(fq_engine *e, const uint64_t *dets, int64_t n, float *out);
+/* The stored affinity table (n = nbasis * nbasis floats, [hole-1][particle-1]). */
+int fq_mlp_affinity(fq_engine *e, float *out, int64_t n);
 /* Training loss sum_s w_s (-log q(ex_s | D_s)) / sum_s w_s of n samples
  * (exc: n x 4 = i, j, a, b with i<j, a<b) and its gradient (nparams floats;
  * grad may be NULL); fp32 forward and backward. */

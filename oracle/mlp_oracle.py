@@ -16,14 +16,26 @@ roundings.  The per-determinant proposal weights are
     f_k = exp(clip(z3_{nb+k}, -60, 60))   particle weights
 stored as float32; every probability below is computed from the stored values.
 
-Proposal for a double excitation {i,j} -> {a,b} from determinant D (occupied
-set O, unoccupied U), mixture with fraction ``mix`` of the uniform fallback:
+Hole-particle affinity.  A learned, determinant-independent table
+theta[h][p] (nbasis x nbasis, after b3 in the parameter vector, zero at
+initialisation) gives A[h][p] = exp(clip(theta, -60, 60)), stored as float32.
+The network component draws an ordered hole pair (h1, h2), the first particle
+p1 paired with h1, then its partner p2 paired with h2:
+    P(h1, h2) = e_h1/Zh * e_h2/(Zh - e_h1)
+    P(p1 | h1) = f_p1 A[h1][p1] / Z1(h1),           Z1(h) = sum_{x in U} f_x A[h][x]
+    P(p2 | h2, p1) = f_p2 A[h2][p2] / Z2(h2, C),     Z2(h, C) = sum_{x in C} f_x A[h][x]
+with C = C(p1) the unoccupied orbitals sharing p2's required spin and irrep,
+excluding p1.  With A = 1 this is the product proposal of per-orbital hole and
+particle weights.  For the unordered double {i,j} -> {a,b} the four paths
+(h1, p1) in {i, j} x {a, b} sum to
+
+    q_mlp = P(i,j) [f_a A_ia/Z1(i) f_b A_jb/Z2(j,C_b) + f_b A_ib/Z1(i) f_a A_ja/Z2(j,C_a)]
+          + P(j,i) [f_a A_ja/Z1(j) f_b A_ib/Z2(i,C_b) + f_b A_jb/Z1(j) f_a A_ia/Z2(i,C_a)]
+
+and the mixture with fraction ``mix`` of the uniform fallback is
     q = (1 - mix) q_mlp + mix q_unif
-    q_mlp = [e_i/Zh * e_j/(Zh-e_i) + e_j/Zh * e_i/(Zh-e_j)]
-            * [f_a/Zp * f_b/Z(C_b \\ a) + f_b/Zp * f_a/Z(C_a \\ b)]
-    q_unif = 2/(nel(nel-1)) * 1/nvirt * [1/|C_b \\ a| + 1/|C_a \\ b|]
-with Zh = sum_O e, Zp = sum_U f, C_x the unoccupied orbitals sharing x's spin
-and irrep (the class the second particle must come from), Z(.) sums of f.
+    q_unif = 2/(nel(nel-1)) * 1/nvirt * [1/|C_b| + 1/|C_a|]
+with C_x the unoccupied orbitals the partner of x may come from (x excluded).
 
 Training loss for samples (D, ex, w): L = sum_s w_s (-log q_mlp(ex_s|D_s)) / sum_s w_s.
 """
@@ -38,7 +50,7 @@ def sizes(nb):
     o = {}
     off = 0
     for name, n in (("W1", d_in * HIDDEN), ("b1", HIDDEN), ("W2", HIDDEN * HIDDEN), ("b2", HIDDEN),
-                    ("W3", HIDDEN * d_out), ("b3", d_out)):
+                    ("W3", HIDDEN * d_out), ("b3", d_out), ("TH", nb * nb)):
         o[name] = (off, n)
         off += n
     return d_in, d_out, o, off
@@ -52,8 +64,22 @@ def unpack(params, nb):
             g("W3").reshape(HIDDEN, d_out), g("b3"))
 
 
-def pack(W1, b1, W2, b2, W3, b3):
-    return np.concatenate([W1.ravel(), b1, W2.ravel(), b2, W3.ravel(), b3])
+def theta(params, nb):
+    """The hole-particle affinity logits, (nb, nb) [hole - 1][particle - 1]."""
+    _d_in, _d_out, o, _ = sizes(nb)
+    p = np.asarray(params, dtype=np.float64)
+    return p[o["TH"][0]:o["TH"][0] + o["TH"][1]].reshape(nb, nb)
+
+
+def affinity(params, nb):
+    """The stored affinity table A = exp(clip(theta)), float32."""
+    return np.exp(np.clip(theta(params, nb), -CLIP, CLIP)).astype(np.float32)
+
+
+def pack(W1, b1, W2, b2, W3, b3, TH=None):
+    nb = len(b3) // 2
+    TH = np.zeros(nb * nb) if TH is None else np.asarray(TH).ravel()
+    return np.concatenate([W1.ravel(), b1, W2.ravel(), b2, W3.ravel(), b3, TH])
 
 
 def init_params(nb, seed):
@@ -66,7 +92,8 @@ def init_params(nb, seed):
     b1 = rng.normal(0, 0.1, HIDDEN)
     b2 = rng.normal(0, 0.1, HIDDEN)
     b3 = rng.normal(0, 0.1, d_out)
-    return pack(W1, b1, W2, b2, W3, b3).astype(np.float32)
+    TH = rng.normal(0, 0.5, (nb, nb))
+    return pack(W1, b1, W2, b2, W3, b3, TH).astype(np.float32)
 
 
 def bf16(x):
@@ -148,60 +175,78 @@ def second_class(occset, orb_sym, nb, i, j, x):
             if b not in occset and b != x and ms(b) == msb and orb_sym[b - 1] == symb]
 
 
-def q_double(wrow, occ_list, orb_sym, nb, i, j, a, b, mix):
+def _aff_or_ones(aff, nb):
+    return np.ones((nb, nb)) if aff is None else np.asarray(aff, dtype=np.float64)
+
+
+def _paths(e, f, A, occset, occ_list, unocc, orb_sym, nb, i, j, a, b):
+    """The four sampling paths of the network component for {i,j} -> {a,b}:
+    (h1, h2, p1, p2, class of p2, probability)."""
+    Zh = sum(e[o - 1] for o in occ_list)
+    P = {(i, j): e[i - 1] / Zh * e[j - 1] / (Zh - e[i - 1]), (j, i): e[j - 1] / Zh * e[i - 1] / (Zh - e[j - 1])}
+    cls = {a: second_class(occset, orb_sym, nb, i, j, a), b: second_class(occset, orb_sym, nb, i, j, b)}
+    out = []
+    for h1, h2 in ((i, j), (j, i)):
+        z1 = sum(f[x - 1] * A[h1 - 1, x - 1] for x in unocc)
+        for p1, p2 in ((a, b), (b, a)):
+            c = cls[p1]
+            z2 = sum(f[x - 1] * A[h2 - 1, x - 1] for x in c)
+            t = P[(h1, h2)] * (f[p1 - 1] * A[h1 - 1, p1 - 1] / z1) * (f[p2 - 1] * A[h2 - 1, p2 - 1] / z2)
+            out.append((h1, h2, p1, p2, c, t))
+    return out
+
+
+def q_double(wrow, occ_list, orb_sym, nb, i, j, a, b, mix, aff=None):
     """Mixture probability of generating the unordered double {i,j}->{a,b}
-    (given that a double was attempted)."""
+    (given that a double was attempted); aff: the stored affinity table
+    (None: all ones)."""
     occset = set(occ_list)
     e = np.asarray(wrow[:nb], dtype=np.float64)
     f = np.asarray(wrow[nb:2 * nb], dtype=np.float64)
+    A = _aff_or_ones(aff, nb)
     nel = len(occ_list)
     unocc = [o for o in range(1, nb + 1) if o not in occset]
-    Zh = sum(e[o - 1] for o in occ_list)
-    Zp = sum(f[o - 1] for o in unocc)
-    hp = e[i - 1] / Zh * e[j - 1] / (Zh - e[i - 1]) + e[j - 1] / Zh * e[i - 1] / (Zh - e[j - 1])
+    q_mlp = sum(t for *_x, t in _paths(e, f, A, occset, occ_list, unocc, orb_sym, nb, i, j, a, b))
     cb = second_class(occset, orb_sym, nb, i, j, a)
     ca = second_class(occset, orb_sym, nb, i, j, b)
-    zb = sum(f[o - 1] for o in cb)
-    za = sum(f[o - 1] for o in ca)
-    pp = f[a - 1] / Zp * f[b - 1] / zb + f[b - 1] / Zp * f[a - 1] / za
-    q_mlp = hp * pp
     q_unif = 2.0 / (nel * (nel - 1)) / len(unocc) * (1.0 / len(cb) + 1.0 / len(ca))
     return (1.0 - mix) * q_mlp + mix * q_unif, q_mlp, q_unif
 
 
-def enumerate_doubles(wrow, occ_list, orb_sym, nb, mix):
+def enumerate_doubles(wrow, occ_list, orb_sym, nb, mix, aff=None):
     """Every double the mixture can produce with its exact probability, by
     summing over the sampling paths (hole pair, first particle, second
     particle) of both components; plus the probability of a null attempt."""
     occset = set(occ_list)
     e = np.asarray(wrow[:nb], dtype=np.float64)
     f = np.asarray(wrow[nb:2 * nb], dtype=np.float64)
+    A = _aff_or_ones(aff, nb)
     nel = len(occ_list)
     unocc = [o for o in range(1, nb + 1) if o not in occset]
     Zh = sum(e[o - 1] for o in occ_list)
-    Zp = sum(f[o - 1] for o in unocc)
     out = {}
     null = 0.0
 
     def add(key, p):
         out[key] = out.get(key, 0.0) + p
 
-    # MLP component: ordered i, j, then a, then b from a's partner class
+    # network component: ordered i, j; a paired with i; b (a's partner class) paired with j
     for i in occ_list:
         for j in occ_list:
             if i == j:
                 continue
             pij = e[i - 1] / Zh * e[j - 1] / (Zh - e[i - 1])
+            z1 = sum(f[x - 1] * A[i - 1, x - 1] for x in unocc)
             for a in unocc:
-                pa = f[a - 1] / Zp
+                pa = f[a - 1] * A[i - 1, a - 1] / z1
                 cb = second_class(occset, orb_sym, nb, i, j, a)
                 if not cb:
                     null += (1 - mix) * pij * pa
                     continue
-                zb = sum(f[o - 1] for o in cb)
+                z2 = sum(f[x - 1] * A[j - 1, x - 1] for x in cb)
                 for b in cb:
                     key = (min(i, j), max(i, j), min(a, b), max(a, b))
-                    add(key, (1 - mix) * pij * pa * f[b - 1] / zb)
+                    add(key, (1 - mix) * pij * pa * f[b - 1] * A[j - 1, b - 1] / z2)
     # uniform component: unordered pair, a uniform over U, b uniform over a's partner class
     npair = nel * (nel - 1) // 2
     for x in range(len(occ_list)):
@@ -218,23 +263,24 @@ def enumerate_doubles(wrow, occ_list, orb_sym, nb, mix):
 
 
 # ---- training loss ---------------------------------------------------------------
-def _nll_and_grad(z3, occ_list, orb_sym, nb, ex):
-    """-log q_mlp and its gradient with respect to the logits of one sample."""
+def _nll_and_grad(z3, th, occ_list, orb_sym, nb, ex):
+    """-log q_mlp of one sample and its gradient with respect to the logits z3
+    (2 nb) and to the affinity rows of the sample's holes i and j (nb each)."""
     i, j, a, b = ex
     occset = set(occ_list)
     u = np.clip(z3[:nb], -CLIP, CLIP)
     v = np.clip(z3[nb:2 * nb], -CLIP, CLIP)
-    inside = np.abs(z3) < CLIP
+    inside = np.abs(z3[:2 * nb]) < CLIP
     e = np.exp(u)
     f = np.exp(v)
+    A = np.exp(np.clip(th, -CLIP, CLIP))
     O = np.zeros(nb, bool)
     O[[o - 1 for o in occ_list]] = True
     U = ~O
+    unocc = [o for o in range(1, nb + 1) if o not in occset]
     Zh = e[O].sum()
-    Zp = f[U].sum()
-    pij = e[i - 1] / Zh * e[j - 1] / (Zh - e[i - 1])
-    pji = e[j - 1] / Zh * e[i - 1] / (Zh - e[j - 1])
-    hp = pij + pji
+    paths = _paths(e, f, A, occset, occ_list, unocc, orb_sym, nb, i, j, a, b)
+    q = sum(t for *_x, t in paths)
 
     def g_hole(first, second):
         g = -e * O / Zh
@@ -245,28 +291,26 @@ def _nll_and_grad(z3, occ_list, orb_sym, nb, ex):
         g[second - 1] += 1.0
         return g
 
-    gu = (pij * g_hole(i, j) + pji * g_hole(j, i)) / hp
-    cb = second_class(occset, orb_sym, nb, i, j, a)
-    ca = second_class(occset, orb_sym, nb, i, j, b)
-    mb = np.zeros(nb, bool)
-    mb[[o - 1 for o in cb]] = True
-    ma = np.zeros(nb, bool)
-    ma[[o - 1 for o in ca]] = True
-    zb = f[mb].sum()
-    za = f[ma].sum()
-    pab = f[a - 1] / Zp * f[b - 1] / zb
-    pba = f[b - 1] / Zp * f[a - 1] / za
-    pp = pab + pba
-
-    def g_part(first, second, m, z):
-        g = -f * U / Zp - f * m / z
-        g[first - 1] += 1.0
-        g[second - 1] += 1.0
-        return g
-
-    gv = (pab * g_part(a, b, mb, zb) + pba * g_part(b, a, ma, za)) / pp
+    gu = np.zeros(nb)
+    grow = {i: np.zeros(nb), j: np.zeros(nb)}  # d log q / d theta[h][.] for h = i, j
+    for h1, h2, p1, p2, c, t in paths:
+        r = t / q
+        gu += r * g_hole(h1, h2)
+        w1 = f * A[h1 - 1] * U
+        g1 = -w1 / w1.sum()
+        g1[p1 - 1] += 1.0
+        m2 = np.zeros(nb, bool)
+        m2[[o - 1 for o in c]] = True
+        w2 = f * A[h2 - 1] * m2
+        g2 = -w2 / w2.sum()
+        g2[p2 - 1] += 1.0
+        grow[h1] += r * g1
+        grow[h2] += r * g2
+    gv = grow[i] + grow[j]
     grad = -np.concatenate([gu, gv]) * inside
-    return -np.log(hp) - np.log(pp), grad
+    gth_i = -grow[i] * (np.abs(th[i - 1]) < CLIP)
+    gth_j = -grow[j] * (np.abs(th[j - 1]) < CLIP)
+    return -np.log(q), grad, gth_i, gth_j
 
 
 def loss_and_grad(params, occ, eps, orb_sym, samples, weights):
@@ -274,6 +318,7 @@ def loss_and_grad(params, occ, eps, orb_sym, samples, weights):
     with respect to the flat parameter vector (float64 forward and backward)."""
     nb = occ.shape[1]
     W1, b1, W2, b2, W3, b3 = unpack(params, nb)
+    th = theta(params, nb)
     x = features(occ, eps)
     z1 = x @ W1 + b1
     a1 = gelu(z1)
@@ -283,14 +328,18 @@ def loss_and_grad(params, occ, eps, orb_sym, samples, weights):
     w = np.asarray(weights, dtype=np.float64)
     wt = w / w.sum()
     dz3 = np.zeros_like(z3)
+    dth = np.zeros((nb, nb))
     loss = 0.0
     for s in range(len(samples)):
         if w[s] == 0.0:
             continue
         ol = [k + 1 for k in range(nb) if occ[s, k] > 0.5]
-        nll, g = _nll_and_grad(z3[s], ol, orb_sym, nb, samples[s])
+        nll, g, gi, gj = _nll_and_grad(z3[s], th, ol, orb_sym, nb, samples[s])
         loss += wt[s] * nll
         dz3[s] = wt[s] * g
+        i, j = samples[s][0], samples[s][1]
+        dth[i - 1] += wt[s] * gi
+        dth[j - 1] += wt[s] * gj
     dW3 = a2.T @ dz3
     db3 = dz3.sum(0)
     dz2 = (dz3 @ W3.T) * gelu_grad(z2)
@@ -299,12 +348,13 @@ def loss_and_grad(params, occ, eps, orb_sym, samples, weights):
     dz1 = (dz2 @ W2.T) * gelu_grad(z1)
     dW1 = x.T @ dz1
     db1 = dz1.sum(0)
-    return loss, pack(dW1, db1, dW2, db2, dW3, db3)
+    return loss, pack(dW1, db1, dW2, db2, dW3, db3, dth)
 
 
 def loss_only(params, occ, eps, orb_sym, samples, weights):
     nb = occ.shape[1]
     z3 = forward(params, occ, eps)
+    th = theta(params, nb)
     w = np.asarray(weights, dtype=np.float64)
     wt = w / w.sum()
     total = 0.0
@@ -312,5 +362,5 @@ def loss_only(params, occ, eps, orb_sym, samples, weights):
         if w[s] == 0.0:
             continue
         ol = [k + 1 for k in range(nb) if occ[s, k] > 0.5]
-        total += wt[s] * _nll_and_grad(z3[s], ol, orb_sym, nb, samples[s])[0]
+        total += wt[s] * _nll_and_grad(z3[s], th, ol, orb_sym, nb, samples[s])[0]
     return total

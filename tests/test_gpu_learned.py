@@ -88,6 +88,12 @@ def _forward(lib, eng, dets, nb):
     return out
 
 
+def _affinity(lib, eng, nb):
+    out = np.zeros((nb, nb), dtype=np.float32)
+    N.check(lib.fq_mlp_affinity(eng, out.ctypes.data_as(P(C.c_float)), nb * nb), "fq_mlp_affinity")
+    return out
+
+
 def _set_params(lib, eng, params):
     params = np.ascontiguousarray(params, dtype=np.float32)
     N.check(lib.fq_mlp_set_params(eng, params.ctypes.data_as(P(C.c_float)), len(params)), "fq_mlp_set_params")
@@ -218,9 +224,12 @@ def test_sampler_pgen_exact_and_frequencies(tmp_path, excit_gen):
                 claimed[key] = p
             if excit_gen == "learned":
                 wrow = _forward(lib, eng, dets[k:k + 1], nb)[0]
+                aff = _affinity(lib, eng, nb)
+                assert np.array_equal(aff, M.affinity(M.init_params(nb, 41) * 3.0, nb))
                 for key, p in claimed.items():
                     if key[0] == 2:
-                        q, _qm, _qu = M.q_double(wrow, occs[k], orb_sym, nb, key[1], key[2], key[3], key[4], 0.1)
+                        q, _qm, _qu = M.q_double(wrow, occs[k], orb_sym, nb, key[1], key[2], key[3], key[4], 0.1,
+                                                 aff)
                         assert abs(p - pdouble * q) <= 1e-12 * p, (key, p, pdouble * q)
             if excit_gen == "heat_bath":
                 import helpers
@@ -264,7 +273,7 @@ def _all_doubles(occ, nb, orb_sym):
                         yield i, j, a, b
 
 
-def _variance_cost(rows, occs, nb, orb_sym, h1, eri, mix):
+def _variance_cost(rows, aff, occs, nb, orb_sym, h1, eri, mix):
     """sum over determinants and doubles of |H|^2 / q: the second moment of the
     spawning estimator per attempt (lower is better; q ~ |H| is optimal)."""
     import exact_small as X
@@ -274,7 +283,7 @@ def _variance_cost(rows, occs, nb, orb_sym, h1, eri, mix):
             hval = abs(X._anti(eri, a, b, i, j))
             if hval == 0.0:
                 continue
-            q = M.q_double(wrow, occ, orb_sym, nb, i, j, a, b, mix)[0]
+            q = M.q_double(wrow, occ, orb_sym, nb, i, j, a, b, mix, aff)[0]
             total += hval * hval / q
     return total
 
@@ -306,15 +315,18 @@ def test_training_reduces_loss_and_spawning_variance(tmp_path):
         occs = [[o for o in range(1, nb + 1) if (f0w >> (o - 1)) & 1]] + occs
         _h, h1, eri = synthetic.synthetic_integrals(norb, seed)
         before = _forward(lib, eng, dets, nb)
+        aff_before = _affinity(lib, eng, nb)
+        assert np.all(aff_before == 1.0)
         steps = 300
         losses = np.zeros(steps)
         N.check(lib.fq_mlp_train(eng, 4096, steps, 2e-3, 77, losses.ctypes.data_as(P(C.c_double))), "fq_mlp_train")
         print(f"  loss first 20 {losses[:20].mean():.4f} last 20 {losses[-20:].mean():.4f}", flush=True)
         assert losses[-20:].mean() < losses[:20].mean() - 0.05
         after = _forward(lib, eng, dets, nb)
-        cost_unif = _variance_cost(before, occs, nb, orb_sym, h1, eri, 1.0)
-        cost_init = _variance_cost(before, occs, nb, orb_sym, h1, eri, 0.1)
-        cost_trained = _variance_cost(after, occs, nb, orb_sym, h1, eri, 0.1)
+        aff_after = _affinity(lib, eng, nb)
+        cost_unif = _variance_cost(before, aff_before, occs, nb, orb_sym, h1, eri, 1.0)
+        cost_init = _variance_cost(before, aff_before, occs, nb, orb_sym, h1, eri, 0.1)
+        cost_trained = _variance_cost(after, aff_after, occs, nb, orb_sym, h1, eri, 0.1)
         print(f"  sum |H|^2/q: uniform {cost_unif:.4f}, untrained mix {cost_init:.4f}, trained mix "
               f"{cost_trained:.4f} (gain over uniform {cost_unif / cost_trained:.2f}x)", flush=True)
         assert cost_trained < 0.9 * cost_unif

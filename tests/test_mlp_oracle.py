@@ -4,7 +4,9 @@
     of the loss itself;
   * the closed-form mixture probability q of every double equals the sum over
     the sampling paths of both mixture components, and the paths' total
-    probability (doubles + null attempts) is 1."""
+    probability (doubles + null attempts) is 1;
+  * with a flat hole-particle affinity the network component factorises into
+    a hole-pair and a particle-pair probability (the product proposal)."""
 import os
 import sys
 
@@ -69,17 +71,38 @@ def test_loss_gradient_matches_central_differences(nsym):
         assert abs(fd - grad[k]) < 1e-7 + 1e-5 * abs(fd), (k, fd, grad[k])
 
 
-@pytest.mark.parametrize("nsym,mix", [(1, 0.1), (2, 0.1), (2, 0.5)])
-def test_mixture_probability_matches_path_sum(nsym, mix):
+@pytest.mark.parametrize("nsym,mix,flat", [(1, 0.1, False), (2, 0.1, False), (2, 0.5, False), (2, 0.1, True)])
+def test_mixture_probability_matches_path_sum(nsym, mix, flat):
     nb, nel = 12, 4
     rng = np.random.default_rng(7 + nsym)
     orb_sym = [int(rng.integers(nsym)) for _ in range(nb // 2) for _ in range(2)]
     occ_list = sorted([1, 2] + [int(rng.choice(range(3, nb + 1, 2))), int(rng.choice(range(4, nb + 1, 2)))])
     wrow = np.exp(rng.normal(0, 1.5, 2 * nb)).astype(np.float32)
-    out, null = M.enumerate_doubles(wrow, occ_list, orb_sym, nb, mix)
+    aff = None if flat else np.exp(rng.normal(0, 1.5, (nb, nb))).astype(np.float32)
+    out, null = M.enumerate_doubles(wrow, occ_list, orb_sym, nb, mix, aff)
     assert abs(sum(out.values()) + null - 1.0) < 1e-12
     assert len(out) > 10
     for (i, j, a, b), p in out.items():
-        q, q_mlp, q_unif = M.q_double(wrow, occ_list, orb_sym, nb, i, j, a, b, mix)
+        q, q_mlp, q_unif = M.q_double(wrow, occ_list, orb_sym, nb, i, j, a, b, mix, aff)
         assert abs(q - p) <= 1e-12 * p, ((i, j, a, b), q, p)
         assert q_unif > 0.0
+
+
+def test_flat_affinity_is_the_product_proposal():
+    nb = 12
+    rng = np.random.default_rng(5)
+    orb_sym = [int(rng.integers(2)) for _ in range(nb // 2) for _ in range(2)]
+    occ_list = [1, 2, 5, 8]
+    occset = set(occ_list)
+    unocc = [o for o in range(1, nb + 1) if o not in occset]
+    wrow = np.exp(rng.normal(0, 1.5, 2 * nb))
+    e, f = wrow[:nb], wrow[nb:]
+    Zh = e[[o - 1 for o in occ_list]].sum()
+    Zp = f[[o - 1 for o in unocc]].sum()
+    out, _null = M.enumerate_doubles(wrow, occ_list, orb_sym, nb, 0.0)
+    for (i, j, a, b), p in out.items():
+        hp = e[i - 1] / Zh * e[j - 1] / (Zh - e[i - 1]) + e[j - 1] / Zh * e[i - 1] / (Zh - e[j - 1])
+        zb = sum(f[o - 1] for o in M.second_class(occset, orb_sym, nb, i, j, a))
+        za = sum(f[o - 1] for o in M.second_class(occset, orb_sym, nb, i, j, b))
+        pp = f[a - 1] / Zp * f[b - 1] / zb + f[b - 1] / Zp * f[a - 1] / za
+        assert abs(p - hp * pp) <= 1e-12 * p
