@@ -73,7 +73,7 @@ class FqhQmcIn(C.Structure):
         ("real_amplitudes", i32), ("real_amplitude_force_32", i32), ("spawn_cutoff", f64),
         ("excit_gen", i32), ("state_size", i64), ("spawned_state_size", i64),
         ("replica_tricks", i32), ("quadrature_initiator", i32), ("rng_mode", i32), ("device", i32),
-        ("vary_shift_from_proje", i32),
+        ("vary_shift_from_proje", i32), ("proposal_mix", f64), ("proposal_min_attempts", i32),
     ]
 
 

@@ -131,6 +131,9 @@ def qmc_in(qmc, fciqmc_opts=None, rng_mode="replay", device=0):
     st.quadrature_initiator = int(bool(f.get("quadrature_initiator", True)))
     st.rng_mode = _RNG[rng_mode]
     st.device = int(device)
+    # learned proposal options (no reference counterpart)
+    st.proposal_mix = float(q.get("proposal_mix", 0.0))
+    st.proposal_min_attempts = int(q.get("proposal_min_attempts", 0))
     return st
 
 

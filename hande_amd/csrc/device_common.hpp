@@ -70,6 +70,7 @@ struct Sys {
   // uniform-fallback fraction of doubles.
   const float* mlp_tab;
   const float* mlp_aff;
+  const float* mlp_flat;  // 2*nbasis ones: the weights of determinants without a row
   int64_t mlp_ld;
   const int64_t* mlp_on;   // per determinant: 1 if the network ran on it this cycle
   const int64_t* mlp_row;  // its row of mlp_tab (exclusive prefix of mlp_on)
@@ -1126,8 +1127,9 @@ __device__ inline int64_t spawn_heat_bath(const Sys& s, Rng& rng, const uint64_t
 // (f: the determinant's particle weights, A: the learned hole-particle
 // affinity).  pgen is the exact mixture probability of the unordered
 // excitation, summed over the four paths (h1, p1) in {i, j} x {a, b}, so the
-// sampler is unbiased whatever the weights; singles use the renormalised
-// generator.
+// sampler is unbiased whatever the weights.  Determinants the network did not
+// run on (fewer attempts than the threshold) use flat weights e = f = 1 with
+// the learned affinity.  Singles use the renormalised generator.
 // ---------------------------------------------------------------------------
 struct PartnerClass {
   int ims, sym;
@@ -1198,17 +1200,15 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
   if (rng.next() < s.psingle) {
     allowed = single_mol(s, rng, f, u, conn, h, pgen);
   } else {
-    // without a network row the mixture is the uniform component alone
-    const double mix = wrow ? s.mix : 1.0;
-    const float* ew = wrow;
-    const float* fw = wrow ? wrow + nb : nullptr;
+    const float* row = wrow ? wrow : s.mlp_flat;
+    const double mix = s.mix;
+    const float* ew = row;
+    const float* fw = row + nb;
     uint64_t un[FQ_MAX_WORDS];
     FQ_FOR_WORDS(w, s.W) un[w] = ~f[w] & s.valid[w];
     double zh = 0.0;
-    if (wrow) {
-      FQ_FOR_WORDS(w, s.W) {
-        for (uint64_t x = f[w]; x; x &= x - 1) zh = zh + static_cast<double>(ew[(w << 6) + __builtin_ctzll(x)]);
-      }
+    FQ_FOR_WORDS(w, s.W) {
+      for (uint64_t x = f[w]; x; x &= x - 1) zh = zh + static_cast<double>(ew[(w << 6) + __builtin_ctzll(x)]);
     }
     const int nvirt = nb - nel;
     int i = 0, j = 0, a = 0, b = 0;
@@ -1290,14 +1290,14 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
         b = partner_pick(s, f, fw, aj, cb, a, 0, rng.next() * zbj, true);
     }
     if (ok) {
-      const float* ai = wrow ? s.mlp_aff + static_cast<int64_t>(i - 1) * nb : nullptr;
-      const float* aj = wrow ? s.mlp_aff + static_cast<int64_t>(j - 1) * nb : nullptr;
+      const float* ai = s.mlp_aff + static_cast<int64_t>(i - 1) * nb;
+      const float* aj = s.mlp_aff + static_cast<int64_t>(j - 1) * nb;
       int nbc = 0, nac = 0;
       double zbi = 0.0, zbj = 0.0, zai = 0.0, zaj = 0.0;  // Z2 of holes i, j over a's (b's) partner class
       partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, a), a, nbc, zbi, zbj);
       partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, b), b, nac, zai, zaj);
       const double q_unif = 2.0 / static_cast<double>(nel * (nel - 1)) / nvirt * (1.0 / nbc + 1.0 / nac);
-      if (wrow) {
+      {
         double z1i = 0.0, z1j = 0.0;
         FQ_FOR_WORDS(w, s.W) {
           for (uint64_t y = un[w]; y; y &= y - 1) {
@@ -1313,8 +1313,6 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
         const double q_mlp = pij * (fa * aia / z1i) * (fb * ajb / zbj) + pij * (fb * aib / z1i) * (fa * aja / zaj) +
                              pji * (fa * aja / z1j) * (fb * aib / zbi) + pji * (fb * ajb / z1j) * (fa * aia / zai);
         pgen = s.pdouble * ((1.0 - mix) * q_mlp + mix * q_unif);
-      } else {
-        pgen = s.pdouble * q_unif;
       }
       conn.nexcit = 2;
       conn.from[0] = i < j ? i : j;

@@ -215,6 +215,7 @@ struct fq_engine {
   // learned proposal (excit_gen FQ_EXCIT_LEARNED): network, per-determinant weights
   fqm::Mlp mlp;
   float* mlp_tab = nullptr;  // rows x 2*nbasis, one row per determinant the network runs on
+  float* mlp_flat = nullptr;  // 2*nbasis ones: weights of determinants without a row
   int64_t mlp_tab_rows = 0;
   int64_t* mlp_on = nullptr;   // [cap] network runs on this determinant this cycle
   int64_t* mlp_row = nullptr;  // [cap + 1] exclusive prefix of mlp_on
@@ -2049,6 +2050,12 @@ int setup_learned(fq_engine* e) {
     return FQ_ERR_NOMEM;
   s.mlp_tab = e->mlp_tab;
   s.mlp_aff = e->mlp.aff;
+  {
+    const std::vector<float> ones(2 * static_cast<size_t>(s.nbasis), 1.0f);
+    if (dalloc(&e->mlp_flat, 2 * static_cast<int64_t>(s.nbasis)) != FQ_OK) return FQ_ERR_NOMEM;
+    FQ_HIP(hipMemcpy(e->mlp_flat, ones.data(), sizeof(float) * ones.size(), hipMemcpyHostToDevice));
+  }
+  s.mlp_flat = e->mlp_flat;
   s.mlp_ld = e->mlp.d_out;
   s.mlp_on = e->mlp_on;
   s.mlp_row = e->mlp_row;
@@ -2961,10 +2968,13 @@ int fq_probe_excitations(fq_engine* e, const uint64_t* det, int64_t n, uint64_t 
   rc |= dalloc(&d_row, 2 * static_cast<int64_t>(e->sys.nbasis));
   if (rc == FQ_OK && hipMemcpyAsync(d_det, det, sizeof(uint64_t) * W, hipMemcpyHostToDevice, st) != hipSuccess)
     rc = FQ_ERR_DEVICE;
-  if (rc == FQ_OK && e->par.excit_gen == FQ_EXCIT_LEARNED) rc = fqm::mlp_infer(e->mlp, e->sys, d_det, 1, 1, d_row, st);
+  // learned: the network's row, or (proposal_min_attempts > 1, as for a determinant with one
+  // attempt in a cycle) none, so that the flat weights are drawn from
+  const bool row = e->par.excit_gen == FQ_EXCIT_LEARNED && e->par.proposal_min_attempts <= 1;
+  if (rc == FQ_OK && row) rc = fqm::mlp_infer(e->mlp, e->sys, d_det, 1, 1, d_row, st);
   if (rc == FQ_OK) {
     hipLaunchKernelGGL(k_probe, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, det[0], W > 1 ? det[1] : 0,
-                       d_row, n, seed, d_exc, d_pg, d_h);
+                       row ? d_row : nullptr, n, seed, d_exc, d_pg, d_h);
     if (hipGetLastError() != hipSuccess) rc = FQ_ERR_DEVICE;
   }
   if (rc == FQ_OK) {
@@ -3063,6 +3073,7 @@ void fq_destroy(fq_engine* e) {
   dfree(e->d_hb_bias);
   fqm::mlp_free(e->mlp);
   dfree(e->mlp_tab);
+  dfree(e->mlp_flat);
   dfree(e->mlp_on);
   dfree(e->mlp_row);
   dfree(e->mlp_idx);

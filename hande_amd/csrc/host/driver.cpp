@@ -68,6 +68,8 @@ extern "C" int fqh_fciqmc(const fqh_system* hs, const fqh_qmc_in* in, fqh_report
   // spawn%cutoff = ceiling(cutoff*real_factor) (src/spawn_data.F90:215); 0 for integers (src/qmc.F90:1485)
   p.spawn_cutoff = in->real_amplitudes ? static_cast<int64_t>(std::ceil(in->spawn_cutoff * static_cast<double>(rf))) : 0;
   p.excit_gen = in->excit_gen;
+  p.proposal_mix = in->proposal_mix;
+  p.proposal_min_attempts = in->proposal_min_attempts;
   p.rng_mode = in->rng_mode;
   p.philox_seed = static_cast<uint64_t>(in->rng_seed);
   p.device = in->device;

@@ -106,8 +106,9 @@ typedef struct fq_params {
   double proposal_mix;          /* FQ_EXCIT_LEARNED: fraction of doubles drawn from the uniform
                                    fallback, in (0, 1]; 0 selects 0.1 */
   int32_t proposal_min_attempts; /* FQ_EXCIT_LEARNED: determinants with fewer spawning attempts in a
-                                    cycle skip the network and draw doubles from the fallback alone
-                                    (still exact); 0 runs the network on every determinant */
+                                    cycle skip the network: their doubles use flat hole and particle
+                                    weights with the learned affinity, in the same mixture (still
+                                    exact); 0 runs the network on every determinant */
 } fq_params;
 
 /* Per-cycle results, the quantities the reference accumulates per cycle. */
@@ -245,7 +246,9 @@ int fq_mlp_train(fq_engine *e, int64_t batch, int32_t steps, double lr, uint64_t
  * determinant (W words), Philox stream keyed by seed: exc (n x 6: allowed,
  * nexcit, from1, from2, to1, to2), pgen and the matrix element of each.  The
  * generator is the one the spawning kernel uses (gen_excit_mol,
- * gen_excit_mol_heat_bath or the learned proposal), without attempt_to_spawn. */
+ * gen_excit_mol_heat_bath or the learned proposal), without attempt_to_spawn.
+ * The learned proposal uses the network's row for the determinant, or, when
+ * proposal_min_attempts > 1 (a determinant with one attempt), the flat weights. */
 int fq_probe_excitations(fq_engine *e, const uint64_t *det, int64_t n, uint64_t seed, int32_t *exc, double *pgen,
                          double *hmatel);
 

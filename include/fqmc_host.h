@@ -71,6 +71,8 @@ typedef struct fqh_qmc_in {
   int32_t device;
   int32_t vary_shift_from_proje; /* vary_shift_from = 'proje': the shift starts from the report's projected
                                     energy (src/energy_evaluation.F90:638-648) */
+  double proposal_mix;           /* fq_params.proposal_mix (learned proposal; 0: default) */
+  int32_t proposal_min_attempts; /* fq_params.proposal_min_attempts */
 } fqh_qmc_in;
 
 /* One row of the QMC report table (src/qmc_io.f90:412-508). */

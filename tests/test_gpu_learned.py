@@ -39,7 +39,7 @@ def _system(tmp_path, norb, nel, seed):
     return api.read_in(int_file=path, nel=nel, ms=0)
 
 
-def _engine(system, excit_gen=N.FQ_EXCIT_LEARNED, seed=5, cap=1 << 16, mix=0.1):
+def _engine(system, excit_gen=N.FQ_EXCIT_LEARNED, seed=5, cap=1 << 16, mix=0.1, min_attempts=0):
     lib = N.lib()
     p = N.FqParams()
     p.nspaces = 1
@@ -54,6 +54,7 @@ def _engine(system, excit_gen=N.FQ_EXCIT_LEARNED, seed=5, cap=1 << 16, mix=0.1):
     p.spawn_capacity = cap
     p.device = 0
     p.proposal_mix = mix
+    p.proposal_min_attempts = min_attempts
     eng = C.c_void_p()
     N.check(lib.fq_create(system.view, C.byref(p), C.byref(eng)), "fq_create")
     return lib, eng
@@ -192,14 +193,19 @@ def _chi_square(counts, probs, n):
     return (stat - dof) / np.sqrt(2 * dof)
 
 
-@pytest.mark.parametrize("excit_gen", ["learned", "heat_bath", "renorm"])
+@pytest.mark.parametrize("excit_gen", ["learned", "learned_flat", "heat_bath", "renorm"])
 def test_sampler_pgen_exact_and_frequencies(tmp_path, excit_gen):
+    """learned_flat: a determinant without a network row (flat weights, the
+    learned affinity)."""
     print(f"[probe] {excit_gen}", flush=True)
     norb, nel = 6, 4
     system = _system(tmp_path, norb, nel, 611)
     nb, _eps, orb_sym = _orb_info(system)
+    flat = excit_gen == "learned_flat"
+    if flat:
+        excit_gen = "learned"
     gen = {"learned": N.FQ_EXCIT_LEARNED, "heat_bath": N.FQ_EXCIT_HEAT_BATH, "renorm": N.FQ_EXCIT_RENORM}[excit_gen]
-    lib, eng = _engine(system, gen)
+    lib, eng = _engine(system, gen, min_attempts=2 if flat else 0)
     try:
         if excit_gen == "learned":
             _set_params(lib, eng, M.init_params(nb, 41) * 3.0)  # strongly non-uniform weights
@@ -223,7 +229,7 @@ def test_sampler_pgen_exact_and_frequencies(tmp_path, excit_gen):
                     assert abs(claimed[key] - p) <= 1e-14 * p, key
                 claimed[key] = p
             if excit_gen == "learned":
-                wrow = _forward(lib, eng, dets[k:k + 1], nb)[0]
+                wrow = np.ones(2 * nb, dtype=np.float32) if flat else _forward(lib, eng, dets[k:k + 1], nb)[0]
                 aff = _affinity(lib, eng, nb)
                 assert np.array_equal(aff, M.affinity(M.init_params(nb, 41) * 3.0, nb))
                 for key, p in claimed.items():
@@ -335,14 +341,18 @@ def test_training_reduces_loss_and_spawning_variance(tmp_path):
         lib.fq_destroy(eng)
 
 
-def test_learned_projected_energy_matches_exact(tmp_path):
+@pytest.mark.parametrize("min_attempts", [0, 4])
+def test_learned_projected_energy_matches_exact(tmp_path, min_attempts):
+    """min_attempts 4: most determinants skip the network and draw doubles with
+    flat weights and the affinity; the estimate stays unbiased."""
     norb, nel, seed = 8, 4, 600
     exact, _hf, weight = exact_small.ground_state(norb, nel, seed, with_weight=True)
     assert weight > 0.5
     system = _system(tmp_path, norb, nel, seed)
-    print("[production] synthetic 8o4e learned", flush=True)
+    print(f"[production] synthetic 8o4e learned, min_attempts {min_attempts}", flush=True)
     q = dict(tau=0.01, init_pop=10, mc_cycles=20, nreports=400, target_population=2000,
-             shift_damping=0.05, state_size=100000, spawned_state_size=100000, excit_gen="learned")
+             shift_damping=0.05, state_size=100000, spawned_state_size=100000, excit_gen="learned",
+             proposal_min_attempts=min_attempts)
     e, se, _sh, _sse = replicate_estimates(system, q, range(99, 107))
     print(f"  E_proj={e:.6f} +- {se:.6f} (8 runs), exact {exact:.6f}", flush=True)
     assert abs(e - exact) < 4 * se + 1e-4
