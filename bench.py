@@ -354,9 +354,11 @@ def prepare(lib, system, target, seed, cap, equil_tau=1e-3, hold=300, max_cycles
     return st, po, da, prop.shift[0], prop.eold[0]
 
 
-def production(lib, eng, target, state, tau, cycles, warmup, report=10):
+def production(lib, eng, target, state, tau, cycles, warmup, report=10, prof=None):
     """Warm-up then timed production cycles from a prepared state; returns the
-    per-report (sum H_0j N_j, N_0) series, wall time and attempts."""
+    per-report (sum H_0j N_j, N_0) series, wall time and attempts.  prof (a
+    dict) receives per-cycle means of the learned proposal's time and rows and
+    of the cycle time."""
     st, po, da, shift, eold = state
     N.check(lib.fq_load(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)), po.ctypes.data_as(C.POINTER(C.c_int64)),
                         da.ctypes.data_as(C.POINTER(C.c_double)), len(da)), "fq_load")
@@ -367,9 +369,14 @@ def production(lib, eng, target, state, tau, cycles, warmup, report=10):
     proj, d0, attempts = [], [], 0
     accp = accd = 0.0
     t0 = time.perf_counter()
+    pr = N.FqProfileOut()
+    acc_prof = np.zeros(3)
     for i in range(cycles):
         out = prop.cycle(tau)
         attempts += out.nattempts
+        if prof is not None:
+            N.check(lib.fq_last_profile(eng, C.byref(pr)), "fq_last_profile")
+            acc_prof += (pr.proposal_ms, pr.proposal_rows, pr.total_ms)
         accp += out.proj_energy[0]
         accd += out.D0_population[0]
         if (i + 1) % report == 0:
@@ -379,6 +386,9 @@ def production(lib, eng, target, state, tau, cycles, warmup, report=10):
         if (i + 1) % 500 == 0:
             log(f"[eff] production cycle {i + 1}: N={out.nparticles[0]:.4g}, {time.perf_counter() - t0:.1f}s")
     wall = time.perf_counter() - t0
+    if prof is not None:
+        m = acc_prof / max(cycles, 1)
+        prof.update(proposal_ms=float(m[0]), proposal_rows=float(m[1]), cycle_ms=float(m[2]))
     return np.array(proj), np.array(d0), wall, attempts, prop.last
 
 
@@ -395,6 +405,16 @@ def batch_means_ratio(num, den, nbatch):
     return a.sum() / b.sum(), float(np.std(a / b, ddof=1) / math.sqrt(nbatch))
 
 
+def attempt_coverage(pops, rf, thresholds):
+    """Share of the walkers (so, roughly, of the spawning attempts) on
+    determinants holding at least K walkers, and the number of such
+    determinants, for each threshold K."""
+    a = np.abs(np.asarray(pops, dtype=np.float64).reshape(-1)) / rf
+    tot = a.sum()
+    return {int(k): {"walker_share": float(a[a >= k].sum() / tot), "determinants": int((a >= k).sum())}
+            for k in thresholds}
+
+
 def efficiency_learned(args):
     from hande_amd import reblock
     lib = N.lib()
@@ -403,28 +423,42 @@ def efficiency_learned(args):
     system = api.read_in(int_file=synthetic.write_fcidump(os.path.join(tmp, "FCIDUMP"), norb, nel, seed), nel=nel, ms=0)
     cap = int(max(6 * args.target, 1 << 20))
     state = prepare(lib, system, args.target, 11, cap)
+    thresholds = [int(x) for x in str(args.min_attempts).split(",")]
+    coverage = attempt_coverage(state[1], float(1 << 31), sorted(set([1, 2, 4, 8, 16, 32, 64] + thresholds)))
+    log(f"[eff] walker share on determinants with >= K walkers: "
+        + ", ".join(f"K={k}: {v['walker_share']:.3f} ({v['determinants']})" for k, v in coverage.items()))
     res = {}
-    codes = {"renorm": N.FQ_EXCIT_RENORM, "heat_bath": N.FQ_EXCIT_HEAT_BATH, "learned": N.FQ_EXCIT_LEARNED}
-    for gen in ("renorm", "heat_bath", "learned"):
-        code = codes[gen]
-        eng, _p = make_engine(lib, system, args.target, 23, cap, cap, excit_gen=code, mix=0.1,
-                              min_attempts=args.min_attempts)
+    runs = [("renorm", N.FQ_EXCIT_RENORM, 0), ("heat_bath", N.FQ_EXCIT_HEAT_BATH, 0)]
+    runs += [(f"learned_k{k}", N.FQ_EXCIT_LEARNED, k) for k in thresholds]
+    trained = None  # (parameters, training seconds): one training serves every threshold
+    for gen, code, k in runs:
+        eng, _p = make_engine(lib, system, args.target, 23, cap, cap, excit_gen=code, mix=0.1, min_attempts=k)
         t_train = 0.0
-        losses = None
-        if gen == "learned":
-            st, po, da = state[:3]
-            N.check(lib.fq_load(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)),
-                                po.ctypes.data_as(C.POINTER(C.c_int64)), da.ctypes.data_as(C.POINTER(C.c_double)),
-                                len(da)), "fq_load")
-            losses = np.zeros(args.train_steps)
-            t1 = time.perf_counter()
-            N.check(lib.fq_mlp_train(eng, 16384, args.train_steps, 1e-3, 5,
-                                     losses.ctypes.data_as(C.POINTER(C.c_double))), "fq_mlp_train")
-            t_train = time.perf_counter() - t1
-            log(f"[eff] trained {args.train_steps} steps in {t_train:.1f}s: loss {losses[:10].mean():.4f} -> "
-                f"{losses[-10:].mean():.4f}")
+        if code == N.FQ_EXCIT_LEARNED:
+            if trained is None:
+                st, po, da = state[:3]
+                N.check(lib.fq_load(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                    po.ctypes.data_as(C.POINTER(C.c_int64)), da.ctypes.data_as(C.POINTER(C.c_double)),
+                                    len(da)), "fq_load")
+                losses = np.zeros(args.train_steps)
+                t1 = time.perf_counter()
+                N.check(lib.fq_mlp_train(eng, 16384, args.train_steps, 1e-3, 5,
+                                         losses.ctypes.data_as(C.POINTER(C.c_double))), "fq_mlp_train")
+                t_train = time.perf_counter() - t1
+                npar = C.c_int64()
+                N.check(lib.fq_mlp_nparams(eng, C.byref(npar)), "fq_mlp_nparams")
+                params = np.zeros(npar.value, dtype=np.float32)
+                N.check(lib.fq_mlp_get_params(eng, params.ctypes.data_as(C.POINTER(C.c_float)), npar.value), "get")
+                trained = (params, t_train, float(losses[:10].mean()), float(losses[-10:].mean()))
+                log(f"[eff] trained {args.train_steps} steps in {t_train:.1f}s: loss {trained[2]:.4f} -> "
+                    f"{trained[3]:.4f}")
+            else:
+                params, t_train = trained[0], trained[1]
+                N.check(lib.fq_mlp_set_params(eng, params.ctypes.data_as(C.POINTER(C.c_float)), len(params)), "set")
         # each generator settles its own (initiator) stationary state before the timed cycles
-        proj, d0, wall, att, last = production(lib, eng, args.target, state, 1e-3, args.eff_cycles, args.eff_warmup)
+        prof = {}
+        proj, d0, wall, att, last = production(lib, eng, args.target, state, 1e-3, args.eff_cycles, args.eff_warmup,
+                                               prof=prof)
         lib.fq_destroy(eng)
         rb = reblock.projected_energy(proj, d0)
         e_bm, sigma = batch_means_ratio(proj, d0, EFF_BATCHES)
@@ -434,24 +468,37 @@ def efficiency_learned(args):
                     "sigma_method": f"batch means: {EFF_BATCHES} contiguous batches of the production cycles",
                     "reblock": {"E_proj": system.H00 + rb["energy"], "sigma": rb["error"], "level": rb["level"],
                                 "converged": rb["converged"]},
-                    "walkers": last.nparticles[0], "determinants": int(last.nstates)}
-        log(f"[eff] {gen}: E_proj {res[gen]['E_proj']:.6f} +- {sigma:.6f}, {wall:.1f}s, "
-            f"efficiency {res[gen]['efficiency']:.4g} (reblocking converged: {rb['converged']})")
-    a, b, h = res["learned"], res["renorm"], res["heat_bath"]
-    agree = abs(a["E_proj"] - b["E_proj"]) < 2 * math.hypot(a["sigma"], b["sigma"])
+                    "walkers": last.nparticles[0], "determinants": int(last.nstates), "cycle_ms": prof["cycle_ms"]}
+        if code == N.FQ_EXCIT_LEARNED:
+            res[gen].update(min_attempts=k, proposal_ms=prof["proposal_ms"], network_rows=prof["proposal_rows"])
+        log(f"[eff] {gen}: E_proj {res[gen]['E_proj']:.6f} +- {sigma:.6f}, {wall:.1f}s "
+            f"({prof['cycle_ms']:.2f} ms/cycle, proposal {prof['proposal_ms']:.2f} ms on {prof['proposal_rows']:.0f} "
+            f"rows), efficiency {res[gen]['efficiency']:.4g} (reblocking converged: {rb['converged']})")
+    b, h = res["renorm"], res["heat_bath"]
+    head = f"learned_k{thresholds[0]}"
+    a = res[head]
+
+    def agree(x, y):
+        return bool(abs(x["E_proj"] - y["E_proj"]) < 2 * math.hypot(x["sigma"], y["sigma"]))
+
     print(json.dumps({"metric": "E_proj efficiency 1/(sigma^2 t), learned vs renorm (the plain path)",
                       "unit": "ratio", "value": a["efficiency"] / b["efficiency"], "higher_is_better": True,
-                      "energies_agree_2sigma": bool(agree),
+                      "energies_agree_2sigma": agree(a, b),
                       "learned_vs_heat_bath": a["efficiency"] / h["efficiency"],
-                      "energies_agree_2sigma_heat_bath": bool(abs(a["E_proj"] - h["E_proj"])
-                                                              < 2 * math.hypot(a["sigma"], h["sigma"])),
+                      "energies_agree_2sigma_heat_bath": agree(a, h),
+                      "heat_bath_vs_renorm": h["efficiency"] / b["efficiency"],
+                      "by_threshold": {g: {"vs_renorm": r["efficiency"] / b["efficiency"],
+                                           "vs_heat_bath": r["efficiency"] / h["efficiency"]}
+                                       for g, r in res.items() if g.startswith("learned")},
+                      "walker_coverage": coverage,
                       "n_gpus": 1, "data": "synthetic",
                       "config": {"workload": f"i-FCIQMC mol40: synthetic molecular integrals {norb} spatial orbitals, "
                                              f"{nel} electrons, C1, real_64, initiator 3, tau 1e-3, target "
                                              f"{args.target:g}, {args.eff_cycles} production cycles",
-                                 "proposal_mix": 0.1, "proposal_min_attempts": args.min_attempts,
+                                 "proposal_mix": 0.1, "proposal_min_attempts": thresholds[0],
                                  "train_steps": args.train_steps, "warmup_cycles": args.eff_warmup},
-                      "learned": a, "renorm": b, "heat_bath": h}), flush=True)
+                      "training": {"seconds": trained[1], "loss_first10": trained[2], "loss_last10": trained[3]},
+                      "runs": res}), flush=True)
 
 
 def c3_instances(n=64, seed=2024):
@@ -608,7 +655,9 @@ def main():
                          "reblocking; 1000 batched)")
     ap.add_argument("--train-steps", type=int, default=1000)
     ap.add_argument("--eff-warmup", type=int, default=500, help="untimed cycles per generator before measuring")
-    ap.add_argument("--min-attempts", type=int, default=1)
+    ap.add_argument("--min-attempts", default="16,4,64",
+                    help="learned proposal: comma-separated attempt thresholds for the network (the first is "
+                         "the headline value); determinants below use flat weights with the learned affinity")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--prep-cycles", type=int, default=3000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

@@ -62,6 +62,7 @@ class FqProfileOut(C.Structure):
         ("det_pass_ms", f64), ("spawn_ms", f64), ("sort_ms", f64), ("reduce_ms", f64), ("merge_ms", f64),
         ("total_ms", f64), ("nstates_in", i64), ("nattempts", i64), ("nspawned", i64), ("nruns", i64),
         ("nstates_out", i64), ("radix_passes", i32), ("spawn_launches", i32), ("spawn_kernel_ms", f64),
+        ("proposal_ms", f64), ("proposal_rows", i64),
     ]
 
 

@@ -2444,6 +2444,7 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
       int64_t ninf = 0;
       FQ_HIP(hipMemcpyAsync(&ninf, e->mlp_row + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
       FQ_HIP(hipStreamSynchronize(st));
+      e->prof.proposal_rows = ninf;
       if (ninf > e->mlp_tab_rows) {  // grow the weight table (rare: rows follow the inferred determinants)
         const int64_t rows = std::min<int64_t>(e->cap, std::max<int64_t>(ninf, 2 * e->mlp_tab_rows));
         dfree(e->mlp_tab);
@@ -2524,6 +2525,10 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
     pr.reduce_ms = t[3];
     pr.merge_ms = t[4];
     pr.total_ms = t[5];
+    float tp = 0.0f;
+    if (!replay) (void)hipEventElapsedTime(&tp, e->pev[1], e->pev[2]);
+    pr.proposal_ms = e->par.excit_gen == FQ_EXCIT_LEARNED ? tp : 0.0;
+    if (e->par.excit_gen != FQ_EXCIT_LEARNED || n == 0) pr.proposal_rows = 0;
     pr.nstates_in = n;
     pr.nattempts = e->h_stats.total_attempts;
     pr.nspawned = ns;

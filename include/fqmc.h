@@ -190,6 +190,8 @@ typedef struct fq_profile_out {
   int32_t radix_passes;  /* 8-bit digit passes of the sort */
   int32_t spawn_launches;    /* launches of the spawning-attempt kernel this cycle */
   double spawn_kernel_ms;    /* summed device time of those launches (event pair around each) */
+  double proposal_ms;        /* learned proposal: network marking, inference and attempt offsets */
+  int64_t proposal_rows;     /* learned proposal: determinants the network ran on */
 } fq_profile_out;
 int fq_last_profile(fq_engine *e, fq_profile_out *out);
 
