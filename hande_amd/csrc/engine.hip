@@ -99,28 +99,55 @@ struct Stats {
 };
 
 // ---------------------------------------------------------------------------
-// Prefix sums: tile sums, one-block scan of the tile sums, tile scans (three
-// launches at most; one when the input fits a tile).  out[n] = total.
+// Prefix sums.  Up to kSmallScanMax entries: one 1024-thread block walks the
+// array in chunks with a running carry (one launch).  Up to kPrefixTiles
+// tiles: per-tile sums, then each tile adds the sums of the tiles before it
+// itself and scans its entries (two launches).  More: the tile sums are
+// scanned by the one-block scan in between (three launches).  out[n] = total.
 // ---------------------------------------------------------------------------
 constexpr int kScanItems = 16;
 constexpr int64_t kScanTile = static_cast<int64_t>(kBlock) * kScanItems;
+constexpr int kSmallBlock = 1024, kSmallItems = 8;
+constexpr int64_t kSmallScanMax = 65536;
+constexpr int64_t kPrefixTiles = 2048;
+
+// Inclusive scan of x over the 64 lanes of a wavefront.
+__device__ __forceinline__ int64_t wave_inclusive_scan(int64_t x, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+__device__ __forceinline__ int64_t wave_sum(int64_t x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+// Sum of x over a kBlock-thread block (every thread receives it).
+__device__ __forceinline__ int64_t block_sum(int64_t x, int64_t* wsh) {
+  x = wave_sum(x);
+  if ((threadIdx.x & 63) == 0) wsh[threadIdx.x >> 6] = x;
+  __syncthreads();
+  int64_t t = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) t += wsh[w];
+  __syncthreads();
+  return t;
+}
 
 __global__ __launch_bounds__(kBlock) void k_tile_sums(const int64_t* __restrict__ in, int64_t n,
                                                       int64_t* __restrict__ sums) {
-  __shared__ int64_t sh[kBlock];
+  __shared__ int64_t wsh[kBlock / 64];
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
   int64_t acc = 0;
   for (int k = 0; k < kScanItems; ++k) {
     const int64_t i = base + k * kBlock + threadIdx.x;
     if (i < n) acc += in[i];
   }
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int h = kBlock / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) sums[blockIdx.x] = sh[0];
+  acc = block_sum(acc, wsh);
+  if (threadIdx.x == 0) sums[blockIdx.x] = acc;
 }
 
 // Inclusive Hillis-Steele scan of sh[0..kBlock) in place.
@@ -133,28 +160,23 @@ __device__ __forceinline__ void block_inclusive_scan(int64_t* sh) {
   }
 }
 
-// Exclusive scan of nb tile sums on one block, in chunks with a running carry.
-__global__ __launch_bounds__(kBlock) void k_scan_sums(const int64_t* __restrict__ sums, int64_t nb,
-                                                      int64_t* __restrict__ off) {
-  __shared__ int64_t sh[kBlock];
-  int64_t carry = 0;
-  for (int64_t c0 = 0; c0 < nb; c0 += kBlock) {
-    const int64_t i = c0 + threadIdx.x;
-    const int64_t v = i < nb ? sums[i] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    block_inclusive_scan(sh);
-    if (i < nb) off[i] = carry + sh[threadIdx.x] - v;
-    carry += sh[kBlock - 1];
-    __syncthreads();
-  }
-}
-
-// Exclusive scan of one tile plus its offset; the tile holding element n-1
-// also writes out[n] = total.
+// Exclusive scan of one tile (kScanItems consecutive entries per thread) plus
+// the tile's offset: with prefix, the sum of part[0..tile) (part: tile sums);
+// otherwise part[tile] (part: exclusive tile offsets).  The tile holding
+// element n-1 also writes out[n] = total.
 __global__ __launch_bounds__(kBlock) void k_tile_scan(const int64_t* __restrict__ in, int64_t n,
-                                                      const int64_t* __restrict__ off, int64_t* __restrict__ out) {
-  __shared__ int64_t sh[kBlock];
+                                                      const int64_t* __restrict__ part, int prefix,
+                                                      int64_t* __restrict__ out) {
+  __shared__ int64_t wsh[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int64_t off;
+  if (prefix) {
+    int64_t pre = 0;
+    for (int64_t k = threadIdx.x; k < blockIdx.x; k += kBlock) pre += part[k];
+    off = block_sum(pre, wsh);
+  } else {
+    off = part[blockIdx.x];
+  }
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
   int64_t v[kScanItems];
   int64_t local = 0;
@@ -163,16 +185,59 @@ __global__ __launch_bounds__(kBlock) void k_tile_scan(const int64_t* __restrict_
     v[k] = i < n ? in[i] : 0;
     local += v[k];
   }
-  sh[threadIdx.x] = local;
+  const int64_t x = wave_inclusive_scan(local, lane);
+  if (lane == 63) wsh[wave] = x;
   __syncthreads();
-  block_inclusive_scan(sh);
-  int64_t run = sh[threadIdx.x] - local + (off ? off[blockIdx.x] : 0);
+  int64_t wpre = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) wpre += w < wave ? wsh[w] : 0;
+  int64_t run = off + wpre + x - local;
   for (int k = 0; k < kScanItems; ++k) {
     const int64_t i = base + k;
     if (i < n) out[i] = run;
     run += v[k];
     if (i == n - 1) out[n] = run;
   }
+}
+
+// Exclusive scan of n <= kSmallScanMax entries on one block of kSmallBlock
+// threads: chunks of kSmallBlock * kSmallItems with a running carry.
+__global__ __launch_bounds__(kSmallBlock) void k_scan_small(const int64_t* __restrict__ in, int64_t n,
+                                                            int64_t* __restrict__ out) {
+  __shared__ int64_t wsh[kSmallBlock / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < n; c0 += static_cast<int64_t>(kSmallBlock) * kSmallItems) {
+    const int64_t base = c0 + static_cast<int64_t>(tid) * kSmallItems;
+    int64_t v[kSmallItems];
+    int64_t local = 0;
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      const int64_t i = base + k;
+      v[k] = i < n ? in[i] : 0;
+      local += v[k];
+    }
+    const int64_t x = wave_inclusive_scan(local, lane);
+    if (lane == 63) wsh[wave] = x;
+    __syncthreads();
+    int64_t wpre = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kSmallBlock / 64; ++w) {
+      const int64_t t = wsh[w];
+      wpre += w < wave ? t : 0;
+      total += t;
+    }
+    int64_t run = carry + wpre + x - local;
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      const int64_t i = base + k;
+      if (i < n) out[i] = run;
+      run += v[k];
+    }
+    carry += total;
+    __syncthreads();  // wsh is rewritten by the next chunk
+  }
+  if (tid == 0) out[n] = carry;
 }
 
 __global__ void k_set_zero_total(int64_t* __restrict__ out) { out[0] = 0; }
@@ -340,17 +405,23 @@ int scan_exclusive(fq_engine* e, const int64_t* in, int64_t* out, int64_t n) {
     hipLaunchKernelGGL(k_set_zero_total, dim3(1), dim3(1), 0, e->stream, out);
     return FQ_OK;
   }
-  const int64_t nb = ceil_div(n, kScanTile);
-  if (nb == 1) {
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kBlock), 0, e->stream, in, n, nullptr, out);
+  if (n <= kSmallScanMax) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kSmallBlock), 0, e->stream, in, n, out);
     return FQ_OK;
   }
+  const int64_t nb = ceil_div(n, kScanTile);
   int64_t* sums = scan_scratch(e, 0, nb);
-  int64_t* offs = scan_scratch(e, 1, nb);
-  if (!sums || !offs) return FQ_ERR_NOMEM;
+  if (!sums) return FQ_ERR_NOMEM;
   hipLaunchKernelGGL(k_tile_sums, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, e->stream, in, n, sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, e->stream, sums, nb, offs);
-  hipLaunchKernelGGL(k_tile_scan, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, e->stream, in, n, offs, out);
+  if (nb <= kPrefixTiles) {
+    hipLaunchKernelGGL(k_tile_scan, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, e->stream, in, n, sums, 1,
+                       out);
+    return FQ_OK;
+  }
+  int64_t* offs = scan_scratch(e, 1, nb + 1);
+  if (!offs) return FQ_ERR_NOMEM;
+  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kSmallBlock), 0, e->stream, sums, nb, offs);
+  hipLaunchKernelGGL(k_tile_scan, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, e->stream, in, n, offs, 0, out);
   return FQ_OK;
 }
 
@@ -1595,9 +1666,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter_new(Sys s, int64_t nruns, in
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_abs_pop(int64_t n, int64_t cap, int S, const int64_t* __restrict__ pop,
-                                                    int64_t* __restrict__ part_i) {
+// Sums of |pop| over the new list (its length read from the stats the
+// insertion wrote); grid-stride.
+__global__ __launch_bounds__(kBlock) void k_abs_pop(const Stats* __restrict__ stn, int64_t cap, int S,
+                                                    const int64_t* __restrict__ pop, int64_t* __restrict__ part_i) {
   __shared__ unsigned long long sh[FQ_MAX_SPACES][kBlock];
+  const int64_t n = stn->new_n;
   // exact integer sums: any grid and order give the same totals
   for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
     unsigned long long v = 0;
@@ -1816,14 +1890,15 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
                        e->det[o], e->pop[o], e->kii[o], e->batched ? e->d_sys_arr : nullptr, use_lds ? 1 : 0);
   }
   FQ_HIP(hipEventRecord(e->pev[8], st));
+  // the new list holds at most n + nruns entries
+  const unsigned nba = std::min<unsigned>(grid_for(std::min(n + nruns, cap)), 2048);
+  hipLaunchKernelGGL(k_abs_pop, dim3(nba), dim3(kBlock), 0, st, e->d_stats, cap, e->S, e->pop[o], e->part_i);
+  hipLaunchKernelGGL(k_sum_abs, dim3(1), dim3(kBlock), 0, st, e->part_i, static_cast<int>(nba), e->d_stats);
   rc = sync_stats(e);
   if (rc != FQ_OK) return rc;
   e->cur = o;
   e->n = e->h_stats.new_n;
-  const unsigned nba = std::min<unsigned>(grid_for(e->n), 2048);
-  hipLaunchKernelGGL(k_abs_pop, dim3(nba), dim3(kBlock), 0, st, e->n, cap, e->S, e->pop[o], e->part_i);
-  hipLaunchKernelGGL(k_sum_abs, dim3(1), dim3(kBlock), 0, st, e->part_i, static_cast<int>(nba), e->d_stats);
-  return sync_stats(e);
+  return FQ_OK;
 }
 
 void fill_out(fq_engine* e, fq_cycle_out* out, int64_t rng_used) {
