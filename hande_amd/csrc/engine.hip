@@ -281,6 +281,8 @@ struct fq_engine {
   fqm::Mlp mlp;
   float* mlp_tab = nullptr;  // rows x 2*nbasis, one row per determinant the network runs on
   float* mlp_flat = nullptr;  // 2*nbasis ones: weights of determinants without a row
+  const int64_t* p_nruns = nullptr;  // device address of this cycle's run count
+  int64_t run_bound = 0;             // host bound of it (the count itself in replay mode)
   int64_t mlp_tab_rows = 0;
   int64_t* mlp_on = nullptr;   // [cap] network runs on this determinant this cycle
   int64_t* mlp_row = nullptr;  // [cap + 1] exclusive prefix of mlp_on
@@ -1289,12 +1291,13 @@ __global__ __launch_bounds__(kBlock) void k_pack_runs(int64_t ntiles, int64_t ns
 // (src/spawn_data.F90:859-1101).  The run's first entry is its first in
 // sorted order: the reference's quicksort order in replay mode, the earliest
 // spawned in production (stable sort of an attempt-ordered list).
-__global__ void k_reduce_runs(Sys s, int64_t nruns, const int64_t* __restrict__ run_start,
+__global__ void k_reduce_runs(Sys s, const int64_t* __restrict__ pnr, const int64_t* __restrict__ run_start,
                               const uint64_t* __restrict__ sdet, const uint32_t* __restrict__ perm,
                               const int64_t* __restrict__ gpop, const int64_t* __restrict__ gflag, int64_t scap,
                               uint64_t* __restrict__ rdet, int64_t* __restrict__ rpop, int64_t* __restrict__ rflag,
                               int64_t* __restrict__ rstate) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nruns = *pnr;
   if (r >= nruns) return;
   const int64_t a = run_start[r], b = run_start[r + 1];
   int64_t tot[FQ_MAX_SPACES] = {0, 0}, init[FQ_MAX_SPACES] = {0, 0};
@@ -1371,8 +1374,10 @@ __device__ __forceinline__ int64_t main_lower_bound(const uint64_t* __restrict__
 
 // Lower bounds in the main list of the first and last key of each merge
 // block's runs (one thread per block of kBlock runs).
-__global__ void k_merge_bounds(int64_t nruns, int64_t n, int64_t cap, int W, const uint64_t* __restrict__ det,
-                               const uint64_t* __restrict__ rdet, int64_t scap, int64_t* __restrict__ mwin) {
+__global__ void k_merge_bounds(const int64_t* __restrict__ pnr, int64_t n, int64_t cap, int W,
+                               const uint64_t* __restrict__ det, const uint64_t* __restrict__ rdet, int64_t scap,
+                               int64_t* __restrict__ mwin) {
+  const int64_t nruns = *pnr;
   const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t r0 = b * kBlock;
   if (r0 >= nruns) return;
@@ -1388,14 +1393,16 @@ __global__ void k_merge_bounds(int64_t nruns, int64_t n, int64_t cap, int W, con
 // Runs are sorted, so a block's runs land between the lower bounds of its
 // first and last keys: two searches of the whole list per block, then each
 // thread searches only that window (merge-path partition of the two lists).
-__global__ __launch_bounds__(kBlock) void k_merge_runs(Sys s, int64_t nruns, int64_t n, int64_t cap,
+__global__ __launch_bounds__(kBlock) void k_merge_runs(Sys s, const int64_t* __restrict__ pnr, int64_t n, int64_t cap,
                                                        const uint64_t* __restrict__ det,
                                                        int64_t* __restrict__ pop_new, const uint64_t* __restrict__ rdet,
                                                        int64_t* __restrict__ rpop, const int64_t* __restrict__ rflag,
                                                        int64_t scap, const int64_t* __restrict__ mwin,
                                                        int64_t* __restrict__ rpos, int64_t* __restrict__ rstate) {
+  const int64_t nruns = *pnr;
   __shared__ uint64_t wdet[FQ_MAX_WORDS][kMergeWin];  // the window's determinants when it fits
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
+  if (r0 >= nruns) return;  // whole block past the runs (the grid covers a bound); no window
   // every key of the block lies in [key(r0), key(r1)], so its lower bound
   // lies in [wlo, whi): whi = min(bound(r1) + 1, n)
   const int64_t b0 = mwin[2 * blockIdx.x], b1 = mwin[2 * blockIdx.x + 1];
@@ -1546,13 +1553,15 @@ __global__ __launch_bounds__(kBlock) void k_round_main(Sys s, int64_t n, int64_t
 }
 
 // round_low_population_spawns for new-determinant candidates.
-__global__ __launch_bounds__(kBlock) void k_round_new(Sys s, int64_t nruns, int64_t* __restrict__ rpop,
+__global__ __launch_bounds__(kBlock) void k_round_new(Sys s, const int64_t* __restrict__ pnr, int64_t rb,
+                                                      int64_t* __restrict__ rpop,
                                                       const int64_t* __restrict__ rstate, int64_t scap, int replay,
                                                       const double* __restrict__ rbuf, int64_t rbase,
                                                       const int64_t* __restrict__ ndraw_off, uint64_t seed,
                                                       uint32_t cyc, uint64_t* __restrict__ nmask,
                                                       int64_t* __restrict__ ncnt) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nruns = *pnr;
   bool alive = false;
   if (r < nruns && rstate[r] == 2) {
     if (s.real_amps) {
@@ -1567,7 +1576,7 @@ __global__ __launch_bounds__(kBlock) void k_round_new(Sys s, int64_t nruns, int6
     }
     for (int sp = 0; sp < s.S; ++sp) alive = alive || rpop[sp * scap + r] != 0;
   }
-  store_ballot(alive, r, nruns, nmask, ncnt);
+  store_ballot(alive, r, rb, nmask, ncnt);  // words up to the bound rb >= nruns (zero past nruns)
 }
 
 __global__ void k_zero(int64_t* __restrict__ p, int64_t n) {
@@ -1575,8 +1584,9 @@ __global__ void k_zero(int64_t* __restrict__ p, int64_t n) {
   if (i < n) p[i] = 0;
 }
 
-__global__ void k_ins_count(int64_t nruns, const uint64_t* __restrict__ nmask, const int64_t* __restrict__ rpos,
-                            int64_t* __restrict__ ins) {
+__global__ void k_ins_count(const int64_t* __restrict__ pnr, const uint64_t* __restrict__ nmask,
+                            const int64_t* __restrict__ rpos, int64_t* __restrict__ ins) {
+  const int64_t nruns = *pnr;
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= nruns || !bit_get(nmask, r)) return;
   atomicAdd(reinterpret_cast<unsigned long long*>(&ins[rpos[r]]), 1ull);
@@ -1584,9 +1594,12 @@ __global__ void k_ins_count(int64_t nruns, const uint64_t* __restrict__ nmask, c
 
 // Decide the new list length; insertion is skipped when the main list would
 // overflow (insert_new_walkers' psip_list%error path).
-__global__ void k_finalize_counts(int64_t n, int64_t nruns, const int64_t* __restrict__ moff,
+// pnr: the number of spawned runs (nullptr: none); also recorded in the stats.
+__global__ void k_finalize_counts(int64_t n, const int64_t* __restrict__ pnr, const int64_t* __restrict__ moff,
                                   const uint64_t* __restrict__ mmask, const int64_t* __restrict__ noff,
                                   const uint64_t* __restrict__ nmask, int64_t cap, Stats* __restrict__ st) {
+  const int64_t nruns = pnr ? *pnr : 0;
+  st->nruns = nruns;
   const int64_t nmain = bit_rank(moff, mmask, n), nnew = bit_rank(noff, nmask, nruns);
   if (nmain + nnew > cap) {
     st->main_overflow = 1;
@@ -1611,8 +1624,9 @@ __global__ void k_scatter_main(Sys s, int64_t n, int64_t cap, const uint64_t* __
 }
 
 // Surviving new runs in order: list[rank among survivors] = run index.
-__global__ void k_new_list(int64_t nruns, const uint64_t* __restrict__ nmask, const int64_t* __restrict__ noff,
-                           int64_t* __restrict__ list) {
+__global__ void k_new_list(const int64_t* __restrict__ pnr, const uint64_t* __restrict__ nmask,
+                           const int64_t* __restrict__ noff, int64_t* __restrict__ list) {
+  const int64_t nruns = *pnr;
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r < nruns && bit_get(nmask, r)) list[bit_rank(noff, nmask, r)] = r;
 }
@@ -1620,7 +1634,8 @@ __global__ void k_new_list(int64_t nruns, const uint64_t* __restrict__ nmask, co
 // Insert the surviving new runs, one thread each (list order = rank order),
 // with K_ii from the diagonal rule; a molecular system's dense pair tables are
 // staged in LDS when jk_lds (2 * norb^2 doubles) is nonzero.
-__global__ __launch_bounds__(kBlock) void k_scatter_new(Sys s, int64_t nruns, int64_t cap, int64_t scap,
+__global__ __launch_bounds__(kBlock) void k_scatter_new(Sys s, const int64_t* __restrict__ pnr, int64_t cap,
+                                                        int64_t scap,
                                                         const uint64_t* __restrict__ rdet,
                                                         const int64_t* __restrict__ rpop,
                                                         const int64_t* __restrict__ rpos,
@@ -1634,6 +1649,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_new(Sys s, int64_t nruns, in
                                                         double* __restrict__ kii_out, const Sys* __restrict__ sys_arr,
                                                         int jk_lds) {
   extern __shared__ double jk[];
+  const int64_t nruns = *pnr;
   const int64_t nnew = bit_rank(noff, nmask, nruns);
   const int64_t k0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
   if (k0 >= nnew || st->main_overflow) return;  // whole block: no surviving run left
@@ -1737,7 +1753,8 @@ int sync_stats(fq_engine* e) {
   return FQ_OK;
 }
 
-// Sort the spawned list (ns entries) and reduce runs; leaves nruns in h_stats.
+// Sort the spawned list (ns entries) and reduce runs; leaves the device
+// address of the run count in p_nruns and a host bound of it in run_bound.
 int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
   const int W = e->W;
   const int64_t scap = e->scap;
@@ -1805,16 +1822,23 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
     hipLaunchKernelGGL(k_pack_runs, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0, e->stream, nt, ns, e->run_tcnt,
                        e->run_toff, e->run_stage, e->run_start);
   }
-  int64_t nruns = 0;
+  // The number of runs stays on the device (run_toff[nt]); kernels size their
+  // grids by a host bound: ns in production, the exact count in replay (read
+  // there anyway, for the rounding draws).
+  int64_t bound = 0;
+  e->p_nruns = nullptr;
   if (ns > 0) {
-    FQ_HIP(hipMemcpyAsync(&nruns, e->run_toff + ceil_div(ns, kRunTile), sizeof(int64_t), hipMemcpyDeviceToHost,
-                          e->stream));
-    FQ_HIP(hipStreamSynchronize(e->stream));
-    hipLaunchKernelGGL(k_reduce_runs, dim3(grid_for(nruns)), dim3(kBlock), 0, e->stream, e->sys, nruns,
+    e->p_nruns = e->run_toff + ceil_div(ns, kRunTile);
+    bound = ns;
+    if (replay) {
+      FQ_HIP(hipMemcpyAsync(&bound, e->p_nruns, sizeof(int64_t), hipMemcpyDeviceToHost, e->stream));
+      FQ_HIP(hipStreamSynchronize(e->stream));
+    }
+    hipLaunchKernelGGL(k_reduce_runs, dim3(grid_for(bound)), dim3(kBlock), 0, e->stream, e->sys, e->p_nruns,
                        e->run_start, e->sdet, perm, e->gpop, e->gflag, scap, e->rdet, e->rpop, e->rflag, e->rstate);
   }
   FQ_HIP(hipEventRecord(e->pev[6], e->stream));
-  e->h_stats.nruns = nruns;
+  e->run_bound = bound;
   return FQ_OK;
 }
 
@@ -1823,15 +1847,16 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
 int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng_pos) {
   int rc = sort_and_reduce(e, ns, replay);
   if (rc != FQ_OK) return rc;
-  const int64_t nruns = e->h_stats.nruns;
+  const int64_t nruns = e->run_bound;  // a bound in production, the count in replay
+  const int64_t* pnr = e->p_nruns;
   const int64_t n = e->n, cap = e->cap, scap = e->scap;
   const int c = e->cur;
   hipStream_t st = e->stream;
   FQ_HIP(hipEventRecord(e->pev[7], st));
   if (nruns > 0) {
-    hipLaunchKernelGGL(k_merge_bounds, dim3(grid_for(grid_for(nruns))), dim3(kBlock), 0, st, nruns, n, cap, e->W,
+    hipLaunchKernelGGL(k_merge_bounds, dim3(grid_for(grid_for(nruns))), dim3(kBlock), 0, st, pnr, n, cap, e->W,
                        e->det[c], e->rdet, scap, e->mwin);
-    hipLaunchKernelGGL(k_merge_runs, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, n, cap, e->det[c],
+    hipLaunchKernelGGL(k_merge_runs, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, pnr, n, cap, e->det[c],
                        e->pop_new, e->rdet, e->rpop, e->rflag, scap, e->mwin, e->rpos, e->rstate);
   }
   // Rounding draws (replay: main list first, then new candidates, in order).
@@ -1856,7 +1881,7 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
   hipLaunchKernelGGL(k_round_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->pop_new,
                      replay ? 1 : 0, e->rng_buf, rng_pos, e->mdraw_off, seed, cyc, e->mmask, e->mcnt);
   if (nruns > 0)
-    hipLaunchKernelGGL(k_round_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, nruns, e->rpop, e->rstate,
+    hipLaunchKernelGGL(k_round_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, pnr, nruns, e->rpop, e->rstate,
                        scap, replay ? 1 : 0, e->rng_buf, rng_pos + main_draws, e->ndraw_off, seed, cyc, e->nmask,
                        e->ncnt);
   rng_pos += main_draws + new_draws;
@@ -1870,22 +1895,23 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
   }
   hipLaunchKernelGGL(k_zero, dim3(grid_for(n + 1)), dim3(kBlock), 0, st, e->ins, n + 1);
   if (nruns > 0)
-    hipLaunchKernelGGL(k_ins_count, dim3(grid_for(nruns)), dim3(kBlock), 0, st, nruns, e->nmask, e->rpos, e->ins);
+    hipLaunchKernelGGL(k_ins_count, dim3(grid_for(nruns)), dim3(kBlock), 0, st, pnr, e->nmask, e->rpos, e->ins);
   rc = scan_exclusive(e, e->ins, e->ins_scan, n + 1);
   if (rc != FQ_OK) return rc;
-  hipLaunchKernelGGL(k_finalize_counts, dim3(1), dim3(1), 0, st, n, nruns, e->moff, e->mmask, e->noff, e->nmask, cap,
+  hipLaunchKernelGGL(k_finalize_counts, dim3(1), dim3(1), 0, st, n, nruns > 0 ? pnr : nullptr, e->moff, e->mmask,
+                     e->noff, e->nmask, cap,
                      e->d_stats);
   const int o = 1 - c;
   hipLaunchKernelGGL(k_scatter_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->det[c], e->pop_new,
                      e->kii[c], e->mmask, e->moff, e->ins_scan, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
   if (nruns > 0) {
-    hipLaunchKernelGGL(k_new_list, dim3(grid_for(nruns)), dim3(kBlock), 0, st, nruns, e->nmask, e->noff, e->nlist);
+    hipLaunchKernelGGL(k_new_list, dim3(grid_for(nruns)), dim3(kBlock), 0, st, pnr, e->nmask, e->noff, e->nlist);
     // dense pair tables in LDS for the diagonal when they fit in 32 KB
     const size_t jk = e->sys.system == FQ_SYS_READ_IN && !e->batched
                           ? 2 * sizeof(double) * static_cast<size_t>(e->sys.norb) * e->sys.norb
                           : 0;
     const bool use_lds = jk > 0 && jk <= 32768;
-    hipLaunchKernelGGL(k_scatter_new, dim3(grid_for(nruns)), dim3(kBlock), use_lds ? jk : 0, st, e->sys, nruns, cap,
+    hipLaunchKernelGGL(k_scatter_new, dim3(grid_for(nruns)), dim3(kBlock), use_lds ? jk : 0, st, e->sys, pnr, cap,
                        scap, e->rdet, e->rpop, e->rpos, e->nmask, e->noff, e->nlist, e->mmask, e->moff, e->d_stats,
                        e->det[o], e->pop[o], e->kii[o], e->batched ? e->d_sys_arr : nullptr, use_lds ? 1 : 0);
   }
