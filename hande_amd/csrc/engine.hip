@@ -282,6 +282,7 @@ struct fq_engine {
   float* mlp_tab = nullptr;  // rows x 2*nbasis, one row per determinant the network runs on
   float* mlp_flat = nullptr;  // 2*nbasis ones: weights of determinants without a row
   const int64_t* p_nruns = nullptr;  // device address of this cycle's run count
+  int64_t att_bound = -1;            // bound of the next cycle's attempts (-1: unknown)
   int64_t run_bound = 0;             // host bound of it (the count itself in replay mode)
   int64_t mlp_tab_rows = 0;
   int64_t* mlp_on = nullptr;   // [cap] network runs on this determinant this cycle
@@ -556,11 +557,20 @@ constexpr int kSpawnTile = 4 * kBlock;  // attempts per block tile
 
 // Parents of each spawning tile's first and last attempt (one thread per
 // tile, so the searches' latencies overlap instead of opening every block).
+// Parent windows of the tiles of the batch [t_begin, t_end), t_end clamped to
+// the attempt total att_off[nent] (the host may only know a bound of it);
+// tiles of the grid past the total get a zero count.
 __global__ void k_tile_bounds(int64_t nent, const int64_t* __restrict__ att_off, int64_t t_begin, int64_t t_end,
-                              int64_t* __restrict__ tile_win) {
-  const int64_t ntiles = (t_end - t_begin + kSpawnTile - 1) / kSpawnTile;
+                              int64_t* __restrict__ tile_win, int64_t* __restrict__ tile_cnt) {
+  const int64_t nt_batch = (t_end - t_begin + kSpawnTile - 1) / kSpawnTile;  // tiles the host sized for
+  const int64_t tot = att_off[nent];
+  t_end = t_end < tot ? t_end : tot;
+  const int64_t ntiles = t_end > t_begin ? (t_end - t_begin + kSpawnTile - 1) / kSpawnTile : 0;
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tile >= ntiles) return;
+  if (tile >= ntiles) {
+    if (tile < nt_batch) tile_cnt[tile] = 0;
+    return;
+  }
   const int64_t t0 = t_begin + tile * kSpawnTile;
   const int64_t t1 = (t0 + kSpawnTile < t_end) ? t0 + kSpawnTile : t_end;
   const int64_t p0 = parent_of(att_off, t0, 0, nent - 1);
@@ -618,7 +628,8 @@ __global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, 
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long below = (1ull << lane) - 1;
-  const int64_t ntiles = (t_end - t_begin + kSpawnTile - 1) / kSpawnTile;
+  if (const int64_t tot = att_off[nent]; tot < t_end) t_end = tot;  // the host may pass a bound
+  const int64_t ntiles = t_end > t_begin ? (t_end - t_begin + kSpawnTile - 1) / kSpawnTile : 0;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t t0 = t_begin + tile * kSpawnTile;
     const int64_t t1 = (t0 + kSpawnTile < t_end) ? t0 + kSpawnTile : t_end;
@@ -783,8 +794,10 @@ __global__ __launch_bounds__(kBlock) void k_pack_tiles(Sys s, int64_t ntiles, co
   }
 }
 
-__global__ void k_add_spawned(const int64_t* __restrict__ pos, int64_t nb, Stats* __restrict__ st) {
+__global__ void k_add_spawned(const int64_t* __restrict__ pos, int64_t nb, const int64_t* __restrict__ ptot,
+                              Stats* __restrict__ st) {
   st->nspawn += pos[nb];
+  st->total_attempts = *ptot;
 }
 
 // ---------------------------------------------------------------------------
@@ -1924,6 +1937,7 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
   if (rc != FQ_OK) return rc;
   e->cur = o;
   e->n = e->h_stats.new_n;
+  e->att_bound = -1;
   return FQ_OK;
 }
 
@@ -2227,14 +2241,15 @@ bool hb_lds_fits(int device, int nel) {
 
 // Spawned entries of a batch of nbt attempts starting at t_begin: scan the
 // per-tile success counts, pack the tiles in order, advance the spawned count.
-int pack_spawned(fq_engine* e, int64_t nbt, int64_t t_begin) {
+// ptot: the attempt total on the device (recorded in the stats).
+int pack_spawned(fq_engine* e, int64_t nbt, int64_t t_begin, const int64_t* ptot) {
   const int64_t nt = ceil_div(nbt, kSpawnTile);
   int rc = scan_exclusive(e, e->tile_cnt, e->tile_off, nt);
   if (rc != FQ_OK) return rc;
   hipLaunchKernelGGL(k_pack_tiles, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0, e->stream, e->sys, nt,
                      e->tile_cnt, e->tile_off, e->st_det, e->st_pop, e->st_ent, e->bcap, t_begin, e->flags,
                      e->scap, e->sdet, e->srec, e->d_stats);
-  hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, e->stream, e->tile_off, nt, e->d_stats);
+  hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, e->stream, e->tile_off, nt, ptot, e->d_stats);
   return FQ_OK;
 }
 
@@ -2479,6 +2494,7 @@ int fq_load(fq_engine* e, const uint64_t* states, const int64_t* pops, const dou
   if (n > 0) FQ_HIP(hipMemcpyAsync(e->kii[0], dat, 8 * n, hipMemcpyHostToDevice, e->stream));
   FQ_HIP(hipStreamSynchronize(e->stream));
   e->n = n;
+  e->att_bound = -1;
   return FQ_OK;
 }
 
@@ -2562,40 +2578,57 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
     const int64_t nent = e->S * n;
     int rc = scan_exclusive(e, e->nat, e->att_off, nent);
     if (rc != FQ_OK) return rc;
-    int64_t total = 0;
-    FQ_HIP(hipMemcpyAsync(&total, e->att_off + nent, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    FQ_HIP(hipStreamSynchronize(st));
+    // The spawning launches cover attempts [0, bound): after a production
+    // cycle the host holds a bound of this cycle's total (every entry makes at
+    // most floor(|N|) + 1 attempts, |N| for integer populations), so it need
+    // not wait for the prefix sum; otherwise it reads the total.  The kernels
+    // clamp to the total on the device.
+    int64_t bound = e->att_bound;
+    if (bound <= 0) {  // unknown (or an empty list): the exact total
+      FQ_HIP(hipMemcpyAsync(&bound, e->att_off + nent, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+      FQ_HIP(hipStreamSynchronize(st));
+    }
     FQ_HIP(hipEventRecord(e->pev[2], st));
     e->spawn_launches = 0;
     if (int rs = sync_sysp(e); rs != FQ_OK) return rs;
-    for (int64_t tb = 0; tb < total; tb += e->bcap) {
-      const int64_t te = std::min(total, tb + e->bcap);
-      const int64_t nbt = te - tb;
-      const unsigned g = static_cast<unsigned>(std::min<int64_t>(ceil_div(nbt, kSpawnTile), 1 << 20));
-      const int li = e->spawn_launches++;
-      hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(ceil_div(nbt, kSpawnTile))), dim3(kBlock), 0, st, nent,
-                         e->att_off, tb, te, e->tile_win);
-      if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li], st));
-      const int gen = fqd::gen_of(e->sys.system, e->sys.excit_gen);
-      auto kern = gen == fqd::kGenHubK       ? k_spawn<fqd::kGenHubK, false>
-                  : gen == fqd::kGenHeatBath ? k_spawn<fqd::kGenHeatBath, false>
-                  : gen == fqd::kGenLearned  ? k_spawn<fqd::kGenLearned, false>
-                                             : k_spawn<fqd::kGenMolRenorm, false>;
-      const size_t lds = gen == fqd::kGenHeatBath ? hb_lds_bytes(e->sys.nel)
-                         : (gen == fqd::kGenMolRenorm || gen == fqd::kGenLearned)
-                             ? sizeof(uint64_t) * e->W * kSpawnTile
-                             : 0;
-      hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), lds, st, e->d_sysp, nent, cap, tb, te, e->att_off, e->det[c],
-                         e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
-                         e->tile_cnt, e->tile_win, nullptr, nullptr, nullptr);
-      if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
-      rc = pack_spawned(e, nbt, tb);
-      if (rc != FQ_OK) return rc;
-    }
+    auto spawn_range = [&](int64_t from, int64_t to) -> int {
+      for (int64_t tb = from; tb < to;) {
+        const int64_t te = std::min(to, tb + e->bcap);
+        const int64_t nbt = te - tb;
+        const unsigned g = static_cast<unsigned>(std::min<int64_t>(ceil_div(nbt, kSpawnTile), 1 << 20));
+        const int li = e->spawn_launches++;
+        hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(ceil_div(nbt, kSpawnTile))), dim3(kBlock), 0, st, nent,
+                           e->att_off, tb, te, e->tile_win, e->tile_cnt);
+        if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li], st));
+        const int gen = fqd::gen_of(e->sys.system, e->sys.excit_gen);
+        auto kern = gen == fqd::kGenHubK       ? k_spawn<fqd::kGenHubK, false>
+                    : gen == fqd::kGenHeatBath ? k_spawn<fqd::kGenHeatBath, false>
+                    : gen == fqd::kGenLearned  ? k_spawn<fqd::kGenLearned, false>
+                                               : k_spawn<fqd::kGenMolRenorm, false>;
+        const size_t lds = gen == fqd::kGenHeatBath ? hb_lds_bytes(e->sys.nel)
+                           : (gen == fqd::kGenMolRenorm || gen == fqd::kGenLearned)
+                               ? sizeof(uint64_t) * e->W * kSpawnTile
+                               : 0;
+        hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), lds, st, e->d_sysp, nent, cap, tb, te, e->att_off,
+                           e->det[c], e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop,
+                           e->st_ent, e->tile_cnt, e->tile_win, nullptr, nullptr, nullptr);
+        if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
+        const int prc = pack_spawned(e, nbt, tb, e->att_off + nent);
+        if (prc != FQ_OK) return prc;
+        tb = te;
+      }
+      return FQ_OK;
+    };
+    rc = spawn_range(0, bound);
+    if (rc != FQ_OK) return rc;
     FQ_HIP(hipEventRecord(e->pev[3], st));
     rc = sync_stats(e);
     if (rc != FQ_OK) return rc;
-    e->h_stats.total_attempts = total;
+    if (const int64_t total = e->h_stats.total_attempts; total > bound) {  // the bound cannot be exceeded
+      rc = spawn_range(bound, total);
+      if (rc == FQ_OK) rc = sync_stats(e);
+      if (rc != FQ_OK) return rc;
+    }
     ns = std::min<int64_t>(e->h_stats.nspawn, e->scap);
   }
   FQ_HIP(hipEventRecord(e->ev[1], st));
@@ -2606,6 +2639,12 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
   FQ_HIP(hipEventSynchronize(e->ev[2]));
   // restore spawn-phase quantities the annihilation's stat resets do not touch
   e->h_stats.total_attempts = keep.total_attempts;
+  if (!replay) {  // next cycle's attempts <= sum floor(|N|) (+1 per entry with real amplitudes)
+    unsigned long long sum = 0;
+    for (int sp = 0; sp < e->S; ++sp) sum += e->h_stats.abs_pop[sp];
+    e->att_bound = static_cast<int64_t>(sum / static_cast<unsigned long long>(e->sys.rf)) +
+                   (e->sys.real_amps ? e->n * e->S : 0);
+  }
   const int64_t used = rng_pos - e->rng_pos;
   e->rng_pos = rng_pos;
   (void)hipEventElapsedTime(&e->t_spawn, e->ev[0], e->ev[1]);
@@ -2850,14 +2889,14 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
     const unsigned g = static_cast<unsigned>(std::min<int64_t>(ceil_div(nbt, kSpawnTile), 1 << 20));
     const int li = e->spawn_launches++;
     hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(ceil_div(nbt, kSpawnTile))), dim3(kBlock), 0, st, nent,
-                       e->att_off, tb, te, e->tile_win);
+                       e->att_off, tb, te, e->tile_win, e->tile_cnt);
     if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li], st));
     auto kern = e->sys.system == FQ_SYS_HUB_K ? k_spawn<fqd::kGenHubK, true> : k_spawn<fqd::kGenMolRenorm, true>;
     hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->d_sysp, nent, cap, tb, te, e->att_off, e->det[c],
                        e->pop[c], 0.0, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
                        e->tile_cnt, e->tile_win, e->d_sys_arr, e->d_iatt, e->d_ipar);
     if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
-    rc = pack_spawned(e, nbt, tb);
+    rc = pack_spawned(e, nbt, tb, e->att_off + nent);
     if (rc != FQ_OK) return rc;
   }
   FQ_HIP(hipEventRecord(e->pev[3], st));
