@@ -1186,9 +1186,13 @@ __device__ inline int partner_pick(const Sys& s, const uint64_t* f, const float*
   return last;
 }
 
+// aff_in / flat_in: copies of s.mlp_aff / s.mlp_flat (in LDS in the spawning
+// kernel); nullptr reads the device tables.
 template <class Rng>
 __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, const float* wrow, Excit& conn,
-                                   double& h, double& pgen) {
+                                   double& h, double& pgen, const float* aff_in = nullptr,
+                                   const float* flat_in = nullptr) {
+  const float* aff = aff_in ? aff_in : s.mlp_aff;
   const SymUnocc u = sym_unocc(s, f);
   pgen = 1.0;
   h = 0.0;
@@ -1200,7 +1204,7 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
   if (rng.next() < s.psingle) {
     allowed = single_mol(s, rng, f, u, conn, h, pgen);
   } else {
-    const float* row = wrow ? wrow : s.mlp_flat;
+    const float* row = wrow ? wrow : (flat_in ? flat_in : s.mlp_flat);
     const double mix = s.mix;
     const float* ew = row;
     const float* fw = row + nb;
@@ -1257,8 +1261,8 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
           }
         }
       if (!j) j = lastj;
-      const float* ai = s.mlp_aff + static_cast<int64_t>(i - 1) * nb;
-      const float* aj = s.mlp_aff + static_cast<int64_t>(j - 1) * nb;
+      const float* ai = aff + (i - 1) * nb;
+      const float* aj = aff + (j - 1) * nb;
       double z1 = 0.0;
       FQ_FOR_WORDS(w, s.W) {
         for (uint64_t y = un[w]; y; y &= y - 1) {
@@ -1290,8 +1294,8 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
         b = partner_pick(s, f, fw, aj, cb, a, 0, rng.next() * zbj, true);
     }
     if (ok) {
-      const float* ai = s.mlp_aff + static_cast<int64_t>(i - 1) * nb;
-      const float* aj = s.mlp_aff + static_cast<int64_t>(j - 1) * nb;
+      const float* ai = aff + (i - 1) * nb;
+      const float* aj = aff + (j - 1) * nb;
       int nbc = 0, nac = 0;
       double zbi = 0.0, zbj = 0.0, zai = 0.0, zaj = 0.0;  // Z2 of holes i, j over a's (b's) partner class
       partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, a), a, nbc, zbi, zbj);
@@ -1332,9 +1336,9 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
 
 template <class Rng>
 __device__ inline int64_t spawn_learned(const Sys& s, Rng& rng, const uint64_t* f, const float* wrow, int64_t parent,
-                                        double tau, Excit& conn) {
+                                        double tau, Excit& conn, const float* aff, const float* flat) {
   double h = 0.0, pgen = 1.0;
-  gen_learned(s, rng, f, wrow, conn, h, pgen);
+  gen_learned(s, rng, f, wrow, conn, h, pgen, aff, flat);
   return attempt_to_spawn(s, rng, tau, h, pgen, parent);
 }
 
@@ -1365,7 +1369,8 @@ __host__ __device__ inline int gen_of(int system, int excit_gen) {
 // heat-bath generator's weight storage (a local array when not given).
 template <int G = kGenAny, class Rng>
 __device__ inline int64_t spawn_attempt(const Sys& s, Rng& rng, const uint64_t* f, const float* wrow,
-                                        int64_t parent, double tau, uint64_t* child, HbBuf hb = HbBuf{}) {
+                                        int64_t parent, double tau, uint64_t* child, HbBuf hb = HbBuf{},
+                                        const float* aff = nullptr, const float* flat = nullptr) {
   Excit conn;
   const int g = G == kGenAny ? gen_of(s.system, s.excit_gen) : G;
   int64_t n;
@@ -1380,7 +1385,7 @@ __device__ inline int64_t spawn_attempt(const Sys& s, Rng& rng, const uint64_t* 
     }
   }
   else if ((G == kGenAny || G == kGenLearned) && g == kGenLearned) {
-    n = spawn_learned(s, rng, f, wrow, parent, tau, conn);
+    n = spawn_learned(s, rng, f, wrow, parent, tau, conn, aff, flat);
   } else {
     n = spawn_mol(s, rng, f, parent, tau, conn);
   }

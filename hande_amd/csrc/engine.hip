@@ -622,9 +622,20 @@ __global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, 
   __shared__ int32_t rent[kPart ? kSpawnTile : 1];    // parent entry - lo
   __shared__ int32_t wcs[kBlock / 64];
   uint64_t* rdet = nullptr;                            // [w * kSpawnTile + r], dynamic LDS
+  const float* aff_lds = nullptr;  // learned: affinity table then flat row, after rdet
+  const float* flat_lds = nullptr;
   if constexpr (kPart) {
     extern __shared__ uint64_t part_lds[];
     rdet = part_lds;
+    if constexpr (G == fqd::kGenLearned) {
+      float* a = reinterpret_cast<float*>(part_lds + s.W * kSpawnTile);
+      const int na = s.nbasis * s.nbasis;
+      for (int k = threadIdx.x; k < na + 2 * s.nbasis; k += kBlock)
+        a[k] = k < na ? s.mlp_aff[k] : s.mlp_flat[k - na];
+      aff_lds = a;
+      flat_lds = a + na;
+      __syncthreads();
+    }
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long below = (1ull << lane) - 1;
@@ -676,7 +687,7 @@ __global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, 
         fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
                         cyc ^ (fqd::kDrawAttempt << 24));
         const float* wrow = (G == fqd::kGenLearned && s.mlp_on[i]) ? s.mlp_tab + s.mlp_row[i] * s.mlp_ld : nullptr;
-        nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child, hb);
+        nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child, hb, aff_lds, flat_lds);
       }
     };
     if constexpr (kPart) {
@@ -2239,6 +2250,21 @@ bool hb_lds_fits(int device, int nel) {
   return fa.sharedSizeBytes + hb_lds_bytes(nel) <= static_cast<size_t>(maxlds);
 }
 
+// Dynamic LDS of the learned spawning kernel: the tile's parked child
+// determinants, the affinity table and the flat weight row.
+size_t learned_lds_bytes(int W, int nbasis) {
+  const size_t nb = static_cast<size_t>(nbasis);
+  return sizeof(uint64_t) * W * kSpawnTile + sizeof(float) * (nb * nb + 2 * nb);
+}
+bool learned_lds_fits(int device, int W, int nbasis) {
+  hipFuncAttributes fa;
+  int maxlds = 0;
+  if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_spawn<fqd::kGenLearned, false>)) != hipSuccess ||
+      hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+    return false;
+  return fa.sharedSizeBytes + learned_lds_bytes(W, nbasis) <= static_cast<size_t>(maxlds);
+}
+
 // Spawned entries of a batch of nbt attempts starting at t_begin: scan the
 // per-tile success counts, pack the tiles in order, advance the spawned count.
 // ptot: the attempt total on the device (recorded in the stats).
@@ -2355,6 +2381,10 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
     return FQ_ERR_NO_DEVICE;
   }
   if (params->excit_gen == FQ_EXCIT_HEAT_BATH && !hb_lds_fits(e->device, sys->nel)) {
+    delete e;
+    return FQ_ERR_ARG;
+  }
+  if (params->excit_gen == FQ_EXCIT_LEARNED && !learned_lds_fits(e->device, sys->nwords, sys->nbasis)) {
     delete e;
     return FQ_ERR_ARG;
   }
@@ -2605,10 +2635,10 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
                     : gen == fqd::kGenHeatBath ? k_spawn<fqd::kGenHeatBath, false>
                     : gen == fqd::kGenLearned  ? k_spawn<fqd::kGenLearned, false>
                                                : k_spawn<fqd::kGenMolRenorm, false>;
-        const size_t lds = gen == fqd::kGenHeatBath ? hb_lds_bytes(e->sys.nel)
-                           : (gen == fqd::kGenMolRenorm || gen == fqd::kGenLearned)
-                               ? sizeof(uint64_t) * e->W * kSpawnTile
-                               : 0;
+        const size_t lds = gen == fqd::kGenHeatBath    ? hb_lds_bytes(e->sys.nel)
+                           : gen == fqd::kGenMolRenorm ? sizeof(uint64_t) * e->W * kSpawnTile
+                           : gen == fqd::kGenLearned   ? learned_lds_bytes(e->W, e->sys.nbasis)
+                                                       : 0;
         hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), lds, st, e->d_sysp, nent, cap, tb, te, e->att_off,
                            e->det[c], e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop,
                            e->st_ent, e->tile_cnt, e->tile_win, nullptr, nullptr, nullptr);

@@ -133,9 +133,10 @@ typedef struct fq_engine fq_engine;
  * bound in init_proc_pointers (src/qmc.F90:217-703).
  * FQ_ERR_ARG for an inconsistent request, including the heat-bath or learned
  * generator on a system other than read_in or with more than 32 electrons,
- * and the heat-bath generator when its per-attempt weights (8 bytes per
+ * the heat-bath generator when its per-attempt weights (8 bytes per
  * electron per thread of a 256-thread workgroup) do not fit the device's
- * workgroup LDS; FQ_ERR_BIASED when the heat-bath tables cannot reach some
+ * workgroup LDS, and the learned generator when its affinity table (4 nbasis^2
+ * bytes) with the spawning tile's staging does not; FQ_ERR_BIASED when the heat-bath tables cannot reach some
  * single excitation (the reference stops there). */
 int fq_create(const fq_system *sys, const fq_params *params, fq_engine **out);
 
