@@ -430,31 +430,25 @@ def efficiency_learned(args):
     res = {}
     runs = [("renorm", N.FQ_EXCIT_RENORM, 0), ("heat_bath", N.FQ_EXCIT_HEAT_BATH, 0)]
     runs += [(f"learned_k{k}", N.FQ_EXCIT_LEARNED, k) for k in thresholds]
-    trained = None  # (parameters, training seconds): one training serves every threshold
+    training = {}
     for gen, code, k in runs:
         eng, _p = make_engine(lib, system, args.target, 23, cap, cap, excit_gen=code, mix=0.1, min_attempts=k)
         t_train = 0.0
         if code == N.FQ_EXCIT_LEARNED:
-            if trained is None:
-                st, po, da = state[:3]
-                N.check(lib.fq_load(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)),
-                                    po.ctypes.data_as(C.POINTER(C.c_int64)), da.ctypes.data_as(C.POINTER(C.c_double)),
-                                    len(da)), "fq_load")
-                losses = np.zeros(args.train_steps)
-                t1 = time.perf_counter()
-                N.check(lib.fq_mlp_train(eng, 16384, args.train_steps, 1e-3, 5,
-                                         losses.ctypes.data_as(C.POINTER(C.c_double))), "fq_mlp_train")
-                t_train = time.perf_counter() - t1
-                npar = C.c_int64()
-                N.check(lib.fq_mlp_nparams(eng, C.byref(npar)), "fq_mlp_nparams")
-                params = np.zeros(npar.value, dtype=np.float32)
-                N.check(lib.fq_mlp_get_params(eng, params.ctypes.data_as(C.POINTER(C.c_float)), npar.value), "get")
-                trained = (params, t_train, float(losses[:10].mean()), float(losses[-10:].mean()))
-                log(f"[eff] trained {args.train_steps} steps in {t_train:.1f}s: loss {trained[2]:.4f} -> "
-                    f"{trained[3]:.4f}")
-            else:
-                params, t_train = trained[0], trained[1]
-                N.check(lib.fq_mlp_set_params(eng, params.ctypes.data_as(C.POINTER(C.c_float)), len(params)), "set")
+            # trained for its threshold: samples of determinants below it are scored with flat weights
+            st, po, da = state[:3]
+            N.check(lib.fq_load(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                po.ctypes.data_as(C.POINTER(C.c_int64)), da.ctypes.data_as(C.POINTER(C.c_double)),
+                                len(da)), "fq_load")
+            losses = np.zeros(args.train_steps)
+            t1 = time.perf_counter()
+            N.check(lib.fq_mlp_train(eng, 16384, args.train_steps, 1e-3, 5,
+                                     losses.ctypes.data_as(C.POINTER(C.c_double))), "fq_mlp_train")
+            t_train = time.perf_counter() - t1
+            training[gen] = {"seconds": t_train, "loss_first10": float(losses[:10].mean()),
+                             "loss_last10": float(losses[-10:].mean())}
+            log(f"[eff] {gen}: trained {args.train_steps} steps in {t_train:.1f}s: loss "
+                f"{training[gen]['loss_first10']:.4f} -> {training[gen]['loss_last10']:.4f}")
         # each generator settles its own (initiator) stationary state before the timed cycles
         prof = {}
         proj, d0, wall, att, last = production(lib, eng, args.target, state, 1e-3, args.eff_cycles, args.eff_warmup,
@@ -497,7 +491,7 @@ def efficiency_learned(args):
                                              f"{args.target:g}, {args.eff_cycles} production cycles",
                                  "proposal_mix": 0.1, "proposal_min_attempts": thresholds[0],
                                  "train_steps": args.train_steps, "warmup_cycles": args.eff_warmup},
-                      "training": {"seconds": trained[1], "loss_first10": trained[2], "loss_last10": trained[3]},
+                      "training": training,
                       "runs": res}), flush=True)
 
 
@@ -655,7 +649,7 @@ def main():
                          "reblocking; 1000 batched)")
     ap.add_argument("--train-steps", type=int, default=1000)
     ap.add_argument("--eff-warmup", type=int, default=500, help="untimed cycles per generator before measuring")
-    ap.add_argument("--min-attempts", default="16,4,64",
+    ap.add_argument("--min-attempts", default="64,16",
                     help="learned proposal: comma-separated attempt thresholds for the network (the first is "
                          "the headline value); determinants below use flat weights with the learned affinity")
     ap.add_argument("--instances", type=int, default=64)

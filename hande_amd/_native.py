@@ -116,7 +116,7 @@ def _declare(lib):
         "fq_mlp_get_params": (C.c_int, [vp, P(C.c_float), i64]),
         "fq_mlp_forward": (C.c_int, [vp, P(u64), i64, P(C.c_float)]),
         "fq_mlp_affinity": (C.c_int, [vp, P(C.c_float), i64]),
-        "fq_mlp_loss_grad": (C.c_int, [vp, P(u64), P(i32), P(f64), i64, P(f64), P(C.c_float)]),
+        "fq_mlp_loss_grad": (C.c_int, [vp, P(u64), P(i32), P(f64), P(i32), i64, P(f64), P(C.c_float)]),
         "fq_mlp_train": (C.c_int, [vp, i64, i32, f64, u64, P(f64)]),
         "fq_probe_excitations": (C.c_int, [vp, P(u64), i64, u64, P(i32), P(f64), P(f64)]),
         "fq_destroy": (None, [vp]),

@@ -3058,8 +3058,8 @@ int fq_mlp_forward(fq_engine* e, const uint64_t* dets, int64_t n, float* out) {
   return rc;
 }
 
-int fq_mlp_loss_grad(fq_engine* e, const uint64_t* dets, const int32_t* exc, const double* w, int64_t n,
-                     double* loss, float* grad) {
+int fq_mlp_loss_grad(fq_engine* e, const uint64_t* dets, const int32_t* exc, const double* w, const int32_t* flat,
+                     int64_t n, double* loss, float* grad) {
   if (!e || !dets || !exc || !w || !loss || n < 1 || e->par.excit_gen != FQ_EXCIT_LEARNED) return FQ_ERR_ARG;
   fqm::Mlp& m = e->mlp;
   if (n > m.bcap) return FQ_ERR_CAPACITY;
@@ -3082,6 +3082,8 @@ int fq_mlp_loss_grad(fq_engine* e, const uint64_t* dets, const int32_t* exc, con
     FQ_HIP(hipMemcpyAsync(m.sdet + q * m.bcap, soa.data() + q * n, sizeof(uint64_t) * n, hipMemcpyHostToDevice, st));
   FQ_HIP(hipMemcpyAsync(m.sexc, exc, sizeof(int32_t) * 4 * n, hipMemcpyHostToDevice, st));
   FQ_HIP(hipMemcpyAsync(m.sw, w, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  const std::vector<int32_t> fl = flat ? std::vector<int32_t>(flat, flat + n) : std::vector<int32_t>(n, 0);
+  FQ_HIP(hipMemcpyAsync(m.sflat, fl.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
   int rc = fqm::mlp_train_step(m, e->sys, n, 0.0, false, st);
   if (rc != FQ_OK) return rc;
   double sums[2] = {0.0, 0.0};
@@ -3101,7 +3103,8 @@ int fq_mlp_train(fq_engine* e, int64_t batch, int32_t steps, double lr, uint64_t
   hipStream_t st = e->stream;
   const int c = e->cur;
   for (int32_t k = 0; k < steps; ++k) {
-    int rc = fqm::mlp_sample(m, e->sys, e->det[c], e->pop[c], e->cap, e->n, batch, seed,
+    int rc = fqm::mlp_sample(m, e->sys, e->det[c], e->pop[c], e->cap, e->n, batch,
+                             static_cast<int64_t>(e->par.proposal_min_attempts), seed,
                              static_cast<uint32_t>(m.adam_t), st, scan_cb, e);
     if (rc != FQ_OK) return rc;
     rc = fqm::mlp_train_step(m, e->sys, batch, lr, true, st);

@@ -221,11 +221,14 @@ __global__ void k_dsum(int64_t n, const double* __restrict__ x, double* __restri
 // The network component sums four paths (h1, h2, p1, p2): the first particle
 // p1 is drawn with h1's affinity over the unoccupied orbitals (normaliser
 // Z1(h1)), its partner p2 with h2's affinity over p1's partner class (Z2).
+// A sample flagged flat (sflat, may be null) is scored as a determinant the
+// network does not run on: hole and particle weights 1 (logits 0), so only the
+// affinity receives its gradient.
 __global__ void k_mlp_loss_grad(fqd::Sys s, int64_t ns, int nb, int d_out, const uint64_t* __restrict__ sdet,
                                 int64_t bcap, const int32_t* __restrict__ sexc, const double* __restrict__ sw,
-                                const double* __restrict__ wsum, const float* __restrict__ z3,
-                                const float* __restrict__ th, float* __restrict__ d3, float* __restrict__ dth,
-                                double* __restrict__ sloss) {
+                                const int32_t* __restrict__ sflat, const double* __restrict__ wsum,
+                                const float* __restrict__ z3, const float* __restrict__ th, float* __restrict__ d3,
+                                float* __restrict__ dth, double* __restrict__ sloss) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= ns) return;
   const float* z = z3 + t * d_out;
@@ -241,8 +244,9 @@ __global__ void k_mlp_loss_grad(fqd::Sys s, int64_t ns, int nb, int d_out, const
   uint64_t f[FQ_MAX_WORDS] = {0, 0};
   for (int w = 0; w < s.W; ++w) f[w] = sdet[static_cast<int64_t>(w) * bcap + t];
   const int i = sexc[4 * t], j = sexc[4 * t + 1], a = sexc[4 * t + 2], b = sexc[4 * t + 3];
-  auto ew = [&](int o) { return exp(static_cast<double>(fminf(fmaxf(z[o - 1], -kClip), kClip))); };
-  auto fw = [&](int o) { return exp(static_cast<double>(fminf(fmaxf(z[nb + o - 1], -kClip), kClip))); };
+  const bool flat = sflat && sflat[t];
+  auto ew = [&](int o) { return flat ? 1.0 : exp(static_cast<double>(fminf(fmaxf(z[o - 1], -kClip), kClip))); };
+  auto fw = [&](int o) { return flat ? 1.0 : exp(static_cast<double>(fminf(fmaxf(z[nb + o - 1], -kClip), kClip))); };
   auto aff = [&](int hole, int o) {
     return exp(static_cast<double>(fminf(fmaxf(th[(hole - 1) * nb + o - 1], -kClip), kClip)));
   };
@@ -299,7 +303,7 @@ __global__ void k_mlp_loss_grad(fqd::Sys s, int64_t ns, int nb, int d_out, const
       gj = r1 * (db - (inb ? wj / zbj : 0.0)) + r2 * (da - (ina ? wj / zaj : 0.0)) + r3 * (da - wj / z1j) +
            r4 * (db - wj / z1j);
     }
-    const bool in_h = fabsf(z[o - 1]) < kClip, in_p = fabsf(z[nb + o - 1]) < kClip;
+    const bool in_h = !flat && fabsf(z[o - 1]) < kClip, in_p = !flat && fabsf(z[nb + o - 1]) < kClip;
     d[o - 1] = in_h ? static_cast<float>(-wt * gu) : 0.0f;
     d[nb + o - 1] = in_p ? static_cast<float>(-wt * (gi + gj)) : 0.0f;
     gth[o - 1] = fabsf(th[(i - 1) * nb + o - 1]) < kClip ? static_cast<float>(-wt * gi) : 0.0f;
@@ -367,10 +371,13 @@ __global__ void k_det_weight(int64_t n, int64_t cap, int S, const int64_t* __res
   out[d] = w;
 }
 
+// One heat-bath training sample per thread; its determinant is flagged flat
+// when even rounded up it makes fewer than min_att attempts (the network does
+// not run on it in production).
 __global__ void k_hb_sample(fqd::Sys s, const uint64_t* __restrict__ det, int64_t cap, int64_t n,
-                            const int64_t* __restrict__ cum, int64_t ns, uint64_t seed, uint32_t step,
-                            uint64_t* __restrict__ sdet, int64_t bcap, int32_t* __restrict__ sexc,
-                            double* __restrict__ sw) {
+                            const int64_t* __restrict__ cum, const int64_t* __restrict__ dw, int64_t min_att,
+                            int64_t ns, uint64_t seed, uint32_t step, uint64_t* __restrict__ sdet, int64_t bcap,
+                            int32_t* __restrict__ sexc, double* __restrict__ sw, int32_t* __restrict__ sflat) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= ns) return;
   fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32), step ^ (fqd::kDrawTrain << 24));
@@ -397,6 +404,7 @@ __global__ void k_hb_sample(fqd::Sys s, const uint64_t* __restrict__ det, int64_
   sexc[4 * t + 2] = conn.to[0];
   sexc[4 * t + 3] = conn.to[1];
   sw[t] = use ? fabs(h) / pgen : 0.0;
+  sflat[t] = dw[lo] < min_att ? 1 : 0;  // dw = 1 + sum of floor|N|
 }
 
 template <class T>
@@ -469,6 +477,7 @@ int mlp_alloc(Mlp& m, int nb, int W, int64_t bcap) {
   rc |= dalloc(&m.sdet, bcap * W);
   rc |= dalloc(&m.sexc, bcap * 4);
   rc |= dalloc(&m.sw, bcap);
+  rc |= dalloc(&m.sflat, bcap);
   rc |= dalloc(&m.sloss, bcap);
   rc |= dalloc(&m.dsum, 2);
   rc |= dalloc(&m.aff, static_cast<int64_t>(nb) * nb);
@@ -497,6 +506,7 @@ void mlp_free(Mlp& m) {
   dfree(m.sdet);
   dfree(m.sexc);
   dfree(m.sw);
+  dfree(m.sflat);
   dfree(m.sloss);
   dfree(m.dsum);
   dfree(m.aff);
@@ -566,7 +576,7 @@ int mlp_train_step(Mlp& m, const fqd::Sys& s, int64_t ns, double lr, bool apply,
   gemm(B, dout, H, m.a2, H, 0, m.p + m.o_w3, dout, 0, m.p + m.o_b3, m.z3, dout, st);
   hipLaunchKernelGGL(k_dsum, dim3(1), dim3(kBlock), 0, st, ns, m.sw, m.dsum);
   hipLaunchKernelGGL(k_mlp_loss_grad, dim3(blocks(ns)), dim3(kBlock), 0, st, s, ns, m.nb, dout, m.sdet, m.bcap,
-                     m.sexc, m.sw, m.dsum, m.z3, m.p + m.o_th, m.d3, m.dth, m.sloss);
+                     m.sexc, m.sw, m.sflat, m.dsum, m.z3, m.p + m.o_th, m.d3, m.dth, m.sloss);
   hipLaunchKernelGGL(k_dsum, dim3(1), dim3(kBlock), 0, st, ns, m.sloss, m.dsum + 1);
   const int nchunk = static_cast<int>(ceil_div(ns, kAffChunk));
   hipLaunchKernelGGL(k_aff_grad_part, dim3(static_cast<unsigned>(m.nb), static_cast<unsigned>(nchunk)),
@@ -603,7 +613,7 @@ int mlp_train_step(Mlp& m, const fqd::Sys& s, int64_t ns, double lr, bool apply,
 }
 
 int mlp_sample(Mlp& m, const fqd::Sys& s, const uint64_t* det, const int64_t* pop, int64_t cap, int64_t n,
-               int64_t ns, uint64_t seed, uint32_t step, hipStream_t st,
+               int64_t ns, int64_t min_att, uint64_t seed, uint32_t step, hipStream_t st,
                int (*scan)(void* ctx, const int64_t* in, int64_t* out, int64_t n), void* scan_ctx) {
   if (n < 1 || ns < 1 || ns > m.bcap) return FQ_ERR_ARG;
   if (m.scum_cap < n + 1) {
@@ -619,8 +629,8 @@ int mlp_sample(Mlp& m, const fqd::Sys& s, const uint64_t* det, const int64_t* po
   hipLaunchKernelGGL(k_det_weight, dim3(blocks(n)), dim3(kBlock), 0, st, n, cap, s.S, pop, shift, m.swts);
   int rc = scan(scan_ctx, m.swts, m.scum, n);
   if (rc != FQ_OK) return rc;
-  hipLaunchKernelGGL(k_hb_sample, dim3(blocks(ns)), dim3(kBlock), 0, st, s, det, cap, n, m.scum, ns, seed, step,
-                     m.sdet, m.bcap, m.sexc, m.sw);
+  hipLaunchKernelGGL(k_hb_sample, dim3(blocks(ns)), dim3(kBlock), 0, st, s, det, cap, n, m.scum, m.swts, min_att,
+                     ns, seed, step, m.sdet, m.bcap, m.sexc, m.sw, m.sflat);
   MLP_HIP(hipGetLastError());
   return FQ_OK;
 }

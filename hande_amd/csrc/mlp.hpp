@@ -48,6 +48,7 @@ struct Mlp {
   uint64_t* sdet = nullptr;  // [W][bcap]
   int32_t* sexc = nullptr;   // [bcap][4]: i, j, a, b (i<j, a<b)
   double* sw = nullptr;      // sample weights
+  int32_t* sflat = nullptr;  // 1: sample scored with flat weights (its determinant gets no network row)
   double* sloss = nullptr;   // per-sample weighted NLL
   double* dsum = nullptr;    // [2]: sum of weights, loss
   int64_t* swts = nullptr;   // determinant sampling weights
@@ -70,9 +71,11 @@ int mlp_infer(const Mlp& m, const fqd::Sys& s, const uint64_t* det, int64_t cap,
 // leaves sum(w) and the weighted loss in dsum.  apply: one Adam step (lr).
 int mlp_train_step(Mlp& m, const fqd::Sys& s, int64_t ns, double lr, bool apply, hipStream_t st);
 // ns heat-bath samples from the main list (determinant chosen with
-// probability proportional to 1 + floor(|pop|/real_factor), summed over spaces).
+// probability proportional to 1 + floor(|pop|/real_factor), summed over
+// spaces); a sample whose determinant has 1 + sum floor|N| < min_att is flagged
+// flat (scored with the weights production gives determinants without a row).
 int mlp_sample(Mlp& m, const fqd::Sys& s, const uint64_t* det, const int64_t* pop, int64_t cap, int64_t n,
-               int64_t ns, uint64_t seed, uint32_t step, hipStream_t st,
+               int64_t ns, int64_t min_att, uint64_t seed, uint32_t step, hipStream_t st,
                int (*scan)(void* ctx, const int64_t* in, int64_t* out, int64_t n), void* scan_ctx);
 
 }  // namespace fqm

@@ -236,12 +236,16 @@ int fq_mlp_forward(fq_engine *e, const uint64_t *dets, int64_t n, float *out);
 int fq_mlp_affinity(fq_engine *e, float *out, int64_t n);
 /* Training loss sum_s w_s (-log q(ex_s | D_s)) / sum_s w_s of n samples
  * (exc: n x 4 = i, j, a, b with i<j, a<b) and its gradient (nparams floats;
- * grad may be NULL); fp32 forward and backward. */
-int fq_mlp_loss_grad(fq_engine *e, const uint64_t *dets, const int32_t *exc, const double *w, int64_t n,
-                     double *loss, float *grad);
+ * grad may be NULL); fp32 forward and backward.  flat (n flags, may be NULL):
+ * a flagged sample is scored with the weights of a determinant the network
+ * does not run on (flat hole and particle weights, the affinity). */
+int fq_mlp_loss_grad(fq_engine *e, const uint64_t *dets, const int32_t *exc, const double *w, const int32_t *flat,
+                     int64_t n, double *loss, float *grad);
 /* steps Adam steps (learning rate lr), each on batch heat-bath samples drawn on
  * the device from the current walker list (determinant chosen with weight
- * 1 + floor|N_i|); losses (may be NULL) receives each step's loss. */
+ * 1 + floor|N_i|); a sample whose determinant has 1 + floor|N_i| below
+ * proposal_min_attempts is scored flat, as production draws for it.  losses
+ * (may be NULL) receives each step's loss. */
 int fq_mlp_train(fq_engine *e, int64_t batch, int32_t steps, double lr, uint64_t seed, double *losses);
 
 /* ---- Excitation probe (molecular systems) --------------------------------------

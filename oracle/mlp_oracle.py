@@ -38,6 +38,8 @@ and the mixture with fraction ``mix`` of the uniform fallback is
 with C_x the unoccupied orbitals the partner of x may come from (x excluded).
 
 Training loss for samples (D, ex, w): L = sum_s w_s (-log q_mlp(ex_s|D_s)) / sum_s w_s.
+A sample flagged flat (its determinant gets no network row in production) is
+scored with e = f = 1 (logits 0): only the affinity receives its gradient.
 """
 import numpy as np
 
@@ -313,9 +315,10 @@ def _nll_and_grad(z3, th, occ_list, orb_sym, nb, ex):
     return -np.log(q), grad, gth_i, gth_j
 
 
-def loss_and_grad(params, occ, eps, orb_sym, samples, weights):
+def loss_and_grad(params, occ, eps, orb_sym, samples, weights, flat=None):
     """Weighted mean NLL of the samples' doubles under q_mlp, and its gradient
-    with respect to the flat parameter vector (float64 forward and backward)."""
+    with respect to the flat parameter vector (float64 forward and backward);
+    flat: per-sample flags (scored with zero logits, no network gradient)."""
     nb = occ.shape[1]
     W1, b1, W2, b2, W3, b3 = unpack(params, nb)
     th = theta(params, nb)
@@ -334,9 +337,10 @@ def loss_and_grad(params, occ, eps, orb_sym, samples, weights):
         if w[s] == 0.0:
             continue
         ol = [k + 1 for k in range(nb) if occ[s, k] > 0.5]
-        nll, g, gi, gj = _nll_and_grad(z3[s], th, ol, orb_sym, nb, samples[s])
+        is_flat = flat is not None and flat[s]
+        nll, g, gi, gj = _nll_and_grad(np.zeros_like(z3[s]) if is_flat else z3[s], th, ol, orb_sym, nb, samples[s])
         loss += wt[s] * nll
-        dz3[s] = wt[s] * g
+        dz3[s] = 0.0 if is_flat else wt[s] * g
         i, j = samples[s][0], samples[s][1]
         dth[i - 1] += wt[s] * gi
         dth[j - 1] += wt[s] * gj
@@ -351,7 +355,7 @@ def loss_and_grad(params, occ, eps, orb_sym, samples, weights):
     return loss, pack(dW1, db1, dW2, db2, dW3, db3, dth)
 
 
-def loss_only(params, occ, eps, orb_sym, samples, weights):
+def loss_only(params, occ, eps, orb_sym, samples, weights, flat=None):
     nb = occ.shape[1]
     z3 = forward(params, occ, eps)
     th = theta(params, nb)
@@ -362,5 +366,6 @@ def loss_only(params, occ, eps, orb_sym, samples, weights):
         if w[s] == 0.0:
             continue
         ol = [k + 1 for k in range(nb) if occ[s, k] > 0.5]
-        total += wt[s] * _nll_and_grad(z3[s], th, ol, orb_sym, nb, samples[s])[0]
+        zs = np.zeros_like(z3[s]) if (flat is not None and flat[s]) else z3[s]
+        total += wt[s] * _nll_and_grad(zs, th, ol, orb_sym, nb, samples[s])[0]
     return total

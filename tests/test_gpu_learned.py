@@ -156,13 +156,15 @@ def test_mlp_loss_and_gradient_match_oracle(tmp_path):
         _set_params(lib, eng, params)
         rng = np.random.default_rng(9)
         dets, occs, exc, w = _random_samples(nb, 4, orb_sym, 300, rng)
+        flat = (np.arange(len(w)) % 5 == 2).astype(np.int32)  # some samples scored with flat weights
         loss = C.c_double()
         grad = np.zeros(len(params), dtype=np.float32)
         N.check(lib.fq_mlp_loss_grad(eng, np.ascontiguousarray(dets).ctypes.data_as(P(C.c_uint64)),
-                                     exc.ctypes.data_as(P(C.c_int32)), w.ctypes.data_as(P(C.c_double)), len(w),
+                                     exc.ctypes.data_as(P(C.c_int32)), w.ctypes.data_as(P(C.c_double)),
+                                     flat.ctypes.data_as(P(C.c_int32)), len(w),
                                      C.byref(loss), grad.ctypes.data_as(P(C.c_float))), "fq_mlp_loss_grad")
         ref_loss, ref_grad = M.loss_and_grad(params.astype(np.float64), M.occupations(dets, nb), eps, orb_sym,
-                                             [tuple(int(x) for x in e) for e in exc], w)
+                                             [tuple(int(x) for x in e) for e in exc], w, flat.astype(bool))
         print(f"  loss {loss.value:.8f} oracle {ref_loss:.8f}", flush=True)
         assert abs(loss.value - ref_loss) < 1e-5 * abs(ref_loss)
         _d_in, _d_out, off, _n = M.sizes(nb)

@@ -50,13 +50,14 @@ def _random_case(nb, nel, nsamp, seed, nsym=1):
     return occ, eps, orb_sym, samples, rng.uniform(0.1, 2.0, nsamp)
 
 
-@pytest.mark.parametrize("nsym", [1, 2])
-def test_loss_gradient_matches_central_differences(nsym):
+@pytest.mark.parametrize("nsym,with_flat", [(1, False), (2, False), (2, True)])
+def test_loss_gradient_matches_central_differences(nsym, with_flat):
     nb, nel = 12, 4
     occ, eps, orb_sym, samples, w = _random_case(nb, nel, 24, 3 + nsym, nsym)
+    flat = (np.arange(len(w)) % 3 == 0) if with_flat else None
     params = M.init_params(nb, 11).astype(np.float64)
-    loss, grad = M.loss_and_grad(params, occ, eps, orb_sym, samples, w)
-    assert abs(loss - M.loss_only(params, occ, eps, orb_sym, samples, w)) < 1e-12
+    loss, grad = M.loss_and_grad(params, occ, eps, orb_sym, samples, w, flat)
+    assert abs(loss - M.loss_only(params, occ, eps, orb_sym, samples, w, flat)) < 1e-12
     rng = np.random.default_rng(1)
     _d_in, _d_out, off, n = M.sizes(nb)
     # probe entries of every block
@@ -67,7 +68,8 @@ def test_loss_gradient_matches_central_differences(nsym):
         p1[k] += h
         p2 = params.copy()
         p2[k] -= h
-        fd = (M.loss_only(p1, occ, eps, orb_sym, samples, w) - M.loss_only(p2, occ, eps, orb_sym, samples, w)) / (2 * h)
+        fd = (M.loss_only(p1, occ, eps, orb_sym, samples, w, flat)
+              - M.loss_only(p2, occ, eps, orb_sym, samples, w, flat)) / (2 * h)
         assert abs(fd - grad[k]) < 1e-7 + 1e-5 * abs(fd), (k, fd, grad[k])
 
 
