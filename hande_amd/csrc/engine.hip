@@ -99,8 +99,8 @@ struct Stats {
 };
 
 // ---------------------------------------------------------------------------
-// Prefix sums.  Up to kSmallScanMax entries: one 1024-thread block walks the
-// array in chunks with a running carry (one launch).  Up to kPrefixTiles
+// Prefix sums.  Up to kSmallScanMax entries: one 1024-thread block (one
+// launch).  Up to kPrefixTiles
 // tiles: per-tile sums, then each tile adds the sums of the tiles before it
 // itself and scans its entries (two launches).  More: the tile sums are
 // scanned by the one-block scan in between (three launches).  out[n] = total.
@@ -108,7 +108,7 @@ struct Stats {
 constexpr int kScanItems = 16;
 constexpr int64_t kScanTile = static_cast<int64_t>(kBlock) * kScanItems;
 constexpr int kSmallBlock = 1024, kSmallItems = 8;
-constexpr int64_t kSmallScanMax = 65536;
+constexpr int64_t kSmallScanMax = kSmallBlock * kSmallItems;  // one chunk: a single block is latency-bound
 constexpr int64_t kPrefixTiles = 2048;
 
 // Inclusive scan of x over the 64 lanes of a wavefront.
@@ -200,8 +200,9 @@ __global__ __launch_bounds__(kBlock) void k_tile_scan(const int64_t* __restrict_
   }
 }
 
-// Exclusive scan of n <= kSmallScanMax entries on one block of kSmallBlock
-// threads: chunks of kSmallBlock * kSmallItems with a running carry.
+// Exclusive scan of n entries on one block of kSmallBlock threads: chunks of
+// kSmallBlock * kSmallItems with a running carry (one chunk for the small
+// scans; many for the tile sums of a very long array).
 __global__ __launch_bounds__(kSmallBlock) void k_scan_small(const int64_t* __restrict__ in, int64_t n,
                                                             int64_t* __restrict__ out) {
   __shared__ int64_t wsh[kSmallBlock / 64];
