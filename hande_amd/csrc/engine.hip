@@ -1264,35 +1264,47 @@ __global__ __launch_bounds__(kBlock) void k_run_heads(const uint64_t* __restrict
                                                       int64_t* __restrict__ gpop, int64_t* __restrict__ gflag,
                                                       int64_t* __restrict__ run_stage,
                                                       int64_t* __restrict__ tile_cnt) {
-  __shared__ int32_t wcount[kBlock / 64];
+  constexpr int R = kRunTile / kBlock;  // rounds of a tile
+  __shared__ int32_t wcount[R][kBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t ntiles = (ns + kRunTile - 1) / kRunTile;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    int32_t run = 0;
-    for (int k = 0; k < kRunTile / kBlock; ++k) {
+    // every round's loads and gathers first (independent, so they overlap),
+    // then the ballot ranks of all rounds after one barrier
+    bool head[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
       const int64_t i = tile * kRunTile + k * kBlock + threadIdx.x;
-      bool head = false;
+      head[k] = false;
       if (i < ns) {
-        head = i == 0;
+        bool h = i == 0;
 #pragma unroll
         for (int w = 0; w < FQ_MAX_WORDS; ++w)
-          if (w < kw && i > 0) head = head || keys[w * scap + i] != keys[w * scap + i - 1];
+          if (w < kw && i > 0) h = h || keys[w * scap + i] != keys[w * scap + i - 1];
+        head[k] = h;
         const int64_t* rec = srec + static_cast<int64_t>(perm[i]) * rec_stride(S);
         for (int sp = 0; sp < S; ++sp) gpop[sp * scap + i] = rec[sp];
         gflag[i] = rec[S];
       }
-      const unsigned long long bal = __ballot(head);
-      if (lane == 0) wcount[wave] = __popcll(bal);
-      __syncthreads();
-      int32_t pos = run + __popcll(bal & ((1ull << lane) - 1));
+    }
+    unsigned long long bal[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      bal[k] = __ballot(head[k]);
+      if (lane == 0) wcount[k][wave] = __popcll(bal[k]);
+    }
+    __syncthreads();
+    int32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      int32_t pos = run + __popcll(bal[k] & ((1ull << lane) - 1));
       int32_t round_total = 0;
       for (int w = 0; w < kBlock / 64; ++w) {
-        if (w < wave) pos += wcount[w];
-        round_total += wcount[w];
+        if (w < wave) pos += wcount[k][w];
+        round_total += wcount[k][w];
       }
-      if (head) run_stage[tile * kRunTile + pos] = i;
+      if (head[k]) run_stage[tile * kRunTile + pos] = tile * kRunTile + k * kBlock + threadIdx.x;
       run += round_total;
-      __syncthreads();
     }
     if (threadIdx.x == 0) tile_cnt[tile] = run;
     __syncthreads();
