@@ -1337,6 +1337,11 @@ __global__ void k_reduce_runs(Sys s, const int64_t* __restrict__ pnr, const int6
   const int64_t nruns = *pnr;
   if (r >= nruns) return;
   const int64_t a = run_start[r], b = run_start[r + 1];
+  // the run's determinant (a gather through the permutation) is loaded first,
+  // so its latency overlaps the run's population sum
+  const uint32_t j = perm[a];
+  uint64_t d[FQ_MAX_WORDS] = {0, 0};
+  FQ_FOR_WORDS(w, s.W) d[w] = sdet[w * scap + j];
   int64_t tot[FQ_MAX_SPACES] = {0, 0}, init[FQ_MAX_SPACES] = {0, 0};
   int events[FQ_MAX_SPACES] = {0, 0};
   const int64_t f0 = gflag[a];
@@ -1377,8 +1382,7 @@ __global__ void k_reduce_runs(Sys s, const int64_t* __restrict__ pnr, const int6
     }
     rpop[sp * scap + r] = tot[sp];
   }
-  const uint32_t j = perm[a];
-  FQ_FOR_WORDS(w, s.W) rdet[w * scap + r] = sdet[w * scap + j];
+  FQ_FOR_WORDS(w, s.W) rdet[w * scap + r] = d[w];
   rflag[r] = fl;
   rstate[r] = any ? 1 : 0;
 }
