@@ -1217,6 +1217,11 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
     const int nvirt = nb - nel;
     int i = 0, j = 0, a = 0, b = 0;
     bool ok = true;
+    // normalisers of the network component: Z1 of holes i, j; Z2 of holes i, j
+    // over b's partner class given a (zbi, zbj); known after a network draw
+    double z1i = 0.0, z1j = 0.0, zbi = 0.0, zbj = 0.0;
+    int nbc = 0;
+    bool have_z = false;
     if (rng.next() < mix) {
       // uniform component: unordered hole pair (choose_ij_mol's index map), a, partner
       const int ind = static_cast<int>(rng.next() * nel * (nel - 1) / 2) + 1;
@@ -1226,7 +1231,6 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
       j = select_orb(f, s.W, j_ind - 1);
       a = select_orb(un, s.W, static_cast<int>(rng.next() * nvirt));
       const PartnerClass cb = partner_class(s, i, j, a);
-      int nbc = 0;
       double zi = 0.0, zj = 0.0;
       partner_stats(s, f, nullptr, nullptr, nullptr, cb, a, nbc, zi, zj);
       if (nbc == 0)
@@ -1263,14 +1267,15 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
       if (!j) j = lastj;
       const float* ai = aff + (i - 1) * nb;
       const float* aj = aff + (j - 1) * nb;
-      double z1 = 0.0;
       FQ_FOR_WORDS(w, s.W) {
         for (uint64_t y = un[w]; y; y &= y - 1) {
           const int o = (w << 6) + __builtin_ctzll(y);
-          z1 = z1 + static_cast<double>(fw[o]) * static_cast<double>(ai[o]);
+          const double v = static_cast<double>(fw[o]);
+          z1i = z1i + v * static_cast<double>(ai[o]);
+          z1j = z1j + v * static_cast<double>(aj[o]);
         }
       }
-      x = rng.next() * z1;
+      x = rng.next() * z1i;
       cum = 0.0;
       int lasta = 0;
       FQ_FOR_WORDS(w, s.W) if (!a)
@@ -1285,9 +1290,8 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
         }
       if (!a) a = lasta;
       const PartnerClass cb = partner_class(s, i, j, a);
-      int nbc = 0;
-      double zbi = 0.0, zbj = 0.0;
       partner_stats(s, f, fw, ai, aj, cb, a, nbc, zbi, zbj);
+      have_z = true;
       if (nbc == 0)
         ok = false;
       else
@@ -1296,13 +1300,10 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
     if (ok) {
       const float* ai = aff + (i - 1) * nb;
       const float* aj = aff + (j - 1) * nb;
-      int nbc = 0, nac = 0;
-      double zbi = 0.0, zbj = 0.0, zai = 0.0, zaj = 0.0;  // Z2 of holes i, j over a's (b's) partner class
-      partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, a), a, nbc, zbi, zbj);
-      partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, b), b, nac, zai, zaj);
-      const double q_unif = 2.0 / static_cast<double>(nel * (nel - 1)) / nvirt * (1.0 / nbc + 1.0 / nac);
-      {
-        double z1i = 0.0, z1j = 0.0;
+      int nac = 0;
+      double zai = 0.0, zaj = 0.0;  // Z2 of holes i, j over a's partner class given b
+      if (!have_z) {  // a uniform draw: the network normalisers of (i, j, a)
+        partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, a), a, nbc, zbi, zbj);
         FQ_FOR_WORDS(w, s.W) {
           for (uint64_t y = un[w]; y; y &= y - 1) {
             const int o = (w << 6) + __builtin_ctzll(y);
@@ -1311,6 +1312,10 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
             z1j = z1j + v * static_cast<double>(aj[o]);
           }
         }
+      }
+      partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, b), b, nac, zai, zaj);
+      const double q_unif = 2.0 / static_cast<double>(nel * (nel - 1)) / nvirt * (1.0 / nbc + 1.0 / nac);
+      {
         const double ei = ew[i - 1], ej = ew[j - 1], fa = fw[a - 1], fb = fw[b - 1];
         const double aia = ai[a - 1], aib = ai[b - 1], aja = aj[a - 1], ajb = aj[b - 1];
         const double pij = ei / zh * ej / (zh - ei), pji = ej / zh * ei / (zh - ej);
