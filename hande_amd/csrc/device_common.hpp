@@ -71,6 +71,12 @@ struct Sys {
   const float* mlp_tab;
   const float* mlp_aff;
   const float* mlp_flat;  // 2*nbasis ones: the weights of determinants without a row
+  // affinity row sums: mlp_rs[h-1] over all orbitals, mlp_crs[(h-1)*ncls + c]
+  // over spin/irrep class c = (ims-1)*nsym_pg + sym (ncls = 2*nsym_pg), whose
+  // members are the bits of mlp_cmask[c*FQ_MAX_WORDS ..]
+  const double* mlp_rs;
+  const double* mlp_crs;
+  const uint64_t* mlp_cmask;
   int64_t mlp_ld;
   const int64_t* mlp_on;   // per determinant: 1 if the network ran on it this cycle
   const int64_t* mlp_row;  // its row of mlp_tab (exclusive prefix of mlp_on)
@@ -1188,6 +1194,35 @@ __device__ inline int partner_pick(const Sys& s, const uint64_t* f, const float*
 
 // aff_in / flat_in: copies of s.mlp_aff / s.mlp_flat (in LDS in the spawning
 // kernel); nullptr reads the device tables.
+// Flat weights (e = f = 1, determinants without a network row): normalisers
+// as the affinity row sums less their occupied terms, O(nel) instead of
+// O(nvirt).
+__device__ inline double occ_aff_sum(const Sys& s, const uint64_t* f, const uint64_t* mask, const float* arow) {
+  double z = 0.0;
+  FQ_FOR_WORDS(w, s.W) {
+    for (uint64_t y = mask ? (f[w] & mask[w]) : f[w]; y; y &= y - 1)
+      z = z + static_cast<double>(arow[(w << 6) + __builtin_ctzll(y)]);
+  }
+  return z;
+}
+// partner_stats with flat weights: count of class c's unoccupied members
+// other than x and the sums of A[hi][.] and A[hj][.] over them.
+__device__ inline void flat_partner_stats(const Sys& s, const uint64_t* f, int hi, int hj, const float* ai,
+                                          const float* aj, const PartnerClass& c, int x, int& count, double& zi,
+                                          double& zj) {
+  count = 0;
+  zi = zj = 0.0;
+  if (!c.ok) return;
+  const int ci = (c.ims - 1) * s.nsym_pg + c.sym, ncls = 2 * s.nsym_pg;
+  const uint64_t* m = s.mlp_cmask + ci * FQ_MAX_WORDS;
+  int occ_in = 0;
+  FQ_FOR_WORDS(w, s.W) occ_in += __popcll(m[w] & f[w]);
+  const bool x_in = btest(m, x);
+  count = nbss(s, c.ims, c.sym) - occ_in - (x_in ? 1 : 0);
+  zi = s.mlp_crs[(hi - 1) * ncls + ci] - occ_aff_sum(s, f, m, ai) - (x_in ? static_cast<double>(ai[x - 1]) : 0.0);
+  zj = s.mlp_crs[(hj - 1) * ncls + ci] - occ_aff_sum(s, f, m, aj) - (x_in ? static_cast<double>(aj[x - 1]) : 0.0);
+}
+
 template <class Rng>
 __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, const float* wrow, Excit& conn,
                                    double& h, double& pgen, const float* aff_in = nullptr,
@@ -1204,6 +1239,7 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
   if (rng.next() < s.psingle) {
     allowed = single_mol(s, rng, f, u, conn, h, pgen);
   } else {
+    const bool flat = wrow == nullptr;  // e = f = 1
     const float* row = wrow ? wrow : (flat_in ? flat_in : s.mlp_flat);
     const double mix = s.mix;
     const float* ew = row;
@@ -1267,12 +1303,17 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
       if (!j) j = lastj;
       const float* ai = aff + (i - 1) * nb;
       const float* aj = aff + (j - 1) * nb;
-      FQ_FOR_WORDS(w, s.W) {
-        for (uint64_t y = un[w]; y; y &= y - 1) {
-          const int o = (w << 6) + __builtin_ctzll(y);
-          const double v = static_cast<double>(fw[o]);
-          z1i = z1i + v * static_cast<double>(ai[o]);
-          z1j = z1j + v * static_cast<double>(aj[o]);
+      if (flat) {
+        z1i = s.mlp_rs[i - 1] - occ_aff_sum(s, f, nullptr, ai);
+        z1j = s.mlp_rs[j - 1] - occ_aff_sum(s, f, nullptr, aj);
+      } else {
+        FQ_FOR_WORDS(w, s.W) {
+          for (uint64_t y = un[w]; y; y &= y - 1) {
+            const int o = (w << 6) + __builtin_ctzll(y);
+            const double v = static_cast<double>(fw[o]);
+            z1i = z1i + v * static_cast<double>(ai[o]);
+            z1j = z1j + v * static_cast<double>(aj[o]);
+          }
         }
       }
       x = rng.next() * z1i;
@@ -1290,7 +1331,10 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
         }
       if (!a) a = lasta;
       const PartnerClass cb = partner_class(s, i, j, a);
-      partner_stats(s, f, fw, ai, aj, cb, a, nbc, zbi, zbj);
+      if (flat)
+        flat_partner_stats(s, f, i, j, ai, aj, cb, a, nbc, zbi, zbj);
+      else
+        partner_stats(s, f, fw, ai, aj, cb, a, nbc, zbi, zbj);
       have_z = true;
       if (nbc == 0)
         ok = false;
@@ -1302,7 +1346,11 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
       const float* aj = aff + (j - 1) * nb;
       int nac = 0;
       double zai = 0.0, zaj = 0.0;  // Z2 of holes i, j over a's partner class given b
-      if (!have_z) {  // a uniform draw: the network normalisers of (i, j, a)
+      if (!have_z && flat) {  // a uniform draw: the network normalisers of (i, j, a)
+        flat_partner_stats(s, f, i, j, ai, aj, partner_class(s, i, j, a), a, nbc, zbi, zbj);
+        z1i = s.mlp_rs[i - 1] - occ_aff_sum(s, f, nullptr, ai);
+        z1j = s.mlp_rs[j - 1] - occ_aff_sum(s, f, nullptr, aj);
+      } else if (!have_z) {
         partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, a), a, nbc, zbi, zbj);
         FQ_FOR_WORDS(w, s.W) {
           for (uint64_t y = un[w]; y; y &= y - 1) {
@@ -1313,7 +1361,10 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
           }
         }
       }
-      partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, b), b, nac, zai, zaj);
+      if (flat)
+        flat_partner_stats(s, f, i, j, ai, aj, partner_class(s, i, j, b), b, nac, zai, zaj);
+      else
+        partner_stats(s, f, fw, ai, aj, partner_class(s, i, j, b), b, nac, zai, zaj);
       const double q_unif = 2.0 / static_cast<double>(nel * (nel - 1)) / nvirt * (1.0 / nbc + 1.0 / nac);
       {
         const double ei = ew[i - 1], ej = ew[j - 1], fa = fw[a - 1], fb = fw[b - 1];

@@ -2185,6 +2185,8 @@ int setup_learned(fq_engine* e) {
   Sys& s = e->sys;
   int rc = fqm::mlp_alloc(e->mlp, s.nbasis, s.W, kTrainBatchMax);
   if (rc != FQ_OK) return rc;
+  rc = fqm::mlp_set_classes(e->mlp, s.nbss, s.ssbf, s.nsym_pg, s.max_sym_spin);
+  if (rc != FQ_OK) return rc;
   rc = fqm::mlp_init(e->mlp, e->par.philox_seed, e->stream);
   if (rc != FQ_OK) return rc;
   e->mlp_tab_rows = std::min<int64_t>(e->cap, int64_t{1} << 16);
@@ -2193,6 +2195,9 @@ int setup_learned(fq_engine* e) {
     return FQ_ERR_NOMEM;
   s.mlp_tab = e->mlp_tab;
   s.mlp_aff = e->mlp.aff;
+  s.mlp_rs = e->mlp.rs;
+  s.mlp_crs = e->mlp.crs;
+  s.mlp_cmask = e->mlp.cmask;
   {
     const std::vector<float> ones(2 * static_cast<size_t>(s.nbasis), 1.0f);
     if (dalloc(&e->mlp_flat, 2 * static_cast<int64_t>(s.nbasis)) != FQ_OK) return FQ_ERR_NOMEM;

@@ -339,6 +339,35 @@ __global__ void k_aff_grad_sum(int nb, int nchunk, const float* __restrict__ par
   g[k] = acc;
 }
 
+// Row sums of the stored affinity (one thread per hole h, orbital order) and
+// per spin/irrep class (members in the class table's order); thread 0 also
+// writes the class bit masks.
+__global__ void k_aff_sums(int nb, const float* __restrict__ aff, const int* __restrict__ nbss,
+                           const int* __restrict__ ssbf, int nsym_pg, int max_sym_spin, double* __restrict__ rs,
+                           double* __restrict__ crs, uint64_t* __restrict__ cmask) {
+  const int h = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int ncls = 2 * nsym_pg;
+  if (h == 0)
+    for (int c = 0; c < ncls; ++c) {
+      uint64_t mw[FQ_MAX_WORDS] = {0, 0};
+      for (int k = 0; k < nbss[c]; ++k) {
+        const int o = ssbf[c * max_sym_spin + k] - 1;
+        mw[o >> 6] |= uint64_t{1} << (o & 63);
+      }
+      for (int w = 0; w < FQ_MAX_WORDS; ++w) cmask[c * FQ_MAX_WORDS + w] = mw[w];
+    }
+  if (h >= nb) return;
+  const float* row = aff + static_cast<int64_t>(h) * nb;
+  double z = 0.0;
+  for (int p = 0; p < nb; ++p) z = z + static_cast<double>(row[p]);
+  rs[h] = z;
+  for (int c = 0; c < ncls; ++c) {
+    double zc = 0.0;
+    for (int k = 0; k < nbss[c]; ++k) zc = zc + static_cast<double>(row[ssbf[c * max_sym_spin + k] - 1]);
+    crs[h * ncls + c] = zc;
+  }
+}
+
 // The stored affinity table: exp(clamp(theta, -60, 60)) rounded to float.
 __global__ void k_aff_table(int64_t n, const float* __restrict__ th, float* __restrict__ aff) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -510,6 +539,9 @@ void mlp_free(Mlp& m) {
   dfree(m.sloss);
   dfree(m.dsum);
   dfree(m.aff);
+  dfree(m.rs);
+  dfree(m.crs);
+  dfree(m.cmask);
   dfree(m.dth);
   dfree(m.aff_part);
   dfree(m.scum);
@@ -524,8 +556,22 @@ int mlp_refresh(Mlp& m, hipStream_t st) {
                      m.o_w2, m.o_w3, m.w1t, m.w2t, m.w3t);
   const int64_t na = static_cast<int64_t>(m.nb) * m.nb;
   hipLaunchKernelGGL(k_aff_table, dim3(blocks(na)), dim3(kBlock), 0, st, na, m.p + m.o_th, m.aff);
+  if (m.nbss)
+    hipLaunchKernelGGL(k_aff_sums, dim3(blocks(m.nb)), dim3(kBlock), 0, st, m.nb, m.aff, m.nbss, m.ssbf, m.nsym_pg,
+                       m.max_sym_spin, m.rs, m.crs, m.cmask);
   MLP_HIP(hipGetLastError());
   return FQ_OK;
+}
+
+int mlp_set_classes(Mlp& m, const int* nbss, const int* ssbf, int nsym_pg, int max_sym_spin) {
+  m.nbss = nbss;
+  m.ssbf = ssbf;
+  m.nsym_pg = nsym_pg;
+  m.max_sym_spin = max_sym_spin;
+  int rc = dalloc(&m.rs, m.nb);
+  rc |= dalloc(&m.crs, static_cast<int64_t>(m.nb) * 2 * nsym_pg);
+  rc |= dalloc(&m.cmask, static_cast<int64_t>(2) * nsym_pg * FQ_MAX_WORDS);
+  return rc == FQ_OK ? FQ_OK : FQ_ERR_NOMEM;
 }
 
 int mlp_set(Mlp& m, const float* host, hipStream_t st) {

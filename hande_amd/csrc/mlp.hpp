@@ -38,6 +38,14 @@ struct Mlp {
   uint16_t* w2t = nullptr;  // [256][256]
   uint16_t* w3t = nullptr;  // [n3][256]
   float* aff = nullptr;     // [nb][nb] exp(clamp(theta)), read by the generator
+  // affinity row sums for flat-weight draws (refreshed with aff): rs[h] over
+  // all orbitals, crs[h][c] over spin/irrep class c, class bit masks cmask[c][w]
+  double* rs = nullptr;
+  double* crs = nullptr;
+  uint64_t* cmask = nullptr;
+  const int* nbss = nullptr;  // the system's class tables (device), set before mlp_init
+  const int* ssbf = nullptr;
+  int nsym_pg = 1, max_sym_spin = 0;
   // training batch storage (capacity bcap samples)
   int64_t bcap = 0;
   float *x = nullptr, *z1 = nullptr, *a1 = nullptr, *z2 = nullptr, *a2 = nullptr, *z3 = nullptr;
@@ -57,6 +65,8 @@ struct Mlp {
 };
 
 int mlp_alloc(Mlp& m, int nb, int W, int64_t bcap);
+// The system's spin/irrep classes (device tables), for the flat-draw sums.
+int mlp_set_classes(Mlp& m, const int* nbss, const int* ssbf, int nsym_pg, int max_sym_spin);
 void mlp_free(Mlp& m);
 // Deterministic initial parameters from seed (host RNG), uploaded; Adam reset.
 int mlp_init(Mlp& m, uint64_t seed, hipStream_t st);
