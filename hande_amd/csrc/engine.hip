@@ -302,8 +302,6 @@ struct fq_engine {
   uint64_t* mmask = nullptr;    // cap/64+2: surviving main entries, one bit each
   int64_t* mcnt = nullptr;      // survivors per 64-entry word
   int64_t* moff = nullptr;      // their exclusive prefix
-  int64_t* ins = nullptr;       // cap+2
-  int64_t* ins_scan = nullptr;  // cap+2
   int64_t* mdraw = nullptr;     // cap+1 (replay rounding draws)
   int64_t* mdraw_off = nullptr;
   double* part_d = nullptr;     // 4 * (cap/256+1) block partials
@@ -339,6 +337,7 @@ struct fq_engine {
   int64_t* rpos = nullptr;       // scap
   int64_t* rstate = nullptr;     // scap: 0 dead, 1 hit, 2 new candidate
   int64_t* nlist = nullptr;      // scap: surviving new runs in order
+  int64_t* nrpos = nullptr;      // scap: their insertion positions (non-decreasing)
   int64_t* mwin = nullptr;       // main-list bounds of each merge block's first and last run
   uint64_t* nmask = nullptr;     // scap/64+2: surviving new runs, one bit each
   int64_t* ncnt = nullptr;       // survivors per 64-run word
@@ -1625,14 +1624,6 @@ __global__ void k_zero(int64_t* __restrict__ p, int64_t n) {
   if (i < n) p[i] = 0;
 }
 
-__global__ void k_ins_count(const int64_t* __restrict__ pnr, const uint64_t* __restrict__ nmask,
-                            const int64_t* __restrict__ rpos, int64_t* __restrict__ ins) {
-  const int64_t nruns = *pnr;
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= nruns || !bit_get(nmask, r)) return;
-  atomicAdd(reinterpret_cast<unsigned long long*>(&ins[rpos[r]]), 1ull);
-}
-
 // Decide the new list length; insertion is skipped when the main list would
 // overflow (insert_new_walkers' psip_list%error path).
 // pnr: the number of spawned runs (nullptr: none); also recorded in the stats.
@@ -1650,26 +1641,48 @@ __global__ void k_finalize_counts(int64_t n, const int64_t* __restrict__ pnr, co
   }
 }
 
+// Surviving main entries move up by their rank among survivors plus the number
+// of surviving new runs inserted at or before them: nrpos lists those runs'
+// insertion positions in rank order, non-decreasing because the runs are
+// sorted, so the count is an upper bound search.
 __global__ void k_scatter_main(Sys s, int64_t n, int64_t cap, const uint64_t* __restrict__ det,
                                const int64_t* __restrict__ pop_new, const double* __restrict__ kii,
                                const uint64_t* __restrict__ mmask, const int64_t* __restrict__ moff,
-                               const int64_t* __restrict__ ins_scan, const Stats* __restrict__ st,
+                               const int64_t* __restrict__ nrpos, const Stats* __restrict__ st,
                                uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
                                double* __restrict__ kii_out) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n || !bit_get(mmask, i)) return;
-  const int64_t o = bit_rank(moff, mmask, i) + (st->main_overflow ? 0 : ins_scan[i + 1]);
+  int64_t shift = 0;
+  if (!st->main_overflow) {
+    int64_t lo = 0, hi = st->new_n - bit_rank(moff, mmask, n);  // surviving new runs
+    while (lo < hi) {  // first k with nrpos[k] > i
+      const int64_t mid = (lo + hi) >> 1;
+      if (nrpos[mid] <= i)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    shift = lo;
+  }
+  const int64_t o = bit_rank(moff, mmask, i) + shift;
   for (int w = 0; w < s.W; ++w) det_out[w * cap + o] = det[w * cap + i];
   for (int sp = 0; sp < s.S; ++sp) pop_out[sp * cap + o] = pop_new[sp * cap + i];
   kii_out[o] = kii[i];
 }
 
 // Surviving new runs in order: list[rank among survivors] = run index.
+// The surviving new runs in rank order, with their insertion positions.
 __global__ void k_new_list(const int64_t* __restrict__ pnr, const uint64_t* __restrict__ nmask,
-                           const int64_t* __restrict__ noff, int64_t* __restrict__ list) {
+                           const int64_t* __restrict__ noff, const int64_t* __restrict__ rpos,
+                           int64_t* __restrict__ list, int64_t* __restrict__ nrpos) {
   const int64_t nruns = *pnr;
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r < nruns && bit_get(nmask, r)) list[bit_rank(noff, nmask, r)] = r;
+  if (r < nruns && bit_get(nmask, r)) {
+    const int64_t k = bit_rank(noff, nmask, r);
+    list[k] = r;
+    nrpos[k] = rpos[r];
+  }
 }
 
 // Insert the surviving new runs, one thread each (list order = rank order),
@@ -1934,19 +1947,16 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
   } else {
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, e->noff, 1);
   }
-  hipLaunchKernelGGL(k_zero, dim3(grid_for(n + 1)), dim3(kBlock), 0, st, e->ins, n + 1);
   if (nruns > 0)
-    hipLaunchKernelGGL(k_ins_count, dim3(grid_for(nruns)), dim3(kBlock), 0, st, pnr, e->nmask, e->rpos, e->ins);
-  rc = scan_exclusive(e, e->ins, e->ins_scan, n + 1);
-  if (rc != FQ_OK) return rc;
+    hipLaunchKernelGGL(k_new_list, dim3(grid_for(nruns)), dim3(kBlock), 0, st, pnr, e->nmask, e->noff, e->rpos,
+                       e->nlist, e->nrpos);
   hipLaunchKernelGGL(k_finalize_counts, dim3(1), dim3(1), 0, st, n, nruns > 0 ? pnr : nullptr, e->moff, e->mmask,
                      e->noff, e->nmask, cap,
                      e->d_stats);
   const int o = 1 - c;
   hipLaunchKernelGGL(k_scatter_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->det[c], e->pop_new,
-                     e->kii[c], e->mmask, e->moff, e->ins_scan, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
+                     e->kii[c], e->mmask, e->moff, e->nrpos, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
   if (nruns > 0) {
-    hipLaunchKernelGGL(k_new_list, dim3(grid_for(nruns)), dim3(kBlock), 0, st, pnr, e->nmask, e->noff, e->nlist);
     // dense pair tables in LDS for the diagonal when they fit in 32 KB
     const size_t jk = e->sys.system == FQ_SYS_READ_IN && !e->batched
                           ? 2 * sizeof(double) * static_cast<size_t>(e->sys.norb) * e->sys.norb
@@ -2317,8 +2327,6 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->mmask, cap / 64 + 2);
   rc |= dalloc(&e->mcnt, cap / 64 + 2);
   rc |= dalloc(&e->moff, cap / 64 + 2);
-  rc |= dalloc(&e->ins, cap + 2);
-  rc |= dalloc(&e->ins_scan, cap + 2);
   rc |= dalloc(&e->mdraw, cap + 1);
   rc |= dalloc(&e->mdraw_off, cap + 1);
   rc |= dalloc(&e->part_d, 4 * (cap / kBlock + 2));
@@ -2348,6 +2356,7 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->rpos, scap);
   rc |= dalloc(&e->rstate, scap);
   rc |= dalloc(&e->nlist, scap);
+  rc |= dalloc(&e->nrpos, scap);
   rc |= dalloc(&e->d_sysp, 1);
   rc |= dalloc(&e->mwin, 2 * (scap / kBlock + 2));
   rc |= dalloc(&e->nmask, scap / 64 + 2);
@@ -3210,8 +3219,6 @@ void fq_destroy(fq_engine* e) {
   dfree(e->mmask);
   dfree(e->mcnt);
   dfree(e->moff);
-  dfree(e->ins);
-  dfree(e->ins_scan);
   dfree(e->mdraw);
   dfree(e->mdraw_off);
   dfree(e->part_d);
@@ -3238,6 +3245,7 @@ void fq_destroy(fq_engine* e) {
   dfree(e->rpos);
   dfree(e->rstate);
   dfree(e->nlist);
+  dfree(e->nrpos);
   dfree(e->d_sysp);
   dfree(e->mwin);
   dfree(e->nmask);
