@@ -117,10 +117,13 @@ struct Philox {
   __device__ Philox(uint64_t key, uint32_t a, uint32_t b, uint32_t c)
       : k0(static_cast<uint32_t>(key)), k1(static_cast<uint32_t>(key >> 32)), c0(a), c1(b), c2(c), block(0),
         o0(0), o1(0), o2(0), o3(0), second(false), failed(false) {}
+  // Each product's high and low words come from one 32x32->64 multiply
+  // (v_mad_u64_u32) rather than separate high and low multiplies.
   __device__ static void round(uint32_t* x, uint32_t ka, uint32_t kb) {
-    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-    uint32_t hi0 = __umulhi(M0, x[0]), lo0 = M0 * x[0];
-    uint32_t hi1 = __umulhi(M1, x[2]), lo1 = M1 * x[2];
+    const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * x[0];
+    const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * x[2];
+    const uint32_t hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+    const uint32_t hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
     uint32_t y0 = hi1 ^ x[1] ^ ka, y1 = lo1, y2 = hi0 ^ x[3] ^ kb, y3 = lo0;
     x[0] = y0;
     x[1] = y1;
