@@ -77,6 +77,12 @@ struct Sys {
   const double* mlp_rs;
   const double* mlp_crs;
   const uint64_t* mlp_cmask;
+  // alias tables of the rows: mlp_au/ak[(h-1)*nbasis + p-1]; per class
+  // mlp_cu/ck[((h-1)*ncls + c)*max_sym_spin + k] over the class's members
+  const double* mlp_au;
+  const int* mlp_ak;
+  const double* mlp_cu;
+  const int* mlp_ck;
   int64_t mlp_ld;
   const int64_t* mlp_on;   // per determinant: 1 if the network ran on it this cycle
   const int64_t* mlp_row;  // its row of mlp_tab (exclusive prefix of mlp_on)
@@ -969,6 +975,32 @@ __host__ __device__ inline int alias_pick(int n, const HbBuf& wt, double tot, do
   }
   return kk;
 }
+// generate_alias_tables (lib/local/alias.f90:101-184) for n <= 128 weights w
+// (1-based results: u scaled probabilities, k aliases in 1..n).
+__host__ __device__ inline void build_alias(int n, const double* w, double tot, double* u, int* k) {
+  uint8_t under[128], over[128];
+  int nunder = 0, nover = 0;
+  const double scale = static_cast<double>(n) / tot;
+  for (int i = 1; i <= n; ++i) {
+    u[i - 1] = w[i - 1] * scale;
+    if (u[i - 1] <= 1.0)
+      under[nunder++] = static_cast<uint8_t>(i);
+    else
+      over[nover++] = static_cast<uint8_t>(i);
+    k[i - 1] = i;
+  }
+  while (nover > 0 && nunder > 0) {
+    const int ov = over[nover - 1], un = under[nunder - 1];
+    k[un - 1] = ov;
+    nunder = nunder - 1;
+    u[ov - 1] = u[ov - 1] - (1.0 - u[un - 1]);
+    if (u[ov - 1] < 1.0) {
+      under[nunder++] = over[nover - 1];
+      nover = nover - 1;
+    }
+  }
+}
+
 // select_weighted_value_precalc (lib/local/alias.f90:13-62) on a stored table.
 __device__ __forceinline__ int alias_draw(int n, const double* u, const int* k, double r) {
   double x = r * n;
@@ -1197,6 +1229,8 @@ __device__ inline int partner_pick(const Sys& s, const uint64_t* f, const float*
 
 // aff_in / flat_in: copies of s.mlp_aff / s.mlp_flat (in LDS in the spawning
 // kernel); nullptr reads the device tables.
+constexpr int kFlatTries = 16;  // alias redraws before the exact walk takes over
+
 // Flat weights (e = f = 1, determinants without a network row): normalisers
 // as the affinity row sums less their occupied terms, O(nel) instead of
 // O(nvirt).
@@ -1319,7 +1353,13 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
           }
         }
       }
-      x = rng.next() * z1i;
+      if (flat) {  // A[i][.] over all orbitals by its alias table, occupied ones redrawn
+        for (int t = 0; t < kFlatTries && !a; ++t) {
+          const int p = alias_draw(nb, s.mlp_au + (i - 1) * nb, s.mlp_ak + (i - 1) * nb, rng.next());
+          if (!btest(f, p)) a = p;
+        }
+      }
+      x = a ? 0.0 : rng.next() * z1i;  // the walk below, exact, when the redraws ran out
       cum = 0.0;
       int lasta = 0;
       FQ_FOR_WORDS(w, s.W) if (!a)
@@ -1339,10 +1379,20 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
       else
         partner_stats(s, f, fw, ai, aj, cb, a, nbc, zbi, zbj);
       have_z = true;
-      if (nbc == 0)
+      if (nbc == 0) {
         ok = false;
-      else
-        b = partner_pick(s, f, fw, aj, cb, a, 0, rng.next() * zbj, true);
+      } else {
+        if (flat) {  // A[j][.] over the class by its alias table, occupied members and a redrawn
+          const int ci = (cb.ims - 1) * s.nsym_pg + cb.sym, n = nbss(s, cb.ims, cb.sym);
+          const int64_t o = (static_cast<int64_t>(j - 1) * 2 * s.nsym_pg + ci) * s.max_sym_spin;
+          for (int t = 0; t < kFlatTries && !b; ++t) {
+            const int q = alias_draw(n, s.mlp_cu + o, s.mlp_ck + o, rng.next());
+            const int p = ssbf(s, cb.ims, cb.sym, q - 1);
+            if (p != a && !btest(f, p)) b = p;
+          }
+        }
+        if (!b) b = partner_pick(s, f, fw, aj, cb, a, 0, rng.next() * zbj, true);
+      }
     }
     if (ok) {
       const float* ai = aff + (i - 1) * nb;

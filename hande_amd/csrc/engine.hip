@@ -2079,30 +2079,7 @@ __global__ void k_hb_alias(int64_t ntab, int n, const double* __restrict__ w, co
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= ntab) return;
   if (!(fabs(gate[t / gate_div]) > 0.0) || !(fabs(tot[t]) > 0.0)) return;
-  const double* wt = w + t * n;
-  double* ut = u + t * n;
-  int* kt = k + t * n;
-  uint8_t under[128], over[128];
-  int nunder = 0, nover = 0;
-  const double scale = static_cast<double>(n) / tot[t];
-  for (int i = 1; i <= n; ++i) {
-    ut[i - 1] = wt[i - 1] * scale;
-    if (ut[i - 1] <= 1.0)
-      under[nunder++] = static_cast<uint8_t>(i);
-    else
-      over[nover++] = static_cast<uint8_t>(i);
-    kt[i - 1] = i;
-  }
-  while (nover > 0 && nunder > 0) {
-    const int ov = over[nover - 1], un = under[nunder - 1];
-    kt[un - 1] = ov;
-    nunder = nunder - 1;
-    ut[ov - 1] = ut[ov - 1] - (1.0 - ut[un - 1]);
-    if (ut[ov - 1] < 1.0) {
-      under[nunder++] = over[nover - 1];
-      nover = nover - 1;
-    }
-  }
+  fqd::build_alias(n, w + t * n, tot[t], u + t * n, k + t * n);
 }
 
 int build_heat_bath(fq_engine* e) {
@@ -2208,6 +2185,10 @@ int setup_learned(fq_engine* e) {
   s.mlp_rs = e->mlp.rs;
   s.mlp_crs = e->mlp.crs;
   s.mlp_cmask = e->mlp.cmask;
+  s.mlp_au = e->mlp.au;
+  s.mlp_ak = e->mlp.ak;
+  s.mlp_cu = e->mlp.cu;
+  s.mlp_ck = e->mlp.ck;
   {
     const std::vector<float> ones(2 * static_cast<size_t>(s.nbasis), 1.0f);
     if (dalloc(&e->mlp_flat, 2 * static_cast<int64_t>(s.nbasis)) != FQ_OK) return FQ_ERR_NOMEM;

@@ -368,6 +368,30 @@ __global__ void k_aff_sums(int nb, const float* __restrict__ aff, const int* __r
   }
 }
 
+// Alias tables of the affinity rows (one thread per (hole, table): table 0 the
+// whole row, table 1 + c class c's members), after k_aff_sums.
+__global__ void k_aff_alias(int nb, const float* __restrict__ aff, const int* __restrict__ nbss,
+                            const int* __restrict__ ssbf, int nsym_pg, int max_sym_spin,
+                            const double* __restrict__ rs, const double* __restrict__ crs, double* __restrict__ au,
+                            int* __restrict__ ak, double* __restrict__ cu, int* __restrict__ ck) {
+  const int ncls = 2 * nsym_pg;
+  const int t = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nb * (1 + ncls)) return;
+  const int h = t / (1 + ncls), c = t % (1 + ncls) - 1;
+  const float* row = aff + static_cast<int64_t>(h) * nb;
+  double w[128];
+  if (c < 0) {
+    for (int p = 0; p < nb; ++p) w[p] = static_cast<double>(row[p]);
+    fqd::build_alias(nb, w, rs[h], au + static_cast<int64_t>(h) * nb, ak + static_cast<int64_t>(h) * nb);
+  } else {
+    const int n = nbss[c];
+    if (n == 0) return;
+    for (int q = 0; q < n; ++q) w[q] = static_cast<double>(row[ssbf[c * max_sym_spin + q] - 1]);
+    const int64_t o = (static_cast<int64_t>(h) * ncls + c) * max_sym_spin;
+    fqd::build_alias(n, w, crs[h * ncls + c], cu + o, ck + o);
+  }
+}
+
 // The stored affinity table: exp(clamp(theta, -60, 60)) rounded to float.
 __global__ void k_aff_table(int64_t n, const float* __restrict__ th, float* __restrict__ aff) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -542,6 +566,10 @@ void mlp_free(Mlp& m) {
   dfree(m.rs);
   dfree(m.crs);
   dfree(m.cmask);
+  dfree(m.au);
+  dfree(m.ak);
+  dfree(m.cu);
+  dfree(m.ck);
   dfree(m.dth);
   dfree(m.aff_part);
   dfree(m.scum);
@@ -556,9 +584,13 @@ int mlp_refresh(Mlp& m, hipStream_t st) {
                      m.o_w2, m.o_w3, m.w1t, m.w2t, m.w3t);
   const int64_t na = static_cast<int64_t>(m.nb) * m.nb;
   hipLaunchKernelGGL(k_aff_table, dim3(blocks(na)), dim3(kBlock), 0, st, na, m.p + m.o_th, m.aff);
-  if (m.nbss)
+  if (m.nbss) {
     hipLaunchKernelGGL(k_aff_sums, dim3(blocks(m.nb)), dim3(kBlock), 0, st, m.nb, m.aff, m.nbss, m.ssbf, m.nsym_pg,
                        m.max_sym_spin, m.rs, m.crs, m.cmask);
+    const int ntab = m.nb * (1 + 2 * m.nsym_pg);
+    hipLaunchKernelGGL(k_aff_alias, dim3(blocks(ntab)), dim3(kBlock), 0, st, m.nb, m.aff, m.nbss, m.ssbf,
+                       m.nsym_pg, m.max_sym_spin, m.rs, m.crs, m.au, m.ak, m.cu, m.ck);
+  }
   MLP_HIP(hipGetLastError());
   return FQ_OK;
 }
@@ -571,6 +603,10 @@ int mlp_set_classes(Mlp& m, const int* nbss, const int* ssbf, int nsym_pg, int m
   int rc = dalloc(&m.rs, m.nb);
   rc |= dalloc(&m.crs, static_cast<int64_t>(m.nb) * 2 * nsym_pg);
   rc |= dalloc(&m.cmask, static_cast<int64_t>(2) * nsym_pg * FQ_MAX_WORDS);
+  rc |= dalloc(&m.au, static_cast<int64_t>(m.nb) * m.nb);
+  rc |= dalloc(&m.ak, static_cast<int64_t>(m.nb) * m.nb);
+  rc |= dalloc(&m.cu, static_cast<int64_t>(m.nb) * 2 * nsym_pg * max_sym_spin);
+  rc |= dalloc(&m.ck, static_cast<int64_t>(m.nb) * 2 * nsym_pg * max_sym_spin);
   return rc == FQ_OK ? FQ_OK : FQ_ERR_NOMEM;
 }
 

@@ -43,6 +43,10 @@ struct Mlp {
   double* rs = nullptr;
   double* crs = nullptr;
   uint64_t* cmask = nullptr;
+  // alias tables of the affinity rows for flat-weight draws: whole row
+  // (au/ak [h][p]) and per class (cu/ck [h][c][k] over the class's members)
+  double *au = nullptr, *cu = nullptr;
+  int *ak = nullptr, *ck = nullptr;
   const int* nbss = nullptr;  // the system's class tables (device), set before mlp_init
   const int* ssbf = nullptr;
   int nsym_pg = 1, max_sym_spin = 0;
