@@ -1644,26 +1644,58 @@ __global__ void k_finalize_counts(int64_t n, const int64_t* __restrict__ pnr, co
 // Surviving main entries move up by their rank among survivors plus the number
 // of surviving new runs inserted at or before them: nrpos lists those runs'
 // insertion positions in rank order, non-decreasing because the runs are
-// sorted, so the count is an upper bound search.
-__global__ void k_scatter_main(Sys s, int64_t n, int64_t cap, const uint64_t* __restrict__ det,
-                               const int64_t* __restrict__ pop_new, const double* __restrict__ kii,
-                               const uint64_t* __restrict__ mmask, const int64_t* __restrict__ moff,
-                               const int64_t* __restrict__ nrpos, const Stats* __restrict__ st,
-                               uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
-                               double* __restrict__ kii_out) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+// sorted, so the count is an upper-bound search.  Each block first bounds the
+// counts of its 256 entries with two searches and stages that window of nrpos
+// in LDS when it fits (dense insertion), so each thread searches only there.
+constexpr int kInsWin = 1024;
+__device__ __forceinline__ int64_t upper_bound_pos(const int64_t* __restrict__ v, int64_t lo, int64_t hi, int64_t x) {
+  while (lo < hi) {  // first k in [lo, hi) with v[k] > x
+    const int64_t mid = (lo + hi) >> 1;
+    if (v[mid] <= x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+__global__ __launch_bounds__(kBlock) void k_scatter_main(Sys s, int64_t n, int64_t cap,
+                                                         const uint64_t* __restrict__ det,
+                                                         const int64_t* __restrict__ pop_new,
+                                                         const double* __restrict__ kii,
+                                                         const uint64_t* __restrict__ mmask,
+                                                         const int64_t* __restrict__ moff,
+                                                         const int64_t* __restrict__ nrpos, const Stats* __restrict__ st,
+                                                         uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
+                                                         double* __restrict__ kii_out) {
+  __shared__ int64_t win[kInsWin];
+  __shared__ int64_t cb[2];
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
+  if (i0 >= n) return;  // whole block
+  const int64_t i1 = (i0 + kBlock < n ? i0 + kBlock : n) - 1;
+  const bool ovf = st->main_overflow != 0;
+  const int64_t nnew = ovf ? 0 : st->new_n - bit_rank(moff, mmask, n);  // surviving new runs
+  if (threadIdx.x < 2) cb[threadIdx.x] = upper_bound_pos(nrpos, 0, nnew, threadIdx.x == 0 ? i0 - 1 : i1);
+  __syncthreads();
+  const int64_t c0 = cb[0], c1 = cb[1];
+  const bool staged = c1 - c0 <= kInsWin;
+  if (staged)
+    for (int64_t q = threadIdx.x; q < c1 - c0; q += kBlock) win[q] = nrpos[c0 + q];
+  __syncthreads();
+  const int64_t i = i0 + threadIdx.x;
   if (i >= n || !bit_get(mmask, i)) return;
-  int64_t shift = 0;
-  if (!st->main_overflow) {
-    int64_t lo = 0, hi = st->new_n - bit_rank(moff, mmask, n);  // surviving new runs
-    while (lo < hi) {  // first k with nrpos[k] > i
-      const int64_t mid = (lo + hi) >> 1;
-      if (nrpos[mid] <= i)
-        lo = mid + 1;
+  int64_t shift;
+  if (staged) {
+    int a = 0, b = static_cast<int>(c1 - c0);
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (win[mid] <= i)
+        a = mid + 1;
       else
-        hi = mid;
+        b = mid;
     }
-    shift = lo;
+    shift = c0 + a;
+  } else {
+    shift = upper_bound_pos(nrpos, c0, c1, i);
   }
   const int64_t o = bit_rank(moff, mmask, i) + shift;
   for (int w = 0; w < s.W; ++w) det_out[w * cap + o] = det[w * cap + i];
