@@ -1026,7 +1026,8 @@ __device__ __forceinline__ int key_cmp(const uint64_t* keys, int64_t stride, int
 
 // Radix passes use DB-bit digits (8 or 9; 1 << DB buckets): the host mixes
 // the two widths to cover a key word in the fewest passes (34 bits: 9+9+8+8).
-// Histogram: hist[d * nb + block] = count of digit d in the block's tile.
+// Histogram, tile-major: hist[block * NB + d] = count of digit d in the
+// block's tile, so a tile's counts (and later its offsets) are contiguous.
 template <int DB>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* __restrict__ kw, int64_t n, int shift,
                                                        int64_t* __restrict__ hist) {
@@ -1040,7 +1041,65 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* __restric
     if (i < n) atomicAdd(&cnt[(kw[i] >> shift) & (NB - 1)], 1);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < NB; d += kBlock) hist[static_cast<int64_t>(d) * gridDim.x + blockIdx.x] = cnt[d];
+  for (int d = threadIdx.x; d < NB; d += kBlock) hist[static_cast<int64_t>(blockIdx.x) * NB + d] = cnt[d];
+}
+
+// Digit offsets of every tile from the tile-major counts: offset(b, d) =
+// (tiles of all digits below d) + (tiles above b with digit d).  Three steps
+// over chunks of `rows` tiles (the host keeps the chunks at most
+// kHistChunks), every access contiguous along the digits: per-chunk column
+// sums, per-digit chunk offsets plus the digit starts (one block), then the
+// tiles' offsets within their chunk.
+constexpr int64_t kHistChunks = 256;
+template <int DB>
+__global__ __launch_bounds__(kBlock) void k_hist_colsum(const int64_t* __restrict__ hist, int64_t nb, int64_t rows,
+                                                        int64_t* __restrict__ csum) {
+  constexpr int NB = 1 << DB;
+  const int d = static_cast<int>(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t c = blockIdx.y, b0 = c * rows, b1 = b0 + rows < nb ? b0 + rows : nb;
+  int64_t acc = 0;
+  for (int64_t b = b0; b < b1; ++b) acc += hist[b * NB + d];
+  csum[c * NB + d] = acc;
+}
+template <int DB>
+__global__ __launch_bounds__(kBlock) void k_hist_chunkoff(int64_t nchunk, int64_t* __restrict__ csum) {
+  constexpr int NB = 1 << DB, DPT = NB / kBlock;  // DPT consecutive digits per thread
+  __shared__ int64_t part[kBlock];
+  int64_t tot[DPT], mine = 0;
+  for (int j = 0; j < DPT; ++j) {  // digit totals (loads only)
+    const int d = threadIdx.x * DPT + j;
+    int64_t t = 0;
+    for (int64_t c = 0; c < nchunk; ++c) t += csum[c * NB + d];
+    tot[j] = t;
+    mine += t;
+  }
+  part[threadIdx.x] = mine;
+  __syncthreads();
+  block_inclusive_scan(part);
+  int64_t start = part[threadIdx.x] - mine;
+  for (int j = 0; j < DPT; ++j) {  // chunk offsets: digit start plus the chunks before
+    const int d = threadIdx.x * DPT + j;
+    int64_t run = start;
+    for (int64_t c = 0; c < nchunk; ++c) {
+      const int64_t v = csum[c * NB + d];
+      csum[c * NB + d] = run;
+      run += v;
+    }
+    start += tot[j];
+  }
+}
+template <int DB>
+__global__ __launch_bounds__(kBlock) void k_hist_colscan(const int64_t* __restrict__ hist, int64_t nb, int64_t rows,
+                                                         const int64_t* __restrict__ coff, int64_t* __restrict__ hoff) {
+  constexpr int NB = 1 << DB;
+  const int d = static_cast<int>(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t c = blockIdx.y, b0 = c * rows, b1 = b0 + rows < nb ? b0 + rows : nb;
+  int64_t run = coff[c * NB + d];
+  for (int64_t b = b0; b < b1; ++b) {
+    const int64_t v = hist[b * NB + d];
+    hoff[b * NB + d] = run;
+    run += v;
+  }
 }
 
 // Radix pass scatter, stable, staged in LDS.  The tile's elements are ranked
@@ -1050,7 +1109,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* __restric
 // offsets (an exclusive scan of the tile's histogram); the tile is then
 // written out in that order, so consecutive threads store consecutive
 // addresses of one digit's run (global base of digit d for this tile:
-// hist_scan[d * nb + tile]).  KW key words per element.
+// hist_scan[tile * NB + d]).  KW key words per element.
 template <int KW, int DB>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                           int64_t n, int w_digit, int shift,
@@ -1067,13 +1126,12 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __rest
   __shared__ int16_t wcnt[kBlock / 64][NB];
   __shared__ int64_t part[kBlock];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t nb = gridDim.x;
   // tile digit offsets: exclusive scan over the NB counts, DPT consecutive per thread
   int64_t c[DPT], tsum = 0;
   for (int j = 0; j < DPT; ++j) {
     const int d = tid * DPT + j;
-    c[j] = hist[static_cast<int64_t>(d) * nb + blockIdx.x];
-    gbase[d] = hist_scan[static_cast<int64_t>(d) * nb + blockIdx.x];
+    c[j] = hist[static_cast<int64_t>(blockIdx.x) * NB + d];
+    gbase[d] = hist_scan[static_cast<int64_t>(blockIdx.x) * NB + d];
     run[d] = 0;
     tsum += c[j];
   }
@@ -1882,8 +1940,16 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
           const int db = pidx < n9 ? 9 : 8;
           auto hist = db == 9 ? k_radix_hist<9> : k_radix_hist<8>;
           hipLaunchKernelGGL(hist, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src] + w * scap, ns, shift, e->hist);
-          int rc = scan_exclusive(e, e->hist, e->hist_scan, (int64_t{1} << db) * nb);
-          if (rc != FQ_OK) return rc;
+          const int64_t rows = std::max<int64_t>(16, ceil_div(nb, kHistChunks)), nchunk = ceil_div(nb, rows);
+          int64_t* csum = scan_scratch(e, 2, nchunk * 512);
+          if (!csum) return FQ_ERR_NOMEM;
+          const dim3 cg(static_cast<unsigned>((1 << db) / kBlock), static_cast<unsigned>(nchunk));
+          hipLaunchKernelGGL(db == 9 ? k_hist_colsum<9> : k_hist_colsum<8>, cg, dim3(kBlock), 0, e->stream, e->hist, nb,
+                             rows, csum);
+          hipLaunchKernelGGL(db == 9 ? k_hist_chunkoff<9> : k_hist_chunkoff<8>, dim3(1), dim3(kBlock), 0, e->stream,
+                             nchunk, csum);
+          hipLaunchKernelGGL(db == 9 ? k_hist_colscan<9> : k_hist_colscan<8>, cg, dim3(kBlock), 0, e->stream, e->hist,
+                             nb, rows, csum, e->hist_scan);
           auto scatter = kw == 1 ? (db == 9 ? k_radix_scatter<1, 9> : k_radix_scatter<1, 8>)
                                  : (db == 9 ? k_radix_scatter<2, 9> : k_radix_scatter<2, 8>);
           hipLaunchKernelGGL(scatter, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src], e->vbuf[src], ns, w, shift,
