@@ -1048,9 +1048,10 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* __restric
 // (tiles of all digits below d) + (tiles above b with digit d).  Three steps
 // over chunks of `rows` tiles (the host keeps the chunks at most
 // kHistChunks), every access contiguous along the digits: per-chunk column
-// sums, per-digit chunk offsets plus the digit starts (one block), then the
-// tiles' offsets within their chunk.
-constexpr int64_t kHistChunks = 256;
+// sums, per-digit chunk offsets (one block per digit) with the digit totals,
+// then the tiles' offsets (chunk offset plus digit start plus the tiles
+// above within the chunk).
+constexpr int64_t kHistChunks = kBlock;  // one thread per chunk in k_hist_chunkscan
 template <int DB>
 __global__ __launch_bounds__(kBlock) void k_hist_colsum(const int64_t* __restrict__ hist, int64_t nb, int64_t rows,
                                                         int64_t* __restrict__ csum) {
@@ -1061,40 +1062,55 @@ __global__ __launch_bounds__(kBlock) void k_hist_colsum(const int64_t* __restric
   for (int64_t b = b0; b < b1; ++b) acc += hist[b * NB + d];
   csum[c * NB + d] = acc;
 }
+// Per digit (one block each, one thread per chunk, nchunk <= kBlock): the
+// chunks' exclusive offsets within the digit, in place, and the digit total.
 template <int DB>
-__global__ __launch_bounds__(kBlock) void k_hist_chunkoff(int64_t nchunk, int64_t* __restrict__ csum) {
-  constexpr int NB = 1 << DB, DPT = NB / kBlock;  // DPT consecutive digits per thread
+__global__ __launch_bounds__(kBlock) void k_hist_chunkscan(int64_t nchunk, int64_t* __restrict__ csum,
+                                                           int64_t* __restrict__ dtot) {
+  constexpr int NB = 1 << DB;
+  __shared__ int64_t wsh[kBlock / 64];
+  const int d = blockIdx.x, c = threadIdx.x, lane = c & 63, wave = c >> 6;
+  const int64_t v = c < nchunk ? csum[static_cast<int64_t>(c) * NB + d] : 0;
+  const int64_t x = wave_inclusive_scan(v, lane);
+  if (lane == 63) wsh[wave] = x;
+  __syncthreads();
+  int64_t pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    pre += w < wave ? wsh[w] : 0;
+    tot += wsh[w];
+  }
+  if (c < nchunk) csum[static_cast<int64_t>(c) * NB + d] = pre + x - v;
+  if (c == 0) dtot[d] = tot;
+}
+// Tile offsets within each chunk plus the chunk's offset and the digit's
+// start (an exclusive scan of the digit totals, redone by every block).
+template <int DB>
+__global__ __launch_bounds__(kBlock) void k_hist_colscan(const int64_t* __restrict__ hist, int64_t nb, int64_t rows,
+                                                         const int64_t* __restrict__ coff,
+                                                         const int64_t* __restrict__ dtot, int64_t* __restrict__ hoff) {
+  constexpr int NB = 1 << DB, DPT = NB / kBlock;
   __shared__ int64_t part[kBlock];
-  int64_t tot[DPT], mine = 0;
-  for (int j = 0; j < DPT; ++j) {  // digit totals (loads only)
-    const int d = threadIdx.x * DPT + j;
-    int64_t t = 0;
-    for (int64_t c = 0; c < nchunk; ++c) t += csum[c * NB + d];
-    tot[j] = t;
-    mine += t;
+  __shared__ int64_t dstart[NB];
+  int64_t t[DPT], mine = 0;
+#pragma unroll
+  for (int j = 0; j < DPT; ++j) {
+    t[j] = dtot[threadIdx.x * DPT + j];
+    mine += t[j];
   }
   part[threadIdx.x] = mine;
   __syncthreads();
   block_inclusive_scan(part);
-  int64_t start = part[threadIdx.x] - mine;
-  for (int j = 0; j < DPT; ++j) {  // chunk offsets: digit start plus the chunks before
-    const int d = threadIdx.x * DPT + j;
-    int64_t run = start;
-    for (int64_t c = 0; c < nchunk; ++c) {
-      const int64_t v = csum[c * NB + d];
-      csum[c * NB + d] = run;
-      run += v;
-    }
-    start += tot[j];
+  int64_t st = part[threadIdx.x] - mine;
+#pragma unroll
+  for (int j = 0; j < DPT; ++j) {
+    dstart[threadIdx.x * DPT + j] = st;
+    st += t[j];
   }
-}
-template <int DB>
-__global__ __launch_bounds__(kBlock) void k_hist_colscan(const int64_t* __restrict__ hist, int64_t nb, int64_t rows,
-                                                         const int64_t* __restrict__ coff, int64_t* __restrict__ hoff) {
-  constexpr int NB = 1 << DB;
+  __syncthreads();
   const int d = static_cast<int>(blockIdx.x) * kBlock + threadIdx.x;
   const int64_t c = blockIdx.y, b0 = c * rows, b1 = b0 + rows < nb ? b0 + rows : nb;
-  int64_t run = coff[c * NB + d];
+  int64_t run = dstart[d] + coff[c * NB + d];
   for (int64_t b = b0; b < b1; ++b) {
     const int64_t v = hist[b * NB + d];
     hoff[b * NB + d] = run;
@@ -1942,14 +1958,15 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
           hipLaunchKernelGGL(hist, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src] + w * scap, ns, shift, e->hist);
           const int64_t rows = std::max<int64_t>(16, ceil_div(nb, kHistChunks)), nchunk = ceil_div(nb, rows);
           int64_t* csum = scan_scratch(e, 2, nchunk * 512);
-          if (!csum) return FQ_ERR_NOMEM;
+          int64_t* dtot = scan_scratch(e, 3, 512);
+          if (!csum || !dtot) return FQ_ERR_NOMEM;
           const dim3 cg(static_cast<unsigned>((1 << db) / kBlock), static_cast<unsigned>(nchunk));
           hipLaunchKernelGGL(db == 9 ? k_hist_colsum<9> : k_hist_colsum<8>, cg, dim3(kBlock), 0, e->stream, e->hist, nb,
                              rows, csum);
-          hipLaunchKernelGGL(db == 9 ? k_hist_chunkoff<9> : k_hist_chunkoff<8>, dim3(1), dim3(kBlock), 0, e->stream,
-                             nchunk, csum);
+          hipLaunchKernelGGL(db == 9 ? k_hist_chunkscan<9> : k_hist_chunkscan<8>, dim3(1u << db), dim3(kBlock), 0,
+                             e->stream, nchunk, csum, dtot);
           hipLaunchKernelGGL(db == 9 ? k_hist_colscan<9> : k_hist_colscan<8>, cg, dim3(kBlock), 0, e->stream, e->hist,
-                             nb, rows, csum, e->hist_scan);
+                             nb, rows, csum, dtot, e->hist_scan);
           auto scatter = kw == 1 ? (db == 9 ? k_radix_scatter<1, 9> : k_radix_scatter<1, 8>)
                                  : (db == 9 ? k_radix_scatter<2, 9> : k_radix_scatter<2, 8>);
           hipLaunchKernelGGL(scatter, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src], e->vbuf[src], ns, w, shift,
