@@ -431,8 +431,10 @@ def efficiency_learned(args):
     runs = [("renorm", N.FQ_EXCIT_RENORM, 0), ("heat_bath", N.FQ_EXCIT_HEAT_BATH, 0)]
     runs += [(f"learned_k{k}", N.FQ_EXCIT_LEARNED, k) for k in thresholds]
     training = {}
-    for gen, code, k in runs:
-        eng, _p = make_engine(lib, system, args.target, 23, cap, cap, excit_gen=code, mix=0.1, min_attempts=k)
+    seeds = [int(x) for x in str(args.eff_seeds).split(",")]
+    per_seed = {}
+    for (gen, code, k), sd in ((r, sd) for sd in seeds for r in runs):
+        eng, _p = make_engine(lib, system, args.target, sd, cap, cap, excit_gen=code, mix=0.1, min_attempts=k)
         t_train = 0.0
         if code == N.FQ_EXCIT_LEARNED:
             # trained for its threshold: samples of determinants below it are scored with flat weights
@@ -445,10 +447,10 @@ def efficiency_learned(args):
             N.check(lib.fq_mlp_train(eng, 16384, args.train_steps, 1e-3, 5,
                                      losses.ctypes.data_as(C.POINTER(C.c_double))), "fq_mlp_train")
             t_train = time.perf_counter() - t1
-            training[gen] = {"seconds": t_train, "loss_first10": float(losses[:10].mean()),
-                             "loss_last10": float(losses[-10:].mean())}
-            log(f"[eff] {gen}: trained {args.train_steps} steps in {t_train:.1f}s: loss "
-                f"{training[gen]['loss_first10']:.4f} -> {training[gen]['loss_last10']:.4f}")
+            training[f"{gen}_seed{sd}"] = {"seconds": t_train, "loss_first10": float(losses[:10].mean()),
+                                           "loss_last10": float(losses[-10:].mean())}
+            log(f"[eff] {gen} seed {sd}: trained {args.train_steps} steps in {t_train:.1f}s: loss "
+                f"{losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}")
         # each generator settles its own (initiator) stationary state before the timed cycles
         prof = {}
         proj, d0, wall, att, last = production(lib, eng, args.target, state, 1e-3, args.eff_cycles, args.eff_warmup,
@@ -457,17 +459,24 @@ def efficiency_learned(args):
         rb = reblock.projected_energy(proj, d0)
         e_bm, sigma = batch_means_ratio(proj, d0, EFF_BATCHES)
         t = wall + t_train
-        res[gen] = {"E_proj": system.H00 + e_bm, "sigma": sigma, "wall_s": wall, "train_s": t_train,
+        r = {"seed": sd, "E_proj": system.H00 + e_bm, "sigma": sigma, "wall_s": wall, "train_s": t_train,
                     "efficiency": reblock.efficiency(sigma, t), "attempts_per_s": att / wall,
                     "sigma_method": f"batch means: {EFF_BATCHES} contiguous batches of the production cycles",
                     "reblock": {"E_proj": system.H00 + rb["energy"], "sigma": rb["error"], "level": rb["level"],
                                 "converged": rb["converged"]},
                     "walkers": last.nparticles[0], "determinants": int(last.nstates), "cycle_ms": prof["cycle_ms"]}
         if code == N.FQ_EXCIT_LEARNED:
-            res[gen].update(min_attempts=k, proposal_ms=prof["proposal_ms"], network_rows=prof["proposal_rows"])
-        log(f"[eff] {gen}: E_proj {res[gen]['E_proj']:.6f} +- {sigma:.6f}, {wall:.1f}s "
+            r.update(min_attempts=k, proposal_ms=prof["proposal_ms"], network_rows=prof["proposal_rows"])
+        per_seed.setdefault(gen, []).append(r)
+        log(f"[eff] {gen} seed {sd}: E_proj {r['E_proj']:.6f} +- {sigma:.6f}, {wall:.1f}s "
             f"({prof['cycle_ms']:.2f} ms/cycle, proposal {prof['proposal_ms']:.2f} ms on {prof['proposal_rows']:.0f} "
-            f"rows), efficiency {res[gen]['efficiency']:.4g} (reblocking converged: {rb['converged']})")
+            f"rows), efficiency {r['efficiency']:.4g} (reblocking converged: {rb['converged']})")
+    # per generator: the mean over seeds of each run's efficiency (independent chains)
+    for gen, rs in per_seed.items():
+        effs = np.array([r["efficiency"] for r in rs])
+        res[gen] = {"efficiency": float(effs.mean()), "efficiency_min": float(effs.min()),
+                    "efficiency_max": float(effs.max()), "sigma": float(np.mean([r["sigma"] for r in rs])),
+                    "E_proj": float(np.mean([r["E_proj"] for r in rs])), "seeds": rs}
     b, h = res["renorm"], res["heat_bath"]
     head = f"learned_k{thresholds[0]}"
     a = res[head]
@@ -490,7 +499,8 @@ def efficiency_learned(args):
                                              f"{nel} electrons, C1, real_64, initiator 3, tau 1e-3, target "
                                              f"{args.target:g}, {args.eff_cycles} production cycles",
                                  "proposal_mix": 0.1, "proposal_min_attempts": thresholds[0],
-                                 "train_steps": args.train_steps, "warmup_cycles": args.eff_warmup},
+                                 "train_steps": args.train_steps, "warmup_cycles": args.eff_warmup,
+                                 "seeds": seeds},
                       "training": training,
                       "runs": res}), flush=True)
 
@@ -648,6 +658,8 @@ def main():
                     help="timed cycles of an efficiency mode (default 3000 learned: 300 reports for the "
                          "reblocking; 1000 batched)")
     ap.add_argument("--train-steps", type=int, default=1000)
+    ap.add_argument("--eff-seeds", default="23", help="learned efficiency: production seeds (comma-separated); "
+                    "each generator runs one independent chain per seed and reports the mean efficiency")
     ap.add_argument("--eff-warmup", type=int, default=500, help="untimed cycles per generator before measuring")
     ap.add_argument("--min-attempts", default="64,16",
                     help="learned proposal: comma-separated attempt thresholds for the network (the first is "
