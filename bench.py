@@ -472,10 +472,15 @@ def efficiency_learned(args):
             f"({prof['cycle_ms']:.2f} ms/cycle, proposal {prof['proposal_ms']:.2f} ms on {prof['proposal_rows']:.0f} "
             f"rows), efficiency {r['efficiency']:.4g} (reblocking converged: {rb['converged']})")
     # per generator: the mean over seeds of each run's efficiency (independent chains)
+    # (a chain whose batch means are not all finite gives no efficiency and is left out)
     for gen, rs in per_seed.items():
-        effs = np.array([r["efficiency"] for r in rs])
-        res[gen] = {"efficiency": float(effs.mean()), "efficiency_min": float(effs.min()),
-                    "efficiency_max": float(effs.max()), "sigma": float(np.mean([r["sigma"] for r in rs])),
+        effs = np.array([r["efficiency"] for r in rs], dtype=float)
+        ok = np.isfinite(effs)
+        res[gen] = {"efficiency": float(effs[ok].mean()) if ok.any() else float("nan"),
+                    "efficiency_min": float(effs[ok].min()) if ok.any() else float("nan"),
+                    "efficiency_max": float(effs[ok].max()) if ok.any() else float("nan"),
+                    "chains_with_efficiency": int(ok.sum()),
+                    "sigma": float(np.nanmean([r["sigma"] for r in rs])),
                     "E_proj": float(np.mean([r["E_proj"] for r in rs])), "seeds": rs}
     b, h = res["renorm"], res["heat_bath"]
     head = f"learned_k{thresholds[0]}"
