@@ -338,6 +338,7 @@ struct fq_engine {
   int64_t* rstate = nullptr;     // scap: 0 dead, 1 hit, 2 new candidate
   int64_t* nlist = nullptr;      // scap: surviving new runs in order
   int64_t* nrpos = nullptr;      // scap: their insertion positions (non-decreasing)
+  int64_t* bstart = nullptr;     // cap/256 + 2: new runs before each 256-entry block of the main list
   int64_t* mwin = nullptr;       // main-list bounds of each merge block's first and last run
   uint64_t* nmask = nullptr;     // scap/64+2: surviving new runs, one bit each
   int64_t* ncnt = nullptr;       // survivors per 64-run word
@@ -1718,10 +1719,26 @@ __global__ void k_finalize_counts(int64_t n, const int64_t* __restrict__ pnr, co
 // Surviving main entries move up by their rank among survivors plus the number
 // of surviving new runs inserted at or before them: nrpos lists those runs'
 // insertion positions in rank order, non-decreasing because the runs are
-// sorted, so the count is an upper-bound search.  Each block first bounds the
-// counts of its 256 entries with two searches and stages that window of nrpos
-// in LDS when it fits (dense insertion), so each thread searches only there.
+// sorted, so the count is an upper-bound search.  Each block takes the counts
+// bounding its 256 entries from bstart (k_block_starts) and stages that window
+// of nrpos in LDS when it fits (dense insertion), so each thread searches only
+// there.
 constexpr int kInsWin = 1024;
+// bstart[b] = number of surviving new runs inserted before main entry 256 b
+// (nrpos < 256 b), b in [0, nblk]: thread k (k <= nnew) fills the blocks whose
+// first entry lies in (nrpos[k-1], nrpos[k]].
+__global__ void k_block_starts(const int64_t* __restrict__ nrpos, const Stats* __restrict__ st,
+                               const int64_t* __restrict__ moff, const uint64_t* __restrict__ mmask, int64_t n,
+                               int64_t bound, int64_t nblk, int64_t* __restrict__ bstart) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k > bound) return;
+  const int64_t nnew = st->main_overflow ? 0 : st->new_n - bit_rank(moff, mmask, n);
+  if (k > nnew) return;
+  const int64_t prev = k == 0 ? -1 : nrpos[k - 1];
+  const int64_t b_lo = prev < 0 ? 0 : prev / kBlock + 1;
+  const int64_t b_hi = k == nnew ? nblk : (nrpos[k] / kBlock < nblk ? nrpos[k] / kBlock : nblk);
+  for (int64_t b = b_lo; b <= b_hi; ++b) bstart[b] = k;
+}
 __device__ __forceinline__ int64_t upper_bound_pos(const int64_t* __restrict__ v, int64_t lo, int64_t hi, int64_t x) {
   while (lo < hi) {  // first k in [lo, hi) with v[k] > x
     const int64_t mid = (lo + hi) >> 1;
@@ -1738,19 +1755,16 @@ __global__ __launch_bounds__(kBlock) void k_scatter_main(Sys s, int64_t n, int64
                                                          const double* __restrict__ kii,
                                                          const uint64_t* __restrict__ mmask,
                                                          const int64_t* __restrict__ moff,
-                                                         const int64_t* __restrict__ nrpos, const Stats* __restrict__ st,
+                                                         const int64_t* __restrict__ nrpos,
+                                                         const int64_t* __restrict__ bstart, const Stats* __restrict__ st,
                                                          uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
                                                          double* __restrict__ kii_out) {
   __shared__ int64_t win[kInsWin];
-  __shared__ int64_t cb[2];
   const int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
   if (i0 >= n) return;  // whole block
-  const int64_t i1 = (i0 + kBlock < n ? i0 + kBlock : n) - 1;
   const bool ovf = st->main_overflow != 0;
-  const int64_t nnew = ovf ? 0 : st->new_n - bit_rank(moff, mmask, n);  // surviving new runs
-  if (threadIdx.x < 2) cb[threadIdx.x] = upper_bound_pos(nrpos, 0, nnew, threadIdx.x == 0 ? i0 - 1 : i1);
-  __syncthreads();
-  const int64_t c0 = cb[0], c1 = cb[1];
+  // c0 = runs inserted before i0 exactly; c1 >= runs inserted at or before i1
+  const int64_t c0 = ovf ? 0 : bstart[blockIdx.x], c1 = ovf ? 0 : bstart[blockIdx.x + 1];
   const bool staged = c1 - c0 <= kInsWin;
   if (staged)
     for (int64_t q = threadIdx.x; q < c1 - c0; q += kBlock) win[q] = nrpos[c0 + q];
@@ -2069,8 +2083,11 @@ int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng
                      e->noff, e->nmask, cap,
                      e->d_stats);
   const int o = 1 - c;
+  const int64_t nblk = ceil_div(n, kBlock);
+  hipLaunchKernelGGL(k_block_starts, dim3(grid_for(nruns + 1)), dim3(kBlock), 0, st, e->nrpos, e->d_stats, e->moff,
+                     e->mmask, n, nruns, nblk, e->bstart);
   hipLaunchKernelGGL(k_scatter_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->det[c], e->pop_new,
-                     e->kii[c], e->mmask, e->moff, e->nrpos, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
+                     e->kii[c], e->mmask, e->moff, e->nrpos, e->bstart, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
   if (nruns > 0) {
     // dense pair tables in LDS for the diagonal when they fit in 32 KB
     const size_t jk = e->sys.system == FQ_SYS_READ_IN && !e->batched
@@ -2453,6 +2470,7 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->rstate, scap);
   rc |= dalloc(&e->nlist, scap);
   rc |= dalloc(&e->nrpos, scap);
+  rc |= dalloc(&e->bstart, cap / kBlock + 2);
   rc |= dalloc(&e->d_sysp, 1);
   rc |= dalloc(&e->mwin, 2 * (scap / kBlock + 2));
   rc |= dalloc(&e->nmask, scap / 64 + 2);
@@ -3342,6 +3360,7 @@ void fq_destroy(fq_engine* e) {
   dfree(e->rstate);
   dfree(e->nlist);
   dfree(e->nrpos);
+  dfree(e->bstart);
   dfree(e->d_sysp);
   dfree(e->mwin);
   dfree(e->nmask);
