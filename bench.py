@@ -434,7 +434,8 @@ def efficiency_learned(args):
     seeds = [int(x) for x in str(args.eff_seeds).split(",")]
     per_seed = {}
     for (gen, code, k), sd in ((r, sd) for sd in seeds for r in runs):
-        eng, _p = make_engine(lib, system, args.target, sd, cap, cap, excit_gen=code, mix=0.1, min_attempts=k)
+        eng, _p = make_engine(lib, system, args.target, sd, cap, cap, excit_gen=code, mix=args.proposal_mix,
+                              min_attempts=k)
         t_train = 0.0
         if code == N.FQ_EXCIT_LEARNED:
             # trained for its threshold: samples of determinants below it are scored with flat weights
@@ -503,7 +504,7 @@ def efficiency_learned(args):
                       "config": {"workload": f"i-FCIQMC mol40: synthetic molecular integrals {norb} spatial orbitals, "
                                              f"{nel} electrons, C1, real_64, initiator 3, tau 1e-3, target "
                                              f"{args.target:g}, {args.eff_cycles} production cycles",
-                                 "proposal_mix": 0.1, "proposal_min_attempts": thresholds[0],
+                                 "proposal_mix": args.proposal_mix, "proposal_min_attempts": thresholds[0],
                                  "train_steps": args.train_steps, "warmup_cycles": args.eff_warmup,
                                  "seeds": seeds},
                       "training": training,
@@ -663,6 +664,8 @@ def main():
                     help="timed cycles of an efficiency mode (default 3000 learned: 300 reports for the "
                          "reblocking; 1000 batched)")
     ap.add_argument("--train-steps", type=int, default=1000)
+    ap.add_argument("--proposal-mix", type=float, default=0.1,
+                    help="learned proposal: fraction of doubles from the uniform fallback")
     ap.add_argument("--eff-seeds", default="23", help="learned efficiency: production seeds (comma-separated); "
                     "each generator runs one independent chain per seed and reports the mean efficiency")
     ap.add_argument("--eff-warmup", type=int, default=500, help="untimed cycles per generator before measuring")
