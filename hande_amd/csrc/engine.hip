@@ -284,6 +284,7 @@ struct fq_engine {
   float* mlp_flat = nullptr;  // 2*nbasis ones: weights of determinants without a row
   const int64_t* p_nruns = nullptr;  // device address of this cycle's run count
   int64_t att_bound = -1;            // bound of the next cycle's attempts (-1: unknown)
+  bool keys_ready = false;           // this cycle's sort keys were made before the spawned length was read
   int64_t run_bound = 0;             // host bound of it (the count itself in replay mode)
   int64_t mlp_tab_rows = 0;
   int64_t* mlp_on = nullptr;   // [cap] network runs on this determinant this cycle
@@ -1217,9 +1218,12 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __rest
 // main list's order with ceil(log2 C(nbasis, nel)) key bits instead of nbasis
 // (34 instead of 52 for 10 electrons in 52 spin orbitals).  binom holds
 // C(p, i) for p < nbasis, 1 <= i <= nel as (low, high) 64-bit halves.
-__global__ void k_rank_keys(const uint64_t* __restrict__ sdet, int64_t ns, int W, int64_t scap,
-                            const uint64_t* __restrict__ binom, int nel, int rank_words,
+// stn (may be null): the spawned-list length is read from the stats on the
+// device (ns is then only the grid's bound).
+__global__ void k_rank_keys(const uint64_t* __restrict__ sdet, int64_t ns, const Stats* __restrict__ stn, int W,
+                            int64_t scap, const uint64_t* __restrict__ binom, int nel, int rank_words,
                             uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  if (stn) ns = stn->nspawn < scap ? stn->nspawn : scap;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   unsigned __int128 rank = 0;
@@ -1237,8 +1241,9 @@ __global__ void k_rank_keys(const uint64_t* __restrict__ sdet, int64_t ns, int W
   vals[i] = static_cast<uint32_t>(i);
 }
 
-__global__ void k_iota_copy_keys(const uint64_t* __restrict__ sdet, int64_t ns, int W, int64_t scap,
-                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+__global__ void k_iota_copy_keys(const uint64_t* __restrict__ sdet, int64_t ns, const Stats* __restrict__ stn, int W,
+                                 int64_t scap, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  if (stn) ns = stn->nspawn < scap ? stn->nspawn : scap;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   for (int w = 0; w < W; ++w) keys[w * scap + i] = sdet[w * scap + i];
@@ -1927,16 +1932,30 @@ int sync_stats(fq_engine* e) {
   return FQ_OK;
 }
 
+// Sort keys and slots of the spawned list: combinatorial ranks, or the
+// determinant words when those are no longer.  stn: read the length on the
+// device (ns is then a bound), so the keys can be made before the host knows it.
+void launch_sort_keys(fq_engine* e, int64_t ns, const Stats* stn) {
+  if (e->rank_words)
+    hipLaunchKernelGGL(k_rank_keys, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->sdet, ns, stn, e->W, e->scap,
+                       e->d_binom, e->sys.nel, e->rank_words, e->kbuf[0], e->vbuf[0]);
+  else
+    hipLaunchKernelGGL(k_iota_copy_keys, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->sdet, ns, stn, e->W,
+                       e->scap, e->kbuf[0], e->vbuf[0]);
+}
+
 // Sort the spawned list (ns entries) and reduce runs; leaves the device
 // address of the run count in p_nruns and a host bound of it in run_bound.
 int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
+  const bool keys_ready = e->keys_ready;  // made by the cycle for this spawned list
+  e->keys_ready = false;
   const int W = e->W;
   const int64_t scap = e->scap;
   int kw = W, kbits = e->key_bits;  // key words and significant bits the sort orders
   uint32_t* perm = nullptr;
   uint64_t* keys = nullptr;
   e->radix_passes = 0;
-  FQ_HIP(hipEventRecord(e->pev[4], e->stream));
+  if (!keys_ready) FQ_HIP(hipEventRecord(e->pev[4], e->stream));  // else recorded before the keys
   if (ns > 0) {
     if (replay) {
       hipLaunchKernelGGL(k_replay_qsort, dim3(1), dim3(1), 0, e->stream, e->sdet, ns, W, scap, e->vbuf[0],
@@ -1947,12 +1966,8 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
       if (e->rank_words) {
         kw = e->rank_words;
         kbits = e->rank_bits;
-        hipLaunchKernelGGL(k_rank_keys, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->sdet, ns, W, scap,
-                           e->d_binom, e->sys.nel, e->rank_words, e->kbuf[0], e->vbuf[0]);
-      } else {
-        hipLaunchKernelGGL(k_iota_copy_keys, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->sdet, ns, W, scap,
-                           e->kbuf[0], e->vbuf[0]);
       }
+      if (!keys_ready) launch_sort_keys(e, ns, nullptr);
       const int64_t nb = ceil_div(ns, kRadixTile);
       if (e->hist_cap < 512 * nb + 1) {
         dfree(e->hist);
@@ -2797,12 +2812,18 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
     rc = spawn_range(0, bound);
     if (rc != FQ_OK) return rc;
     FQ_HIP(hipEventRecord(e->pev[3], st));
+    // the sort keys (at most one spawned entry per attempt) run while the host
+    // waits for the spawned-list length
+    FQ_HIP(hipEventRecord(e->pev[4], st));
+    launch_sort_keys(e, std::min(bound, e->scap), e->d_stats);
+    e->keys_ready = true;
     rc = sync_stats(e);
     if (rc != FQ_OK) return rc;
     if (const int64_t total = e->h_stats.total_attempts; total > bound) {  // the bound cannot be exceeded
       rc = spawn_range(bound, total);
       if (rc == FQ_OK) rc = sync_stats(e);
       if (rc != FQ_OK) return rc;
+      e->keys_ready = false;
     }
     ns = std::min<int64_t>(e->h_stats.nspawn, e->scap);
   }
