@@ -563,7 +563,7 @@ constexpr int kSpawnTile = 4 * kBlock;  // attempts per block tile
 __device__ __forceinline__ int64_t wave_parent_of(const int64_t* __restrict__ att_off, int64_t t, int64_t lo,
                                                   int64_t hi, int lane) {
   while (hi - lo > 63) {
-    const int64_t step = (hi - lo + 63) / 64;
+    const int64_t step = (hi - lo) / 64 + 1;  // 64 * step > hi - lo: pivot 63's stretch reaches hi
     const int64_t p = lo + lane * step;
     const unsigned long long m = __ballot(p <= hi && att_off[p] <= t);  // lanes 0..L (lane 0 holds)
     lo = lo + static_cast<int64_t>(63 - __clzll(m)) * step;
