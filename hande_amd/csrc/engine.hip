@@ -557,48 +557,27 @@ __device__ __forceinline__ int64_t parent_of(const int64_t* __restrict__ att_off
 
 constexpr int kSpawnTile = 4 * kBlock;  // attempts per block tile
 
-// parent_of by one wavefront: last k in [lo, hi] with att_off[k] <= t (given
-// att_off[lo] <= t), narrowing by 64 pivots a step (4 dependent loads for
-// 1.6e7 entries instead of 24).  Every lane of the wave must call it.
-__device__ __forceinline__ int64_t wave_parent_of(const int64_t* __restrict__ att_off, int64_t t, int64_t lo,
-                                                  int64_t hi, int lane) {
-  while (hi - lo > 63) {
-    const int64_t step = (hi - lo) / 64 + 1;  // 64 * step > hi - lo: pivot 63's stretch reaches hi
-    const int64_t p = lo + lane * step;
-    const unsigned long long m = __ballot(p <= hi && att_off[p] <= t);  // lanes 0..L (lane 0 holds)
-    lo = lo + static_cast<int64_t>(63 - __clzll(m)) * step;
-    hi = lo + step - 1 < hi ? lo + step - 1 : hi;
-  }
-  const int64_t p = lo + lane;
-  const unsigned long long m = __ballot(p <= hi && att_off[p] <= t);
-  return lo + (63 - __clzll(m));
-}
-
-// Parents of each spawning tile's first and last attempt, one wavefront per
-// tile, for the batch [t_begin, t_end); t_end clamped to the attempt total
-// att_off[nent] (the host may only know a bound of it); tiles of the batch
-// past the total get a zero count.
-__global__ __launch_bounds__(kBlock) void k_tile_bounds(int64_t nent, const int64_t* __restrict__ att_off,
-                                                        int64_t t_begin, int64_t t_end,
-                                                        int64_t* __restrict__ tile_win, int64_t* __restrict__ tile_cnt) {
+// Parents of each spawning tile's first and last attempt (one thread per
+// tile, so the searches' latencies overlap instead of opening every block).
+// Parent windows of the tiles of the batch [t_begin, t_end), t_end clamped to
+// the attempt total att_off[nent] (the host may only know a bound of it);
+// tiles of the grid past the total get a zero count.
+__global__ void k_tile_bounds(int64_t nent, const int64_t* __restrict__ att_off, int64_t t_begin, int64_t t_end,
+                              int64_t* __restrict__ tile_win, int64_t* __restrict__ tile_cnt) {
   const int64_t nt_batch = (t_end - t_begin + kSpawnTile - 1) / kSpawnTile;  // tiles the host sized for
   const int64_t tot = att_off[nent];
   t_end = t_end < tot ? t_end : tot;
   const int64_t ntiles = t_end > t_begin ? (t_end - t_begin + kSpawnTile - 1) / kSpawnTile : 0;
-  const int64_t tile = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;  // one per wave
-  const int lane = threadIdx.x & 63;
+  const int64_t tile = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (tile >= ntiles) {
-    if (tile < nt_batch && lane == 0) tile_cnt[tile] = 0;
+    if (tile < nt_batch) tile_cnt[tile] = 0;
     return;
   }
   const int64_t t0 = t_begin + tile * kSpawnTile;
   const int64_t t1 = (t0 + kSpawnTile < t_end) ? t0 + kSpawnTile : t_end;
-  const int64_t p0 = wave_parent_of(att_off, t0, 0, nent - 1, lane);
-  const int64_t p1 = wave_parent_of(att_off, t1 - 1, p0, nent - 1, lane);
-  if (lane == 0) {
-    tile_win[2 * tile] = p0;
-    tile_win[2 * tile + 1] = p1;
-  }
+  const int64_t p0 = parent_of(att_off, t0, 0, nent - 1);
+  tile_win[2 * tile] = p0;
+  tile_win[2 * tile + 1] = parent_of(att_off, t1 - 1, p0, nent - 1);
 }
 constexpr int kSpawnWin = 1024;          // parent-window entries staged in LDS
 
@@ -2808,7 +2787,7 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
         const int64_t nbt = te - tb;
         const unsigned g = static_cast<unsigned>(std::min<int64_t>(ceil_div(nbt, kSpawnTile), 1 << 20));
         const int li = e->spawn_launches++;
-        hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(ceil_div(nbt, kSpawnTile) * 64)), dim3(kBlock), 0, st, nent,
+        hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(ceil_div(nbt, kSpawnTile))), dim3(kBlock), 0, st, nent,
                            e->att_off, tb, te, e->tile_win, e->tile_cnt);
         if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li], st));
         const int gen = fqd::gen_of(e->sys.system, e->sys.excit_gen);
@@ -3105,7 +3084,7 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
     const int64_t nbt = te - tb;
     const unsigned g = static_cast<unsigned>(std::min<int64_t>(ceil_div(nbt, kSpawnTile), 1 << 20));
     const int li = e->spawn_launches++;
-    hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(ceil_div(nbt, kSpawnTile) * 64)), dim3(kBlock), 0, st, nent,
+    hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(ceil_div(nbt, kSpawnTile))), dim3(kBlock), 0, st, nent,
                        e->att_off, tb, te, e->tile_win, e->tile_cnt);
     if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li], st));
     auto kern = e->sys.system == FQ_SYS_HUB_K ? k_spawn<fqd::kGenHubK, true> : k_spawn<fqd::kGenMolRenorm, true>;
