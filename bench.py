@@ -38,6 +38,7 @@ import math
 import os
 import sys
 import tempfile
+import threading
 import time
 
 import numpy as np
@@ -248,23 +249,68 @@ def spawn_kernel_bytes(prof, W=1, S=1):
     return prof.nstates_in * (8 * W + 8 * S) + prof.nspawned * (8 * W + 8 * S + 8)
 
 
+BOX_CPU_SHARE = 16  # host cores the GPU box grants one GPU's jobs, whatever os.cpu_count() shows
+
+
 def host_cores():
+    """Threads for the CPU baseline: the cores this process may run on, at most
+    the box's CPU share for one GPU (16), and the count visible, both reported."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return max(1, min(BOX_CPU_SHARE, n)), n
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_flags():
+    """Compiler flags of oracle/liboracle.so (from its Makefile)."""
+    try:
+        with open(os.path.join(ROOT, "oracle", "Makefile")) as f:
+            for line in f:
+                if line.startswith("CXXFLAGS"):
+                    return "g++ " + line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def hash_owner(states, W, nranks):
+    """Owner rank of each determinant: a 64-bit mix of its words mod nranks, so
+    the walkers spread over the ranks as the reference's hashed ownership
+    spreads them over MPI ranks (src/spawning.F90:770-838)."""
+    h = np.zeros(len(states) // W, dtype=np.uint64)
+    words = states.reshape(-1, W)
+    with np.errstate(over="ignore"):
+        for w in range(W):
+            x = words[:, w] ^ h
+            x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            h = x ^ (x >> np.uint64(31))
+    return (h % np.uint64(nranks)).astype(np.int64)
 
 
 def cpu_baseline(lib, system, eng, p, shift, eold, tau, budget_s=10.0):
     """Time the CPU oracle on the GPU's current walker list.
 
     Parallel form: one oracle instance per host core (threads; the oracle's
-    ctypes calls release the GIL), each owning a contiguous slice of the
-    determinants with its own seed, as the reference's MPI ranks each own a
-    share of the list (no inter-rank exchange is timed, which favours the CPU).
-    Serial form: one instance over the whole list on one core.  Each form runs
-    whole cycles until ``budget_s`` seconds have passed (at least one)."""
+    ctypes calls release the GIL), each owning the determinants a hash assigns
+    to it (sorted, with its own seed), as the reference's MPI ranks own a
+    hashed share of the list; no inter-rank exchange is timed, which favours
+    the CPU.  Every instance is created and loaded first; then all start
+    together and run whole cycles until a common deadline ``budget_s`` later,
+    and the rate is the sum of the attempts over the longest instance's wall
+    time.  Serial form: one instance over the whole list on one core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import helpers
     orc = helpers.oracle()
@@ -273,45 +319,59 @@ def cpu_baseline(lib, system, eng, p, shift, eold, tau, budget_s=10.0):
     st, po, da = download_list(lib, eng, W, S)
     nd = len(da)
     walkers = np.abs(po).sum() / 2**31
+    P, visible = host_cores()
 
-    def run(lo, hi, seed, stop_at):
+    def make(idx, seed):
         h = orc.orc_create(system.view, C.byref(p), seed)
-        sl = slice(lo, hi)
-        sst = np.ascontiguousarray(st.reshape(nd, W)[sl]).reshape(-1)
-        spo = np.ascontiguousarray(po.reshape(nd, S)[sl]).reshape(-1)
-        sda = np.ascontiguousarray(da[sl])
+        sst = np.ascontiguousarray(st.reshape(nd, W)[idx]).reshape(-1)
+        spo = np.ascontiguousarray(po.reshape(nd, S)[idx]).reshape(-1)
+        sda = np.ascontiguousarray(da[idx])
         orc.orc_load(h, sst.ctypes.data_as(C.POINTER(C.c_uint64)), spo.ctypes.data_as(C.POINTER(C.c_int64)),
-                     sda.ctypes.data_as(C.POINTER(C.c_double)), hi - lo)
+                     sda.ctypes.data_as(C.POINTER(C.c_double)), len(sda))
+        return h
+
+    def run(h, barrier, budget):
         out = N.FqCycleOut()
         att, cyc = 0, 0
+        if barrier is not None:
+            barrier.wait()
         t0 = time.perf_counter()
+        stop_at = t0 + budget
         while True:
             orc.orc_cycle(h, shift, eold, tau, C.byref(out))
             att += out.nattempts
             cyc += 1
             now = time.perf_counter()
-            if now >= stop_at or cyc >= 50:
+            if now >= stop_at:
                 break
-        orc.orc_destroy(h)
         return att, cyc, now - t0
 
-    P = host_cores()
-    bounds = [nd * k // P for k in range(P + 1)]
-    t0 = time.perf_counter()
+    owner = hash_owner(st, W, P)
+    hs = [make(np.nonzero(owner == k)[0], 7 + k) for k in range(P)]
+    barrier = threading.Barrier(P)
     with concurrent.futures.ThreadPoolExecutor(max_workers=P) as ex:
-        futs = [ex.submit(run, bounds[k], bounds[k + 1], 7 + k, t0 + budget_s) for k in range(P)]
+        futs = [ex.submit(run, hs[k], barrier, budget_s) for k in range(P)]
         res = [f.result() for f in futs]
-    wall_p = time.perf_counter() - t0
+    for h in hs:
+        orc.orc_destroy(h)
+    wall_p = max(r[2] for r in res)
     att_p = sum(r[0] for r in res)
     cyc_p = min(r[1] for r in res)
-    att_1, cyc_1, wall_1 = run(0, nd, 7, time.perf_counter() + budget_s)
+    h1 = make(np.arange(nd), 7)
+    att_1, cyc_1, wall_1 = run(h1, None, budget_s)
+    orc.orc_destroy(h1)
     return {"value": att_p / wall_p, "unit": "spawn attempts/s", "cores": P, "kind": "port",
-            "sample": f"{P} oracle instances (serial C++ restatement of the reference cycle), one per host core, "
-                      f"each on 1/{P} of the {nd} determinants of the same {walkers:.3g}-walker state, "
-                      f">= {cyc_p} whole cycles each, {att_p} attempts in {wall_p:.1f} s",
+            "sample": f"{P} oracle instances (serial C++ restatement of the reference cycle), one thread each, "
+                      f"each owning a hashed 1/{P} share of the {nd} determinants of the same {walkers:.3g}-walker "
+                      f"state, all running whole cycles for {budget_s:g} s from a common start "
+                      f"(>= {cyc_p} cycles each), {att_p} attempts in {wall_p:.1f} s",
+            "cpu_model": host_cpu_model(), "logical_cpus_visible": visible,
+            "cpu_share": f"{BOX_CPU_SHARE} cores (the GPU box's share for one GPU)",
+            "compiler": oracle_flags(),
             "single_core": {"value": att_1 / wall_1, "cores": 1,
                             "sample": f"one instance over all {nd} determinants, {cyc_1} cycles, "
-                                      f"{att_1} attempts in {wall_1:.1f} s"}}
+                                      f"{att_1} attempts in {wall_1:.1f} s"},
+            "parallel_efficiency": (att_p / wall_p) / (P * att_1 / wall_1)}
 
 
 def copy_probe(mib):

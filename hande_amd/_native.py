@@ -11,9 +11,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfqmc.so")
 
 FQ_OK = 0
+FQ_ERR_OVERFLOW = -8
 FQ_ERR = {
     -1: "FQ_ERR_ARG", -2: "FQ_ERR_NOMEM", -3: "FQ_ERR_DEVICE", -4: "FQ_ERR_RNG_EXHAUSTED",
-    -5: "FQ_ERR_CAPACITY", -6: "FQ_ERR_NO_DEVICE", -7: "FQ_ERR_BIASED",
+    -5: "FQ_ERR_CAPACITY", -6: "FQ_ERR_NO_DEVICE", -7: "FQ_ERR_BIASED", -8: "FQ_ERR_OVERFLOW",
 }
 FQ_SYS_HUB_K, FQ_SYS_READ_IN = 1, 2
 FQ_EXCIT_RENORM, FQ_EXCIT_NO_RENORM, FQ_EXCIT_HEAT_BATH, FQ_EXCIT_LEARNED = 0, 1, 2, 3

@@ -160,8 +160,23 @@ def fciqmc(sys, qmc, fciqmc=None, rng_mode="replay", device=0, lib=None, **_):
     max_rows = st.nreports + 1
     rows = (N.FqhReportRow * max_rows)()
     nrows = C.c_int()
-    N.check(lib.fqh_fciqmc(sys.handle, C.byref(st), rows, max_rows, C.byref(nrows)), "fciqmc")
-    return rows_to_table(rows, nrows.value, 2 if st.replica_tricks else 1)
+    rc = lib.fqh_fciqmc(sys.handle, C.byref(st), rows, max_rows, C.byref(nrows))
+    table = rows_to_table(rows, nrows.value, 2 if st.replica_tricks else 1)
+    if rc == N.FQ_ERR_OVERFLOW:
+        raise ListOverflow(table)
+    N.check(rc, "fciqmc")
+    return table
+
+
+class ListOverflow(RuntimeError):
+    """The spawned or main walker list filled up and the report loop stopped
+    early, as the reference stops on its spawn%error / psip_list%error flags
+    (src/fciqmc.f90:415-422).  ``table`` holds the rows reported until then."""
+
+    def __init__(self, table):
+        super().__init__(f"walker list overflow: report loop stopped after {len(table)} rows "
+                         "(raise state_size / spawned_state_size)")
+        self.table = table
 
 
 def load_deck(text, base_dir=".", lib=None):

@@ -120,6 +120,7 @@ extern "C" int fqh_fciqmc(const fqh_system* hs, const fqh_qmc_in* in, fqh_report
     r.time_per_cycle = tcyc;
   };
   emit(0, 0, 0.0, 0.0);
+  bool overflowed = false;
   for (int ireport = 1; ireport <= in->nreports; ++ireport) {
     double eproj_old[FQ_MAX_SPACES] = {0, 0};
     for (int s = 0; s < S; ++s) eproj_old[s] = (std::fabs(d0[s]) < 2.2250738585072014e-308) ? 0.0 : proj[s] / d0[s];
@@ -180,10 +181,18 @@ extern "C" int fqh_fciqmc(const fqh_system* hs, const fqh_qmc_in* in, fqh_report
       }
     }
     emit(static_cast<int64_t>(ireport) * in->mc_cycles, nspawn_events, rspawn, tcyc);
-    if (error) break;
+    if (error) {
+      overflowed = true;
+      break;
+    }
   }
   fq_destroy(e);
   *nrows = nr;
+  // A full spawned or main list stops the report loop after the current
+  // report, as the reference's sticky spawn%error / psip_list%error flags do
+  // (src/fciqmc.f90:415-422); the caller sees it as FQ_ERR_OVERFLOW with the
+  // rows emitted so far.
+  if (rc == FQ_OK && overflowed) rc = FQ_ERR_OVERFLOW;
   return rc;
 }
 

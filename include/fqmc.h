@@ -37,6 +37,8 @@ extern "C" {
 #define FQ_ERR_NO_DEVICE (-6)      /* no usable GPU */
 #define FQ_ERR_BIASED (-7)         /* heat-bath tables cannot reach every allowed single excitation
                                       (the stop_all of src/excit_gen_heat_bath_mol.F90:185-202) */
+#define FQ_ERR_OVERFLOW (-8)       /* report loop stopped early: the spawned or main list filled up
+                                      (spawn%error / psip_list%error, src/fciqmc.f90:415-422) */
 
 /* ---- enumerations ------------------------------------------------------ */
 enum fq_system_kind { FQ_SYS_HUB_K = 1, FQ_SYS_READ_IN = 2 };
