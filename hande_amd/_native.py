@@ -12,9 +12,11 @@ LIB_PATH = os.path.join(HERE, "libfqmc.so")
 
 FQ_OK = 0
 FQ_ERR_OVERFLOW = -8
+FQ_ERR_SCRATCH = -9
 FQ_ERR = {
     -1: "FQ_ERR_ARG", -2: "FQ_ERR_NOMEM", -3: "FQ_ERR_DEVICE", -4: "FQ_ERR_RNG_EXHAUSTED",
     -5: "FQ_ERR_CAPACITY", -6: "FQ_ERR_NO_DEVICE", -7: "FQ_ERR_BIASED", -8: "FQ_ERR_OVERFLOW",
+    -9: "FQ_ERR_SCRATCH",
 }
 FQ_SYS_HUB_K, FQ_SYS_READ_IN = 1, 2
 FQ_EXCIT_RENORM, FQ_EXCIT_NO_RENORM, FQ_EXCIT_HEAT_BATH, FQ_EXCIT_LEARNED = 0, 1, 2, 3
@@ -104,6 +106,7 @@ def _declare(lib):
         "fq_push_rng": (C.c_int, [vp, P(f64), i64]),
         "fq_rng_available": (C.c_int, [vp, P(i64)]),
         "fq_cycle": (C.c_int, [vp, P(f64), P(f64), f64, P(FqCycleOut)]),
+        "fq_cycle_noalloc": (C.c_int, [vp, P(f64), P(f64), f64, P(FqCycleOut)]),
         "fq_annihilate": (C.c_int, [vp, P(u64), P(i64), P(i64), i64, P(FqCycleOut)]),
         "fq_nstates": (C.c_int, [vp, P(i64)]),
         "fq_download": (C.c_int, [vp, P(u64), P(i64), P(f64), i64, P(i64)]),

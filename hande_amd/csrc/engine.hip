@@ -42,6 +42,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "device_common.hpp"
@@ -49,6 +50,26 @@
 #include "mlp.hpp"
 
 using fqd::Sys;
+
+// ---------------------------------------------------------------------------
+// Fixed scratch of fq_cycle_noalloc, for callers that cannot allocate device
+// memory.  One region per scratch slot (int64 elements): prefix-sum partials
+// (slots 0-3) and the radix sort's tile histograms and their offsets (slots 4
+// and 5).  Every device holds its own copy until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns with that
+// device current (the array's name is a host-side stand-in).  Calls that
+// overlap on one device share it and race.
+// ---------------------------------------------------------------------------
+constexpr int kScratchSlots = 6;
+constexpr int64_t kScratchSlotElems[kScratchSlots] = {int64_t{1} << 16, int64_t{1} << 16, int64_t{1} << 17,
+                                                      int64_t{1} << 10, int64_t{3600000}, int64_t{3600000}};
+__host__ __device__ constexpr int64_t scratch_slot_offset(int slot) {
+  int64_t off = 0;
+  for (int k = 0; k < slot; ++k) off += (kScratchSlotElems[k] + 31) / 32 * 32;  // 256-byte aligned regions
+  return off;
+}
+constexpr int64_t kScratchElems = scratch_slot_offset(kScratchSlots);
+static __device__ __align__(256) int64_t scratch[kScratchElems];
 
 namespace {
 
@@ -349,6 +370,10 @@ struct fq_engine {
   // scan scratch
   std::vector<int64_t*> scan_tmp;
   std::vector<int64_t> scan_tmp_cap;
+  // fq_cycle_noalloc: scratch slots come from the device's fixed array
+  int64_t* fixed_scratch = nullptr;  // device address of `scratch` on this engine's device (null: unavailable)
+  bool noalloc = false;              // inside fq_cycle_noalloc
+  bool scratch_short = false;        // a request exceeded its fixed slot
   // replay stream
   double* rng_buf = nullptr;
   int64_t rng_cap = 0, rng_pos = 0, rng_end = 0;
@@ -387,8 +412,40 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
-// Scratch buffer `slot` of at least `need` elements (grown on demand).
+// Device address of the fixed scratch array on `device` (current), looked up
+// once per device; null if the lookup fails.
+int64_t* fixed_scratch_address(int device) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static int64_t* addr[kMaxDevices] = {};
+  static bool looked[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!looked[device]) {
+    void* p = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess) {
+      fprintf(stderr, "fqmc: hipGetSymbolAddress(scratch) on device %d failed: %s\n", device,
+              hipGetErrorString(err));
+      p = nullptr;
+    }
+    addr[device] = static_cast<int64_t*>(p);
+    looked[device] = true;
+  }
+  return addr[device];
+}
+
+// Scratch buffer `slot` of at least `need` elements (grown on demand; inside
+// fq_cycle_noalloc, the slot's region of the fixed array, or null when the
+// request exceeds it).
 int64_t* scan_scratch(fq_engine* e, int slot, int64_t need) {
+  if (e->noalloc) {
+    if (slot >= kScratchSlots || need > kScratchSlotElems[slot]) {
+      e->scratch_short = true;
+      return nullptr;
+    }
+    return e->fixed_scratch + scratch_slot_offset(slot);
+  }
   while (static_cast<int>(e->scan_tmp.size()) <= slot) {
     e->scan_tmp.push_back(nullptr);
     e->scan_tmp_cap.push_back(0);
@@ -1969,12 +2026,19 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
       }
       if (!keys_ready) launch_sort_keys(e, ns, nullptr);
       const int64_t nb = ceil_div(ns, kRadixTile);
-      if (e->hist_cap < 512 * nb + 1) {
+      int64_t *hist = e->hist, *hist_scan = e->hist_scan;
+      if (e->noalloc) {
+        hist = scan_scratch(e, 4, 512 * nb + 1);
+        hist_scan = scan_scratch(e, 5, 512 * nb + 1);
+        if (!hist || !hist_scan) return FQ_ERR_NOMEM;
+      } else if (e->hist_cap < 512 * nb + 1) {
         dfree(e->hist);
         dfree(e->hist_scan);
         if (dalloc(&e->hist, 512 * nb + 1) != FQ_OK || dalloc(&e->hist_scan, 512 * nb + 1) != FQ_OK)
           return FQ_ERR_NOMEM;
         e->hist_cap = 512 * nb + 1;
+        hist = e->hist;
+        hist_scan = e->hist_scan;
       }
       int src = 0;
       for (int w = 0; w < kw; ++w) {
@@ -1983,23 +2047,23 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
         const int passes = std::max(1, (bits + 8) / 9), n9 = std::max(0, std::min(passes, bits - 8 * passes));
         for (int pidx = 0, shift = 0; pidx < passes; ++pidx) {
           const int db = pidx < n9 ? 9 : 8;
-          auto hist = db == 9 ? k_radix_hist<9> : k_radix_hist<8>;
-          hipLaunchKernelGGL(hist, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src] + w * scap, ns, shift, e->hist);
+          auto khist = db == 9 ? k_radix_hist<9> : k_radix_hist<8>;
+          hipLaunchKernelGGL(khist, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src] + w * scap, ns, shift, hist);
           const int64_t rows = std::max<int64_t>(16, ceil_div(nb, kHistChunks)), nchunk = ceil_div(nb, rows);
           int64_t* csum = scan_scratch(e, 2, nchunk * 512);
           int64_t* dtot = scan_scratch(e, 3, 512);
           if (!csum || !dtot) return FQ_ERR_NOMEM;
           const dim3 cg(static_cast<unsigned>((1 << db) / kBlock), static_cast<unsigned>(nchunk));
-          hipLaunchKernelGGL(db == 9 ? k_hist_colsum<9> : k_hist_colsum<8>, cg, dim3(kBlock), 0, e->stream, e->hist, nb,
+          hipLaunchKernelGGL(db == 9 ? k_hist_colsum<9> : k_hist_colsum<8>, cg, dim3(kBlock), 0, e->stream, hist, nb,
                              rows, csum);
           hipLaunchKernelGGL(db == 9 ? k_hist_chunkscan<9> : k_hist_chunkscan<8>, dim3(1u << db), dim3(kBlock), 0,
                              e->stream, nchunk, csum, dtot);
-          hipLaunchKernelGGL(db == 9 ? k_hist_colscan<9> : k_hist_colscan<8>, cg, dim3(kBlock), 0, e->stream, e->hist,
-                             nb, rows, csum, dtot, e->hist_scan);
+          hipLaunchKernelGGL(db == 9 ? k_hist_colscan<9> : k_hist_colscan<8>, cg, dim3(kBlock), 0, e->stream, hist,
+                             nb, rows, csum, dtot, hist_scan);
           auto scatter = kw == 1 ? (db == 9 ? k_radix_scatter<1, 9> : k_radix_scatter<1, 8>)
                                  : (db == 9 ? k_radix_scatter<2, 9> : k_radix_scatter<2, 8>);
           hipLaunchKernelGGL(scatter, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src], e->vbuf[src], ns, w, shift,
-                             e->hist, e->hist_scan, e->kbuf[1 - src], e->vbuf[1 - src], scap);
+                             hist, hist_scan, e->kbuf[1 - src], e->vbuf[1 - src], scap);
           src = 1 - src;
           shift += db;
           ++e->radix_passes;
@@ -2552,6 +2616,7 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
     delete e;
     return FQ_ERR_DEVICE;
   }
+  e->fixed_scratch = fixed_scratch_address(e->device);
   e->par = *params;
   e->W = sys->nwords;
   e->S = params->nspaces;
@@ -2706,7 +2771,7 @@ int fq_rng_available(fq_engine* e, int64_t* n) {
   return FQ_OK;
 }
 
-int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
+static int cycle_body(fq_engine* e, const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
   if (!e || !shift || !eproj_old || !out || e->batched) return FQ_ERR_ARG;
   FQ_HIP(hipSetDevice(e->device));
   const bool replay = e->par.rng_mode == FQ_RNG_REPLAY;
@@ -2715,7 +2780,10 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
   hipStream_t st = e->stream;
   const uint32_t cyc = static_cast<uint32_t>(e->cycle_index++);
   const double sh1 = e->S > 1 ? shift[1] : 0.0, e1 = e->S > 1 ? eproj_old[1] : 0.0;
-  FQ_HIP(hipMemsetAsync(e->d_stats, 0, sizeof(Stats), st));
+  // per-cycle counters cleared by a kernel (no memset node in the stream)
+  static_assert(sizeof(Stats) % sizeof(int64_t) == 0, "Stats is cleared as int64 words");
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, reinterpret_cast<int64_t*>(e->d_stats),
+                     static_cast<int64_t>(sizeof(Stats) / sizeof(int64_t)));
   FQ_HIP(hipEventRecord(e->ev[0], st));
   FQ_HIP(hipEventRecord(e->pev[0], st));
   int64_t rng_pos = e->rng_pos;
@@ -2881,6 +2949,30 @@ int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double 
   }
   fill_out(e, out, used);
   return FQ_OK;
+}
+
+int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
+  return cycle_body(e, shift, eproj_old, tau, out);
+}
+
+int fq_cycle_noalloc(fq_engine* e, const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
+  if (!e || !shift || !eproj_old || !out || e->batched) return FQ_ERR_ARG;
+  // replay streams and the learned proposal's weight table grow on demand
+  if (e->par.rng_mode != FQ_RNG_PHILOX || e->par.excit_gen == FQ_EXCIT_LEARNED) return FQ_ERR_ARG;
+  if (!e->fixed_scratch) return FQ_ERR_DEVICE;
+  e->noalloc = true;
+  e->scratch_short = false;
+  const int rc = cycle_body(e, shift, eproj_old, tau, out);
+  e->noalloc = false;
+  if (e->scratch_short) {
+    fprintf(stderr, "fqmc: fq_cycle_noalloc: this cycle needs more scratch than the fixed %lld bytes hold "
+            "(spawned list above %lld entries); the walker list is unchanged\n",
+            static_cast<long long>(sizeof(int64_t) * kScratchElems),
+            static_cast<long long>((kScratchSlotElems[4] - 1) / 512 * kRadixTile));
+    (void)hipStreamSynchronize(e->stream);
+    return FQ_ERR_SCRATCH;
+  }
+  return rc;
 }
 
 int fq_annihilate(fq_engine* e, const uint64_t* sstates, const int64_t* spops, const int64_t* sflags, int64_t ns,

@@ -39,6 +39,7 @@ extern "C" {
                                       (the stop_all of src/excit_gen_heat_bath_mol.F90:185-202) */
 #define FQ_ERR_OVERFLOW (-8)       /* report loop stopped early: the spawned or main list filled up
                                       (spawn%error / psip_list%error, src/fciqmc.f90:415-422) */
+#define FQ_ERR_SCRATCH (-9)        /* fq_cycle_noalloc: the cycle needs more than the fixed scratch holds */
 
 /* ---- enumerations ------------------------------------------------------ */
 enum fq_system_kind { FQ_SYS_HUB_K = 1, FQ_SYS_READ_IN = 2 };
@@ -159,6 +160,19 @@ int fq_rng_available(fq_engine *e, int64_t *n);
  * src/fciqmc.f90:278,370).  In replay mode, returns FQ_ERR_RNG_EXHAUSTED if
  * the stream ran dry; the engine state is then undefined. */
 int fq_cycle(fq_engine *e, const double *shift, const double *eproj_old, double tau, fq_cycle_out *out);
+
+/* fq_cycle without device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  fq_cycle
+ * grows its prefix-sum and radix-histogram scratch on demand; this variant
+ * takes them from a fixed device array of the library (about 58 MiB per
+ * device, held until the process ends) and calls no hipMalloc or hipFree.
+ * Limits: production RNG, generators other than the learned proposal (FQ_ERR_ARG
+ * otherwise), at most 14,399,488 spawned entries in the cycle (FQ_ERR_SCRATCH,
+ * the walker list then unchanged).  The array is shared by every engine on a
+ * device: calls that overlap on one device race.  Results equal fq_cycle's bit
+ * for bit. */
+int fq_cycle_noalloc(fq_engine *e, const double *shift, const double *eproj_old, double tau, fq_cycle_out *out);
 
 /* Annihilate a caller-supplied spawned list into the main list
  * (direct_annihilation without the spawning loop; src/annihilation.f90:9-214
