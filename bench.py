@@ -50,6 +50,9 @@ from hande_amd import _native as N  # noqa: E402
 from hande_amd import api, synthetic  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# Vector ALU issue peak of one MI355X: 256 CUs x 4 SIMDs x 16 lanes per clock
+# at 2.4 GHz (a wave64 full-rate 32-bit instruction takes 4 cycles on a SIMD).
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 
 
 def log(*a):
@@ -164,32 +167,46 @@ class Propagator:
         return out
 
 
-def sort_key_words(nbasis, nel):
-    """64-bit words of the production sort key: the combinatorial rank of a
-    determinant needs ceil(log2 C(nbasis, nel)) bits."""
-    return 1 if math.comb(nbasis, nel) <= (1 << 64) else 2
+def sort_key_words(nbasis, nel, spawn_capacity):
+    """64-bit words of the sort key: the combinatorial rank of a determinant
+    (ceil(log2 C(nbasis, nel)) bits) above its spawned-list slot
+    (ceil(log2 spawn_capacity) bits)."""
+    bits = max(1, (math.comb(nbasis, nel) - 1).bit_length()) + max(1, (spawn_capacity - 1).bit_length())
+    return 1 if bits <= 64 else 2
 
 
-def phase_bytes(prof, W=1, S=1, kw=None):
-    """Algorithmic HBM bytes of each phase of one cycle (compulsory traffic);
-    kw: words of the sort key (W when not given)."""
-    kw = W if kw is None else kw
+def entry_bytes(W, S):
+    """Bytes of a main-list entry (determinant words, populations, K_ii) and of
+    a spawned entry (determinant words, populations, flag)."""
+    return 8 * W + 8 * S + 8, 8 * W + 8 * S + 8
+
+
+def compulsory_bytes(prof, W=1, S=1):
+    """The hot path's compulsory HBM bytes per cycle: the main list read by the
+    spawning/death pass and by annihilation and written once, the spawned
+    list written by spawning and read by annihilation
+    (3 N_d B_m + 2 N_s B_s with the written list N_d' B_m)."""
+    bm, bs = entry_bytes(W, S)
+    nd, ns, nout = prof.nstates_in, prof.nspawned, prof.nstates_out
+    return {"cycle": 2 * nd * bm + nout * bm + 2 * ns * bs,
+            "annihilation": nd * bm + nout * bm + ns * bs}
+
+
+def phase_bytes(prof, W=1, S=1, kw=1):
+    """Algorithmic HBM bytes of each phase of one cycle as the kernels move
+    them; kw: words of the sort key."""
     nd, ns, nr, nout = prof.nstates_in, prof.nspawned, prof.nruns, prof.nstates_out
-    bm = 8 * W + 8 * S + 8          # main-list entry: det words, pops, K_ii
-    bs = 8 * W + 8 * S + 8          # spawned entry: det words, pops, flag
+    bm, bs = entry_bytes(W, S)
     return {
         # read det/pop/K_ii, write post-death pops, attempt counts, flags
         "det_pass": nd * (bm + 8 * S + 8 * S + 8),
-        # parents (det, pop, flag) and attempt offsets once, spawned entries written
-        "spawn": nd * (8 * W + 8 * S + 8 + 8 * S) + ns * bs,
-        # rank keys written once; each 8-bit LSD pass reads keys+indices twice
-        # (histogram, scatter) and writes them once
-        "sort": ns * (8 * W + 8 * kw + 4) + prof.radix_passes * ns * (3 * (8 * kw + 4)),
-        # run heads: sorted keys and permutation read, pops/flags gathered and
-        # written in sorted order; reduction: those read back, run starts, the
-        # run's determinant, unique runs written
-        "reduce": ns * (8 * kw + 4 + 2 * (8 * S + 8)) + ns * (8 * S + 8) + nr * (16 + 4 + 8 * W)
-                  + nr * (8 * W + 8 * S + 8 + 8),
+        # parents (det, pop, flag) and attempt offsets once, spawned entries and their sort keys written
+        "spawn": nd * (8 * W + 8 * S + 8 + 8 * S) + ns * (bs + 8 * kw),
+        # one read of the keys for every pass's digit counts; each pass reads and writes them once
+        "sort": ns * 8 * kw + prof.radix_passes * ns * 2 * 8 * kw,
+        # sorted keys read (and each key's neighbour), each entry's populations
+        # and flag gathered by slot, each run's determinant gathered, runs written
+        "reduce": ns * (2 * 8 * kw + 8 * S + 8) + nr * (8 * W) + nr * (8 * W + 8 * S + 8 + 8),
         # main list read twice (merge search ~ touched lines ignored) and written once, runs read
         "merge": nd * (2 * bm + 3 * 8) + nout * bm + nr * (8 * W + 8 * S + 8 + 8 * 3),
     }
@@ -239,7 +256,8 @@ def issue_summary(workload):
     c = v["raw_per_launch"]
     return {"valu_busy": c["SQ_ACTIVE_INST_VALU"] * 4 / (1024 * c["GRBM_GUI_ACTIVE"] / 8),
             "valu_lane_utilization": c["SQ_THREAD_CYCLES_VALU"] / (64 * c["SQ_ACTIVE_INST_VALU"]),
-            "valu_insts_per_launch": c["SQ_INSTS_VALU"], "source": "profiles/k_spawn_mol26_valu_raw.json"}
+            "valu_insts_per_launch": c["SQ_INSTS_VALU"], "lane_ops_per_launch": c["SQ_THREAD_CYCLES_VALU"],
+            "source": "profiles/k_spawn_mol26_valu_raw.json"}
 
 
 def spawn_kernel_bytes(prof, W=1, S=1):
@@ -414,19 +432,21 @@ def prepare(lib, system, target, seed, cap, equil_tau=1e-3, hold=300, max_cycles
     return st, po, da, prop.shift[0], prop.eold[0]
 
 
-def production(lib, eng, target, state, tau, cycles, warmup, report=10, prof=None):
+def production(lib, eng, target, state, tau, cycles, warmup, report=10, prof=None, pull=0.0):
     """Warm-up then timed production cycles from a prepared state; returns the
-    per-report (sum H_0j N_j, N_0) series, wall time and attempts.  prof (a
-    dict) receives per-cycle means of the learned proposal's time and rows and
-    of the cycle time."""
+    per-report (sum H_0j N_j, N_0) series, the per-report walker counts, wall
+    time and attempts.  pull > 0 holds every generator's population at the
+    target (an integral term on the shift), so generators are compared at
+    matched walker numbers.  prof (a dict) receives per-cycle means of the
+    learned proposal's time and rows and of the cycle time."""
     st, po, da, shift, eold = state
     N.check(lib.fq_load(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)), po.ctypes.data_as(C.POINTER(C.c_int64)),
                         da.ctypes.data_as(C.POINTER(C.c_double)), len(da)), "fq_load")
-    prop = Propagator(lib, eng, target)
+    prop = Propagator(lib, eng, target, pull=pull)
     prop.shift[0], prop.eold[0], prop.vary = shift, eold, True
     for _ in range(warmup):
         prop.cycle(tau)
-    proj, d0, attempts = [], [], 0
+    proj, d0, walkers, attempts = [], [], [], 0
     accp = accd = 0.0
     t0 = time.perf_counter()
     pr = N.FqProfileOut()
@@ -442,6 +462,7 @@ def production(lib, eng, target, state, tau, cycles, warmup, report=10, prof=Non
         if (i + 1) % report == 0:
             proj.append(accp)
             d0.append(accd)
+            walkers.append(out.nparticles[0])
             accp = accd = 0.0
         if (i + 1) % 500 == 0:
             log(f"[eff] production cycle {i + 1}: N={out.nparticles[0]:.4g}, {time.perf_counter() - t0:.1f}s")
@@ -449,20 +470,7 @@ def production(lib, eng, target, state, tau, cycles, warmup, report=10, prof=Non
     if prof is not None:
         m = acc_prof / max(cycles, 1)
         prof.update(proposal_ms=float(m[0]), proposal_rows=float(m[1]), cycle_ms=float(m[2]))
-    return np.array(proj), np.array(d0), wall, attempts, prop.last
-
-
-EFF_BATCHES = 6
-
-
-def batch_means_ratio(num, den, nbatch):
-    """Ratio of means sum(num)/sum(den) and its standard error from the spread
-    of the ratios of nbatch contiguous batches (batch means; the same batches
-    for every generator compared)."""
-    n = len(num) // nbatch * nbatch
-    a = np.asarray(num[:n], float).reshape(nbatch, -1).sum(axis=1)
-    b = np.asarray(den[:n], float).reshape(nbatch, -1).sum(axis=1)
-    return a.sum() / b.sum(), float(np.std(a / b, ddof=1) / math.sqrt(nbatch))
+    return np.array(proj), np.array(d0), np.array(walkers), wall, attempts, prop.last
 
 
 def attempt_coverage(pops, rf, thresholds):
@@ -476,6 +484,12 @@ def attempt_coverage(pops, rf, thresholds):
 
 
 def efficiency_learned(args):
+    """E_proj efficiency 1/(sigma^2 t) of the generators on mol40 at matched
+    walker numbers (every run's shift holds the population at the target).
+    sigma is the Flyvbjerg--Petersen reblocked error of the ratio estimator
+    at the Wolff/Lee optimal block (pyblock/pyhande semantics); a run whose
+    reblocking finds no optimal block reports no efficiency, and ratios are
+    given only between runs that both converged."""
     from hande_amd import reblock
     lib = N.lib()
     norb, nel, seed = synthetic.MOLECULAR_CONFIGS["mol40"]
@@ -483,22 +497,21 @@ def efficiency_learned(args):
     system = api.read_in(int_file=synthetic.write_fcidump(os.path.join(tmp, "FCIDUMP"), norb, nel, seed), nel=nel, ms=0)
     cap = int(max(6 * args.target, 1 << 20))
     state = prepare(lib, system, args.target, 11, cap)
-    thresholds = [int(x) for x in str(args.min_attempts).split(",")]
-    coverage = attempt_coverage(state[1], float(1 << 31), sorted(set([1, 2, 4, 8, 16, 32, 64] + thresholds)))
+    k = int(str(args.min_attempts).split(",")[0])
+    coverage = attempt_coverage(state[1], float(1 << 31), sorted({1, 4, 16, 64, k}))
     log(f"[eff] walker share on determinants with >= K walkers: "
-        + ", ".join(f"K={k}: {v['walker_share']:.3f} ({v['determinants']})" for k, v in coverage.items()))
-    res = {}
-    runs = [("renorm", N.FQ_EXCIT_RENORM, 0), ("heat_bath", N.FQ_EXCIT_HEAT_BATH, 0)]
-    runs += [(f"learned_k{k}", N.FQ_EXCIT_LEARNED, k) for k in thresholds]
-    training = {}
-    seeds = [int(x) for x in str(args.eff_seeds).split(",")]
-    per_seed = {}
-    for (gen, code, k), sd in ((r, sd) for sd in seeds for r in runs):
+        + ", ".join(f"K={kk}: {v['walker_share']:.3f} ({v['determinants']})" for kk, v in coverage.items()))
+    gens = {"renorm": (N.FQ_EXCIT_RENORM, 0), "heat_bath": (N.FQ_EXCIT_HEAT_BATH, 0),
+            "learned": (N.FQ_EXCIT_LEARNED, k)}
+    wanted = [g for g in str(args.eff_gens).split(",") if g]
+    res, training = {}, {}
+    sd = int(str(args.eff_seeds).split(",")[0])
+    for gen in wanted:
+        code, kk = gens[gen]
         eng, _p = make_engine(lib, system, args.target, sd, cap, cap, excit_gen=code, mix=args.proposal_mix,
-                              min_attempts=k)
+                              min_attempts=kk)
         t_train = 0.0
         if code == N.FQ_EXCIT_LEARNED:
-            # trained for its threshold: samples of determinants below it are scored with flat weights
             st, po, da = state[:3]
             N.check(lib.fq_load(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)),
                                 po.ctypes.data_as(C.POINTER(C.c_int64)), da.ctypes.data_as(C.POINTER(C.c_double)),
@@ -508,67 +521,52 @@ def efficiency_learned(args):
             N.check(lib.fq_mlp_train(eng, 16384, args.train_steps, 1e-3, 5,
                                      losses.ctypes.data_as(C.POINTER(C.c_double))), "fq_mlp_train")
             t_train = time.perf_counter() - t1
-            training[f"{gen}_seed{sd}"] = {"seconds": t_train, "loss_first10": float(losses[:10].mean()),
-                                           "loss_last10": float(losses[-10:].mean())}
-            log(f"[eff] {gen} seed {sd}: trained {args.train_steps} steps in {t_train:.1f}s: loss "
+            training = {"seconds": t_train, "loss_first10": float(losses[:10].mean()),
+                        "loss_last10": float(losses[-10:].mean())}
+            log(f"[eff] learned: trained {args.train_steps} steps in {t_train:.1f}s: loss "
                 f"{losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}")
-        # each generator settles its own (initiator) stationary state before the timed cycles
         prof = {}
-        proj, d0, wall, att, last = production(lib, eng, args.target, state, 1e-3, args.eff_cycles, args.eff_warmup,
-                                               prof=prof)
+        proj, d0, walkers, wall, att, last = production(lib, eng, args.target, state, 1e-3, args.eff_cycles,
+                                                        args.eff_warmup, prof=prof, pull=1.0)
         lib.fq_destroy(eng)
         rb = reblock.projected_energy(proj, d0)
-        e_bm, sigma = batch_means_ratio(proj, d0, EFF_BATCHES)
         t = wall + t_train
-        r = {"seed": sd, "E_proj": system.H00 + e_bm, "sigma": sigma, "wall_s": wall, "train_s": t_train,
-                    "efficiency": reblock.efficiency(sigma, t), "attempts_per_s": att / wall,
-                    "sigma_method": f"batch means: {EFF_BATCHES} contiguous batches of the production cycles",
-                    "reblock": {"E_proj": system.H00 + rb["energy"], "sigma": rb["error"], "level": rb["level"],
-                                "converged": rb["converged"]},
-                    "walkers": last.nparticles[0], "determinants": int(last.nstates), "cycle_ms": prof["cycle_ms"]}
+        r = {"E_proj": system.H00 + rb["energy"], "sigma": rb["error"], "converged": rb["converged"],
+             "reblock_level": rb["level"], "reports": len(proj), "wall_s": wall, "train_s": t_train,
+             "efficiency": reblock.efficiency(rb["error"], t) if rb["converged"] else None,
+             "attempts_per_s": att / wall, "walkers_mean": float(walkers.mean()),
+             "walkers_sd": float(walkers.std()), "determinants": int(last.nstates),
+             "cycle_ms": prof["cycle_ms"], "seed": sd}
         if code == N.FQ_EXCIT_LEARNED:
-            r.update(min_attempts=k, proposal_ms=prof["proposal_ms"], network_rows=prof["proposal_rows"])
-        per_seed.setdefault(gen, []).append(r)
-        log(f"[eff] {gen} seed {sd}: E_proj {r['E_proj']:.6f} +- {sigma:.6f}, {wall:.1f}s "
-            f"({prof['cycle_ms']:.2f} ms/cycle, proposal {prof['proposal_ms']:.2f} ms on {prof['proposal_rows']:.0f} "
-            f"rows), efficiency {r['efficiency']:.4g} (reblocking converged: {rb['converged']})")
-    # per generator: the mean over seeds of each run's efficiency (independent chains)
-    # (a chain whose batch means are not all finite gives no efficiency and is left out)
-    for gen, rs in per_seed.items():
-        effs = np.array([r["efficiency"] for r in rs], dtype=float)
-        ok = np.isfinite(effs)
-        res[gen] = {"efficiency": float(effs[ok].mean()) if ok.any() else float("nan"),
-                    "efficiency_min": float(effs[ok].min()) if ok.any() else float("nan"),
-                    "efficiency_max": float(effs[ok].max()) if ok.any() else float("nan"),
-                    "chains_with_efficiency": int(ok.sum()),
-                    "sigma": float(np.nanmean([r["sigma"] for r in rs])),
-                    "E_proj": float(np.mean([r["E_proj"] for r in rs])), "seeds": rs}
-    b, h = res["renorm"], res["heat_bath"]
-    head = f"learned_k{thresholds[0]}"
-    a = res[head]
+            r.update(min_attempts=kk, proposal_ms=prof["proposal_ms"], network_rows=prof["proposal_rows"],
+                     proposal_mix=args.proposal_mix)
+        res[gen] = r
+        log(f"[eff] {gen}: E_proj {r['E_proj']:.6f} +- {r['sigma']:.6f} (reblocked, level {rb['level']} of "
+            f"{len(proj)} reports, converged {rb['converged']}), walkers {r['walkers_mean']:.4g}, {wall:.1f}s, "
+            f"efficiency {r['efficiency']}")
 
-    def agree(x, y):
-        return bool(abs(x["E_proj"] - y["E_proj"]) < 2 * math.hypot(x["sigma"], y["sigma"]))
+    def ratio(a, b):
+        x, y = res.get(a), res.get(b)
+        if not x or not y or x["efficiency"] is None or y["efficiency"] is None:
+            return None
+        return {"ratio": x["efficiency"] / y["efficiency"],
+                "energies_agree_2sigma": bool(abs(x["E_proj"] - y["E_proj"]) < 2 * math.hypot(x["sigma"], y["sigma"]))}
 
-    print(json.dumps({"metric": "E_proj efficiency 1/(sigma^2 t), learned vs renorm (the plain path)",
-                      "unit": "ratio", "value": a["efficiency"] / b["efficiency"], "higher_is_better": True,
-                      "energies_agree_2sigma": agree(a, b),
-                      "learned_vs_heat_bath": a["efficiency"] / h["efficiency"],
-                      "energies_agree_2sigma_heat_bath": agree(a, h),
-                      "heat_bath_vs_renorm": h["efficiency"] / b["efficiency"],
-                      "by_threshold": {g: {"vs_renorm": r["efficiency"] / b["efficiency"],
-                                           "vs_heat_bath": r["efficiency"] / h["efficiency"]}
-                                       for g, r in res.items() if g.startswith("learned")},
-                      "walker_coverage": coverage,
-                      "n_gpus": 1, "data": "synthetic",
+    print(json.dumps({"metric": "E_proj efficiency 1/(sigma^2 t) on mol40 at matched walker numbers",
+                      "unit": "ratio", "higher_is_better": True,
+                      "learned_vs_renorm": ratio("learned", "renorm"),
+                      "learned_vs_heat_bath": ratio("learned", "heat_bath"),
+                      "heat_bath_vs_renorm": ratio("heat_bath", "renorm"),
+                      "walker_coverage": coverage, "n_gpus": 1, "data": "synthetic",
                       "config": {"workload": f"i-FCIQMC mol40: synthetic molecular integrals {norb} spatial orbitals, "
                                              f"{nel} electrons, C1, real_64, initiator 3, tau 1e-3, target "
-                                             f"{args.target:g}, {args.eff_cycles} production cycles",
-                                 "proposal_mix": args.proposal_mix, "proposal_min_attempts": thresholds[0],
-                                 "train_steps": args.train_steps, "warmup_cycles": args.eff_warmup,
-                                 "seeds": seeds},
-                      "training": training,
-                      "runs": res}), flush=True)
+                                             f"{args.target:g} held by the shift, {args.eff_cycles} production "
+                                             f"cycles in reports of 10",
+                                 "proposal_mix": args.proposal_mix, "proposal_min_attempts": k,
+                                 "train_steps": args.train_steps, "warmup_cycles": args.eff_warmup, "seed": sd,
+                                 "sigma": "Flyvbjerg-Petersen reblocking, Wolff/Lee optimal block, ratio with "
+                                          "covariance"},
+                      "training": training, "runs": res}), flush=True)
 
 
 def c3_instances(n=64, seed=2024):
@@ -603,13 +601,19 @@ def integer_params(seed, cap):
 
 def efficiency_batched(args):
     """64 instances: one batched engine against the same instances propagated
-    one engine at a time.  Batched chains equal the solo chains bit for bit, so
-    sigma is the same and the efficiency ratio is the wall-time ratio."""
+    one engine at a time, both from the same prepared lists at cycle 0 with
+    the same Philox keys (instance k: philox_seed + k) and the same shift
+    control (a function of each instance's own outputs), so every instance
+    follows one chain in both forms; the chains are checked equal report by
+    report.  sigma of each instance's E_proj is its reblocked error (as
+    pyhande computes it); the efficiency of a form is the sum over instances
+    of 1/(sigma_k^2 t) with t the form's wall time."""
+    from hande_amd import reblock
     lib = N.lib()
     decks, targets = c3_instances(args.instances)
     K = len(decks)
     systems = [api.hubbard_k(**d) for d in decks]
-    tau = 0.005
+    tau, every = 0.005, 5
     cap = int(6 * targets.sum()) + (1 << 20)
     arr = (N.FqSystem * K)(*[s.view for s in systems])
     beng = C.c_void_p()
@@ -627,76 +631,124 @@ def efficiency_batched(args):
     n_old = np.zeros(K)
     outs = (N.FqCycleOut * K)()
 
-    def batch_cycle(c, prep=False):
+    def batch_cycle():
         N.check(lib.fq_cycle_batch(beng, shift.ctypes.data_as(C.POINTER(C.c_double)),
                                    eold.ctypes.data_as(C.POINTER(C.c_double)), taus.ctypes.data_as(C.POINTER(C.c_double)),
                                    outs), "fq_cycle_batch")
-        if (c + 1) % 5 == 0:
-            n = np.array([outs[k].nparticles[0] for k in range(K)])
-            upd = vary & (n_old > 0)
-            if prep:
-                # preparation: stronger damping plus an integral pull to the
-                # target, from a twentieth of it, so that strongly correlated
-                # rings (growth ~10% per cycle at zero shift) do not overshoot
-                shift[upd] -= (np.log(n[upd] / n_old[upd]) * 0.2 + 0.04 * np.log(n[upd] / targets[upd])) / (tau * 5)
-                vary[:] |= n > targets / 20
-            else:
-                shift[upd] -= np.log(n[upd] / n_old[upd]) * 0.05 / (tau * 5)
-                vary[:] |= n > targets
-            n_old[:] = n
 
+    # preparation (untimed): stronger damping plus an integral pull to the
+    # target, from a twentieth of it, so that strongly correlated rings
+    # (growth ~10% per cycle at zero shift) do not overshoot
     c = 0
     t0 = time.perf_counter()
     while c < args.prep_cycles and not (vary.all() and c > 400):
-        batch_cycle(c, prep=True)
+        batch_cycle()
         c += 1
+        if c % every == 0:
+            n = np.array([outs[k].nparticles[0] for k in range(K)])
+            upd = vary & (n_old > 0)
+            shift[upd] -= (np.log(n[upd] / n_old[upd]) * 0.2 + 0.04 * np.log(n[upd] / targets[upd])) / (tau * every)
+            vary[:] |= n > targets / 20
+            n_old[:] = n
         if c % 200 == 0:
             log(f"[batch] prep cycle {c}: {sum(outs[k].nparticles[0] for k in range(K)):.4g} walkers, "
                 f"{int(vary.sum())}/{K} instances controlled, {time.perf_counter() - t0:.1f}s")
-    log(f"[batch] prepared {K} instances in {c} cycles ({time.perf_counter() - t0:.1f}s): "
-        f"{sum(outs[k].nparticles[0] for k in range(K)):.4g} walkers")
-    # timed batched cycles
-    attempts = 0
-    t1 = time.perf_counter()
-    for i in range(args.eff_cycles):
-        batch_cycle(c + i)
-        attempts += sum(outs[k].nattempts for k in range(K))
-        if (i + 1) % 200 == 0:
-            log(f"[batch] timed cycle {i + 1}: {time.perf_counter() - t1:.1f}s")
-    t_batch = time.perf_counter() - t1
-    # the same instances, one engine each, from the batched list
     st, po, da = download_list(lib, beng, 1)
     lib.fq_destroy(beng)
-    t_solo = 0.0
-    att_solo = 0
+    shift0, nold0 = shift.copy(), n_old.copy()
+    log(f"[batch] prepared {K} instances in {c} cycles ({time.perf_counter() - t0:.1f}s): {po.sum():.4g} walkers")
+    cycles = args.eff_cycles
+
+    class Control:
+        """The production shift update (damping 0.05 every 5 cycles, the
+        reference's rule) and E_proj report sums of one set of instances."""
+
+        def __init__(self, ks):
+            self.ks = np.asarray(ks)
+            self.shift, self.n_old = shift0[self.ks].copy(), nold0[self.ks].copy()
+            self.eold = np.zeros(len(self.ks))
+            self.accp, self.accd = np.zeros(len(self.ks)), np.zeros(len(self.ks))
+            self.proj, self.d0, self.count = [], [], 0
+
+        def update(self, o):  # o: sequence of FqCycleOut for self.ks
+            self.count += 1
+            self.accp += [x.proj_energy[0] for x in o]
+            self.accd += [x.D0_population[0] for x in o]
+            if self.count % every == 0:
+                n = np.array([x.nparticles[0] for x in o])
+                self.shift -= np.log(n / self.n_old) * 0.05 / (tau * every)
+                self.n_old = n
+                self.eold = np.where(self.accd != 0, self.accp / np.where(self.accd != 0, self.accd, 1), 0.0)
+                self.proj.append(self.accp.copy())
+                self.d0.append(self.accd.copy())
+                self.accp[:] = 0.0
+                self.accd[:] = 0.0
+
+    # batched form, from the prepared lists at cycle 0
+    beng = C.c_void_p()
+    N.check(lib.fq_create_batch(arr, K, C.byref(integer_params(77, cap)), C.byref(beng)), "fq_create_batch")
+    binst = (st >> np.uint64(48)).astype(np.int32)
+    bst = np.ascontiguousarray(st & np.uint64((1 << 48) - 1))
+    N.check(lib.fq_load_batch(beng, binst.ctypes.data_as(C.POINTER(C.c_int32)), bst.ctypes.data_as(C.POINTER(C.c_uint64)),
+                              po.ctypes.data_as(C.POINTER(C.c_int64)), da.ctypes.data_as(C.POINTER(C.c_double)),
+                              len(da)), "load")
+    cb = Control(range(K))
+    attempts = 0
+    t1 = time.perf_counter()
+    for i in range(cycles):
+        shift[:], eold[:] = cb.shift, cb.eold
+        batch_cycle()
+        cb.update([outs[k] for k in range(K)])
+        attempts += sum(outs[k].nattempts for k in range(K))
+        if (i + 1) % 500 == 0:
+            log(f"[batch] batched cycle {i + 1}: {time.perf_counter() - t1:.1f}s")
+    t_batch = time.perf_counter() - t1
+    lib.fq_destroy(beng)
+    # the same instances, one engine each
+    t_solo, att_solo, same = 0.0, 0, True
+    sproj, sd0 = [], []
     for k in range(K):
-        sel = (st >> np.uint64(48)) == np.uint64(k)
-        sk = np.ascontiguousarray(st[sel] & np.uint64((1 << 48) - 1))
-        pk = np.ascontiguousarray(po[sel])
-        dk = np.ascontiguousarray(da[sel])
+        sel = binst == k
+        sk, pk, dk = (np.ascontiguousarray(x[sel]) for x in (bst, po, da))
         eng = C.c_void_p()
-        N.check(lib.fq_create(systems[k].view, C.byref(integer_params(77 + k, max(8 * len(dk), 1 << 16))),
-                              C.byref(eng)), "fq_create")
+        N.check(lib.fq_create(systems[k].view, C.byref(integer_params(77 + k, cap)), C.byref(eng)), "fq_create")
         N.check(lib.fq_load(eng, sk.ctypes.data_as(C.POINTER(C.c_uint64)), pk.ctypes.data_as(C.POINTER(C.c_int64)),
                             dk.ctypes.data_as(C.POINTER(C.c_double)), len(dk)), "fq_load")
-        sh = (C.c_double * 2)(shift[k], 0.0)
-        eo = (C.c_double * 2)(0.0, 0.0)
+        ck = Control([k])
         o = N.FqCycleOut()
         t2 = time.perf_counter()
-        for i in range(args.eff_cycles):
-            N.check(lib.fq_cycle(eng, sh, eo, tau, C.byref(o)), "fq_cycle")
+        for i in range(cycles):
+            N.check(lib.fq_cycle(eng, (C.c_double * 2)(ck.shift[0], 0.0), (C.c_double * 2)(ck.eold[0], 0.0), tau,
+                                 C.byref(o)), "fq_cycle")
+            ck.update([o])
             att_solo += o.nattempts
         t_solo += time.perf_counter() - t2
         lib.fq_destroy(eng)
+        p_k, d_k = np.array(ck.proj)[:, 0], np.array(ck.d0)[:, 0]
+        same = same and np.array_equal(p_k, np.array(cb.proj)[:, k]) and np.array_equal(d_k, np.array(cb.d0)[:, k])
+        sproj.append(p_k)
+        sd0.append(d_k)
         if (k + 1) % 8 == 0:
             log(f"[batch] one engine per instance: {k + 1}/{K} done, {t_solo:.1f}s")
+    per, inv_var = [], []
+    for k in range(K):
+        rb = reblock.projected_energy(np.array(cb.proj)[:, k], np.array(cb.d0)[:, k])
+        per.append({"E_proj": systems[k].H00 + rb["energy"], "sigma": rb["error"], "converged": rb["converged"]})
+        if rb["converged"] and rb["error"] > 0:
+            inv_var.append(1.0 / rb["error"] ** 2)
+    s_inv = float(np.sum(inv_var))
+    eff_b, eff_s = s_inv / t_batch, s_inv / t_solo
     print(json.dumps({"metric": "E_proj efficiency 1/(sigma^2 t), batched vs one engine per instance",
-                      "value": t_solo / t_batch, "unit": "ratio", "higher_is_better": True, "n_gpus": 1,
-                      "data": "synthetic",
+                      "value": eff_b / eff_s if s_inv > 0 else None, "unit": "ratio", "higher_is_better": True,
+                      "n_gpus": 1, "data": "synthetic", "chains_identical": bool(same),
+                      "instances_converged": len(inv_var),
+                      "efficiency": {"batched": eff_b, "one_engine_per_instance": eff_s,
+                                     "definition": "sum over converged instances of 1/(sigma_k^2 t)"},
                       "config": {"workload": f"{K} Hubbard-k rings of 10-18 sites, U/t 1-8, targets 1e4-1e6, "
-                                             f"integer populations, tau {tau}, {args.eff_cycles} cycles"},
+                                             f"integer populations, tau {tau}, {cycles} cycles, reports of {every}"},
                       "batched": {"wall_s": t_batch, "attempts_per_s": attempts / t_batch},
-                      "one_engine_per_instance": {"wall_s": t_solo, "attempts_per_s": att_solo / t_solo}}), flush=True)
+                      "one_engine_per_instance": {"wall_s": t_solo, "attempts_per_s": att_solo / t_solo},
+                      "instances": per}), flush=True)
 
 
 def main():
@@ -721,17 +773,18 @@ def main():
                     help="measure the E_proj statistical efficiency of the learned proposal (against heat-bath, "
                          "mol40) or of batched instances (against one engine per instance) instead")
     ap.add_argument("--eff-cycles", type=int, default=None,
-                    help="timed cycles of an efficiency mode (default 3000 learned: 300 reports for the "
-                         "reblocking; 1000 batched)")
+                    help="timed cycles of an efficiency mode (default 20000 learned: 2000 reports of 10 for the "
+                         "reblocking; 5000 batched: 1000 reports of 5)")
     ap.add_argument("--train-steps", type=int, default=1000)
     ap.add_argument("--proposal-mix", type=float, default=0.1,
                     help="learned proposal: fraction of doubles from the uniform fallback")
-    ap.add_argument("--eff-seeds", default="23", help="learned efficiency: production seeds (comma-separated); "
-                    "each generator runs one independent chain per seed and reports the mean efficiency")
+    ap.add_argument("--eff-seeds", default="23", help="learned efficiency: production seed")
+    ap.add_argument("--eff-gens", default="renorm,heat_bath,learned",
+                    help="learned efficiency: generators to run (comma-separated; ratios need both runs)")
     ap.add_argument("--eff-warmup", type=int, default=500, help="untimed cycles per generator before measuring")
-    ap.add_argument("--min-attempts", default="64,16",
-                    help="learned proposal: comma-separated attempt thresholds for the network (the first is "
-                         "the headline value); determinants below use flat weights with the learned affinity")
+    ap.add_argument("--min-attempts", default="64",
+                    help="learned proposal: attempt threshold for the network; determinants below use flat "
+                         "weights with the learned affinity")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--prep-cycles", type=int, default=3000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -739,7 +792,7 @@ def main():
                          "byte counters FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.eff_cycles is None:
-        args.eff_cycles = 3000 if args.efficiency == "learned" else 1000
+        args.eff_cycles = 20000 if args.efficiency == "learned" else 5000
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -820,8 +873,9 @@ def main():
     last = prop.last
     phases = ["det_pass", "spawn", "sort", "reduce", "merge"]
     avg_ms = {ph: float(np.mean([getattr(pr, ph + "_ms") for pr in profs])) for ph in phases}
-    kw = sort_key_words(system.view.nbasis, system.view.nel)
+    kw = sort_key_words(system.view.nbasis, system.view.nel, cap)
     avg_bytes = {ph: float(np.mean([phase_bytes(pr, W, S, kw)[ph] for pr in profs])) for ph in phases}
+    comp = {k: float(np.mean([compulsory_bytes(pr, W, S)[k] for pr in profs])) for k in ("cycle", "annihilation")}
     launches = sum(pr.spawn_launches for pr in profs)
     k_ms = sum(pr.spawn_kernel_ms for pr in profs) / launches
     k_bytes = sum(spawn_kernel_bytes(pr, W, S) for pr in profs) / launches
@@ -846,6 +900,15 @@ def main():
                 f"{nel} electrons, C1, real_64, initiator 3, tau 1e-3, renorm, target {args.target:g}"
                 + (", replica pair" if S == 2 else ""))
     traffic = traffic_summary(workload)
+    issue = issue_summary(workload)
+    cycle_ms = float(np.mean([pr.total_ms for pr in profs]))
+    annih_phase_ms = avg_ms["sort"] + avg_ms["reduce"] + avg_ms["merge"]
+    valu = None
+    if issue:
+        lane_ops = issue["lane_ops_per_launch"] / (k_ms * 1e-3)
+        valu = {"achieved": lane_ops / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12, "unit": "Tlane-op/s",
+                "frac": lane_ops / VALU_PEAK_LANE_OPS, "busy": issue["valu_busy"],
+                "lane_utilization": issue["valu_lane_utilization"], "source": issue["source"]}
     result = {
         "metric": "spawn attempts/s",
         "value": value,
@@ -867,13 +930,24 @@ def main():
             "parallelism": "1 GPU",
         },
         "annihilated_entries_per_s": annih_rate,
+        # the dominant kernel in HBM terms (achieved / peak / frac / traffic), with
+        # what actually limits it: VALU issue (busy >= 0.9 of SIMD cycles, lanes
+        # lost to divergence), priced against the VALU lane-op peak in "valu";
+        # "annihilation" and "cycle" are the HBM-bound phases on their
+        # compulsory bytes (3 N_d B_m + 2 N_s B_s per cycle)
         "roofline": {"bound": "hbm", "kernel": "k_spawn", "dominant": dominant, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "limiter": "valu-issue" if valu else None, "valu": valu,
                      "launch_ms": k_ms, "bytes_per_launch": k_bytes, "launches": launches,
-                     "phase_ms": avg_ms, "phase_bytes": avg_bytes,
-                     # the kernel is bound by VALU issue, not HBM: counters of the committed profile
-                     "issue": issue_summary(workload)},
+                     "annihilation": {"ms": annih_phase_ms, "compulsory_bytes": comp["annihilation"],
+                                      "achieved": comp["annihilation"] / (annih_phase_ms * 1e-3) / 1e9,
+                                      "unit": "GB/s",
+                                      "frac": comp["annihilation"] / (annih_phase_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "cycle": {"ms": cycle_ms, "compulsory_bytes": comp["cycle"],
+                               "achieved": comp["cycle"] / (cycle_ms * 1e-3) / 1e9, "unit": "GB/s",
+                               "frac": comp["cycle"] / (cycle_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "phase_ms": avg_ms, "phase_bytes": avg_bytes},
         "cpu_baseline": cpu,
     }
     lib.fq_destroy(eng)

@@ -89,6 +89,14 @@ class FqhReportRow(C.Structure):
     ]
 
 
+class FqhRestart(C.Structure):
+    _fields_ = [
+        ("nstates", i64), ("capacity", i64), ("states", P(C.c_uint64)), ("pops", P(i64)), ("dat", P(f64)),
+        ("mc_cycles_done", i64), ("shift", f64 * FQ_MAX_SPACES), ("vary_shift", C.c_int32 * FQ_MAX_SPACES),
+        ("proj_energy", f64 * FQ_MAX_SPACES), ("D0_population", f64 * FQ_MAX_SPACES),
+    ]
+
+
 def _declare(lib):
     vp = C.c_void_p
     sig = {
@@ -107,6 +115,14 @@ def _declare(lib):
         "fq_rng_available": (C.c_int, [vp, P(i64)]),
         "fq_cycle": (C.c_int, [vp, P(f64), P(f64), f64, P(FqCycleOut)]),
         "fq_cycle_noalloc": (C.c_int, [vp, P(f64), P(f64), f64, P(FqCycleOut)]),
+        "fq_cycle_spawn": (C.c_int, [vp, P(f64), P(f64), f64, P(FqCycleOut)]),
+        "fq_spawned_list": (C.c_int, [vp, P(u64), P(i64), P(i64), i64, P(i64)]),
+        "fq_cycle_finish": (C.c_int, [vp, P(u64), P(i64), P(i64), i64, P(FqCycleOut)]),
+        "fqh_fciqmc_ranks": (C.c_int, [vp, P(FqhQmcIn), C.c_int, P(FqhReportRow), C.c_int, P(C.c_int)]),
+        "fqh_fciqmc_restart": (C.c_int, [vp, P(FqhQmcIn), P(FqhRestart), P(FqhRestart), P(FqhReportRow), C.c_int,
+                                         P(C.c_int)]),
+        "fqh_murmur2": (C.c_uint32, [C.c_char_p, C.c_int, C.c_uint32]),
+        "fqh_owner_rank": (C.c_int, [P(u64), C.c_int, C.c_int]),
         "fq_annihilate": (C.c_int, [vp, P(u64), P(i64), P(i64), i64, P(FqCycleOut)]),
         "fq_nstates": (C.c_int, [vp, P(i64)]),
         "fq_download": (C.c_int, [vp, P(u64), P(i64), P(f64), i64, P(i64)]),
@@ -148,6 +164,15 @@ def lib(path=None):
                 "(hipcc, gfx950). There is no CPU fallback.")
         _LIB = _declare(C.CDLL(LIB_PATH))
     return _LIB
+
+
+def capacity_from(size, element_bytes, spawned=False):
+    """Slots from a state_size-style option: >= 0 is a count, < 0 is MB per
+    core (src/particle_t_utils.f90:84-101), as the report-loop driver reads it."""
+    if size >= 0:
+        return int(size)
+    mb = -float(size) * 1.0e6
+    return int(mb / (2.0 * element_bytes) if spawned else mb / element_bytes)
 
 
 def check(rc, what):

@@ -153,18 +153,58 @@ def rows_to_table(rows, nrows, nspaces=1):
     return np.array(out)
 
 
-def fciqmc(sys, qmc, fciqmc=None, rng_mode="replay", device=0, lib=None, **_):
-    """fciqmc{...}: run FCIQMC on the GPU engine and return the report table."""
+def pop_bit_shift(st):
+    """Bits of fixed-point fraction in the population encoding (real_bit_shift,
+    src/qmc.F90:1462-1479): 31 for real_64 amplitudes, 11 for real_32, 0 for integers."""
+    return (11 if st.real_amplitude_force_32 else 31) if st.real_amplitudes else 0
+
+
+def fciqmc(sys, qmc, fciqmc=None, rng_mode="replay", device=0, lib=None, nprocs=1, restart=None,
+           return_state=False, **_):
+    """fciqmc{...}: run FCIQMC on the GPU engine and return the report table.
+
+    nprocs > 1 (replay mode) replays a run of that many MPI ranks: one engine
+    per emulated rank on the device, spawned walkers exchanged on the host.
+    restart (a ``restart.RestartState`` or the path of a HANDE restart file)
+    continues a stored run, as ``restart = { read = ... }`` does; the file's
+    shift damping applies unless qmc sets one (src/qmc.F90:186-192).
+    return_state=True returns (table, RestartState after the last report), the
+    state ``restart = { write = ... }`` stores (``restart.write_restart``)."""
+    from . import restart as R
     lib = lib or sys._lib
     st = qmc_in(qmc, fciqmc, rng_mode=rng_mode, device=device)
+    S = 2 if st.replica_tricks else 1
+    pbs = pop_bit_shift(st)
+    rs = None
+    if restart is not None:
+        rs = restart if isinstance(restart, R.RestartState) else R.read_restart(
+            restart, sys.view.nbasis, S, pbs)
+        if rs.pop_bit_shift != pbs:
+            raise R.RestartError("restart state encoded with another population scale")
+        if "shift_damping" not in qmc and rs.shift_damping is not None:
+            st.shift_damping = rs.shift_damping
     max_rows = st.nreports + 1
     rows = (N.FqhReportRow * max_rows)()
     nrows = C.c_int()
-    rc = lib.fqh_fciqmc(sys.handle, C.byref(st), rows, max_rows, C.byref(nrows))
-    table = rows_to_table(rows, nrows.value, 2 if st.replica_tricks else 1)
+    end = None
+    if nprocs > 1:
+        if rs is not None or return_state:
+            raise ValueError("restarts are written and read on one rank")
+        rc = lib.fqh_fciqmc_ranks(sys.handle, C.byref(st), int(nprocs), rows, max_rows, C.byref(nrows))
+    elif rs is None and not return_state:
+        rc = lib.fqh_fciqmc(sys.handle, C.byref(st), rows, max_rows, C.byref(nrows))
+    else:
+        start = R.RestartBuffers.from_state(rs) if rs is not None else None
+        cap = N.capacity_from(st.state_size, 8 * sys.view.nwords + 16 * S)
+        end = R.RestartBuffers(sys.view.nwords, S, cap) if return_state else None
+        rc = lib.fqh_fciqmc_restart(sys.handle, C.byref(st), C.byref(start.st) if start else None,
+                                    C.byref(end.st) if end else None, rows, max_rows, C.byref(nrows))
+    table = rows_to_table(rows, nrows.value, S)
     if rc == N.FQ_ERR_OVERFLOW:
         raise ListOverflow(table)
     N.check(rc, "fciqmc")
+    if return_state:
+        return table, end.to_state(S, pbs, like=rs)
     return table
 
 
@@ -177,6 +217,29 @@ class ListOverflow(RuntimeError):
         super().__init__(f"walker list overflow: report loop stopped after {len(table)} rows "
                          "(raise state_size / spawned_state_size)")
         self.table = table
+
+
+def restart_file(opts, base_dir=".", rank=0):
+    """Path of the restart file a deck's ``restart = { read = Y }`` names
+    (restart_stem.Y.pX.H5 with stem HANDE.RS, src/restart_hdf5.F90:112-127;
+    read = true picks the highest existing Y), or None if it reads none."""
+    import glob
+    import os
+    import re
+    r = opts.get("restart") or {}
+    rid = r.get("read")
+    if rid is None or rid is False:
+        return None
+    if rid is True or (isinstance(rid, (int, float)) and rid < 0):
+        ids = []
+        for f in glob.glob(os.path.join(base_dir, f"HANDE.RS.*.p{rank}.H5")):
+            m = re.search(r"HANDE\.RS\.(\d+)\.p\d+\.H5$", f)
+            if m:
+                ids.append(int(m.group(1)))
+        if not ids:
+            raise FileNotFoundError(f"no restart files in {base_dir}")
+        rid = max(ids)
+    return os.path.join(base_dir, f"HANDE.RS.{int(rid)}.p{rank}.H5")
 
 
 def load_deck(text, base_dir=".", lib=None):

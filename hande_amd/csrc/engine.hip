@@ -43,6 +43,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "device_common.hpp"
@@ -53,16 +54,16 @@ using fqd::Sys;
 
 // ---------------------------------------------------------------------------
 // Fixed scratch of fq_cycle_noalloc, for callers that cannot allocate device
-// memory.  One region per scratch slot (int64 elements): prefix-sum partials
-// (slots 0-3) and the radix sort's tile histograms and their offsets (slots 4
-// and 5).  Every device holds its own copy until the process ends; host code
-// reaches it only through the address hipGetSymbolAddress returns with that
-// device current (the array's name is a host-side stand-in).  Calls that
-// overlap on one device share it and race.
+// memory.  One region per scratch slot (int64 elements): the prefix sums'
+// tile sums (slot 0) and tile offsets (slot 1), sized for every scan an
+// engine can issue (at most 2^31 entries, tiles of 4096).  Every device holds
+// its own copy until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns with that device current (the array's
+// name is a host-side stand-in).  Calls that overlap on one device share it
+// and race.
 // ---------------------------------------------------------------------------
-constexpr int kScratchSlots = 6;
-constexpr int64_t kScratchSlotElems[kScratchSlots] = {int64_t{1} << 16, int64_t{1} << 16, int64_t{1} << 17,
-                                                      int64_t{1} << 10, int64_t{3600000}, int64_t{3600000}};
+constexpr int kScratchSlots = 2;
+constexpr int64_t kScratchSlotElems[kScratchSlots] = {(int64_t{1} << 19) + 64, (int64_t{1} << 19) + 64};
 __host__ __device__ constexpr int64_t scratch_slot_offset(int slot) {
   int64_t off = 0;
   for (int k = 0; k < slot; ++k) off += (kScratchSlotElems[k] + 31) / 32 * 32;  // 256-byte aligned regions
@@ -79,8 +80,6 @@ constexpr int kBlock = 256;
 constexpr int kInstShift = 48;
 constexpr uint64_t kOrbMask = (uint64_t{1} << kInstShift) - 1;
 constexpr int kMaxInst = 1 << 16;
-constexpr int kRadixItems = 8;
-constexpr int kRadixTile = kBlock * kRadixItems;
 constexpr int kRunTile = 4 * kBlock;  // sorted entries per run-detection tile
 constexpr int kMergeWin = 1024;       // main-list window entries a merge block stages in LDS
 // Words per spawned-list record (populations, flag), padded to 16-byte
@@ -305,7 +304,7 @@ struct fq_engine {
   float* mlp_flat = nullptr;  // 2*nbasis ones: weights of determinants without a row
   const int64_t* p_nruns = nullptr;  // device address of this cycle's run count
   int64_t att_bound = -1;            // bound of the next cycle's attempts (-1: unknown)
-  bool keys_ready = false;           // this cycle's sort keys were made before the spawned length was read
+  bool keys_ready = false;           // the spawning pass made this cycle's sort keys
   int64_t run_bound = 0;             // host bound of it (the count itself in replay mode)
   int64_t mlp_tab_rows = 0;
   int64_t* mlp_on = nullptr;   // [cap] network runs on this determinant this cycle
@@ -338,21 +337,17 @@ struct fq_engine {
   int64_t* tile_win = nullptr;  // parents of each tile's first and last attempt
   // spawned list
   int64_t scap = 0;
-  uint64_t* sdet = nullptr;  // W*scap
-  int64_t* srec = nullptr;   // scap records: S populations then the flag (rec_stride(S) words each)
-  // sort buffers
-  uint64_t* kbuf[2] = {nullptr, nullptr};  // W*scap
-  uint32_t* vbuf[2] = {nullptr, nullptr};  // scap
-  int64_t* hist = nullptr;                 // 256*nb+1
-  int64_t* hist_scan = nullptr;
-  int64_t hist_cap = 0;
-  // runs
-  int64_t* run_stage = nullptr;  // scap: run starts compacted per tile
-  int64_t* run_tcnt = nullptr;   // runs per tile (scap / kRunTile + 2)
-  int64_t* run_toff = nullptr;   // their exclusive prefix
-  int64_t* gpop = nullptr;       // S*scap: spawned populations in sorted order
-  int64_t* gflag = nullptr;      // scap: spawned flags in sorted order
-  int64_t* run_start = nullptr;  // scap+1
+  uint64_t* sdet = nullptr;  // W*scap (in kbuf[1])
+  int64_t* srec = nullptr;   // scap records: S populations then the flag, rec_stride(S) words each (in pbuf[1])
+  // sort: keys (value << flag_bits | flags) and payload (populations), ping-pong
+  int sort_kw = 1, flag_bits = 1, value_bits = 1;
+  uint64_t* kbuf[2] = {nullptr, nullptr};  // sort_kw * scap
+  int64_t* pbuf[2] = {nullptr, nullptr};   // S * scap
+  uint32_t* perm = nullptr;                // scap: replay quicksort order
+  int64_t* pass_cnt = nullptr;             // 512 * kSortBlocks digit counts of a pass's blocks
+  int64_t* pass_off = nullptr;             // their exclusive prefix (+ total)
+  int64_t* run_tcnt = nullptr;             // run heads per run tile
+  int64_t* run_toff = nullptr;             // their exclusive prefix; [ntiles] = the run count
   uint64_t* rdet = nullptr;      // W*scap
   int64_t* rpop = nullptr;       // S*scap
   int64_t* rflag = nullptr;      // scap
@@ -370,6 +365,12 @@ struct fq_engine {
   // scan scratch
   std::vector<int64_t*> scan_tmp;
   std::vector<int64_t> scan_tmp_cap;
+  // fq_cycle_spawn .. fq_cycle_finish: a cycle split at the spawned list (replay mode)
+  bool pending = false;
+  uint32_t pend_cyc = 0;
+  int64_t pend_rng_pos = 0;   // stream position after the spawning pass
+  int64_t pend_ns = 0;        // entries this engine spawned
+  Stats pend_stats{};         // the spawning pass's counters
   // fq_cycle_noalloc: scratch slots come from the device's fixed array
   int64_t* fixed_scratch = nullptr;  // device address of `scratch` on this engine's device (null: unavailable)
   bool noalloc = false;              // inside fq_cycle_noalloc
@@ -833,20 +834,88 @@ __global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Sort keys.  A spawned entry's key is v << flag_bits | flags over kw 64-bit
+// words (kw = 1 or 2): v is the determinant's combinatorial rank (or, for
+// batched instances, its encoded word 0, instance index included), flags its
+// initiator flag bits (one per space).  Ordering the v fields orders
+// determinants as the main list does (last word most significant,
+// src/bit_utils.F90:452-480); the LSD radix passes order only the digits of
+// v and are stable, so equal determinants keep spawned (attempt) order.  The
+// entry's populations travel with the key through the passes (the payload),
+// so the run reduction reads everything contiguously, and the determinant is
+// recovered from v.
+//
+// Combinatorial rank: for a bit string with set bits p_1 < ... < p_k
+// (0-based), rank = sum_i C(p_i, i).  For a fixed number of set bits this is
+// a bijection onto [0, C(nbasis, k)) that preserves the strings' numeric
+// order, so the sort needs ceil(log2 C(nbasis, nel)) bits instead of nbasis
+// (34 instead of 52 for 10 electrons in 52 spin orbitals).  binom holds
+// C(p, i) for p < nbasis, 1 <= i <= nel as (low, high) 64-bit halves.
+// ---------------------------------------------------------------------------
+struct KeyPlan {
+  int kw;          // key words
+  int flag_bits;   // width of the flag field (the number of spaces)
+  int rank;        // 1: v = combinatorial rank; 0: v = determinant word 0
+  int nel, nbasis;
+  const uint64_t* binom;
+  int binom_lds;   // stage the table in dynamic LDS (k_pack_tiles, k_runs)
+};
+constexpr int kMaxPasses = 16;
+struct DigitPlan {
+  int np;                  // passes
+  int shift[kMaxPasses];   // first key bit of each pass's digit
+  int bits[kMaxPasses];    // digit width (8 or 9)
+};
+// Keys per thread of a sort tile: the tile's keys and payload sit in LDS.
+__host__ __device__ constexpr int sort_items(int kw, int S) { return kw + S <= 2 ? 16 : 8; }
+constexpr int kSortBlocks = 512;  // blocks of a radix pass (each owns a contiguous range of tiles)
+
+__device__ __forceinline__ void make_key(const KeyPlan& kp, const uint64_t* f, int W, uint64_t flags, uint64_t* key) {
+  unsigned __int128 v = 0;
+  if (kp.rank) {
+    int k = 0;
+    FQ_FOR_WORDS(w, W) {
+      for (uint64_t x = f[w]; x && k < kp.nel; x &= x - 1) {
+        const int p = (w << 6) + __builtin_ctzll(x);
+        if (p >= kp.nbasis) break;  // not a determinant of this system (caller-supplied list)
+        const uint64_t* c = kp.binom + 2 * (static_cast<int64_t>(p) * kp.nel + k);
+        v += (static_cast<unsigned __int128>(c[1]) << 64) | c[0];
+        ++k;
+      }
+    }
+  } else {
+    v = f[0];
+  }
+  const unsigned __int128 k = (v << kp.flag_bits) | (flags & ((uint64_t{1} << kp.flag_bits) - 1));
+  key[0] = static_cast<uint64_t>(k);
+  key[1] = static_cast<uint64_t>(k >> 64);
+}
+
 // Pack each tile's compacted successes into the spawned list after the entries
-// of earlier batches (st->nspawn), in attempt order; one block per tile.
+// of earlier batches (st->nspawn), in attempt order, as sort keys and payload
+// (the populations); blocks loop over tiles.
 __global__ __launch_bounds__(kBlock) void k_pack_tiles(Sys s, int64_t ntiles, const int64_t* __restrict__ tile_cnt,
                                                        const int64_t* __restrict__ tile_off,
                                                        const uint64_t* __restrict__ st_det,
                                                        const int64_t* __restrict__ st_pop,
                                                        const int64_t* __restrict__ st_ent, int64_t bcap,
                                                        int64_t t_begin, const int64_t* __restrict__ flags,
-                                                       int64_t scap, uint64_t* __restrict__ sdet,
-                                                       int64_t* __restrict__ srec, Stats* __restrict__ st) {
-  const int64_t tile = blockIdx.x;
-  if (tile >= ntiles) return;
+                                                       int64_t scap, KeyPlan kp, uint64_t* __restrict__ keys,
+                                                       int64_t* __restrict__ pay, Stats* __restrict__ st) {
+  // the binomial table of the keys' ranks in LDS when the launch provides room
+  // for it (blocks loop over tiles, so each stages it once)
+  extern __shared__ uint64_t binom_lds[];
+  if (kp.rank && kp.binom_lds) {
+    const int nw = 2 * kp.nbasis * kp.nel;
+    for (int k = threadIdx.x; k < nw; k += kBlock) binom_lds[k] = kp.binom[k];
+    __syncthreads();
+    kp.binom = binom_lds;
+  }
+  const int64_t nspawn0 = st->nspawn;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
   const int64_t cnt = tile_cnt[tile];
-  const int64_t base = st->nspawn + tile_off[tile];
+  const int64_t base = nspawn0 + tile_off[tile];
   for (int64_t j = threadIdx.x; j < cnt; j += kBlock) {
     const int64_t u = tile * kSpawnTile + j;
     const int64_t slot = base + j;
@@ -857,10 +926,14 @@ __global__ __launch_bounds__(kBlock) void k_pack_tiles(Sys s, int64_t ntiles, co
     const int64_t e = st_ent[u];
     const int sp = static_cast<int>(e % s.S);
     const int64_t i = e / s.S;
-    for (int w = 0; w < s.W; ++w) sdet[w * scap + slot] = st_det[w * bcap + u];
-    int64_t* rec = srec + slot * rec_stride(s.S);
-    for (int q = 0; q < s.S; ++q) rec[q] = (q == sp) ? st_pop[u] : 0;
-    rec[s.S] = s.initiator ? flags[i] : 0;
+    uint64_t f[FQ_MAX_WORDS] = {0, 0};
+    FQ_FOR_WORDS(w, s.W) f[w] = st_det[w * bcap + u];
+    const int64_t v = st_pop[u];
+    for (int q = 0; q < s.S; ++q) pay[q * scap + slot] = (q == sp) ? v : 0;
+    uint64_t k[2];
+    make_key(kp, f, s.W, s.initiator ? static_cast<uint64_t>(flags[i]) : 0, k);
+    for (int w = 0; w < kp.kw; ++w) keys[w * scap + slot] = k[w];
+  }
   }
 }
 
@@ -874,7 +947,8 @@ __global__ void k_add_spawned(const int64_t* __restrict__ pos, int64_t nb, const
 // Batched instances
 // ---------------------------------------------------------------------------
 // bound[k] = first position in the sorted list whose instance index is >= k.
-__global__ void k_inst_bounds(const uint64_t* __restrict__ w0, int64_t n, int ninst, int64_t* __restrict__ bound) {
+__global__ void k_inst_bounds(const uint64_t* __restrict__ w0, int64_t n, int ninst, int shift,
+                              int64_t* __restrict__ bound) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > ninst) return;
   if (k == ninst) {
@@ -884,7 +958,7 @@ __global__ void k_inst_bounds(const uint64_t* __restrict__ w0, int64_t n, int ni
   int64_t lo = 0, hi = n;
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
-    if (static_cast<int>(w0[mid] >> kInstShift) < k)
+    if (static_cast<int>(w0[mid] >> shift) < k)
       lo = mid + 1;
     else
       hi = mid;
@@ -1083,234 +1157,261 @@ __device__ __forceinline__ int key_cmp(const uint64_t* keys, int64_t stride, int
   return 0;
 }
 
-// Radix passes use DB-bit digits (8 or 9; 1 << DB buckets): the host mixes
-// the two widths to cover a key word in the fewest passes (34 bits: 9+9+8+8).
-// Histogram, tile-major: hist[block * NB + d] = count of digit d in the
-// block's tile, so a tile's counts (and later its offsets) are contiguous.
-template <int DB>
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* __restrict__ kw, int64_t n, int shift,
-                                                       int64_t* __restrict__ hist) {
-  constexpr int NB = 1 << DB;
-  __shared__ int cnt[NB];
-  for (int d = threadIdx.x; d < NB; d += kBlock) cnt[d] = 0;
-  __syncthreads();
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRadixTile;
-  for (int r = 0; r < kRadixItems; ++r) {
-    const int64_t i = base + r * kBlock + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[(kw[i] >> shift) & (NB - 1)], 1);
+// Sort passes use DB-bit digits (8 or 9; 1 << DB buckets) of the key's value
+// field: the host mixes the two widths to cover it in the fewest passes
+// (34 bits: 9+9+8+8).  Digit of a KW-word key starting at bit `shift`.
+template <int KW>
+__device__ __forceinline__ uint32_t key_digit(const uint64_t* k, int shift, uint32_t mask) {
+  if constexpr (KW == 1) {
+    return static_cast<uint32_t>(k[0] >> shift) & mask;
+  } else {
+    const uint64_t x = shift >= 64 ? (k[1] >> (shift - 64))
+                       : shift == 0 ? k[0]
+                                    : ((k[0] >> shift) | (k[1] << (64 - shift)));
+    return static_cast<uint32_t>(x) & mask;
   }
-  __syncthreads();
-  for (int d = threadIdx.x; d < NB; d += kBlock) hist[static_cast<int64_t>(blockIdx.x) * NB + d] = cnt[d];
 }
 
-// Digit offsets of every tile from the tile-major counts: offset(b, d) =
-// (tiles of all digits below d) + (tiles above b with digit d).  Three steps
-// over chunks of `rows` tiles (the host keeps the chunks at most
-// kHistChunks), every access contiguous along the digits: per-chunk column
-// sums, per-digit chunk offsets (one block per digit) with the digit totals,
-// then the tiles' offsets (chunk offset plus digit start plus the tiles
-// above within the chunk).
-constexpr int64_t kHistChunks = kBlock;  // one thread per chunk in k_hist_chunkscan
+// Lanes of the wavefront holding the same DB-bit digit as this lane (among
+// the `valid` ones): DB ballots.
 template <int DB>
-__global__ __launch_bounds__(kBlock) void k_hist_colsum(const int64_t* __restrict__ hist, int64_t nb, int64_t rows,
-                                                        int64_t* __restrict__ csum) {
-  constexpr int NB = 1 << DB;
-  const int d = static_cast<int>(blockIdx.x) * kBlock + threadIdx.x;
-  const int64_t c = blockIdx.y, b0 = c * rows, b1 = b0 + rows < nb ? b0 + rows : nb;
-  int64_t acc = 0;
-  for (int64_t b = b0; b < b1; ++b) acc += hist[b * NB + d];
-  csum[c * NB + d] = acc;
+__device__ __forceinline__ unsigned long long match_digit(uint32_t d, bool valid) {
+  unsigned long long peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < DB; ++b) {
+    const bool bit = (d >> b) & 1;
+    const unsigned long long m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
 }
-// Per digit (one block each, one thread per chunk, nchunk <= kBlock): the
-// chunks' exclusive offsets within the digit, in place, and the digit total.
-template <int DB>
-__global__ __launch_bounds__(kBlock) void k_hist_chunkscan(int64_t nchunk, int64_t* __restrict__ csum,
-                                                           int64_t* __restrict__ dtot) {
-  constexpr int NB = 1 << DB;
-  __shared__ int64_t wsh[kBlock / 64];
-  const int d = blockIdx.x, c = threadIdx.x, lane = c & 63, wave = c >> 6;
-  const int64_t v = c < nchunk ? csum[static_cast<int64_t>(c) * NB + d] : 0;
-  const int64_t x = wave_inclusive_scan(v, lane);
-  if (lane == 63) wsh[wave] = x;
+
+// Exclusive scan of x over the block (every thread receives its prefix);
+// tot receives the block total.
+__device__ __forceinline__ uint32_t block_exclusive_u32(uint32_t x, uint32_t* wsh, uint32_t& tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) wsh[wave] = inc;
   __syncthreads();
-  int64_t pre = 0, tot = 0;
+  uint32_t pre = 0;
+  tot = 0;
 #pragma unroll
   for (int w = 0; w < kBlock / 64; ++w) {
     pre += w < wave ? wsh[w] : 0;
     tot += wsh[w];
   }
-  if (c < nchunk) csum[static_cast<int64_t>(c) * NB + d] = pre + x - v;
-  if (c == 0) dtot[d] = tot;
-}
-// Tile offsets within each chunk plus the chunk's offset and the digit's
-// start (an exclusive scan of the digit totals, redone by every block).
-template <int DB>
-__global__ __launch_bounds__(kBlock) void k_hist_colscan(const int64_t* __restrict__ hist, int64_t nb, int64_t rows,
-                                                         const int64_t* __restrict__ coff,
-                                                         const int64_t* __restrict__ dtot, int64_t* __restrict__ hoff) {
-  constexpr int NB = 1 << DB, DPT = NB / kBlock;
-  __shared__ int64_t part[kBlock];
-  __shared__ int64_t dstart[NB];
-  int64_t t[DPT], mine = 0;
-#pragma unroll
-  for (int j = 0; j < DPT; ++j) {
-    t[j] = dtot[threadIdx.x * DPT + j];
-    mine += t[j];
-  }
-  part[threadIdx.x] = mine;
   __syncthreads();
-  block_inclusive_scan(part);
-  int64_t st = part[threadIdx.x] - mine;
-#pragma unroll
-  for (int j = 0; j < DPT; ++j) {
-    dstart[threadIdx.x * DPT + j] = st;
-    st += t[j];
-  }
-  __syncthreads();
-  const int d = static_cast<int>(blockIdx.x) * kBlock + threadIdx.x;
-  const int64_t c = blockIdx.y, b0 = c * rows, b1 = b0 + rows < nb ? b0 + rows : nb;
-  int64_t run = dstart[d] + coff[c * NB + d];
-  for (int64_t b = b0; b < b1; ++b) {
-    const int64_t v = hist[b * NB + d];
-    hoff[b * NB + d] = run;
-    run += v;
-  }
+  return pre + inc - x;
 }
 
-// Radix pass scatter, stable, staged in LDS.  The tile's elements are ranked
-// in input order (rounds of 256; per round DB wavefront ballots give each
-// lane's rank among equal digits of its wave, per-wave digit counts in LDS
-// give the rest) and placed in LDS sorted by digit at the tile's digit
-// offsets (an exclusive scan of the tile's histogram); the tile is then
-// written out in that order, so consecutive threads store consecutive
-// addresses of one digit's run (global base of digit d for this tile:
-// hist_scan[tile * NB + d]).  KW key words per element.
-template <int KW, int DB>
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                          int64_t n, int w_digit, int shift,
-                                                          const int64_t* __restrict__ hist,
-                                                          const int64_t* __restrict__ hist_scan,
-                                                          uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                          int64_t stride) {
-  constexpr int NB = 1 << DB, DPT = NB / kBlock;  // buckets, buckets per thread
-  __shared__ uint64_t sk[KW][kRadixTile];
-  __shared__ uint32_t sv[kRadixTile];
-  __shared__ int64_t gbase[NB];
-  __shared__ int32_t loff[NB];
-  __shared__ int32_t run[NB];
-  __shared__ int16_t wcnt[kBlock / 64][NB];
-  __shared__ int64_t part[kBlock];
+// Radix pass, reduce-then-scan over a fixed grid: block b owns the sort tiles
+// [b * tpb, (b + 1) * tpb) (tiles of kBlock * sort_items keys).  k_pass_count writes the
+// block's digit counts digit-major, cnt[d * nblk + b]; their exclusive scan
+// gives every block the global start of each digit's keys from its range, and
+// k_pass_scatter walks its tiles in order, ranks each tile's keys by digit in
+// input order (stable), and writes keys and payload (S population words) to
+// their sorted positions.  No block waits on another.
+template <int KW, int S, int DB>
+__global__ __launch_bounds__(kBlock) void k_pass_count(const uint64_t* __restrict__ keys, int64_t n, int64_t stride,
+                                                       int shift, int64_t tpb, int64_t* __restrict__ cnt) {
+  constexpr int NB = 1 << DB, R = sort_items(KW, S), T = kBlock * R;
+  __shared__ uint32_t h[NB];
+  for (int d = threadIdx.x; d < NB; d += kBlock) h[d] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t lo = static_cast<int64_t>(blockIdx.x) * tpb * T;
+  const int64_t hi = lo + tpb * T < n ? lo + tpb * T : n;
+  for (int64_t base = lo; base < hi; base += T) {  // block-uniform rounds (wavefront ballots)
+    uint32_t d[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {  // every load of the tile in flight before the ballots
+      const int64_t i = base + r * kBlock + threadIdx.x;
+      uint64_t k[KW];
+#pragma unroll
+      for (int w = 0; w < KW; ++w) k[w] = i < hi ? keys[w * stride + i] : 0;
+      d[r] = key_digit<KW>(k, shift, NB - 1);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool valid = base + r * kBlock + threadIdx.x < hi;
+      const unsigned long long peers = match_digit<DB>(d[r], valid);
+      if (valid && (peers & ((1ull << lane) - 1)) == 0)
+        atomicAdd(&h[d[r]], static_cast<uint32_t>(__popcll(peers)));
+    }
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < NB; d += kBlock) cnt[static_cast<int64_t>(d) * gridDim.x + blockIdx.x] = h[d];
+}
+
+// Per digit d (one block each, nb <= kSortBlocks block counts in its row):
+// the row's exclusive prefix, in place, and the digit total in tot[d].
+__global__ __launch_bounds__(kSortBlocks) void k_pass_rowscan(int nb, int64_t* __restrict__ cnt,
+                                                              int64_t* __restrict__ tot) {
+  __shared__ int64_t wsh[kSortBlocks / 64];
+  const int d = blockIdx.x, b = threadIdx.x, lane = b & 63, wave = b >> 6;
+  const int64_t v = b < nb ? cnt[static_cast<int64_t>(d) * nb + b] : 0;
+  const int64_t x = wave_inclusive_scan(v, lane);
+  if (lane == 63) wsh[wave] = x;
+  __syncthreads();
+  int64_t pre = 0, t = 0;
+#pragma unroll
+  for (int w = 0; w < kSortBlocks / 64; ++w) {
+    pre += w < wave ? wsh[w] : 0;
+    t += wsh[w];
+  }
+  if (b < nb) cnt[static_cast<int64_t>(d) * nb + b] = pre + x - v;
+  if (b == 0) tot[d] = t;
+}
+
+template <int KW, int S, int DB>
+__global__ __launch_bounds__(kBlock) void k_pass_scatter(const uint64_t* __restrict__ kin,
+                                                         const int64_t* __restrict__ pin, uint64_t* __restrict__ kout,
+                                                         int64_t* __restrict__ pout, int64_t n, int64_t stride,
+                                                         int shift, int64_t tpb, const int64_t* __restrict__ off,
+                                                         const int64_t* __restrict__ dtot) {
+  constexpr int NB = 1 << DB, R = sort_items(KW, S), T = kBlock * R, NW = kBlock / 64;
+  constexpr uint32_t MASK = NB - 1;
+  __shared__ uint64_t sk[KW][T];
+  __shared__ int64_t sp[S][T];
+  __shared__ uint32_t wc[NW][NB];  // per-wave running digit counts, then per-wave exclusive prefixes
+  __shared__ uint32_t toff[NB];    // tile-local start of each digit
+  __shared__ int64_t run[NB];      // global position of the next key of each digit from this block
+  __shared__ uint32_t wsh[NW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // tile digit offsets: exclusive scan over the NB counts, DPT consecutive per thread
-  int64_t c[DPT], tsum = 0;
-  for (int j = 0; j < DPT; ++j) {
-    const int d = tid * DPT + j;
-    c[j] = hist[static_cast<int64_t>(blockIdx.x) * NB + d];
-    gbase[d] = hist_scan[static_cast<int64_t>(blockIdx.x) * NB + d];
-    run[d] = 0;
-    tsum += c[j];
-  }
-  part[tid] = tsum;
-  __syncthreads();
-  block_inclusive_scan(part);
-  int64_t acc = part[tid] - tsum;
-  for (int j = 0; j < DPT; ++j) {
-    loff[tid * DPT + j] = static_cast<int32_t>(acc);
-    acc += c[j];
-  }
-  const int64_t tbase = static_cast<int64_t>(blockIdx.x) * kRadixTile;
-  const int tn = static_cast<int>(n - tbase < kRadixTile ? n - tbase : kRadixTile);
-  for (int r = 0; r < kRadixItems; ++r) {
-    for (int k = tid; k < (kBlock / 64) * NB; k += kBlock) (&wcnt[0][0])[k] = 0;
-    __syncthreads();
-    const int li = r * kBlock + tid;
-    const bool valid = li < tn;
-    uint64_t kv[KW] = {};
-    uint32_t v = 0;
-    int digit = 0;
-    if (valid) {
+  {
+    // digit starts: exclusive scan of the digit totals (every block), plus
+    // the keys of this digit in the blocks before this one
+    constexpr int DPT = NB / kBlock;
+    uint32_t t[DPT], sum = 0;
 #pragma unroll
-      for (int w = 0; w < KW; ++w) kv[w] = kin[w * stride + tbase + li];
-      v = vin[tbase + li];
-      digit = static_cast<int>(((w_digit == 0 ? kv[0] : kv[KW - 1]) >> shift) & (NB - 1));
+    for (int j = 0; j < DPT; ++j) {
+      t[j] = static_cast<uint32_t>(dtot[tid * DPT + j]);
+      sum += t[j];
     }
-    unsigned long long peers = __ballot(valid);
-    for (int b = 0; b < DB; ++b) {
-      const bool bit = (digit >> b) & 1;
-      const unsigned long long m = __ballot(bit);
-      peers &= bit ? m : ~m;
-    }
-    const int rank = __popcll(peers & ((1ull << lane) - 1));
-    if (valid && rank == 0) wcnt[wave][digit] = static_cast<int16_t>(__popcll(peers));
-    __syncthreads();
-    if (valid) {
-      int pos = loff[digit] + run[digit] + rank;
-      for (int ww = 0; ww < wave; ++ww) pos += wcnt[ww][digit];
+    uint32_t all;
+    uint32_t st = block_exclusive_u32(sum, wsh, all);
 #pragma unroll
-      for (int w = 0; w < KW; ++w) sk[w][pos] = kv[w];
-      sv[pos] = v;
+    for (int j = 0; j < DPT; ++j) {
+      const int d = tid * DPT + j;
+      run[d] = static_cast<int64_t>(st) + off[static_cast<int64_t>(d) * gridDim.x + blockIdx.x];
+      st += t[j];
     }
-    __syncthreads();
-    for (int d = tid; d < NB; d += kBlock) {
-      int add = 0;
-      for (int ww = 0; ww < kBlock / 64; ++ww) add += wcnt[ww][d];
-      run[d] += add;
-    }
-    __syncthreads();  // wcnt is cleared by the next round
   }
-  for (int p = tid; p < tn; p += kBlock) {
-    const uint64_t dw = w_digit == 0 ? sk[0][p] : sk[KW - 1][p];
-    const int d = static_cast<int>((dw >> shift) & (NB - 1));
-    const int64_t g = gbase[d] + (p - loff[d]);
+  const int64_t lo = static_cast<int64_t>(blockIdx.x) * tpb * T;
+  const int64_t hi = lo + tpb * T < n ? lo + tpb * T : n;
+  for (int64_t base = lo; base < hi; base += T) {
+    for (int k = tid; k < NW * NB; k += kBlock) (&wc[0][0])[k] = 0;
+    __syncthreads();
+    // wavefront w ranks keys [base + w*64*R, base + (w+1)*64*R) round by round
+    const int64_t wbase = base + static_cast<int64_t>(wave) * 64 * R;
+    uint64_t key[R][KW];
+    int64_t pay[R][S];
+    uint32_t dig[R], rnk[R];
 #pragma unroll
-    for (int w = 0; w < KW; ++w) kout[w * stride + g] = sk[w][p];
-    vout[g] = sv[p];
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = wbase + r * 64 + lane;
+      const bool v = i < hi;
+#pragma unroll
+      for (int w = 0; w < KW; ++w) key[r][w] = v ? kin[w * stride + i] : 0;
+#pragma unroll
+      for (int q = 0; q < S; ++q) pay[r][q] = v ? pin[q * stride + i] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool valid = wbase + r * 64 + lane < hi;
+      const uint32_t d = key_digit<KW>(key[r], shift, MASK);
+      dig[r] = d;
+      const unsigned long long peers = match_digit<DB>(d, valid);
+      const uint32_t below = static_cast<uint32_t>(__popcll(peers & ((1ull << lane) - 1)));
+      const uint32_t old = valid ? wc[wave][d] : 0;
+      __builtin_amdgcn_wave_barrier();
+      if (valid && below == 0) wc[wave][d] = old + static_cast<uint32_t>(__popcll(peers));
+      __builtin_amdgcn_wave_barrier();
+      rnk[r] = old + below;
+    }
+    __syncthreads();
+    // tile digit counts (wc becomes the waves' exclusive prefixes per digit)
+    constexpr int DPT = NB / kBlock;
+    uint32_t cnt[DPT], tsum = 0;
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+      const int d = tid * DPT + j;
+      uint32_t c = 0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        const uint32_t x = wc[w][d];
+        wc[w][d] = c;
+        c += x;
+      }
+      cnt[j] = c;
+      tsum += c;
+    }
+    uint32_t ttot;
+    uint32_t to = block_exclusive_u32(tsum, wsh, ttot);
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+      toff[tid * DPT + j] = to;
+      to += cnt[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (wbase + r * 64 + lane < hi) {
+        const uint32_t d = dig[r];
+        const uint32_t pos = toff[d] + wc[wave][d] + rnk[r];
+#pragma unroll
+        for (int w = 0; w < KW; ++w) sk[w][pos] = key[r][w];
+#pragma unroll
+        for (int q = 0; q < S; ++q) sp[q][pos] = pay[r][q];
+      }
+    }
+    __syncthreads();
+    const int tn = static_cast<int>(hi - base < T ? hi - base : T);
+    for (int p = tid; p < tn; p += kBlock) {
+      uint64_t k[KW];
+#pragma unroll
+      for (int w = 0; w < KW; ++w) k[w] = sk[w][p];
+      const uint32_t d = key_digit<KW>(k, shift, MASK);
+      const int64_t g = run[d] + (p - static_cast<int>(toff[d]));
+#pragma unroll
+      for (int w = 0; w < KW; ++w) kout[w * stride + g] = k[w];
+#pragma unroll
+      for (int q = 0; q < S; ++q) pout[q * stride + g] = sp[q][p];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) run[tid * DPT + j] += cnt[j];
+    // (the next tile's first barrier orders these updates before their use)
   }
 }
 
-// Sort keys as combinatorial ranks: for a bit string with set bits
-// p_1 < p_2 < ... < p_k (0-based), rank = sum_i C(p_i, i).  For a fixed number
-// of set bits this is a bijection onto [0, C(nbasis, k)) that preserves the
-// numeric order of the strings, so sorting ranks sorts determinants in the
-// main list's order with ceil(log2 C(nbasis, nel)) key bits instead of nbasis
-// (34 instead of 52 for 10 electrons in 52 spin orbitals).  binom holds
-// C(p, i) for p < nbasis, 1 <= i <= nel as (low, high) 64-bit halves.
-// stn (may be null): the spawned-list length is read from the stats on the
-// device (ns is then only the grid's bound).
-__global__ void k_rank_keys(const uint64_t* __restrict__ sdet, int64_t ns, const Stats* __restrict__ stn, int W,
-                            int64_t scap, const uint64_t* __restrict__ binom, int nel, int rank_words,
-                            uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  if (stn) ns = stn->nspawn < scap ? stn->nspawn : scap;
+// Sort keys and payload from the spawned list (determinant words, records of
+// populations and flag): entry i gets those of entry perm[i] (perm may be
+// null: the identity).
+__global__ void k_make_keys(KeyPlan kp, int S, const uint64_t* __restrict__ sdet, const int64_t* __restrict__ srec,
+                            int W, int64_t ns, int64_t scap, const uint32_t* __restrict__ perm,
+                            uint64_t* __restrict__ keys, int64_t* __restrict__ pay) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= ns) return;
-  unsigned __int128 rank = 0;
-  int k = 0;
-  for (int w = 0; w < W; ++w) {
-    for (uint64_t x = sdet[w * scap + i]; x; x &= x - 1) {
-      const int p = (w << 6) + __builtin_ctzll(x);
-      const uint64_t* c = binom + 2 * (static_cast<int64_t>(p) * nel + k);
-      rank += (static_cast<unsigned __int128>(c[1]) << 64) | c[0];
-      ++k;
-    }
-  }
-  keys[i] = static_cast<uint64_t>(rank);
-  if (rank_words > 1) keys[scap + i] = static_cast<uint64_t>(rank >> 64);
-  vals[i] = static_cast<uint32_t>(i);
-}
-
-__global__ void k_iota_copy_keys(const uint64_t* __restrict__ sdet, int64_t ns, const Stats* __restrict__ stn, int W,
-                                 int64_t scap, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  if (stn) ns = stn->nspawn < scap ? stn->nspawn : scap;
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= ns) return;
-  for (int w = 0; w < W; ++w) keys[w * scap + i] = sdet[w * scap + i];
-  vals[i] = static_cast<uint32_t>(i);
+  const int64_t src = perm ? static_cast<int64_t>(perm[i]) : i;
+  uint64_t f[FQ_MAX_WORDS] = {0, 0};
+  FQ_FOR_WORDS(w, W) f[w] = sdet[w * scap + src];
+  const int64_t* rec = srec + src * rec_stride(S);
+  uint64_t k[2];
+  make_key(kp, f, W, static_cast<uint64_t>(rec[S]), k);
+  for (int w = 0; w < kp.kw; ++w) keys[w * scap + i] = k[w];
+  for (int q = 0; q < S; ++q) pay[q * scap + i] = rec[q];
 }
 
 // Replay: the reference's quicksort (src/sort.f90:213-396) on an index array,
 // one thread; comparisons read the spawned bit strings.
 __global__ void k_replay_qsort(const uint64_t* __restrict__ sdet, int64_t ns, int W, int64_t scap,
-                               uint32_t* __restrict__ idx, uint64_t* __restrict__ keys_sorted) {
+                               uint32_t* __restrict__ idx) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   for (int64_t i = 0; i < ns; ++i) idx[i] = static_cast<uint32_t>(i);
   auto gt = [&](uint32_t a, uint32_t b) { return key_cmp(sdet, scap, W, a, b) > 0; };
@@ -1381,146 +1482,220 @@ __global__ void k_replay_qsort(const uint64_t* __restrict__ sdet, int64_t ns, in
       }
     }
   }
-  for (int64_t i = 0; i < ns; ++i)
-    for (int w = 0; w < W; ++w) keys_sorted[w * scap + i] = sdet[w * scap + idx[i]];
 }
 
 // ---------------------------------------------------------------------------
 // Kernels: self-annihilation of the sorted spawned list
 // ---------------------------------------------------------------------------
-// Runs of equal keys in the sorted list, per tile of kRunTile entries: the
-// tile's run heads (first entry, or key differing from the previous one) are
-// ranked in order by wavefront ballots and written contiguously from the
-// tile's first slot of run_stage, with the tile's count in tile_cnt; each
-// entry's populations and flag are gathered through the sort permutation into
-// sorted order (gpop, gflag) for the run reduction.
-__global__ __launch_bounds__(kBlock) void k_run_heads(const uint64_t* __restrict__ keys, int64_t ns, int kw,
-                                                      int64_t scap, int S, const uint32_t* __restrict__ perm,
-                                                      const int64_t* __restrict__ srec,
-                                                      int64_t* __restrict__ gpop, int64_t* __restrict__ gflag,
-                                                      int64_t* __restrict__ run_stage,
-                                                      int64_t* __restrict__ tile_cnt) {
-  constexpr int R = kRunTile / kBlock;  // rounds of a tile
-  __shared__ int32_t wcount[R][kBlock / 64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t ntiles = (ns + kRunTile - 1) / kRunTile;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    // every round's loads and gathers first (independent, so they overlap),
-    // then the ballot ranks of all rounds after one barrier
-    bool head[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int64_t i = tile * kRunTile + k * kBlock + threadIdx.x;
-      head[k] = false;
-      if (i < ns) {
-        bool h = i == 0;
-#pragma unroll
-        for (int w = 0; w < FQ_MAX_WORDS; ++w)
-          if (w < kw && i > 0) h = h || keys[w * scap + i] != keys[w * scap + i - 1];
-        head[k] = h;
-        const int64_t* rec = srec + static_cast<int64_t>(perm[i]) * rec_stride(S);
-        for (int sp = 0; sp < S; ++sp) gpop[sp * scap + i] = rec[sp];
-        gflag[i] = rec[S];
-      }
-    }
-    unsigned long long bal[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      bal[k] = __ballot(head[k]);
-      if (lane == 0) wcount[k][wave] = __popcll(bal[k]);
-    }
-    __syncthreads();
-    int32_t run = 0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      int32_t pos = run + __popcll(bal[k] & ((1ull << lane) - 1));
-      int32_t round_total = 0;
-      for (int w = 0; w < kBlock / 64; ++w) {
-        if (w < wave) pos += wcount[k][w];
-        round_total += wcount[k][w];
-      }
-      if (head[k]) run_stage[tile * kRunTile + pos] = tile * kRunTile + k * kBlock + threadIdx.x;
-      run += round_total;
-    }
-    if (threadIdx.x == 0) tile_cnt[tile] = run;
-    __syncthreads();
-  }
+// Equal determinants are adjacent in the sorted keys (equal value fields).
+template <int KW>
+__device__ __forceinline__ bool same_value(const uint64_t* a, const uint64_t* b, int low_bits) {
+  if constexpr (KW == 1) return (a[0] >> low_bits) == (b[0] >> low_bits);
+  else return a[1] == b[1] && (a[0] >> low_bits) == (b[0] >> low_bits);
 }
 
-// run_start[tile_off[t] + j] for the tile's j-th run; run_start[nruns] = ns.
-__global__ __launch_bounds__(kBlock) void k_pack_runs(int64_t ntiles, int64_t ns,
-                                                      const int64_t* __restrict__ tile_cnt,
-                                                      const int64_t* __restrict__ tile_off,
-                                                      const int64_t* __restrict__ run_stage,
-                                                      int64_t* __restrict__ run_start) {
-  const int64_t tile = blockIdx.x;
-  if (tile >= ntiles) return;
-  const int64_t cnt = tile_cnt[tile], base = tile_off[tile];
-  for (int64_t j = threadIdx.x; j < cnt; j += kBlock) run_start[base + j] = run_stage[tile * kRunTile + j];
-  if (tile == ntiles - 1 && threadIdx.x == 0) run_start[tile_off[ntiles]] = ns;
-}
-
-// One thread per run of equal keys: annihilate_spawn_t(_initiator)
-// (src/spawn_data.F90:859-1101).  The run's first entry is its first in
-// sorted order: the reference's quicksort order in replay mode, the earliest
-// spawned in production (stable sort of an attempt-ordered list).
-__global__ void k_reduce_runs(Sys s, const int64_t* __restrict__ pnr, const int64_t* __restrict__ run_start,
-                              const uint64_t* __restrict__ sdet, const uint32_t* __restrict__ perm,
-                              const int64_t* __restrict__ gpop, const int64_t* __restrict__ gflag, int64_t scap,
-                              uint64_t* __restrict__ rdet, int64_t* __restrict__ rpop, int64_t* __restrict__ rflag,
-                              int64_t* __restrict__ rstate) {
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t nruns = *pnr;
-  if (r >= nruns) return;
-  const int64_t a = run_start[r], b = run_start[r + 1];
-  // the run's determinant (a gather through the permutation) is loaded first,
-  // so its latency overlaps the run's population sum
-  const uint32_t j = perm[a];
-  uint64_t d[FQ_MAX_WORDS] = {0, 0};
-  FQ_FOR_WORDS(w, s.W) d[w] = sdet[w * scap + j];
-  int64_t tot[FQ_MAX_SPACES] = {0, 0}, init[FQ_MAX_SPACES] = {0, 0};
-  int events[FQ_MAX_SPACES] = {0, 0};
-  const int64_t f0 = gflag[a];
-  for (int sp = 0; sp < s.S; ++sp) {
-    const int64_t v = gpop[sp * scap + a];
-    tot[sp] = v;
-    if (!((f0 >> sp) & 1)) {
-      init[sp] = v;
+// The determinant of a combinatorial rank: electron by electron from the
+// highest, the largest p with C(p, i) <= the remaining rank is the i-th
+// lowest set bit.  The highest is found by a binary search (7 steps cover 128
+// orbitals), the others by stepping down from the one above (p only
+// decreases; low-lying determinants have small gaps).  binom: the table (in
+// LDS when staged).
+template <bool WIDE>
+__device__ __forceinline__ void unrank(const KeyPlan& kp, const uint64_t* __restrict__ binom,
+                                       unsigned __int128 rank, uint64_t* f) {
+  using V = typename std::conditional<WIDE, unsigned __int128, uint64_t>::type;
+  auto C = [&](int p, int i) -> V {
+    const uint64_t* t = binom + 2 * (p * kp.nel + (i - 1));
+    if constexpr (WIDE) {
+      return (static_cast<unsigned __int128>(t[1]) << 64) | t[0];
     } else {
-      const int32_t v32 = static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(v) & 0xffffffffULL));
-      events[sp] = v32 >= 0 ? 1 : -1;
+      return t[1] ? ~uint64_t{0} : t[0];  // above the 64-bit ranks: larger than any of them
+    }
+  };
+  V r = static_cast<V>(rank);
+  f[0] = f[1] = 0;
+  const int nel = kp.nel;
+  int lo = nel - 1, hi = kp.nbasis - 1;
+#pragma unroll
+  for (int step = 0; step < 7; ++step) {
+    const int mid = (lo + hi + 1) >> 1;
+    const bool ok = C(mid, nel) <= r;
+    const bool live = lo < hi;
+    lo = live && ok ? mid : lo;
+    hi = live && !ok ? mid - 1 : hi;
+  }
+  int p = lo;
+  for (int i = nel; i >= 1; --i) {
+    V c = C(p, i);
+    while (c > r && p > 0) c = C(--p, i);
+    f[p >> 6] |= uint64_t{1} << (p & 63);
+    r -= c;
+    --p;
+  }
+}
+
+// Run heads of each tile of kRunTile sorted keys (first key, or value
+// differing from the previous key): their count per tile.
+template <int KW>
+__global__ __launch_bounds__(kBlock) void k_run_count(const uint64_t* __restrict__ keys, int64_t ns, int64_t stride,
+                                                      int fb, int64_t* __restrict__ tcnt) {
+  constexpr int R = kRunTile / kBlock;
+  __shared__ int64_t wsh[kBlock / 64];
+  const int64_t tile = blockIdx.x;
+  int64_t c = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t i = tile * kRunTile + r * kBlock + threadIdx.x;
+    if (i < ns) {
+      uint64_t k[KW], prev[KW];
+#pragma unroll
+      for (int w = 0; w < KW; ++w) {
+        k[w] = keys[w * stride + i];
+        prev[w] = i > 0 ? keys[w * stride + i - 1] : 0;
+      }
+      c += (i == 0 || !same_value<KW>(k, prev, fb)) ? 1 : 0;
     }
   }
-  for (int64_t k = a + 1; k < b; ++k) {
-    const int64_t fk = gflag[k];
-    for (int sp = 0; sp < s.S; ++sp) {
-      const int64_t v = gpop[sp * scap + k];
-      if (!((fk >> sp) & 1))
-        init[sp] += v;
-      else if (v < 0)
-        events[sp] -= 1;
-      else if (v > 0)
-        events[sp] += 1;
-      tot[sp] += v;
+  const int64_t t = block_sum(c, wsh);
+  if (threadIdx.x == 0) tcnt[tile] = t;
+}
+
+// annihilate_spawn_t(_initiator) (src/spawn_data.F90:859-1101) over the
+// sorted keys and populations, one block per tile of kRunTile keys: the
+// tile's run heads (first key, or value differing from the previous key) are
+// listed in order in LDS by wavefront ballots, from the tile's run offset
+// (toff, the exclusive scan of k_run_count); then thread t reduces heads t,
+// t + kBlock, ... : the run (which may extend past the tile) from the
+// contiguous sorted populations and the flag bits of the keys, its
+// determinant from its key, written at the run's index.  The run's first
+// entry is its first in sorted order: the reference's quicksort order in
+// replay mode, the earliest spawned in production (the stable sort of an
+// attempt-ordered list).
+template <int KW, bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_runs(Sys s, KeyPlan kp, const uint64_t* __restrict__ keys,
+                                                 const int64_t* __restrict__ pay, int64_t ns, int64_t stride,
+                                                 const int64_t* __restrict__ toff, uint64_t* __restrict__ rdet,
+                                                 int64_t* __restrict__ rpop, int64_t* __restrict__ rflag,
+                                                 int64_t* __restrict__ rstate) {
+  constexpr int R = kRunTile / kBlock;
+  __shared__ int32_t wcount[R][kBlock / 64];
+  __shared__ int32_t hpos[kRunTile];  // tile-relative index of each head, in order
+  extern __shared__ uint64_t binom_lds[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool staged = kp.rank && kp.binom_lds;
+  if (staged) {
+    const int nw = 2 * kp.nbasis * kp.nel;
+    for (int k = tid; k < nw; k += kBlock) binom_lds[k] = kp.binom[k];
+  }
+  const int64_t tile = blockIdx.x, t0 = tile * kRunTile;
+  const int fb = kp.flag_bits;
+  bool head[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t i = t0 + r * kBlock + tid;
+    head[r] = false;
+    if (i < ns) {
+      uint64_t k[KW], prev[KW];
+#pragma unroll
+      for (int w = 0; w < KW; ++w) {
+        k[w] = keys[w * stride + i];
+        prev[w] = i > 0 ? keys[w * stride + i - 1] : 0;
+      }
+      head[r] = i == 0 || !same_value<KW>(k, prev, fb);
     }
   }
-  int64_t fl = 0;
-  bool any = false;
-  for (int sp = 0; sp < s.S; ++sp) {
-    any = any || tot[sp] != 0;
-    if (s.initiator) {
-      const int st = tot[sp] >= 0 ? 1 : -1, si = init[sp] >= 0 ? 1 : -1;
-      if (init[sp] != 0 && st == si) {
-      } else if (abs(events[sp]) > 1) {
-      } else {
-        fl += int64_t{1} << sp;
+  unsigned long long bal[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    bal[r] = __ballot(head[r]);
+    if (lane == 0) wcount[r][wave] = __popcll(bal[r]);
+  }
+  __syncthreads();
+  int32_t total = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    int32_t p = total + __popcll(bal[r] & ((1ull << lane) - 1));
+    for (int w = 0; w < kBlock / 64; ++w) {
+      if (w < wave) p += wcount[r][w];
+      total += wcount[r][w];
+    }
+    if (head[r]) hpos[p] = r * kBlock + tid;
+  }
+  __syncthreads();  // also orders the table's staging
+  const int64_t rbase = toff[tile];
+  const uint64_t fmask = (uint64_t{1} << fb) - 1;
+  for (int h = tid; h < total; h += kBlock) {
+    const int64_t i = t0 + hpos[h];
+    uint64_t kk[KW];
+#pragma unroll
+    for (int w = 0; w < KW; ++w) kk[w] = keys[w * stride + i];
+    int64_t end;
+    if (h + 1 < total) {
+      end = t0 + hpos[h + 1];
+    } else {  // the tile's last run may continue into the next tiles
+      end = t0 + kRunTile < ns ? t0 + kRunTile : ns;
+      for (; end < ns; ++end) {
+        uint64_t kj[KW];
+#pragma unroll
+        for (int w = 0; w < KW; ++w) kj[w] = keys[w * stride + end];
+        if (!same_value<KW>(kj, kk, fb)) break;
       }
     }
-    rpop[sp * scap + r] = tot[sp];
+    int64_t tot[FQ_MAX_SPACES] = {0, 0}, init[FQ_MAX_SPACES] = {0, 0};
+    int events[FQ_MAX_SPACES] = {0, 0};
+    const int64_t f0 = static_cast<int64_t>(kk[0] & fmask);
+    for (int sp = 0; sp < s.S; ++sp) {
+      const int64_t v = pay[sp * stride + i];
+      tot[sp] = v;
+      if (!((f0 >> sp) & 1)) {
+        init[sp] = v;
+      } else {  // the first entry's event count: sign of the population as a default integer
+        const int32_t v32 = static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(v) & 0xffffffffULL));
+        events[sp] = v32 >= 0 ? 1 : -1;
+      }
+    }
+    for (int64_t j = i + 1; j < end; ++j) {
+      const int64_t fk = static_cast<int64_t>(keys[j] & fmask);  // flags sit in word 0
+      for (int sp = 0; sp < s.S; ++sp) {
+        const int64_t v = pay[sp * stride + j];
+        if (!((fk >> sp) & 1))
+          init[sp] += v;
+        else if (v < 0)
+          events[sp] -= 1;
+        else if (v > 0)
+          events[sp] += 1;
+        tot[sp] += v;
+      }
+    }
+    const int64_t run = rbase + h;
+    int64_t fl = 0;
+    bool any = false;
+    for (int sp = 0; sp < s.S; ++sp) {
+      any = any || tot[sp] != 0;
+      if (s.initiator) {
+        const int st = tot[sp] >= 0 ? 1 : -1, si = init[sp] >= 0 ? 1 : -1;
+        if (init[sp] != 0 && st == si) {
+        } else if (abs(events[sp]) > 1) {
+        } else {
+          fl += int64_t{1} << sp;
+        }
+      }
+      rpop[sp * stride + run] = tot[sp];
+    }
+    uint64_t d[2] = {0, 0};
+    const unsigned __int128 v = (KW == 1 ? static_cast<unsigned __int128>(kk[0])
+                                         : (static_cast<unsigned __int128>(kk[KW - 1]) << 64) | kk[0]) >> fb;
+    if (!kp.rank) {
+      d[0] = static_cast<uint64_t>(v);  // batched instances: the encoded word itself
+    } else if (staged) {
+      unrank<WIDE>(kp, binom_lds, v, d);
+    } else {
+      unrank<WIDE>(kp, kp.binom, v, d);
+    }
+    FQ_FOR_WORDS(w, s.W) rdet[w * stride + run] = d[w];
+    rflag[run] = fl;
+    rstate[run] = any ? 1 : 0;
   }
-  FQ_FOR_WORDS(w, s.W) rdet[w * scap + r] = d[w];
-  rflag[r] = fl;
-  rstate[r] = any ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1989,117 +2164,161 @@ int sync_stats(fq_engine* e) {
   return FQ_OK;
 }
 
-// Sort keys and slots of the spawned list: combinatorial ranks, or the
-// determinant words when those are no longer.  stn: read the length on the
-// device (ns is then a bound), so the keys can be made before the host knows it.
-void launch_sort_keys(fq_engine* e, int64_t ns, const Stats* stn) {
-  if (e->rank_words)
-    hipLaunchKernelGGL(k_rank_keys, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->sdet, ns, stn, e->W, e->scap,
-                       e->d_binom, e->sys.nel, e->rank_words, e->kbuf[0], e->vbuf[0]);
-  else
-    hipLaunchKernelGGL(k_iota_copy_keys, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, e->sdet, ns, stn, e->W,
-                       e->scap, e->kbuf[0], e->vbuf[0]);
+KeyPlan key_plan(const fq_engine* e) {
+  KeyPlan kp{};
+  kp.kw = e->sort_kw;
+  kp.flag_bits = e->flag_bits;
+  kp.rank = e->rank_words ? 1 : 0;
+  kp.nel = e->sys.nel;
+  kp.nbasis = e->sys.nbasis;
+  kp.binom = e->d_binom;
+  return kp;
+}
+
+// Bytes of the rank table, and whether kernels may stage it in LDS.
+size_t binom_bytes(const fq_engine* e) {
+  return e->rank_words ? sizeof(uint64_t) * 2 * e->sys.nbasis * e->sys.nel : 0;
+}
+constexpr size_t kBinomLdsMax = 48 * 1024;
+
+// Digits of the sort: the fewest passes of 8- or 9-bit digits covering the
+// key's value field (bits flag_bits .. flag_bits + value_bits - 1).
+DigitPlan digit_plan(const fq_engine* e) {
+  DigitPlan dp{};
+  const int bits = e->value_bits;
+  const int passes = std::max(1, (bits + 8) / 9), n9 = std::max(0, std::min(passes, bits - 8 * passes));
+  int shift = e->flag_bits;
+  for (int p = 0; p < passes; ++p) {
+    dp.shift[p] = shift;
+    dp.bits[p] = p < n9 ? 9 : 8;
+    shift += dp.bits[p];
+  }
+  dp.np = passes;
+  return dp;
+}
+
+// Key layout and sort buffers for the engine's value width.
+int setup_sort(fq_engine* e) {
+  const int vb = e->rank_words ? e->rank_bits : e->key_bits;
+  const int fb = e->S;
+  if (vb + fb > 128) return FQ_ERR_ARG;
+  const int kw = vb + fb > 64 ? 2 : 1;
+  if (e->batched && kw != 1) return FQ_ERR_ARG;  // the instance bounds read one key word
+  const int64_t scap = e->scap;
+  e->value_bits = vb;
+  e->flag_bits = fb;
+  int rc = FQ_OK;
+  // The spawned list as determinant words and records (sdet, srec) is only
+  // read before the sort (replay walk, caller-supplied lists, rank exchange);
+  // it shares the memory of the sort's second buffers.
+  if (kw != e->sort_kw || !e->kbuf[0]) {
+    for (int b = 0; b < 2; ++b) {
+      dfree(e->kbuf[b]);
+      rc |= dalloc(&e->kbuf[b], (b == 0 ? kw : std::max(kw, e->W)) * scap);
+    }
+  }
+  e->sort_kw = kw;
+  for (int b = 0; b < 2; ++b)
+    if (!e->pbuf[b]) rc |= dalloc(&e->pbuf[b], (b == 0 ? e->S : std::max(e->S, rec_stride(e->S))) * scap);
+  e->sdet = e->kbuf[1];
+  e->srec = e->pbuf[1];
+  if (!e->perm && e->par.rng_mode == FQ_RNG_REPLAY) rc |= dalloc(&e->perm, scap);
+  if (!e->pass_cnt) rc |= dalloc(&e->pass_cnt, 512 * kSortBlocks);
+  if (!e->pass_off) rc |= dalloc(&e->pass_off, 512 * kSortBlocks + 1);
+  if (!e->run_tcnt) rc |= dalloc(&e->run_tcnt, ceil_div(scap, kRunTile) + 2);
+  if (!e->run_toff) rc |= dalloc(&e->run_toff, ceil_div(scap, kRunTile) + 2);
+  return rc == FQ_OK ? FQ_OK : FQ_ERR_NOMEM;
+}
+
+// Sort keys and payload made from the spawned list's determinants and records
+// (when the spawning pass did not make them): perm (may be null) gives the order.
+void launch_make_keys(fq_engine* e, int64_t ns, const uint32_t* perm) {
+  hipLaunchKernelGGL(k_make_keys, dim3(grid_for(ns)), dim3(kBlock), 0, e->stream, key_plan(e), e->S, e->sdet, e->srec,
+                     e->W, ns, e->scap, perm, e->kbuf[0], e->pbuf[0]);
+}
+
+// One radix pass from buffer `src` to the other.
+template <int KW, int S, int DB>
+int sort_pass(fq_engine* e, int src, int64_t ns, int shift) {
+  constexpr int T = kBlock * sort_items(KW, S);
+  const int64_t ntiles = ceil_div(ns, T);
+  const int64_t nblk = std::min<int64_t>(ntiles, kSortBlocks), tpb = ceil_div(ntiles, nblk);
+  const int64_t nb = ceil_div(ntiles, tpb);  // blocks that own tiles
+  hipLaunchKernelGGL((k_pass_count<KW, S, DB>), dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, e->stream,
+                     e->kbuf[src], ns, e->scap, shift, tpb, e->pass_cnt);
+  hipLaunchKernelGGL(k_pass_rowscan, dim3(1u << DB), dim3(kSortBlocks), 0, e->stream, static_cast<int>(nb),
+                     e->pass_cnt, e->pass_off);
+  hipLaunchKernelGGL((k_pass_scatter<KW, S, DB>), dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, e->stream,
+                     e->kbuf[src], e->pbuf[src], e->kbuf[1 - src], e->pbuf[1 - src], ns, e->scap, shift, tpb,
+                     e->pass_cnt, e->pass_off);
+  return FQ_OK;
+}
+template <int KW, int S>
+int sort_pass_db(fq_engine* e, int src, int64_t ns, int shift, int db) {
+  return db == 9 ? sort_pass<KW, S, 9>(e, src, ns, shift) : sort_pass<KW, S, 8>(e, src, ns, shift);
 }
 
 // Sort the spawned list (ns entries) and reduce runs; leaves the device
 // address of the run count in p_nruns and a host bound of it in run_bound.
 int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
-  const bool keys_ready = e->keys_ready;  // made by the cycle for this spawned list
+  const bool keys_ready = e->keys_ready;  // made by the spawning pass for this spawned list
   e->keys_ready = false;
-  const int W = e->W;
   const int64_t scap = e->scap;
-  int kw = W, kbits = e->key_bits;  // key words and significant bits the sort orders
-  uint32_t* perm = nullptr;
-  uint64_t* keys = nullptr;
+  const int kw = e->sort_kw, S = e->S;
+  hipStream_t st = e->stream;
+  int src = 0;
   e->radix_passes = 0;
-  if (!keys_ready) FQ_HIP(hipEventRecord(e->pev[4], e->stream));  // else recorded before the keys
+  FQ_HIP(hipEventRecord(e->pev[4], st));
   if (ns > 0) {
     if (replay) {
-      hipLaunchKernelGGL(k_replay_qsort, dim3(1), dim3(1), 0, e->stream, e->sdet, ns, W, scap, e->vbuf[0],
-                         e->kbuf[0]);
-      perm = e->vbuf[0];
-      keys = e->kbuf[0];
+      hipLaunchKernelGGL(k_replay_qsort, dim3(1), dim3(1), 0, st, e->sdet, ns, e->W, scap, e->perm);
+      launch_make_keys(e, ns, e->perm);
     } else {
-      if (e->rank_words) {
-        kw = e->rank_words;
-        kbits = e->rank_bits;
+      if (!keys_ready) launch_make_keys(e, ns, nullptr);
+      const DigitPlan dp = digit_plan(e);
+      for (int p = 0; p < dp.np; ++p) {
+        int rc;
+        if (kw == 1)
+          rc = S == 1 ? sort_pass_db<1, 1>(e, src, ns, dp.shift[p], dp.bits[p])
+                      : sort_pass_db<1, 2>(e, src, ns, dp.shift[p], dp.bits[p]);
+        else
+          rc = S == 1 ? sort_pass_db<2, 1>(e, src, ns, dp.shift[p], dp.bits[p])
+                      : sort_pass_db<2, 2>(e, src, ns, dp.shift[p], dp.bits[p]);
+        if (rc != FQ_OK) return rc;
+        src = 1 - src;
+        ++e->radix_passes;
       }
-      if (!keys_ready) launch_sort_keys(e, ns, nullptr);
-      const int64_t nb = ceil_div(ns, kRadixTile);
-      int64_t *hist = e->hist, *hist_scan = e->hist_scan;
-      if (e->noalloc) {
-        hist = scan_scratch(e, 4, 512 * nb + 1);
-        hist_scan = scan_scratch(e, 5, 512 * nb + 1);
-        if (!hist || !hist_scan) return FQ_ERR_NOMEM;
-      } else if (e->hist_cap < 512 * nb + 1) {
-        dfree(e->hist);
-        dfree(e->hist_scan);
-        if (dalloc(&e->hist, 512 * nb + 1) != FQ_OK || dalloc(&e->hist_scan, 512 * nb + 1) != FQ_OK)
-          return FQ_ERR_NOMEM;
-        e->hist_cap = 512 * nb + 1;
-        hist = e->hist;
-        hist_scan = e->hist_scan;
-      }
-      int src = 0;
-      for (int w = 0; w < kw; ++w) {
-        const int bits = std::min(64, kbits - 64 * w);
-        // fewest passes of 8- or 9-bit digits covering the word's bits
-        const int passes = std::max(1, (bits + 8) / 9), n9 = std::max(0, std::min(passes, bits - 8 * passes));
-        for (int pidx = 0, shift = 0; pidx < passes; ++pidx) {
-          const int db = pidx < n9 ? 9 : 8;
-          auto khist = db == 9 ? k_radix_hist<9> : k_radix_hist<8>;
-          hipLaunchKernelGGL(khist, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src] + w * scap, ns, shift, hist);
-          const int64_t rows = std::max<int64_t>(16, ceil_div(nb, kHistChunks)), nchunk = ceil_div(nb, rows);
-          int64_t* csum = scan_scratch(e, 2, nchunk * 512);
-          int64_t* dtot = scan_scratch(e, 3, 512);
-          if (!csum || !dtot) return FQ_ERR_NOMEM;
-          const dim3 cg(static_cast<unsigned>((1 << db) / kBlock), static_cast<unsigned>(nchunk));
-          hipLaunchKernelGGL(db == 9 ? k_hist_colsum<9> : k_hist_colsum<8>, cg, dim3(kBlock), 0, e->stream, hist, nb,
-                             rows, csum);
-          hipLaunchKernelGGL(db == 9 ? k_hist_chunkscan<9> : k_hist_chunkscan<8>, dim3(1u << db), dim3(kBlock), 0,
-                             e->stream, nchunk, csum, dtot);
-          hipLaunchKernelGGL(db == 9 ? k_hist_colscan<9> : k_hist_colscan<8>, cg, dim3(kBlock), 0, e->stream, hist,
-                             nb, rows, csum, dtot, hist_scan);
-          auto scatter = kw == 1 ? (db == 9 ? k_radix_scatter<1, 9> : k_radix_scatter<1, 8>)
-                                 : (db == 9 ? k_radix_scatter<2, 9> : k_radix_scatter<2, 8>);
-          hipLaunchKernelGGL(scatter, dim3(nb), dim3(kBlock), 0, e->stream, e->kbuf[src], e->vbuf[src], ns, w, shift,
-                             hist, hist_scan, e->kbuf[1 - src], e->vbuf[1 - src], scap);
-          src = 1 - src;
-          shift += db;
-          ++e->radix_passes;
-        }
-      }
-      perm = e->vbuf[src];
-      keys = e->kbuf[src];
     }
   }
-  FQ_HIP(hipEventRecord(e->pev[5], e->stream));
+  FQ_HIP(hipEventRecord(e->pev[5], st));
+  const int64_t nt = ceil_div(ns, kRunTile);
+  int64_t* nruns = e->run_toff + nt;
   if (ns > 0) {
-    const int64_t nt = ceil_div(ns, kRunTile);
-    hipLaunchKernelGGL(k_run_heads, dim3(static_cast<unsigned>(std::min<int64_t>(nt, 1 << 20))), dim3(kBlock), 0,
-                       e->stream, keys, ns, kw, scap, e->S, perm, e->srec, e->gpop, e->gflag, e->run_stage,
-                       e->run_tcnt);
-    int rc = scan_exclusive(e, e->run_tcnt, e->run_toff, nt);
+    hipLaunchKernelGGL(kw == 1 ? k_run_count<1> : k_run_count<2>, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0,
+                       st, e->kbuf[src], ns, scap, e->flag_bits, e->run_tcnt);
+    const int rc = scan_exclusive(e, e->run_tcnt, e->run_toff, nt);
     if (rc != FQ_OK) return rc;
-    hipLaunchKernelGGL(k_pack_runs, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0, e->stream, nt, ns, e->run_tcnt,
-                       e->run_toff, e->run_stage, e->run_start);
+    KeyPlan kp = key_plan(e);
+    const size_t tab = binom_bytes(e);
+    kp.binom_lds = tab > 0 && tab <= kBinomLdsMax;
+    auto kern = kw == 1 ? k_runs<1, false> : k_runs<2, true>;  // one-word keys: 64-bit rank arithmetic
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nt)), dim3(kBlock), kp.binom_lds ? tab : 0, st, e->sys, kp,
+                       e->kbuf[src], e->pbuf[src], ns, scap, e->run_toff, e->rdet, e->rpop, e->rflag, e->rstate);
   }
-  // The number of runs stays on the device (run_toff[nt]); kernels size their
-  // grids by a host bound: ns in production, the exact count in replay (read
-  // there anyway, for the rounding draws).
+  // The number of runs stays on the device; kernels size their grids by a
+  // host bound: ns in production, the exact count in replay (read there
+  // anyway, for the rounding draws).
   int64_t bound = 0;
   e->p_nruns = nullptr;
   if (ns > 0) {
-    e->p_nruns = e->run_toff + ceil_div(ns, kRunTile);
+    e->p_nruns = nruns;
     bound = ns;
     if (replay) {
-      FQ_HIP(hipMemcpyAsync(&bound, e->p_nruns, sizeof(int64_t), hipMemcpyDeviceToHost, e->stream));
-      FQ_HIP(hipStreamSynchronize(e->stream));
+      FQ_HIP(hipMemcpyAsync(&bound, nruns, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+      FQ_HIP(hipStreamSynchronize(st));
     }
-    hipLaunchKernelGGL(k_reduce_runs, dim3(grid_for(bound)), dim3(kBlock), 0, e->stream, e->sys, e->p_nruns,
-                       e->run_start, e->sdet, perm, e->gpop, e->gflag, scap, e->rdet, e->rpop, e->rflag, e->rstate);
   }
-  FQ_HIP(hipEventRecord(e->pev[6], e->stream));
+  FQ_HIP(hipEventRecord(e->pev[6], st));
   e->run_bound = bound;
   return FQ_OK;
 }
@@ -2348,8 +2567,7 @@ int build_heat_bath(fq_engine* e) {
 template <class T>
 int upload(T** dst, const T* src, int64_t count, hipStream_t st);
 
-// Binomial table and key width for the production sort on combinatorial ranks
-// (used when the rank needs fewer bits than the bit string).
+// Binomial table and value width of the sort keys (combinatorial ranks).
 int setup_rank_keys(fq_engine* e) {
   const int nb = e->sys.nbasis, nel = e->sys.nel;
   if (nel < 1 || nel > nb) return FQ_OK;
@@ -2371,7 +2589,6 @@ int setup_rank_keys(fq_engine* e) {
   unsigned __int128 top = row[nel] - 1;  // largest rank: C(nb, nel) - 1
   int bits = 1;
   while (bits < 128 && (top >> bits) != 0) ++bits;
-  if (bits >= nb) return FQ_OK;  // no gain over the bit strings
   e->rank_bits = bits;
   e->rank_words = bits > 64 ? 2 : 1;
   return upload(&e->d_binom, tab.data(), static_cast<int64_t>(tab.size()), e->stream);
@@ -2496,9 +2713,12 @@ int pack_spawned(fq_engine* e, int64_t nbt, int64_t t_begin, const int64_t* ptot
   const int64_t nt = ceil_div(nbt, kSpawnTile);
   int rc = scan_exclusive(e, e->tile_cnt, e->tile_off, nt);
   if (rc != FQ_OK) return rc;
-  hipLaunchKernelGGL(k_pack_tiles, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0, e->stream, e->sys, nt,
-                     e->tile_cnt, e->tile_off, e->st_det, e->st_pop, e->st_ent, e->bcap, t_begin, e->flags,
-                     e->scap, e->sdet, e->srec, e->d_stats);
+  KeyPlan kp = key_plan(e);
+  const size_t tab = binom_bytes(e);
+  kp.binom_lds = tab > 0 && tab <= kBinomLdsMax;
+  hipLaunchKernelGGL(k_pack_tiles, dim3(static_cast<unsigned>(std::min<int64_t>(nt, 2048))), dim3(kBlock),
+                     kp.binom_lds ? tab : 0, e->stream, e->sys, nt, e->tile_cnt, e->tile_off, e->st_det, e->st_pop,
+                     e->st_ent, e->bcap, t_begin, e->flags, e->scap, kp, e->kbuf[0], e->pbuf[0], e->d_stats);
   hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, e->stream, e->tile_off, nt, ptot, e->d_stats);
   return FQ_OK;
 }
@@ -2519,29 +2739,21 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->mmask, cap / 64 + 2);
   rc |= dalloc(&e->mcnt, cap / 64 + 2);
   rc |= dalloc(&e->moff, cap / 64 + 2);
-  rc |= dalloc(&e->mdraw, cap + 1);
-  rc |= dalloc(&e->mdraw_off, cap + 1);
+  const bool replay = e->par.rng_mode == FQ_RNG_REPLAY;
+  if (replay) {  // rounding-draw counts of the replay stream
+    rc |= dalloc(&e->mdraw, cap + 1);
+    rc |= dalloc(&e->mdraw_off, cap + 1);
+  }
   rc |= dalloc(&e->part_d, 4 * (cap / kBlock + 2));
   rc |= dalloc(&e->part_i, 2 * (cap / kBlock + 2));
-  e->bcap = scap;
+  // spawning attempts are staged in batches of at most bcap (one slot each)
+  e->bcap = std::min<int64_t>(scap, int64_t{1} << 26);
   rc |= dalloc(&e->st_det, W * e->bcap);
   rc |= dalloc(&e->st_pop, e->bcap);
   rc |= dalloc(&e->st_ent, e->bcap);
   rc |= dalloc(&e->tile_cnt, e->bcap / kSpawnTile + 2);
   rc |= dalloc(&e->tile_off, e->bcap / kSpawnTile + 2);
   rc |= dalloc(&e->tile_win, 2 * (e->bcap / kSpawnTile + 2));
-  rc |= dalloc(&e->sdet, W * scap);
-  rc |= dalloc(&e->srec, rec_stride(S) * scap);
-  for (int b = 0; b < 2; ++b) {
-    rc |= dalloc(&e->kbuf[b], W * scap);
-    rc |= dalloc(&e->vbuf[b], scap);
-  }
-  rc |= dalloc(&e->run_stage, scap);
-  rc |= dalloc(&e->run_tcnt, scap / kRunTile + 2);
-  rc |= dalloc(&e->run_toff, scap / kRunTile + 2);
-  rc |= dalloc(&e->gpop, S * scap);
-  rc |= dalloc(&e->gflag, scap);
-  rc |= dalloc(&e->run_start, scap + 1);
   rc |= dalloc(&e->rdet, W * scap);
   rc |= dalloc(&e->rpop, S * scap);
   rc |= dalloc(&e->rflag, scap);
@@ -2555,8 +2767,10 @@ int alloc_engine(fq_engine* e) {
   rc |= dalloc(&e->nmask, scap / 64 + 2);
   rc |= dalloc(&e->ncnt, scap / 64 + 2);
   rc |= dalloc(&e->noff, scap / 64 + 2);
-  rc |= dalloc(&e->ndraw, scap + 1);
-  rc |= dalloc(&e->ndraw_off, scap + 1);
+  if (replay) {
+    rc |= dalloc(&e->ndraw, scap + 1);
+    rc |= dalloc(&e->ndraw_off, scap + 1);
+  }
   rc |= dalloc(&e->d_stats, 1);
   return rc == FQ_OK ? FQ_OK : FQ_ERR_NOMEM;
 }
@@ -2707,11 +2921,12 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   s.kmat = e->d_kmat;
   s.s1j = e->d_s1j;
   s.s1k = e->d_s1k;
-  if (rc == FQ_OK && params->rng_mode == FQ_RNG_PHILOX) rc = setup_rank_keys(e);
+  if (rc == FQ_OK) rc = setup_rank_keys(e);
   if (rc == FQ_OK && (params->excit_gen == FQ_EXCIT_HEAT_BATH || params->excit_gen == FQ_EXCIT_LEARNED))
     rc = build_heat_bath(e);
   if (rc == FQ_OK && params->excit_gen == FQ_EXCIT_LEARNED) rc = setup_learned(e);
   if (rc == FQ_OK) rc = alloc_engine(e);
+  if (rc == FQ_OK) rc = setup_sort(e);
   for (int k = 0; k < 4 && rc == FQ_OK; ++k)
     if (hipEventCreate(&e->ev[k]) != hipSuccess) rc = FQ_ERR_DEVICE;
   for (int k = 0; k < 9 && rc == FQ_OK; ++k)
@@ -2880,18 +3095,13 @@ static int cycle_body(fq_engine* e, const double* shift, const double* eproj_old
     rc = spawn_range(0, bound);
     if (rc != FQ_OK) return rc;
     FQ_HIP(hipEventRecord(e->pev[3], st));
-    // the sort keys (at most one spawned entry per attempt) run while the host
-    // waits for the spawned-list length
-    FQ_HIP(hipEventRecord(e->pev[4], st));
-    launch_sort_keys(e, std::min(bound, e->scap), e->d_stats);
-    e->keys_ready = true;
+    e->keys_ready = true;  // the packing kernels wrote the sort keys
     rc = sync_stats(e);
     if (rc != FQ_OK) return rc;
     if (const int64_t total = e->h_stats.total_attempts; total > bound) {  // the bound cannot be exceeded
       rc = spawn_range(bound, total);
       if (rc == FQ_OK) rc = sync_stats(e);
       if (rc != FQ_OK) return rc;
-      e->keys_ready = false;
     }
     ns = std::min<int64_t>(e->h_stats.nspawn, e->scap);
   }
@@ -2952,7 +3162,102 @@ static int cycle_body(fq_engine* e, const double* shift, const double* eproj_old
 }
 
 int fq_cycle(fq_engine* e, const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
+  if (e && e->pending) return FQ_ERR_ARG;
   return cycle_body(e, shift, eproj_old, tau, out);
+}
+
+int fq_cycle_spawn(fq_engine* e, const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
+  if (!e || !shift || !eproj_old || !out || e->batched || e->pending) return FQ_ERR_ARG;
+  if (e->par.rng_mode != FQ_RNG_REPLAY) return FQ_ERR_ARG;
+  FQ_HIP(hipSetDevice(e->device));
+  const int c = e->cur;
+  hipStream_t st = e->stream;
+  const uint32_t cyc = static_cast<uint32_t>(e->cycle_index++);
+  const double sh1 = e->S > 1 ? shift[1] : 0.0, e1 = e->S > 1 ? eproj_old[1] : 0.0;
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, reinterpret_cast<int64_t*>(e->d_stats),
+                     static_cast<int64_t>(sizeof(Stats) / sizeof(int64_t)));
+  hipLaunchKernelGGL(k_replay_walk, dim3(1), dim3(1), 0, st, e->sys, e->n, e->cap, e->det[c], e->pop[c], e->kii[c],
+                     e->pop_new, tau, shift[0], sh1, eproj_old[0], e1, e->rng_buf, e->rng_pos, e->rng_end, e->scap,
+                     e->sdet, e->srec, e->d_stats);
+  int rc = sync_stats(e);
+  if (rc != FQ_OK) return rc;
+  if (e->h_stats.rng_failed) return FQ_ERR_RNG_EXHAUSTED;
+  e->pending = true;
+  e->pend_cyc = cyc;
+  e->pend_rng_pos = e->h_stats.rng_pos;
+  e->pend_ns = std::min<int64_t>(e->h_stats.nspawn, e->scap);
+  e->pend_stats = e->h_stats;
+  std::memset(out, 0, sizeof(*out));
+  for (int sp = 0; sp < e->S; ++sp) {
+    out->proj_energy[sp] = e->h_stats.proj[sp];
+    out->D0_population[sp] = e->h_stats.d0[sp];
+  }
+  out->nspawn_events = e->pend_ns;
+  out->ndeath = e->h_stats.ndeath;
+  out->nattempts = e->h_stats.total_attempts;
+  out->rng_consumed = e->pend_rng_pos - e->rng_pos;
+  out->spawn_overflow = e->h_stats.spawn_overflow ? 1 : 0;
+  return FQ_OK;
+}
+
+int fq_spawned_list(fq_engine* e, uint64_t* states, int64_t* pops, int64_t* flags, int64_t capacity, int64_t* n) {
+  if (!e || !n || !e->pending) return FQ_ERR_ARG;
+  const int64_t ns = e->pend_ns;
+  *n = ns;
+  if (capacity < ns) return FQ_ERR_CAPACITY;
+  if (ns == 0) return FQ_OK;
+  FQ_HIP(hipSetDevice(e->device));
+  const int W = e->W, S = e->S, RS = rec_stride(S);
+  std::vector<uint64_t> dw(static_cast<size_t>(W) * ns);
+  std::vector<int64_t> rec(static_cast<size_t>(RS) * ns);
+  for (int w = 0; w < W; ++w)
+    FQ_HIP(hipMemcpyAsync(dw.data() + w * ns, e->sdet + w * e->scap, 8 * ns, hipMemcpyDeviceToHost, e->stream));
+  FQ_HIP(hipMemcpyAsync(rec.data(), e->srec, 8 * RS * ns, hipMemcpyDeviceToHost, e->stream));
+  FQ_HIP(hipStreamSynchronize(e->stream));
+  for (int64_t i = 0; i < ns; ++i) {
+    if (states)
+      for (int w = 0; w < W; ++w) states[i * W + w] = dw[w * ns + i];
+    if (pops)
+      for (int sp = 0; sp < S; ++sp) pops[i * S + sp] = rec[i * RS + sp];
+    if (flags) flags[i] = rec[i * RS + S];
+  }
+  return FQ_OK;
+}
+
+int fq_cycle_finish(fq_engine* e, const uint64_t* sstates, const int64_t* spops, const int64_t* sflags, int64_t ns,
+                    fq_cycle_out* out) {
+  if (!e || !out || !e->pending || ns < 0 || (ns > 0 && (!sstates || !spops))) return FQ_ERR_ARG;
+  if (ns > e->scap) return FQ_ERR_CAPACITY;
+  FQ_HIP(hipSetDevice(e->device));
+  const int W = e->W, S = e->S, RS = rec_stride(S);
+  std::vector<uint64_t> dw(static_cast<size_t>(W) * std::max<int64_t>(ns, 1));
+  std::vector<int64_t> rec(static_cast<size_t>(RS) * std::max<int64_t>(ns, 1), 0);
+  for (int64_t i = 0; i < ns; ++i) {
+    for (int w = 0; w < W; ++w) dw[w * ns + i] = sstates[i * W + w];
+    for (int sp = 0; sp < S; ++sp) rec[i * RS + sp] = spops[i * S + sp];
+    rec[i * RS + S] = (sflags && e->par.initiator) ? sflags[i] : 0;
+  }
+  hipStream_t st = e->stream;
+  for (int w = 0; w < W && ns > 0; ++w)
+    FQ_HIP(hipMemcpyAsync(e->sdet + w * e->scap, dw.data() + w * ns, 8 * ns, hipMemcpyHostToDevice, st));
+  if (ns > 0) FQ_HIP(hipMemcpyAsync(e->srec, rec.data(), 8 * RS * ns, hipMemcpyHostToDevice, st));
+  e->pending = false;
+  int64_t rng_pos = e->pend_rng_pos;
+  int rc = annihilate(e, ns, true, e->pend_cyc, rng_pos);
+  if (rc != FQ_OK) return rc;
+  const int64_t used = rng_pos - e->rng_pos;
+  e->rng_pos = rng_pos;
+  const Stats& k = e->pend_stats;
+  for (int sp = 0; sp < S; ++sp) {
+    e->h_stats.proj[sp] = k.proj[sp];
+    e->h_stats.d0[sp] = k.d0[sp];
+  }
+  e->h_stats.nspawn = e->pend_ns;
+  e->h_stats.ndeath = k.ndeath;
+  e->h_stats.total_attempts = k.total_attempts;
+  e->h_stats.spawn_overflow = k.spawn_overflow;
+  fill_out(e, out, used);
+  return FQ_OK;
 }
 
 int fq_cycle_noalloc(fq_engine* e, const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
@@ -2965,10 +3270,8 @@ int fq_cycle_noalloc(fq_engine* e, const double* shift, const double* eproj_old,
   const int rc = cycle_body(e, shift, eproj_old, tau, out);
   e->noalloc = false;
   if (e->scratch_short) {
-    fprintf(stderr, "fqmc: fq_cycle_noalloc: this cycle needs more scratch than the fixed %lld bytes hold "
-            "(spawned list above %lld entries); the walker list is unchanged\n",
-            static_cast<long long>(sizeof(int64_t) * kScratchElems),
-            static_cast<long long>((kScratchSlotElems[4] - 1) / 512 * kRadixTile));
+    fprintf(stderr, "fqmc: fq_cycle_noalloc: this cycle needs more scratch than the fixed %lld bytes hold; "
+            "the walker list is unchanged\n", static_cast<long long>(sizeof(int64_t) * kScratchElems));
     (void)hipStreamSynchronize(e->stream);
     return FQ_ERR_SCRATCH;
   }
@@ -2977,7 +3280,7 @@ int fq_cycle_noalloc(fq_engine* e, const double* shift, const double* eproj_old,
 
 int fq_annihilate(fq_engine* e, const uint64_t* sstates, const int64_t* spops, const int64_t* sflags, int64_t ns,
                   fq_cycle_out* out) {
-  if (!e || !out || ns < 0 || (ns > 0 && (!sstates || !spops))) return FQ_ERR_ARG;
+  if (!e || !out || ns < 0 || (ns > 0 && (!sstates || !spops)) || e->pending) return FQ_ERR_ARG;
   if (ns > e->scap) return FQ_ERR_CAPACITY;
   FQ_HIP(hipSetDevice(e->device));
   const bool replay = e->par.rng_mode == FQ_RNG_REPLAY;
@@ -3074,6 +3377,11 @@ int fq_create_batch(const fq_system* systems, int32_t ninst, const fq_params* pa
   int ib = 0;
   while ((1 << ib) < ninst) ++ib;
   e->key_bits = kInstShift + std::max(ib, 1);
+  rc = setup_sort(e);
+  if (rc != FQ_OK) {
+    fq_destroy(e);
+    return rc;
+  }
   fq_params small = *params;
   small.main_capacity = small.spawn_capacity = 1;
   std::vector<Sys> hs(static_cast<size_t>(ninst));
@@ -3155,7 +3463,7 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
   FQ_HIP(hipEventRecord(e->pev[0], st));
   int64_t* ob = e->d_ibound;
   int64_t* nbd = e->d_ibound + (ninst + 1);
-  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[c], n, ninst, ob);
+  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[c], n, ninst, kInstShift, ob);
   if (n > 0)
     hipLaunchKernelGGL(k_det_pass_batch, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, e->d_sys_arr, ob, e->d_ipar,
                        ninst, n, cap, e->det[c], e->pop[c], e->kii[c], e->pop_new, e->nat, e->flags,
@@ -3188,17 +3496,20 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
     if (rc != FQ_OK) return rc;
   }
   FQ_HIP(hipEventRecord(e->pev[3], st));
+  e->keys_ready = true;  // the packing kernels wrote the sort keys
   rc = sync_stats(e);
   if (rc != FQ_OK) return rc;
   const int64_t ns = std::min<int64_t>(e->h_stats.nspawn, e->scap);
   // spawned entries are in attempt order, hence grouped by instance
-  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->sdet, ns, ninst, e->d_isb);
+  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->kbuf[0], ns, ninst,
+                     kInstShift + e->flag_bits, e->d_isb);
   const Stats keep = e->h_stats;
   int64_t rng_pos = e->rng_pos;
   rc = annihilate(e, ns, false, cyc, rng_pos);
   if (rc != FQ_OK) return rc;
   const int o = e->cur;
-  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[o], e->n, ninst, nbd);
+  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[o], e->n, ninst, kInstShift,
+                     nbd);
   hipLaunchKernelGGL(k_inst_stats, dim3(ninst), dim3(kBlock), 0, st, ninst, S, cap, ob, nbd, e->d_isb, e->d_cterm,
                      e->d_cdeath, e->att_off, e->pop[o], e->d_iout_d, e->d_iout_i);
   std::vector<double> od(static_cast<size_t>(2 * S * ninst));
@@ -3437,7 +3748,6 @@ void fq_destroy(fq_engine* e) {
     dfree(e->pop[b]);
     dfree(e->kii[b]);
     dfree(e->kbuf[b]);
-    dfree(e->vbuf[b]);
   }
   dfree(e->pop_new);
   dfree(e->nat);
@@ -3456,16 +3766,15 @@ void fq_destroy(fq_engine* e) {
   dfree(e->tile_cnt);
   dfree(e->tile_off);
   dfree(e->tile_win);
-  dfree(e->sdet);
-  dfree(e->srec);
-  dfree(e->hist);
-  dfree(e->hist_scan);
-  dfree(e->run_stage);
+  e->sdet = nullptr;  // aliases of kbuf[1], pbuf[1]
+  e->srec = nullptr;
+  dfree(e->perm);
+  dfree(e->pbuf[0]);
+  dfree(e->pbuf[1]);
+  dfree(e->pass_cnt);
+  dfree(e->pass_off);
   dfree(e->run_tcnt);
   dfree(e->run_toff);
-  dfree(e->gpop);
-  dfree(e->gflag);
-  dfree(e->run_start);
   dfree(e->rdet);
   dfree(e->rpop);
   dfree(e->rflag);

@@ -10,6 +10,7 @@
 // :1433-1454).  The per-cycle work runs in the engine (fq_cycle).  In replay
 // mode this driver owns the dSFMT stream (seed rng_seed + iproc with iproc = 0,
 // src/fciqmc.f90:206) and keeps the engine's device copy topped up.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -55,7 +56,13 @@ class StreamFeeder {
 
 extern "C" int fqh_fciqmc(const fqh_system* hs, const fqh_qmc_in* in, fqh_report_row* rows, int max_rows,
                           int* nrows) {
+  return fqh_fciqmc_restart(hs, in, nullptr, nullptr, rows, max_rows, nrows);
+}
+
+extern "C" int fqh_fciqmc_restart(const fqh_system* hs, const fqh_qmc_in* in, const fqh_restart* start,
+                                  fqh_restart* end, fqh_report_row* rows, int max_rows, int* nrows) {
   if (!hs || !in || !rows || !nrows) return FQ_ERR_ARG;
+  if (start && start->nstates > 0 && (!start->states || !start->pops || !start->dat)) return FQ_ERR_ARG;
   const fq_system* sys = fqh_view(hs);
   fq_params p{};
   p.nspaces = in->replica_tricks ? 2 : 1;
@@ -82,26 +89,45 @@ extern "C" int fqh_fciqmc(const fqh_system* hs, const fqh_qmc_in* in, fqh_report
   if (rc != FQ_OK) return rc;
   const bool replay = in->rng_mode == FQ_RNG_REPLAY;
   StreamFeeder feeder(e, static_cast<uint32_t>(static_cast<int32_t>(in->rng_seed)));
-  // initial_distribution: the reference determinant with nint(D0_pop)*real_factor.
-  std::vector<int64_t> pop0(S, static_cast<int64_t>(std::llround(in->init_pop)) * rf);
-  double dat0 = 0.0;
-  rc = fq_load(e, sys->f0, pop0.data(), &dat0, 1);
+  double shift[FQ_MAX_SPACES] = {0, 0}, proj[FQ_MAX_SPACES] = {0, 0}, d0[FQ_MAX_SPACES] = {0, 0};
+  double ntot_old[FQ_MAX_SPACES] = {0, 0}, nparticles[FQ_MAX_SPACES] = {0, 0};
+  bool vary[FQ_MAX_SPACES] = {false, false};
+  int64_t nstates = 1, cycles0 = 0;
+  if (!start) {
+    // initial_distribution: the reference determinant with nint(D0_pop)*real_factor.
+    std::vector<int64_t> pop0(S, static_cast<int64_t>(std::llround(in->init_pop)) * rf);
+    double dat0 = 0.0;
+    rc = fq_load(e, sys->f0, pop0.data(), &dat0, 1);
+    for (int s = 0; s < S; ++s) {
+      shift[s] = in->initial_shift;
+      nparticles[s] = std::fabs(static_cast<double>(pop0[s]) / static_cast<double>(rf));
+      ntot_old[s] = nparticles[s];
+      // initial_ci_projected_energy over the single reference determinant
+      d0[s] = static_cast<double>(pop0[s]) / static_cast<double>(rf);
+      proj[s] = 0.0;
+    }
+  } else {
+    // read_restart_hdf5 + init_estimators: the stored list, shift and flags; the
+    // population recounted as sum |pop| / real_factor in list order.
+    rc = fq_load(e, start->states, start->pops, start->dat, start->nstates);
+    nstates = start->nstates;
+    cycles0 = start->mc_cycles_done;
+    for (int s = 0; s < S; ++s) {
+      shift[s] = start->shift[s];
+      vary[s] = start->vary_shift[s] != 0;
+      proj[s] = start->proj_energy[s];
+      d0[s] = start->D0_population[s];
+      double t = 0.0;
+      for (int64_t i = 0; i < start->nstates; ++i)
+        t = t + std::fabs(static_cast<double>(start->pops[i * S + s]) / static_cast<double>(rf));
+      nparticles[s] = t;
+      ntot_old[s] = t;
+    }
+  }
   if (rc != FQ_OK) {
     fq_destroy(e);
     return rc;
   }
-  double shift[FQ_MAX_SPACES] = {0, 0}, proj[FQ_MAX_SPACES] = {0, 0}, d0[FQ_MAX_SPACES] = {0, 0};
-  double ntot_old[FQ_MAX_SPACES] = {0, 0}, nparticles[FQ_MAX_SPACES] = {0, 0};
-  bool vary[FQ_MAX_SPACES] = {false, false};
-  for (int s = 0; s < S; ++s) {
-    shift[s] = in->initial_shift;
-    nparticles[s] = std::fabs(static_cast<double>(pop0[s]) / static_cast<double>(rf));
-    ntot_old[s] = nparticles[s];
-    // initial_ci_projected_energy over the single reference determinant
-    d0[s] = static_cast<double>(pop0[s]) / static_cast<double>(rf);
-    proj[s] = 0.0;
-  }
-  int64_t nstates = 1;
   int nr = 0;
   auto emit = [&](int64_t iter, int64_t nspawn, double rspawn, double tcyc) {
     if (nr >= max_rows) return;
@@ -119,8 +145,9 @@ extern "C" int fqh_fciqmc(const fqh_system* hs, const fqh_qmc_in* in, fqh_report
     r.rspawn = rspawn;
     r.time_per_cycle = tcyc;
   };
-  emit(0, 0, 0.0, 0.0);
+  emit(cycles0, 0, 0.0, 0.0);
   bool overflowed = false;
+  int reports_done = 0;
   for (int ireport = 1; ireport <= in->nreports; ++ireport) {
     double eproj_old[FQ_MAX_SPACES] = {0, 0};
     for (int s = 0; s < S; ++s) eproj_old[s] = (std::fabs(d0[s]) < 2.2250738585072014e-308) ? 0.0 : proj[s] / d0[s];
@@ -180,10 +207,22 @@ extern "C" int fqh_fciqmc(const fqh_system* hs, const fqh_qmc_in* in, fqh_report
         shift[s] = in->vary_shift_from_proje ? proj[s] / d0[s] : in->vary_shift_from;
       }
     }
-    emit(static_cast<int64_t>(ireport) * in->mc_cycles, nspawn_events, rspawn, tcyc);
+    emit(cycles0 + static_cast<int64_t>(ireport) * in->mc_cycles, nspawn_events, rspawn, tcyc);
+    reports_done = ireport;
     if (error) {
       overflowed = true;
       break;
+    }
+  }
+  if (rc == FQ_OK && end) {
+    // dump_restart_hdf5: the list, shift, flags and the last report's estimators
+    rc = fq_download(e, end->states, end->pops, end->dat, end->capacity, &end->nstates);
+    end->mc_cycles_done = cycles0 + static_cast<int64_t>(reports_done) * in->mc_cycles;
+    for (int s = 0; s < FQ_MAX_SPACES; ++s) {
+      end->shift[s] = s < S ? shift[s] : 0.0;
+      end->vary_shift[s] = s < S && vary[s] ? 1 : 0;
+      end->proj_energy[s] = s < S ? proj[s] : 0.0;
+      end->D0_population[s] = s < S ? d0[s] : 0.0;
     }
   }
   fq_destroy(e);
@@ -192,6 +231,249 @@ extern "C" int fqh_fciqmc(const fqh_system* hs, const fqh_qmc_in* in, fqh_report
   // report, as the reference's sticky spawn%error / psip_list%error flags do
   // (src/fciqmc.f90:415-422); the caller sees it as FQ_ERR_OVERFLOW with the
   // rows emitted so far.
+  if (rc == FQ_OK && overflowed) rc = FQ_ERR_OVERFLOW;
+  return rc;
+}
+
+// MurmurHash2 (Appleby; the reference's copy is lib/external/MurmurHash2.c:16):
+// 32-bit hash of len bytes read as little-endian 32-bit words.
+extern "C" uint32_t fqh_murmur2(const void* key, int len, uint32_t seed) {
+  const unsigned char* data = static_cast<const unsigned char*>(key);
+  const uint32_t m = 0x5bd1e995u;
+  uint32_t h = seed ^ static_cast<uint32_t>(len);
+  for (; len >= 4; data += 4, len -= 4) {
+    uint32_t k;
+    std::memcpy(&k, data, 4);  // little-endian host
+    k *= m;
+    k ^= k >> 24;
+    k *= m;
+    h *= m;
+    h ^= k;
+  }
+  switch (len) {
+    case 3: h ^= static_cast<uint32_t>(data[2]) << 16; [[fallthrough]];
+    case 2: h ^= static_cast<uint32_t>(data[1]) << 8; [[fallthrough]];
+    case 1: h ^= data[0]; h *= m;
+  }
+  h ^= h >> 13;
+  h *= m;
+  h ^= h >> 15;
+  return h;
+}
+
+// The rank owning a determinant (assign_particle_processor with hash_shift 0
+// and one processor-map slot per rank, src/spawning.F90:770-838,
+// src/load_balancing.F90:141-170): MurmurHash2 with seed 7 (src/qmc.F90:1496)
+// of the first ceiling(nbasis/32)*4 bytes of the bit string
+// (lib/local/hash.f90), the signed hash modulo nprocs as Fortran's modulo.
+extern "C" int fqh_owner_rank(const uint64_t* f, int nbasis, int nprocs) {
+  if (!f || nprocs < 1) return -1;
+  unsigned char bytes[8 * FQ_MAX_WORDS] = {};
+  const int nw = std::min((nbasis + 63) / 64, FQ_MAX_WORDS);
+  for (int w = 0; w < nw; ++w)
+    for (int b = 0; b < 8; ++b) bytes[8 * w + b] = static_cast<unsigned char>(f[w] >> (8 * b));
+  const int32_t hash = static_cast<int32_t>(fqh_murmur2(bytes, (nbasis + 31) / 32 * 4, 7u));
+  int r = hash % nprocs;
+  return r < 0 ? r + nprocs : r;
+}
+
+// The report loop on nprocs emulated MPI ranks, one engine each on the same
+// device, replay mode: rank r consumes dSFMT seeded rng_seed + r
+// (src/fciqmc.f90:206), owns the determinants that hash to it (the reference
+// determinant's initial walkers start on its owner, src/qmc.F90:1574-1578),
+// spawns (fq_cycle_spawn), and receives the spawned entries addressed to it in
+// rank order, each source's in creation order (the per-destination blocks of
+// comm_spawn_t, src/spawn_data.F90:624-739), before annihilating them
+// (fq_cycle_finish).  Report quantities are the ranks' sums
+// (update_energy_estimators' MPI_Allreduce, src/energy_evaluation.F90:126-201),
+// R_spawn their mean (:579).  A host-side exchange: the replay of multi-rank
+// reference runs on one GPU, and the exchange step a sharded engine performs.
+extern "C" int fqh_fciqmc_ranks(const fqh_system* hs, const fqh_qmc_in* in, int nprocs, fqh_report_row* rows,
+                                int max_rows, int* nrows) {
+  if (!hs || !in || !rows || !nrows || nprocs < 1) return FQ_ERR_ARG;
+  if (nprocs == 1) return fqh_fciqmc(hs, in, rows, max_rows, nrows);
+  if (in->rng_mode != FQ_RNG_REPLAY) return FQ_ERR_ARG;
+  const fq_system* sys = fqh_view(hs);
+  fq_params p{};
+  p.nspaces = in->replica_tricks ? 2 : 1;
+  p.initiator = in->initiator;
+  p.initiator_pop = in->initiator_threshold;
+  p.quadrature_initiator = in->quadrature_initiator;
+  p.real_amplitudes = in->real_amplitudes;
+  p.pop_bit_shift = in->real_amplitudes ? (in->real_amplitude_force_32 ? 11 : 31) : 0;
+  const int64_t rf = int64_t{1} << p.pop_bit_shift;
+  p.spawn_cutoff = in->real_amplitudes ? static_cast<int64_t>(std::ceil(in->spawn_cutoff * static_cast<double>(rf))) : 0;
+  p.excit_gen = in->excit_gen;
+  p.rng_mode = FQ_RNG_REPLAY;
+  p.device = in->device;
+  const int S = p.nspaces;
+  const int W = sys->nwords;
+  p.main_capacity = capacity_from(in->state_size, 8 * W + 8 * S + 8 * S, false);
+  p.spawn_capacity = capacity_from(in->spawned_state_size, 8 * (W + S + (in->initiator ? 1 : 0)), true);
+  std::vector<fq_engine*> eng(nprocs, nullptr);
+  std::vector<StreamFeeder> feeders;
+  int rc = FQ_OK;
+  for (int r = 0; r < nprocs && rc == FQ_OK; ++r) rc = fq_create(sys, &p, &eng[r]);
+  auto cleanup = [&]() {
+    for (fq_engine* e : eng)
+      if (e) fq_destroy(e);
+  };
+  if (rc != FQ_OK) {
+    cleanup();
+    return rc;
+  }
+  for (int r = 0; r < nprocs; ++r)
+    feeders.emplace_back(eng[r], static_cast<uint32_t>(static_cast<int32_t>(in->rng_seed) + r));
+  std::vector<int64_t> pop0(S, static_cast<int64_t>(std::llround(in->init_pop)) * rf);
+  double dat0 = 0.0;
+  const int d0_rank = fqh_owner_rank(sys->f0, sys->nbasis, nprocs);
+  rc = fq_load(eng[d0_rank], sys->f0, pop0.data(), &dat0, 1);
+  if (rc != FQ_OK) {
+    cleanup();
+    return rc;
+  }
+  std::vector<double> npart(static_cast<size_t>(nprocs) * S, 0.0);  // per-rank particle totals
+  std::vector<int64_t> nst(nprocs, 0);
+  for (int s = 0; s < S; ++s) npart[d0_rank * S + s] = std::fabs(static_cast<double>(pop0[s]) / static_cast<double>(rf));
+  nst[d0_rank] = 1;
+  double shift[FQ_MAX_SPACES] = {0, 0}, proj[FQ_MAX_SPACES] = {0, 0}, d0[FQ_MAX_SPACES] = {0, 0};
+  double ntot_old[FQ_MAX_SPACES] = {0, 0};
+  bool vary[FQ_MAX_SPACES] = {false, false};
+  auto total = [&](int s) {
+    double t = 0.0;
+    for (int r = 0; r < nprocs; ++r) t = t + npart[r * S + s];
+    return t;
+  };
+  for (int s = 0; s < S; ++s) {
+    shift[s] = in->initial_shift;
+    ntot_old[s] = total(s);
+    d0[s] = static_cast<double>(pop0[s]) / static_cast<double>(rf);
+    proj[s] = 0.0;
+  }
+  int nr = 0;
+  auto emit = [&](int64_t iter, int64_t nspawn, double rspawn, double tcyc) {
+    if (nr >= max_rows) return;
+    fqh_report_row& row = rows[nr++];
+    std::memset(&row, 0, sizeof(row));
+    row.iterations = iter;
+    for (int s = 0; s < S; ++s) {
+      row.shift[s] = shift[s];
+      row.proj_energy[s] = proj[s];
+      row.D0_population[s] = d0[s];
+      row.nparticles[s] = ntot_old[s];
+    }
+    int64_t t = 0;
+    for (int r = 0; r < nprocs; ++r) t += nst[r];
+    row.nstates = t;
+    row.nspawn_events = nspawn;
+    row.rspawn = rspawn;
+    row.time_per_cycle = tcyc;
+  };
+  emit(0, 0, 0.0, 0.0);
+  bool overflowed = false;
+  std::vector<fq_cycle_out> out(nprocs);
+  std::vector<std::vector<uint64_t>> sst(nprocs);
+  std::vector<std::vector<int64_t>> spo(nprocs), sfl(nprocs);
+  std::vector<uint64_t> rst;
+  std::vector<int64_t> rpo, rfl;
+  for (int ireport = 1; ireport <= in->nreports && rc == FQ_OK; ++ireport) {
+    double eproj_old[FQ_MAX_SPACES] = {0, 0};
+    for (int s = 0; s < S; ++s) eproj_old[s] = (std::fabs(d0[s]) < 2.2250738585072014e-308) ? 0.0 : proj[s] / d0[s];
+    std::vector<double> rproj(static_cast<size_t>(nprocs) * S, 0.0), rd0(static_cast<size_t>(nprocs) * S, 0.0);
+    std::vector<double> rsp(nprocs, 0.0);
+    std::vector<int64_t> nsp(nprocs, 0);
+    bool error = false;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int ic = 0; ic < in->mc_cycles && rc == FQ_OK; ++ic) {
+      std::vector<int64_t> nattempts(nprocs);
+      for (int r = 0; r < nprocs && rc == FQ_OK; ++r) {
+        nattempts[r] = static_cast<int64_t>(std::llround(2 * npart[r * S]));
+        double wsum = 0.0;
+        for (int s = 0; s < S; ++s) wsum += npart[r * S + s];
+        rc = feeders[r].ensure(4096 + 8 * S * nst[r] + static_cast<int64_t>(48.0 * (wsum + S * nst[r])));
+        if (rc == FQ_OK) rc = fq_cycle_spawn(eng[r], shift, eproj_old, in->tau, &out[r]);
+        if (rc != FQ_OK) break;
+        const int64_t ns = out[r].nspawn_events;
+        sst[r].resize(static_cast<size_t>(W) * ns);
+        spo[r].resize(static_cast<size_t>(S) * ns);
+        sfl[r].resize(ns);
+        int64_t got = 0;
+        rc = fq_spawned_list(eng[r], sst[r].data(), spo[r].data(), sfl[r].data(), ns, &got);
+      }
+      for (int r = 0; r < nprocs && rc == FQ_OK; ++r) {  // the entries rank r receives, in rank order
+        rst.clear();
+        rpo.clear();
+        rfl.clear();
+        for (int q = 0; q < nprocs; ++q) {
+          const int64_t nq = static_cast<int64_t>(sfl[q].size());
+          for (int64_t i = 0; i < nq; ++i) {
+            if (fqh_owner_rank(&sst[q][i * W], sys->nbasis, nprocs) != r) continue;
+            rst.insert(rst.end(), sst[q].begin() + i * W, sst[q].begin() + (i + 1) * W);
+            rpo.insert(rpo.end(), spo[q].begin() + i * S, spo[q].begin() + (i + 1) * S);
+            rfl.push_back(sfl[q][i]);
+          }
+        }
+        fq_cycle_out fin;
+        rc = fq_cycle_finish(eng[r], rst.data(), rpo.data(), rfl.data(), static_cast<int64_t>(rfl.size()), &fin);
+        if (rc != FQ_OK) break;
+        for (int s = 0; s < S; ++s) {
+          rproj[r * S + s] = rproj[r * S + s] + out[r].proj_energy[s];
+          rd0[r * S + s] = rd0[r * S + s] + out[r].D0_population[s];
+          npart[r * S + s] = fin.nparticles[s];
+        }
+        nst[r] = fin.nstates;
+        nsp[r] = out[r].nspawn_events;
+        double rate = 0.0;
+        if (nattempts[r] > 0) {
+          const double ndeath_real = static_cast<double>(out[r].ndeath) / static_cast<double>(rf);
+          rate = (static_cast<double>(nsp[r]) + ndeath_real) / static_cast<double>(nattempts[r]);
+        }
+        rsp[r] = rsp[r] + rate;
+        error = error || out[r].spawn_overflow || fin.main_overflow;
+      }
+    }
+    if (rc != FQ_OK) break;
+    const double tcyc =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / in->mc_cycles;
+    double ntot[FQ_MAX_SPACES], rspawn = 0.0;
+    int64_t nspawn_events = 0;
+    for (int s = 0; s < S; ++s) {
+      proj[s] = d0[s] = 0.0;
+      for (int r = 0; r < nprocs; ++r) {
+        proj[s] = proj[s] + rproj[r * S + s];
+        d0[s] = d0[s] + rd0[r * S + s];
+      }
+      ntot[s] = total(s);
+      proj[s] = proj[s] / in->mc_cycles;
+      d0[s] = d0[s] / in->mc_cycles;
+    }
+    for (int r = 0; r < nprocs; ++r) {
+      rspawn = rspawn + rsp[r];
+      nspawn_events += nsp[r];
+    }
+    rspawn = rspawn / (in->mc_cycles * nprocs);
+    for (int s = 0; s < S; ++s) {
+      if (vary[s]) {
+        const double denom = 1.0 * in->tau * in->mc_cycles;
+        shift[s] = shift[s] - std::log(ntot[s] / ntot_old[s]) * in->shift_damping / denom -
+                   std::log(ntot[s] / in->target_population) * 0.0 / denom;
+      }
+    }
+    for (int s = 0; s < S; ++s) ntot_old[s] = ntot[s];
+    for (int s = 0; s < S; ++s) {
+      if (!vary[s] && ntot[s] > in->target_population) {
+        vary[s] = true;
+        shift[s] = in->vary_shift_from_proje ? proj[s] / d0[s] : in->vary_shift_from;
+      }
+    }
+    emit(static_cast<int64_t>(ireport) * in->mc_cycles, nspawn_events, rspawn, tcyc);
+    if (error) {
+      overflowed = true;
+      break;
+    }
+  }
+  cleanup();
+  *nrows = nr;
   if (rc == FQ_OK && overflowed) rc = FQ_ERR_OVERFLOW;
   return rc;
 }

@@ -136,6 +136,8 @@ typedef struct fq_engine fq_engine;
  * bound in init_proc_pointers (src/qmc.F90:217-703).
  * FQ_ERR_ARG for an inconsistent request, including the heat-bath or learned
  * generator on a system other than read_in or with more than 32 electrons,
+ * spawned lists whose sort keys would not fit 128 bits (ceil(log2 C(nbasis, nel))
+ * plus ceil(log2 spawn_capacity) above 128: half-filled systems near 128 spin orbitals),
  * the heat-bath generator when its per-attempt weights (8 bytes per
  * electron per thread of a 256-thread workgroup) do not fit the device's
  * workgroup LDS, and the learned generator when its affinity table (4 nbasis^2
@@ -161,15 +163,30 @@ int fq_rng_available(fq_engine *e, int64_t *n);
  * the stream ran dry; the engine state is then undefined. */
 int fq_cycle(fq_engine *e, const double *shift, const double *eproj_old, double tau, fq_cycle_out *out);
 
+/* A replay-mode cycle split at the spawned list, for emulated MPI ranks (a
+ * multi-rank reference run replayed on one GPU; the exchange step of a sharded
+ * engine).  fq_cycle_spawn runs the loop over determinants (estimators,
+ * spawning, death) and keeps the spawned list on the device (out: proj_energy,
+ * D0_population, nspawn_events = entries spawned, ndeath, nattempts);
+ * fq_spawned_list copies it (creation order: states nwords*n, pops nspaces*n,
+ * flags n; capacity entries); fq_cycle_finish annihilates the caller's list
+ * (the entries this rank receives) into the post-death walker list
+ * (direct_annihilation, src/annihilation.f90:9) and completes out.  fq_cycle
+ * and fq_annihilate return FQ_ERR_ARG while a split cycle is pending. */
+int fq_cycle_spawn(fq_engine *e, const double *shift, const double *eproj_old, double tau, fq_cycle_out *out);
+int fq_spawned_list(fq_engine *e, uint64_t *states, int64_t *pops, int64_t *flags, int64_t capacity, int64_t *n);
+int fq_cycle_finish(fq_engine *e, const uint64_t *sstates, const int64_t *spops, const int64_t *sflags, int64_t ns,
+                    fq_cycle_out *out);
+
 /* fq_cycle without device allocation.  No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  fq_cycle
- * grows its prefix-sum and radix-histogram scratch on demand; this variant
- * takes them from a fixed device array of the library (about 58 MiB per
- * device, held until the process ends) and calls no hipMalloc or hipFree.
- * Limits: production RNG, generators other than the learned proposal (FQ_ERR_ARG
- * otherwise), at most 14,399,488 spawned entries in the cycle (FQ_ERR_SCRATCH,
- * the walker list then unchanged).  The array is shared by every engine on a
+ * grows its prefix-sum scratch on demand; this variant takes it from a fixed
+ * device array of the library (8 MiB per device, held until the process ends,
+ * enough for the longest scan an engine issues) and calls no hipMalloc or
+ * hipFree.  Limits: production RNG, generators other than the learned proposal
+ * (FQ_ERR_ARG otherwise); FQ_ERR_SCRATCH (walker list unchanged) if a scan
+ * should ever exceed the array.  The array is shared by every engine on a
  * device: calls that overlap on one device race.  Results equal fq_cycle's bit
  * for bit. */
 int fq_cycle_noalloc(fq_engine *e, const double *shift, const double *eproj_old, double tau, fq_cycle_out *out);

@@ -93,6 +93,52 @@ typedef struct fqh_report_row {
  * status code. */
 int fqh_fciqmc(const fqh_system *s, const fqh_qmc_in *in, fqh_report_row *rows, int max_rows, int *nrows);
 
+/* do_fciqmc on nprocs emulated MPI ranks (replay mode): one engine per rank on
+ * the same device, rank r drawing from dSFMT seeded rng_seed + r and owning the
+ * determinants that hash to it; each cycle's spawned entries are exchanged on
+ * the host as comm_spawn_t does (src/spawn_data.F90:624-739) and the report
+ * columns are the ranks' sums, as after the reference's MPI_Allreduce
+ * (src/energy_evaluation.F90:126-201).  nprocs = 1 is fqh_fciqmc. */
+int fqh_fciqmc_ranks(const fqh_system *s, const fqh_qmc_in *in, int nprocs, fqh_report_row *rows, int max_rows,
+                     int *nrows);
+
+/* The state a restart file carries between two runs (src/restart_hdf5.F90:30-90:
+ * qmc/psips determinants, populations and data; qmc/state ncycles, shift and
+ * vary shift; qmc/reference "reference population @ t-1"; qmc/state
+ * "projected energy").  Populations are in the run's encoding (times
+ * 2^pop_bit_shift for real amplitudes); dat is K_ii = <D|H|D> - H_00 per state. */
+typedef struct fqh_restart {
+  int64_t nstates;      /* walkers in states / pops / dat */
+  int64_t capacity;     /* room in states / pops / dat (written by fqh_fciqmc_restart's end state) */
+  uint64_t *states;     /* nwords x nstates, determinant-major, sorted as the main list */
+  int64_t *pops;        /* nspaces x nstates, determinant-major */
+  double *dat;          /* nstates */
+  int64_t mc_cycles_done;
+  double shift[FQ_MAX_SPACES];
+  int32_t vary_shift[FQ_MAX_SPACES];
+  double proj_energy[FQ_MAX_SPACES];    /* sum_j H_0j N_j of the last report */
+  double D0_population[FQ_MAX_SPACES];  /* N_0 of the last report */
+} fqh_restart;
+
+/* do_fciqmc restarted from `start` (read_restart_hdf5 and the restart branches
+ * of init_qmc / init_estimators / do_fciqmc, src/restart_hdf5.F90:539-927,
+ * src/qmc.F90:105-172,827-908, src/fciqmc.f90:147-262 with a restart file of
+ * version >= 2, so the first row's projected energy and N_0 are the stored
+ * ones): the walker list, shift, vary-shift flags and iteration count continue
+ * from `start`, the replay stream restarts from rng_seed (a file without an
+ * rng group), and the total population is recounted from the populations.
+ * start = NULL is fqh_fciqmc.  end (may be NULL) receives the state after the
+ * last report (dump_restart_hdf5, src/restart_hdf5.F90:307-537): FQ_ERR_CAPACITY
+ * with end->nstates set if end->capacity is too small. */
+int fqh_fciqmc_restart(const fqh_system *s, const fqh_qmc_in *in, const fqh_restart *start, fqh_restart *end,
+                       fqh_report_row *rows, int max_rows, int *nrows);
+
+/* MurmurHash2 of len bytes (lib/external/MurmurHash2.c:16) and the rank owning
+ * a determinant among nprocs (assign_particle_processor with hash seed 7 and
+ * no hash shift, src/spawning.F90:770-838; -1 for bad arguments). */
+uint32_t fqh_murmur2(const void *key, int len, uint32_t seed);
+int fqh_owner_rank(const uint64_t *f, int nbasis, int nprocs);
+
 /* The replay stream: n close-open dSFMT-19937 draws for rng_seed, in the
  * order get_rand_close_open returns them
  * (lib/dSFMT_F03_interface/dSFMT_interface.F90:192-250,387-409). */

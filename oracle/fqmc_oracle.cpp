@@ -44,6 +44,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -212,8 +213,16 @@ class Oracle {
 
   // ---- one MC cycle (src/fciqmc.f90:307-399) ---------------------------------
   void cycle(const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
+    cycle_spawn(shift, eproj_old, tau, out);
+    cycle_finish(out);
+  }
+
+  // The cycle up to the spawned list (estimators, spawning, death): the list
+  // is then in `spawned` in creation order, ready to be exchanged between
+  // emulated ranks before cycle_finish annihilates it.
+  void cycle_spawn(const double* shift, const double* eproj_old, double tau, fq_cycle_out* out) {
     std::memset(out, 0, sizeof(*out));
-    uint64_t rng0 = rng_.consumed();
+    rng0_ = rng_.consumed();
     spawned.clear();
     int64_t ndeath = 0;
     int64_t attempts = 0;
@@ -243,16 +252,21 @@ class Oracle {
       }
     }
     out->nspawn_events = static_cast<int64_t>(spawned.size());
+    out->ndeath = ndeath;
+    out->nattempts = attempts;
+  }
+
+  // direct_annihilation of `spawned` (as received) and the cycle's totals.
+  void cycle_finish(fq_cycle_out* out) {
     annihilate();
     for (int s = 0; s < S; ++s) out->nparticles[s] = nparticles[s];
     out->nstates = static_cast<int64_t>(states.size());
-    out->ndeath = ndeath;
-    out->nattempts = attempts;
-    out->rng_consumed = static_cast<int64_t>(rng_.consumed() - rng0);
+    out->rng_consumed = static_cast<int64_t>(rng_.consumed() - rng0_);
     out->spawn_overflow = spawn_error;
     out->main_overflow = main_error;
     ++cyc_;
   }
+  std::vector<Spawned>& spawned_list() { return spawned; }
 
   // Annihilate an externally supplied spawned list (entries in the given order).
   void annihilate_external(const uint64_t* sst, const int64_t* sp, const int64_t* sf, int64_t ns,
@@ -301,6 +315,7 @@ class Oracle {
   bool prod_ = false;
   uint64_t pseed_ = 0;
   uint32_t cyc_ = 0;  // cycles run so far
+  uint64_t rng0_ = 0;  // draws consumed before the current cycle
   PhiloxStream ph_;
   void set_production(uint64_t seed) {
     prod_ = true;
@@ -1442,10 +1457,68 @@ int64_t capacity_from(int64_t size, int64_t element_bytes, bool spawned) {
   return static_cast<int64_t>(spawned ? mb / (2.0 * element_bytes) : mb / element_bytes);
 }
 
+// MurmurHash2 (Appleby; lib/external/MurmurHash2.c:16, the reference's
+// copy): 32-bit hash of len bytes read as little-endian 32-bit words.
+uint32_t murmur2(const unsigned char* data, int len, uint32_t seed) {
+  const uint32_t m = 0x5bd1e995u;
+  uint32_t h = seed ^ static_cast<uint32_t>(len);
+  for (; len >= 4; data += 4, len -= 4) {
+    uint32_t k = static_cast<uint32_t>(data[0]) | (static_cast<uint32_t>(data[1]) << 8) |
+                 (static_cast<uint32_t>(data[2]) << 16) | (static_cast<uint32_t>(data[3]) << 24);
+    k *= m;
+    k ^= k >> 24;
+    k *= m;
+    h *= m;
+    h ^= k;
+  }
+  switch (len) {
+    case 3: h ^= static_cast<uint32_t>(data[2]) << 16; [[fallthrough]];
+    case 2: h ^= static_cast<uint32_t>(data[1]) << 8; [[fallthrough]];
+    case 1: h ^= data[0]; h *= m;
+  }
+  h ^= h >> 13;
+  h *= m;
+  h ^= h >> 15;
+  return h;
+}
+
+// assign_particle_processor with hash_shift 0 and one slot per rank
+// (src/spawning.F90:770-838, proc_map(i) = mod(i, nprocs),
+// src/load_balancing.F90:141-170): murmurhash_bit_string over
+// ceiling(nbasis/32)*4 bytes of the bit string, seed 7 (src/qmc.F90:1496,
+// lib/local/hash.f90), the signed hash taken modulo nprocs (Fortran modulo).
+int owner_rank(const uint64_t* f, int nbasis, int nprocs) {
+  unsigned char bytes[8 * FQ_MAX_WORDS];
+  for (int w = 0; w < FQ_MAX_WORDS; ++w)
+    for (int b = 0; b < 8; ++b) bytes[8 * w + b] = static_cast<unsigned char>(f[w] >> (8 * b));
+  const int nbytes = (nbasis + 31) / 32 * 4;
+  const int32_t hash = static_cast<int32_t>(murmur2(bytes, nbytes, 7u));
+  int r = hash % nprocs;
+  if (r < 0) r += nprocs;
+  return r;
+}
+
 // Report loop of do_fciqmc (src/fciqmc.f90:276-464) with the estimator and
 // shift bookkeeping of src/energy_evaluation.F90:425-711 and
-// src/qmc_common.F90:929-1017,1240-1304.
-int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows, int max_rows, int* nrows) {
+// src/qmc_common.F90:929-1017,1240-1304, on nprocs emulated MPI ranks: rank r
+// draws from dSFMT seeded rng_seed + r, owns the determinants that hash to it
+// (the reference determinant's initial walkers start there), spawns into a
+// list of per-destination blocks in creation order, and receives the blocks
+// addressed to it in rank order (comm_spawn_t, src/spawn_data.F90:624-739)
+// before annihilating them.  Report quantities are the ranks' sums
+// (MPI_Allreduce, src/energy_evaluation.F90:126-201), R_spawn their mean.
+//
+// start (may be NULL) restarts from a stored state instead of the initial
+// distribution, as read_restart_hdf5 and the restart branches of init_qmc /
+// init_estimators / do_fciqmc do for a version-2 file (src/restart_hdf5.F90:
+// 539-927, src/qmc.F90:827-908, src/fciqmc.f90:147-262): each stored walker
+// goes to the rank owning it, shift / vary flags / N_0 / projected energy and
+// the iteration count continue, the population is recounted.  end (may be
+// NULL, nprocs = 1 only) receives the state after the last report
+// (dump_restart_hdf5, src/restart_hdf5.F90:307-537).
+int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows, int max_rows, int* nrows,
+               int nprocs, const fqh_restart* start = nullptr, fqh_restart* end = nullptr) {
+  if (nprocs < 1 || (end && nprocs != 1)) return -1;
   fq_params p{};
   p.nspaces = in->replica_tricks ? 2 : 1;
   p.initiator = in->initiator;
@@ -1461,19 +1534,63 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
   const int W = sys->nwords;
   p.main_capacity = capacity_from(in->state_size, 8 * W + 8 * S + 8 * S, false);
   p.spawn_capacity = capacity_from(in->spawned_state_size, 8 * (W + S + (in->initiator ? 1 : 0)), true);
-  Oracle o(sys, &p, static_cast<uint32_t>(static_cast<int32_t>(in->rng_seed)));
-  // initial_distribution (src/qmc.F90:1507-1560)
-  std::vector<int64_t> pop0(S, static_cast<int64_t>(std::llround(in->init_pop)) * rf);
-  std::vector<double> dat0(1, 0.0);
-  o.load(sys->f0, pop0.data(), dat0.data(), 1);
+  std::vector<std::unique_ptr<Oracle>> ranks;
+  for (int r = 0; r < nprocs; ++r)
+    ranks.emplace_back(new Oracle(sys, &p, static_cast<uint32_t>(static_cast<int32_t>(in->rng_seed) + r)));
+  if (!start) {
+    // initial_distribution (src/qmc.F90:1507-1580): on the reference determinant's rank
+    std::vector<int64_t> pop0(S, static_cast<int64_t>(std::llround(in->init_pop)) * rf);
+    std::vector<double> dat0(1, 0.0);
+    const int d0_rank = owner_rank(sys->f0, sys->nbasis, nprocs);
+    ranks[d0_rank]->load(sys->f0, pop0.data(), dat0.data(), 1);
+  } else {
+    for (int r = 0; r < nprocs; ++r) {
+      std::vector<uint64_t> st;
+      std::vector<int64_t> pp;
+      std::vector<double> dd;
+      for (int64_t i = 0; i < start->nstates; ++i) {
+        if (owner_rank(start->states + i * W, sys->nbasis, nprocs) != r) continue;
+        st.insert(st.end(), start->states + i * W, start->states + (i + 1) * W);
+        pp.insert(pp.end(), start->pops + i * S, start->pops + (i + 1) * S);
+        dd.push_back(start->dat[i]);
+      }
+      ranks[r]->load(st.data(), pp.data(), dd.data(), static_cast<int64_t>(dd.size()));
+    }
+  }
+  const int64_t cycles0 = start ? start->mc_cycles_done : 0;
   double shift[FQ_MAX_SPACES], proj[FQ_MAX_SPACES], d0[FQ_MAX_SPACES], ntot_old[FQ_MAX_SPACES];
   bool vary[FQ_MAX_SPACES];
+  auto total_particles = [&](int s) {
+    double t = 0.0;
+    for (auto& o : ranks) t = t + o->nparticles[s];
+    return t;
+  };
+  auto total_states = [&]() {
+    int64_t t = 0;
+    for (auto& o : ranks) t += static_cast<int64_t>(o->states.size());
+    return t;
+  };
   for (int s = 0; s < S; ++s) {
-    shift[s] = in->initial_shift;
-    vary[s] = false;
-    ntot_old[s] = o.nparticles[s];
+    shift[s] = start ? start->shift[s] : in->initial_shift;
+    vary[s] = start ? start->vary_shift[s] != 0 : false;
+    ntot_old[s] = total_particles(s);
+    proj[s] = d0[s] = 0.0;
   }
-  o.projected_energy(proj, d0);
+  if (start) {
+    for (int s = 0; s < S; ++s) {
+      proj[s] = start->proj_energy[s];
+      d0[s] = start->D0_population[s];
+    }
+  } else {
+    for (auto& o : ranks) {
+      double pr[FQ_MAX_SPACES], dr[FQ_MAX_SPACES];
+      o->projected_energy(pr, dr);
+      for (int s = 0; s < S; ++s) {
+        proj[s] = proj[s] + pr[s];
+        d0[s] = d0[s] + dr[s];
+      }
+    }
+  }
   int nr = 0;
   auto emit = [&](int64_t iter, int64_t nspawn, double rspawn) {
     if (nr >= max_rows) return;
@@ -1486,42 +1603,70 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
       r.D0_population[s] = d0[s];
       r.nparticles[s] = ntot_old[s];
     }
-    r.nstates = static_cast<int64_t>(o.states.size());
+    r.nstates = total_states();
     r.nspawn_events = nspawn;
     r.rspawn = rspawn;
   };
-  emit(0, 0, 0.0);
+  emit(cycles0, 0, 0.0);
+  int reports_done = 0;
+  std::vector<fq_cycle_out> out(nprocs);
+  std::vector<std::vector<Spawned>> sent(nprocs);
   for (int ireport = 1; ireport <= in->nreports; ++ireport) {
     double eproj_old[FQ_MAX_SPACES];
     for (int s = 0; s < S; ++s) eproj_old[s] = (std::fabs(d0[s]) < 2.2250738585072014e-308) ? 0.0 : proj[s] / d0[s];
-    double rspawn = 0.0;
-    for (int s = 0; s < S; ++s) proj[s] = d0[s] = 0.0;
-    int64_t nspawn_events = 0;
+    // per-rank report accumulators
+    std::vector<double> rproj(nprocs * S, 0.0), rd0(nprocs * S, 0.0), rspawn(nprocs, 0.0);
+    std::vector<int64_t> nspawn_events(nprocs, 0);
     bool error = false;
     for (int ic = 0; ic < in->mc_cycles; ++ic) {
-      const int64_t nattempts = static_cast<int64_t>(std::llround(2 * o.nparticles[0]));
-      fq_cycle_out out;
-      o.cycle(shift, eproj_old, in->tau, &out);
-      for (int s = 0; s < S; ++s) {
-        proj[s] = proj[s] + out.proj_energy[s];
-        d0[s] = d0[s] + out.D0_population[s];
+      std::vector<int64_t> nattempts(nprocs);
+      for (int r = 0; r < nprocs; ++r) {
+        nattempts[r] = static_cast<int64_t>(std::llround(2 * ranks[r]->nparticles[0]));
+        ranks[r]->cycle_spawn(shift, eproj_old, in->tau, &out[r]);
       }
-      nspawn_events = out.nspawn_events;
-      double rate = 0.0;
-      if (nattempts > 0) {
-        double ndeath_real = static_cast<double>(out.ndeath) / static_cast<double>(rf);
-        rate = (static_cast<double>(nspawn_events) + ndeath_real) / static_cast<double>(nattempts);
+      if (nprocs > 1) {  // the exchange: rank r receives, in rank order, the blocks addressed to it
+        for (int r = 0; r < nprocs; ++r) sent[r].swap(ranks[r]->spawned_list());
+        for (int r = 0; r < nprocs; ++r) {
+          std::vector<Spawned>& recv = ranks[r]->spawned_list();
+          recv.clear();
+          for (int q = 0; q < nprocs; ++q)
+            for (const Spawned& e : sent[q])
+              if (owner_rank(e.f, sys->nbasis, nprocs) == r) recv.push_back(e);
+        }
       }
-      rspawn = rspawn + rate;
-      error = error || out.spawn_overflow || out.main_overflow;
+      for (int r = 0; r < nprocs; ++r) {
+        ranks[r]->cycle_finish(&out[r]);
+        for (int s = 0; s < S; ++s) {
+          rproj[r * S + s] = rproj[r * S + s] + out[r].proj_energy[s];
+          rd0[r * S + s] = rd0[r * S + s] + out[r].D0_population[s];
+        }
+        nspawn_events[r] = out[r].nspawn_events;
+        double rate = 0.0;
+        if (nattempts[r] > 0) {
+          double ndeath_real = static_cast<double>(out[r].ndeath) / static_cast<double>(rf);
+          rate = (static_cast<double>(nspawn_events[r]) + ndeath_real) / static_cast<double>(nattempts[r]);
+        }
+        rspawn[r] = rspawn[r] + rate;
+        error = error || out[r].spawn_overflow || out[r].main_overflow;
+      }
     }
-    double ntot[FQ_MAX_SPACES];
+    double ntot[FQ_MAX_SPACES], rs = 0.0;
+    int64_t nsp = 0;
     for (int s = 0; s < S; ++s) {
-      ntot[s] = o.nparticles[s];
+      proj[s] = d0[s] = 0.0;
+      for (int r = 0; r < nprocs; ++r) {
+        proj[s] = proj[s] + rproj[r * S + s];
+        d0[s] = d0[s] + rd0[r * S + s];
+      }
+      ntot[s] = total_particles(s);
       proj[s] = proj[s] / in->mc_cycles;
       d0[s] = d0[s] / in->mc_cycles;
     }
-    rspawn = rspawn / in->mc_cycles;
+    for (int r = 0; r < nprocs; ++r) {
+      rs = rs + rspawn[r];
+      nsp += nspawn_events[r];
+    }
+    rs = rs / (in->mc_cycles * nprocs);
     for (int s = 0; s < S; ++s) {
       if (vary[s]) {
         const double denom = 1.0 * in->tau * in->mc_cycles;
@@ -1536,10 +1681,24 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
         shift[s] = in->vary_shift_from_proje ? proj[s] / d0[s] : in->vary_shift_from;
       }
     }
-    emit(static_cast<int64_t>(ireport) * in->mc_cycles, nspawn_events, rspawn);
+    emit(cycles0 + static_cast<int64_t>(ireport) * in->mc_cycles, nsp, rs);
+    reports_done = ireport;
     if (error) break;
   }
   *nrows = nr;
+  if (end) {
+    Oracle& o = *ranks[0];
+    end->nstates = static_cast<int64_t>(o.states.size());
+    if (end->nstates > end->capacity) return -5;
+    o.download(end->states, end->pops, end->dat);
+    end->mc_cycles_done = cycles0 + static_cast<int64_t>(reports_done) * in->mc_cycles;
+    for (int s = 0; s < FQ_MAX_SPACES; ++s) {
+      end->shift[s] = s < S ? shift[s] : 0.0;
+      end->vary_shift[s] = s < S && vary[s] ? 1 : 0;
+      end->proj_energy[s] = s < S ? proj[s] : 0.0;
+      end->D0_population[s] = s < S ? d0[s] : 0.0;
+    }
+  }
   return 0;
 }
 
@@ -1627,7 +1786,25 @@ int64_t orc_hb_generate(void* h, const uint64_t* f, const double* draws, int64_t
 }
 
 int orc_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows, int max_rows, int* nrows) {
-  return run_fciqmc(sys, in, rows, max_rows, nrows);
+  return run_fciqmc(sys, in, rows, max_rows, nrows, 1);
+}
+// The report loop continued from a stored state (see run_fciqmc).
+int orc_fciqmc_restart(const fq_system* sys, const fqh_qmc_in* in, const fqh_restart* start, fqh_restart* end,
+                       fqh_report_row* rows, int max_rows, int* nrows) {
+  return run_fciqmc(sys, in, rows, max_rows, nrows, 1, start, end);
+}
+// The report loop on nprocs emulated MPI ranks (see run_fciqmc).
+int orc_fciqmc_ranks(const fq_system* sys, const fqh_qmc_in* in, int nprocs, fqh_report_row* rows, int max_rows,
+                     int* nrows) {
+  return run_fciqmc(sys, in, rows, max_rows, nrows, nprocs);
+}
+// MurmurHash2 of len bytes (the oracle's restatement, for the vector test).
+uint32_t orc_murmur2(const unsigned char* data, int len, uint32_t seed) { return murmur2(data, len, seed); }
+// Owner rank of a determinant among nprocs ranks.
+int orc_owner_rank(const uint64_t* f, int nbasis, int nprocs) {
+  uint64_t g[FQ_MAX_WORDS] = {0, 0};
+  for (int w = 0; w < (nbasis + 63) / 64 && w < FQ_MAX_WORDS; ++w) g[w] = f[w];
+  return owner_rank(g, nbasis, nprocs);
 }
 
 }  // extern "C"

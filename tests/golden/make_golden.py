@@ -6,6 +6,9 @@ dSFMT checker built by ``make -C oracle ref``):
     python tests/golden/make_golden.py
 
 Outputs (small JSON data files, no reference source):
+  * murmur_vectors.json -- MurmurHash2 values written by the reference's own C
+    source (lib/external/MurmurHash2.c), the hash that assigns determinants to
+    MPI ranks.
   * dsfmt_vectors.json -- close-open dSFMT-19937 stream values written by the
     reference's own C source (lib/dSFMT-src-2.2.3/dSFMT.c) with the 50000-double
     refill of lib/dSFMT_F03_interface/dSFMT_interface.F90:387-409.
@@ -28,6 +31,7 @@ REF = os.environ.get("HANDE_REFERENCE", "/root/reference")
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 DSFMT_REF = os.path.join(ROOT, "oracle", "_ref", "dsfmt_ref")
+MURMUR_REF = os.path.join(ROOT, "oracle", "_ref", "murmur_ref")
 
 QMC_TESTS = [
     # (name, directory under test_suite, input file)
@@ -43,6 +47,11 @@ QMC_TESTS = [
     ("Ne-RHF-aug-cc-pVDZ", "ifciqmc/np1/Ne-RHF-aug-cc-pVDZ", "hande_dump.in"),
     ("H2-RHF-cc-pVTZ-Lz", "fciqmc/np1/H2-RHF-cc-pVTZ-Lz", "Lz.in"),
     ("H2-sto-3g-replica", "fciqmc/np1/H2-sto-3g-replica", "h2.in"),
+    # two MPI ranks (real_64 populations, initiator): replayed on emulated ranks
+    ("hubbard_k_14s_real_64_np2", "ifciqmc_real_64/np2/hubbard_k_14s_14e_U2_real_64", "hubbard.in"),
+    # a run continued from the reference's restart file (HDF5, integer populations
+    # read into real_64 amplitudes)
+    ("He2-aug-cc-pVDZ_real_64_restart", "fciqmc_real_64/np1/He2-aug-cc-pVDZ_real_64_restart", "restart.in"),
 ]
 
 # long tables are stored truncated (the tests compare a prefix)
@@ -77,6 +86,26 @@ def dsfmt_vectors():
             "sha256": hashlib.sha256("\n".join(txt).encode()).hexdigest(),
         })
     return out
+
+
+def murmur_vectors():
+    """MurmurHash2 of byte strings written by the reference's own C source
+    (lib/external/MurmurHash2.c): every length 0..24 and determinant-like
+    4-, 8- and 16-byte strings, seeds 7 (the rank hash) and others."""
+    import random
+    rnd = random.Random(2024)
+    cases = []
+    for seed in (7, 0, 1, 0xdeadbeef):
+        for n in range(25):
+            cases.append((seed, bytes(rnd.getrandbits(8) for _ in range(n))))
+    for _ in range(200):
+        n = rnd.choice([4, 8, 16])
+        cases.append((7, bytes(rnd.getrandbits(8) for _ in range(n))))
+    inp = "".join(f"{sd} {b.hex()}\n" if b else f"{sd}\n" for sd, b in cases)
+    txt = subprocess.run([MURMUR_REF], input=inp, check=True, capture_output=True, text=True).stdout.split()
+    assert len(txt) == len(cases)
+    return {"source": "lib/external/MurmurHash2.c via oracle/ref_murmur_driver.c",
+            "cases": [{"seed": sd, "bytes": b.hex(), "hash": int(h)} for (sd, b), h in zip(cases, txt)]}
 
 
 def parse_report_table(lines):
@@ -181,6 +210,8 @@ def main():
         sys.exit("build oracle/_ref/dsfmt_ref first (make -C oracle ref)")
     with open(os.path.join(HERE, "dsfmt_vectors.json"), "w") as f:
         json.dump(dsfmt_vectors(), f, indent=1)
+    with open(os.path.join(HERE, "murmur_vectors.json"), "w") as f:
+        json.dump(murmur_vectors(), f, indent=1)
 
     goldens = {}
     for name, d, inp in QMC_TESTS + FCI_TESTS:
@@ -195,6 +226,8 @@ def main():
             "basis": parse_basis(lines),
         }
         entry.update(parse_symmetry(lines))
+        m = re.search(r"Number of MPI processes running on:\s*(\d+)", text)
+        entry["nprocs"] = int(m.group(1)) if m else 1
         js = parse_json_block(text)
         if js is not None:
             entry["json"] = js
@@ -211,11 +244,13 @@ def main():
                         eig.append(float(l2.split()[1]))
                     break
             entry["eigenvalues"] = eig
-        # data files the test reads (integral dumps) are referenced, not copied
+        # data files the deck reads (integral dumps, restart files), stored gzipped
         entry["data_files"] = []
-        for x in os.listdir(path):
-            if x in ("FCIDUMP", "INTDUMP") and not name.startswith("fci_") and not name.endswith("-Lz"):
-                # the integral dump the deck reads: test input data, stored gzipped
+        for x in sorted(os.listdir(path)):
+            is_restart = re.fullmatch(r"HANDE\.RS\.\d+\.p\d+\.H5", x) is not None and "restart" in inp
+            if is_restart or (x in ("FCIDUMP", "INTDUMP") and not name.startswith("fci_")
+                              and not name.endswith("-Lz")):
+                # the integral dump (or restart file) the deck reads: test input data, stored gzipped
                 dst = os.path.join(HERE, f"{name}.{x}.gz")
                 with open(os.path.join(path, x), "rb") as src, open(dst, "wb") as raw, \
                         gzip.GzipFile(filename="", mode="wb", compresslevel=9, fileobj=raw, mtime=0) as out:

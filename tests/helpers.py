@@ -50,6 +50,12 @@ def oracle():
     lib.orc_set_production.argtypes = [vp, C.c_uint64]
     lib.orc_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     lib.orc_fciqmc.argtypes = [P(N.FqSystem), P(N.FqhQmcIn), P(N.FqhReportRow), C.c_int, P(C.c_int)]
+    lib.orc_fciqmc_ranks.argtypes = [P(N.FqSystem), P(N.FqhQmcIn), C.c_int, P(N.FqhReportRow), C.c_int, P(C.c_int)]
+    lib.orc_fciqmc_restart.argtypes = [P(N.FqSystem), P(N.FqhQmcIn), P(N.FqhRestart), P(N.FqhRestart),
+                                       P(N.FqhReportRow), C.c_int, P(C.c_int)]
+    lib.orc_murmur2.restype = C.c_uint32
+    lib.orc_murmur2.argtypes = [C.c_char_p, C.c_int, C.c_uint32]
+    lib.orc_owner_rank.argtypes = [P(C.c_uint64), C.c_int, C.c_int]
     lib.orc_hb_biased.argtypes = [vp]
     lib.orc_hb_table.restype = C.c_int64
     lib.orc_hb_table.argtypes = [vp, C.c_char_p, P(C.c_double), C.c_int64]
@@ -59,10 +65,18 @@ def oracle():
     return lib
 
 
-def oracle_report(orc, system, st):
+def oracle_report(orc, system, st, nprocs=1, start=None, end=None):
+    """The oracle's report table; start / end are restart.RestartBuffers."""
     rows = (N.FqhReportRow * (st.nreports + 1))()
     nr = C.c_int()
-    orc.orc_fciqmc(system.view, C.byref(st), rows, st.nreports + 1, C.byref(nr))
+    if start is not None or end is not None:
+        rc = orc.orc_fciqmc_restart(system.view, C.byref(st), C.byref(start.st) if start else None,
+                                    C.byref(end.st) if end else None, rows, st.nreports + 1, C.byref(nr))
+        assert rc == 0, rc
+    elif nprocs == 1:
+        orc.orc_fciqmc(system.view, C.byref(st), rows, st.nreports + 1, C.byref(nr))
+    else:
+        orc.orc_fciqmc_ranks(system.view, C.byref(st), nprocs, rows, st.nreports + 1, C.byref(nr))
     return api.rows_to_table(rows, nr.value, 2 if st.replica_tricks else 1)
 
 

@@ -93,3 +93,44 @@ def test_hubbard_4x4_configuration_lockstep():
     p.main_capacity = p.spawn_capacity = 1 << 18
     outs = production_lockstep(system, p, 60, 1000, shift=-18.0, tau=0.005, label="4x4 production")
     assert outs[-1].nstates > 5000
+
+
+def test_replay_two_rank_real_64_golden():
+    """The reference's two-rank real_64 initiator run
+    (test_suite/ifciqmc_real_64/np2/hubbard_k_14s_14e_U2_real_64) replayed on
+    two emulated ranks, one engine each on the GPU, spawned walkers exchanged
+    by MurmurHash2 ownership: the first 300 of its 600 report rows (the
+    oracle's emulation matches all 600) against the golden table at 1e-10."""
+    e = goldens()["hubbard_k_14s_real_64_np2"]
+    system, opts = api.load_deck(e["input"])
+    q = dict(opts["qmc"])
+    q["nreports"] = 300
+    print("[replay] two emulated ranks, 300 reports", flush=True)
+    got = api.fciqmc(system, q, opts.get("fciqmc"), rng_mode="replay", device=0, nprocs=2)
+    compare_tables(got, e["report"][:301])
+
+
+def test_replay_restart_golden_and_dump(tmp_path):
+    """The reference's restart test (test_suite/fciqmc_real_64/np1/
+    He2-aug-cc-pVDZ_real_64_restart): the engine continues from the
+    reference's HDF5 restart file (integer populations read into real_64
+    amplitudes) and reproduces all 101 rows of its benchmark table; the state
+    it dumps after the last report is bit-identical to the oracle's."""
+    from hande_amd import restart as R
+    from helpers import deck, oracle_report
+    name = "He2-aug-cc-pVDZ_real_64_restart"
+    system, opts = deck(name, str(tmp_path))
+    path = api.restart_file(opts, str(tmp_path))
+    print(f"[replay restart] {name} from {path}", flush=True)
+    got, end = api.fciqmc(system, opts["qmc"], opts.get("fciqmc"), rng_mode="replay", restart=path,
+                          return_state=True)
+    compare_tables(got, goldens()[name]["report"])
+    st = api.qmc_in(opts["qmc"], opts.get("fciqmc"))
+    rs = R.read_restart(path, system.view.nbasis, 1, api.pop_bit_shift(st))
+    oend = R.RestartBuffers(system.view.nwords, 1, 1 << 16)
+    oracle_report(oracle(), system, st, start=R.RestartBuffers.from_state(rs), end=oend)
+    ref = oend.to_state(1, api.pop_bit_shift(st))
+    assert end.nstates == ref.nstates and end.mc_cycles_done == ref.mc_cycles_done == 10000
+    assert np.array_equal(end.states, ref.states) and np.array_equal(end.pops, ref.pops)
+    assert np.array_equal(end.dat, ref.dat)
+    assert np.array_equal(end.shift, ref.shift) and np.array_equal(end.proj_energy, ref.proj_energy)

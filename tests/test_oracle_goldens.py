@@ -44,3 +44,32 @@ def test_oracle_reproduces_molecular_golden(name, tmp_path):
     st = api.qmc_in(opts["qmc"], opts.get("fciqmc"))
     st.nreports = MOLECULAR[name]
     compare_tables(oracle_report(oracle(), system, st), e["report"][:MOLECULAR[name] + 1])
+
+
+def test_oracle_reproduces_two_rank_real_64_golden():
+    """The reference's two-rank real_64 initiator run
+    (test_suite/ifciqmc_real_64/np2/hubbard_k_14s_14e_U2_real_64) on two
+    emulated ranks: dSFMT seeds rng_seed + rank, MurmurHash2 ownership (seed 7),
+    per-destination spawn blocks received in rank order, summed estimators.
+    The CPU suite checks the first 150 of its 600 report rows (all 600 match)."""
+    e = goldens()["hubbard_k_14s_real_64_np2"]
+    assert e["nprocs"] == 2
+    system, opts = api.load_deck(e["input"], lib=N.lib())
+    st = api.qmc_in(opts["qmc"], opts.get("fciqmc"))
+    st.nreports = 150
+    compare_tables(oracle_report(oracle(), system, st, nprocs=2), e["report"][:151])
+
+
+def test_murmurhash2_matches_reference_vectors():
+    """The rank hash (MurmurHash2, lib/external/MurmurHash2.c) of the oracle and
+    of the host driver against values written by the reference's own C source."""
+    import ctypes as C
+    import json
+    import os
+    vec = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "murmur_vectors.json")))
+    orc = oracle()
+    lib = N.lib()
+    for case in vec["cases"]:
+        b = bytes.fromhex(case["bytes"])
+        assert orc.orc_murmur2(b, len(b), case["seed"]) == case["hash"], case
+        assert lib.fqh_murmur2(b, len(b), case["seed"]) == case["hash"], case
