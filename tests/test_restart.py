@@ -1,0 +1,135 @@
+"""HANDE restart files: the HDF5 subset reader/writer and the restarted report loop.
+
+The reader is pinned by the reference's own restart file
+(test_suite/fciqmc_real_64/np1/He2-aug-cc-pVDZ_real_64_restart/HANDE.RS.0.p0.H5):
+the run that restarts from it prints the stored shift, projected energy,
+reference population, total population and state count as its first report
+row, so those printed values check what the reader decodes.  The restarted
+oracle run then reproduces every row of that test's benchmark table."""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+from hande_amd import api, hdf5, restart as R, _native as N
+from helpers import GOLDEN, compare_tables, deck, goldens, oracle, oracle_report
+
+NAME = "He2-aug-cc-pVDZ_real_64_restart"
+
+
+def _reference_file_bytes():
+    with gzip.open(os.path.join(GOLDEN, f"{NAME}.HANDE.RS.0.p0.H5.gz"), "rb") as f:
+        return f.read()
+
+
+def test_reader_decodes_reference_restart_file():
+    t = hdf5.read(_reference_file_bytes())
+    assert sorted(t) == ["basis", "metadata", "qmc", "rng"]
+    assert t["rng"] == {}
+    assert int(t["metadata"]["restart version"]) == 2 and int(t["metadata"]["i0_length"]) == 32
+    assert t["metadata"]["uuid"] == b"6e91142c-489c-410f-80d7-57a0229bc73e"
+    row0 = goldens()[NAME]["report"][0]  # iterations, shift, sum H_0j N_j, N_0, # H psips, # states
+    ps = t["qmc"]["psips"]
+    assert ps["determinants"].shape == (int(row0[5]), 1)
+    assert int(t["qmc"]["state"]["ncycles"]) == int(row0[0])
+    assert float(t["qmc"]["state"]["shift"][0]) == pytest.approx(row0[1], rel=1e-10)
+    assert float(t["qmc"]["state"]["projected energy"][0]) == pytest.approx(row0[2], rel=1e-10)
+    assert float(t["qmc"]["reference"]["reference population @ t-1"][0]) == pytest.approx(row0[3], rel=1e-10)
+    assert float(np.abs(ps["populations"]).sum()) == row0[4] == float(ps["total population"][0])
+    # the reference determinant (orbitals 1-4, the deck's "det") is bits 0-3
+    assert int(t["qmc"]["reference"]["reference determinant"][0]) == 0b1111
+    assert int(ps["determinants"][0, 0]) == 0b1111
+
+
+def test_restart_state_converts_words_and_scale():
+    rs = R.read_restart(_reference_file_bytes(), nbasis=20, nspaces=1, pop_bit_shift=31)
+    raw = hdf5.read(_reference_file_bytes())["qmc"]["psips"]
+    assert rs.states.dtype == np.uint64 and rs.states.shape == (141, 1)
+    assert np.array_equal(rs.states[:, 0], raw["determinants"][:, 0].astype(np.uint64))
+    assert np.array_equal(rs.pops[:, 0], raw["populations"][:, 0].astype(np.int64) << 31)
+    assert rs.total_population()[0] == 1159.0
+    assert rs.mc_cycles_done == 9000 and bool(rs.vary_shift[0]) and rs.shift_damping == 0.05
+    # 32-bit words pair into 64-bit ones low word first (bit_string_32_to_64)
+    w = R._words64(np.array([[0x1, 0x2, 0x4]], np.int32), 32, 96)
+    assert w.tolist() == [[0x0000000200000001, 0x4]]
+    # change_pop_scaling_64 divides truncating toward zero
+    assert R._rescale(np.array([[7], [-7]]), 4, 1).ravel().tolist() == [1, -1]
+    assert R._rescale(np.array([[3], [-3]]), 1, 8).ravel().tolist() == [24, -24]
+
+
+def test_writer_round_trips():
+    t = hdf5.read(_reference_file_bytes())
+    t2 = hdf5.read(hdf5.write(None, t))
+
+    def same(a, b):
+        if isinstance(a, dict):
+            assert sorted(a) == sorted(b)
+            for k in a:
+                same(a[k], b[k])
+        elif isinstance(a, bytes):
+            assert a == b
+        else:
+            a, b = np.asarray(a), np.asarray(b)
+            assert a.dtype == b.dtype and a.shape == b.shape and np.array_equal(a, b)
+
+    same(t, t2)
+    # more links than one symbol node holds, nested groups, every dtype the writer takes
+    big = {f"d{i:02d}": np.arange(i + 1, dtype=np.int64) for i in range(40)}
+    big["g"] = {"f": np.float64(-2.5), "s": "text", "i32": np.int32(-3), "a": np.ones((3, 2))}
+    got = hdf5.read(hdf5.write(None, big))
+    assert sorted(got) == sorted(big) and np.array_equal(got["d39"], np.arange(40))
+    assert got["g"]["f"] == -2.5 and got["g"]["s"] == b"text" and got["g"]["i32"] == -3
+    assert got["g"]["a"].shape == (3, 2)
+
+
+def test_reader_refuses_what_it_does_not_implement():
+    with pytest.raises(hdf5.HDF5Error):
+        hdf5.read(b"not an hdf5 file at all")
+    with pytest.raises(R.RestartError):
+        R.read_restart(_reference_file_bytes(), nbasis=22, nspaces=1, pop_bit_shift=31)
+    with pytest.raises(R.RestartError):
+        R.read_restart(_reference_file_bytes(), nbasis=20, nspaces=2, pop_bit_shift=31)
+
+
+def _run(tmp_path, nreports, start_state, end_capacity=1 << 16):
+    system, opts = deck(NAME, str(tmp_path))
+    st = api.qmc_in(opts["qmc"], opts.get("fciqmc"))
+    st.nreports = nreports
+    start = R.RestartBuffers.from_state(start_state)
+    end = R.RestartBuffers(system.view.nwords, 1, end_capacity)
+    table = oracle_report(oracle(), system, st, start=start, end=end)
+    return system, opts, st, table, end.to_state(1, api.pop_bit_shift(st), like=start_state)
+
+
+def test_oracle_reproduces_restart_golden(tmp_path):
+    """The whole benchmark table of the reference's restart test (100 reports)."""
+    system, opts = deck(NAME, str(tmp_path))
+    st = api.qmc_in(opts["qmc"], opts.get("fciqmc"))
+    path = api.restart_file(opts, str(tmp_path))
+    assert os.path.basename(path) == "HANDE.RS.0.p0.H5"
+    rs = R.read_restart(path, system.view.nbasis, 1, api.pop_bit_shift(st))
+    _, _, _, table, _ = _run(tmp_path, st.nreports, rs)
+    compare_tables(table, goldens()[NAME]["report"])
+
+
+def test_dump_then_restart_continues_the_state(tmp_path):
+    """A state dumped after 20 reports, written and read back as HDF5, restarts
+    with that report's row; and reading what the writer wrote gives back the
+    same list and estimators."""
+    rs0 = R.read_restart(_reference_file_bytes(), 20, 1, 31)
+    system, opts, st, t1, end = _run(tmp_path, 20, rs0)
+    assert end.mc_cycles_done == 9000 + 20 * 10
+    assert end.nstates == int(t1[-1][5])
+    f = tmp_path / "HANDE.RS.1.p0.H5"
+    R.write_restart(str(f), end, nbasis=20)
+    back = R.read_restart(str(f), 20, 1, 31)
+    assert np.array_equal(back.states, end.states) and np.array_equal(back.pops, end.pops)
+    assert np.array_equal(back.dat, end.dat)
+    assert back.mc_cycles_done == end.mc_cycles_done
+    assert np.array_equal(back.shift, end.shift) and np.array_equal(back.vary_shift, end.vary_shift)
+    assert np.array_equal(back.proj_energy, end.proj_energy)
+    assert np.array_equal(back.D0_population, end.D0_population)
+    assert api.restart_file({"restart": {"read": True}}, str(tmp_path)) == str(f)
+    _, _, _, t2, _ = _run(tmp_path, 3, back)
+    assert np.array_equal(t2[0][:6], t1[-1][:6])
