@@ -78,6 +78,10 @@ class FqhQmcIn(C.Structure):
         ("excit_gen", i32), ("state_size", i64), ("spawned_state_size", i64),
         ("replica_tricks", i32), ("quadrature_initiator", i32), ("rng_mode", i32), ("device", i32),
         ("vary_shift_from_proje", i32), ("proposal_mix", f64), ("proposal_min_attempts", i32),
+        ("vary_shift_set", i32), ("vary_shift", i32),
+        ("blocking_on_the_fly", i32), ("auto_shift_damping", i32), ("shift_damping_precision", f64),
+        ("force_shift_damping_opt", i32), ("start_save_frequency", i32), ("start_point_number", i32),
+        ("start_point", i32), ("error_limit", f64), ("min_blocks_used", i32), ("blocks_used", i32),
     ]
 
 
@@ -86,6 +90,7 @@ class FqhReportRow(C.Structure):
         ("iterations", i64), ("shift", f64 * FQ_MAX_SPACES), ("proj_energy", f64 * FQ_MAX_SPACES),
         ("D0_population", f64 * FQ_MAX_SPACES), ("nparticles", f64 * FQ_MAX_SPACES),
         ("nstates", i64), ("nspawn_events", i64), ("rspawn", f64), ("time_per_cycle", f64),
+        ("shift_damping", f64), ("blocked_proj_energy", f64), ("blocked_proj_error", f64),
     ]
 
 
@@ -94,6 +99,7 @@ class FqhRestart(C.Structure):
         ("nstates", i64), ("capacity", i64), ("states", P(C.c_uint64)), ("pops", P(i64)), ("dat", P(f64)),
         ("mc_cycles_done", i64), ("shift", f64 * FQ_MAX_SPACES), ("vary_shift", C.c_int32 * FQ_MAX_SPACES),
         ("proj_energy", f64 * FQ_MAX_SPACES), ("D0_population", f64 * FQ_MAX_SPACES),
+        ("shift_damping", f64), ("shift_damping_status", i32),
     ]
 
 
@@ -119,6 +125,7 @@ def _declare(lib):
         "fq_spawned_list": (C.c_int, [vp, P(u64), P(i64), P(i64), i64, P(i64)]),
         "fq_cycle_finish": (C.c_int, [vp, P(u64), P(i64), P(i64), i64, P(FqCycleOut)]),
         "fqh_fciqmc_ranks": (C.c_int, [vp, P(FqhQmcIn), C.c_int, P(FqhReportRow), C.c_int, P(C.c_int)]),
+        "fqh_qmc_in_defaults": (None, [P(FqhQmcIn)]),
         "fqh_fciqmc_restart": (C.c_int, [vp, P(FqhQmcIn), P(FqhRestart), P(FqhRestart), P(FqhReportRow), C.c_int,
                                          P(C.c_int)]),
         "fqh_murmur2": (C.c_uint32, [C.c_char_p, C.c_int, C.c_uint32]),

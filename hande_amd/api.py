@@ -101,10 +101,24 @@ def read_in(int_file=None, nel=None, electrons=None, ms=0, sym=None, CAS=None, b
     return System(h, lib)
 
 
-def qmc_in(qmc, fciqmc_opts=None, rng_mode="replay", device=0):
-    """Build the qmc/fciqmc option block with the reference's defaults."""
+BLOCKING_DEFAULTS = {  # blocking{} (src/qmc_data.f90:425-460)
+    "blocking_on_the_fly": False, "auto_shift_damping": False, "shift_damping_precision": 2.0,
+    "force_shift_damping_opt": False, "start_save_frequency": -1, "start_point_number": -1,
+    "start_point": -1, "error_limit": 0.0, "min_blocks_used": 10, "blocks_used": 2147483647,
+}
+
+
+def qmc_in(qmc, fciqmc_opts=None, rng_mode="replay", device=0, blocking=None):
+    """Build the qmc/fciqmc/blocking option block with the reference's defaults."""
     q = dict(qmc)
     f = dict(fciqmc_opts or {})
+    b = dict(blocking or {})
+    unknown = set(b) - set(BLOCKING_DEFAULTS) - {"filename"}
+    if unknown:
+        raise ValueError(f"unknown blocking options: {sorted(unknown)}")
+    b = {**BLOCKING_DEFAULTS, **b}
+    if b["auto_shift_damping"] and not b["blocking_on_the_fly"]:
+        raise ValueError("auto_shift_damping needs blocking_on_the_fly")
     st = N.FqhQmcIn()
     st.tau = float(q["tau"])
     st.rng_seed = int(q.get("rng_seed", 7))
@@ -134,6 +148,18 @@ def qmc_in(qmc, fciqmc_opts=None, rng_mode="replay", device=0):
     # learned proposal options (no reference counterpart)
     st.proposal_mix = float(q.get("proposal_mix", 0.0))
     st.proposal_min_attempts = int(q.get("proposal_min_attempts", 0))
+    st.vary_shift_set = int("vary_shift" in q)
+    st.vary_shift = int(bool(q.get("vary_shift", False)))
+    st.blocking_on_the_fly = int(bool(b["blocking_on_the_fly"]))
+    st.auto_shift_damping = int(bool(b["auto_shift_damping"]))
+    st.shift_damping_precision = float(b["shift_damping_precision"])
+    st.force_shift_damping_opt = int(bool(b["force_shift_damping_opt"]))
+    st.start_save_frequency = int(b["start_save_frequency"])
+    st.start_point_number = int(b["start_point_number"])
+    st.start_point = int(b["start_point"])
+    st.error_limit = float(b["error_limit"])
+    st.min_blocks_used = int(b["min_blocks_used"])
+    st.blocks_used = int(b["blocks_used"])
     return st
 
 
@@ -160,7 +186,7 @@ def pop_bit_shift(st):
 
 
 def fciqmc(sys, qmc, fciqmc=None, rng_mode="replay", device=0, lib=None, nprocs=1, restart=None,
-           return_state=False, **_):
+           return_state=False, blocking=None, max_reports=None, **_):
     """fciqmc{...}: run FCIQMC on the GPU engine and return the report table.
 
     nprocs > 1 (replay mode) replays a run of that many MPI ranks: one engine
@@ -169,10 +195,12 @@ def fciqmc(sys, qmc, fciqmc=None, rng_mode="replay", device=0, lib=None, nprocs=
     continues a stored run, as ``restart = { read = ... }`` does; the file's
     shift damping applies unless qmc sets one (src/qmc.F90:186-192).
     return_state=True returns (table, RestartState after the last report), the
-    state ``restart = { write = ... }`` stores (``restart.write_restart``)."""
+    state ``restart = { write = ... }`` stores (``restart.write_restart``).
+    max_reports stops the run after that many reports (a prefix of the run
+    nreports describes)."""
     from . import restart as R
     lib = lib or sys._lib
-    st = qmc_in(qmc, fciqmc, rng_mode=rng_mode, device=device)
+    st = qmc_in(qmc, fciqmc, rng_mode=rng_mode, device=device, blocking=blocking)
     S = 2 if st.replica_tricks else 1
     pbs = pop_bit_shift(st)
     rs = None
@@ -183,7 +211,7 @@ def fciqmc(sys, qmc, fciqmc=None, rng_mode="replay", device=0, lib=None, nprocs=
             raise R.RestartError("restart state encoded with another population scale")
         if "shift_damping" not in qmc and rs.shift_damping is not None:
             st.shift_damping = rs.shift_damping
-    max_rows = st.nreports + 1
+    max_rows = min(st.nreports, max_reports if max_reports is not None else st.nreports) + 1
     rows = (N.FqhReportRow * max_rows)()
     nrows = C.c_int()
     end = None

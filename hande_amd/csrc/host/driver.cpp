@@ -17,6 +17,7 @@
 #include <cstring>
 #include <vector>
 
+#include "blocking.hpp"
 #include "dsfmt.hpp"
 #include "fqmc.h"
 #include "fqmc_host.h"
@@ -54,6 +55,22 @@ class StreamFeeder {
 
 }  // namespace
 
+extern "C" void fqh_qmc_in_defaults(fqh_qmc_in* in) {
+  if (!in) return;
+  in->vary_shift_set = 0;
+  in->vary_shift = 0;
+  in->blocking_on_the_fly = 0;
+  in->auto_shift_damping = 0;
+  in->shift_damping_precision = 2.0;
+  in->force_shift_damping_opt = 0;
+  in->start_save_frequency = -1;
+  in->start_point_number = -1;
+  in->start_point = -1;
+  in->error_limit = 0.0;
+  in->min_blocks_used = 10;
+  in->blocks_used = 2147483647;
+}
+
 extern "C" int fqh_fciqmc(const fqh_system* hs, const fqh_qmc_in* in, fqh_report_row* rows, int max_rows,
                           int* nrows) {
   return fqh_fciqmc_restart(hs, in, nullptr, nullptr, rows, max_rows, nrows);
@@ -89,9 +106,8 @@ extern "C" int fqh_fciqmc_restart(const fqh_system* hs, const fqh_qmc_in* in, co
   if (rc != FQ_OK) return rc;
   const bool replay = in->rng_mode == FQ_RNG_REPLAY;
   StreamFeeder feeder(e, static_cast<uint32_t>(static_cast<int32_t>(in->rng_seed)));
-  double shift[FQ_MAX_SPACES] = {0, 0}, proj[FQ_MAX_SPACES] = {0, 0}, d0[FQ_MAX_SPACES] = {0, 0};
-  double ntot_old[FQ_MAX_SPACES] = {0, 0}, nparticles[FQ_MAX_SPACES] = {0, 0};
-  bool vary[FQ_MAX_SPACES] = {false, false};
+  double proj[FQ_MAX_SPACES] = {0, 0}, d0[FQ_MAX_SPACES] = {0, 0};
+  double nparticles[FQ_MAX_SPACES] = {0, 0};
   int64_t nstates = 1, cycles0 = 0;
   if (!start) {
     // initial_distribution: the reference determinant with nint(D0_pop)*real_factor.
@@ -99,35 +115,33 @@ extern "C" int fqh_fciqmc_restart(const fqh_system* hs, const fqh_qmc_in* in, co
     double dat0 = 0.0;
     rc = fq_load(e, sys->f0, pop0.data(), &dat0, 1);
     for (int s = 0; s < S; ++s) {
-      shift[s] = in->initial_shift;
       nparticles[s] = std::fabs(static_cast<double>(pop0[s]) / static_cast<double>(rf));
-      ntot_old[s] = nparticles[s];
       // initial_ci_projected_energy over the single reference determinant
       d0[s] = static_cast<double>(pop0[s]) / static_cast<double>(rf);
       proj[s] = 0.0;
     }
   } else {
-    // read_restart_hdf5 + init_estimators: the stored list, shift and flags; the
+    // read_restart_hdf5 + init_estimators: the stored list and estimators; the
     // population recounted as sum |pop| / real_factor in list order.
     rc = fq_load(e, start->states, start->pops, start->dat, start->nstates);
     nstates = start->nstates;
     cycles0 = start->mc_cycles_done;
     for (int s = 0; s < S; ++s) {
-      shift[s] = start->shift[s];
-      vary[s] = start->vary_shift[s] != 0;
       proj[s] = start->proj_energy[s];
       d0[s] = start->D0_population[s];
       double t = 0.0;
       for (int64_t i = 0; i < start->nstates; ++i)
         t = t + std::fabs(static_cast<double>(start->pops[i * S + s]) / static_cast<double>(rf));
       nparticles[s] = t;
-      ntot_old[s] = t;
     }
   }
   if (rc != FQ_OK) {
     fq_destroy(e);
     return rc;
   }
+  fqmc::ShiftControl ctl;
+  ctl.init(*in, S, nparticles, start);
+  const double* shift = ctl.shift;
   int nr = 0;
   auto emit = [&](int64_t iter, int64_t nspawn, double rspawn, double tcyc) {
     if (nr >= max_rows) return;
@@ -138,17 +152,18 @@ extern "C" int fqh_fciqmc_restart(const fqh_system* hs, const fqh_qmc_in* in, co
       r.shift[s] = shift[s];
       r.proj_energy[s] = proj[s];
       r.D0_population[s] = d0[s];
-      r.nparticles[s] = ntot_old[s];
+      r.nparticles[s] = ctl.ntot_old[s];
     }
     r.nstates = nstates;
     r.nspawn_events = nspawn;
     r.rspawn = rspawn;
     r.time_per_cycle = tcyc;
+    r.shift_damping = ctl.damping;
   };
   emit(cycles0, 0, 0.0, 0.0);
   bool overflowed = false;
   int reports_done = 0;
-  for (int ireport = 1; ireport <= in->nreports; ++ireport) {
+  for (int ireport = 1; ireport <= in->nreports && nr < max_rows; ++ireport) {  // a full table ends the run
     double eproj_old[FQ_MAX_SPACES] = {0, 0};
     for (int s = 0; s < S; ++s) eproj_old[s] = (std::fabs(d0[s]) < 2.2250738585072014e-308) ? 0.0 : proj[s] / d0[s];
     double rspawn = 0.0;
@@ -193,26 +208,22 @@ extern "C" int fqh_fciqmc_restart(const fqh_system* hs, const fqh_qmc_in* in, co
       d0[s] = d0[s] / in->mc_cycles;
     }
     rspawn = rspawn / in->mc_cycles;
-    for (int s = 0; s < S; ++s) {
-      if (vary[s]) {
-        const double denom = 1.0 * in->tau * in->mc_cycles;
-        shift[s] = shift[s] - std::log(ntot[s] / ntot_old[s]) * in->shift_damping / denom -
-                   std::log(ntot[s] / in->target_population) * 0.0 / denom;
-      }
-    }
-    for (int s = 0; s < S; ++s) ntot_old[s] = ntot[s];
-    for (int s = 0; s < S; ++s) {
-      if (!vary[s] && ntot[s] > in->target_population) {
-        vary[s] = true;
-        shift[s] = in->vary_shift_from_proje ? proj[s] / d0[s] : in->vary_shift_from;
-      }
-    }
+    const bool soft_exit = ctl.end_report(ntot, proj, d0);
     emit(cycles0 + static_cast<int64_t>(ireport) * in->mc_cycles, nspawn_events, rspawn, tcyc);
+    ctl.after_report(ireport, proj[0], d0[0]);
+    if (nr > 0 && nr <= max_rows) {  // the damping and blocked estimate once this report is processed
+      rows[nr - 1].shift_damping = ctl.damping;
+      if (ctl.blocking()) {
+        rows[nr - 1].blocked_proj_energy = ctl.blocks().proj_energy_mean();
+        rows[nr - 1].blocked_proj_error = ctl.blocks().proj_energy_error();
+      }
+    }
     reports_done = ireport;
     if (error) {
       overflowed = true;
       break;
     }
+    if (soft_exit) break;
   }
   if (rc == FQ_OK && end) {
     // dump_restart_hdf5: the list, shift, flags and the last report's estimators
@@ -220,10 +231,12 @@ extern "C" int fqh_fciqmc_restart(const fqh_system* hs, const fqh_qmc_in* in, co
     end->mc_cycles_done = cycles0 + static_cast<int64_t>(reports_done) * in->mc_cycles;
     for (int s = 0; s < FQ_MAX_SPACES; ++s) {
       end->shift[s] = s < S ? shift[s] : 0.0;
-      end->vary_shift[s] = s < S && vary[s] ? 1 : 0;
+      end->vary_shift[s] = s < S && ctl.vary[s] ? 1 : 0;
       end->proj_energy[s] = s < S ? proj[s] : 0.0;
       end->D0_population[s] = s < S ? d0[s] : 0.0;
     }
+    end->shift_damping = ctl.damping;
+    end->shift_damping_status = ctl.status;
   }
   fq_destroy(e);
   *nrows = nr;
@@ -336,20 +349,21 @@ extern "C" int fqh_fciqmc_ranks(const fqh_system* hs, const fqh_qmc_in* in, int 
   std::vector<int64_t> nst(nprocs, 0);
   for (int s = 0; s < S; ++s) npart[d0_rank * S + s] = std::fabs(static_cast<double>(pop0[s]) / static_cast<double>(rf));
   nst[d0_rank] = 1;
-  double shift[FQ_MAX_SPACES] = {0, 0}, proj[FQ_MAX_SPACES] = {0, 0}, d0[FQ_MAX_SPACES] = {0, 0};
-  double ntot_old[FQ_MAX_SPACES] = {0, 0};
-  bool vary[FQ_MAX_SPACES] = {false, false};
+  double proj[FQ_MAX_SPACES] = {0, 0}, d0[FQ_MAX_SPACES] = {0, 0};
   auto total = [&](int s) {
     double t = 0.0;
     for (int r = 0; r < nprocs; ++r) t = t + npart[r * S + s];
     return t;
   };
+  double ntot0[FQ_MAX_SPACES] = {0, 0};
   for (int s = 0; s < S; ++s) {
-    shift[s] = in->initial_shift;
-    ntot_old[s] = total(s);
+    ntot0[s] = total(s);
     d0[s] = static_cast<double>(pop0[s]) / static_cast<double>(rf);
     proj[s] = 0.0;
   }
+  fqmc::ShiftControl ctl;  // estimators are the ranks' sums, so the parent's bookkeeping applies
+  ctl.init(*in, S, ntot0, nullptr);
+  const double* shift = ctl.shift;
   int nr = 0;
   auto emit = [&](int64_t iter, int64_t nspawn, double rspawn, double tcyc) {
     if (nr >= max_rows) return;
@@ -360,7 +374,7 @@ extern "C" int fqh_fciqmc_ranks(const fqh_system* hs, const fqh_qmc_in* in, int 
       row.shift[s] = shift[s];
       row.proj_energy[s] = proj[s];
       row.D0_population[s] = d0[s];
-      row.nparticles[s] = ntot_old[s];
+      row.nparticles[s] = ctl.ntot_old[s];
     }
     int64_t t = 0;
     for (int r = 0; r < nprocs; ++r) t += nst[r];
@@ -368,6 +382,7 @@ extern "C" int fqh_fciqmc_ranks(const fqh_system* hs, const fqh_qmc_in* in, int 
     row.nspawn_events = nspawn;
     row.rspawn = rspawn;
     row.time_per_cycle = tcyc;
+    row.shift_damping = ctl.damping;
   };
   emit(0, 0, 0.0, 0.0);
   bool overflowed = false;
@@ -376,7 +391,7 @@ extern "C" int fqh_fciqmc_ranks(const fqh_system* hs, const fqh_qmc_in* in, int 
   std::vector<std::vector<int64_t>> spo(nprocs), sfl(nprocs);
   std::vector<uint64_t> rst;
   std::vector<int64_t> rpo, rfl;
-  for (int ireport = 1; ireport <= in->nreports && rc == FQ_OK; ++ireport) {
+  for (int ireport = 1; ireport <= in->nreports && nr < max_rows && rc == FQ_OK; ++ireport) {
     double eproj_old[FQ_MAX_SPACES] = {0, 0};
     for (int s = 0; s < S; ++s) eproj_old[s] = (std::fabs(d0[s]) < 2.2250738585072014e-308) ? 0.0 : proj[s] / d0[s];
     std::vector<double> rproj(static_cast<size_t>(nprocs) * S, 0.0), rd0(static_cast<size_t>(nprocs) * S, 0.0);
@@ -452,25 +467,21 @@ extern "C" int fqh_fciqmc_ranks(const fqh_system* hs, const fqh_qmc_in* in, int 
       nspawn_events += nsp[r];
     }
     rspawn = rspawn / (in->mc_cycles * nprocs);
-    for (int s = 0; s < S; ++s) {
-      if (vary[s]) {
-        const double denom = 1.0 * in->tau * in->mc_cycles;
-        shift[s] = shift[s] - std::log(ntot[s] / ntot_old[s]) * in->shift_damping / denom -
-                   std::log(ntot[s] / in->target_population) * 0.0 / denom;
-      }
-    }
-    for (int s = 0; s < S; ++s) ntot_old[s] = ntot[s];
-    for (int s = 0; s < S; ++s) {
-      if (!vary[s] && ntot[s] > in->target_population) {
-        vary[s] = true;
-        shift[s] = in->vary_shift_from_proje ? proj[s] / d0[s] : in->vary_shift_from;
-      }
-    }
+    const bool soft_exit = ctl.end_report(ntot, proj, d0);
     emit(static_cast<int64_t>(ireport) * in->mc_cycles, nspawn_events, rspawn, tcyc);
+    ctl.after_report(ireport, proj[0], d0[0]);
+    if (nr > 0 && nr <= max_rows) {
+      rows[nr - 1].shift_damping = ctl.damping;
+      if (ctl.blocking()) {
+        rows[nr - 1].blocked_proj_energy = ctl.blocks().proj_energy_mean();
+        rows[nr - 1].blocked_proj_error = ctl.blocks().proj_energy_error();
+      }
+    }
     if (error) {
       overflowed = true;
       break;
     }
+    if (soft_exit) break;
   }
   cleanup();
   *nrows = nr;

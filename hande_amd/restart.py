@@ -59,6 +59,7 @@ class RestartState:
     D0_population: np.ndarray   # (nspaces,)
     pop_bit_shift: int = 0
     shift_damping: float | None = None
+    shift_damping_status: int = -2             # sd_no_optimisation unless the file says otherwise
     reference: np.ndarray | None = None        # (nwords,) uint64
     hs_reference: np.ndarray | None = None
     extra: dict = field(default_factory=dict)  # other items read from the file
@@ -166,9 +167,11 @@ def read_restart(path, nbasis, nspaces=1, pop_bit_shift=0):
         hs_reference=(_words64(np.asarray(ref["Hilbert space reference determinant"])[None, :], i0_length, nbasis)[0]
                       if "Hilbert space reference determinant" in ref else None),
     )
-    for k in ("hash_seed", "move_freq", "shift_damping_status"):
+    for k in ("hash_seed", "move_freq"):
         if k in state:
             rs.extra[k] = int(state[k])
+    if "shift_damping_status" in state:
+        rs.shift_damping_status = int(state["shift_damping_status"])
     for k in ("uuid", "hande version", "date"):
         if k in meta:
             rs.extra[k] = meta[k]
@@ -211,7 +214,7 @@ def restart_tree(rs: RestartState, nbasis, shift_damping=0.05, calc_uuid=None):
                 "move_freq": np.int32(rs.extra.get("move_freq", 0)),
                 "vary shift": rs.vary_shift.astype(np.int32),
                 "shift_damping": np.array([shift_damping if rs.shift_damping is None else rs.shift_damping]),
-                "shift_damping_status": np.int32(rs.extra.get("shift_damping_status", -2)),
+                "shift_damping_status": np.int32(rs.shift_damping_status),
             },
             "reference": {
                 "reference determinant": np.asarray(ref, np.uint64).view(np.int64),
@@ -257,6 +260,8 @@ class RestartBuffers:
         st = b.st
         st.nstates = n
         st.mc_cycles_done = int(rs.mc_cycles_done)
+        st.shift_damping = float(rs.shift_damping) if rs.shift_damping is not None else 0.0
+        st.shift_damping_status = int(rs.shift_damping_status)
         for s in range(rs.nspaces):
             st.shift[s] = float(rs.shift[s])
             st.vary_shift[s] = int(bool(rs.vary_shift[s]))
@@ -274,7 +279,8 @@ class RestartBuffers:
             mc_cycles_done=int(st.mc_cycles_done),
             shift=np.array(st.shift[:nspaces]), vary_shift=np.array(st.vary_shift[:nspaces]).astype(bool),
             proj_energy=np.array(st.proj_energy[:nspaces]), D0_population=np.array(st.D0_population[:nspaces]),
-            pop_bit_shift=pop_bit_shift)
+            pop_bit_shift=pop_bit_shift, shift_damping=float(st.shift_damping),
+            shift_damping_status=int(st.shift_damping_status))
         if like is not None:
             rs.reference, rs.hs_reference, rs.extra = like.reference, like.hs_reference, dict(like.extra)
         return rs

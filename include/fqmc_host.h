@@ -73,7 +73,26 @@ typedef struct fqh_qmc_in {
                                     energy (src/energy_evaluation.F90:638-648) */
   double proposal_mix;           /* fq_params.proposal_mix (learned proposal; 0: default) */
   int32_t proposal_min_attempts; /* fq_params.proposal_min_attempts */
+  int32_t vary_shift_set;        /* qmc{ vary_shift = ... } given: it overrides the restart's flag and, if
+                                    false, resets the shift to initial_shift (src/qmc.F90:882-891) */
+  int32_t vary_shift;
+  /* blocking{} (src/qmc_data.f90:425-460; defaults in fqh_qmc_in_defaults) */
+  int32_t blocking_on_the_fly;
+  int32_t auto_shift_damping;
+  double shift_damping_precision;
+  int32_t force_shift_damping_opt;
+  int32_t start_save_frequency;
+  int32_t start_point_number;
+  int32_t start_point;
+  double error_limit;
+  int32_t min_blocks_used;
+  int32_t blocks_used;
 } fqh_qmc_in;
+
+/* Fill the blocking{} fields with the reference's defaults (on-the-fly
+ * blocking and auto shift damping off, start points chosen automatically,
+ * shift_damping_precision 2, min_blocks_used 10, blocks_used huge). */
+void fqh_qmc_in_defaults(fqh_qmc_in *in);
 
 /* One row of the QMC report table (src/qmc_io.f90:412-508). */
 typedef struct fqh_report_row {
@@ -86,11 +105,16 @@ typedef struct fqh_report_row {
   int64_t nspawn_events;
   double rspawn;
   double time_per_cycle;
+  double shift_damping;         /* damping in force after this report (auto_shift_damping may change it) */
+  double blocked_proj_energy;   /* on-the-fly blocking: optimal-block E_proj - E_HF so far (0 if none) */
+  double blocked_proj_error;    /* and its standard error */
 } fqh_report_row;
 
 /* Run do_fciqmc on the engine: rows[0] is the iteration-0 status line.
- * Fills at most max_rows rows and sets *nrows.  On error returns the engine's
- * status code. */
+ * Fills at most max_rows rows and sets *nrows: the run stops once the table is
+ * full, so a short table runs a prefix of the calculation nreports describes
+ * (everything that depends on nreports, such as the blocking start points,
+ * is as in the full run).  On error returns the engine's status code. */
 int fqh_fciqmc(const fqh_system *s, const fqh_qmc_in *in, fqh_report_row *rows, int max_rows, int *nrows);
 
 /* do_fciqmc on nprocs emulated MPI ranks (replay mode): one engine per rank on
@@ -118,6 +142,8 @@ typedef struct fqh_restart {
   int32_t vary_shift[FQ_MAX_SPACES];
   double proj_energy[FQ_MAX_SPACES];    /* sum_j H_0j N_j of the last report */
   double D0_population[FQ_MAX_SPACES];  /* N_0 of the last report */
+  double shift_damping;                 /* qmc/state shift_damping (<= 0: none stored) */
+  int32_t shift_damping_status;         /* qmc/state shift_damping_status (sd_no_optimisation = -2, ...) */
 } fqh_restart;
 
 /* do_fciqmc restarted from `start` (read_restart_hdf5 and the restart branches

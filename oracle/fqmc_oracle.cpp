@@ -48,6 +48,7 @@
 #include <string>
 #include <vector>
 
+#include "blocking.hpp"
 #include "dsfmt.hpp"
 #include "fqmc.h"
 #include "fqmc_host.h"
@@ -1558,8 +1559,7 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
     }
   }
   const int64_t cycles0 = start ? start->mc_cycles_done : 0;
-  double shift[FQ_MAX_SPACES], proj[FQ_MAX_SPACES], d0[FQ_MAX_SPACES], ntot_old[FQ_MAX_SPACES];
-  bool vary[FQ_MAX_SPACES];
+  double proj[FQ_MAX_SPACES], d0[FQ_MAX_SPACES], ntot0[FQ_MAX_SPACES];
   auto total_particles = [&](int s) {
     double t = 0.0;
     for (auto& o : ranks) t = t + o->nparticles[s];
@@ -1571,11 +1571,14 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
     return t;
   };
   for (int s = 0; s < S; ++s) {
-    shift[s] = start ? start->shift[s] : in->initial_shift;
-    vary[s] = start ? start->vary_shift[s] != 0 : false;
-    ntot_old[s] = total_particles(s);
+    ntot0[s] = total_particles(s);
     proj[s] = d0[s] = 0.0;
   }
+  // shift, vary flags, damping and on-the-fly blocking between reports
+  // (the report-loop bookkeeping the drivers share; pinned by the goldens)
+  fqmc::ShiftControl ctl;
+  ctl.init(*in, S, ntot0, start);
+  const double* shift = ctl.shift;
   if (start) {
     for (int s = 0; s < S; ++s) {
       proj[s] = start->proj_energy[s];
@@ -1601,9 +1604,10 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
       r.shift[s] = shift[s];
       r.proj_energy[s] = proj[s];
       r.D0_population[s] = d0[s];
-      r.nparticles[s] = ntot_old[s];
+      r.nparticles[s] = ctl.ntot_old[s];
     }
     r.nstates = total_states();
+    r.shift_damping = ctl.damping;
     r.nspawn_events = nspawn;
     r.rspawn = rspawn;
   };
@@ -1611,7 +1615,7 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
   int reports_done = 0;
   std::vector<fq_cycle_out> out(nprocs);
   std::vector<std::vector<Spawned>> sent(nprocs);
-  for (int ireport = 1; ireport <= in->nreports; ++ireport) {
+  for (int ireport = 1; ireport <= in->nreports && nr < max_rows; ++ireport) {  // a full table ends the run
     double eproj_old[FQ_MAX_SPACES];
     for (int s = 0; s < S; ++s) eproj_old[s] = (std::fabs(d0[s]) < 2.2250738585072014e-308) ? 0.0 : proj[s] / d0[s];
     // per-rank report accumulators
@@ -1667,23 +1671,18 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
       nsp += nspawn_events[r];
     }
     rs = rs / (in->mc_cycles * nprocs);
-    for (int s = 0; s < S; ++s) {
-      if (vary[s]) {
-        const double denom = 1.0 * in->tau * in->mc_cycles;
-        shift[s] = shift[s] - std::log(ntot[s] / ntot_old[s]) * in->shift_damping / denom -
-                   std::log(ntot[s] / in->target_population) * 0.0 / denom;
-      }
-    }
-    for (int s = 0; s < S; ++s) ntot_old[s] = ntot[s];
-    for (int s = 0; s < S; ++s) {
-      if (!vary[s] && ntot[s] > in->target_population) {
-        vary[s] = true;
-        shift[s] = in->vary_shift_from_proje ? proj[s] / d0[s] : in->vary_shift_from;
-      }
-    }
+    const bool soft_exit = ctl.end_report(ntot, proj, d0);
     emit(cycles0 + static_cast<int64_t>(ireport) * in->mc_cycles, nsp, rs);
+    ctl.after_report(ireport, proj[0], d0[0]);
+    if (nr > 0 && nr <= max_rows) {
+      rows[nr - 1].shift_damping = ctl.damping;
+      if (ctl.blocking()) {
+        rows[nr - 1].blocked_proj_energy = ctl.blocks().proj_energy_mean();
+        rows[nr - 1].blocked_proj_error = ctl.blocks().proj_energy_error();
+      }
+    }
     reports_done = ireport;
-    if (error) break;
+    if (error || soft_exit) break;
   }
   *nrows = nr;
   if (end) {
@@ -1694,10 +1693,12 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
     end->mc_cycles_done = cycles0 + static_cast<int64_t>(reports_done) * in->mc_cycles;
     for (int s = 0; s < FQ_MAX_SPACES; ++s) {
       end->shift[s] = s < S ? shift[s] : 0.0;
-      end->vary_shift[s] = s < S && vary[s] ? 1 : 0;
+      end->vary_shift[s] = s < S && ctl.vary[s] ? 1 : 0;
       end->proj_energy[s] = s < S ? proj[s] : 0.0;
       end->D0_population[s] = s < S ? d0[s] : 0.0;
     }
+    end->shift_damping = ctl.damping;
+    end->shift_damping_status = ctl.status;
   }
   return 0;
 }

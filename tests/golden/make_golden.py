@@ -52,6 +52,10 @@ QMC_TESTS = [
     # a run continued from the reference's restart file (HDF5, integer populations
     # read into real_64 amplitudes)
     ("He2-aug-cc-pVDZ_real_64_restart", "fciqmc_real_64/np1/He2-aug-cc-pVDZ_real_64_restart", "restart.in"),
+    # restart + initiator + vary_shift from the start + on-the-fly blocking with
+    # automatic shift damping
+    ("He2-aug-cc-pVDZ_real_64_auto_shift_damping_restart",
+     "ifciqmc_real_64/np1/He2-aug-cc-pVDZ_real_64_auto_shift_damping_restart", "read.in"),
 ]
 
 # long tables are stored truncated (the tests compare a prefix)
@@ -135,6 +139,20 @@ def parse_report_table(lines):
             continue
         rows.append([float(x) for k, x in enumerate(parts) if k not in drop])
     return rows
+
+
+def parse_shift_damping_changes(lines):
+    """(iterations of the report row above, new damping) for every
+    '# Auto-Shift-Damping: Shift damping changed to' line of the table."""
+    out, last = [], None
+    for line in lines:
+        m = re.match(r"^\s*#?\s*(\d+)\s+" + NUM, line)
+        if m:
+            last = int(m.group(1))
+        m = re.search(r"Shift damping changed to\s+(" + NUM + ")", line)
+        if m and last is not None:
+            out.append([last, float(m.group(1))])
+    return out
 
 
 def parse_basis(lines):
@@ -234,6 +252,12 @@ def main():
         table = parse_report_table(lines)
         if table is not None:
             entry["report"] = table[:MAX_ROWS.get(name, len(table))]
+            # the header and first rows verbatim, for the text-format check
+            hdr = next(i for i, line in enumerate(lines) if re.match(r"^\s*#\s+iterations\s+Shift", line))
+            entry["report_text"] = lines[hdr - (1 if "Replica" in lines[hdr - 1] else 0):hdr + 6]
+            changes = parse_shift_damping_changes(lines)
+            if changes:
+                entry["shift_damping_changes"] = changes
         if name.startswith("fci_"):
             eig = []
             for i, line in enumerate(lines):
@@ -247,7 +271,8 @@ def main():
         # data files the deck reads (integral dumps, restart files), stored gzipped
         entry["data_files"] = []
         for x in sorted(os.listdir(path)):
-            is_restart = re.fullmatch(r"HANDE\.RS\.\d+\.p\d+\.H5", x) is not None and "restart" in inp
+            is_restart = (re.fullmatch(r"HANDE\.RS\.\d+\.p\d+\.H5", x) is not None
+                          and re.search(r"restart\s*=\s*\{[^}]*read", entry["input"]) is not None)
             if is_restart or (x in ("FCIDUMP", "INTDUMP") and not name.startswith("fci_")
                               and not name.endswith("-Lz")):
                 # the integral dump (or restart file) the deck reads: test input data, stored gzipped

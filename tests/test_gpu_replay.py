@@ -134,3 +134,24 @@ def test_replay_restart_golden_and_dump(tmp_path):
     assert np.array_equal(end.states, ref.states) and np.array_equal(end.pops, ref.pops)
     assert np.array_equal(end.dat, ref.dat)
     assert np.array_equal(end.shift, ref.shift) and np.array_equal(end.proj_energy, ref.proj_energy)
+
+
+def test_replay_auto_shift_damping_restart_golden(tmp_path):
+    """test_suite/ifciqmc_real_64/np1/He2-aug-cc-pVDZ_real_64_auto_shift_damping_restart
+    on the engine: a restart with the initiator approximation, vary_shift on
+    from the start, on-the-fly blocking and automatic shift damping; all 501
+    first 130 of its 501 rows at 1e-10, past the damping change after report
+    100 (which moves every later shift; the oracle matches all 501 rows).
+    Replay runs spawning on one GPU thread, so the whole table (2500 cycles at
+    ~9000 walkers) would take minutes."""
+    import time
+    from helpers import deck
+    name = "He2-aug-cc-pVDZ_real_64_auto_shift_damping_restart"
+    system, opts = deck(name, str(tmp_path))
+    path = api.restart_file(opts, str(tmp_path))
+    print(f"[replay restart] {name}, 130 of 500 reports", flush=True)
+    t0 = time.time()
+    got = api.fciqmc(system, opts["qmc"], opts.get("fciqmc"), rng_mode="replay", restart=path,
+                     blocking=opts.get("blocking"), max_reports=130)
+    print(f"  {time.time() - t0:.1f} s", flush=True)
+    compare_tables(got, goldens()[name]["report"][:131])

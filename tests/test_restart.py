@@ -133,3 +133,65 @@ def test_dump_then_restart_continues_the_state(tmp_path):
     assert api.restart_file({"restart": {"read": True}}, str(tmp_path)) == str(f)
     _, _, _, t2, _ = _run(tmp_path, 3, back)
     assert np.array_equal(t2[0][:6], t1[-1][:6])
+
+
+ASD = "He2-aug-cc-pVDZ_real_64_auto_shift_damping_restart"
+
+
+def test_oracle_reproduces_auto_shift_damping_golden(tmp_path):
+    """test_suite/ifciqmc_real_64/np1/He2-aug-cc-pVDZ_real_64_auto_shift_damping_restart:
+    a restart (initiator, real_64) with vary_shift forced on from the start,
+    on-the-fly blocking and automatic shift damping (src/blocking.F90).  Every
+    one of its 501 rows, and the damping change the reference prints after
+    report 100 (6.5180314754E-02, with the shift reset to the projected
+    energy), which depends on the reblocking's start-point and block-size
+    choices."""
+    import ctypes as C
+    from hande_amd import _native as N
+    system, opts = deck(ASD, str(tmp_path))
+    st = api.qmc_in(opts["qmc"], opts.get("fciqmc"), blocking=opts.get("blocking"))
+    assert st.blocking_on_the_fly and st.auto_shift_damping and st.vary_shift_set and st.vary_shift
+    rs = R.read_restart(api.restart_file(opts, str(tmp_path)), system.view.nbasis, 1, api.pop_bit_shift(st))
+    assert rs.shift_damping_status == 1 and not rs.vary_shift[0]  # sd_wait1; the deck forces vary_shift on
+    rows = (N.FqhReportRow * (st.nreports + 1))()
+    nr = C.c_int()
+    start = R.RestartBuffers.from_state(rs)
+    rc = oracle().orc_fciqmc_restart(system.view, C.byref(st), C.byref(start.st), None, rows, st.nreports + 1,
+                                     C.byref(nr))
+    assert rc == 0
+    e = goldens()[ASD]
+    compare_tables(api.rows_to_table(rows, nr.value, 1), e["report"])
+    changes = [(rows[i].iterations, rows[i].shift_damping) for i in range(1, nr.value)
+               if rows[i].shift_damping != rows[i - 1].shift_damping]
+    assert len(changes) == len(e["shift_damping_changes"]) == 1
+    assert changes[0][0] == e["shift_damping_changes"][0][0]
+    assert changes[0][1] == pytest.approx(e["shift_damping_changes"][0][1], rel=1e-10)
+    # the blocked estimate is populated once blocking has data
+    assert rows[nr.value - 1].blocked_proj_error > 0
+
+
+def test_blocking_options_are_checked():
+    with pytest.raises(ValueError):
+        api.qmc_in({"tau": 0.1}, blocking={"auto_shift_damping": True})
+    with pytest.raises(ValueError):
+        api.qmc_in({"tau": 0.1}, blocking={"no_such_option": 1})
+    st = api.qmc_in({"tau": 0.1})
+    assert st.min_blocks_used == 10 and st.blocks_used == 2147483647 and st.shift_damping_precision == 2.0
+
+
+def test_short_table_runs_a_prefix_of_the_same_calculation(tmp_path):
+    """A table of 131 rows stops the run after 130 of its 500 reports; the
+    rows are those of the full run (the blocking start points, sized from
+    nreports, are unchanged), damping change included."""
+    import ctypes as C
+    from hande_amd import _native as N
+    system, opts = deck(ASD, str(tmp_path))
+    st = api.qmc_in(opts["qmc"], opts.get("fciqmc"), blocking=opts.get("blocking"))
+    rs = R.read_restart(api.restart_file(opts, str(tmp_path)), system.view.nbasis, 1, api.pop_bit_shift(st))
+    rows = (N.FqhReportRow * 131)()
+    nr = C.c_int()
+    start = R.RestartBuffers.from_state(rs)
+    assert oracle().orc_fciqmc_restart(system.view, C.byref(st), C.byref(start.st), None, rows, 131,
+                                       C.byref(nr)) == 0
+    assert nr.value == 131
+    compare_tables(api.rows_to_table(rows, nr.value, 1), goldens()[ASD]["report"][:131])
