@@ -348,6 +348,8 @@ struct fq_engine {
   int64_t* pass_off = nullptr;             // their exclusive prefix (+ total)
   int64_t* run_tcnt = nullptr;             // run heads per run tile
   int64_t* run_toff = nullptr;             // their exclusive prefix; [ntiles] = the run count
+  int64_t* run_tfirst = nullptr;           // first head of each run tile (-1: none)
+  int64_t* run_tpre = nullptr;             // sums of each run tile's entries before its first head
   uint64_t* rdet = nullptr;      // W*scap
   int64_t* rpop = nullptr;       // S*scap
   int64_t* rflag = nullptr;      // scap
@@ -383,6 +385,11 @@ struct fq_engine {
   Stats h_stats{};
   uint64_t cycle_index = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // the spawned-list length, spawn overflow and attempt total, copied behind
+  // the spawning kernels into pinned memory (h_len) and marked by ev_len
+  int64_t* h_len = nullptr;
+  hipEvent_t ev_len = nullptr;
+  int64_t spawn_bound = 0;  // attempts the cycle's spawning launches covered
   float t_spawn = 0, t_annih = 0, t_total = 0;
   // phase events: 0 cycle start, 1 det pass end, 2 spawn start, 3 spawn end,
   // 4 sort start, 5 sort end, 6 reduce end, 7 merge start, 8 merge end
@@ -639,6 +646,64 @@ __global__ void k_tile_bounds(int64_t nent, const int64_t* __restrict__ att_off,
 }
 constexpr int kSpawnWin = 1024;          // parent-window entries staged in LDS
 
+// ---------------------------------------------------------------------------
+// Sort keys.  A spawned entry's key is v << flag_bits | flags over kw 64-bit
+// words (kw = 1 or 2): v is the determinant's combinatorial rank (or, for
+// batched instances, its encoded word 0, instance index included), flags its
+// initiator flag bits (one per space).  Ordering the v fields orders
+// determinants as the main list does (last word most significant,
+// src/bit_utils.F90:452-480); the LSD radix passes order only the digits of
+// v and are stable, so equal determinants keep spawned (attempt) order.  The
+// entry's populations travel with the key through the passes (the payload),
+// so the run reduction reads everything contiguously, and the determinant is
+// recovered from v.
+//
+// Combinatorial rank: for a bit string with set bits p_1 < ... < p_k
+// (0-based), rank = sum_i C(p_i, i).  For a fixed number of set bits this is
+// a bijection onto [0, C(nbasis, k)) that preserves the strings' numeric
+// order, so the sort needs ceil(log2 C(nbasis, nel)) bits instead of nbasis
+// (34 instead of 52 for 10 electrons in 52 spin orbitals).  binom holds
+// C(p, i) for p < nbasis, 1 <= i <= nel as (low, high) 64-bit halves.
+// ---------------------------------------------------------------------------
+struct KeyPlan {
+  int kw;          // key words
+  int flag_bits;   // width of the flag field (the number of spaces)
+  int rank;        // 1: v = combinatorial rank; 0: v = determinant word 0
+  int nel, nbasis;
+  const uint64_t* binom;
+  int binom_lds;   // stage the table in dynamic LDS (k_pack_tiles, k_runs)
+};
+constexpr int kMaxPasses = 16;
+struct DigitPlan {
+  int np;                  // passes
+  int shift[kMaxPasses];   // first key bit of each pass's digit
+  int bits[kMaxPasses];    // digit width (8 or 9)
+};
+// Keys per thread of a sort tile: the tile's keys and payload sit in LDS.
+__host__ __device__ constexpr int sort_items(int kw, int S) { return kw + S <= 2 ? 16 : 8; }
+constexpr int kSortBlocks = 512;  // blocks of a radix pass (each owns a contiguous range of tiles)
+
+__device__ __forceinline__ void make_key(const KeyPlan& kp, const uint64_t* f, int W, uint64_t flags, uint64_t* key) {
+  unsigned __int128 v = 0;
+  if (kp.rank) {
+    int k = 0;
+    FQ_FOR_WORDS(w, W) {
+      for (uint64_t x = f[w]; x && k < kp.nel; x &= x - 1) {
+        const int p = (w << 6) + __builtin_ctzll(x);
+        if (p >= kp.nbasis) break;  // not a determinant of this system (caller-supplied list)
+        const uint64_t* c = kp.binom + 2 * (static_cast<int64_t>(p) * kp.nel + k);
+        v += (static_cast<unsigned __int128>(c[1]) << 64) | c[0];
+        ++k;
+      }
+    }
+  } else {
+    v = f[0];
+  }
+  const unsigned __int128 k = (v << kp.flag_bits) | (flags & ((uint64_t{1} << kp.flag_bits) - 1));
+  key[0] = static_cast<uint64_t>(k);
+  key[1] = static_cast<uint64_t>(k >> 64);
+}
+
 // Production: one thread per spawning attempt.  Each block takes tiles of
 // kSpawnTile consecutive attempts of the batch [t_begin, t_end); the parents
 // of a tile's first and last attempts bound a short window of the attempt
@@ -655,6 +720,7 @@ __global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, 
                                                   uint64_t* __restrict__ st_det, int64_t* __restrict__ st_pop,
                                                   int64_t* __restrict__ st_ent, int64_t* __restrict__ tile_cnt,
                                                   const int64_t* __restrict__ tile_win,
+                                                  const int64_t* __restrict__ pflags,
                                                   const Sys* __restrict__ sys_arr, const int64_t* __restrict__ iatt,
                                                   const double* __restrict__ itau) {
   // the system descriptor is read from device memory (scalar loads on use)
@@ -824,7 +890,8 @@ __global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, 
         const int64_t u = (t0 - t_begin) + pos;
         FQ_FOR_WORDS(w, s.W) st_det[w * bcap + u] = child[w];
         st_pop[u] = nspawn;
-        st_ent[u] = e;
+        // the spawning space and the parent's initiator flags (the sort key's flag bits)
+        st_ent[u] = (e % s.S) | (s.initiator ? pflags[e / s.S] << 8 : 0);
       }
       run += round_total;
       __syncthreads();
@@ -834,79 +901,21 @@ __global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, 
   }
 }
 
-// ---------------------------------------------------------------------------
-// Sort keys.  A spawned entry's key is v << flag_bits | flags over kw 64-bit
-// words (kw = 1 or 2): v is the determinant's combinatorial rank (or, for
-// batched instances, its encoded word 0, instance index included), flags its
-// initiator flag bits (one per space).  Ordering the v fields orders
-// determinants as the main list does (last word most significant,
-// src/bit_utils.F90:452-480); the LSD radix passes order only the digits of
-// v and are stable, so equal determinants keep spawned (attempt) order.  The
-// entry's populations travel with the key through the passes (the payload),
-// so the run reduction reads everything contiguously, and the determinant is
-// recovered from v.
-//
-// Combinatorial rank: for a bit string with set bits p_1 < ... < p_k
-// (0-based), rank = sum_i C(p_i, i).  For a fixed number of set bits this is
-// a bijection onto [0, C(nbasis, k)) that preserves the strings' numeric
-// order, so the sort needs ceil(log2 C(nbasis, nel)) bits instead of nbasis
-// (34 instead of 52 for 10 electrons in 52 spin orbitals).  binom holds
-// C(p, i) for p < nbasis, 1 <= i <= nel as (low, high) 64-bit halves.
-// ---------------------------------------------------------------------------
-struct KeyPlan {
-  int kw;          // key words
-  int flag_bits;   // width of the flag field (the number of spaces)
-  int rank;        // 1: v = combinatorial rank; 0: v = determinant word 0
-  int nel, nbasis;
-  const uint64_t* binom;
-  int binom_lds;   // stage the table in dynamic LDS (k_pack_tiles, k_runs)
-};
-constexpr int kMaxPasses = 16;
-struct DigitPlan {
-  int np;                  // passes
-  int shift[kMaxPasses];   // first key bit of each pass's digit
-  int bits[kMaxPasses];    // digit width (8 or 9)
-};
-// Keys per thread of a sort tile: the tile's keys and payload sit in LDS.
-__host__ __device__ constexpr int sort_items(int kw, int S) { return kw + S <= 2 ? 16 : 8; }
-constexpr int kSortBlocks = 512;  // blocks of a radix pass (each owns a contiguous range of tiles)
-
-__device__ __forceinline__ void make_key(const KeyPlan& kp, const uint64_t* f, int W, uint64_t flags, uint64_t* key) {
-  unsigned __int128 v = 0;
-  if (kp.rank) {
-    int k = 0;
-    FQ_FOR_WORDS(w, W) {
-      for (uint64_t x = f[w]; x && k < kp.nel; x &= x - 1) {
-        const int p = (w << 6) + __builtin_ctzll(x);
-        if (p >= kp.nbasis) break;  // not a determinant of this system (caller-supplied list)
-        const uint64_t* c = kp.binom + 2 * (static_cast<int64_t>(p) * kp.nel + k);
-        v += (static_cast<unsigned __int128>(c[1]) << 64) | c[0];
-        ++k;
-      }
-    }
-  } else {
-    v = f[0];
-  }
-  const unsigned __int128 k = (v << kp.flag_bits) | (flags & ((uint64_t{1} << kp.flag_bits) - 1));
-  key[0] = static_cast<uint64_t>(k);
-  key[1] = static_cast<uint64_t>(k >> 64);
-}
-
 // Pack each tile's compacted successes into the spawned list after the entries
-// of earlier batches (st->nspawn), in attempt order, as sort keys and payload
-// (the populations); blocks loop over tiles.
-__global__ __launch_bounds__(kBlock) void k_pack_tiles(Sys s, int64_t ntiles, const int64_t* __restrict__ tile_cnt,
+// of earlier batches (st->nspawn), in attempt order, as sort keys (made here,
+// the binomial table staged in LDS) and payload (the population in its
+// space's word, 0 in the other); blocks loop over tiles.  st_ent holds the
+// spawning space and, from bit 8, the parent's initiator flags.
+__global__ __launch_bounds__(kBlock) void k_pack_tiles(int W, int S, int64_t ntiles,
+                                                       const int64_t* __restrict__ tile_cnt,
                                                        const int64_t* __restrict__ tile_off,
                                                        const uint64_t* __restrict__ st_det,
                                                        const int64_t* __restrict__ st_pop,
                                                        const int64_t* __restrict__ st_ent, int64_t bcap,
-                                                       int64_t t_begin, const int64_t* __restrict__ flags,
                                                        int64_t scap, KeyPlan kp, uint64_t* __restrict__ keys,
                                                        int64_t* __restrict__ pay, Stats* __restrict__ st) {
-  // the binomial table of the keys' ranks in LDS when the launch provides room
-  // for it (blocks loop over tiles, so each stages it once)
   extern __shared__ uint64_t binom_lds[];
-  if (kp.rank && kp.binom_lds) {
+  if (kp.rank && kp.binom_lds) {  // blocks loop over tiles, so each stages the table once
     const int nw = 2 * kp.nbasis * kp.nel;
     for (int k = threadIdx.x; k < nw; k += kBlock) binom_lds[k] = kp.binom[k];
     __syncthreads();
@@ -914,26 +923,25 @@ __global__ __launch_bounds__(kBlock) void k_pack_tiles(Sys s, int64_t ntiles, co
   }
   const int64_t nspawn0 = st->nspawn;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-  const int64_t cnt = tile_cnt[tile];
-  const int64_t base = nspawn0 + tile_off[tile];
-  for (int64_t j = threadIdx.x; j < cnt; j += kBlock) {
-    const int64_t u = tile * kSpawnTile + j;
-    const int64_t slot = base + j;
-    if (slot >= scap) {
-      st->spawn_overflow = 1;
-      continue;
+    const int64_t cnt = tile_cnt[tile];
+    const int64_t base = nspawn0 + tile_off[tile];
+    for (int64_t j = threadIdx.x; j < cnt; j += kBlock) {
+      const int64_t u = tile * kSpawnTile + j;
+      const int64_t slot = base + j;
+      if (slot >= scap) {
+        st->spawn_overflow = 1;
+        continue;
+      }
+      const int64_t e = st_ent[u];
+      const int sp = static_cast<int>(e & 0xff);
+      uint64_t f[FQ_MAX_WORDS] = {0, 0};
+      FQ_FOR_WORDS(w, W) f[w] = st_det[w * bcap + u];
+      const int64_t v = st_pop[u];
+      for (int q = 0; q < S; ++q) pay[q * scap + slot] = (q == sp) ? v : 0;
+      uint64_t k[2];
+      make_key(kp, f, W, static_cast<uint64_t>(e) >> 8, k);
+      for (int w = 0; w < kp.kw; ++w) keys[w * scap + slot] = k[w];
     }
-    const int64_t e = st_ent[u];
-    const int sp = static_cast<int>(e % s.S);
-    const int64_t i = e / s.S;
-    uint64_t f[FQ_MAX_WORDS] = {0, 0};
-    FQ_FOR_WORDS(w, s.W) f[w] = st_det[w * bcap + u];
-    const int64_t v = st_pop[u];
-    for (int q = 0; q < s.S; ++q) pay[q * scap + slot] = (q == sp) ? v : 0;
-    uint64_t k[2];
-    make_key(kp, f, s.W, s.initiator ? static_cast<uint64_t>(flags[i]) : 0, k);
-    for (int w = 0; w < kp.kw; ++w) keys[w * scap + slot] = k[w];
-  }
   }
 }
 
@@ -1209,20 +1217,33 @@ __device__ __forceinline__ uint32_t block_exclusive_u32(uint32_t x, uint32_t* ws
   return pre + inc - x;
 }
 
+// Length of the spawned list: the device count (clamped to the host bound n,
+// the capacity at most) when pn is given, else n itself.
+__device__ __forceinline__ int64_t list_len(const int64_t* __restrict__ pn, int64_t n) {
+  if (!pn) return n;
+  const int64_t v = *pn;
+  return v < n ? v : n;
+}
+
 // Radix pass, reduce-then-scan over a fixed grid: block b owns the sort tiles
-// [b * tpb, (b + 1) * tpb) (tiles of kBlock * sort_items keys).  k_pass_count writes the
+// [b * tpb, (b + 1) * tpb) (tiles of kBlock * sort_items keys; tpb from the
+// list length on the device and the grid, so the host need not know the
+// length, only a bound of it).  k_pass_count writes the
 // block's digit counts digit-major, cnt[d * nblk + b]; their exclusive scan
 // gives every block the global start of each digit's keys from its range, and
 // k_pass_scatter walks its tiles in order, ranks each tile's keys by digit in
 // input order (stable), and writes keys and payload (S population words) to
 // their sorted positions.  No block waits on another.
 template <int KW, int S, int DB>
-__global__ __launch_bounds__(kBlock) void k_pass_count(const uint64_t* __restrict__ keys, int64_t n, int64_t stride,
-                                                       int shift, int64_t tpb, int64_t* __restrict__ cnt) {
+__global__ __launch_bounds__(kBlock) void k_pass_count(const uint64_t* __restrict__ keys,
+                                                       const int64_t* __restrict__ pn, int64_t nb, int64_t stride,
+                                                       int shift, int64_t* __restrict__ cnt) {
   constexpr int NB = 1 << DB, R = sort_items(KW, S), T = kBlock * R;
   __shared__ uint32_t h[NB];
   for (int d = threadIdx.x; d < NB; d += kBlock) h[d] = 0;
   __syncthreads();
+  const int64_t n = list_len(pn, nb);
+  const int64_t tpb = ((n + T - 1) / T + gridDim.x - 1) / gridDim.x;
   const int lane = threadIdx.x & 63;
   const int64_t lo = static_cast<int64_t>(blockIdx.x) * tpb * T;
   const int64_t hi = lo + tpb * T < n ? lo + tpb * T : n;
@@ -1271,10 +1292,13 @@ __global__ __launch_bounds__(kSortBlocks) void k_pass_rowscan(int nb, int64_t* _
 template <int KW, int S, int DB>
 __global__ __launch_bounds__(kBlock) void k_pass_scatter(const uint64_t* __restrict__ kin,
                                                          const int64_t* __restrict__ pin, uint64_t* __restrict__ kout,
-                                                         int64_t* __restrict__ pout, int64_t n, int64_t stride,
-                                                         int shift, int64_t tpb, const int64_t* __restrict__ off,
+                                                         int64_t* __restrict__ pout, const int64_t* __restrict__ pn,
+                                                         int64_t nb, int64_t stride, int shift,
+                                                         const int64_t* __restrict__ off,
                                                          const int64_t* __restrict__ dtot) {
   constexpr int NB = 1 << DB, R = sort_items(KW, S), T = kBlock * R, NW = kBlock / 64;
+  const int64_t n = list_len(pn, nb);
+  const int64_t tpb = ((n + T - 1) / T + gridDim.x - 1) / gridDim.x;
   constexpr uint32_t MASK = NB - 1;
   __shared__ uint64_t sk[KW][T];
   __shared__ int64_t sp[S][T];
@@ -1534,18 +1558,56 @@ __device__ __forceinline__ void unrank(const KeyPlan& kp, const uint64_t* __rest
   }
 }
 
-// Run heads of each tile of kRunTile sorted keys (first key, or value
-// differing from the previous key): their count per tile.
-template <int KW>
-__global__ __launch_bounds__(kBlock) void k_run_count(const uint64_t* __restrict__ keys, int64_t ns, int64_t stride,
-                                                      int fb, int64_t* __restrict__ tcnt) {
-  constexpr int R = kRunTile / kBlock;
+// One sorted entry's share of its run's sums in annihilate_spawn_t_initiator
+// (src/spawn_data.F90:937-1101), per space: c[sp] the population; c[S + sp]
+// the population if it was spawned from an initiator (flag clear), else 0;
+// c[2S + sp] for a flagged (non-initiator) spawn its event count, the sign of
+// the population (0 for 0) -- except a run's first entry, which counts the
+// sign of its population's low 32 bits read as a default integer.  Every sum
+// is additive, so a run's totals may be formed in any grouping.
+template <int S>
+__device__ __forceinline__ void entry_share(int64_t flags, const int64_t* pop, bool head, int64_t* c) {
+#pragma unroll
+  for (int sp = 0; sp < S; ++sp) {
+    const int64_t v = pop[sp];
+    const bool fl = (flags >> sp) & 1;
+    c[sp] = v;
+    c[S + sp] = fl ? 0 : v;
+    int64_t ev = 0;
+    if (fl) {
+      if (head) {
+        const int32_t v32 = static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(v) & 0xffffffffULL));
+        ev = v32 >= 0 ? 1 : -1;
+      } else {
+        ev = v < 0 ? -1 : (v > 0 ? 1 : 0);
+      }
+    }
+    c[2 * S + sp] = ev;
+  }
+}
+
+// Per tile of kRunTile sorted keys: the number of run heads (first key, or
+// value differing from the previous key), the position of the first head (-1
+// if none), and the sums (entry_share) of the entries before it -- the part of
+// a run that began in an earlier tile.
+template <int KW, int S>
+__global__ __launch_bounds__(kBlock) void k_run_count(const uint64_t* __restrict__ keys,
+                                                      const int64_t* __restrict__ pay, const int64_t* __restrict__ pn,
+                                                      int64_t nsb, int64_t stride, int fb,
+                                                      int64_t* __restrict__ tcnt, int64_t* __restrict__ tfirst,
+                                                      int64_t* __restrict__ tpre) {
+  constexpr int R = kRunTile / kBlock, NV = 3 * S;
+  const int64_t ns = list_len(pn, nsb);
   __shared__ int64_t wsh[kBlock / 64];
-  const int64_t tile = blockIdx.x;
+  __shared__ int32_t wfirst[kBlock / 64];
+  const int64_t tile = blockIdx.x, t0 = tile * kRunTile;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int64_t c = 0;
+  int first_here = kRunTile;  // this wavefront's first head in the tile
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int64_t i = tile * kRunTile + r * kBlock + threadIdx.x;
+    const int64_t i = t0 + r * kBlock + threadIdx.x;
+    bool h = false;
     if (i < ns) {
       uint64_t k[KW], prev[KW];
 #pragma unroll
@@ -1553,148 +1615,269 @@ __global__ __launch_bounds__(kBlock) void k_run_count(const uint64_t* __restrict
         k[w] = keys[w * stride + i];
         prev[w] = i > 0 ? keys[w * stride + i - 1] : 0;
       }
-      c += (i == 0 || !same_value<KW>(k, prev, fb)) ? 1 : 0;
+      h = i == 0 || !same_value<KW>(k, prev, fb);
     }
+    c += h ? 1 : 0;
+    const unsigned long long b = __ballot(h);
+    if (b && first_here == kRunTile) first_here = r * kBlock + wave * 64 + __builtin_ctzll(b);
   }
-  const int64_t t = block_sum(c, wsh);
-  if (threadIdx.x == 0) tcnt[tile] = t;
+  if (lane == 0) wfirst[wave] = first_here;
+  const int64_t t = block_sum(c, wsh);  // its barriers also publish wfirst
+  int first = kRunTile;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) first = wfirst[w] < first ? wfirst[w] : first;
+  const uint64_t fmask = (uint64_t{1} << fb) - 1;
+  int64_t acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0;
+  if (first > 0) {  // block-uniform
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = r * kBlock + static_cast<int>(threadIdx.x);
+      const int64_t i = t0 + j;
+      if (j < first && i < ns) {
+        int64_t pop[S], cv[NV];
+#pragma unroll
+        for (int q = 0; q < S; ++q) pop[q] = pay[q * stride + i];
+        entry_share<S>(static_cast<int64_t>(keys[i] & fmask), pop, false, cv);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += cv[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = block_sum(acc[k], wsh);
+  }
+  if (threadIdx.x == 0) {
+    tcnt[tile] = t;
+    tfirst[tile] = first < kRunTile ? first : -1;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tpre[tile * NV + k] = acc[k];
+  }
+}
+
+// Segmented scan element: f = a run head lies in the span, v = the sums
+// since the last head in it (or over the whole span).  a precedes b.
+template <int NV>
+__device__ __forceinline__ void seg_combine(int fa, const int64_t* va, int& fb, int64_t* vb) {
+  if (!fb) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) vb[k] += va[k];
+  }
+  fb |= fa;
 }
 
 // annihilate_spawn_t(_initiator) (src/spawn_data.F90:859-1101) over the
-// sorted keys and populations, one block per tile of kRunTile keys: the
-// tile's run heads (first key, or value differing from the previous key) are
-// listed in order in LDS by wavefront ballots, from the tile's run offset
-// (toff, the exclusive scan of k_run_count); then thread t reduces heads t,
-// t + kBlock, ... : the run (which may extend past the tile) from the
-// contiguous sorted populations and the flag bits of the keys, its
-// determinant from its key, written at the run's index.  The run's first
-// entry is its first in sorted order: the reference's quicksort order in
-// replay mode, the earliest spawned in production (the stable sort of an
+// sorted keys and populations, one block per tile of kRunTile keys.  The tile
+// is staged in LDS; thread t takes entries [R t, R t + R), forms each entry's
+// share (entry_share), and a segmented scan over the block's threads gives
+// every run ending in the tile its totals whatever its length (a run no
+// longer falls to one thread).  A run is written by the thread holding its
+// last entry, if its head lies in the tile; the tile's last run adds the
+// partial sums of the following tiles (k_run_count) up to the next head.  Run
+// index = the tile's run offset (toff) + the head's rank in the tile; the
+// determinant is recovered from the key by unranking.  The run's first entry
+// is its first in sorted order: the reference's quicksort order in replay
+// mode, the earliest spawned in production (the stable sort of an
 // attempt-ordered list).
-template <int KW, bool WIDE>
+template <int KW, int S, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_runs(Sys s, KeyPlan kp, const uint64_t* __restrict__ keys,
                                                  const int64_t* __restrict__ pay, int64_t ns, int64_t stride,
-                                                 const int64_t* __restrict__ toff, uint64_t* __restrict__ rdet,
-                                                 int64_t* __restrict__ rpop, int64_t* __restrict__ rflag,
-                                                 int64_t* __restrict__ rstate) {
-  constexpr int R = kRunTile / kBlock;
-  __shared__ int32_t wcount[R][kBlock / 64];
-  __shared__ int32_t hpos[kRunTile];  // tile-relative index of each head, in order
+                                                 const int64_t* __restrict__ toff, const int64_t* __restrict__ tfirst,
+                                                 const int64_t* __restrict__ tpre, int64_t ntiles,
+                                                 uint64_t* __restrict__ rdet, int64_t* __restrict__ rpop,
+                                                 int64_t* __restrict__ rflag, int64_t* __restrict__ rstate) {
+  constexpr int R = kRunTile / kBlock, NV = 3 * S, NW = kBlock / 64;
+  __shared__ uint64_t sk[KW][kRunTile];
+  __shared__ int64_t spop[S][kRunTile];
+  __shared__ uint64_t rkey[KW][kRunTile];  // each run's key, by its index in the tile
+  __shared__ uint64_t prevk[KW];
+  __shared__ int64_t wv[NW][NV];
+  __shared__ int32_t wf[NW];
+  __shared__ uint32_t wsh[NW];
   extern __shared__ uint64_t binom_lds[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t tile = blockIdx.x, t0 = tile * kRunTile;
+  if (t0 >= ns) return;  // block-uniform
   const bool staged = kp.rank && kp.binom_lds;
   if (staged) {
     const int nw = 2 * kp.nbasis * kp.nel;
     for (int k = tid; k < nw; k += kBlock) binom_lds[k] = kp.binom[k];
   }
-  const int64_t tile = blockIdx.x, t0 = tile * kRunTile;
+  const int n = static_cast<int>(ns - t0 < kRunTile ? ns - t0 : kRunTile);
   const int fb = kp.flag_bits;
-  bool head[R];
+  const uint64_t fmask = (uint64_t{1} << fb) - 1;
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int64_t i = t0 + r * kBlock + tid;
-    head[r] = false;
-    if (i < ns) {
-      uint64_t k[KW], prev[KW];
+  for (int r = 0; r < R; ++r) {  // coalesced staging
+    const int j = r * kBlock + tid;
+    if (j < n) {
+#pragma unroll
+      for (int w = 0; w < KW; ++w) sk[w][j] = keys[w * stride + t0 + j];
+#pragma unroll
+      for (int q = 0; q < S; ++q) spop[q][j] = pay[q * stride + t0 + j];
+    }
+  }
+  if (tid < KW) prevk[tid] = t0 > 0 ? keys[tid * stride + t0 - 1] : 0;
+  __syncthreads();  // also orders the table's staging
+  // this thread's entries: head flags, shares, head count, sums since its last head
+  bool hd[R];
+  int64_t cv[R][NV], agg[NV];
+  int hc = 0, any = 0;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) agg[k] = 0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int j = tid * R + q;
+    hd[q] = false;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) cv[q][k] = 0;
+    if (j < n) {
+      uint64_t k0[KW], kp0[KW];
 #pragma unroll
       for (int w = 0; w < KW; ++w) {
-        k[w] = keys[w * stride + i];
-        prev[w] = i > 0 ? keys[w * stride + i - 1] : 0;
+        k0[w] = sk[w][j];
+        kp0[w] = j > 0 ? sk[w][j - 1] : prevk[w];
       }
-      head[r] = i == 0 || !same_value<KW>(k, prev, fb);
+      hd[q] = (t0 + j == 0) || !same_value<KW>(k0, kp0, fb);
+      int64_t pop[S];
+#pragma unroll
+      for (int sp = 0; sp < S; ++sp) pop[sp] = spop[sp][j];
+      entry_share<S>(static_cast<int64_t>(k0[0] & fmask), pop, hd[q], cv[q]);
+      if (hd[q]) {
+        ++hc;
+        any = 1;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) agg[k] = cv[q][k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) agg[k] += cv[q][k];
+      }
     }
   }
-  unsigned long long bal[R];
+  // segmented inclusive scan over the wavefront, then over the waves
+  int f = any;
+  int64_t v[NV];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    bal[r] = __ballot(head[r]);
-    if (lane == 0) wcount[r][wave] = __popcll(bal[r]);
+  for (int k = 0; k < NV; ++k) v[k] = agg[k];
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int fo = __shfl_up(f, d, 64);
+    int64_t vo[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) vo[k] = __shfl_up(v[k], d, 64);
+    if (lane >= d) seg_combine<NV>(fo, vo, f, v);
   }
-  __syncthreads();
-  int32_t total = 0;
+  if (lane == 63) {
+    wf[wave] = f;
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    int32_t p = total + __popcll(bal[r] & ((1ull << lane) - 1));
-    for (int w = 0; w < kBlock / 64; ++w) {
-      if (w < wave) p += wcount[r][w];
-      total += wcount[r][w];
+    for (int k = 0; k < NV; ++k) wv[wave][k] = v[k];
+  }
+  // exclusive within the wave: the inclusive value of the lane before
+  int ef = __shfl_up(f, 1, 64);
+  int64_t ev[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) ev[k] = __shfl_up(v[k], 1, 64);
+  if (lane == 0) {
+    ef = 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) ev[k] = 0;
+  }
+  uint32_t htot;
+  const uint32_t hbase = block_exclusive_u32(static_cast<uint32_t>(hc), wsh, htot);  // barriers publish wf, wv
+  {
+    int pf = 0;  // the waves before this one, folded in order
+    int64_t pv[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) pv[k] = 0;
+    for (int u = 0; u < wave; ++u) {
+      int64_t tv[NV];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) tv[k] = wv[u][k];
+      int tf = wf[u];
+      seg_combine<NV>(pf, pv, tf, tv);
+      pf = tf;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) pv[k] = tv[k];
     }
-    if (head[r]) hpos[p] = r * kBlock + tid;
+    seg_combine<NV>(pf, pv, ef, ev);  // ef, ev: everything in the tile before this thread's entries
   }
-  __syncthreads();  // also orders the table's staging
+  // walk the entries again: a run ends where the next entry heads another
   const int64_t rbase = toff[tile];
-  const uint64_t fmask = (uint64_t{1} << fb) - 1;
-  for (int h = tid; h < total; h += kBlock) {
-    const int64_t i = t0 + hpos[h];
+  int owned = ef;  // the current run's head lies in this tile
+  int64_t run_sum[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) run_sum[k] = ev[k];
+  int64_t hr = static_cast<int64_t>(hbase) - 1;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int j = tid * R + q;
+    if (j >= n) break;
+    if (hd[q]) {
+      owned = 1;
+      ++hr;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) run_sum[k] = cv[q][k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) run_sum[k] += cv[q][k];
+    }
     uint64_t kk[KW];
 #pragma unroll
-    for (int w = 0; w < KW; ++w) kk[w] = keys[w * stride + i];
-    int64_t end;
-    if (h + 1 < total) {
-      end = t0 + hpos[h + 1];
-    } else {  // the tile's last run may continue into the next tiles
-      end = t0 + kRunTile < ns ? t0 + kRunTile : ns;
-      for (; end < ns; ++end) {
-        uint64_t kj[KW];
+    for (int w = 0; w < KW; ++w) kk[w] = sk[w][j];
+    bool end;
+    if (j + 1 < n) {
+      uint64_t kn[KW];
 #pragma unroll
-        for (int w = 0; w < KW; ++w) kj[w] = keys[w * stride + end];
-        if (!same_value<KW>(kj, kk, fb)) break;
+      for (int w = 0; w < KW; ++w) kn[w] = sk[w][j + 1];
+      end = !same_value<KW>(kn, kk, fb);
+    } else {
+      end = true;
+      if (t0 + n < ns && owned && tfirst[tile + 1] != 0) {
+        // the tile's last run continues: add the following tiles' parts up to the next head
+        for (int64_t u = tile + 1; u < ntiles; ++u) {
+#pragma unroll
+          for (int k = 0; k < NV; ++k) run_sum[k] += tpre[u * NV + k];
+          if (tfirst[u] >= 0) break;
+        }
       }
     }
-    int64_t tot[FQ_MAX_SPACES] = {0, 0}, init[FQ_MAX_SPACES] = {0, 0};
-    int events[FQ_MAX_SPACES] = {0, 0};
-    const int64_t f0 = static_cast<int64_t>(kk[0] & fmask);
-    for (int sp = 0; sp < s.S; ++sp) {
-      const int64_t v = pay[sp * stride + i];
-      tot[sp] = v;
-      if (!((f0 >> sp) & 1)) {
-        init[sp] = v;
-      } else {  // the first entry's event count: sign of the population as a default integer
-        const int32_t v32 = static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(v) & 0xffffffffULL));
-        events[sp] = v32 >= 0 ? 1 : -1;
-      }
-    }
-    for (int64_t j = i + 1; j < end; ++j) {
-      const int64_t fk = static_cast<int64_t>(keys[j] & fmask);  // flags sit in word 0
-      for (int sp = 0; sp < s.S; ++sp) {
-        const int64_t v = pay[sp * stride + j];
-        if (!((fk >> sp) & 1))
-          init[sp] += v;
-        else if (v < 0)
-          events[sp] -= 1;
-        else if (v > 0)
-          events[sp] += 1;
-        tot[sp] += v;
-      }
-    }
-    const int64_t run = rbase + h;
+    if (!end || !owned) continue;
+    const int64_t run = rbase + hr;
     int64_t fl = 0;
-    bool any = false;
-    for (int sp = 0; sp < s.S; ++sp) {
-      any = any || tot[sp] != 0;
+    bool nz = false;
+#pragma unroll
+    for (int sp = 0; sp < S; ++sp) {
+      const int64_t tot = run_sum[sp], init = run_sum[S + sp], evs = run_sum[2 * S + sp];
+      nz = nz || tot != 0;
       if (s.initiator) {
-        const int st = tot[sp] >= 0 ? 1 : -1, si = init[sp] >= 0 ? 1 : -1;
-        if (init[sp] != 0 && st == si) {
-        } else if (abs(events[sp]) > 1) {
+        const int st = tot >= 0 ? 1 : -1, si = init >= 0 ? 1 : -1;
+        if (init != 0 && st == si) {
+        } else if (evs > 1 || evs < -1) {
         } else {
           fl += int64_t{1} << sp;
         }
       }
-      rpop[sp * stride + run] = tot[sp];
+      rpop[sp * stride + run] = tot;
     }
-    uint64_t d[2] = {0, 0};
-    const unsigned __int128 v = (KW == 1 ? static_cast<unsigned __int128>(kk[0])
-                                         : (static_cast<unsigned __int128>(kk[KW - 1]) << 64) | kk[0]) >> fb;
-    if (!kp.rank) {
-      d[0] = static_cast<uint64_t>(v);  // batched instances: the encoded word itself
-    } else if (staged) {
-      unrank<WIDE>(kp, binom_lds, v, d);
-    } else {
-      unrank<WIDE>(kp, kp.binom, v, d);
-    }
-    FQ_FOR_WORDS(w, s.W) rdet[w * stride + run] = d[w];
+#pragma unroll
+    for (int w = 0; w < KW; ++w) rkey[w][hr] = kk[w];
     rflag[run] = fl;
-    rstate[run] = any ? 1 : 0;
+    rstate[run] = nz ? 1 : 0;
+  }
+  // the runs' determinants, every lane busy: unranked from their keys
+  __syncthreads();
+  for (int h = tid; h < static_cast<int>(htot); h += kBlock) {
+    uint64_t d[2] = {0, 0};
+    const unsigned __int128 val = (KW == 1 ? static_cast<unsigned __int128>(rkey[0][h])
+                                           : (static_cast<unsigned __int128>(rkey[KW - 1][h]) << 64) | rkey[0][h]) >>
+                                  fb;
+    if (!kp.rank) {
+      d[0] = static_cast<uint64_t>(val);  // batched instances: the encoded word itself
+    } else if (staged) {
+      unrank<WIDE>(kp, binom_lds, val, d);
+    } else {
+      unrank<WIDE>(kp, kp.binom, val, d);
+    }
+    FQ_FOR_WORDS(w, s.W) rdet[w * stride + rbase + h] = d[w];
   }
 }
 
@@ -2227,6 +2410,8 @@ int setup_sort(fq_engine* e) {
   if (!e->pass_off) rc |= dalloc(&e->pass_off, 512 * kSortBlocks + 1);
   if (!e->run_tcnt) rc |= dalloc(&e->run_tcnt, ceil_div(scap, kRunTile) + 2);
   if (!e->run_toff) rc |= dalloc(&e->run_toff, ceil_div(scap, kRunTile) + 2);
+  if (!e->run_tfirst) rc |= dalloc(&e->run_tfirst, ceil_div(scap, kRunTile) + 2);
+  if (!e->run_tpre) rc |= dalloc(&e->run_tpre, 3 * e->S * (ceil_div(scap, kRunTile) + 2));
   return rc == FQ_OK ? FQ_OK : FQ_ERR_NOMEM;
 }
 
@@ -2238,29 +2423,30 @@ void launch_make_keys(fq_engine* e, int64_t ns, const uint32_t* perm) {
 }
 
 // One radix pass from buffer `src` to the other.
+// ns: the list length, or a bound of it when pns (the device count) is given.
 template <int KW, int S, int DB>
-int sort_pass(fq_engine* e, int src, int64_t ns, int shift) {
+int sort_pass(fq_engine* e, int src, int64_t ns, const int64_t* pns, int shift) {
   constexpr int T = kBlock * sort_items(KW, S);
-  const int64_t ntiles = ceil_div(ns, T);
-  const int64_t nblk = std::min<int64_t>(ntiles, kSortBlocks), tpb = ceil_div(ntiles, nblk);
-  const int64_t nb = ceil_div(ntiles, tpb);  // blocks that own tiles
+  const int64_t nb = std::min<int64_t>(ceil_div(ns, T), kSortBlocks);
   hipLaunchKernelGGL((k_pass_count<KW, S, DB>), dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, e->stream,
-                     e->kbuf[src], ns, e->scap, shift, tpb, e->pass_cnt);
+                     e->kbuf[src], pns, ns, e->scap, shift, e->pass_cnt);
   hipLaunchKernelGGL(k_pass_rowscan, dim3(1u << DB), dim3(kSortBlocks), 0, e->stream, static_cast<int>(nb),
                      e->pass_cnt, e->pass_off);
   hipLaunchKernelGGL((k_pass_scatter<KW, S, DB>), dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, e->stream,
-                     e->kbuf[src], e->pbuf[src], e->kbuf[1 - src], e->pbuf[1 - src], ns, e->scap, shift, tpb,
+                     e->kbuf[src], e->pbuf[src], e->kbuf[1 - src], e->pbuf[1 - src], pns, ns, e->scap, shift,
                      e->pass_cnt, e->pass_off);
   return FQ_OK;
 }
 template <int KW, int S>
-int sort_pass_db(fq_engine* e, int src, int64_t ns, int shift, int db) {
-  return db == 9 ? sort_pass<KW, S, 9>(e, src, ns, shift) : sort_pass<KW, S, 8>(e, src, ns, shift);
+int sort_pass_db(fq_engine* e, int src, int64_t ns, const int64_t* pns, int shift, int db) {
+  return db == 9 ? sort_pass<KW, S, 9>(e, src, ns, pns, shift) : sort_pass<KW, S, 8>(e, src, ns, pns, shift);
 }
 
-// Sort the spawned list (ns entries) and reduce runs; leaves the device
-// address of the run count in p_nruns and a host bound of it in run_bound.
-int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
+// Sort the spawned list and reduce runs; leaves the device address of the run
+// count in p_nruns and a host bound of it in run_bound.  The list holds ns
+// entries, or, when pns is given (production, no host wait for the length),
+// *pns entries with ns a bound of them.
+int sort_and_reduce(fq_engine* e, int64_t ns, const int64_t* pns, bool replay) {
   const bool keys_ready = e->keys_ready;  // made by the spawning pass for this spawned list
   e->keys_ready = false;
   const int64_t scap = e->scap;
@@ -2279,31 +2465,44 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
       for (int p = 0; p < dp.np; ++p) {
         int rc;
         if (kw == 1)
-          rc = S == 1 ? sort_pass_db<1, 1>(e, src, ns, dp.shift[p], dp.bits[p])
-                      : sort_pass_db<1, 2>(e, src, ns, dp.shift[p], dp.bits[p]);
+          rc = S == 1 ? sort_pass_db<1, 1>(e, src, ns, pns, dp.shift[p], dp.bits[p])
+                      : sort_pass_db<1, 2>(e, src, ns, pns, dp.shift[p], dp.bits[p]);
         else
-          rc = S == 1 ? sort_pass_db<2, 1>(e, src, ns, dp.shift[p], dp.bits[p])
-                      : sort_pass_db<2, 2>(e, src, ns, dp.shift[p], dp.bits[p]);
+          rc = S == 1 ? sort_pass_db<2, 1>(e, src, ns, pns, dp.shift[p], dp.bits[p])
+                      : sort_pass_db<2, 2>(e, src, ns, pns, dp.shift[p], dp.bits[p]);
         if (rc != FQ_OK) return rc;
         src = 1 - src;
         ++e->radix_passes;
       }
     }
   }
+  if (pns) {  // the passes are queued: now the exact length (copied before them)
+    FQ_HIP(hipEventSynchronize(e->ev_len));
+    // the attempts cannot exceed the bound the spawning launches covered (every
+    // entry makes at most floor(|N|) + 1 of them, |N| for integer populations)
+    if (e->h_len[2] > e->spawn_bound) return FQ_ERR_DEVICE;
+    ns = std::min<int64_t>(e->h_len[0], e->scap);
+    pns = nullptr;
+  }
   FQ_HIP(hipEventRecord(e->pev[5], st));
   const int64_t nt = ceil_div(ns, kRunTile);
   int64_t* nruns = e->run_toff + nt;
   if (ns > 0) {
-    hipLaunchKernelGGL(kw == 1 ? k_run_count<1> : k_run_count<2>, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0,
-                       st, e->kbuf[src], ns, scap, e->flag_bits, e->run_tcnt);
+    auto cnt = kw == 1 ? (S == 1 ? k_run_count<1, 1> : k_run_count<1, 2>)
+                       : (S == 1 ? k_run_count<2, 1> : k_run_count<2, 2>);
+    hipLaunchKernelGGL(cnt, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0, st, e->kbuf[src], e->pbuf[src],
+                       nullptr, ns, scap, e->flag_bits, e->run_tcnt, e->run_tfirst, e->run_tpre);
     const int rc = scan_exclusive(e, e->run_tcnt, e->run_toff, nt);
     if (rc != FQ_OK) return rc;
     KeyPlan kp = key_plan(e);
     const size_t tab = binom_bytes(e);
     kp.binom_lds = tab > 0 && tab <= kBinomLdsMax;
-    auto kern = kw == 1 ? k_runs<1, false> : k_runs<2, true>;  // one-word keys: 64-bit rank arithmetic
+    // one-word keys: 64-bit rank arithmetic
+    auto kern = kw == 1 ? (S == 1 ? k_runs<1, 1, false> : k_runs<1, 2, false>)
+                        : (S == 1 ? k_runs<2, 1, true> : k_runs<2, 2, true>);
     hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nt)), dim3(kBlock), kp.binom_lds ? tab : 0, st, e->sys, kp,
-                       e->kbuf[src], e->pbuf[src], ns, scap, e->run_toff, e->rdet, e->rpop, e->rflag, e->rstate);
+                       e->kbuf[src], e->pbuf[src], ns, scap, e->run_toff, e->run_tfirst, e->run_tpre, nt, e->rdet,
+                       e->rpop, e->rflag, e->rstate);
   }
   // The number of runs stays on the device; kernels size their grids by a
   // host bound: ns in production, the exact count in replay (read there
@@ -2325,8 +2524,8 @@ int sort_and_reduce(fq_engine* e, int64_t ns, bool replay) {
 
 // Annihilation proper: merge runs, round, compact, insert.  ns spawned
 // entries are in the spawned list; pop_new holds the post-death populations.
-int annihilate(fq_engine* e, int64_t ns, bool replay, uint32_t cyc, int64_t& rng_pos) {
-  int rc = sort_and_reduce(e, ns, replay);
+int annihilate(fq_engine* e, int64_t ns, const int64_t* pns, bool replay, uint32_t cyc, int64_t& rng_pos) {
+  int rc = sort_and_reduce(e, ns, pns, replay);
   if (rc != FQ_OK) return rc;
   const int64_t nruns = e->run_bound;  // a bound in production, the count in replay
   const int64_t* pnr = e->p_nruns;
@@ -2713,12 +2912,13 @@ int pack_spawned(fq_engine* e, int64_t nbt, int64_t t_begin, const int64_t* ptot
   const int64_t nt = ceil_div(nbt, kSpawnTile);
   int rc = scan_exclusive(e, e->tile_cnt, e->tile_off, nt);
   if (rc != FQ_OK) return rc;
+  (void)t_begin;
   KeyPlan kp = key_plan(e);
   const size_t tab = binom_bytes(e);
   kp.binom_lds = tab > 0 && tab <= kBinomLdsMax;
   hipLaunchKernelGGL(k_pack_tiles, dim3(static_cast<unsigned>(std::min<int64_t>(nt, 2048))), dim3(kBlock),
-                     kp.binom_lds ? tab : 0, e->stream, e->sys, nt, e->tile_cnt, e->tile_off, e->st_det, e->st_pop,
-                     e->st_ent, e->bcap, t_begin, e->flags, e->scap, kp, e->kbuf[0], e->pbuf[0], e->d_stats);
+                     kp.binom_lds ? tab : 0, e->stream, e->W, e->S, nt, e->tile_cnt, e->tile_off, e->st_det,
+                     e->st_pop, e->st_ent, e->bcap, e->scap, kp, e->kbuf[0], e->pbuf[0], e->d_stats);
   hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, e->stream, e->tile_off, nt, ptot, e->d_stats);
   return FQ_OK;
 }
@@ -2933,6 +3133,9 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
     if (hipEventCreate(&e->pev[k]) != hipSuccess) rc = FQ_ERR_DEVICE;
   for (int k = 0; k < 2 * fq_engine::kSpawnEvents && rc == FQ_OK; ++k)
     if (hipEventCreate(&e->sev[k]) != hipSuccess) rc = FQ_ERR_DEVICE;
+  if (rc == FQ_OK && hipEventCreateWithFlags(&e->ev_len, hipEventDisableTiming) != hipSuccess) rc = FQ_ERR_DEVICE;
+  if (rc == FQ_OK && hipHostMalloc(reinterpret_cast<void**>(&e->h_len), 4 * sizeof(int64_t)) != hipSuccess)
+    rc = FQ_ERR_NOMEM;
   if (rc != FQ_OK) {
     fq_destroy(e);
     return rc;
@@ -3084,7 +3287,7 @@ static int cycle_body(fq_engine* e, const double* shift, const double* eproj_old
                                                        : 0;
         hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), lds, st, e->d_sysp, nent, cap, tb, te, e->att_off,
                            e->det[c], e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop,
-                           e->st_ent, e->tile_cnt, e->tile_win, nullptr, nullptr, nullptr);
+                           e->st_ent, e->tile_cnt, e->tile_win, e->flags, nullptr, nullptr, nullptr);
         if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
         const int prc = pack_spawned(e, nbt, tb, e->att_off + nent);
         if (prc != FQ_OK) return prc;
@@ -3096,23 +3299,22 @@ static int cycle_body(fq_engine* e, const double* shift, const double* eproj_old
     if (rc != FQ_OK) return rc;
     FQ_HIP(hipEventRecord(e->pev[3], st));
     e->keys_ready = true;  // the packing kernels wrote the sort keys
-    rc = sync_stats(e);
-    if (rc != FQ_OK) return rc;
-    if (const int64_t total = e->h_stats.total_attempts; total > bound) {  // the bound cannot be exceeded
-      rc = spawn_range(bound, total);
-      if (rc == FQ_OK) rc = sync_stats(e);
-      if (rc != FQ_OK) return rc;
-    }
-    ns = std::min<int64_t>(e->h_stats.nspawn, e->scap);
+    // The spawned-list length and the attempt total are copied to pinned host
+    // memory behind the spawning kernels; the radix passes read the length on
+    // the device (their grids sized by a bound: successes <= attempts <=
+    // bound), so the host waits for the copy only once they are queued, and
+    // the device does not idle meanwhile.
+    FQ_HIP(hipMemcpyAsync(e->h_len, &e->d_stats->nspawn, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    FQ_HIP(hipEventRecord(e->ev_len, st));
+    e->spawn_bound = bound;
+    ns = std::min<int64_t>(bound, e->scap);
   }
   FQ_HIP(hipEventRecord(e->ev[1], st));
-  const Stats keep = e->h_stats;
-  int rc = annihilate(e, ns, replay, cyc, rng_pos);
+  int rc = annihilate(e, ns, replay ? nullptr : &e->d_stats->nspawn, replay, cyc, rng_pos);
   if (rc != FQ_OK) return rc;
   FQ_HIP(hipEventRecord(e->ev[2], st));
   FQ_HIP(hipEventSynchronize(e->ev[2]));
-  // restore spawn-phase quantities the annihilation's stat resets do not touch
-  e->h_stats.total_attempts = keep.total_attempts;
+  ns = std::min<int64_t>(e->h_stats.nspawn, e->scap);
   if (!replay) {  // next cycle's attempts <= sum floor(|N|) (+1 per entry with real amplitudes)
     unsigned long long sum = 0;
     for (int sp = 0; sp < e->S; ++sp) sum += e->h_stats.abs_pop[sp];
@@ -3243,7 +3445,7 @@ int fq_cycle_finish(fq_engine* e, const uint64_t* sstates, const int64_t* spops,
   if (ns > 0) FQ_HIP(hipMemcpyAsync(e->srec, rec.data(), 8 * RS * ns, hipMemcpyHostToDevice, st));
   e->pending = false;
   int64_t rng_pos = e->pend_rng_pos;
-  int rc = annihilate(e, ns, true, e->pend_cyc, rng_pos);
+  int rc = annihilate(e, ns, nullptr, true, e->pend_cyc, rng_pos);
   if (rc != FQ_OK) return rc;
   const int64_t used = rng_pos - e->rng_pos;
   e->rng_pos = rng_pos;
@@ -3305,7 +3507,7 @@ int fq_annihilate(fq_engine* e, const uint64_t* sstates, const int64_t* spops, c
   FQ_HIP(hipStreamSynchronize(st));
   const uint32_t cyc = static_cast<uint32_t>(e->cycle_index++);
   int64_t rng_pos = e->rng_pos;
-  int rc = annihilate(e, ns, replay, cyc, rng_pos);
+  int rc = annihilate(e, ns, nullptr, replay, cyc, rng_pos);
   if (rc != FQ_OK) return rc;
   const int64_t used = rng_pos - e->rng_pos;
   e->rng_pos = rng_pos;
@@ -3490,7 +3692,7 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
     auto kern = e->sys.system == FQ_SYS_HUB_K ? k_spawn<fqd::kGenHubK, true> : k_spawn<fqd::kGenMolRenorm, true>;
     hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->d_sysp, nent, cap, tb, te, e->att_off, e->det[c],
                        e->pop[c], 0.0, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
-                       e->tile_cnt, e->tile_win, e->d_sys_arr, e->d_iatt, e->d_ipar);
+                       e->tile_cnt, e->tile_win, e->flags, e->d_sys_arr, e->d_iatt, e->d_ipar);
     if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
     rc = pack_spawned(e, nbt, tb, e->att_off + nent);
     if (rc != FQ_OK) return rc;
@@ -3505,7 +3707,7 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
                      kInstShift + e->flag_bits, e->d_isb);
   const Stats keep = e->h_stats;
   int64_t rng_pos = e->rng_pos;
-  rc = annihilate(e, ns, false, cyc, rng_pos);
+  rc = annihilate(e, ns, nullptr, false, cyc, rng_pos);
   if (rc != FQ_OK) return rc;
   const int o = e->cur;
   hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[o], e->n, ninst, kInstShift,
@@ -3775,6 +3977,8 @@ void fq_destroy(fq_engine* e) {
   dfree(e->pass_off);
   dfree(e->run_tcnt);
   dfree(e->run_toff);
+  dfree(e->run_tfirst);
+  dfree(e->run_tpre);
   dfree(e->rdet);
   dfree(e->rpop);
   dfree(e->rflag);
@@ -3840,6 +4044,8 @@ void fq_destroy(fq_engine* e) {
     if (e->pev[k]) (void)hipEventDestroy(e->pev[k]);
   for (int k = 0; k < 2 * fq_engine::kSpawnEvents; ++k)
     if (e->sev[k]) (void)hipEventDestroy(e->sev[k]);
+  if (e->ev_len) (void)hipEventDestroy(e->ev_len);
+  if (e->h_len) (void)hipHostFree(e->h_len);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }

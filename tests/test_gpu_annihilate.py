@@ -117,6 +117,28 @@ def test_annihilate_production_matches_stable_oracle(hub, initiator, seed):
     assert _run(lib, orc, hub, st, p, main, pops, sk, sp, sf, stable=True) > 0
 
 
+@pytest.mark.parametrize("mode", ["philox", "replay"])
+def test_annihilate_long_runs_across_tiles(hub, mode):
+    """Runs far longer than a reduction tile (1024 sorted entries): keys spawned
+    1023, 1024, 1025, 2500 and 7000 times, beside single spawns, with random
+    initiator flags and signs (first-entry event quirk included): each run's
+    totals cross tile boundaries, and a tile may hold no run head at all."""
+    print(f"[annihilate] long runs, {mode}", flush=True)
+    lib, orc = N.lib(), oracle()
+    rng = np.random.default_rng(77)
+    st, p = _mk(hub, True, True, mode)
+    p.spawn_capacity = 1 << 16
+    rf = 1 << p.pop_bit_shift
+    main, pops, sk, _, _ = _random_state(rng, hub, 300, 2000, rf, True, big=True)
+    heavy = rng.choice(np.unique(sk), 5, replace=False)
+    reps = np.repeat(heavy, [1023, 1024, 1025, 2500, 7000])
+    sk = rng.permutation(np.concatenate([sk, reps]))
+    ns = len(sk)
+    sp = (rng.integers(1, 4, ns) * rng.choice([-1, 1], ns) * (3 * rf // 2)).astype(np.int64)
+    sf = rng.integers(0, 2, ns).astype(np.int64)
+    assert _run(lib, orc, hub, st, p, main, pops, sk, sp, sf, stable=(mode == "philox")) > 0
+
+
 def test_annihilate_molecular_new_kii(tmp_path):
     """New determinants get K_ii = <D|H|D> - H00 from the device diagonal rule."""
     from helpers import deck
