@@ -82,6 +82,8 @@ class FqhQmcIn(C.Structure):
         ("blocking_on_the_fly", i32), ("auto_shift_damping", i32), ("shift_damping_precision", f64),
         ("force_shift_damping_opt", i32), ("start_save_frequency", i32), ("start_point_number", i32),
         ("start_point", i32), ("error_limit", f64), ("min_blocks_used", i32), ("blocks_used", i32),
+        ("semi_stoch_size", i32), ("semi_stoch_start_iter", i32), ("semi_stoch_shift_iter", i32),
+        ("semi_stoch_separate", i32),
     ]
 
 

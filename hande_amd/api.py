@@ -108,11 +108,13 @@ BLOCKING_DEFAULTS = {  # blocking{} (src/qmc_data.f90:425-460)
 }
 
 
-def qmc_in(qmc, fciqmc_opts=None, rng_mode="replay", device=0, blocking=None):
-    """Build the qmc/fciqmc/blocking option block with the reference's defaults."""
+def qmc_in(qmc, fciqmc_opts=None, rng_mode="replay", device=0, blocking=None, semi_stoch=None):
+    """Build the qmc/fciqmc/blocking/semi_stoch option block with the
+    reference's defaults."""
     q = dict(qmc)
     f = dict(fciqmc_opts or {})
     b = dict(blocking or {})
+    ss = dict(semi_stoch or {})
     unknown = set(b) - set(BLOCKING_DEFAULTS) - {"filename"}
     if unknown:
         raise ValueError(f"unknown blocking options: {sorted(unknown)}")
@@ -160,6 +162,29 @@ def qmc_in(qmc, fciqmc_opts=None, rng_mode="replay", device=0, blocking=None):
     st.error_limit = float(b["error_limit"])
     st.min_blocks_used = int(b["min_blocks_used"])
     st.blocks_used = int(b["blocks_used"])
+    # semi_stoch{} (src/lua_hande_calc.f90:1620-1740): space 'high', separate
+    # (the default) or combined annihilation; real amplitudes are switched on
+    # as the reference does; shift_start_iteration replaces start_iteration
+    unknown = set(ss) - {"space", "size", "start_iteration", "shift_start_iteration", "separate_annihilation"}
+    if unknown:
+        raise ValueError(f"unsupported semi_stoch options: {sorted(unknown)}")
+    if ss:
+        if ss.get("space") != "high":
+            raise ValueError("semi_stoch: only space = 'high' is implemented")
+        if "size" not in ss:
+            raise ValueError("semi_stoch: target space size not given")
+        st.real_amplitudes = 1
+        st.semi_stoch_size = int(ss["size"])
+        st.semi_stoch_separate = 1 if ss.get("separate_annihilation", True) else 0
+        if "shift_start_iteration" in ss:
+            st.semi_stoch_start_iter = 2147483647
+            st.semi_stoch_shift_iter = int(ss["shift_start_iteration"])
+        else:
+            st.semi_stoch_start_iter = int(ss.get("start_iteration", 1))
+            st.semi_stoch_shift_iter = -1
+    else:
+        st.semi_stoch_size, st.semi_stoch_start_iter, st.semi_stoch_shift_iter = 0, 1, -1
+        st.semi_stoch_separate = 1
     return st
 
 

@@ -69,6 +69,10 @@ extern "C" void fqh_qmc_in_defaults(fqh_qmc_in* in) {
   in->error_limit = 0.0;
   in->min_blocks_used = 10;
   in->blocks_used = 2147483647;
+  in->semi_stoch_size = 0;
+  in->semi_stoch_start_iter = 1;
+  in->semi_stoch_shift_iter = -1;
+  in->semi_stoch_separate = 1;
 }
 
 extern "C" int fqh_fciqmc(const fqh_system* hs, const fqh_qmc_in* in, fqh_report_row* rows, int max_rows,

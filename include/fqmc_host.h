@@ -87,6 +87,17 @@ typedef struct fqh_qmc_in {
   double error_limit;
   int32_t min_blocks_used;
   int32_t blocks_used;
+  /* semi_stoch{} (src/qmc_data.f90:304-330), space "high": the
+   * semi_stoch_size most populated determinants become the deterministic core
+   * at iteration semi_stoch_start_iter (or semi_stoch_shift_iter iterations
+   * after the shift starts varying, if not -1); size 0: off.
+   * semi_stoch_separate: 1 = separate annihilation (the reference's default:
+   * the projection is added to the core populations directly), 0 = combined
+   * (its result joins the spawned list). */
+  int32_t semi_stoch_size;
+  int32_t semi_stoch_start_iter;
+  int32_t semi_stoch_shift_iter;
+  int32_t semi_stoch_separate;
 } fqh_qmc_in;
 
 /* Fill the blocking{} fields with the reference's defaults (on-the-fly

@@ -132,7 +132,14 @@ struct PhiloxStream {
     return static_cast<double>(v) * (1.0 / 4503599627370496.0);
   }
 };
-enum : uint32_t { kDrawAttempt = 1, kDrawNattempts = 2, kDrawDeath = 3, kDrawRoundMain = 4, kDrawRoundSpawn = 5 };
+enum : uint32_t {
+  kDrawAttempt = 1,
+  kDrawNattempts = 2,
+  kDrawDeath = 3,
+  kDrawRoundMain = 4,
+  kDrawRoundSpawn = 5,
+  kDrawDeterm = 8  // rounding of the deterministic projection's spawns
+};
 
 class Oracle {
  public:
@@ -228,11 +235,19 @@ class Oracle {
     int64_t ndeath = 0;
     int64_t attempts = 0;
     int64_t t = 0;  // production: global attempt index, det-major
+    std::vector<double> vec;     // semi-stochastic: the core determinants' populations (determ%vector)
+    std::vector<int64_t> dpos;   // and their list positions (determ%indices)
     for (size_t idet = 0; idet < states.size(); ++idet) {
       Decoded dd = decode_full(states[idet].f);
       double real_pop[FQ_MAX_SPACES];
       for (int s = 0; s < S; ++s) real_pop[s] = static_cast<double>(pops[idet * S + s]) / rf;
-      int64_t flag = parent_flag(real_pop);
+      // set_determ_info / set_parent_flag: a core determinant is an initiator
+      const bool dparent = is_determ(states[idet].f);
+      if (dparent) {
+        for (int s = 0; s < S; ++s) vec.push_back(real_pop[s]);
+        dpos.push_back(static_cast<int64_t>(idet));
+      }
+      int64_t flag = dparent ? 0 : parent_flag(real_pop);
       for (int s = 0; s < S; ++s) {
         Excit c = get_excitation(states[idet].f, f0.data());
         update_proj_energy(states[idet].f, dd.occ, real_pop[s], c, out->proj_energy[s], out->D0_population[s]);
@@ -246,16 +261,97 @@ class Oracle {
           draw_stream(static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32), kDrawAttempt);
           Excit conn;
           int64_t nspawn = spawn(dd, states[idet].f, parent, tau, conn);
+          if (nspawn != 0 && dparent) {  // core-to-core spawns are the deterministic projection's
+            uint64_t child[FQ_MAX_WORDS] = {0, 0};
+            excited_det(states[idet].f, conn, child);
+            if (is_determ(child)) nspawn = 0;
+          }
           if (nspawn != 0) create_spawned(states[idet].f, conn, nspawn, s, flag);
         }
-        draw_stream(static_cast<uint32_t>(i64), hi, kDrawDeath);
-        death(tau, dat[idet], shift[s], eproj_old[s], pops[idet * S + s], nparticles[s], ndeath);
+        if (!dparent) {  // a core determinant's death is part of the deterministic projection
+          draw_stream(static_cast<uint32_t>(i64), hi, kDrawDeath);
+          death(tau, dat[idet], shift[s], eproj_old[s], pops[idet * S + s], nparticles[s], ndeath);
+        }
       }
+    }
+    if (determ_.on) {
+      if (determ_.separate)
+        determ_separate(vec, dpos, shift, eproj_old, tau);
+      else
+        determ_projection(vec, shift, eproj_old, tau);
     }
     out->nspawn_events = static_cast<int64_t>(spawned.size());
     out->ndeath = ndeath;
     out->nattempts = attempts;
   }
+
+  // init_semi_stoch_t with space "high" (src/semi_stoch.F90:134-377,
+  // create_high_pop_space :1198-1316, find_most_populated_dets :1318-1385,
+  // create_determ_hamil :533-685, one rank): the target_size most populated
+  // determinants of the current list (sum over spaces of |N|, kept by the
+  // reference's replace-the-minimum scan), sorted as the main list, and the
+  // Hamiltonian among them as a CSR matrix, H_ij = <D_i|H|D_j> (H_ii - H_00 on
+  // the diagonal, elements of magnitude <= 1e-12 dropped).  For separate
+  // annihilation the stored matrix is its transpose, T_ij = <D_j|H|D_i>: row i
+  // then lists the entries the reference's transposed product
+  // (csrpgemv, lib/local/csr.f90:148-204) adds into element i, in the order it
+  // adds them.  Every core determinant is in the list, so
+  // add_determ_dets_to_psip_list inserts nothing.
+  void semi_stoch_init(int target_size, bool separate) {
+    const int64_t n = static_cast<int64_t>(states.size());
+    const int64_t nd = std::min<int64_t>(target_size, n);
+    auto absum = [&](int64_t i) {
+      int64_t t = 0;
+      for (int s = 0; s < S; ++s) t += std::llabs(pops[i * S + s]);
+      return t;
+    };
+    std::vector<int64_t> idx(nd), pop(nd);
+    int64_t min_pop = 0, min_ind = 0;
+    for (int64_t i = 0; i < nd; ++i) {
+      idx[i] = i;
+      pop[i] = absum(i);
+      if (i == 0 || pop[i] < min_pop) {
+        min_pop = pop[i];
+        min_ind = i;
+      }
+    }
+    for (int64_t i = nd; i < n; ++i) {
+      const int64_t a = absum(i);
+      if (a > min_pop) {
+        idx[min_ind] = i;
+        pop[min_ind] = a;
+        min_pop = pop[0];
+        min_ind = 0;
+        for (int64_t j = 1; j < nd; ++j)
+          if (pop[j] < min_pop) {
+            min_pop = pop[j];
+            min_ind = j;
+          }
+      }
+    }
+    determ_ = DetermSpace{};
+    for (int64_t i = 0; i < nd; ++i) determ_.dets.push_back(states[idx[i]]);
+    std::sort(determ_.dets.begin(), determ_.dets.end(),
+              [&](const Det& a, const Det& b) { return cmp(a.f, b.f) == 1; });
+    const int64_t D = nd;
+    determ_.row_ptr.assign(D + 1, 0);
+    for (int64_t i = 0; i < D; ++i) {
+      determ_.row_ptr[i] = static_cast<int64_t>(determ_.mat.size());
+      for (int64_t j = 0; j < D; ++j) {
+        double h = separate ? hmatel(determ_.dets[j].f, determ_.dets[i].f) : hmatel(determ_.dets[i].f, determ_.dets[j].f);
+        if (i == j) h = h - sys_.H00;
+        h = 1.0 * h;  // the quasi-Newton weight, 1 here
+        if (std::fabs(h) > 1.0e-12) {
+          determ_.mat.push_back(h);
+          determ_.col.push_back(j);
+        }
+      }
+    }
+    determ_.row_ptr[D] = static_cast<int64_t>(determ_.mat.size());
+    determ_.on = true;
+    determ_.separate = separate;
+  }
+  int64_t determ_size() const { return determ_.on ? static_cast<int64_t>(determ_.dets.size()) : 0; }
 
   // direct_annihilation of `spawned` (as received) and the cycle's totals.
   void cycle_finish(fq_cycle_out* out) {
@@ -300,6 +396,120 @@ class Oracle {
   std::vector<double> h1, eri;
   std::vector<uint64_t> excit_mask;
   std::vector<Spawned> spawned;
+
+  // the semi-stochastic core space (semi_stoch_t)
+  struct DetermSpace {
+    bool on = false;
+    bool separate = false;       // separate (true) or combined annihilation
+    std::vector<Det> dets;        // sorted as the main list
+    std::vector<int64_t> row_ptr;  // CSR rows in that order
+    std::vector<int64_t> col;
+    std::vector<double> mat;
+  } determ_;
+
+  bool is_determ(const uint64_t* f) const {  // check_if_determ
+    if (!determ_.on) return false;
+    size_t lo = 0, hi = determ_.dets.size();
+    while (lo < hi) {
+      const size_t mid = (lo + hi) / 2;
+      const int c = cmp(determ_.dets[mid].f, f);
+      if (c == 0) return true;
+      if (c == 1)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return false;
+  }
+
+  // get_hmatel for two determinants (src/hamiltonian_molecular.f90:12-71 and
+  // the Hubbard-k rules): the Slater--Condon element by excitation level.
+  double hmatel(const uint64_t* f1, const uint64_t* f2) {
+    Excit e = get_excitation(f1, f2);
+    if (e.nexcit == 0) return sc0(f1);
+    if (sys_.system == FQ_SYS_HUB_K) {
+      if (e.nexcit != 2) return 0.0;
+      double h = two_e_hub_k(e.from[0], e.from[1], e.to[0], e.to[1]);
+      return e.perm ? -h : h;
+    }
+    if (e.nexcit == 1) return sc1_excit(decode_occ(f1), e.from[0], e.to[0], e.perm);
+    if (e.nexcit == 2) return sc2_excit(e.from[0], e.from[1], e.to[0], e.to[1], e.perm);
+    return 0.0;
+  }
+
+  // determ_proj_combined_annihil + create_spawned_particle_determ
+  // (src/semi_stoch.F90:897-975,1096-1146): for each core determinant (CSR
+  // row) and space, -tau (sum_j H_ij N_j + (E_old*0 - S) N_i) (the quasi-Newton
+  // weight difference is 0 without quasi-Newton), encoded with one stochastic
+  // rounding draw, appended to the spawned list (a zero included) as from an
+  // initiator.
+  void determ_projection(const std::vector<double>& vec, const double* shift, const double* eproj_old, double tau) {
+    const int64_t D = static_cast<int64_t>(determ_.dets.size());
+    for (int64_t row = 0; row < D; ++row) {
+      for (int s = 0; s < S; ++s) {
+        double out = 0.0;
+        for (int64_t k = determ_.row_ptr[row]; k < determ_.row_ptr[row + 1]; ++k)
+          out = out + determ_.mat[k] * vec[determ_.col[k] * S + s];
+        const double rho = 1.0 - 1.0;  // quasi_newton_pop_control - the row's weight
+        out = -tau * (out + (eproj_old[s] * rho - 1.0 * shift[s]) * vec[row * S + s]);
+        draw_stream(static_cast<uint32_t>(row), static_cast<uint32_t>(s) << 28, kDrawDeterm);
+        const double scaled = out * static_cast<double>(rf);
+        const double sgn = std::copysign(1.0, out);
+        int64_t nspawn = static_cast<int64_t>(std::fabs(scaled));
+        if (std::fabs(scaled) - static_cast<double>(nspawn) > rand()) nspawn = nspawn + 1;
+        nspawn = nspawn * static_cast<int64_t>(std::llround(sgn));
+        if (static_cast<int64_t>(spawned.size()) + 1 > par_.spawn_capacity) {
+          spawn_error = true;
+          continue;
+        }
+        Spawned sp;
+        std::memset(&sp, 0, sizeof(sp));
+        for (int w = 0; w < W; ++w) sp.f[w] = determ_.dets[row].f[w];
+        sp.pop[s] = nspawn;
+        sp.flag = 0;
+        spawned.push_back(sp);
+      }
+    }
+  }
+  // determ_proj_separate_annihil + deterministic_annihilation
+  // (src/semi_stoch.F90:1009-1094, src/annihilation.f90:488-535): per core
+  // determinant and space, y_i = (-tau (E_old*0 - S)) v_i, then
+  // y_i += (-tau T_ik) v_k over row i of the stored transpose (the
+  // reference's y(icol) += a*m*x(irow) loop, in its order); once every y is
+  // formed, each is encoded with one stochastic rounding draw per space (row
+  // by row, spaces inner) and added to the core determinant's population in
+  // the main list (these are not spawning events).
+  void determ_separate(const std::vector<double>& vec, const std::vector<int64_t>& dpos, const double* shift,
+                       const double* eproj_old, double tau) {
+    const int64_t D = static_cast<int64_t>(determ_.dets.size());
+    std::vector<double> y(static_cast<size_t>(D) * S);
+    const double rho = 1.0 - 1.0;  // quasi_newton_pop_control - the row's weight
+    const double a = -1.0 * tau;
+    for (int s = 0; s < S; ++s) {
+      const double c = -(tau * (eproj_old[s] * rho - shift[s] * 1.0));
+      for (int64_t i = 0; i < D; ++i) {
+        double v = c * vec[i * S + s];
+        for (int64_t k = determ_.row_ptr[i]; k < determ_.row_ptr[i + 1]; ++k)
+          v = v + a * determ_.mat[k] * vec[determ_.col[k] * S + s];
+        y[i * S + s] = v;
+      }
+    }
+    for (int64_t i = 0; i < D; ++i) {
+      int64_t* p = &pops[dpos[i] * S];
+      for (int s = 0; s < S; ++s) {
+        draw_stream(static_cast<uint32_t>(i), static_cast<uint32_t>(s) << 28, kDrawDeterm);
+        double scaled = y[i * S + s] * static_cast<double>(rf);
+        const int64_t sgn = scaled < 0.0 ? -1 : 1;
+        scaled = std::fabs(scaled);
+        int64_t nspawn = static_cast<int64_t>(scaled);
+        if (scaled - static_cast<double>(nspawn) > rand()) nspawn = nspawn + 1;
+        const int64_t old = p[s];
+        p[s] = p[s] + sgn * nspawn;
+        nparticles[s] = nparticles[s] + static_cast<double>(std::llabs(p[s]) - std::llabs(old)) / rf;
+      }
+    }
+  }
+
 
 
  public:
@@ -1387,14 +1597,15 @@ class Oracle {
   void remove_unoccupied() {
     size_t k = 0;
     for (size_t i = 0; i < states.size(); ++i) {
-      if (par_.real_amplitudes) {
+      const bool core = is_determ(states[i].f);
+      if (par_.real_amplitudes && !core) {
         int64_t old[FQ_MAX_SPACES];
         for (int s = 0; s < S; ++s) old[s] = pops[i * S + s];
         draw_stream(static_cast<uint32_t>(i), static_cast<uint32_t>(static_cast<uint64_t>(i) >> 32), kDrawRoundMain);
         stoch_round(&pops[i * S]);
         add_np(&pops[i * S], old);
       }
-      if (!allzero(&pops[i * S])) {
+      if (core || !allzero(&pops[i * S])) {
         states[k] = states[i];
         for (int s = 0; s < S; ++s) pops[k * S + s] = pops[i * S + s];
         dat[k] = dat[i];
@@ -1519,7 +1730,7 @@ int owner_rank(const uint64_t* f, int nbasis, int nprocs) {
 // (dump_restart_hdf5, src/restart_hdf5.F90:307-537).
 int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows, int max_rows, int* nrows,
                int nprocs, const fqh_restart* start = nullptr, fqh_restart* end = nullptr) {
-  if (nprocs < 1 || (end && nprocs != 1)) return -1;
+  if (nprocs < 1 || (end && nprocs != 1) || (in->semi_stoch_size > 0 && nprocs != 1)) return -1;
   fq_params p{};
   p.nspaces = in->replica_tricks ? 2 : 1;
   p.initiator = in->initiator;
@@ -1613,6 +1824,14 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
   };
   emit(cycles0, 0, 0.0);
   int reports_done = 0;
+  // the iteration the semi-stochastic core is built (src/fciqmc.f90:229-235)
+  int64_t ss_iter = std::max<int64_t>(in->semi_stoch_start_iter, cycles0 + 1);
+  auto all_vary = [&]() {
+    for (int s = 0; s < S; ++s)
+      if (!ctl.vary[s]) return false;
+    return true;
+  };
+  if (all_vary() && in->semi_stoch_shift_iter != -1) ss_iter = cycles0 + 1;
   std::vector<fq_cycle_out> out(nprocs);
   std::vector<std::vector<Spawned>> sent(nprocs);
   for (int ireport = 1; ireport <= in->nreports && nr < max_rows; ++ireport) {  // a full table ends the run
@@ -1623,6 +1842,9 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
     std::vector<int64_t> nspawn_events(nprocs, 0);
     bool error = false;
     for (int ic = 0; ic < in->mc_cycles; ++ic) {
+      const int64_t iter = cycles0 + static_cast<int64_t>(ireport - 1) * in->mc_cycles + ic + 1;
+      if (in->semi_stoch_size > 0 && iter == ss_iter)
+        ranks[0]->semi_stoch_init(in->semi_stoch_size, in->semi_stoch_separate != 0);
       std::vector<int64_t> nattempts(nprocs);
       for (int r = 0; r < nprocs; ++r) {
         nattempts[r] = static_cast<int64_t>(std::llround(2 * ranks[r]->nparticles[0]));
@@ -1671,7 +1893,10 @@ int run_fciqmc(const fq_system* sys, const fqh_qmc_in* in, fqh_report_row* rows,
       nsp += nspawn_events[r];
     }
     rs = rs / (in->mc_cycles * nprocs);
+    const bool vary_before = all_vary();
     const bool soft_exit = ctl.end_report(ntot, proj, d0);
+    if (!vary_before && all_vary() && in->semi_stoch_shift_iter != -1)  // end_report_loop, src/qmc_common.F90:1203
+      ss_iter = in->semi_stoch_shift_iter + cycles0 + static_cast<int64_t>(ireport) * in->mc_cycles + 1;
     emit(cycles0 + static_cast<int64_t>(ireport) * in->mc_cycles, nsp, rs);
     ctl.after_report(ireport, proj[0], d0[0]);
     if (nr > 0 && nr <= max_rows) {

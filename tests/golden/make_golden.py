@@ -56,6 +56,10 @@ QMC_TESTS = [
     # automatic shift damping
     ("He2-aug-cc-pVDZ_real_64_auto_shift_damping_restart",
      "ifciqmc_real_64/np1/He2-aug-cc-pVDZ_real_64_auto_shift_damping_restart", "read.in"),
+    # semi-stochastic projection (space "high", core chosen at iteration 1000):
+    # combined annihilation, then separate annihilation with a replica pair
+    ("He2-aug-cc-pVDZ_real_32_SS", "fciqmc_real_32/np1/He2-aug-cc-pVDZ_real_32_SS", "he2.in"),
+    ("He2-aug-cc-pVDZ_real_32_SS_replica", "fciqmc_real_32/np1/He2-aug-cc-pVDZ_real_32_SS_replica", "he2.in"),
 ]
 
 # long tables are stored truncated (the tests compare a prefix)

@@ -15,19 +15,21 @@ CASES = [k for k, e in goldens().items()
 def test_cases_cover_plain_restart_and_replica_free_tables():
     assert "He2-aug-cc-pVDZ_real_64_restart" in CASES and "hubbard_k_1d_n4" in CASES
     assert len(CASES) >= 10
+    assert "He2-aug-cc-pVDZ_real_32_SS_replica" in CASES  # a two-space table
 
 
 @pytest.mark.parametrize("name", CASES)
 def test_report_lines_match_reference_text(name):
     e = goldens()[name]
     text = e["report_text"]
-    hdr = report.header(1)
-    assert hdr[-1] == text[0]
+    nspaces = 2 if "Replica" in text[0] else 1  # a replica pair's table has a 'Replica i' line on top
+    hdr = report.header(nspaces)
+    assert hdr == text[:nspaces]
     # data rows only: bloom warnings (bloom_stats_warning) are comment lines without a leading number
-    rows = [line for line in text[1:] if line.lstrip(" #").split(" ")[0].isdigit()]
+    rows = [line for line in text[nspaces:] if line.lstrip(" #").split(" ")[0].isdigit()]
     assert len(rows) >= 3
     for i, (vals, line) in enumerate(zip(e["report"], rows)):
-        assert report.row(vals, 1, comment=(i == 0)) == line
+        assert report.row(vals, nspaces, comment=(i == 0)) == line
 
 
 def test_replica_header_and_table():
