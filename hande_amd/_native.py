@@ -123,6 +123,8 @@ def _declare(lib):
         "fq_rng_available": (C.c_int, [vp, P(i64)]),
         "fq_cycle": (C.c_int, [vp, P(f64), P(f64), f64, P(FqCycleOut)]),
         "fq_cycle_noalloc": (C.c_int, [vp, P(f64), P(f64), f64, P(FqCycleOut)]),
+        "fq_semi_stoch_init": (C.c_int, [vp, i64, C.c_int]),
+        "fq_semi_stoch_info": (C.c_int, [vp, P(i64), P(i64), P(u64), i64]),
         "fq_cycle_spawn": (C.c_int, [vp, P(f64), P(f64), f64, P(FqCycleOut)]),
         "fq_spawned_list": (C.c_int, [vp, P(u64), P(i64), P(i64), i64, P(i64)]),
         "fq_cycle_finish": (C.c_int, [vp, P(u64), P(i64), P(i64), i64, P(FqCycleOut)]),

@@ -211,7 +211,7 @@ def pop_bit_shift(st):
 
 
 def fciqmc(sys, qmc, fciqmc=None, rng_mode="replay", device=0, lib=None, nprocs=1, restart=None,
-           return_state=False, blocking=None, max_reports=None, **_):
+           return_state=False, blocking=None, max_reports=None, semi_stoch=None, **_):
     """fciqmc{...}: run FCIQMC on the GPU engine and return the report table.
 
     nprocs > 1 (replay mode) replays a run of that many MPI ranks: one engine
@@ -222,10 +222,11 @@ def fciqmc(sys, qmc, fciqmc=None, rng_mode="replay", device=0, lib=None, nprocs=
     return_state=True returns (table, RestartState after the last report), the
     state ``restart = { write = ... }`` stores (``restart.write_restart``).
     max_reports stops the run after that many reports (a prefix of the run
-    nreports describes)."""
+    nreports describes).  semi_stoch is the semi_stoch{} table (space
+    "high"; one rank)."""
     from . import restart as R
     lib = lib or sys._lib
-    st = qmc_in(qmc, fciqmc, rng_mode=rng_mode, device=device, blocking=blocking)
+    st = qmc_in(qmc, fciqmc, rng_mode=rng_mode, device=device, blocking=blocking, semi_stoch=semi_stoch)
     S = 2 if st.replica_tricks else 1
     pbs = pop_bit_shift(st)
     rs = None

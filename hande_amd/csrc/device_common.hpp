@@ -90,6 +90,10 @@ struct Sys {
   int initiator, quadrature, real_amps, excit_gen;
   double initiator_pop;
   int64_t rf, spawn_cutoff;
+  // Semi-stochastic core (combined annihilation): its ndeterm determinants,
+  // sorted as the main list, word-major (determ[w * ndeterm + k]); 0: off.
+  const uint64_t* determ;
+  int64_t ndeterm;
 };
 
 // ---------------------------------------------------------------------------
@@ -177,7 +181,8 @@ enum : uint32_t {
   kDrawRoundMain = 4,
   kDrawRoundSpawn = 5,
   kDrawTrain = 6,
-  kDrawProbe = 7
+  kDrawProbe = 7,
+  kDrawDeterm = 8  // rounding of the deterministic projection's spawns
 };
 
 // ---------------------------------------------------------------------------
@@ -404,6 +409,46 @@ __device__ inline double sc2_excit(const Sys& s, int i, int j, int a, int b, boo
 
 // <D|H|D> in the reference's summation order (slater_condon0_*).
 __device__ inline double diagonal_mol(const Sys& s, const uint64_t* f, const double* J, const double* K);
+__device__ inline double diagonal(const Sys& s, const uint64_t* f);
+
+// Index of f among the semi-stochastic core determinants, -1 if it is not one
+// (check_if_determ, src/semi_stoch.F90:795-824, as a binary search of the
+// sorted core).
+__device__ inline int64_t core_index(const Sys& s, const uint64_t* f) {
+  int64_t lo = 0, hi = s.ndeterm;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    int c = 0;
+#pragma unroll
+    for (int w = FQ_MAX_WORDS - 1; w >= 0; --w)
+      if (w < s.W && c == 0) {
+        const uint64_t x = s.determ[w * s.ndeterm + mid];
+        c = x < f[w] ? -1 : (x > f[w] ? 1 : 0);
+      }
+    if (c == 0) return mid;
+    if (c < 0)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return -1;
+}
+
+// get_hmatel(f1, f2) (src/hamiltonian_molecular.f90:12-71; Hubbard-k in
+// momentum space: only double excitations couple): the Slater--Condon element
+// by excitation level, single excitations over f1's occupied orbitals.
+__device__ inline double hmatel(const Sys& s, const uint64_t* f1, const uint64_t* f2) {
+  const Excit e = get_excitation(f1, f2, s.W, s.nel);
+  if (e.nexcit == 0) return diagonal(s, f1);
+  if (s.system == FQ_SYS_HUB_K) {
+    if (e.nexcit != 2) return 0.0;
+    const double h = two_e_hub_k(s, e.from[0], e.from[1], e.to[0], e.to[1]);
+    return e.perm ? -h : h;
+  }
+  if (e.nexcit == 1) return sc1_excit(s, f1, e.from[0], e.to[0], e.perm);
+  if (e.nexcit == 2) return sc2_excit(s, e.from[0], e.from[1], e.to[0], e.to[1], e.perm);
+  return 0.0;
+}
 __device__ inline double diagonal(const Sys& s, const uint64_t* f) {
   if (s.system == FQ_SYS_HUB_K) {
     double h = static_cast<double>(s.nalpha * s.nbeta) * s.coulomb_k;
@@ -547,6 +592,19 @@ __device__ inline int64_t stoch_round_pop(Rng& rng, int64_t p, int64_t rf) {
     return (static_cast<double>(ap) > r) ? fsign64(rf, p) : 0;
   }
   return p;
+}
+
+// A deterministic-projection amplitude encoded with one stochastic rounding
+// draw (create_spawned_particle_determ, src/semi_stoch.F90:1096-1146;
+// deterministic_annihilation, src/annihilation.f90:488-535): the integer part
+// of |amp| * rf, one more with probability its fraction, with amp's sign.
+template <class Rng>
+__device__ inline int64_t stoch_encode(Rng& rng, double amp, int64_t rf) {
+  const double scaled = amp * static_cast<double>(rf);
+  const double a = fabs(scaled);
+  int64_t k = static_cast<int64_t>(a);
+  if (a - static_cast<double>(k) > rng.next()) k = k + 1;
+  return scaled < 0.0 ? -k : k;
 }
 
 // ---------------------------------------------------------------------------

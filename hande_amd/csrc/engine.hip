@@ -39,10 +39,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
+#include <queue>
 #include <type_traits>
 #include <vector>
 
@@ -117,6 +120,24 @@ struct Stats {
   int64_t rng_failed;
   unsigned long long abs_pop[FQ_MAX_SPACES];  // sum |pop| after annihilation (encoded)
 };
+
+// The semi-stochastic core as the projection kernels see it: CSR rows of the
+// stored Hamiltonian (row i: H_ij over j in combined mode, the transpose
+// H_ji in separate mode), the cycle's core populations vec[i * S + s] and
+// the main-list position of each core determinant.
+struct DetermDev {
+  const int64_t* row;
+  const int32_t* col;
+  const double* mat;
+  double* vec;
+  int64_t* pos;
+  int64_t nd;
+  int separate;
+};
+// A determinant-pass flag bit above the initiator flags: the parent is a core
+// determinant (its spawns onto the core are the deterministic projection's).
+constexpr int64_t kCoreFlag = int64_t{1} << 40;
+constexpr int64_t kInitFlagMask = 0xff;
 
 // ---------------------------------------------------------------------------
 // Prefix sums.  Up to kSmallScanMax entries: one 1024-thread block (one
@@ -390,6 +411,18 @@ struct fq_engine {
   int64_t* h_len = nullptr;
   hipEvent_t ev_len = nullptr;
   int64_t spawn_bound = 0;  // attempts the cycle's spawning launches covered
+  // semi-stochastic core (fq_semi_stoch_init): its determinants (word-major,
+  // sorted as the main list; also in sys.determ), the Hamiltonian among them
+  // as CSR rows (the transpose for separate annihilation), and per cycle the
+  // core populations and their main-list positions
+  int64_t ndeterm = 0, determ_nnz = 0;
+  int determ_separate = 0;
+  uint64_t* d_determ = nullptr;
+  int64_t* d_drow = nullptr;
+  int32_t* d_dcol = nullptr;
+  double* d_dmat = nullptr;
+  double* d_dvec = nullptr;
+  int64_t* d_dpos = nullptr;
   float t_spawn = 0, t_annih = 0, t_total = 0;
   // phase events: 0 cycle start, 1 det pass end, 2 spawn start, 3 spawn end,
   // 4 sort start, 5 sort end, 6 reduce end, 7 merge start, 8 merge end
@@ -506,13 +539,17 @@ __device__ __forceinline__ void load_det(const uint64_t* det, int64_t cap, int W
 // Production: per determinant (loop body of src/fciqmc.f90:315-374 minus the
 // spawning attempts).  Block partial sums (projected energy, reference
 // population, deaths) are reduced in a fixed order by k_sum_partials, so the
-// estimators are deterministic and no global atomics are issued.
+// estimators are deterministic and no global atomics are issued.  A
+// semi-stochastic core determinant (set_determ_info) records its population
+// and position for the projection, is flagged an initiator with kCoreFlag,
+// and skips death.
 __global__ __launch_bounds__(kBlock) void k_det_pass(Sys s, int64_t n, int64_t cap, const uint64_t* __restrict__ det,
                                                      const int64_t* __restrict__ pop, const double* __restrict__ kii,
                                                      int64_t* __restrict__ pop_new, int64_t* __restrict__ nat,
                                                      int64_t* __restrict__ flags, double tau, double sh0, double sh1,
                                                      double e0, double e1, uint64_t seed, uint32_t cyc,
-                                                     double* __restrict__ part_d, int64_t* __restrict__ part_i) {
+                                                     double* __restrict__ part_d, int64_t* __restrict__ part_i,
+                                                     double* __restrict__ dvec, int64_t* __restrict__ dpos) {
   __shared__ double sh_d[2 * FQ_MAX_SPACES][kBlock];
   __shared__ int64_t sh_death[kBlock];
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -527,16 +564,23 @@ __global__ __launch_bounds__(kBlock) void k_det_pass(Sys s, int64_t n, int64_t c
       p[sp] = pop[sp * cap + i];
       rp[sp] = static_cast<double>(p[sp]) / static_cast<double>(s.rf);
     }
-    flags[i] = fqd::parent_flag(s, rp);
+    const int64_t ci = s.ndeterm > 0 ? fqd::core_index(s, f) : -1;
+    if (ci >= 0) {
+      for (int sp = 0; sp < s.S; ++sp) dvec[ci * s.S + sp] = rp[sp];
+      dpos[ci] = i;
+    }
+    flags[i] = ci >= 0 ? kCoreFlag : fqd::parent_flag(s, rp);
     const double K = kii[i];
     for (int sp = 0; sp < s.S; ++sp) {
       fqd::proj_energy_term(s, f, rp[sp], proj[sp], d0[sp]);
       const uint32_t hi = static_cast<uint32_t>(i >> 32) ^ (static_cast<uint32_t>(sp) << 28);
       fqd::Philox rng(seed, static_cast<uint32_t>(i), hi, cyc ^ (fqd::kDrawNattempts << 24));
       nat[i * s.S + sp] = fqd::decide_nattempts(rng, rp[sp]);
-      fqd::Philox rd(seed, static_cast<uint32_t>(i), hi, cyc ^ (fqd::kDrawDeath << 24));
       int64_t q = p[sp];
-      ndeath += fqd::death(rd, tau, K, sp == 0 ? sh0 : sh1, sp == 0 ? e0 : e1, q);
+      if (ci < 0) {
+        fqd::Philox rd(seed, static_cast<uint32_t>(i), hi, cyc ^ (fqd::kDrawDeath << 24));
+        ndeath += fqd::death(rd, tau, K, sp == 0 ? sh0 : sh1, sp == 0 ? e0 : e1, q);
+      }
       pop_new[sp * cap + i] = q;
     }
   }
@@ -814,6 +858,8 @@ __global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, 
                         cyc ^ (fqd::kDrawAttempt << 24));
         const float* wrow = (G == fqd::kGenLearned && s.mlp_on[i]) ? s.mlp_tab + s.mlp_row[i] * s.mlp_ld : nullptr;
         nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child, hb, aff_lds, flat_lds);
+        // a core parent's spawn onto the core is the deterministic projection's
+        if (s.ndeterm > 0 && nspawn != 0 && (pflags[i] & kCoreFlag) && fqd::core_index(s, child) >= 0) nspawn = 0;
       }
     };
     if constexpr (kPart) {
@@ -891,7 +937,7 @@ __global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, 
         FQ_FOR_WORDS(w, s.W) st_det[w * bcap + u] = child[w];
         st_pop[u] = nspawn;
         // the spawning space and the parent's initiator flags (the sort key's flag bits)
-        st_ent[u] = (e % s.S) | (s.initiator ? pflags[e / s.S] << 8 : 0);
+        st_ent[u] = (e % s.S) | (s.initiator ? (pflags[e / s.S] & kInitFlagMask) << 8 : 0);
       }
       run += round_total;
       __syncthreads();
@@ -1095,11 +1141,40 @@ __global__ __launch_bounds__(kBlock) void k_inst_stats(int ninst, int S, int64_t
 }
 
 // Replay: the whole spawning/death loop on one thread, in reference order.
+// determ_proj_combined_annihil (src/semi_stoch.F90:897-975) for one core
+// determinant and space: -tau (sum_j H_ij v_j + (E_old * 0 - S) v_i), the
+// row summed in column order (csrpgemv_single_row, lib/local/csr.f90:206-235);
+// 0 is rho - w, the quasi-Newton weight difference without quasi-Newton.
+__device__ __forceinline__ double determ_combined(const DetermDev& dd, int S, int64_t row, int sp, double tau,
+                                                  double shift, double eold) {
+  double out = 0.0;
+  for (int64_t k = dd.row[row]; k < dd.row[row + 1]; ++k) out = out + dd.mat[k] * dd.vec[dd.col[k] * S + sp];
+  return -tau * (out + (eold * (1.0 - 1.0) - 1.0 * shift) * dd.vec[row * S + sp]);
+}
+// determ_proj_separate_annihil (src/semi_stoch.F90:1009-1094): the shift term
+// -(tau (E_old * 0 - S)) v_i, then (-tau T_ik) v_k added over the stored
+// transpose's row i in order, as the reference's transposed product
+// (csrpgemv, lib/local/csr.f90:148-204) accumulates element i.
+__device__ __forceinline__ double determ_separate(const DetermDev& dd, int S, int64_t row, int sp, double tau,
+                                                  double shift, double eold) {
+  const double a = -1.0 * tau;
+  double y = -(tau * (eold * (1.0 - 1.0) - shift * 1.0)) * dd.vec[row * S + sp];
+  for (int64_t k = dd.row[row]; k < dd.row[row + 1]; ++k) y = y + a * dd.mat[k] * dd.vec[dd.col[k] * S + sp];
+  return y;
+}
+
+// Semi-stochastic core (dd.nd > 0, set_determ_info and the determ_parent
+// branches of src/fciqmc.f90:330-372): a core parent is an initiator, its
+// spawns onto the core are cancelled and its death is left to the
+// deterministic projection, which follows the loop (determ_projection,
+// src/semi_stoch.F90:859-1094) with one rounding draw per core determinant
+// and space.
 __global__ void k_replay_walk(Sys s, int64_t n, int64_t cap, const uint64_t* __restrict__ det,
                               const int64_t* __restrict__ pop, const double* __restrict__ kii,
                               int64_t* __restrict__ pop_new, double tau, double sh0, double sh1, double e0,
                               double e1, const double* __restrict__ rbuf, int64_t rpos, int64_t rend, int64_t scap,
-                              uint64_t* __restrict__ sdet, int64_t* __restrict__ srec, Stats* __restrict__ st) {
+                              uint64_t* __restrict__ sdet, int64_t* __restrict__ srec, Stats* __restrict__ st,
+                              DetermDev dd) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   fqd::ReplayRng rng{rbuf, rpos, rend, false};
   double proj[FQ_MAX_SPACES] = {0.0, 0.0}, d0[FQ_MAX_SPACES] = {0.0, 0.0};
@@ -1114,14 +1189,20 @@ __global__ void k_replay_walk(Sys s, int64_t n, int64_t cap, const uint64_t* __r
       p[sp] = pop[sp * cap + i];
       rp[sp] = static_cast<double>(p[sp]) / static_cast<double>(s.rf);
     }
-    const int64_t flag = fqd::parent_flag(s, rp);
+    const int64_t ci = dd.nd > 0 ? fqd::core_index(s, f) : -1;
+    if (ci >= 0) {
+      for (int sp = 0; sp < s.S; ++sp) dd.vec[ci * s.S + sp] = rp[sp];
+      dd.pos[ci] = i;
+    }
+    const int64_t flag = ci >= 0 ? 0 : fqd::parent_flag(s, rp);
     for (int sp = 0; sp < s.S; ++sp) {
       fqd::proj_energy_term(s, f, rp[sp], proj[sp], d0[sp]);
       const int nat = fqd::decide_nattempts(rng, rp[sp]);
       attempts += nat;
       for (int a = 0; a < nat && !rng.failed; ++a) {
         uint64_t child[FQ_MAX_WORDS] = {0, 0};
-        const int64_t ns = fqd::spawn_attempt(s, rng, f, nullptr, p[sp], tau, child);
+        int64_t ns = fqd::spawn_attempt(s, rng, f, nullptr, p[sp], tau, child);
+        if (ns != 0 && ci >= 0 && fqd::core_index(s, child) >= 0) ns = 0;
         if (ns != 0) {
           if (nspawn_total + 1 > scap) {
             overflow = true;
@@ -1135,9 +1216,37 @@ __global__ void k_replay_walk(Sys s, int64_t n, int64_t cap, const uint64_t* __r
         }
       }
       int64_t q = p[sp];
-      ndeath += fqd::death(rng, tau, kii[i], sp == 0 ? sh0 : sh1, sp == 0 ? e0 : e1, q);
+      if (ci < 0) ndeath += fqd::death(rng, tau, kii[i], sp == 0 ? sh0 : sh1, sp == 0 ? e0 : e1, q);
       p[sp] = q;
       pop_new[sp * cap + i] = q;
+    }
+  }
+  if (dd.nd > 0 && !rng.failed) {
+    const double sh[FQ_MAX_SPACES] = {sh0, sh1}, eo[FQ_MAX_SPACES] = {e0, e1};
+    if (!dd.separate) {
+      for (int64_t row = 0; row < dd.nd && !rng.failed; ++row)
+        for (int sp = 0; sp < s.S; ++sp) {
+          const int64_t ns = fqd::stoch_encode(rng, determ_combined(dd, s.S, row, sp, tau, sh[sp], eo[sp]), s.rf);
+          if (nspawn_total + 1 > scap) {
+            overflow = true;
+            continue;
+          }
+          const int64_t slot = nspawn_total++;
+          FQ_FOR_WORDS(w, s.W) sdet[w * scap + slot] = s.determ[w * dd.nd + row];
+          int64_t* rec = srec + slot * rec_stride(s.S);
+          for (int q = 0; q < s.S; ++q) rec[q] = (q == sp) ? ns : 0;
+          rec[s.S] = 0;
+        }
+    } else {
+      // every row's projection first, then the rounding draws row by row
+      for (int64_t row = 0; row < dd.nd; ++row)
+        for (int sp = 0; sp < s.S; ++sp)
+          dd.vec[(dd.nd + row) * s.S + sp] = determ_separate(dd, s.S, row, sp, tau, sh[sp], eo[sp]);
+      for (int64_t row = 0; row < dd.nd && !rng.failed; ++row)
+        for (int sp = 0; sp < s.S; ++sp) {
+          const int64_t ns = fqd::stoch_encode(rng, dd.vec[(dd.nd + row) * s.S + sp], s.rf);
+          pop_new[sp * cap + dd.pos[row]] += ns;
+        }
     }
   }
   for (int sp = 0; sp < FQ_MAX_SPACES; ++sp) {
@@ -1882,6 +1991,101 @@ __global__ __launch_bounds__(kBlock) void k_runs(Sys s, KeyPlan kp, const uint64
 }
 
 // ---------------------------------------------------------------------------
+// Kernels: semi-stochastic projection
+// ---------------------------------------------------------------------------
+// Element (i, j) of the core Hamiltonian as create_determ_hamil stores it
+// (src/semi_stoch.F90:533-685): <D_i|H|D_j>, H_ii - H_00 on the diagonal,
+// times the quasi-Newton weight (1 here); the transpose (<D_j|H|D_i>) for
+// separate annihilation.
+__device__ __forceinline__ double determ_elem(const Sys& s, const uint64_t* fi, int64_t i, int64_t j, int transp) {
+  uint64_t fj[FQ_MAX_WORDS];
+#pragma unroll
+  for (int w = 0; w < FQ_MAX_WORDS; ++w) fj[w] = w < s.W ? s.determ[w * s.ndeterm + j] : 0;
+  double h = transp ? fqd::hmatel(s, fj, fi) : fqd::hmatel(s, fi, fj);
+  if (i == j) h = h - s.H00;
+  return 1.0 * h;
+}
+constexpr double kDetermEps = 1.0e-12;  // depsilon: smaller elements are not stored
+
+// Non-zero elements per core row: one block per row, threads over columns.
+__global__ __launch_bounds__(kBlock) void k_determ_count(Sys s, int transp, int64_t* __restrict__ cnt) {
+  __shared__ int64_t wsh[kBlock / 64];
+  const int64_t i = blockIdx.x;
+  uint64_t fi[FQ_MAX_WORDS];
+#pragma unroll
+  for (int w = 0; w < FQ_MAX_WORDS; ++w) fi[w] = w < s.W ? s.determ[w * s.ndeterm + i] : 0;
+  int64_t c = 0;
+  for (int64_t j = threadIdx.x; j < s.ndeterm; j += kBlock) c += fabs(determ_elem(s, fi, i, j, transp)) > kDetermEps;
+  c = block_sum(c, wsh);
+  if (threadIdx.x == 0) cnt[i] = c;
+}
+
+// The row's elements in column order from row[i] (block-wide ranks per chunk
+// of kBlock columns).
+__global__ __launch_bounds__(kBlock) void k_determ_fill(Sys s, int transp, const int64_t* __restrict__ row,
+                                                        int32_t* __restrict__ col, double* __restrict__ mat) {
+  __shared__ uint32_t wsh[kBlock / 64];
+  const int64_t i = blockIdx.x;
+  uint64_t fi[FQ_MAX_WORDS];
+#pragma unroll
+  for (int w = 0; w < FQ_MAX_WORDS; ++w) fi[w] = w < s.W ? s.determ[w * s.ndeterm + i] : 0;
+  int64_t base = row[i];
+  for (int64_t j0 = 0; j0 < s.ndeterm; j0 += kBlock) {  // block-uniform trip count
+    const int64_t j = j0 + threadIdx.x;
+    double h = 0.0;
+    bool keep = false;
+    if (j < s.ndeterm) {
+      h = determ_elem(s, fi, i, j, transp);
+      keep = fabs(h) > kDetermEps;
+    }
+    uint32_t tot;
+    const uint32_t off = block_exclusive_u32(keep ? 1u : 0u, wsh, tot);
+    if (keep) {
+      col[base + off] = static_cast<int32_t>(j);
+      mat[base + off] = h;
+    }
+    base += tot;
+  }
+}
+
+// Production deterministic projection, one thread per core determinant and
+// space, each with its own draw keyed (row, space, cycle).  Combined
+// annihilation writes the encoded amplitude to the spawned list after the
+// stochastic spawns (key and payload, as an initiator's spawn; the count is
+// added by k_add_nspawn); separate annihilation adds it to the core
+// determinant's post-death population.
+__global__ __launch_bounds__(kBlock) void k_determ_proj(Sys s, KeyPlan kp, DetermDev dd, double tau, double sh0,
+                                                        double sh1, double e0, double e1, uint64_t seed, uint32_t cyc,
+                                                        int64_t cap, int64_t* __restrict__ pop_new,
+                                                        uint64_t* __restrict__ keys, int64_t* __restrict__ pay,
+                                                        int64_t scap, Stats* __restrict__ st) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= dd.nd * s.S) return;
+  const int64_t row = t / s.S;
+  const int sp = static_cast<int>(t - row * s.S);
+  const double shift = sp == 0 ? sh0 : sh1, eold = sp == 0 ? e0 : e1;
+  fqd::Philox rng(seed, static_cast<uint32_t>(row), static_cast<uint32_t>(sp) << 28, cyc ^ (fqd::kDrawDeterm << 24));
+  if (dd.separate) {
+    pop_new[sp * cap + dd.pos[row]] += fqd::stoch_encode(rng, determ_separate(dd, s.S, row, sp, tau, shift, eold), s.rf);
+    return;
+  }
+  const int64_t k = fqd::stoch_encode(rng, determ_combined(dd, s.S, row, sp, tau, shift, eold), s.rf);
+  const int64_t slot = st->nspawn + t;
+  if (slot >= scap) {
+    st->spawn_overflow = 1;
+    return;
+  }
+  for (int q = 0; q < s.S; ++q) pay[q * scap + slot] = q == sp ? k : 0;
+  uint64_t f[FQ_MAX_WORDS];
+#pragma unroll
+  for (int w = 0; w < FQ_MAX_WORDS; ++w) f[w] = w < s.W ? s.determ[w * dd.nd + row] : 0;
+  uint64_t key[2];
+  make_key(kp, f, s.W, 0, key);
+  for (int w = 0; w < kp.kw; ++w) keys[w * scap + slot] = key[w];
+}
+__global__ void k_add_nspawn(Stats* __restrict__ st, int64_t m) { st->nspawn += m; }
+
+// ---------------------------------------------------------------------------
 // Kernels: merge with the main list, rounding, insertion
 // ---------------------------------------------------------------------------
 // Per surviving run: binary search of its key in the (old) main list; a hit
@@ -2027,11 +2231,22 @@ __device__ __forceinline__ int round_draws(const int64_t* p, int64_t stride, int
   return c;
 }
 
+// A main-list entry of the semi-stochastic core (kept whatever its
+// population, never rounded: remove_unoccupied_dets, src/annihilation.f90:537-600).
+__device__ __forceinline__ bool is_core_entry(const Sys& s, const uint64_t* __restrict__ det, int64_t cap,
+                                              int64_t i) {
+  if (s.ndeterm == 0) return false;
+  uint64_t f[FQ_MAX_WORDS];
+#pragma unroll
+  for (int w = 0; w < FQ_MAX_WORDS; ++w) f[w] = w < s.W ? det[w * cap + i] : 0;
+  return fqd::core_index(s, f) >= 0;
+}
+
 __global__ void k_count_round_main(Sys s, int64_t n, int64_t cap, const int64_t* __restrict__ pop_new,
-                                   int64_t* __restrict__ mdraw) {
+                                   const uint64_t* __restrict__ det, int64_t* __restrict__ mdraw) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  mdraw[i] = s.real_amps ? round_draws(pop_new, cap, i, s.S, s.rf) : 0;
+  mdraw[i] = (s.real_amps && !is_core_entry(s, det, cap, i)) ? round_draws(pop_new, cap, i, s.S, s.rf) : 0;
 }
 __global__ void k_count_round_new(Sys s, int64_t nruns, const int64_t* __restrict__ rpop,
                                   const int64_t* __restrict__ rstate, int64_t scap, int64_t* __restrict__ ndraw) {
@@ -2066,10 +2281,12 @@ __global__ __launch_bounds__(kBlock) void k_round_main(Sys s, int64_t n, int64_t
                                                        int replay, const double* __restrict__ rbuf, int64_t rbase,
                                                        const int64_t* __restrict__ mdraw_off, uint64_t seed,
                                                        uint32_t cyc, uint64_t* __restrict__ mmask,
-                                                       int64_t* __restrict__ mcnt) {
+                                                       int64_t* __restrict__ mcnt, const uint64_t* __restrict__ det) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   bool alive = false;
-  if (i < n) {
+  if (i < n && is_core_entry(s, det, cap, i)) {
+    alive = true;
+  } else if (i < n) {
     if (s.real_amps) {
       if (replay) {
         fqd::ReplayRng rng{rbuf, rbase + mdraw_off[i], rbase + mdraw_off[i + 1], false};
@@ -2543,7 +2760,7 @@ int annihilate(fq_engine* e, int64_t ns, const int64_t* pns, bool replay, uint32
   int64_t main_draws = 0, new_draws = 0;
   if (replay && e->sys.real_amps) {
     hipLaunchKernelGGL(k_count_round_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->pop_new,
-                       e->mdraw);
+                       e->det[c], e->mdraw);
     rc = scan_exclusive(e, e->mdraw, e->mdraw_off, n);
     if (rc != FQ_OK) return rc;
     if (nruns > 0) {
@@ -2559,7 +2776,7 @@ int annihilate(fq_engine* e, int64_t ns, const int64_t* pns, bool replay, uint32
   }
   const uint64_t seed = e->par.philox_seed;
   hipLaunchKernelGGL(k_round_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->pop_new,
-                     replay ? 1 : 0, e->rng_buf, rng_pos, e->mdraw_off, seed, cyc, e->mmask, e->mcnt);
+                     replay ? 1 : 0, e->rng_buf, rng_pos, e->mdraw_off, seed, cyc, e->mmask, e->mcnt, e->det[c]);
   if (nruns > 0)
     hipLaunchKernelGGL(k_round_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, pnr, nruns, e->rpop, e->rstate,
                        scap, replay ? 1 : 0, e->rng_buf, rng_pos + main_draws, e->ndraw_off, seed, cyc, e->nmask,
@@ -3148,6 +3365,24 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   return FQ_OK;
 }
 
+DetermDev determ_dev(const fq_engine* e) {
+  return DetermDev{e->d_drow, e->d_dcol, e->d_dmat, e->d_dvec, e->d_dpos, e->ndeterm, e->determ_separate};
+}
+
+void release_determ(fq_engine* e) {
+  dfree(e->d_determ);
+  dfree(e->d_drow);
+  dfree(e->d_dcol);
+  dfree(e->d_dmat);
+  dfree(e->d_dvec);
+  dfree(e->d_dpos);
+  if (e->sys.ndeterm > 0) e->sys_dirty = true;
+  e->ndeterm = 0;
+  e->determ_nnz = 0;
+  e->sys.determ = nullptr;
+  e->sys.ndeterm = 0;
+}
+
 int fq_load(fq_engine* e, const uint64_t* states, const int64_t* pops, const double* dat, int64_t n) {
   if (!e || n < 0 || (n > 0 && (!states || !pops || !dat))) return FQ_ERR_ARG;
   if (n > e->cap) return FQ_ERR_CAPACITY;
@@ -3168,6 +3403,113 @@ int fq_load(fq_engine* e, const uint64_t* states, const int64_t* pops, const dou
   FQ_HIP(hipStreamSynchronize(e->stream));
   e->n = n;
   e->att_bound = -1;
+  release_determ(e);  // a new list: no semi-stochastic core until fq_semi_stoch_init
+  return FQ_OK;
+}
+
+// init_semi_stoch_t with space "high" (src/semi_stoch.F90:134-377,
+// create_high_pop_space :1198-1316, find_most_populated_dets :1318-1385):
+// the target_size most populated entries of the current list (sum over
+// spaces of |N|).  The reference keeps them with a replace-the-minimum scan
+// (a later entry replaces the first smallest kept one when strictly larger);
+// a min-heap ordered by (population, slot) replaces the same entries.  The
+// core is kept sorted as the main list (every core determinant is in it, so
+// add_determ_dets_to_psip_list inserts nothing) and its Hamiltonian is built
+// on the device (create_determ_hamil :533-685).
+int fq_semi_stoch_init(fq_engine* e, int64_t target_size, int separate) {
+  if (!e || target_size < 0 || e->batched || e->pending) return FQ_ERR_ARG;
+  FQ_HIP(hipSetDevice(e->device));
+  FQ_HIP(hipStreamSynchronize(e->stream));
+  release_determ(e);
+  const int64_t n = e->n, nd = std::min(target_size, n);
+  if (nd == 0) return FQ_OK;
+  if (nd > INT32_MAX) return FQ_ERR_ARG;  // column indices are 32-bit
+  const int W = e->W, S = e->S, c = e->cur;
+  hipStream_t st = e->stream;
+  std::vector<int64_t> pw(static_cast<size_t>(S) * n);
+  std::vector<uint64_t> dw(static_cast<size_t>(W) * n);
+  for (int sp = 0; sp < S; ++sp)
+    FQ_HIP(hipMemcpyAsync(pw.data() + sp * n, e->pop[c] + sp * e->cap, 8 * n, hipMemcpyDeviceToHost, st));
+  for (int w = 0; w < W; ++w)
+    FQ_HIP(hipMemcpyAsync(dw.data() + w * n, e->det[c] + w * e->cap, 8 * n, hipMemcpyDeviceToHost, st));
+  FQ_HIP(hipStreamSynchronize(st));
+  auto absum = [&](int64_t i) {
+    int64_t t = 0;
+    for (int sp = 0; sp < S; ++sp) t += std::llabs(pw[sp * n + i]);
+    return t;
+  };
+  using Slot = std::pair<int64_t, int64_t>;  // (population, slot)
+  std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+  std::vector<int64_t> idx(nd);
+  for (int64_t i = 0; i < nd; ++i) {
+    idx[i] = i;
+    heap.push({absum(i), i});
+  }
+  for (int64_t i = nd; i < n; ++i) {
+    const int64_t a = absum(i);
+    if (a > heap.top().first) {
+      const int64_t slot = heap.top().second;
+      heap.pop();
+      idx[slot] = i;
+      heap.push({a, slot});
+    }
+  }
+  std::sort(idx.begin(), idx.end());  // list order
+  std::vector<uint64_t> core(static_cast<size_t>(W) * nd);
+  for (int w = 0; w < W; ++w)
+    for (int64_t k = 0; k < nd; ++k) core[w * nd + k] = dw[w * n + idx[k]];
+  int rc = dalloc(&e->d_determ, W * nd);
+  rc |= dalloc(&e->d_drow, nd + 1);
+  rc |= dalloc(&e->d_dvec, 2 * nd * S);  // populations, then (separate) the projected amplitudes
+  rc |= dalloc(&e->d_dpos, nd);
+  int64_t* cnt = nullptr;
+  rc |= dalloc(&cnt, nd);
+  if (rc != FQ_OK) {
+    dfree(cnt);
+    release_determ(e);
+    return FQ_ERR_NOMEM;
+  }
+  FQ_HIP(hipMemcpyAsync(e->d_determ, core.data(), 8 * W * nd, hipMemcpyHostToDevice, st));
+  e->sys.determ = e->d_determ;
+  e->sys.ndeterm = nd;
+  e->sys_dirty = true;
+  const int transp = separate ? 1 : 0;
+  hipLaunchKernelGGL(k_determ_count, dim3(static_cast<unsigned>(nd)), dim3(kBlock), 0, st, e->sys, transp, cnt);
+  rc = scan_exclusive(e, cnt, e->d_drow, nd);
+  int64_t nnz = 0;
+  if (rc == FQ_OK) {
+    FQ_HIP(hipMemcpyAsync(&nnz, e->d_drow + nd, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    FQ_HIP(hipStreamSynchronize(st));
+    rc = dalloc(&e->d_dcol, nnz) | dalloc(&e->d_dmat, nnz);
+  }
+  if (rc == FQ_OK) {
+    hipLaunchKernelGGL(k_determ_fill, dim3(static_cast<unsigned>(nd)), dim3(kBlock), 0, st, e->sys, transp,
+                       e->d_drow, e->d_dcol, e->d_dmat);
+    if (hipStreamSynchronize(st) != hipSuccess) rc = FQ_ERR_DEVICE;
+  }
+  dfree(cnt);
+  if (rc != FQ_OK) {
+    release_determ(e);
+    return rc == FQ_ERR_DEVICE ? rc : FQ_ERR_NOMEM;
+  }
+  e->ndeterm = nd;
+  e->determ_nnz = nnz;
+  e->determ_separate = transp;
+  return FQ_OK;
+}
+
+int fq_semi_stoch_info(fq_engine* e, int64_t* ndeterm, int64_t* nnz, uint64_t* states, int64_t capacity) {
+  if (!e || !ndeterm) return FQ_ERR_ARG;
+  *ndeterm = e->ndeterm;
+  if (nnz) *nnz = e->determ_nnz;
+  if (!states || e->ndeterm == 0) return FQ_OK;
+  if (capacity < e->ndeterm) return FQ_ERR_CAPACITY;
+  FQ_HIP(hipSetDevice(e->device));
+  std::vector<uint64_t> dw(static_cast<size_t>(e->W) * e->ndeterm);
+  FQ_HIP(hipMemcpyAsync(dw.data(), e->d_determ, 8 * dw.size(), hipMemcpyDeviceToHost, e->stream));
+  FQ_HIP(hipStreamSynchronize(e->stream));
+  for (int64_t k = 0; k < e->ndeterm; ++k)
+    for (int w = 0; w < e->W; ++w) states[k * e->W + w] = dw[w * e->ndeterm + k];
   return FQ_OK;
 }
 
@@ -3209,7 +3551,7 @@ static int cycle_body(fq_engine* e, const double* shift, const double* eproj_old
   if (replay) {
     hipLaunchKernelGGL(k_replay_walk, dim3(1), dim3(1), 0, st, e->sys, n, cap, e->det[c], e->pop[c], e->kii[c],
                        e->pop_new, tau, shift[0], sh1, eproj_old[0], e1, e->rng_buf, e->rng_pos, e->rng_end, e->scap,
-                       e->sdet, e->srec, e->d_stats);
+                       e->sdet, e->srec, e->d_stats, determ_dev(e));
     FQ_HIP(hipEventRecord(e->pev[1], st));
     FQ_HIP(hipEventRecord(e->pev[2], st));
     FQ_HIP(hipEventRecord(e->pev[3], st));
@@ -3223,7 +3565,7 @@ static int cycle_body(fq_engine* e, const double* shift, const double* eproj_old
     if (n > 0)
       hipLaunchKernelGGL(k_det_pass, dim3(nb), dim3(kBlock), 0, st, e->sys, n, cap, e->det[c], e->pop[c], e->kii[c],
                          e->pop_new, e->nat, e->flags, tau, shift[0], sh1, eproj_old[0], e1, e->par.philox_seed, cyc,
-                         e->part_d, e->part_i);
+                         e->part_d, e->part_i, e->d_dvec, e->d_dpos);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, e->part_d, e->part_i,
                        n > 0 ? static_cast<int>(nb) : 0, e->d_stats);
     FQ_HIP(hipEventRecord(e->pev[1], st));
@@ -3297,6 +3639,17 @@ static int cycle_body(fq_engine* e, const double* shift, const double* eproj_old
     };
     rc = spawn_range(0, bound);
     if (rc != FQ_OK) return rc;
+    int64_t ndet_entries = 0;  // the deterministic projection's spawned entries (combined annihilation)
+    if (e->ndeterm > 0) {
+      const int64_t m = e->ndeterm * e->S;
+      hipLaunchKernelGGL(k_determ_proj, dim3(grid_for(m)), dim3(kBlock), 0, st, e->sys, key_plan(e), determ_dev(e),
+                         tau, shift[0], sh1, eproj_old[0], e1, e->par.philox_seed, cyc, cap, e->pop_new,
+                         e->kbuf[0], e->pbuf[0], e->scap, e->d_stats);
+      if (!e->determ_separate) {
+        hipLaunchKernelGGL(k_add_nspawn, dim3(1), dim3(1), 0, st, e->d_stats, m);
+        ndet_entries = m;
+      }
+    }
     FQ_HIP(hipEventRecord(e->pev[3], st));
     e->keys_ready = true;  // the packing kernels wrote the sort keys
     // The spawned-list length and the attempt total are copied to pinned host
@@ -3307,7 +3660,7 @@ static int cycle_body(fq_engine* e, const double* shift, const double* eproj_old
     FQ_HIP(hipMemcpyAsync(e->h_len, &e->d_stats->nspawn, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     FQ_HIP(hipEventRecord(e->ev_len, st));
     e->spawn_bound = bound;
-    ns = std::min<int64_t>(bound, e->scap);
+    ns = std::min<int64_t>(bound + ndet_entries, e->scap);
   }
   FQ_HIP(hipEventRecord(e->ev[1], st));
   int rc = annihilate(e, ns, replay ? nullptr : &e->d_stats->nspawn, replay, cyc, rng_pos);
@@ -3380,7 +3733,7 @@ int fq_cycle_spawn(fq_engine* e, const double* shift, const double* eproj_old, d
                      static_cast<int64_t>(sizeof(Stats) / sizeof(int64_t)));
   hipLaunchKernelGGL(k_replay_walk, dim3(1), dim3(1), 0, st, e->sys, e->n, e->cap, e->det[c], e->pop[c], e->kii[c],
                      e->pop_new, tau, shift[0], sh1, eproj_old[0], e1, e->rng_buf, e->rng_pos, e->rng_end, e->scap,
-                     e->sdet, e->srec, e->d_stats);
+                     e->sdet, e->srec, e->d_stats, determ_dev(e));
   int rc = sync_stats(e);
   if (rc != FQ_OK) return rc;
   if (e->h_stats.rng_failed) return FQ_ERR_RNG_EXHAUSTED;
@@ -3955,6 +4308,7 @@ void fq_destroy(fq_engine* e) {
   dfree(e->nat);
   dfree(e->att_off);
   dfree(e->flags);
+  release_determ(e);
   dfree(e->mmask);
   dfree(e->mcnt);
   dfree(e->moff);

@@ -167,6 +167,16 @@ extern "C" int fqh_fciqmc_restart(const fqh_system* hs, const fqh_qmc_in* in, co
   emit(cycles0, 0, 0.0, 0.0);
   bool overflowed = false;
   int reports_done = 0;
+  // the iteration the semi-stochastic core is built (src/fciqmc.f90:227-235):
+  // start_iter, or shift_iter iterations after the shift starts varying (at
+  // once if it already varies)
+  int64_t ss_iter = std::max<int64_t>(in->semi_stoch_start_iter, cycles0 + 1);
+  auto all_vary = [&]() {
+    for (int s = 0; s < S; ++s)
+      if (!ctl.vary[s]) return false;
+    return true;
+  };
+  if (all_vary() && in->semi_stoch_shift_iter != -1) ss_iter = cycles0 + 1;
   for (int ireport = 1; ireport <= in->nreports && nr < max_rows; ++ireport) {  // a full table ends the run
     double eproj_old[FQ_MAX_SPACES] = {0, 0};
     for (int s = 0; s < S; ++s) eproj_old[s] = (std::fabs(d0[s]) < 2.2250738585072014e-308) ? 0.0 : proj[s] / d0[s];
@@ -176,6 +186,11 @@ extern "C" int fqh_fciqmc_restart(const fqh_system* hs, const fqh_qmc_in* in, co
     bool error = false;
     const auto t0 = std::chrono::steady_clock::now();
     for (int ic = 0; ic < in->mc_cycles; ++ic) {
+      const int64_t iter = cycles0 + static_cast<int64_t>(ireport - 1) * in->mc_cycles + ic + 1;
+      if (in->semi_stoch_size > 0 && iter == ss_iter) {
+        rc = fq_semi_stoch_init(e, in->semi_stoch_size, in->semi_stoch_separate);
+        if (rc != FQ_OK) break;
+      }
       const int64_t nattempts = static_cast<int64_t>(std::llround(2 * nparticles[0]));
       if (replay) {
         double wsum = 0.0;
@@ -212,7 +227,10 @@ extern "C" int fqh_fciqmc_restart(const fqh_system* hs, const fqh_qmc_in* in, co
       d0[s] = d0[s] / in->mc_cycles;
     }
     rspawn = rspawn / in->mc_cycles;
+    const bool vary_before = all_vary();
     const bool soft_exit = ctl.end_report(ntot, proj, d0);
+    if (!vary_before && all_vary() && in->semi_stoch_shift_iter != -1)  // end_report_loop, src/qmc_common.F90:1203
+      ss_iter = in->semi_stoch_shift_iter + cycles0 + static_cast<int64_t>(ireport) * in->mc_cycles + 1;
     emit(cycles0 + static_cast<int64_t>(ireport) * in->mc_cycles, nspawn_events, rspawn, tcyc);
     ctl.after_report(ireport, proj[0], d0[0]);
     if (nr > 0 && nr <= max_rows) {  // the damping and blocked estimate once this report is processed
@@ -309,7 +327,7 @@ extern "C" int fqh_fciqmc_ranks(const fqh_system* hs, const fqh_qmc_in* in, int 
                                 int max_rows, int* nrows) {
   if (!hs || !in || !rows || !nrows || nprocs < 1) return FQ_ERR_ARG;
   if (nprocs == 1) return fqh_fciqmc(hs, in, rows, max_rows, nrows);
-  if (in->rng_mode != FQ_RNG_REPLAY) return FQ_ERR_ARG;
+  if (in->rng_mode != FQ_RNG_REPLAY || in->semi_stoch_size > 0) return FQ_ERR_ARG;  // semi-stochastic: one rank
   const fq_system* sys = fqh_view(hs);
   fq_params p{};
   p.nspaces = in->replica_tricks ? 2 : 1;

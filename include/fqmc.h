@@ -149,6 +149,23 @@ int fq_create(const fq_system *sys, const fq_params *params, fq_engine **out);
  * restart).  states: nwords*n, pops: nspaces*n, dat: n (K_ii). */
 int fq_load(fq_engine *e, const uint64_t *states, const int64_t *pops, const double *dat, int64_t n);
 
+/* Semi-stochastic projection (init_semi_stoch_t, src/semi_stoch.F90:134-377,
+ * space "high"): make the target_size most populated determinants of the
+ * current list (sum over spaces of |N|, chosen as find_most_populated_dets
+ * does, :1318-1385) the deterministic core, and build the Hamiltonian among
+ * them on the device (create_determ_hamil, :533-685).  From the next cycle the
+ * core is projected exactly (determ_projection, :859-1094): separate = 1 adds
+ * the encoded result to the core populations (separate annihilation, the
+ * reference's default), 0 appends it to the spawned list (combined
+ * annihilation); core parents are initiators, their spawns onto the core and
+ * their deaths are left to the projection, and core determinants stay in the
+ * list whatever their population.  target_size 0 switches it off; fq_load
+ * also clears the core.  Not for batched engines or during a split cycle. */
+int fq_semi_stoch_init(fq_engine *e, int64_t target_size, int separate);
+/* The core's size, its Hamiltonian's stored elements (nnz, may be NULL) and,
+ * if states is not NULL, its determinants (list order, nwords each). */
+int fq_semi_stoch_info(fq_engine *e, int64_t *ndeterm, int64_t *nnz, uint64_t *states, int64_t capacity);
+
 /* Append draws to the device replay stream (FQ_RNG_REPLAY only).  The host
  * generates them with dSFMT exactly as get_rand_close_open hands them out
  * (lib/dSFMT_F03_interface/dSFMT_interface.F90:387-409). */

@@ -133,7 +133,8 @@ int fqh_fciqmc(const fqh_system *s, const fqh_qmc_in *in, fqh_report_row *rows, 
  * determinants that hash to it; each cycle's spawned entries are exchanged on
  * the host as comm_spawn_t does (src/spawn_data.F90:624-739) and the report
  * columns are the ranks' sums, as after the reference's MPI_Allreduce
- * (src/energy_evaluation.F90:126-201).  nprocs = 1 is fqh_fciqmc. */
+ * (src/energy_evaluation.F90:126-201).  nprocs = 1 is fqh_fciqmc; a
+ * semi-stochastic run (semi_stoch_size > 0) needs nprocs = 1. */
 int fqh_fciqmc_ranks(const fqh_system *s, const fqh_qmc_in *in, int nprocs, fqh_report_row *rows, int max_rows,
                      int *nrows);
 

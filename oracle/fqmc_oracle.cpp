@@ -194,9 +194,19 @@ class Oracle {
       for (int64_t i = 0; i < n; ++i) acc = acc + std::fabs(static_cast<double>(pops[i * S + s]) / rf);
       nparticles[s] = acc;
     }
+    determ_ = DetermSpace{};  // a new list has no semi-stochastic core
   }
 
   uint64_t rng_consumed() const { return rng_.consumed(); }
+  // the core's determinants (list order) and its stored Hamiltonian's size
+  int64_t determ_info(uint64_t* st, int64_t* nnz) const {
+    const int64_t D = determ_size();
+    if (nnz) *nnz = D ? static_cast<int64_t>(determ_.mat.size()) : 0;
+    if (st)
+      for (int64_t k = 0; k < D; ++k)
+        for (int w = 0; w < W; ++w) st[k * W + w] = determ_.dets[k].f[w];
+    return D;
+  }
 
   void download(uint64_t* st, int64_t* p, double* d) const {
     for (size_t i = 0; i < states.size(); ++i) {
@@ -300,6 +310,10 @@ class Oracle {
   void semi_stoch_init(int target_size, bool separate) {
     const int64_t n = static_cast<int64_t>(states.size());
     const int64_t nd = std::min<int64_t>(target_size, n);
+    if (nd <= 0) {  // an empty core: off
+      determ_ = DetermSpace{};
+      return;
+    }
     auto absum = [&](int64_t i) {
       int64_t t = 0;
       for (int s = 0; s < S; ++s) t += std::llabs(pops[i * S + s]);
@@ -1965,6 +1979,14 @@ uint64_t orc_rng_consumed(void* h) { return static_cast<Oracle*>(h)->rng_consume
 void orc_projected_energy(void* h, double* proj, double* d0) { static_cast<Oracle*>(h)->projected_energy(proj, d0); }
 double orc_nparticles(void* h, int s) { return static_cast<Oracle*>(h)->nparticles[s]; }
 void orc_set_stable_order(void* h, int on) { static_cast<Oracle*>(h)->stable_order = on != 0; }
+// Semi-stochastic core from the current list (fq_semi_stoch_init) and its description.
+int orc_semi_stoch_init(void* h, int64_t target_size, int separate) {
+  static_cast<Oracle*>(h)->semi_stoch_init(static_cast<int>(target_size), separate != 0);
+  return 0;
+}
+int64_t orc_semi_stoch_info(void* h, uint64_t* states, int64_t* nnz) {
+  return static_cast<Oracle*>(h)->determ_info(states, nnz);
+}
 
 // Heat-bath probes for the tests.
 int orc_hb_biased(void* h) { return static_cast<Oracle*>(h)->hb.biased ? 1 : 0; }

@@ -62,6 +62,9 @@ def oracle():
     lib.orc_alias_tables.argtypes = [C.c_int, P(C.c_double), C.c_double, P(C.c_double), P(C.c_int)]
     lib.orc_hb_generate.restype = C.c_int64
     lib.orc_hb_generate.argtypes = [vp, P(C.c_uint64), P(C.c_double), C.c_int64, P(C.c_int), P(C.c_double)]
+    lib.orc_semi_stoch_init.argtypes = [vp, C.c_int64, C.c_int]
+    lib.orc_semi_stoch_info.restype = C.c_int64
+    lib.orc_semi_stoch_info.argtypes = [vp, P(C.c_uint64), P(C.c_int64)]
     return lib
 
 
@@ -212,14 +215,16 @@ def lockstep(system, st, ncycles, init_pop=None, shift=0.0, label=""):
         orc.orc_destroy(h)
 
 
-def production_lockstep(system, p, ncycles, init_pop, shift=0.0, tau=0.01, label="", every=50):
+def production_lockstep(system, p, ncycles, init_pop, shift=0.0, tau=0.01, label="", every=50, core=None):
     """Run the engine in production mode (Philox) and the oracle in its
     production mode with the same seed side by side from init_pop walkers on
     the reference determinant.  Per cycle: spawn/death/attempt counts, list
     length, the walker lists (determinants, populations, K_ii) and the
     reference population must be identical; the projected energy and the
     particle totals (summed in a different order on the GPU) agree to 1e-12
-    relative.  Returns the list of cycle outputs."""
+    relative.  core = (cycle, size, separate) makes a semi-stochastic core
+    before that cycle on both (same determinants, same stored element count).
+    Returns the list of cycle outputs."""
     lib = N.lib()
     orc = oracle()
     W, S = system.view.nwords, p.nspaces
@@ -240,6 +245,22 @@ def production_lockstep(system, p, ncycles, init_pop, shift=0.0, tau=0.01, label
         sh = (C.c_double * 2)(shift, shift)
         eold = (C.c_double * 2)(0.0, 0.0)
         for cyc in range(ncycles):
+            if core is not None and cyc == core[0]:
+                N.check(lib.fq_semi_stoch_init(eng, core[1], core[2]), "fq_semi_stoch_init")
+                orc.orc_semi_stoch_init(h, core[1], core[2])
+                nd, nnz = C.c_int64(), C.c_int64()
+                N.check(lib.fq_semi_stoch_info(eng, C.byref(nd), C.byref(nnz), None, 0), "fq_semi_stoch_info")
+                g_states = np.zeros(max(nd.value, 1) * W, dtype=np.uint64)
+                N.check(lib.fq_semi_stoch_info(eng, C.byref(nd), None,
+                                               g_states.ctypes.data_as(C.POINTER(C.c_uint64)), nd.value),
+                        "fq_semi_stoch_info")
+                o_states = np.zeros(max(nd.value, 1) * W, dtype=np.uint64)
+                o_nnz = C.c_int64()
+                assert orc.orc_semi_stoch_info(h, o_states.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                               C.byref(o_nnz)) == nd.value
+                assert nd.value == min(core[1], o1.nstates if cyc else 1) and nnz.value == o_nnz.value
+                assert np.array_equal(g_states, o_states)
+                print(f"  {label} core: {nd.value} determinants, {nnz.value} stored elements", flush=True)
             o1, o2 = N.FqCycleOut(), N.FqCycleOut()
             N.check(lib.fq_cycle(eng, sh, eold, tau, C.byref(o1)), "fq_cycle")
             orc.orc_cycle(h, sh, eold, tau, C.byref(o2))
