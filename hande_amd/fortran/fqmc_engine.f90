@@ -126,6 +126,15 @@ module fqmc_engine
             integer(c_int32_t) :: rc
         end function fq_download
 
+        ! init_semi_stoch_t for space 'high' (separate = 1: separate annihilation)
+        function fq_semi_stoch_init(engine, target_size, separate) result(rc) bind(c, name='fq_semi_stoch_init')
+            import :: c_ptr, c_int64_t, c_int32_t
+            type(c_ptr), value :: engine
+            integer(c_int64_t), value :: target_size
+            integer(c_int32_t), value :: separate
+            integer(c_int32_t) :: rc
+        end function fq_semi_stoch_init
+
         subroutine fq_destroy(engine) bind(c, name='fq_destroy')
             import :: c_ptr
             type(c_ptr), value :: engine
