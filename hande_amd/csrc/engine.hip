@@ -1764,27 +1764,20 @@ __global__ __launch_bounds__(kBlock) void k_run_count(const uint64_t* __restrict
   }
 }
 
-// Segmented scan element: f = a run head lies in the span, v = the sums
-// since the last head in it (or over the whole span).  a precedes b.
-template <int NV>
-__device__ __forceinline__ void seg_combine(int fa, const int64_t* va, int& fb, int64_t* vb) {
-  if (!fb) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) vb[k] += va[k];
-  }
-  fb |= fa;
-}
+// Runs longer than this are reduced by the whole block (a queue in LDS), so a
+// determinant spawned onto thousands of times does not hold one lane while
+// its wavefront idles; shorter ones by one thread each.
+constexpr int kShortRun = 32;
 
 // annihilate_spawn_t(_initiator) (src/spawn_data.F90:859-1101) over the
-// sorted keys and populations, one block per tile of kRunTile keys.  The tile
-// is staged in LDS; thread t takes entries [R t, R t + R), forms each entry's
-// share (entry_share), and a segmented scan over the block's threads gives
-// every run ending in the tile its totals whatever its length (a run no
-// longer falls to one thread).  A run is written by the thread holding its
-// last entry, if its head lies in the tile; the tile's last run adds the
-// partial sums of the following tiles (k_run_count) up to the next head.  Run
-// index = the tile's run offset (toff) + the head's rank in the tile; the
-// determinant is recovered from the key by unranking.  The run's first entry
+// sorted keys and populations, one block per tile of kRunTile keys.  The
+// tile's run heads are listed in order (wavefront ballots); thread h takes run
+// h, sums its entries' shares (entry_share) over consecutive keys and writes
+// the run -- its index is the tile's run offset (toff) + h, its determinant is
+// recovered from the key by unranking -- so every lane has a run whatever the
+// run lengths.  Runs longer than kShortRun are queued and reduced by the whole
+// block afterwards.  The tile's last run adds the head-free prefixes of the
+// following tiles (k_run_count) up to the next head.  The run's first entry
 // is its first in sorted order: the reference's quicksort order in replay
 // mode, the earliest spawned in production (the stable sort of an
 // attempt-ordered list).
@@ -1796,13 +1789,11 @@ __global__ __launch_bounds__(kBlock) void k_runs(Sys s, KeyPlan kp, const uint64
                                                  uint64_t* __restrict__ rdet, int64_t* __restrict__ rpop,
                                                  int64_t* __restrict__ rflag, int64_t* __restrict__ rstate) {
   constexpr int R = kRunTile / kBlock, NV = 3 * S, NW = kBlock / 64;
-  __shared__ uint64_t sk[KW][kRunTile];
-  __shared__ int64_t spop[S][kRunTile];
-  __shared__ uint64_t rkey[KW][kRunTile];  // each run's key, by its index in the tile
-  __shared__ uint64_t prevk[KW];
-  __shared__ int64_t wv[NW][NV];
-  __shared__ int32_t wf[NW];
-  __shared__ uint32_t wsh[NW];
+  __shared__ int32_t wcount[R][NW];
+  __shared__ int32_t hpos[kRunTile + 1];  // tile-relative index of each head, in order; then the tile length
+  __shared__ int32_t longq[kRunTile];     // runs (by h) left to the whole block
+  __shared__ int32_t nlong;
+  __shared__ int64_t wsh[NW];
   extern __shared__ uint64_t binom_lds[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t tile = blockIdx.x, t0 = tile * kRunTile;
@@ -1815,147 +1806,70 @@ __global__ __launch_bounds__(kBlock) void k_runs(Sys s, KeyPlan kp, const uint64
   const int n = static_cast<int>(ns - t0 < kRunTile ? ns - t0 : kRunTile);
   const int fb = kp.flag_bits;
   const uint64_t fmask = (uint64_t{1} << fb) - 1;
+  bool head[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {  // coalesced staging
+  for (int r = 0; r < R; ++r) {
     const int j = r * kBlock + tid;
+    const int64_t i = t0 + j;
+    head[r] = false;
     if (j < n) {
-#pragma unroll
-      for (int w = 0; w < KW; ++w) sk[w][j] = keys[w * stride + t0 + j];
-#pragma unroll
-      for (int q = 0; q < S; ++q) spop[q][j] = pay[q * stride + t0 + j];
-    }
-  }
-  if (tid < KW) prevk[tid] = t0 > 0 ? keys[tid * stride + t0 - 1] : 0;
-  __syncthreads();  // also orders the table's staging
-  // this thread's entries: head flags, shares, head count, sums since its last head
-  bool hd[R];
-  int64_t cv[R][NV], agg[NV];
-  int hc = 0, any = 0;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) agg[k] = 0;
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int j = tid * R + q;
-    hd[q] = false;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) cv[q][k] = 0;
-    if (j < n) {
-      uint64_t k0[KW], kp0[KW];
+      uint64_t k[KW], prev[KW];
 #pragma unroll
       for (int w = 0; w < KW; ++w) {
-        k0[w] = sk[w][j];
-        kp0[w] = j > 0 ? sk[w][j - 1] : prevk[w];
+        k[w] = keys[w * stride + i];
+        prev[w] = i > 0 ? keys[w * stride + i - 1] : 0;
       }
-      hd[q] = (t0 + j == 0) || !same_value<KW>(k0, kp0, fb);
-      int64_t pop[S];
-#pragma unroll
-      for (int sp = 0; sp < S; ++sp) pop[sp] = spop[sp][j];
-      entry_share<S>(static_cast<int64_t>(k0[0] & fmask), pop, hd[q], cv[q]);
-      if (hd[q]) {
-        ++hc;
-        any = 1;
-#pragma unroll
-        for (int k = 0; k < NV; ++k) agg[k] = cv[q][k];
-      } else {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) agg[k] += cv[q][k];
-      }
+      head[r] = i == 0 || !same_value<KW>(k, prev, fb);
     }
   }
-  // segmented inclusive scan over the wavefront, then over the waves
-  int f = any;
-  int64_t v[NV];
+  unsigned long long bal[R];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = agg[k];
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int fo = __shfl_up(f, d, 64);
-    int64_t vo[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) vo[k] = __shfl_up(v[k], d, 64);
-    if (lane >= d) seg_combine<NV>(fo, vo, f, v);
+  for (int r = 0; r < R; ++r) {
+    bal[r] = __ballot(head[r]);
+    if (lane == 0) wcount[r][wave] = __popcll(bal[r]);
   }
-  if (lane == 63) {
-    wf[wave] = f;
+  if (tid == 0) nlong = 0;
+  __syncthreads();
+  int32_t total = 0;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) wv[wave][k] = v[k];
-  }
-  // exclusive within the wave: the inclusive value of the lane before
-  int ef = __shfl_up(f, 1, 64);
-  int64_t ev[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) ev[k] = __shfl_up(v[k], 1, 64);
-  if (lane == 0) {
-    ef = 0;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) ev[k] = 0;
-  }
-  uint32_t htot;
-  const uint32_t hbase = block_exclusive_u32(static_cast<uint32_t>(hc), wsh, htot);  // barriers publish wf, wv
-  {
-    int pf = 0;  // the waves before this one, folded in order
-    int64_t pv[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) pv[k] = 0;
-    for (int u = 0; u < wave; ++u) {
-      int64_t tv[NV];
-#pragma unroll
-      for (int k = 0; k < NV; ++k) tv[k] = wv[u][k];
-      int tf = wf[u];
-      seg_combine<NV>(pf, pv, tf, tv);
-      pf = tf;
-#pragma unroll
-      for (int k = 0; k < NV; ++k) pv[k] = tv[k];
+  for (int r = 0; r < R; ++r) {
+    int32_t p = total + __popcll(bal[r] & ((1ull << lane) - 1));
+    for (int w = 0; w < NW; ++w) {
+      if (w < wave) p += wcount[r][w];
+      total += wcount[r][w];
     }
-    seg_combine<NV>(pf, pv, ef, ev);  // ef, ev: everything in the tile before this thread's entries
+    if (head[r]) hpos[p] = r * kBlock + tid;
   }
-  // walk the entries again: a run ends where the next entry heads another
+  if (tid == 0) hpos[total] = n;
+  __syncthreads();  // also orders the table's staging
   const int64_t rbase = toff[tile];
-  int owned = ef;  // the current run's head lies in this tile
-  int64_t run_sum[NV];
+  // the shares of run h's entries [a, b) of the tile, from index `from` (stride `step`)
+  auto shares = [&](int a, int b, int from, int step, int64_t* acc) {
+    for (int j = from; j < b; j += step) {
+      int64_t pop[S], cv[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) run_sum[k] = ev[k];
-  int64_t hr = static_cast<int64_t>(hbase) - 1;
+      for (int q = 0; q < S; ++q) pop[q] = pay[q * stride + t0 + j];
+      entry_share<S>(static_cast<int64_t>(keys[t0 + j] & fmask), pop, j == a, cv);
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int j = tid * R + q;
-    if (j >= n) break;
-    if (hd[q]) {
-      owned = 1;
-      ++hr;
-#pragma unroll
-      for (int k = 0; k < NV; ++k) run_sum[k] = cv[q][k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < NV; ++k) run_sum[k] += cv[q][k];
+      for (int k = 0; k < NV; ++k) acc[k] += cv[k];
     }
-    uint64_t kk[KW];
+  };
+  // run h's totals complete: the continuation of the tile's last run, the
+  // initiator rules, and the run's record
+  auto finish = [&](int h, int64_t* acc) {
+    if (h == total - 1 && t0 + n < ns && tfirst[tile + 1] != 0) {
+      for (int64_t u = tile + 1; u < ntiles; ++u) {
 #pragma unroll
-    for (int w = 0; w < KW; ++w) kk[w] = sk[w][j];
-    bool end;
-    if (j + 1 < n) {
-      uint64_t kn[KW];
-#pragma unroll
-      for (int w = 0; w < KW; ++w) kn[w] = sk[w][j + 1];
-      end = !same_value<KW>(kn, kk, fb);
-    } else {
-      end = true;
-      if (t0 + n < ns && owned && tfirst[tile + 1] != 0) {
-        // the tile's last run continues: add the following tiles' parts up to the next head
-        for (int64_t u = tile + 1; u < ntiles; ++u) {
-#pragma unroll
-          for (int k = 0; k < NV; ++k) run_sum[k] += tpre[u * NV + k];
-          if (tfirst[u] >= 0) break;
-        }
+        for (int k = 0; k < NV; ++k) acc[k] += tpre[u * NV + k];
+        if (tfirst[u] >= 0) break;
       }
     }
-    if (!end || !owned) continue;
-    const int64_t run = rbase + hr;
+    const int64_t run = rbase + h;
     int64_t fl = 0;
     bool nz = false;
 #pragma unroll
     for (int sp = 0; sp < S; ++sp) {
-      const int64_t tot = run_sum[sp], init = run_sum[S + sp], evs = run_sum[2 * S + sp];
+      const int64_t tot = acc[sp], init = acc[S + sp], evs = acc[2 * S + sp];
       nz = nz || tot != 0;
       if (s.initiator) {
         const int st = tot >= 0 ? 1 : -1, si = init >= 0 ? 1 : -1;
@@ -1967,18 +1881,16 @@ __global__ __launch_bounds__(kBlock) void k_runs(Sys s, KeyPlan kp, const uint64
       }
       rpop[sp * stride + run] = tot;
     }
-#pragma unroll
-    for (int w = 0; w < KW; ++w) rkey[w][hr] = kk[w];
     rflag[run] = fl;
     rstate[run] = nz ? 1 : 0;
-  }
-  // the runs' determinants, every lane busy: unranked from their keys
-  __syncthreads();
-  for (int h = tid; h < static_cast<int>(htot); h += kBlock) {
+    const int64_t i = t0 + hpos[h];
+    uint64_t kk[KW];
+#pragma unroll
+    for (int w = 0; w < KW; ++w) kk[w] = keys[w * stride + i];
     uint64_t d[2] = {0, 0};
-    const unsigned __int128 val = (KW == 1 ? static_cast<unsigned __int128>(rkey[0][h])
-                                           : (static_cast<unsigned __int128>(rkey[KW - 1][h]) << 64) | rkey[0][h]) >>
-                                  fb;
+    const unsigned __int128 val =
+        (KW == 1 ? static_cast<unsigned __int128>(kk[0])
+                 : (static_cast<unsigned __int128>(kk[KW - 1]) << 64) | kk[0]) >> fb;
     if (!kp.rank) {
       d[0] = static_cast<uint64_t>(val);  // batched instances: the encoded word itself
     } else if (staged) {
@@ -1986,7 +1898,32 @@ __global__ __launch_bounds__(kBlock) void k_runs(Sys s, KeyPlan kp, const uint64
     } else {
       unrank<WIDE>(kp, kp.binom, val, d);
     }
-    FQ_FOR_WORDS(w, s.W) rdet[w * stride + rbase + h] = d[w];
+    FQ_FOR_WORDS(w, s.W) rdet[w * stride + run] = d[w];
+  };
+  for (int h = tid; h < total; h += kBlock) {
+    const int a = hpos[h], b = hpos[h + 1];
+    if (b - a > kShortRun) {
+      longq[atomicAdd(&nlong, 1)] = h;
+      continue;
+    }
+    int64_t acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0;
+    shares(a, b, a, 1, acc);
+    finish(h, acc);
+  }
+  __syncthreads();
+  const int nq = nlong;
+  for (int q = 0; q < nq; ++q) {  // block-uniform
+    const int h = longq[q];
+    const int a = hpos[h], b = hpos[h + 1];
+    int64_t acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0;
+    shares(a, b, a + tid, kBlock, acc);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = block_sum(acc[k], wsh);
+    if (tid == 0) finish(h, acc);
   }
 }
 

@@ -9,7 +9,7 @@ src/excit_gen_heat_bath_mol.F90).
   excitation is rejected at creation, as the reference stops there.
 * Production mode: the mean projected energy of eight independent runs of a
   synthetic system agrees with its exact ground-state energy (dense
-  diagonalisation) within 4 standard errors, for the heat-bath and the
+  diagonalisation) within 2 standard errors, for the heat-bath and the
   renormalised generators.  The system is chosen
   with its ground state 95% on the reference determinant: the synthetic
   integrals are not Hartree--Fock canonical, so single excitations enter the
@@ -63,8 +63,8 @@ def test_heat_bath_biased_system_rejected(tmp_path):
 @pytest.mark.parametrize("excit_gen", ["heat_bath", "renorm"])
 def test_molecular_projected_energy_matches_exact(tmp_path, excit_gen):
     """Production mode: eight independent runs (Philox seeds); their mean
-    projected energy and shift within 4 standard errors (t, 7 degrees of
-    freedom) of the dense diagonalisation."""
+    projected energy and shift within 2 standard errors of the dense
+    diagonalisation."""
     norb, nel, seed = 8, 4, 600
     exact, _hf, weight = exact_small.ground_state(norb, nel, seed, with_weight=True)
     assert weight > 0.5  # E_proj needs a ground state with a large reference weight
@@ -77,8 +77,8 @@ def test_molecular_projected_energy_matches_exact(tmp_path, excit_gen):
     # the target: the first 300 reports are left out
     e, se, sh, sse = replicate_estimates(system, q, range(91, 99), skip=300)
     print(f"  E_proj={e:.6f} +- {se:.6f}, shift={sh:.6f} +- {sse:.6f}, exact {exact:.6f}", flush=True)
-    assert abs(e - exact) < 4 * se + 1e-4
-    assert abs(sh - exact) < 4 * sse + 5e-3
+    assert abs(e - exact) < 2 * se + 1e-4
+    assert abs(sh - exact) < 2 * sse + 5e-3
 
 
 @pytest.mark.parametrize("excit_gen", ["renorm", "heat_bath"])
@@ -86,7 +86,7 @@ def test_two_word_projected_energy_matches_exact(tmp_path, excit_gen):
     """Production mode with determinants of two 64-bit words (34 spatial
     orbitals, 2 electrons; weakly correlated integrals so the ground state is
     89% on the reference): the mean E_proj of eight independent runs within
-    4 standard errors of the dense diagonalisation."""
+    2 standard errors of the dense diagonalisation."""
     norb, nel, seed, cs = 34, 2, 3400, 0.03
     exact, _hf, weight = exact_small.ground_state(norb, nel, seed, with_weight=True, chol_scale=cs)
     assert weight > 0.5
@@ -98,4 +98,4 @@ def test_two_word_projected_energy_matches_exact(tmp_path, excit_gen):
              shift_damping=0.05, state_size=100000, spawned_state_size=100000, excit_gen=excit_gen)
     e, se, _sh, _sse = replicate_estimates(system, q, range(31, 39))
     print(f"  E_proj={e:.6f} +- {se:.6f} (8 runs), exact {exact:.6f}", flush=True)
-    assert abs(e - exact) < 4 * se + 1e-4
+    assert abs(e - exact) < 2 * se + 1e-4

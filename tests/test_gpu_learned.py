@@ -13,7 +13,7 @@
   exactly over every double from the weights, drops.
 * Production runs with the learned generator reproduce the exact ground
   state energy of a small synthetic system: the mean of eight independent
-  runs within 4 standard errors."""
+  runs within 2 standard errors."""
 import ctypes as C
 import os
 import sys
@@ -357,4 +357,4 @@ def test_learned_projected_energy_matches_exact(tmp_path, min_attempts):
              proposal_min_attempts=min_attempts)
     e, se, _sh, _sse = replicate_estimates(system, q, range(99, 107))
     print(f"  E_proj={e:.6f} +- {se:.6f} (8 runs), exact {exact:.6f}", flush=True)
-    assert abs(e - exact) < 4 * se + 1e-4
+    assert abs(e - exact) < 2 * se + 1e-4

@@ -3,7 +3,10 @@
 * Bit-reproducible for a given seed (no atomics order on the hot path),
   different for a different seed.
 * Reblocked projected energy agrees with the reference's exact FCI
-  eigenvalue for the same system (test_suite/fci goldens) within 3 sigma."""
+  eigenvalue for the same system (test_suite/fci goldens) within 2 sigma
+  (plus 1e-4 Ha for the population-control bias of a 3000-walker run, which
+  the 2x2 lattice's very small reblocked error, ~1e-5, would otherwise
+  expose; 5e-3 Ha for the shift)."""
 import numpy as np
 import pytest
 
@@ -45,5 +48,5 @@ def test_projected_energy_matches_fci(fci, deck, initiator):
     print(f"{fci} initiator={initiator}: E_proj={e:.6f} +- {out['error']:.6f} (level {out['level']}), "
           f"shift={system.H00 + sh['mean']:.6f} +- {sh['error']:.6f}, exact {exact:.6f}")
     assert out["converged"] and sh["converged"], "reblocking found no optimal block"
-    assert abs(e - exact) < 3 * out["error"] + 1e-4
-    assert abs(system.H00 + sh["mean"] - exact) < 3 * sh["error"] + 5e-3
+    assert abs(e - exact) < 2 * out["error"] + 1e-4
+    assert abs(system.H00 + sh["mean"] - exact) < 2 * sh["error"] + 5e-3
