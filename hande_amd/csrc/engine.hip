@@ -1627,27 +1627,30 @@ __device__ __forceinline__ bool same_value(const uint64_t* a, const uint64_t* b,
   else return a[1] == b[1] && (a[0] >> low_bits) == (b[0] >> low_bits);
 }
 
-// The determinant of a combinatorial rank: electron by electron from the
-// highest, the largest p with C(p, i) <= the remaining rank is the i-th
-// lowest set bit.  The highest is found by a binary search (7 steps cover 128
-// orbitals), the others by stepping down from the one above (p only
-// decreases; low-lying determinants have small gaps).  binom: the table (in
-// LDS when staged).
+// The determinant of a combinatorial rank, in one pass down the orbitals from
+// the highest occupied one (a binary search; 7 steps cover 128 orbitals): at
+// orbital p with i electrons left, p is occupied iff C(p, i) <= the remaining
+// rank, which then drops by C(p, i).  One table read per orbital and no
+// nested loop keep a wavefront's lanes together (stepping electron by
+// electron lets each lane's gaps serialise the wavefront).  binom: the table
+// (in LDS when staged); sat: it holds C(p, i) saturated at 2^64 - 1, one word
+// each (ranks below 2^64), instead of (low, high) pairs.
 template <bool WIDE>
-__device__ __forceinline__ void unrank(const KeyPlan& kp, const uint64_t* __restrict__ binom,
+__device__ __forceinline__ void unrank(const KeyPlan& kp, const uint64_t* __restrict__ binom, bool sat,
                                        unsigned __int128 rank, uint64_t* f) {
   using V = typename std::conditional<WIDE, unsigned __int128, uint64_t>::type;
+  const int nel = kp.nel;
   auto C = [&](int p, int i) -> V {
-    const uint64_t* t = binom + 2 * (p * kp.nel + (i - 1));
     if constexpr (WIDE) {
+      const uint64_t* t = binom + 2 * (p * nel + (i - 1));
       return (static_cast<unsigned __int128>(t[1]) << 64) | t[0];
     } else {
+      if (sat) return binom[p * nel + (i - 1)];
+      const uint64_t* t = binom + 2 * (p * nel + (i - 1));
       return t[1] ? ~uint64_t{0} : t[0];  // above the 64-bit ranks: larger than any of them
     }
   };
   V r = static_cast<V>(rank);
-  f[0] = f[1] = 0;
-  const int nel = kp.nel;
   int lo = nel - 1, hi = kp.nbasis - 1;
 #pragma unroll
   for (int step = 0; step < 7; ++step) {
@@ -1657,14 +1660,21 @@ __device__ __forceinline__ void unrank(const KeyPlan& kp, const uint64_t* __rest
     lo = live && ok ? mid : lo;
     hi = live && !ok ? mid - 1 : hi;
   }
-  int p = lo;
-  for (int i = nel; i >= 1; --i) {
-    V c = C(p, i);
-    while (c > r && p > 0) c = C(--p, i);
-    f[p >> 6] |= uint64_t{1} << (p & 63);
-    r -= c;
-    --p;
+  uint64_t f0 = 0, f1 = 0;
+  int i = nel;
+  for (int p = lo; i > 0 && p >= 0; --p) {
+    const V c = C(p, i);
+    if (c <= r) {
+      if (p < 64)
+        f0 |= uint64_t{1} << p;
+      else
+        f1 |= uint64_t{1} << (p - 64);
+      r -= c;
+      --i;
+    }
   }
+  f[0] = f0;
+  f[1] = f1;
 }
 
 // One sorted entry's share of its run's sums in annihilate_spawn_t_initiator
@@ -1799,9 +1809,16 @@ __global__ __launch_bounds__(kBlock) void k_runs(Sys s, KeyPlan kp, const uint64
   const int64_t tile = blockIdx.x, t0 = tile * kRunTile;
   if (t0 >= ns) return;  // block-uniform
   const bool staged = kp.rank && kp.binom_lds;
-  if (staged) {
-    const int nw = 2 * kp.nbasis * kp.nel;
-    for (int k = tid; k < nw; k += kBlock) binom_lds[k] = kp.binom[k];
+  if (staged) {  // one-word ranks: the table saturated at 2^64 - 1, one word per entry
+    const int nv = kp.nbasis * kp.nel;
+    for (int k = tid; k < nv; k += kBlock) {
+      if constexpr (WIDE) {
+        binom_lds[2 * k] = kp.binom[2 * k];
+        binom_lds[2 * k + 1] = kp.binom[2 * k + 1];
+      } else {
+        binom_lds[k] = kp.binom[2 * k + 1] ? ~uint64_t{0} : kp.binom[2 * k];
+      }
+    }
   }
   const int n = static_cast<int>(ns - t0 < kRunTile ? ns - t0 : kRunTile);
   const int fb = kp.flag_bits;
@@ -1894,9 +1911,9 @@ __global__ __launch_bounds__(kBlock) void k_runs(Sys s, KeyPlan kp, const uint64
     if (!kp.rank) {
       d[0] = static_cast<uint64_t>(val);  // batched instances: the encoded word itself
     } else if (staged) {
-      unrank<WIDE>(kp, binom_lds, val, d);
+      unrank<WIDE>(kp, binom_lds, !WIDE, val, d);
     } else {
-      unrank<WIDE>(kp, kp.binom, val, d);
+      unrank<WIDE>(kp, kp.binom, false, val, d);
     }
     FQ_FOR_WORDS(w, s.W) rdet[w * stride + run] = d[w];
   };
