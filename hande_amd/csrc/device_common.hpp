@@ -50,6 +50,12 @@ struct Sys {
   const double* s1j;   // norb^3: [p][a][o] = (pa|oo), the Coulomb terms of a single p -> a
   const double* s1k;   // norb^3: [p][a][o] = (po|oa), its exchange terms
   double psingle, pdouble;
+  // recip[k] = 1.0 / k for 0 < k < nrecip, and jpair[ind] the second electron
+  // index int(1.5 + sqrt(2*ind - 1.75)) of pair index ind of choose_ij_mol,
+  // both evaluated on the host (the same correctly rounded values)
+  const double* recip;
+  int nrecip;
+  const int* jpair;
   // Heat-bath tables (excit_gen FQ_EXCIT_HEAT_BATH), i-major over 1-based spin
   // orbitals: hb_ija[hb3(i,j,a)] = hb_ija%weights(a,j,i), hb_ija_tot[hb2(i,j)] =
   // hb_ija%weights_tot(j,i), hb_ijab[hb4(i,j,a,b)] = hb_ijab%weights(b,a,j,i), ...
@@ -168,6 +174,53 @@ struct Philox {
       b = o1;
     }
     second = !second;
+    uint64_t v = (static_cast<uint64_t>(a >> 6) << 26) | (b >> 6);
+    return static_cast<double>(v) * (1.0 / 4503599627370496.0);
+  }
+};
+
+// The same stream as Philox, with its first K blocks generated at
+// construction, where a wavefront's lanes are still converged: the rejection
+// loops of the generators then consume stored words (a move of the remaining
+// words per draw) instead of generating blocks in divergent code, where a
+// block costs the whole wavefront whenever any lane needs one.  Draws past
+// the K blocks are generated one block at a time, as Philox does.
+template <int K>
+struct PhiloxPre {
+  uint32_t k0, k1, c0, c1, c2;
+  uint32_t block;
+  uint32_t w[4 * K];  // unconsumed words, the next draw's pair first
+  int left;           // draws (word pairs) left in w
+  bool failed;
+  __device__ void gen(uint32_t* o) {
+    uint32_t x[4] = {c0, c1, c2, block++};
+    uint32_t ka = k0, kb = k1;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+      Philox::round(x, ka, kb);
+      ka += 0x9E3779B9u;
+      kb += 0xBB67AE85u;
+    }
+    o[0] = x[0];
+    o[1] = x[1];
+    o[2] = x[2];
+    o[3] = x[3];
+  }
+  __device__ PhiloxPre(uint64_t key, uint32_t a, uint32_t b, uint32_t c)
+      : k0(static_cast<uint32_t>(key)), k1(static_cast<uint32_t>(key >> 32)), c0(a), c1(b), c2(c), block(0),
+        left(2 * K), failed(false) {
+#pragma unroll
+    for (int q = 0; q < K; ++q) gen(w + 4 * q);
+  }
+  __device__ double next() {
+    if (left == 0) {
+      gen(w);
+      left = 2;
+    }
+    const uint32_t a = w[0], b = w[1];
+#pragma unroll
+    for (int q = 0; q + 2 < 4 * K; ++q) w[q] = w[q + 2];
+    --left;
     uint64_t v = (static_cast<uint64_t>(a >> 6) << 26) | (b >> 6);
     return static_cast<double>(v) * (1.0 / 4503599627370496.0);
   }
@@ -752,6 +805,10 @@ __device__ inline SymUnocc sym_unocc(const Sys& s, const uint64_t* f) {
   }
   return u;
 }
+// 1.0 / k: the host-made table's entry when it holds k, else the division
+__device__ __forceinline__ double recip_of(const Sys& s, int k) {
+  return k > 0 && k < s.nrecip ? s.recip[k] : 1.0 / static_cast<double>(k);
+}
 __device__ __forceinline__ int ssbf(const Sys& s, int ims, int sym, int k) {
   return s.ssbf[((ims - 1) * s.nsym_pg + sym) * s.max_sym_spin + k];
 }
@@ -799,7 +856,7 @@ __device__ inline bool single_mol(const Sys& s, Rng& rng, const uint64_t* f, con
       }
       conn.from[0] = i;
       conn.to[0] = a;
-      pgen = s.psingle * (1.0 / static_cast<double>(ni * u.get(ims_of(a), s.orb_sym[a - 1])));
+      pgen = s.psingle * recip_of(s, ni * u.get(ims_of(a), s.orb_sym[a - 1]));
       conn.perm = perm1(f, s.W, i, a);
       h = sc1_excit(s, f, i, a, conn.perm);
     } else {  // draws ran out (replay stream exhausted) before an i was found: no excitation
@@ -842,7 +899,7 @@ __device__ inline bool gen_mol(const Sys& s, Rng& rng, const uint64_t* f, Excit&
   } else {
     // choose_ij_mol
     const int ind = static_cast<int>(rng.next() * nel * (nel - 1) / 2) + 1;
-    const int j_ind = static_cast<int>(1.50 + sqrt(2 * ind - 1.750));
+    const int j_ind = s.jpair[ind];
     const int i_ind = ind - ((j_ind - 1) * (j_ind - 2)) / 2;
     const int i = select_orb(f, s.W, i_ind - 1);
     const int j = select_orb(f, s.W, j_ind - 1);
@@ -919,11 +976,11 @@ __device__ inline bool gen_mol(const Sys& s, Rng& rng, const uint64_t* f, Excit&
             n_aij = n_aij - u.get(ims, isyma);
         }
         if (s.orb_sym[a - 1] == s.orb_sym[b - 1]) {
-          p_aijb = 1.0 / static_cast<double>(u.get(imsa, s.orb_sym[a - 1]) - 1);
-          p_bija = 1.0 / static_cast<double>(u.get(imsb, s.orb_sym[b - 1]) - 1);
+          p_aijb = recip_of(s, u.get(imsa, s.orb_sym[a - 1]) - 1);
+          p_bija = recip_of(s, u.get(imsb, s.orb_sym[b - 1]) - 1);
         } else {
-          p_aijb = 1.0 / static_cast<double>(u.get(imsa, s.orb_sym[a - 1]));
-          p_bija = 1.0 / static_cast<double>(u.get(imsb, s.orb_sym[b - 1]));
+          p_aijb = recip_of(s, u.get(imsa, s.orb_sym[a - 1]));
+          p_bija = recip_of(s, u.get(imsb, s.orb_sym[b - 1]));
         }
       } else {
         n_aij = s.nvirt;
@@ -932,10 +989,10 @@ __device__ inline bool gen_mol(const Sys& s, Rng& rng, const uint64_t* f, Excit&
           if (u.get(1, isymb) == 0) n_aij = n_aij - u.get(2, isyma);
           if (u.get(2, isymb) == 0) n_aij = n_aij - u.get(1, isyma);
         }
-        p_aijb = 1.0 / static_cast<double>(u.get(imsa, s.orb_sym[a - 1]));
-        p_bija = 1.0 / static_cast<double>(u.get(imsb, s.orb_sym[b - 1]));
+        p_aijb = recip_of(s, u.get(imsa, s.orb_sym[a - 1]));
+        p_bija = recip_of(s, u.get(imsb, s.orb_sym[b - 1]));
       }
-      const double pd = (1.0 / static_cast<double>(n_aij)) * (p_bija + p_aijb);
+      const double pd = recip_of(s, n_aij) * (p_bija + p_aijb);
       pgen = s.pdouble * pgen_ij * pd;
       conn.perm = perm2(f, s.W, conn);
       h = sc2_excit(s, conn.from[0], conn.from[1], conn.to[0], conn.to[1], conn.perm);
@@ -1356,7 +1413,7 @@ __device__ inline bool gen_learned(const Sys& s, Rng& rng, const uint64_t* f, co
     if (rng.next() < mix) {
       // uniform component: unordered hole pair (choose_ij_mol's index map), a, partner
       const int ind = static_cast<int>(rng.next() * nel * (nel - 1) / 2) + 1;
-      const int j_ind = static_cast<int>(1.50 + sqrt(2 * ind - 1.750));
+      const int j_ind = s.jpair[ind];
       const int i_ind = ind - ((j_ind - 1) * (j_ind - 2)) / 2;
       i = select_orb(f, s.W, i_ind - 1);
       j = select_orb(f, s.W, j_ind - 1);

@@ -39,6 +39,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <climits>
 #include <cstdint>
 #include <cstdio>
@@ -298,6 +299,8 @@ struct fq_engine {
   int *d_sym_table = nullptr, *d_inv_sym = nullptr, *d_orb_sym = nullptr, *d_nbss = nullptr, *d_ssbf = nullptr;
   double *d_sp_eigv = nullptr, *d_h1 = nullptr, *d_eri = nullptr, *d_jmat = nullptr, *d_kmat = nullptr;
   double *d_s1j = nullptr, *d_s1k = nullptr;
+  double* d_recip = nullptr;  // 1.0 / k, k < kRecipN
+  int* d_jpair = nullptr;     // choose_ij_mol's j index per pair index
   // heat-bath tables (excit_gen FQ_EXCIT_HEAT_BATH)
   double *d_hb_i = nullptr, *d_hb_ij = nullptr, *d_hb_ija = nullptr, *d_hb_ija_tot = nullptr, *d_hb_ija_u = nullptr;
   double *d_hb_ijab = nullptr, *d_hb_ijab_tot = nullptr, *d_hb_ijab_u = nullptr;
@@ -665,6 +668,13 @@ __device__ __forceinline__ int64_t parent_of(const int64_t* __restrict__ att_off
 }
 
 constexpr int kSpawnTile = 4 * kBlock;  // attempts per block tile
+// Philox blocks of an attempt's stream generated before its generator runs
+// (two draws each: the single/double choice, the orbital choices with their
+// first rejection redraws, the spawn rounding).  On mol26 (renormalised
+// generator) k_spawn took 474, 465, 454 and 425 us with 0, 2, 3 and 4 blocks;
+// 4 keeps its 80 VGPRs (6 waves per SIMD).
+constexpr int kSpawnPre = 4;
+constexpr int kRecipN = 4096;  // entries of the host-made reciprocal table
 
 // Parents of each spawning tile's first and last attempt (one thread per
 // tile, so the searches' latencies overlap instead of opening every block).
@@ -849,13 +859,13 @@ __global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, 
         const int inst = static_cast<int>(f[0] >> kInstShift);
         f[0] &= kOrbMask;
         const int64_t trel = t - iatt[inst];
-        fqd::Philox rng(seed + static_cast<uint64_t>(inst), static_cast<uint32_t>(trel),
-                        static_cast<uint32_t>(trel >> 32), cyc ^ (fqd::kDrawAttempt << 24));
+        fqd::PhiloxPre<kSpawnPre> rng(seed + static_cast<uint64_t>(inst), static_cast<uint32_t>(trel),
+                                      static_cast<uint32_t>(trel >> 32), cyc ^ (fqd::kDrawAttempt << 24));
         nspawn = fqd::spawn_attempt<G>(sys_arr[inst], rng, f, nullptr, parent, itau[inst], child);
         if (nspawn != 0) child[0] |= static_cast<uint64_t>(inst) << kInstShift;
       } else {
-        fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
-                        cyc ^ (fqd::kDrawAttempt << 24));
+        fqd::PhiloxPre<kSpawnPre> rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32),
+                                      cyc ^ (fqd::kDrawAttempt << 24));
         const float* wrow = (G == fqd::kGenLearned && s.mlp_on[i]) ? s.mlp_tab + s.mlp_row[i] * s.mlp_ld : nullptr;
         nspawn = fqd::spawn_attempt<G>(s, rng, f, wrow, parent, tau, child, hb, aff_lds, flat_lds);
         // a core parent's spawn onto the core is the deterministic projection's
@@ -3278,6 +3288,15 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
     if (rc == FQ_OK) rc = upload(&e->d_kmat, K.data(), static_cast<int64_t>(K.size()), e->stream);
     if (rc == FQ_OK) rc = upload(&e->d_s1j, S1J.data(), static_cast<int64_t>(S1J.size()), e->stream);
     if (rc == FQ_OK) rc = upload(&e->d_s1k, S1K.data(), static_cast<int64_t>(S1K.size()), e->stream);
+    // Host-evaluated divisions and square roots of the generators' integer
+    // arguments (IEEE division and sqrt are correctly rounded on both sides).
+    std::vector<double> R(kRecipN, 0.0);
+    for (int k = 1; k < kRecipN; ++k) R[k] = 1.0 / static_cast<double>(k);
+    const int npairs = sys->nel * (sys->nel - 1) / 2;
+    std::vector<int> JP(static_cast<size_t>(npairs) + 2, 0);
+    for (int ind = 1; ind <= npairs + 1; ++ind) JP[ind] = static_cast<int>(1.50 + std::sqrt(2 * ind - 1.750));
+    if (rc == FQ_OK) rc = upload(&e->d_recip, R.data(), kRecipN, e->stream);
+    if (rc == FQ_OK) rc = upload(&e->d_jpair, JP.data(), static_cast<int64_t>(JP.size()), e->stream);
     if (rc == FQ_OK) FQ_HIP(hipStreamSynchronize(e->stream));
   }
   s.sp_eigv = e->d_sp_eigv;
@@ -3292,6 +3311,9 @@ int fq_create(const fq_system* sys, const fq_params* params, fq_engine** out) {
   s.kmat = e->d_kmat;
   s.s1j = e->d_s1j;
   s.s1k = e->d_s1k;
+  s.recip = e->d_recip;
+  s.nrecip = e->d_recip ? kRecipN : 0;
+  s.jpair = e->d_jpair;
   if (rc == FQ_OK) rc = setup_rank_keys(e);
   if (rc == FQ_OK && (params->excit_gen == FQ_EXCIT_HEAT_BATH || params->excit_gen == FQ_EXCIT_LEARNED))
     rc = build_heat_bath(e);
@@ -4317,6 +4339,8 @@ void fq_destroy(fq_engine* e) {
   dfree(e->d_kmat);
   dfree(e->d_s1j);
   dfree(e->d_s1k);
+  dfree(e->d_recip);
+  dfree(e->d_jpair);
   dfree(e->d_hb_i);
   dfree(e->d_hb_ij);
   dfree(e->d_hb_ija);
