@@ -26,7 +26,10 @@ Prints one JSON line (rank 0).  Extra fields:
   cpu_baseline  the CPU oracle (a serial restatement of the reference loop)
                 timed on the same walker state: one instance per host core on
                 a slice of the determinants each (as MPI ranks would hold), and
-                one instance on one core over the whole list.
+                one instance on one core over the whole list;
+  batched       the batched workload (64 Hubbard-k rings of 10-18 sites) run
+                as graph-captured cycles: cycle time, attempts/s and the
+                schedule the tuner chose (--no-batched skips it).
 Under ``rocprofv3 --selected-regions`` only the timed cycles are profiled
 (roctxProfilerResume/Pause bracket them).
 """
@@ -751,6 +754,74 @@ def efficiency_batched(args):
                       "instances": per}), flush=True)
 
 
+def batched_leg(cycles=200, prep_cycles=1500, graph=True):
+    """The batched workload (64 Hubbard-k rings of 10-18 sites, targets 1e4-1e6,
+    integer populations) with graph-captured cycles: prepared as the batched
+    efficiency mode prepares it, then `cycles` timed graph cycles (their device
+    time from the engine's events around each graph launch); the tuner's
+    candidate timings and its choice are reported."""
+    lib = N.lib()
+    decks, targets = c3_instances(64)
+    K = len(decks)
+    systems = [api.hubbard_k(**d) for d in decks]
+    tau, every = 0.005, 5
+    cap = int(6 * targets.sum()) + (1 << 20)
+    arr = (N.FqSystem * K)(*[s.view for s in systems])
+    eng = C.c_void_p()
+    N.check(lib.fq_create_batch(arr, K, C.byref(integer_params(77, cap)), C.byref(eng)), "fq_create_batch")
+    try:
+        N.check(lib.fq_batch_graph(eng, 1 if graph else 0), "fq_batch_graph")
+        inst = np.arange(K, dtype=np.int32)
+        f0 = np.array([int(s.f0()[0]) for s in systems], dtype=np.uint64)
+        pops = np.full(K, 10, dtype=np.int64)
+        N.check(lib.fq_load_batch(eng, inst.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  f0.ctypes.data_as(C.POINTER(C.c_uint64)), pops.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  np.zeros(K).ctypes.data_as(C.POINTER(C.c_double)), K), "load")
+        shift, eold, taus = np.zeros(K), np.zeros(K), np.full(K, tau)
+        vary, n_old = np.zeros(K, bool), np.zeros(K)
+        outs = (N.FqCycleOut * K)()
+        prof = N.FqProfileOut()
+        t0 = time.perf_counter()
+        dev_ms, wall, attempts = [], 0.0, 0
+        for c in range(prep_cycles + cycles):
+            t1 = time.perf_counter()
+            N.check(lib.fq_cycle_batch(eng, shift.ctypes.data_as(C.POINTER(C.c_double)),
+                                       eold.ctypes.data_as(C.POINTER(C.c_double)),
+                                       taus.ctypes.data_as(C.POINTER(C.c_double)), outs), "fq_cycle_batch")
+            if c >= prep_cycles:
+                wall += time.perf_counter() - t1
+                lib.fq_last_profile(eng, C.byref(prof))
+                dev_ms.append(prof.total_ms)
+                attempts += sum(outs[k].nattempts for k in range(K))
+            if (c + 1) % every == 0:
+                n = np.array([outs[k].nparticles[0] for k in range(K)])
+                upd = vary & (n_old > 0)
+                damp = 0.2 if c < prep_cycles else 0.05
+                pull = 0.04 if c < prep_cycles else 0.0
+                shift[upd] -= (np.log(n[upd] / n_old[upd]) * damp + pull * np.log(n[upd] / targets[upd])) / (tau * every)
+                vary[:] |= n > targets / 20
+                n_old[:] = n
+        info = N.FqScheduleOut()
+        N.check(lib.fq_batch_schedule(eng, C.byref(info)), "fq_batch_schedule")
+        walkers = sum(outs[k].nparticles[0] for k in range(K))
+        log(f"[bench] batched: {walkers:.4g} walkers, {np.median(dev_ms):.3f} ms/cycle device "
+            f"({'graph' if graph else 'ordinary launches'}), "
+            f"{1e3 * wall / cycles:.3f} ms/cycle wall, {time.perf_counter() - t0:.1f}s")
+        return {"workload": f"{K} Hubbard-k rings of 10-18 sites, U/t 1-8, targets 1e4-1e6, integer populations, "
+                            f"tau {tau}",
+                "walkers": walkers, "cycles": cycles, "graph_launches": bool(graph), "attempts_per_s": attempts / wall,
+                "ms_per_cycle_wall": 1e3 * wall / cycles, "ms_per_cycle_device": float(np.median(dev_ms)),
+                "graph": {"graph_cycles": info.graph_cycles, "ordinary_cycles": info.eager_cycles,
+                          "graphs_captured": info.graphs, "tuned": bool(info.tuned),
+                          "attempt_octave": info.octave, "tuned_octaves": info.tuned_octaves,
+                          "chosen": {"digit_bits": info.digit_bits, "spawn_blocks": info.spawn_blocks},
+                          "candidates": [{"digit_bits": info.cand_digit_bits[q],
+                                          "spawn_blocks": info.cand_spawn_blocks[q],
+                                          "median_ms": info.cand_ms[q]} for q in range(info.ncand)]}}
+    finally:
+        lib.fq_destroy(eng)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -769,6 +840,10 @@ def main():
     ap.add_argument("--load-state", default=None, help="start from a list written by --save-state (no preparation)")
     ap.add_argument("--start-frac", type=float, default=1e-3, help="initial HF population / target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched graph-cycle leg")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched leg and print its JSON")
+    ap.add_argument("--batched-graph", type=int, default=1, choices=[0, 1],
+                    help="batched leg: graph-captured cycles (1) or ordinary launches (0)")
     ap.add_argument("--efficiency", choices=["learned", "batched"], default=None,
                     help="measure the E_proj statistical efficiency of the learned proposal (against heat-bath, "
                          "mol40) or of batched instances (against one engine per instance) instead")
@@ -793,6 +868,9 @@ def main():
     args = ap.parse_args()
     if args.eff_cycles is None:
         args.eff_cycles = 20000 if args.efficiency == "learned" else 5000
+    if args.batched_only:
+        print(json.dumps(batched_leg(graph=bool(args.batched_graph))), flush=True)
+        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -951,6 +1029,11 @@ def main():
         "cpu_baseline": cpu,
     }
     lib.fq_destroy(eng)
+    if not args.no_batched and not args.load_state:
+        try:
+            result["batched"] = batched_leg()
+        except Exception as ex:  # a side measurement; the headline line still prints
+            log(f"[bench] batched leg failed: {ex}")
     print(json.dumps(result), flush=True)
 
 

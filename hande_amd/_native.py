@@ -60,6 +60,14 @@ class FqCycleOut(C.Structure):
     ]
 
 
+class FqScheduleOut(C.Structure):
+    _fields_ = [
+        ("graph", i32), ("tuned", i32), ("digit_bits", i32), ("spawn_blocks", i32), ("ncand", i32),
+        ("chosen", i32), ("octave", i32), ("tuned_octaves", i32), ("graph_cycles", i64), ("graphs", i64), ("eager_cycles", i64),
+        ("cand_ms", f64 * 8), ("cand_digit_bits", i32 * 8), ("cand_spawn_blocks", i32 * 8),
+    ]
+
+
 class FqProfileOut(C.Structure):
     _fields_ = [
         ("det_pass_ms", f64), ("spawn_ms", f64), ("sort_ms", f64), ("reduce_ms", f64), ("merge_ms", f64),
@@ -142,12 +150,16 @@ def _declare(lib):
         "fq_create_batch": (C.c_int, [P(FqSystem), i32, P(FqParams), P(vp)]),
         "fq_load_batch": (C.c_int, [vp, P(i32), P(u64), P(i64), P(f64), i64]),
         "fq_cycle_batch": (C.c_int, [vp, P(f64), P(f64), P(f64), P(FqCycleOut)]),
+        "fq_batch_graph": (C.c_int, [vp, i32]),
+        "fq_batch_schedule": (C.c_int, [vp, P(FqScheduleOut)]),
         "fq_mlp_nparams": (C.c_int, [vp, P(i64)]),
         "fq_mlp_set_params": (C.c_int, [vp, P(C.c_float), i64]),
         "fq_mlp_get_params": (C.c_int, [vp, P(C.c_float), i64]),
         "fq_mlp_forward": (C.c_int, [vp, P(u64), i64, P(C.c_float)]),
         "fq_mlp_affinity": (C.c_int, [vp, P(C.c_float), i64]),
         "fq_mlp_loss_grad": (C.c_int, [vp, P(u64), P(i32), P(f64), P(i32), i64, P(f64), P(C.c_float)]),
+        "fq_mlp_variance_grad": (C.c_int, [vp, P(u64), P(i32), P(f64), P(f64), P(i32), i64, P(f64), P(C.c_float)]),
+        "fq_mlp_online": (C.c_int, [vp, i32, i32, i64, f64, i32]),
         "fq_mlp_train": (C.c_int, [vp, i64, i32, f64, u64, P(f64)]),
         "fq_probe_excitations": (C.c_int, [vp, P(u64), i64, u64, P(i32), P(f64), P(f64)]),
         "fq_destroy": (None, [vp]),

@@ -179,6 +179,16 @@ __device__ __forceinline__ int64_t block_sum(int64_t x, int64_t* wsh) {
   return t;
 }
 
+// A list length read on the device when pn is given (launches sized by a
+// bound n: production sorts, graph-captured cycles), clamped to the bound.
+__device__ __forceinline__ int64_t list_len(const int64_t* __restrict__ pn, int64_t n) {
+  if (!pn) return n;
+  const int64_t v = *pn;
+  return v < n ? v : n;
+}
+// The cycle index: read on the device in graph-captured cycles.
+__device__ __forceinline__ uint32_t cycle_of(const uint32_t* __restrict__ pc, uint32_t cyc) { return pc ? *pc : cyc; }
+
 __global__ __launch_bounds__(kBlock) void k_tile_sums(const int64_t* __restrict__ in, int64_t n,
                                                       int64_t* __restrict__ sums) {
   __shared__ int64_t wsh[kBlock / 64];
@@ -299,6 +309,44 @@ struct fq_engine {
   int *d_sym_table = nullptr, *d_inv_sym = nullptr, *d_orb_sym = nullptr, *d_nbss = nullptr, *d_ssbf = nullptr;
   double *d_sp_eigv = nullptr, *d_h1 = nullptr, *d_eri = nullptr, *d_jmat = nullptr, *d_kmat = nullptr;
   double *d_s1j = nullptr, *d_s1k = nullptr;
+  // Graph-captured batched cycles: while one is captured (capturing), the
+  // main list's length is read on the device from dn and the cycle index from
+  // dcyc (the graph's first copy fills both from pinned host memory, h_in),
+  // and launches are sized by bounds.
+  // online training of the learned proposal (fq_mlp_online): steps of batch
+  // samples at the start of every online_every-th cycle
+  int32_t online_every = 0, online_steps = 0;
+  int64_t online_batch = 0;
+  double online_lr = 0.0;
+  int64_t online_runs = 0;
+  bool capturing = false;
+  const int64_t* dn = nullptr;
+  const uint32_t* dcyc = nullptr;
+  bool graph_on = false;
+  char* h_in = nullptr;       // pinned: [n][cycle index][tau, shift, E_proj per instance]
+  char* d_in = nullptr;
+  Stats* h_gstats = nullptr;  // pinned copies the graph writes at its end
+  double* h_god = nullptr;
+  int64_t* h_goi = nullptr;
+  struct GraphEntry {
+    int c;
+    int64_t nb, tb;
+    int sched;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  // the tuner, per octave of the attempt bound: the chosen schedule (-1:
+  // still trying candidates), cycles tried and each candidate's timings
+  int sched[64];
+  int tune_iter[64];
+  std::vector<float> tune_ms[64][8];
+  int last_oct = -1;          // octave of the last graph cycle
+  int64_t graph_cycles = 0, eager_cycles = 0;
+  hipEvent_t gev[2] = {nullptr, nullptr};
+  // schedule knobs (the tuner's candidates): 8-bit radix digits only, and a
+  // cap on the spawning grid (0: one block per tile)
+  int digit8 = 0;
+  int spawn_blocks = 0;
   double* d_recip = nullptr;  // 1.0 / k, k < kRecipN
   int* d_jpair = nullptr;     // choose_ij_mol's j index per pair index
   // heat-bath tables (excit_gen FQ_EXCIT_HEAT_BATH)
@@ -320,6 +368,12 @@ struct fq_engine {
   int64_t* d_iatt = nullptr;         // [ninst]: first attempt of each instance
   double* d_cterm = nullptr;         // [2S][cap]: per-determinant projected-energy and N_0 terms
   int64_t* d_cdeath = nullptr;       // [cap]
+  // per-instance 256-entry block counts and offsets (old list, new list) and
+  // the block partials of the estimator terms (2S), deaths and |N| (1 + S)
+  int64_t *d_iblk_cnt = nullptr, *d_iblk_off = nullptr, *d_nblk_off = nullptr;
+  double* d_ipart_d = nullptr;
+  int64_t* d_ipart_i = nullptr;
+  int64_t ipb = 0;  // partial stride: a bound of the blocks of either list
   double* d_iout_d = nullptr;        // [ninst][2S]
   int64_t* d_iout_i = nullptr;       // [ninst][4 + S]
   // learned proposal (excit_gen FQ_EXCIT_LEARNED): network, per-determinant weights
@@ -776,7 +830,8 @@ __global__ __launch_bounds__(kBlock) void k_spawn(const Sys* __restrict__ sdev, 
                                                   const int64_t* __restrict__ tile_win,
                                                   const int64_t* __restrict__ pflags,
                                                   const Sys* __restrict__ sys_arr, const int64_t* __restrict__ iatt,
-                                                  const double* __restrict__ itau) {
+                                                  const double* __restrict__ itau, const uint32_t* __restrict__ pcyc) {
+  cyc = cycle_of(pcyc, cyc);
   // the system descriptor is read from device memory (scalar loads on use)
   // rather than passed by value: its many table pointers would otherwise sit
   // in scalar registers for the whole kernel and spill into vector lanes
@@ -1002,9 +1057,10 @@ __global__ __launch_bounds__(kBlock) void k_pack_tiles(int W, int S, int64_t nti
 }
 
 __global__ void k_add_spawned(const int64_t* __restrict__ pos, int64_t nb, const int64_t* __restrict__ ptot,
-                              Stats* __restrict__ st) {
+                              Stats* __restrict__ st, int64_t tlim) {
   st->nspawn += pos[nb];
   st->total_attempts = *ptot;
+  if (tlim >= 0 && *ptot > tlim) st->spawn_overflow = 1;  // attempts past the launches' bound were not made
 }
 
 // ---------------------------------------------------------------------------
@@ -1012,9 +1068,10 @@ __global__ void k_add_spawned(const int64_t* __restrict__ pos, int64_t nb, const
 // ---------------------------------------------------------------------------
 // bound[k] = first position in the sorted list whose instance index is >= k.
 __global__ void k_inst_bounds(const uint64_t* __restrict__ w0, int64_t n, int ninst, int shift,
-                              int64_t* __restrict__ bound) {
+                              int64_t* __restrict__ bound, const int64_t* __restrict__ pn) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > ninst) return;
+  n = list_len(pn, n);
   if (k == ninst) {
     bound[k] = n;
     return;
@@ -1043,9 +1100,16 @@ __global__ __launch_bounds__(kBlock) void k_det_pass_batch(Sys s, const Sys* __r
                                                            int64_t* __restrict__ pop_new, int64_t* __restrict__ nat,
                                                            int64_t* __restrict__ flags, uint64_t seed, uint32_t cyc,
                                                            double* __restrict__ cterm,
-                                                           int64_t* __restrict__ cdeath) {
+                                                           int64_t* __restrict__ cdeath,
+                                                           const int64_t* __restrict__ pn,
+                                                           const uint32_t* __restrict__ pcyc) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n) return;
+  if (i >= list_len(pn, n)) {  // past the list in a launch sized by a bound: no attempts
+    for (int sp = 0; sp < s.S; ++sp) nat[i * s.S + sp] = 0;
+    return;
+  }
+  cyc = cycle_of(pcyc, cyc);
   uint64_t f[FQ_MAX_WORDS];
   load_det(det, cap, s.W, i, f);
   const int inst = static_cast<int>(f[0] >> kInstShift);
@@ -1088,47 +1152,96 @@ __global__ void k_inst_att(int ninst, int S, const int64_t* __restrict__ bound, 
   if (k < ninst) iatt[k] = att_off[S * bound[k]];
 }
 
-// Per-instance cycle results, one block per instance: projected-energy terms,
-// N_0 and deaths over the old list, attempts, spawned entries, occupied
-// determinants and sum |N| over the new list.  The floating-point sums repeat
-// the single-instance reduction exactly (blocks of 256 determinants counted
-// from the instance's first, each reduced by the k_det_pass tree, then block b
-// added to partial b mod 256 in order and the k_sum_partials tree), so a
-// batched instance reports the same estimators, bit for bit, as it alone.
-__global__ __launch_bounds__(kBlock) void k_inst_stats(int ninst, int S, int64_t cap, const int64_t* __restrict__ ob,
-                                                       const int64_t* __restrict__ nbd, const int64_t* __restrict__ sb,
-                                                       const double* __restrict__ cterm,
-                                                       const int64_t* __restrict__ cdeath,
-                                                       const int64_t* __restrict__ att_off,
-                                                       const int64_t* __restrict__ pop,
-                                                       double* __restrict__ out_d, int64_t* __restrict__ out_i) {
+// Per-instance cycle results: projected-energy terms, N_0 and deaths over the
+// old list, attempts, spawned entries, occupied determinants and sum |N| over
+// the new list.  The floating-point sums repeat the single-instance reduction
+// exactly: blocks of 256 determinants counted from the instance's first, each
+// reduced by the k_det_pass tree (k_inst_part, one block per such block of
+// every instance), then block b added to partial b mod 256 in order and the
+// k_sum_partials tree (k_inst_stats, one block per instance), so a batched
+// instance reports the same estimators, bit for bit, as it alone.
+// Blocks of 256 entries per instance of a list with instance bounds b.
+__global__ void k_inst_nblk(int ninst, const int64_t* __restrict__ b, int64_t* __restrict__ cnt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ninst) cnt[k] = (b[k + 1] - b[k] + kBlock - 1) / kBlock;
+}
+// The instance owning global block blk: the last k with off[k] <= blk.
+__device__ __forceinline__ int inst_of_block(const int64_t* __restrict__ off, int ninst, int64_t blk) {
+  int lo = 0, hi = ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= blk)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+// One block per 256-entry block of every instance (grid: a bound of their
+// number).  Old list (ob, off): the tree sums of the 2S estimator terms and the
+// deaths; new list (nbd, noff): sum |N| per space.
+__global__ __launch_bounds__(kBlock) void k_inst_part(int ninst, int S, int64_t cap, const int64_t* __restrict__ ob,
+                                                      const int64_t* __restrict__ off, const double* __restrict__ cterm,
+                                                      const int64_t* __restrict__ cdeath,
+                                                      const int64_t* __restrict__ nbd, const int64_t* __restrict__ noff,
+                                                      const int64_t* __restrict__ pop, int64_t pb,
+                                                      double* __restrict__ part_d, int64_t* __restrict__ part_i) {
   __shared__ double shd[2 * FQ_MAX_SPACES][kBlock];
-  __shared__ int64_t shi[1 + FQ_MAX_SPACES][kBlock];
-  const int k = blockIdx.x, tid = threadIdx.x;
-  const int64_t a = ob[k], b = ob[k + 1], na = nbd[k], nbb = nbd[k + 1];
-  const int nq = 2 * S;
-  double acc[2 * FQ_MAX_SPACES] = {0.0, 0.0, 0.0, 0.0};
-  const int64_t nvb = (b - a + kBlock - 1) / kBlock;
-  for (int64_t vb = 0; vb < nvb; ++vb) {
-    const int64_t i = a + vb * kBlock + tid;
-    for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][tid] = (q < nq && i < b) ? cterm[q * cap + i] : 0.0;
+  __shared__ int64_t wsh[kBlock / 64];
+  const int tid = threadIdx.x;
+  const int64_t blk = blockIdx.x;
+  const int64_t nold = off[ninst], nnew = noff[ninst];
+  if (blk < nold) {
+    const int k = inst_of_block(off, ninst, blk);
+    const int64_t i = ob[k] + (blk - off[k]) * kBlock + tid;
+    const bool in = i < ob[k + 1];
+    for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][tid] = (q < 2 * S && in) ? cterm[q * cap + i] : 0.0;
     __syncthreads();
     for (int h = kBlock / 2; h > 0; h >>= 1) {
       if (tid < h)
         for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][tid] += shd[q][tid + h];
       __syncthreads();
     }
-    if (tid == static_cast<int>(vb % kBlock))
-      for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) acc[q] = acc[q] + shd[q][0];
-    __syncthreads();
-  }
-  int64_t acci[1 + FQ_MAX_SPACES] = {0, 0, 0};
-  for (int64_t i = a + tid; i < b; i += kBlock) acci[0] += cdeath[i];
-  for (int64_t i = na + tid; i < nbb; i += kBlock)
-    for (int sp = 0; sp < S; ++sp) {
-      const int64_t p = pop[sp * cap + i];
-      acci[1 + sp] += p < 0 ? -p : p;
+    const int64_t d = block_sum(in ? cdeath[i] : 0, wsh);
+    if (tid == 0) {
+      for (int q = 0; q < 2 * S; ++q) part_d[q * pb + blk] = shd[q][0];
+      part_i[blk] = d;
     }
+  }
+  if (blk < nnew) {
+    const int k = inst_of_block(noff, ninst, blk);
+    const int64_t i = nbd[k] + (blk - noff[k]) * kBlock + tid;
+    const bool in = i < nbd[k + 1];
+    for (int sp = 0; sp < S; ++sp) {
+      int64_t p = in ? pop[sp * cap + i] : 0;
+      p = p < 0 ? -p : p;
+      const int64_t t = block_sum(p, wsh);
+      if (tid == 0) part_i[(1 + sp) * pb + blk] = t;
+    }
+  }
+}
+// One block per instance: its block partials added in the k_sum_partials
+// order, then the outputs.
+__global__ __launch_bounds__(kBlock) void k_inst_stats(int ninst, int S, const int64_t* __restrict__ ob,
+                                                       const int64_t* __restrict__ nbd, const int64_t* __restrict__ sb,
+                                                       const int64_t* __restrict__ off,
+                                                       const int64_t* __restrict__ noff, int64_t pb,
+                                                       const double* __restrict__ part_d,
+                                                       const int64_t* __restrict__ part_i,
+                                                       const int64_t* __restrict__ att_off,
+                                                       double* __restrict__ out_d, int64_t* __restrict__ out_i) {
+  __shared__ double shd[2 * FQ_MAX_SPACES][kBlock];
+  __shared__ int64_t shi[1 + FQ_MAX_SPACES][kBlock];
+  const int k = blockIdx.x, tid = threadIdx.x;
+  const int nq = 2 * S;
+  double acc[2 * FQ_MAX_SPACES] = {0.0, 0.0, 0.0, 0.0};
+  int64_t acci[1 + FQ_MAX_SPACES] = {0, 0, 0};
+  for (int64_t vb = tid; vb < off[k + 1] - off[k]; vb += kBlock) {
+    for (int q = 0; q < nq; ++q) acc[q] = acc[q] + part_d[q * pb + off[k] + vb];
+    acci[0] += part_i[off[k] + vb];
+  }
+  for (int64_t vb = tid; vb < noff[k + 1] - noff[k]; vb += kBlock)
+    for (int sp = 0; sp < S; ++sp) acci[1 + sp] += part_i[(1 + sp) * pb + noff[k] + vb];
   for (int q = 0; q < 2 * FQ_MAX_SPACES; ++q) shd[q][tid] = acc[q];
   for (int q = 0; q < 1 + FQ_MAX_SPACES; ++q) shi[q][tid] = acci[q];
   __syncthreads();
@@ -1140,12 +1253,13 @@ __global__ __launch_bounds__(kBlock) void k_inst_stats(int ninst, int S, int64_t
     __syncthreads();
   }
   if (tid == 0) {
+    const int64_t a = ob[k], b = ob[k + 1];
     for (int q = 0; q < nq; ++q) out_d[k * nq + q] = shd[q][0];
     int64_t* o = out_i + k * (4 + S);
     o[0] = shi[0][0];                          // deaths
     o[1] = att_off[S * b] - att_off[S * a];    // attempts
     o[2] = sb[k + 1] - sb[k];                  // spawned entries
-    o[3] = nbb - na;                           // occupied determinants
+    o[3] = nbd[k + 1] - nbd[k];                // occupied determinants
     for (int sp = 0; sp < S; ++sp) o[4 + sp] = shi[1 + sp][0];
   }
 }
@@ -1338,11 +1452,6 @@ __device__ __forceinline__ uint32_t block_exclusive_u32(uint32_t x, uint32_t* ws
 
 // Length of the spawned list: the device count (clamped to the host bound n,
 // the capacity at most) when pn is given, else n itself.
-__device__ __forceinline__ int64_t list_len(const int64_t* __restrict__ pn, int64_t n) {
-  if (!pn) return n;
-  const int64_t v = *pn;
-  return v < n ? v : n;
-}
 
 // Radix pass, reduce-then-scan over a fixed grid: block b owns the sort tiles
 // [b * tpb, (b + 1) * tpb) (tiles of kBlock * sort_items keys; tpb from the
@@ -1807,7 +1916,9 @@ __global__ __launch_bounds__(kBlock) void k_runs(Sys s, KeyPlan kp, const uint64
                                                  const int64_t* __restrict__ toff, const int64_t* __restrict__ tfirst,
                                                  const int64_t* __restrict__ tpre, int64_t ntiles,
                                                  uint64_t* __restrict__ rdet, int64_t* __restrict__ rpop,
-                                                 int64_t* __restrict__ rflag, int64_t* __restrict__ rstate) {
+                                                 int64_t* __restrict__ rflag, int64_t* __restrict__ rstate,
+                                                 const int64_t* __restrict__ pns) {
+  ns = list_len(pns, ns);
   constexpr int R = kRunTile / kBlock, NV = 3 * S, NW = kBlock / 64;
   __shared__ int32_t wcount[R][NW];
   __shared__ int32_t hpos[kRunTile + 1];  // tile-relative index of each head, in order; then the tile length
@@ -2079,8 +2190,9 @@ __device__ __forceinline__ int64_t main_lower_bound(const uint64_t* __restrict__
 // block's runs (one thread per block of kBlock runs).
 __global__ void k_merge_bounds(const int64_t* __restrict__ pnr, int64_t n, int64_t cap, int W,
                                const uint64_t* __restrict__ det, const uint64_t* __restrict__ rdet, int64_t scap,
-                               int64_t* __restrict__ mwin) {
+                               int64_t* __restrict__ mwin, const int64_t* __restrict__ pn) {
   const int64_t nruns = *pnr;
+  n = list_len(pn, n);
   const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t r0 = b * kBlock;
   if (r0 >= nruns) return;
@@ -2101,8 +2213,10 @@ __global__ __launch_bounds__(kBlock) void k_merge_runs(Sys s, const int64_t* __r
                                                        int64_t* __restrict__ pop_new, const uint64_t* __restrict__ rdet,
                                                        int64_t* __restrict__ rpop, const int64_t* __restrict__ rflag,
                                                        int64_t scap, const int64_t* __restrict__ mwin,
-                                                       int64_t* __restrict__ rpos, int64_t* __restrict__ rstate) {
+                                                       int64_t* __restrict__ rpos, int64_t* __restrict__ rstate,
+                                                       const int64_t* __restrict__ pn) {
   const int64_t nruns = *pnr;
+  n = list_len(pn, n);
   __shared__ uint64_t wdet[FQ_MAX_WORDS][kMergeWin];  // the window's determinants when it fits
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
   if (r0 >= nruns) return;  // whole block past the runs (the grid covers a bound); no window
@@ -2245,12 +2359,15 @@ __global__ __launch_bounds__(kBlock) void k_round_main(Sys s, int64_t n, int64_t
                                                        int replay, const double* __restrict__ rbuf, int64_t rbase,
                                                        const int64_t* __restrict__ mdraw_off, uint64_t seed,
                                                        uint32_t cyc, uint64_t* __restrict__ mmask,
-                                                       int64_t* __restrict__ mcnt, const uint64_t* __restrict__ det) {
+                                                       int64_t* __restrict__ mcnt, const uint64_t* __restrict__ det,
+                                                       const int64_t* __restrict__ pn, const uint32_t* __restrict__ pcyc) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nl = list_len(pn, n);  // entries past it (up to the bound n) are marked dead
+  cyc = cycle_of(pcyc, cyc);
   bool alive = false;
-  if (i < n && is_core_entry(s, det, cap, i)) {
+  if (i < nl && is_core_entry(s, det, cap, i)) {
     alive = true;
-  } else if (i < n) {
+  } else if (i < nl) {
     if (s.real_amps) {
       if (replay) {
         fqd::ReplayRng rng{rbuf, rbase + mdraw_off[i], rbase + mdraw_off[i + 1], false};
@@ -2275,8 +2392,9 @@ __global__ __launch_bounds__(kBlock) void k_round_new(Sys s, const int64_t* __re
                                                       const double* __restrict__ rbuf, int64_t rbase,
                                                       const int64_t* __restrict__ ndraw_off, uint64_t seed,
                                                       uint32_t cyc, uint64_t* __restrict__ nmask,
-                                                      int64_t* __restrict__ ncnt) {
+                                                      int64_t* __restrict__ ncnt, const uint32_t* __restrict__ pcyc) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  cyc = cycle_of(pcyc, cyc);
   const int64_t nruns = *pnr;
   bool alive = false;
   if (r < nruns && rstate[r] == 2) {
@@ -2305,7 +2423,9 @@ __global__ void k_zero(int64_t* __restrict__ p, int64_t n) {
 // pnr: the number of spawned runs (nullptr: none); also recorded in the stats.
 __global__ void k_finalize_counts(int64_t n, const int64_t* __restrict__ pnr, const int64_t* __restrict__ moff,
                                   const uint64_t* __restrict__ mmask, const int64_t* __restrict__ noff,
-                                  const uint64_t* __restrict__ nmask, int64_t cap, Stats* __restrict__ st) {
+                                  const uint64_t* __restrict__ nmask, int64_t cap, Stats* __restrict__ st,
+                                  const int64_t* __restrict__ pn) {
+  n = list_len(pn, n);
   const int64_t nruns = pnr ? *pnr : 0;
   st->nruns = nruns;
   const int64_t nmain = bit_rank(moff, mmask, n), nnew = bit_rank(noff, nmask, nruns);
@@ -2330,9 +2450,14 @@ constexpr int kInsWin = 1024;
 // first entry lies in (nrpos[k-1], nrpos[k]].
 __global__ void k_block_starts(const int64_t* __restrict__ nrpos, const Stats* __restrict__ st,
                                const int64_t* __restrict__ moff, const uint64_t* __restrict__ mmask, int64_t n,
-                               int64_t bound, int64_t nblk, int64_t* __restrict__ bstart) {
+                               int64_t bound, int64_t nblk, int64_t* __restrict__ bstart,
+                               const int64_t* __restrict__ pn) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k > bound) return;
+  if (pn) {  // the list's length and blocks on the device
+    n = list_len(pn, n);
+    nblk = (n + kBlock - 1) / kBlock;
+  }
   const int64_t nnew = st->main_overflow ? 0 : st->new_n - bit_rank(moff, mmask, n);
   if (k > nnew) return;
   const int64_t prev = k == 0 ? -1 : nrpos[k - 1];
@@ -2359,8 +2484,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter_main(Sys s, int64_t n, int64
                                                          const int64_t* __restrict__ nrpos,
                                                          const int64_t* __restrict__ bstart, const Stats* __restrict__ st,
                                                          uint64_t* __restrict__ det_out, int64_t* __restrict__ pop_out,
-                                                         double* __restrict__ kii_out) {
+                                                         double* __restrict__ kii_out, const int64_t* __restrict__ pn) {
   __shared__ int64_t win[kInsWin];
+  n = list_len(pn, n);
   const int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
   if (i0 >= n) return;  // whole block
   const bool ovf = st->main_overflow != 0;
@@ -2522,6 +2648,11 @@ int reserve_rng(fq_engine* e, int64_t extra) {
   return FQ_OK;
 }
 
+// Profiling event, recorded when the cycle is not being captured into a graph.
+void mark(fq_engine* e, hipEvent_t ev) {
+  if (!e->capturing) (void)hipEventRecord(ev, e->stream);
+}
+
 int sync_stats(fq_engine* e) {
   FQ_HIP(hipMemcpyAsync(&e->h_stats, e->d_stats, sizeof(Stats), hipMemcpyDeviceToHost, e->stream));
   FQ_HIP(hipStreamSynchronize(e->stream));
@@ -2550,7 +2681,8 @@ constexpr size_t kBinomLdsMax = 48 * 1024;
 DigitPlan digit_plan(const fq_engine* e) {
   DigitPlan dp{};
   const int bits = e->value_bits;
-  const int passes = std::max(1, (bits + 8) / 9), n9 = std::max(0, std::min(passes, bits - 8 * passes));
+  const int passes = e->digit8 ? std::max(1, (bits + 7) / 8) : std::max(1, (bits + 8) / 9);
+  const int n9 = e->digit8 ? 0 : std::max(0, std::min(passes, bits - 8 * passes));
   int shift = e->flag_bits;
   for (int p = 0; p < passes; ++p) {
     dp.shift[p] = shift;
@@ -2635,7 +2767,7 @@ int sort_and_reduce(fq_engine* e, int64_t ns, const int64_t* pns, bool replay) {
   hipStream_t st = e->stream;
   int src = 0;
   e->radix_passes = 0;
-  FQ_HIP(hipEventRecord(e->pev[4], st));
+  mark(e, e->pev[4]);
   if (ns > 0) {
     if (replay) {
       hipLaunchKernelGGL(k_replay_qsort, dim3(1), dim3(1), 0, st, e->sdet, ns, e->W, scap, e->perm);
@@ -2657,7 +2789,7 @@ int sort_and_reduce(fq_engine* e, int64_t ns, const int64_t* pns, bool replay) {
       }
     }
   }
-  if (pns) {  // the passes are queued: now the exact length (copied before them)
+  if (pns && !e->capturing) {  // the passes are queued: now the exact length (copied before them)
     FQ_HIP(hipEventSynchronize(e->ev_len));
     // the attempts cannot exceed the bound the spawning launches covered (every
     // entry makes at most floor(|N|) + 1 of them, |N| for integer populations)
@@ -2665,14 +2797,14 @@ int sort_and_reduce(fq_engine* e, int64_t ns, const int64_t* pns, bool replay) {
     ns = std::min<int64_t>(e->h_len[0], e->scap);
     pns = nullptr;
   }
-  FQ_HIP(hipEventRecord(e->pev[5], st));
+  mark(e, e->pev[5]);
   const int64_t nt = ceil_div(ns, kRunTile);
   int64_t* nruns = e->run_toff + nt;
   if (ns > 0) {
     auto cnt = kw == 1 ? (S == 1 ? k_run_count<1, 1> : k_run_count<1, 2>)
                        : (S == 1 ? k_run_count<2, 1> : k_run_count<2, 2>);
     hipLaunchKernelGGL(cnt, dim3(static_cast<unsigned>(nt)), dim3(kBlock), 0, st, e->kbuf[src], e->pbuf[src],
-                       nullptr, ns, scap, e->flag_bits, e->run_tcnt, e->run_tfirst, e->run_tpre);
+                       pns, ns, scap, e->flag_bits, e->run_tcnt, e->run_tfirst, e->run_tpre);
     const int rc = scan_exclusive(e, e->run_tcnt, e->run_toff, nt);
     if (rc != FQ_OK) return rc;
     KeyPlan kp = key_plan(e);
@@ -2683,7 +2815,7 @@ int sort_and_reduce(fq_engine* e, int64_t ns, const int64_t* pns, bool replay) {
                         : (S == 1 ? k_runs<2, 1, true> : k_runs<2, 2, true>);
     hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nt)), dim3(kBlock), kp.binom_lds ? tab : 0, st, e->sys, kp,
                        e->kbuf[src], e->pbuf[src], ns, scap, e->run_toff, e->run_tfirst, e->run_tpre, nt, e->rdet,
-                       e->rpop, e->rflag, e->rstate);
+                       e->rpop, e->rflag, e->rstate, pns);
   }
   // The number of runs stays on the device; kernels size their grids by a
   // host bound: ns in production, the exact count in replay (read there
@@ -2698,7 +2830,7 @@ int sort_and_reduce(fq_engine* e, int64_t ns, const int64_t* pns, bool replay) {
       FQ_HIP(hipStreamSynchronize(st));
     }
   }
-  FQ_HIP(hipEventRecord(e->pev[6], st));
+  mark(e, e->pev[6]);
   e->run_bound = bound;
   return FQ_OK;
 }
@@ -2710,15 +2842,18 @@ int annihilate(fq_engine* e, int64_t ns, const int64_t* pns, bool replay, uint32
   if (rc != FQ_OK) return rc;
   const int64_t nruns = e->run_bound;  // a bound in production, the count in replay
   const int64_t* pnr = e->p_nruns;
+  // a graph-captured cycle: n is a bound of the list's length, read on the device from dn
   const int64_t n = e->n, cap = e->cap, scap = e->scap;
+  const int64_t* dn = e->capturing ? e->dn : nullptr;
+  const uint32_t* dcyc = e->capturing ? e->dcyc : nullptr;
   const int c = e->cur;
   hipStream_t st = e->stream;
-  FQ_HIP(hipEventRecord(e->pev[7], st));
+  mark(e, e->pev[7]);
   if (nruns > 0) {
     hipLaunchKernelGGL(k_merge_bounds, dim3(grid_for(grid_for(nruns))), dim3(kBlock), 0, st, pnr, n, cap, e->W,
-                       e->det[c], e->rdet, scap, e->mwin);
+                       e->det[c], e->rdet, scap, e->mwin, dn);
     hipLaunchKernelGGL(k_merge_runs, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, pnr, n, cap, e->det[c],
-                       e->pop_new, e->rdet, e->rpop, e->rflag, scap, e->mwin, e->rpos, e->rstate);
+                       e->pop_new, e->rdet, e->rpop, e->rflag, scap, e->mwin, e->rpos, e->rstate, dn);
   }
   // Rounding draws (replay: main list first, then new candidates, in order).
   int64_t main_draws = 0, new_draws = 0;
@@ -2740,11 +2875,12 @@ int annihilate(fq_engine* e, int64_t ns, const int64_t* pns, bool replay, uint32
   }
   const uint64_t seed = e->par.philox_seed;
   hipLaunchKernelGGL(k_round_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->pop_new,
-                     replay ? 1 : 0, e->rng_buf, rng_pos, e->mdraw_off, seed, cyc, e->mmask, e->mcnt, e->det[c]);
+                     replay ? 1 : 0, e->rng_buf, rng_pos, e->mdraw_off, seed, cyc, e->mmask, e->mcnt, e->det[c], dn,
+                     dcyc);
   if (nruns > 0)
     hipLaunchKernelGGL(k_round_new, dim3(grid_for(nruns)), dim3(kBlock), 0, st, e->sys, pnr, nruns, e->rpop, e->rstate,
                        scap, replay ? 1 : 0, e->rng_buf, rng_pos + main_draws, e->ndraw_off, seed, cyc, e->nmask,
-                       e->ncnt);
+                       e->ncnt, dcyc);
   rng_pos += main_draws + new_draws;
   rc = scan_exclusive(e, e->mcnt, e->moff, ceil_div(n, 64));
   if (rc != FQ_OK) return rc;
@@ -2758,14 +2894,14 @@ int annihilate(fq_engine* e, int64_t ns, const int64_t* pns, bool replay, uint32
     hipLaunchKernelGGL(k_new_list, dim3(grid_for(nruns)), dim3(kBlock), 0, st, pnr, e->nmask, e->noff, e->rpos,
                        e->nlist, e->nrpos);
   hipLaunchKernelGGL(k_finalize_counts, dim3(1), dim3(1), 0, st, n, nruns > 0 ? pnr : nullptr, e->moff, e->mmask,
-                     e->noff, e->nmask, cap,
-                     e->d_stats);
+                     e->noff, e->nmask, cap, e->d_stats, dn);
   const int o = 1 - c;
   const int64_t nblk = ceil_div(n, kBlock);
   hipLaunchKernelGGL(k_block_starts, dim3(grid_for(nruns + 1)), dim3(kBlock), 0, st, e->nrpos, e->d_stats, e->moff,
-                     e->mmask, n, nruns, nblk, e->bstart);
+                     e->mmask, n, nruns, nblk, e->bstart, dn);
   hipLaunchKernelGGL(k_scatter_main, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, n, cap, e->det[c], e->pop_new,
-                     e->kii[c], e->mmask, e->moff, e->nrpos, e->bstart, e->d_stats, e->det[o], e->pop[o], e->kii[o]);
+                     e->kii[c], e->mmask, e->moff, e->nrpos, e->bstart, e->d_stats, e->det[o], e->pop[o], e->kii[o],
+                     dn);
   if (nruns > 0) {
     // dense pair tables in LDS for the diagonal when they fit in 32 KB
     const size_t jk = e->sys.system == FQ_SYS_READ_IN && !e->batched
@@ -2776,11 +2912,12 @@ int annihilate(fq_engine* e, int64_t ns, const int64_t* pns, bool replay, uint32
                        scap, e->rdet, e->rpop, e->rpos, e->nmask, e->noff, e->nlist, e->mmask, e->moff, e->d_stats,
                        e->det[o], e->pop[o], e->kii[o], e->batched ? e->d_sys_arr : nullptr, use_lds ? 1 : 0);
   }
-  FQ_HIP(hipEventRecord(e->pev[8], st));
+  mark(e, e->pev[8]);
   // the new list holds at most n + nruns entries
   const unsigned nba = std::min<unsigned>(grid_for(std::min(n + nruns, cap)), 2048);
   hipLaunchKernelGGL(k_abs_pop, dim3(nba), dim3(kBlock), 0, st, e->d_stats, cap, e->S, e->pop[o], e->part_i);
   hipLaunchKernelGGL(k_sum_abs, dim3(1), dim3(kBlock), 0, st, e->part_i, static_cast<int>(nba), e->d_stats);
+  if (e->capturing) return FQ_OK;  // the caller reads the stats when the graph has run
   rc = sync_stats(e);
   if (rc != FQ_OK) return rc;
   e->cur = o;
@@ -3089,7 +3226,7 @@ bool learned_lds_fits(int device, int W, int nbasis) {
 // Spawned entries of a batch of nbt attempts starting at t_begin: scan the
 // per-tile success counts, pack the tiles in order, advance the spawned count.
 // ptot: the attempt total on the device (recorded in the stats).
-int pack_spawned(fq_engine* e, int64_t nbt, int64_t t_begin, const int64_t* ptot) {
+int pack_spawned(fq_engine* e, int64_t nbt, int64_t t_begin, const int64_t* ptot, int64_t tlim = -1) {
   const int64_t nt = ceil_div(nbt, kSpawnTile);
   int rc = scan_exclusive(e, e->tile_cnt, e->tile_off, nt);
   if (rc != FQ_OK) return rc;
@@ -3100,7 +3237,7 @@ int pack_spawned(fq_engine* e, int64_t nbt, int64_t t_begin, const int64_t* ptot
   hipLaunchKernelGGL(k_pack_tiles, dim3(static_cast<unsigned>(std::min<int64_t>(nt, 2048))), dim3(kBlock),
                      kp.binom_lds ? tab : 0, e->stream, e->W, e->S, nt, e->tile_cnt, e->tile_off, e->st_det,
                      e->st_pop, e->st_ent, e->bcap, e->scap, kp, e->kbuf[0], e->pbuf[0], e->d_stats);
-  hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, e->stream, e->tile_off, nt, ptot, e->d_stats);
+  hipLaunchKernelGGL(k_add_spawned, dim3(1), dim3(1), 0, e->stream, e->tile_off, nt, ptot, e->d_stats, tlim);
   return FQ_OK;
 }
 
@@ -3537,6 +3674,19 @@ static int cycle_body(fq_engine* e, const double* shift, const double* eproj_old
     rng_pos = e->h_stats.rng_pos;
     ns = e->h_stats.nspawn;
   } else {
+    if (e->online_every > 0 && n > 0 && cyc % static_cast<uint32_t>(e->online_every) == 0) {
+      // online training from the current list: this cycle already uses the update
+      fqm::Mlp& m = e->mlp;
+      for (int32_t k = 0; k < e->online_steps; ++k) {
+        int trc = fqm::mlp_sample(m, e->sys, e->det[c], e->pop[c], cap, n, e->online_batch,
+                                  static_cast<int64_t>(e->par.proposal_min_attempts),
+                                  e->par.philox_seed ^ 0x6a09e667f3bcc908ull, static_cast<uint32_t>(m.adam_t), st,
+                                  scan_cb, e);
+        if (trc == FQ_OK) trc = fqm::mlp_train_step(m, e->sys, e->online_batch, e->online_lr, true, st);
+        if (trc != FQ_OK) return trc;
+      }
+      ++e->online_runs;
+    }
     const unsigned nb = grid_for(n);
     if (n > 0)
       hipLaunchKernelGGL(k_det_pass, dim3(nb), dim3(kBlock), 0, st, e->sys, n, cap, e->det[c], e->pop[c], e->kii[c],
@@ -3605,7 +3755,7 @@ static int cycle_body(fq_engine* e, const double* shift, const double* eproj_old
                                                        : 0;
         hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), lds, st, e->d_sysp, nent, cap, tb, te, e->att_off,
                            e->det[c], e->pop[c], tau, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop,
-                           e->st_ent, e->tile_cnt, e->tile_win, e->flags, nullptr, nullptr, nullptr);
+                           e->st_ent, e->tile_cnt, e->tile_win, e->flags, nullptr, nullptr, nullptr, nullptr);
         if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
         const int prc = pack_spawned(e, nbt, tb, e->att_off + nent);
         if (prc != FQ_OK) return prc;
@@ -3933,6 +4083,12 @@ int fq_create_batch(const fq_system* systems, int32_t ninst, const fq_params* pa
     rc |= dalloc(&e->d_iatt, ninst);
     rc |= dalloc(&e->d_cterm, 2 * S * e->cap);
     rc |= dalloc(&e->d_cdeath, e->cap);
+    e->ipb = e->cap / kBlock + ninst + 2;
+    rc |= dalloc(&e->d_iblk_cnt, ninst + 1);
+    rc |= dalloc(&e->d_iblk_off, ninst + 1);
+    rc |= dalloc(&e->d_nblk_off, ninst + 1);
+    rc |= dalloc(&e->d_ipart_d, 2 * S * e->ipb);
+    rc |= dalloc(&e->d_ipart_i, (1 + S) * e->ipb);
     rc |= dalloc(&e->d_iout_d, 2 * S * ninst);
     rc |= dalloc(&e->d_iout_i, (4 + S) * ninst);
     if (rc != FQ_OK) rc = FQ_ERR_NOMEM;
@@ -3971,10 +4127,301 @@ int fq_load_batch(fq_engine* e, const int32_t* inst, const uint64_t* states, con
   return fq_load(e, st.data(), po.data(), da.data(), n);  // the encoded, sorted list
 }
 
+// Per-instance estimators and counts of a batched cycle (after annihilation;
+// old-list bounds ob, new-list bounds nbd, nold and nnew bounds of the two
+// lengths, pnew the new list's populations).
+static int inst_stats(fq_engine* e, const int64_t* ob, const int64_t* nbd, int64_t nold, int64_t nnew,
+                      const int64_t* pnew) {
+  const int ninst = e->ninst, S = e->S;
+  hipStream_t st = e->stream;
+  hipLaunchKernelGGL(k_inst_nblk, dim3(grid_for(ninst)), dim3(kBlock), 0, st, ninst, ob, e->d_iblk_cnt);
+  int rc = scan_exclusive(e, e->d_iblk_cnt, e->d_iblk_off, ninst);
+  if (rc != FQ_OK) return rc;
+  hipLaunchKernelGGL(k_inst_nblk, dim3(grid_for(ninst)), dim3(kBlock), 0, st, ninst, nbd, e->d_iblk_cnt);
+  rc = scan_exclusive(e, e->d_iblk_cnt, e->d_nblk_off, ninst);
+  if (rc != FQ_OK) return rc;
+  // blocks of either list: at most (length / 256) + ninst
+  const int64_t nblk = std::min<int64_t>(std::max<int64_t>(nold, nnew) / kBlock + ninst + 1, e->ipb);
+  hipLaunchKernelGGL(k_inst_part, dim3(static_cast<unsigned>(nblk)), dim3(kBlock), 0, st, ninst, S, e->cap, ob,
+                     e->d_iblk_off, e->d_cterm, e->d_cdeath, nbd, e->d_nblk_off, pnew, e->ipb,
+                     e->d_ipart_d, e->d_ipart_i);
+  hipLaunchKernelGGL(k_inst_stats, dim3(ninst), dim3(kBlock), 0, st, ninst, S, ob, nbd, e->d_isb, e->d_iblk_off,
+                     e->d_nblk_off, e->ipb, e->d_ipart_d, e->d_ipart_i, e->att_off, e->d_iout_d, e->d_iout_i);
+  return FQ_OK;
+}
+
+// Per-instance results of a batched cycle from the k_inst_stats outputs.
+static void batch_outputs(const fq_engine* e, bool spawn_ovf, bool main_ovf, const double* od, const int64_t* oi,
+                          fq_cycle_out* out) {
+  const int S = e->S;
+  const double rf = static_cast<double>(e->sys.rf);
+  for (int k = 0; k < e->ninst; ++k) {
+    fq_cycle_out& r = out[k];
+    std::memset(&r, 0, sizeof(r));
+    const int64_t* q = oi + k * (4 + S);
+    for (int sp = 0; sp < S; ++sp) {
+      r.proj_energy[sp] = od[k * 2 * S + 2 * sp];
+      r.D0_population[sp] = od[k * 2 * S + 2 * sp + 1];
+      r.nparticles[sp] = static_cast<double>(q[4 + sp]) / rf;
+    }
+    r.ndeath = q[0];
+    r.nattempts = q[1];
+    r.nspawn_events = q[2];
+    r.nstates = q[3];
+    r.spawn_overflow = spawn_ovf ? 1 : 0;
+    r.main_overflow = main_ovf ? 1 : 0;
+  }
+}
+
+// ---- graph-captured batched cycles ---------------------------------------------
+// Launch bound of a length: the next multiple of a quarter of its octave (at
+// most 25% past it), so one graph serves while the length stays in its step.
+static int64_t len_step(int64_t x) {
+  if (x <= 4096) return 4096;
+  const int e = 63 - __builtin_clzll(static_cast<unsigned long long>(x - 1));  // 2^e <= x - 1 < 2^(e+1)
+  const int64_t step = int64_t{1} << (e - 2);
+  return ceil_div(x, step) * step;
+}
+// Candidate schedules: (8-bit digits only, spawning grid cap); index 0 is the
+// one ordinary cycles use.
+constexpr int kNumSched = 6;
+constexpr int kSchedDigit8[kNumSched] = {0, 0, 0, 1, 1, 1};
+constexpr int kSchedSpawnBlocks[kNumSched] = {0, 2048, 1024, 0, 2048, 1024};
+constexpr int kTuneReps = 3;  // timed graph cycles per candidate
+
+static size_t graph_in_bytes(const fq_engine* e) { return 16 + sizeof(double) * (1 + 2 * e->S) * e->ninst; }
+
+// The whole batched cycle for list buffer c, launches sized by nb (main list)
+// and tb (attempts), lengths and the cycle index read on the device, inputs
+// copied from h_in first and outputs to the pinned buffers last.  Called while
+// the stream is captured.
+static int enqueue_graph_cycle(fq_engine* e, int64_t nb, int64_t tb) {
+  const int ninst = e->ninst, S = e->S;
+  const int64_t cap = e->cap;
+  const int c = e->cur;
+  hipStream_t st = e->stream;
+  const int64_t* dn = reinterpret_cast<const int64_t*>(e->d_in);
+  const uint32_t* dcyc = reinterpret_cast<const uint32_t*>(e->d_in + 8);
+  const double* par = reinterpret_cast<const double*>(e->d_in + 16);
+  FQ_HIP(hipMemcpyAsync(e->d_in, e->h_in, graph_in_bytes(e), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, reinterpret_cast<int64_t*>(e->d_stats),
+                     static_cast<int64_t>(sizeof(Stats) / sizeof(int64_t)));
+  int64_t* ob = e->d_ibound;
+  int64_t* nbd = e->d_ibound + (ninst + 1);
+  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[c], nb, ninst, kInstShift,
+                     ob, dn);
+  hipLaunchKernelGGL(k_det_pass_batch, dim3(grid_for(nb)), dim3(kBlock), 0, st, e->sys, e->d_sys_arr, ob, par, ninst,
+                     nb, cap, e->det[c], e->pop[c], e->kii[c], e->pop_new, e->nat, e->flags, e->par.philox_seed, 0u,
+                     e->d_cterm, e->d_cdeath, dn, dcyc);
+  // entries past the list have no attempts, so att_off[S nb] is the total
+  const int64_t nent = S * nb;
+  int rc = scan_exclusive(e, e->nat, e->att_off, nent);
+  if (rc != FQ_OK) return rc;
+  hipLaunchKernelGGL(k_inst_att, dim3(grid_for(ninst)), dim3(kBlock), 0, st, ninst, S, ob, e->att_off, e->d_iatt);
+  const unsigned g = static_cast<unsigned>(std::min<int64_t>(
+      ceil_div(tb, kSpawnTile), e->spawn_blocks > 0 ? e->spawn_blocks : int64_t{1} << 20));
+  hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(ceil_div(tb, kSpawnTile))), dim3(kBlock), 0, st, nent, e->att_off,
+                     int64_t{0}, tb, e->tile_win, e->tile_cnt);
+  auto kern = e->sys.system == FQ_SYS_HUB_K ? k_spawn<fqd::kGenHubK, true> : k_spawn<fqd::kGenMolRenorm, true>;
+  hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->d_sysp, nent, cap, int64_t{0}, tb, e->att_off, e->det[c],
+                     e->pop[c], 0.0, e->par.philox_seed, 0u, e->bcap, e->st_det, e->st_pop, e->st_ent, e->tile_cnt,
+                     e->tile_win, e->flags, e->d_sys_arr, e->d_iatt, par, dcyc);
+  rc = pack_spawned(e, tb, 0, e->att_off + nent, tb);
+  if (rc != FQ_OK) return rc;
+  e->keys_ready = true;
+  const int64_t nsb = std::min<int64_t>(tb, e->scap);
+  const int64_t* pns = &e->d_stats->nspawn;
+  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->kbuf[0], nsb, ninst,
+                     kInstShift + e->flag_bits, e->d_isb, pns);
+  int64_t rng_pos = e->rng_pos;
+  rc = annihilate(e, nsb, pns, false, 0u, rng_pos);
+  if (rc != FQ_OK) return rc;
+  const int o = 1 - c;
+  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[o],
+                     std::min<int64_t>(nb + nsb, cap), ninst, kInstShift, nbd, &e->d_stats->new_n);
+  rc = inst_stats(e, ob, nbd, nb, std::min<int64_t>(nb + nsb, cap), e->pop[o]);
+  if (rc != FQ_OK) return rc;
+  FQ_HIP(hipMemcpyAsync(e->h_gstats, e->d_stats, sizeof(Stats), hipMemcpyDeviceToHost, st));
+  FQ_HIP(hipMemcpyAsync(e->h_god, e->d_iout_d, sizeof(double) * 2 * S * ninst, hipMemcpyDeviceToHost, st));
+  FQ_HIP(hipMemcpyAsync(e->h_goi, e->d_iout_i, sizeof(int64_t) * (4 + S) * ninst, hipMemcpyDeviceToHost, st));
+  return FQ_OK;
+}
+
+// The graph of (list buffer, bounds, schedule), captured on first use.
+static int graph_for(fq_engine* e, int64_t nb, int64_t tb, int sched, hipGraphExec_t* out) {
+  for (const auto& g : e->graphs)
+    if (g.c == e->cur && g.nb == nb && g.tb == tb && g.sched == sched) {
+      *out = g.exec;
+      return FQ_OK;
+    }
+  // no allocation may happen inside the capture: size the scan scratch for
+  // the longest scan of the cycle first
+  const int64_t longest = std::max<int64_t>(e->S * nb, tb);
+  if (longest > kSmallScanMax) {
+    const int64_t ntile = ceil_div(longest, kScanTile);
+    if (!scan_scratch(e, 0, ntile) || !scan_scratch(e, 1, ntile + 1)) return FQ_ERR_NOMEM;
+  }
+  e->digit8 = kSchedDigit8[sched];
+  e->spawn_blocks = kSchedSpawnBlocks[sched];
+  const int64_t n_keep = e->n;
+  e->n = nb;  // annihilation sizes its launches by e->n
+  e->dn = reinterpret_cast<const int64_t*>(e->d_in);
+  e->dcyc = reinterpret_cast<const uint32_t*>(e->d_in + 8);
+  e->capturing = true;
+  hipGraph_t graph = nullptr;
+  int rc = FQ_OK;
+  if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    rc = FQ_ERR_DEVICE;
+  } else {
+    rc = enqueue_graph_cycle(e, nb, tb);
+    if (hipStreamEndCapture(e->stream, &graph) != hipSuccess && rc == FQ_OK) rc = FQ_ERR_DEVICE;
+  }
+  e->capturing = false;
+  e->dn = nullptr;
+  e->dcyc = nullptr;
+  e->n = n_keep;
+  e->digit8 = 0;
+  e->spawn_blocks = 0;
+  hipGraphExec_t exec = nullptr;
+  if (rc == FQ_OK && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = FQ_ERR_DEVICE;
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc != FQ_OK) return rc;
+  e->graphs.push_back({e->cur, nb, tb, sched, exec});
+  *out = exec;
+  return FQ_OK;
+}
+
+static float median_of(std::vector<float> v) {
+  std::sort(v.begin(), v.end());
+  return v.empty() ? 0.0f : v[v.size() / 2];
+}
+
+// One batched cycle as a graph launch; *done = false when the cycle must run
+// as ordinary launches (attempt bound unknown or past the staging capacity).
+static int graph_cycle(fq_engine* e, const double* shift, const double* eproj_old, const double* tau,
+                       fq_cycle_out* out, bool* done) {
+  *done = false;
+  const int ninst = e->ninst, S = e->S;
+  if (e->n <= 0 || e->att_bound <= 0) return FQ_OK;
+  const int64_t nb = std::min<int64_t>(len_step(e->n), e->cap);
+  const int64_t tb = std::min<int64_t>(len_step(e->att_bound), e->bcap);
+  if (e->att_bound > tb) return FQ_OK;
+  if (int rs = sync_sysp(e); rs != FQ_OK) return rs;
+  // the tuner works per octave of the attempt bound (the schedules' relative
+  // cost changes with the workload's size): candidates in turn until each has
+  // run kTuneReps times, then the fastest by median
+  const int oct = 63 - __builtin_clzll(static_cast<unsigned long long>(tb));
+  e->last_oct = oct;
+  int sched = e->sched[oct];
+  if (sched < 0) sched = e->tune_iter[oct] % kNumSched;
+  hipGraphExec_t exec = nullptr;
+  int rc = graph_for(e, nb, tb, sched, &exec);
+  if (rc != FQ_OK) return rc;
+  // inputs: the list length, the cycle index, then tau, shift, E_proj per instance
+  const uint32_t cyc = static_cast<uint32_t>(e->cycle_index++);
+  int64_t* hi = reinterpret_cast<int64_t*>(e->h_in);
+  hi[0] = e->n;
+  hi[1] = static_cast<int64_t>(cyc);
+  double* par = reinterpret_cast<double*>(e->h_in + 16);
+  for (int k = 0; k < ninst; ++k) {
+    par[k] = tau[k];
+    for (int sp = 0; sp < S; ++sp) {
+      par[ninst + sp * ninst + k] = shift[k * S + sp];
+      par[(1 + S) * ninst + sp * ninst + k] = eproj_old[k * S + sp];
+    }
+  }
+  FQ_HIP(hipEventRecord(e->gev[0], e->stream));
+  FQ_HIP(hipGraphLaunch(exec, e->stream));
+  FQ_HIP(hipEventRecord(e->gev[1], e->stream));
+  FQ_HIP(hipStreamSynchronize(e->stream));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e->gev[0], e->gev[1]);
+  ++e->graph_cycles;
+  if (e->sched[oct] < 0) {
+    e->tune_ms[oct][sched].push_back(ms);
+    if (++e->tune_iter[oct] >= kNumSched * kTuneReps) {
+      int best = 0;
+      for (int k = 1; k < kNumSched; ++k)
+        if (median_of(e->tune_ms[oct][k]) < median_of(e->tune_ms[oct][best])) best = k;
+      e->sched[oct] = best;
+    }
+  }
+  e->h_stats = *e->h_gstats;
+  if (e->h_stats.spawn_overflow && e->h_stats.total_attempts > tb) return FQ_ERR_DEVICE;  // the bound was wrong
+  e->cur = 1 - e->cur;
+  e->n = e->h_stats.new_n;
+  unsigned long long sum = 0;
+  for (int sp = 0; sp < S; ++sp) sum += e->h_stats.abs_pop[sp];
+  e->att_bound = static_cast<int64_t>(sum / static_cast<unsigned long long>(e->sys.rf));
+  e->t_total = ms;
+  e->prof = fq_profile_out{};
+  e->prof.total_ms = ms;
+  e->prof.nattempts = e->h_stats.total_attempts;
+  e->prof.nspawned = std::min<int64_t>(e->h_stats.nspawn, e->scap);
+  e->prof.nruns = e->h_stats.nruns;
+  e->prof.nstates_out = e->n;
+  batch_outputs(e, e->h_stats.spawn_overflow != 0, e->h_stats.main_overflow != 0, e->h_god, e->h_goi, out);
+  *done = true;
+  return FQ_OK;
+}
+
+int fq_batch_graph(fq_engine* e, int32_t enable) {
+  if (!e || !e->batched) return FQ_ERR_ARG;
+  FQ_HIP(hipSetDevice(e->device));
+  if (enable && !e->h_in) {
+    const size_t nin = graph_in_bytes(e);
+    if (hipHostMalloc(reinterpret_cast<void**>(&e->h_in), nin) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&e->h_gstats), sizeof(Stats)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&e->h_god), sizeof(double) * 2 * e->S * e->ninst) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&e->h_goi), sizeof(int64_t) * (4 + e->S) * e->ninst) != hipSuccess)
+      return FQ_ERR_NOMEM;
+    if (dalloc(&e->d_in, static_cast<int64_t>(nin)) != FQ_OK) return FQ_ERR_NOMEM;
+    for (int k = 0; k < 2; ++k)
+      if (hipEventCreate(&e->gev[k]) != hipSuccess) return FQ_ERR_DEVICE;
+    for (int o = 0; o < 64; ++o) {
+      e->sched[o] = -1;
+      e->tune_iter[o] = 0;
+    }
+  }
+  e->graph_on = enable != 0;
+  return FQ_OK;
+}
+
+int fq_batch_schedule(fq_engine* e, fq_schedule_out* out) {
+  if (!e || !e->batched || !out) return FQ_ERR_ARG;
+  std::memset(out, 0, sizeof(*out));
+  const int oct = e->last_oct;  // the last graph cycle's octave
+  const int ch = oct >= 0 ? e->sched[oct] : -1;
+  const int k = ch < 0 ? 0 : ch;
+  out->graph = e->graph_on ? 1 : 0;
+  out->tuned = ch >= 0 ? 1 : 0;
+  out->chosen = ch;
+  out->octave = oct;
+  for (int o = 0; o < 64; ++o) out->tuned_octaves += e->h_in && e->sched[o] >= 0 ? 1 : 0;
+  out->digit_bits = kSchedDigit8[k] ? 8 : 9;
+  out->spawn_blocks = kSchedSpawnBlocks[k];
+  out->ncand = kNumSched;
+  out->graph_cycles = e->graph_cycles;
+  out->graphs = static_cast<int64_t>(e->graphs.size());
+  out->eager_cycles = e->eager_cycles;
+  for (int q = 0; q < kNumSched; ++q) {
+    out->cand_ms[q] = oct >= 0 ? median_of(e->tune_ms[oct][q]) : 0.0;
+    out->cand_digit_bits[q] = kSchedDigit8[q] ? 8 : 9;
+    out->cand_spawn_blocks[q] = kSchedSpawnBlocks[q];
+  }
+  return FQ_OK;
+}
+
 int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, const double* tau,
                    fq_cycle_out* out) {
   if (!e || !e->batched || !shift || !eproj_old || !tau || !out) return FQ_ERR_ARG;
   FQ_HIP(hipSetDevice(e->device));
+  if (e->graph_on) {
+    bool done = false;
+    const int grc = graph_cycle(e, shift, eproj_old, tau, out, &done);
+    if (grc != FQ_OK || done) return grc;
+  }
+  ++e->eager_cycles;
   const int ninst = e->ninst, S = e->S;
   const int64_t n = e->n, cap = e->cap;
   const int c = e->cur;
@@ -3989,16 +4436,18 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
     }
   }
   FQ_HIP(hipMemcpyAsync(e->d_ipar, par.data(), sizeof(double) * par.size(), hipMemcpyHostToDevice, st));
-  FQ_HIP(hipMemsetAsync(e->d_stats, 0, sizeof(Stats), st));
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, reinterpret_cast<int64_t*>(e->d_stats),
+                     static_cast<int64_t>(sizeof(Stats) / sizeof(int64_t)));
   FQ_HIP(hipEventRecord(e->ev[0], st));
   FQ_HIP(hipEventRecord(e->pev[0], st));
   int64_t* ob = e->d_ibound;
   int64_t* nbd = e->d_ibound + (ninst + 1);
-  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[c], n, ninst, kInstShift, ob);
+  hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[c], n, ninst, kInstShift, ob,
+                     nullptr);
   if (n > 0)
     hipLaunchKernelGGL(k_det_pass_batch, dim3(grid_for(n)), dim3(kBlock), 0, st, e->sys, e->d_sys_arr, ob, e->d_ipar,
                        ninst, n, cap, e->det[c], e->pop[c], e->kii[c], e->pop_new, e->nat, e->flags,
-                       e->par.philox_seed, cyc, e->d_cterm, e->d_cdeath);
+                       e->par.philox_seed, cyc, e->d_cterm, e->d_cdeath, nullptr, nullptr);
   FQ_HIP(hipEventRecord(e->pev[1], st));
   const int64_t nent = S * n;
   int rc = scan_exclusive(e, e->nat, e->att_off, nent);
@@ -4021,7 +4470,7 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
     auto kern = e->sys.system == FQ_SYS_HUB_K ? k_spawn<fqd::kGenHubK, true> : k_spawn<fqd::kGenMolRenorm, true>;
     hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, e->d_sysp, nent, cap, tb, te, e->att_off, e->det[c],
                        e->pop[c], 0.0, e->par.philox_seed, cyc, e->bcap, e->st_det, e->st_pop, e->st_ent,
-                       e->tile_cnt, e->tile_win, e->flags, e->d_sys_arr, e->d_iatt, e->d_ipar);
+                       e->tile_cnt, e->tile_win, e->flags, e->d_sys_arr, e->d_iatt, e->d_ipar, nullptr);
     if (li < fq_engine::kSpawnEvents) FQ_HIP(hipEventRecord(e->sev[2 * li + 1], st));
     rc = pack_spawned(e, nbt, tb, e->att_off + nent);
     if (rc != FQ_OK) return rc;
@@ -4033,16 +4482,16 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
   const int64_t ns = std::min<int64_t>(e->h_stats.nspawn, e->scap);
   // spawned entries are in attempt order, hence grouped by instance
   hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->kbuf[0], ns, ninst,
-                     kInstShift + e->flag_bits, e->d_isb);
+                     kInstShift + e->flag_bits, e->d_isb, nullptr);
   const Stats keep = e->h_stats;
   int64_t rng_pos = e->rng_pos;
   rc = annihilate(e, ns, nullptr, false, cyc, rng_pos);
   if (rc != FQ_OK) return rc;
   const int o = e->cur;
   hipLaunchKernelGGL(k_inst_bounds, dim3(grid_for(ninst + 1)), dim3(kBlock), 0, st, e->det[o], e->n, ninst, kInstShift,
-                     nbd);
-  hipLaunchKernelGGL(k_inst_stats, dim3(ninst), dim3(kBlock), 0, st, ninst, S, cap, ob, nbd, e->d_isb, e->d_cterm,
-                     e->d_cdeath, e->att_off, e->pop[o], e->d_iout_d, e->d_iout_i);
+                     nbd, nullptr);
+  rc = inst_stats(e, ob, nbd, n, e->n, e->pop[o]);
+  if (rc != FQ_OK) return rc;
   std::vector<double> od(static_cast<size_t>(2 * S * ninst));
   std::vector<int64_t> oi(static_cast<size_t>((4 + S) * ninst));
   FQ_HIP(hipMemcpyAsync(od.data(), e->d_iout_d, sizeof(double) * od.size(), hipMemcpyDeviceToHost, st));
@@ -4079,23 +4528,11 @@ int fq_cycle_batch(fq_engine* e, const double* shift, const double* eproj_old, c
       pr.spawn_kernel_ms += ms;
     }
   }
-  const double rf = static_cast<double>(e->sys.rf);
-  for (int k = 0; k < ninst; ++k) {
-    fq_cycle_out& r = out[k];
-    std::memset(&r, 0, sizeof(r));
-    const int64_t* q = oi.data() + k * (4 + S);
-    for (int sp = 0; sp < S; ++sp) {
-      r.proj_energy[sp] = od[k * 2 * S + 2 * sp];
-      r.D0_population[sp] = od[k * 2 * S + 2 * sp + 1];
-      r.nparticles[sp] = static_cast<double>(q[4 + sp]) / rf;
-    }
-    r.ndeath = q[0];
-    r.nattempts = q[1];
-    r.nspawn_events = q[2];
-    r.nstates = q[3];
-    r.spawn_overflow = keep.spawn_overflow ? 1 : 0;
-    r.main_overflow = e->h_stats.main_overflow ? 1 : 0;
-  }
+  batch_outputs(e, keep.spawn_overflow != 0, e->h_stats.main_overflow != 0, od.data(), oi.data(), out);
+  // the next cycle's attempt total: sum |N| over the new list (integer populations)
+  unsigned long long sum = 0;
+  for (int sp = 0; sp < S; ++sp) sum += e->h_stats.abs_pop[sp];
+  e->att_bound = static_cast<int64_t>(sum / static_cast<unsigned long long>(e->sys.rf));
   return FQ_OK;
 }
 
@@ -4158,8 +4595,8 @@ int fq_mlp_forward(fq_engine* e, const uint64_t* dets, int64_t n, float* out) {
   return rc;
 }
 
-int fq_mlp_loss_grad(fq_engine* e, const uint64_t* dets, const int32_t* exc, const double* w, const int32_t* flat,
-                     int64_t n, double* loss, float* grad) {
+static int loss_grad(fq_engine* e, const uint64_t* dets, const int32_t* exc, const double* w, const double* habs,
+                     const int32_t* flat, int64_t n, double* loss, float* grad) {
   if (!e || !dets || !exc || !w || !loss || n < 1 || e->par.excit_gen != FQ_EXCIT_LEARNED) return FQ_ERR_ARG;
   fqm::Mlp& m = e->mlp;
   if (n > m.bcap) return FQ_ERR_CAPACITY;
@@ -4182,15 +4619,43 @@ int fq_mlp_loss_grad(fq_engine* e, const uint64_t* dets, const int32_t* exc, con
     FQ_HIP(hipMemcpyAsync(m.sdet + q * m.bcap, soa.data() + q * n, sizeof(uint64_t) * n, hipMemcpyHostToDevice, st));
   FQ_HIP(hipMemcpyAsync(m.sexc, exc, sizeof(int32_t) * 4 * n, hipMemcpyHostToDevice, st));
   FQ_HIP(hipMemcpyAsync(m.sw, w, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  if (habs) FQ_HIP(hipMemcpyAsync(m.sh, habs, sizeof(double) * n, hipMemcpyHostToDevice, st));
   const std::vector<int32_t> fl = flat ? std::vector<int32_t>(flat, flat + n) : std::vector<int32_t>(n, 0);
   FQ_HIP(hipMemcpyAsync(m.sflat, fl.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  const int keep = m.objective;
+  m.objective = habs ? 1 : 0;
   int rc = fqm::mlp_train_step(m, e->sys, n, 0.0, false, st);
+  m.objective = keep;
   if (rc != FQ_OK) return rc;
   double sums[2] = {0.0, 0.0};
   FQ_HIP(hipMemcpyAsync(sums, m.dsum, sizeof(sums), hipMemcpyDeviceToHost, st));
   if (grad) FQ_HIP(hipMemcpyAsync(grad, m.g, sizeof(float) * m.nparams, hipMemcpyDeviceToHost, st));
   FQ_HIP(hipStreamSynchronize(st));
   *loss = sums[1];
+  return FQ_OK;
+}
+
+int fq_mlp_loss_grad(fq_engine* e, const uint64_t* dets, const int32_t* exc, const double* w, const int32_t* flat,
+                     int64_t n, double* loss, float* grad) {
+  return loss_grad(e, dets, exc, w, nullptr, flat, n, loss, grad);
+}
+
+int fq_mlp_variance_grad(fq_engine* e, const uint64_t* dets, const int32_t* exc, const double* w,
+                         const double* habs, const int32_t* flat, int64_t n, double* loss, float* grad) {
+  if (!habs) return FQ_ERR_ARG;
+  return loss_grad(e, dets, exc, w, habs, flat, n, loss, grad);
+}
+
+int fq_mlp_online(fq_engine* e, int32_t every, int32_t steps, int64_t batch, double lr, int32_t objective) {
+  if (!e || e->par.excit_gen != FQ_EXCIT_LEARNED || every < 0 || steps < 0 || (objective != 0 && objective != 1))
+    return FQ_ERR_ARG;
+  if (every > 0 && (batch < 1 || !(lr > 0.0))) return FQ_ERR_ARG;
+  if (batch > e->mlp.bcap) return FQ_ERR_CAPACITY;
+  e->online_every = every;
+  e->online_steps = steps;
+  e->online_batch = batch;
+  e->online_lr = lr;
+  e->mlp.objective = objective;
   return FQ_OK;
 }
 
@@ -4366,6 +4831,11 @@ void fq_destroy(fq_engine* e) {
   dfree(e->d_iatt);
   dfree(e->d_cterm);
   dfree(e->d_cdeath);
+  dfree(e->d_iblk_cnt);
+  dfree(e->d_iblk_off);
+  dfree(e->d_nblk_off);
+  dfree(e->d_ipart_d);
+  dfree(e->d_ipart_i);
   dfree(e->d_iout_d);
   dfree(e->d_iout_i);
   for (fq_engine* sub : e->subs) fq_destroy(sub);
@@ -4378,6 +4848,14 @@ void fq_destroy(fq_engine* e) {
     if (e->sev[k]) (void)hipEventDestroy(e->sev[k]);
   if (e->ev_len) (void)hipEventDestroy(e->ev_len);
   if (e->h_len) (void)hipHostFree(e->h_len);
+  for (auto& g : e->graphs) (void)hipGraphExecDestroy(g.exec);
+  for (int k = 0; k < 2; ++k)
+    if (e->gev[k]) (void)hipEventDestroy(e->gev[k]);
+  if (e->h_in) (void)hipHostFree(e->h_in);
+  if (e->h_gstats) (void)hipHostFree(e->h_gstats);
+  if (e->h_god) (void)hipHostFree(e->h_god);
+  if (e->h_goi) (void)hipHostFree(e->h_goi);
+  dfree(e->d_in);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }

@@ -46,21 +46,33 @@ __device__ inline float gelu_g(float z) {
 // Rows [row, row+16) of the block's tile times W (stored transposed [N][K]):
 // acc[t] is the 16x16 output tile t (v_mfma_f32_16x16x32_bf16: lane l feeds
 // A[row l%16][k 8(l/16)..+7] and B[k 8(l/16)..+7][col l%16] and holds
-// D[rows 4(l/16)..+3][col l%16]).
+// D[rows 4(l/16)..+3][col l%16]).  Each 32-wide slab of K of W is staged once
+// per block in LDS (slab: the activation buffer this layer does not use),
+// so the block's four waves share one read of the weights; every thread of
+// the block calls this.
+constexpr int kSlabLd = 40;  // slab row stride (bf16): 32 + 8 keeps 16-byte rows and spreads banks
 template <int NT>
 __device__ inline void mfma_rows16(const uint16_t* A, int lda, int K, const uint16_t* __restrict__ Wt, int ntiles,
-                                   f32x4 (&acc)[NT], int row, int kq, int cl) {
+                                   f32x4 (&acc)[NT], int row, int kq, int cl, uint16_t* slab) {
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int ncols = ntiles * 16;
   for (int k0 = 0; k0 < K; k0 += 32) {
+    for (int e = threadIdx.x; e < ncols * 4; e += blockDim.x) {  // 4 x 8 bf16 per row
+      const int n = e >> 2, part = (e & 3) * 8;
+      *reinterpret_cast<bf16x8*>(slab + n * kSlabLd + part) =
+          *reinterpret_cast<const bf16x8*>(Wt + static_cast<int64_t>(n) * K + k0 + part);
+    }
+    __syncthreads();
     const bf16x8 a = *reinterpret_cast<const bf16x8*>(A + row * lda + k0 + kq);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       if (t < ntiles) {
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(Wt + static_cast<int64_t>(t * 16 + cl) * K + k0 + kq);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(slab + (t * 16 + cl) * kSlabLd + kq);
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
       }
     }
+    __syncthreads();
   }
 }
 
@@ -92,7 +104,8 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(int nb, int k1, int n3, in
   const int row = wave * 16 + (lane & 15), kq = (lane >> 4) * 8, cl = lane & 15;
   const int orow = wave * 16 + (lane >> 4) * 4;
   f32x4 acc[16];
-  mfma_rows16<16>(bufA, k1, k1, w1t, 16, acc, row, kq, cl);
+  static_assert(kHidden * kSlabLd <= kInferRows * kHidden, "a weight slab fits the idle activation buffer");
+  mfma_rows16<16>(bufA, k1, k1, w1t, 16, acc, row, kq, cl, bufB);
 #pragma unroll
   for (int t = 0; t < 16; ++t)
 #pragma unroll
@@ -101,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(int nb, int k1, int n3, in
       bufB[(orow + r) * kHidden + col] = f2bf(gelu_f(acc[t][r] + b1[col]));
     }
   __syncthreads();
-  mfma_rows16<16>(bufB, kHidden, kHidden, w2t, 16, acc, row, kq, cl);
+  mfma_rows16<16>(bufB, kHidden, kHidden, w2t, 16, acc, row, kq, cl, bufA);
   __syncthreads();
 #pragma unroll
   for (int t = 0; t < 16; ++t)
@@ -112,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(int nb, int k1, int n3, in
     }
   __syncthreads();
   const int nt3 = n3 / 16;
-  mfma_rows16<16>(bufA, kHidden, kHidden, w3t, nt3, acc, row, kq, cl);
+  mfma_rows16<16>(bufA, kHidden, kHidden, w3t, nt3, acc, row, kq, cl, bufB);
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     if (t >= nt3) break;
@@ -162,27 +175,72 @@ __global__ void k_mlp_feat(int64_t ns, int nb, int d_in, const uint64_t* __restr
   x[t] = occ ? (c < nb ? 1.0f : static_cast<float>(eps[o])) : 0.0f;
 }
 
-// C[M][N] = op(A)[M][K] op(B)[K][N] (+ bias[N]); op(A)[i][k] = ta ? A[k][i] : A[i][k].
-__global__ __launch_bounds__(kBlock) void k_gemm32(int M, int N, int K, const float* __restrict__ A, int lda, int ta,
-                                                   const float* __restrict__ B, int ldb, int tb,
-                                                   const float* __restrict__ bias, float* __restrict__ C, int ldc) {
-  __shared__ float As[16][17], Bs[16][17];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int i = blockIdx.y * 16 + ty, j = blockIdx.x * 16 + tx;
-  float acc = 0.0f;
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    const int ai = blockIdx.y * 16 + ty, ak = k0 + tx;
-    As[ty][tx] = (ai < M && ak < K) ? (ta ? A[static_cast<int64_t>(ak) * lda + ai] : A[static_cast<int64_t>(ai) * lda + ak])
-                                    : 0.0f;
-    const int bk = k0 + ty, bj = blockIdx.x * 16 + tx;
-    Bs[ty][tx] = (bk < K && bj < N) ? (tb ? B[static_cast<int64_t>(bj) * ldb + bk] : B[static_cast<int64_t>(bk) * ldb + bj])
-                                    : 0.0f;
+// C[M][N] = op(A)[M][K] op(B)[K][N] (+ bias[N]); op(A)[i][k] = ta ? A[k][i] : A[i][k],
+// on fp32 matrix cores: a 64x64 tile of C per block, each
+// of its four waves a 32x32 quarter as 2x2 tiles of v_mfma_f32_16x16x4_f32
+// (lane l feeds A[row l%16][k l/16] and B[k l/16][col l%16] and holds
+// D[rows 4(l/16)..+3][col l%16]); K in slabs of 16 staged through LDS with
+// the transposes applied on load.  The accumulation order is fixed by the
+// tile shape, so results are deterministic.
+constexpr int kGemmTile = 64, kGemmSlab = 16;
+__global__ __launch_bounds__(kBlock) void k_gemm_mfma(int M, int N, int K, const float* __restrict__ A, int lda,
+                                                      int ta, const float* __restrict__ B, int ldb, int tb,
+                                                      const float* __restrict__ bias, float* __restrict__ C, int ldc) {
+  __shared__ float As[kGemmTile][kGemmSlab + 1];
+  __shared__ float Bs[kGemmSlab][kGemmTile + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i0 = blockIdx.y * kGemmTile, j0 = blockIdx.x * kGemmTile;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += kGemmSlab) {
+    // 64 x 16 of op(A) and 16 x 64 of op(B): 1024 elements each, 4 per thread
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = q * kBlock + tid;
+      {  // A: consecutive threads along k (row-major A) or along i (transposed A)
+        const int r = ta ? (e & 63) : (e >> 4), kk = ta ? (e >> 6) : (e & 15);
+        const int gi = i0 + r, gk = k0 + kk;
+        As[r][kk] = (gi < M && gk < K)
+                        ? (ta ? A[static_cast<int64_t>(gk) * lda + gi] : A[static_cast<int64_t>(gi) * lda + gk])
+                        : 0.0f;
+      }
+      {  // B: along j (row-major B) or along k (transposed B)
+        const int cidx = tb ? (e >> 4) : (e & 63), kk = tb ? (e & 15) : (e >> 6);
+        const int gj = j0 + cidx, gk = k0 + kk;
+        Bs[kk][cidx] = (gj < N && gk < K)
+                           ? (tb ? B[static_cast<int64_t>(gj) * ldb + gk] : B[static_cast<int64_t>(gk) * ldb + gj])
+                           : 0.0f;
+      }
+    }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) acc = acc + As[ty][kk] * Bs[kk][tx];
+    for (int kk = 0; kk < kGemmSlab; kk += 4) {
+      const int kq = kk + (lane >> 4);
+      float av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = As[wr + a * 16 + (lane & 15)][kq];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bv[b] = Bs[kq][wc + b * 16 + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
     __syncthreads();
   }
-  if (i < M && j < N) C[static_cast<int64_t>(i) * ldc + j] = acc + (bias ? bias[j] : 0.0f);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + wr + a * 16 + (lane >> 4) * 4 + r, gj = j0 + wc + b * 16 + (lane & 15);
+        if (gi < M && gj < N) C[static_cast<int64_t>(gi) * ldc + gj] = acc[a][b][r] + (bias ? bias[gj] : 0.0f);
+      }
 }
 
 __global__ void k_gelu_fwd(int64_t n, const float* __restrict__ z, float* __restrict__ a) {
@@ -228,7 +286,8 @@ __global__ void k_mlp_loss_grad(fqd::Sys s, int64_t ns, int nb, int d_out, const
                                 int64_t bcap, const int32_t* __restrict__ sexc, const double* __restrict__ sw,
                                 const int32_t* __restrict__ sflat, const double* __restrict__ wsum,
                                 const float* __restrict__ z3, const float* __restrict__ th, float* __restrict__ d3,
-                                float* __restrict__ dth, double* __restrict__ sloss) {
+                                float* __restrict__ dth, double* __restrict__ sloss, const double* __restrict__ sh,
+                                int variance) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= ns) return;
   const float* z = z3 + t * d_out;
@@ -282,7 +341,11 @@ __global__ void k_mlp_loss_grad(fqd::Sys s, int64_t ns, int nb, int d_out, const
   const double t4 = pji * (fb * ajb / z1j) * (fa * aia / zai);  // (j, i, b, a)
   const double q = t1 + t2 + t3 + t4;
   const double r1 = t1 / q, r2 = t2 / q, r3 = t3 / q, r4 = t4 / q;
-  sloss[t] = wt * -log(q);
+  // the sample's loss and the factor of its -log q gradient: the NLL weighted
+  // by wt, or the spawning-variance term wt |H| / q, whose gradient is the NLL
+  // gradient weighted by that term
+  const double c = variance ? wt * sh[t] / q : wt;
+  sloss[t] = variance ? c : wt * -log(q);
   for (int o = 1; o <= nb; ++o) {
     double gu = 0.0, gi = 0.0, gj = 0.0;
     if (fqd::btest(f, o)) {
@@ -304,12 +367,12 @@ __global__ void k_mlp_loss_grad(fqd::Sys s, int64_t ns, int nb, int d_out, const
            r4 * (db - wj / z1j);
     }
     const bool in_h = !flat && fabsf(z[o - 1]) < kClip, in_p = !flat && fabsf(z[nb + o - 1]) < kClip;
-    d[o - 1] = in_h ? static_cast<float>(-wt * gu) : 0.0f;
-    d[nb + o - 1] = in_p ? static_cast<float>(-wt * (gi + gj)) : 0.0f;
-    gth[o - 1] = fabsf(th[(i - 1) * nb + o - 1]) < kClip ? static_cast<float>(-wt * gi) : 0.0f;
-    gth[nb + o - 1] = fabsf(th[(j - 1) * nb + o - 1]) < kClip ? static_cast<float>(-wt * gj) : 0.0f;
+    d[o - 1] = in_h ? static_cast<float>(-c * gu) : 0.0f;
+    d[nb + o - 1] = in_p ? static_cast<float>(-c * (gi + gj)) : 0.0f;
+    gth[o - 1] = fabsf(th[(i - 1) * nb + o - 1]) < kClip ? static_cast<float>(-c * gi) : 0.0f;
+    gth[nb + o - 1] = fabsf(th[(j - 1) * nb + o - 1]) < kClip ? static_cast<float>(-c * gj) : 0.0f;
   }
-  for (int c = 2 * nb; c < d_out; ++c) d[c] = 0.0f;
+  for (int k = 2 * nb; k < d_out; ++k) d[k] = 0.0f;
 }
 
 // Affinity gradient, deterministic: block (h, c) sums, in sample order, the
@@ -430,7 +493,8 @@ __global__ void k_det_weight(int64_t n, int64_t cap, int S, const int64_t* __res
 __global__ void k_hb_sample(fqd::Sys s, const uint64_t* __restrict__ det, int64_t cap, int64_t n,
                             const int64_t* __restrict__ cum, const int64_t* __restrict__ dw, int64_t min_att,
                             int64_t ns, uint64_t seed, uint32_t step, uint64_t* __restrict__ sdet, int64_t bcap,
-                            int32_t* __restrict__ sexc, double* __restrict__ sw, int32_t* __restrict__ sflat) {
+                            int32_t* __restrict__ sexc, double* __restrict__ sw, int32_t* __restrict__ sflat,
+                            double* __restrict__ sh) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= ns) return;
   fqd::Philox rng(seed, static_cast<uint32_t>(t), static_cast<uint32_t>(t >> 32), step ^ (fqd::kDrawTrain << 24));
@@ -457,6 +521,7 @@ __global__ void k_hb_sample(fqd::Sys s, const uint64_t* __restrict__ det, int64_
   sexc[4 * t + 2] = conn.to[0];
   sexc[4 * t + 3] = conn.to[1];
   sw[t] = use ? fabs(h) / pgen : 0.0;
+  sh[t] = use ? fabs(h) : 0.0;
   sflat[t] = dw[lo] < min_att ? 1 : 0;  // dw = 1 + sum of floor|N|
 }
 
@@ -475,7 +540,8 @@ void dfree(T*& p) {
 
 void gemm(int M, int N, int K, const float* A, int lda, int ta, const float* B, int ldb, int tb, const float* bias,
           float* C, int ldc, hipStream_t st) {
-  hipLaunchKernelGGL(k_gemm32, dim3(static_cast<unsigned>(ceil_div(N, 16)), static_cast<unsigned>(ceil_div(M, 16))),
+  hipLaunchKernelGGL(k_gemm_mfma, dim3(static_cast<unsigned>(ceil_div(N, kGemmTile)),
+                                      static_cast<unsigned>(ceil_div(M, kGemmTile))),
                      dim3(kBlock), 0, st, M, N, K, A, lda, ta, B, ldb, tb, bias, C, ldc);
 }
 
@@ -530,6 +596,7 @@ int mlp_alloc(Mlp& m, int nb, int W, int64_t bcap) {
   rc |= dalloc(&m.sdet, bcap * W);
   rc |= dalloc(&m.sexc, bcap * 4);
   rc |= dalloc(&m.sw, bcap);
+  rc |= dalloc(&m.sh, bcap);
   rc |= dalloc(&m.sflat, bcap);
   rc |= dalloc(&m.sloss, bcap);
   rc |= dalloc(&m.dsum, 2);
@@ -559,6 +626,7 @@ void mlp_free(Mlp& m) {
   dfree(m.sdet);
   dfree(m.sexc);
   dfree(m.sw);
+  dfree(m.sh);
   dfree(m.sflat);
   dfree(m.sloss);
   dfree(m.dsum);
@@ -658,7 +726,7 @@ int mlp_train_step(Mlp& m, const fqd::Sys& s, int64_t ns, double lr, bool apply,
   gemm(B, dout, H, m.a2, H, 0, m.p + m.o_w3, dout, 0, m.p + m.o_b3, m.z3, dout, st);
   hipLaunchKernelGGL(k_dsum, dim3(1), dim3(kBlock), 0, st, ns, m.sw, m.dsum);
   hipLaunchKernelGGL(k_mlp_loss_grad, dim3(blocks(ns)), dim3(kBlock), 0, st, s, ns, m.nb, dout, m.sdet, m.bcap,
-                     m.sexc, m.sw, m.sflat, m.dsum, m.z3, m.p + m.o_th, m.d3, m.dth, m.sloss);
+                     m.sexc, m.sw, m.sflat, m.dsum, m.z3, m.p + m.o_th, m.d3, m.dth, m.sloss, m.sh, m.objective);
   hipLaunchKernelGGL(k_dsum, dim3(1), dim3(kBlock), 0, st, ns, m.sloss, m.dsum + 1);
   const int nchunk = static_cast<int>(ceil_div(ns, kAffChunk));
   hipLaunchKernelGGL(k_aff_grad_part, dim3(static_cast<unsigned>(m.nb), static_cast<unsigned>(nchunk)),
@@ -712,7 +780,7 @@ int mlp_sample(Mlp& m, const fqd::Sys& s, const uint64_t* det, const int64_t* po
   int rc = scan(scan_ctx, m.swts, m.scum, n);
   if (rc != FQ_OK) return rc;
   hipLaunchKernelGGL(k_hb_sample, dim3(blocks(ns)), dim3(kBlock), 0, st, s, det, cap, n, m.scum, m.swts, min_att,
-                     ns, seed, step, m.sdet, m.bcap, m.sexc, m.sw, m.sflat);
+                     ns, seed, step, m.sdet, m.bcap, m.sexc, m.sw, m.sflat, m.sh);
   MLP_HIP(hipGetLastError());
   return FQ_OK;
 }

@@ -60,6 +60,8 @@ struct Mlp {
   uint64_t* sdet = nullptr;  // [W][bcap]
   int32_t* sexc = nullptr;   // [bcap][4]: i, j, a, b (i<j, a<b)
   double* sw = nullptr;      // sample weights
+  double* sh = nullptr;      // |H_ij| of each sample (the spawning-variance objective's factor)
+  int objective = 0;         // 0: weighted cross-entropy to q* ~ |H|; 1: spawning variance sum |H|^2 / q
   int32_t* sflat = nullptr;  // 1: sample scored with flat weights (its determinant gets no network row)
   double* sloss = nullptr;   // per-sample weighted NLL
   double* dsum = nullptr;    // [2]: sum of weights, loss

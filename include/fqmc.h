@@ -265,6 +265,39 @@ int fq_load_batch(fq_engine *e, const int32_t *inst, const uint64_t *states, con
  * per instance; out[k] receives instance k's results. */
 int fq_cycle_batch(fq_engine *e, const double *shift, const double *eproj_old, const double *tau, fq_cycle_out *out);
 
+/* Graph-captured batched cycles (no counterpart in the reference).  With
+ * enable = 1, fq_cycle_batch runs each cycle as one hipGraph launch: the
+ * cycle's launches are captured once per (list buffer, length step, attempt
+ * step) with grids sized by those steps (four per octave) and list lengths
+ * read on the device, so the host neither waits nor launches inside a cycle;
+ * the cycle's inputs enter through one pinned-memory copy at the graph's
+ * start and its outputs leave through copies at its end.  The first graph
+ * cycles try each candidate schedule (radix digit widths, spawning grid) in
+ * turn and keep the fastest by median cycle time, separately for each octave
+ * of the attempt bound (their relative cost changes with the workload's
+ * size); every schedule gives the
+ * same walker lists, bit for bit, so the chain does not depend on the choice.
+ * A cycle whose attempt bound exceeds the spawning staging capacity, and the
+ * first cycle after fq_load_batch, run as ordinary launches. */
+typedef struct fq_schedule_out {
+  int32_t graph;          /* graph cycles enabled */
+  int32_t tuned;          /* the tuner has chosen for the octave of the last graph cycle's attempt bound */
+  int32_t digit_bits;     /* widest radix digit of the chosen schedule: 9 (8- and 9-bit digits) or 8 */
+  int32_t spawn_blocks;   /* spawning grid cap of the chosen schedule (0: one block per 1024-attempt tile) */
+  int32_t ncand;          /* candidate schedules */
+  int32_t chosen;         /* index of the chosen one (-1 while tuning) */
+  int32_t octave;         /* floor(log2) of the last graph cycle's attempt bound (-1: none yet) */
+  int32_t tuned_octaves;  /* octaves with a chosen schedule (the tuner works per octave) */
+  int64_t graph_cycles;   /* cycles run as graph launches */
+  int64_t graphs;         /* graphs captured */
+  int64_t eager_cycles;   /* cycles run as ordinary launches */
+  double cand_ms[8];      /* median device time of one cycle under each candidate (ms; 0 if untried) */
+  int32_t cand_digit_bits[8];
+  int32_t cand_spawn_blocks[8];
+} fq_schedule_out;
+int fq_batch_graph(fq_engine *e, int32_t enable);
+int fq_batch_schedule(fq_engine *e, fq_schedule_out *out);
+
 /* ---- Learned double-excitation proposal (excit_gen FQ_EXCIT_LEARNED) ---------
  * No counterpart in the reference: an addition beside its heat-bath generator
  * (src/excit_gen_heat_bath_mol.F90), which produces the training samples.
@@ -297,6 +330,20 @@ int fq_mlp_loss_grad(fq_engine *e, const uint64_t *dets, const int32_t *exc, con
  * proposal_min_attempts is scored flat, as production draws for it.  losses
  * (may be NULL) receives each step's loss. */
 int fq_mlp_train(fq_engine *e, int64_t batch, int32_t steps, double lr, uint64_t seed, double *losses);
+/* The spawning-variance objective sum_s w_s |H_s| / q(ex_s | D_s) / sum_s w_s
+ * (habs: |H_s| per sample; with w_s = |H_s| / p_s for samples drawn with
+ * probability p_s an estimate of sum |H|^2 / q) and its gradient, as
+ * fq_mlp_loss_grad. */
+int fq_mlp_variance_grad(fq_engine *e, const uint64_t *dets, const int32_t *exc, const double *w,
+                         const double *habs, const int32_t *flat, int64_t n, double *loss, float *grad);
+/* Online training: with every > 0, a production fq_cycle whose cycle index is
+ * a multiple of every first runs steps Adam steps (learning rate lr) on batch
+ * heat-bath samples of the current walker list, as fq_mlp_train, and the
+ * cycle's proposal is the updated one (the sampler stays exact for any
+ * parameters).  objective: 0 the weighted cross-entropy to q* ~ |H|
+ * (fq_mlp_loss_grad), 1 the spawning variance (fq_mlp_variance_grad); it is
+ * also the objective of fq_mlp_train from then on.  every = 0 stops it. */
+int fq_mlp_online(fq_engine *e, int32_t every, int32_t steps, int64_t batch, double lr, int32_t objective);
 
 /* ---- Excitation probe (molecular systems) --------------------------------------
  * n independent calls of the engine's excitation generator from one

@@ -315,10 +315,14 @@ def _nll_and_grad(z3, th, occ_list, orb_sym, nb, ex):
     return -np.log(q), grad, gth_i, gth_j
 
 
-def loss_and_grad(params, occ, eps, orb_sym, samples, weights, flat=None):
+def loss_and_grad(params, occ, eps, orb_sym, samples, weights, flat=None, habs=None):
     """Weighted mean NLL of the samples' doubles under q_mlp, and its gradient
     with respect to the flat parameter vector (float64 forward and backward);
-    flat: per-sample flags (scored with zero logits, no network gradient)."""
+    flat: per-sample flags (scored with zero logits, no network gradient).
+    With habs (|H_ij| per sample) the loss is instead the spawning variance
+    sum_s w_s |H_s| / q_mlp(s) / sum_s w_s: with w_s = |H_s| / p(s) for
+    samples drawn with probability p, an estimate of sum |H|^2 / q; its
+    gradient is the NLL gradient weighted by w_s |H_s| / q_mlp(s)."""
     nb = occ.shape[1]
     W1, b1, W2, b2, W3, b3 = unpack(params, nb)
     th = theta(params, nb)
@@ -339,11 +343,16 @@ def loss_and_grad(params, occ, eps, orb_sym, samples, weights, flat=None):
         ol = [k + 1 for k in range(nb) if occ[s, k] > 0.5]
         is_flat = flat is not None and flat[s]
         nll, g, gi, gj = _nll_and_grad(np.zeros_like(z3[s]) if is_flat else z3[s], th, ol, orb_sym, nb, samples[s])
-        loss += wt[s] * nll
-        dz3[s] = 0.0 if is_flat else wt[s] * g
+        if habs is None:
+            c = wt[s]
+            loss += c * nll
+        else:  # d(w h / q) = (w h / q) d(-log q)
+            c = wt[s] * habs[s] * np.exp(nll)
+            loss += c
+        dz3[s] = 0.0 if is_flat else c * g
         i, j = samples[s][0], samples[s][1]
-        dth[i - 1] += wt[s] * gi
-        dth[j - 1] += wt[s] * gj
+        dth[i - 1] += c * gi
+        dth[j - 1] += c * gj
     dW3 = a2.T @ dz3
     db3 = dz3.sum(0)
     dz2 = (dz3 @ W3.T) * gelu_grad(z2)
@@ -355,7 +364,7 @@ def loss_and_grad(params, occ, eps, orb_sym, samples, weights, flat=None):
     return loss, pack(dW1, db1, dW2, db2, dW3, db3, dth)
 
 
-def loss_only(params, occ, eps, orb_sym, samples, weights, flat=None):
+def loss_only(params, occ, eps, orb_sym, samples, weights, flat=None, habs=None):
     nb = occ.shape[1]
     z3 = forward(params, occ, eps)
     th = theta(params, nb)
@@ -367,5 +376,6 @@ def loss_only(params, occ, eps, orb_sym, samples, weights, flat=None):
             continue
         ol = [k + 1 for k in range(nb) if occ[s, k] > 0.5]
         zs = np.zeros_like(z3[s]) if (flat is not None and flat[s]) else z3[s]
-        total += wt[s] * _nll_and_grad(zs, th, ol, orb_sym, nb, samples[s])[0]
+        nll = _nll_and_grad(zs, th, ol, orb_sym, nb, samples[s])[0]
+        total += wt[s] * (nll if habs is None else habs[s] * np.exp(nll))
     return total

@@ -107,3 +107,67 @@ def test_batch_follows_solo_chains(initiator):
         lib.fq_destroy(beng)
         for eng in solo:
             lib.fq_destroy(eng)
+
+
+def _cycle_fields(o):
+    return (o.nstates, o.nspawn_events, o.ndeath, o.nattempts, o.nparticles[0], o.proj_energy[0],
+            o.D0_population[0], o.spawn_overflow, o.main_overflow)
+
+
+def test_graph_cycles_match_ordinary_cycles():
+    """fq_batch_graph: the same batch run as graph launches (through the
+    tuner's candidate schedules, then its choice, across several launch-bound
+    steps as the populations grow) and as ordinary launches gives the same
+    outputs and walker lists every cycle.  The ordinary engine's cycles and
+    both list downloads run between graph replays, so each replay follows
+    eager work and host synchronisations."""
+    print(f"[graph] {len(DECKS)} instances, graph vs ordinary cycles", flush=True)
+    lib = N.lib()
+    systems = [api.hubbard_k(**d) for d in DECKS]
+    K = len(systems)
+    arr = (N.FqSystem * K)(*[s.view for s in systems])
+    engs = []
+    try:
+        for graph in (0, 1):
+            eng = C.c_void_p()
+            N.check(lib.fq_create_batch(arr, K, C.byref(_params(99, cap=1 << 20)), C.byref(eng)), "fq_create_batch")
+            engs.append(eng)
+            N.check(lib.fq_batch_graph(eng, graph), "fq_batch_graph")
+            inst = np.arange(K, dtype=np.int32)
+            f0 = np.array([int(s.f0()[0]) for s in systems], dtype=np.uint64)
+            pops = np.array([1200, 800, 2000, 500, 1000], dtype=np.int64)
+            N.check(lib.fq_load_batch(eng, inst.ctypes.data_as(P(C.c_int32)), f0.ctypes.data_as(P(C.c_uint64)),
+                                      pops.ctypes.data_as(P(C.c_int64)), np.zeros(K).ctypes.data_as(P(C.c_double)),
+                                      K), "load")
+        taus = np.array([0.01, 0.02, 0.015, 0.01, 0.02])
+        shifts = np.array([0.3, 0.2, 0.3, 0.4, 0.3])  # growing populations: the launch bounds change
+        eolds = np.array([-0.5, -1.0, -0.3, -2.0, -0.7])
+        outs = [(N.FqCycleOut * K)(), (N.FqCycleOut * K)()]
+        for cyc in range(80):
+            if cyc == 50:
+                shifts[:] = -0.5  # and shrink
+            for g in (0, 1):
+                N.check(lib.fq_cycle_batch(engs[g], shifts.ctypes.data_as(P(C.c_double)),
+                                           eolds.ctypes.data_as(P(C.c_double)), taus.ctypes.data_as(P(C.c_double)),
+                                           outs[g]), "fq_cycle_batch")
+            for k in range(K):
+                assert _cycle_fields(outs[0][k]) == _cycle_fields(outs[1][k]), (cyc, k)
+            a, b = download(lib, engs[0], 1, 1), download(lib, engs[1], 1, 1)
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y), cyc
+            if cyc % 20 == 0:
+                print(f"  cycle {cyc}: {len(a[0])} determinants", flush=True)
+        info = N.FqScheduleOut()
+        N.check(lib.fq_batch_schedule(engs[1], C.byref(info)), "fq_batch_schedule")
+        print(f"  graph cycles {info.graph_cycles}, ordinary {info.eager_cycles}, graphs {info.graphs}, "
+              f"octaves tuned {info.tuned_octaves}, last octave {info.octave} chose {info.chosen} "
+              f"({info.digit_bits}-bit digits, spawn blocks {info.spawn_blocks}), "
+              f"candidates ms {[round(info.cand_ms[q], 3) for q in range(info.ncand)]}", flush=True)
+        assert info.graph == 1 and info.tuned_octaves >= 1 and -1 <= info.chosen < info.ncand
+        # the first cycle after the load runs ordinary launches (its attempt
+        # bound is unknown), as do cycles whose attempts pass the staging capacity
+        assert info.eager_cycles >= 1 and info.graph_cycles + info.eager_cycles == 80
+        assert info.graph_cycles >= 60 and info.graphs >= info.ncand
+    finally:
+        for eng in engs:
+            lib.fq_destroy(eng)

@@ -177,6 +177,89 @@ def test_mlp_loss_and_gradient_match_oracle(tmp_path):
         lib.fq_destroy(eng)
 
 
+def test_mlp_variance_objective_matches_oracle(tmp_path):
+    """fq_mlp_variance_grad (the spawning-variance objective sum w |H| / q)
+    against the oracle's float64 loss and gradient, which central differences
+    pin; the network's fp32 matrix-core training products at 2e-4 of each
+    block's largest gradient."""
+    print("[mlp] variance objective loss/grad", flush=True)
+    system = _system(tmp_path, 8, 4, 808)
+    nb, eps, orb_sym = _orb_info(system)
+    lib, eng = _engine(system)
+    try:
+        params = M.init_params(nb, 29)
+        _set_params(lib, eng, params)
+        rng = np.random.default_rng(13)
+        dets, occs, exc, w = _random_samples(nb, 4, orb_sym, 300, rng)
+        habs = rng.uniform(0.001, 0.3, len(w))
+        flat = (np.arange(len(w)) % 5 == 1).astype(np.int32)
+        loss = C.c_double()
+        grad = np.zeros(len(params), dtype=np.float32)
+        N.check(lib.fq_mlp_variance_grad(eng, np.ascontiguousarray(dets).ctypes.data_as(P(C.c_uint64)),
+                                         exc.ctypes.data_as(P(C.c_int32)), w.ctypes.data_as(P(C.c_double)),
+                                         habs.ctypes.data_as(P(C.c_double)), flat.ctypes.data_as(P(C.c_int32)),
+                                         len(w), C.byref(loss), grad.ctypes.data_as(P(C.c_float))),
+                "fq_mlp_variance_grad")
+        ref_loss, ref_grad = M.loss_and_grad(params.astype(np.float64), M.occupations(dets, nb), eps, orb_sym,
+                                             [tuple(int(x) for x in e) for e in exc], w, flat.astype(bool), habs)
+        print(f"  loss {loss.value:.8f} oracle {ref_loss:.8f}", flush=True)
+        assert abs(loss.value - ref_loss) < 1e-5 * abs(ref_loss)
+        _d_in, _d_out, off, _n = M.sizes(nb)
+        for name, (o, cnt) in off.items():
+            g, r = grad[o:o + cnt].astype(np.float64), ref_grad[o:o + cnt]
+            scale = np.abs(r).max()
+            print(f"  {name}: max |g - ref| / max|ref| = {np.abs(g - r).max() / scale:.2e}", flush=True)
+            assert np.abs(g - r).max() <= 2e-4 * scale + 1e-9
+    finally:
+        lib.fq_destroy(eng)
+
+
+def test_online_training_runs_inside_cycles_and_lowers_variance(tmp_path):
+    """fq_mlp_online: production cycles train the proposal from their own
+    walker lists (objective: spawning variance); the parameters change only on
+    the training cycles, two engines with the same seeds follow the same
+    chain, and the variance objective on fresh heat-bath samples falls."""
+    print("[mlp] online training", flush=True)
+    system = _system(tmp_path, 10, 6, 1010)
+    nb, eps, orb_sym = _orb_info(system)
+    engs = []
+    try:
+        outs = []
+        for rep in range(2):
+            lib, eng = _engine(system)
+            engs.append(eng)
+            N.check(lib.fq_mlp_online(eng, 5, 2, 512, 3e-3, 1), "fq_mlp_online")
+            f0 = system.f0()
+            pop = np.array([200 << 31], dtype=np.int64)  # real amplitudes, 31 fraction bits
+            N.check(lib.fq_load(eng, f0.ctypes.data_as(P(C.c_uint64)), pop.ctypes.data_as(P(C.c_int64)),
+                                np.zeros(1).ctypes.data_as(P(C.c_double)), 1), "fq_load")
+            n = C.c_int64()
+            N.check(lib.fq_mlp_nparams(eng, C.byref(n)), "nparams")
+            p_prev = np.zeros(n.value, dtype=np.float32)
+            N.check(lib.fq_mlp_get_params(eng, p_prev.ctypes.data_as(P(C.c_float)), n.value), "get")
+            rows = []
+            for cyc in range(20):
+                o = N.FqCycleOut()
+                N.check(lib.fq_cycle(eng, (C.c_double * 2)(0.0, 0.0), (C.c_double * 2)(0.0, 0.0), 0.01, C.byref(o)),
+                        "fq_cycle")
+                p = np.zeros(n.value, dtype=np.float32)
+                N.check(lib.fq_mlp_get_params(eng, p.ctypes.data_as(P(C.c_float)), n.value), "get")
+                changed = not np.array_equal(p, p_prev)
+                assert changed == (cyc % 5 == 0), cyc
+                p_prev = p
+                rows.append((o.nstates, o.nattempts, o.nspawn_events, o.nparticles[0], o.proj_energy[0]))
+            outs.append(rows)
+        assert outs[0] == outs[1]
+        # the objective on fresh samples: trained further it falls
+        losses = np.zeros(60)
+        N.check(N.lib().fq_mlp_train(engs[0], 512, 60, 3e-3, 77, losses.ctypes.data_as(P(C.c_double))), "train")
+        print(f"  variance objective: first {losses[:10].mean():.4g}, last {losses[-10:].mean():.4g}", flush=True)
+        assert losses[-10:].mean() < losses[:10].mean()
+    finally:
+        for eng in engs:
+            N.lib().fq_destroy(eng)
+
+
 def _probe(lib, eng, det, n, seed):
     exc = np.zeros((n, 6), dtype=np.int32)
     pg = np.zeros(n)

@@ -50,14 +50,16 @@ def _random_case(nb, nel, nsamp, seed, nsym=1):
     return occ, eps, orb_sym, samples, rng.uniform(0.1, 2.0, nsamp)
 
 
-@pytest.mark.parametrize("nsym,with_flat", [(1, False), (2, False), (2, True)])
-def test_loss_gradient_matches_central_differences(nsym, with_flat):
+@pytest.mark.parametrize("nsym,with_flat,variance", [(1, False, False), (2, False, False), (2, True, False),
+                                                     (1, False, True), (2, True, True)])
+def test_loss_gradient_matches_central_differences(nsym, with_flat, variance):
     nb, nel = 12, 4
     occ, eps, orb_sym, samples, w = _random_case(nb, nel, 24, 3 + nsym, nsym)
     flat = (np.arange(len(w)) % 3 == 0) if with_flat else None
+    habs = np.random.default_rng(3).uniform(0.01, 0.5, len(w)) if variance else None
     params = M.init_params(nb, 11).astype(np.float64)
-    loss, grad = M.loss_and_grad(params, occ, eps, orb_sym, samples, w, flat)
-    assert abs(loss - M.loss_only(params, occ, eps, orb_sym, samples, w, flat)) < 1e-12
+    loss, grad = M.loss_and_grad(params, occ, eps, orb_sym, samples, w, flat, habs)
+    assert abs(loss - M.loss_only(params, occ, eps, orb_sym, samples, w, flat, habs)) < 1e-12 * max(1.0, abs(loss))
     rng = np.random.default_rng(1)
     _d_in, _d_out, off, n = M.sizes(nb)
     # probe entries of every block
@@ -68,9 +70,9 @@ def test_loss_gradient_matches_central_differences(nsym, with_flat):
         p1[k] += h
         p2 = params.copy()
         p2[k] -= h
-        fd = (M.loss_only(p1, occ, eps, orb_sym, samples, w, flat)
-              - M.loss_only(p2, occ, eps, orb_sym, samples, w, flat)) / (2 * h)
-        assert abs(fd - grad[k]) < 1e-7 + 1e-5 * abs(fd), (k, fd, grad[k])
+        fd = (M.loss_only(p1, occ, eps, orb_sym, samples, w, flat, habs)
+              - M.loss_only(p2, occ, eps, orb_sym, samples, w, flat, habs)) / (2 * h)
+        assert abs(fd - grad[k]) < 1e-7 * max(1.0, abs(loss)) + 1e-5 * abs(fd), (k, fd, grad[k])
 
 
 @pytest.mark.parametrize("nsym,mix,flat", [(1, 0.1, False), (2, 0.1, False), (2, 0.5, False), (2, 0.1, True)])
