@@ -515,6 +515,8 @@ def efficiency_learned(args):
                               min_attempts=kk)
         t_train = 0.0
         if code == N.FQ_EXCIT_LEARNED:
+            if args.objective == "variance":  # also the objective of the pre-training below
+                N.check(lib.fq_mlp_online(eng, 0, 0, 1, 1e-3, 1), "fq_mlp_online")
             st, po, da = state[:3]
             N.check(lib.fq_load(eng, st.ctypes.data_as(C.POINTER(C.c_uint64)),
                                 po.ctypes.data_as(C.POINTER(C.c_int64)), da.ctypes.data_as(C.POINTER(C.c_double)),
@@ -525,7 +527,11 @@ def efficiency_learned(args):
                                      losses.ctypes.data_as(C.POINTER(C.c_double))), "fq_mlp_train")
             t_train = time.perf_counter() - t1
             training = {"seconds": t_train, "loss_first10": float(losses[:10].mean()),
-                        "loss_last10": float(losses[-10:].mean())}
+                        "loss_last10": float(losses[-10:].mean()), "objective": args.objective}
+            if args.online_every > 0:
+                N.check(lib.fq_mlp_online(eng, args.online_every, args.online_steps, 16384, 1e-3,
+                                          1 if args.objective == "variance" else 0), "fq_mlp_online")
+                training["online"] = {"every": args.online_every, "steps": args.online_steps, "batch": 16384}
             log(f"[eff] learned: trained {args.train_steps} steps in {t_train:.1f}s: loss "
                 f"{losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}")
         prof = {}
@@ -690,6 +696,7 @@ def efficiency_batched(args):
     # batched form, from the prepared lists at cycle 0
     beng = C.c_void_p()
     N.check(lib.fq_create_batch(arr, K, C.byref(integer_params(77, cap)), C.byref(beng)), "fq_create_batch")
+    N.check(lib.fq_batch_graph(beng, 1), "fq_batch_graph")  # each batched cycle one graph launch
     binst = (st >> np.uint64(48)).astype(np.int32)
     bst = np.ascontiguousarray(st & np.uint64((1 << 48) - 1))
     N.check(lib.fq_load_batch(beng, binst.ctypes.data_as(C.POINTER(C.c_int32)), bst.ctypes.data_as(C.POINTER(C.c_uint64)),
@@ -706,6 +713,8 @@ def efficiency_batched(args):
         if (i + 1) % 500 == 0:
             log(f"[batch] batched cycle {i + 1}: {time.perf_counter() - t1:.1f}s")
     t_batch = time.perf_counter() - t1
+    info = N.FqScheduleOut()
+    N.check(lib.fq_batch_schedule(beng, C.byref(info)), "fq_batch_schedule")
     lib.fq_destroy(beng)
     # the same instances, one engine each
     t_solo, att_solo, same = 0.0, 0, True
@@ -749,7 +758,10 @@ def efficiency_batched(args):
                                      "definition": "sum over converged instances of 1/(sigma_k^2 t)"},
                       "config": {"workload": f"{K} Hubbard-k rings of 10-18 sites, U/t 1-8, targets 1e4-1e6, "
                                              f"integer populations, tau {tau}, {cycles} cycles, reports of {every}"},
-                      "batched": {"wall_s": t_batch, "attempts_per_s": attempts / t_batch},
+                      "batched": {"wall_s": t_batch, "attempts_per_s": attempts / t_batch,
+                                  "graph_cycles": info.graph_cycles, "ordinary_cycles": info.eager_cycles,
+                                  "schedule": {"digit_bits": info.digit_bits, "spawn_blocks": info.spawn_blocks,
+                                               "tuned_octaves": info.tuned_octaves}},
                       "one_engine_per_instance": {"wall_s": t_solo, "attempts_per_s": att_solo / t_solo},
                       "instances": per}), flush=True)
 
@@ -841,6 +853,8 @@ def main():
     ap.add_argument("--start-frac", type=float, default=1e-3, help="initial HF population / target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched graph-cycle leg")
+    ap.add_argument("--main-cap", type=int, default=0, help="main-list capacity (default 4 x target)")
+    ap.add_argument("--spawn-cap", type=int, default=0, help="spawned-list capacity (default 4 x target)")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched leg and print its JSON")
     ap.add_argument("--batched-graph", type=int, default=1, choices=[0, 1],
                     help="batched leg: graph-captured cycles (1) or ordinary launches (0)")
@@ -851,6 +865,11 @@ def main():
                     help="timed cycles of an efficiency mode (default 20000 learned: 2000 reports of 10 for the "
                          "reblocking; 5000 batched: 1000 reports of 5)")
     ap.add_argument("--train-steps", type=int, default=1000)
+    ap.add_argument("--objective", choices=["xent", "variance"], default="variance",
+                    help="learned proposal training objective: cross-entropy to |H| or the spawning variance")
+    ap.add_argument("--online-every", type=int, default=50,
+                    help="learned proposal: train online every this many production cycles (0: pre-training only)")
+    ap.add_argument("--online-steps", type=int, default=2, help="learned proposal: Adam steps per online training")
     ap.add_argument("--proposal-mix", type=float, default=0.1,
                     help="learned proposal: fraction of doubles from the uniform fallback")
     ap.add_argument("--eff-seeds", default="23", help="learned efficiency: production seed")
@@ -894,7 +913,9 @@ def main():
     cap = int(max(4 * args.target, 1 << 20))
     S = args.replicas
     W = system.view.nwords
-    eng, p = make_engine(lib, system, args.target, 7, cap, cap, spaces=S)
+    mcap = args.main_cap or cap
+    scap = args.spawn_cap or cap
+    eng, p = make_engine(lib, system, args.target, 7, mcap, scap, spaces=S)
     rf = 1 << 31
     f0 = system.f0()
     region = ProfilerRegion()
