@@ -148,6 +148,9 @@ class Propagator:
     def cycle(self, tau):
         out = N.FqCycleOut()
         N.check(self.lib.fq_cycle(self.eng, self.shift, self.eold, tau, C.byref(out)), "fq_cycle")
+        if out.spawn_overflow or out.main_overflow:  # a truncated cycle: no number from it is valid
+            raise RuntimeError(f"list capacity exceeded (spawned {out.nspawn_events}, states {out.nstates}); "
+                               f"raise --spawn-cap / --main-cap")
         self.count += 1
         for s in range(self.S):
             self.proj[s] += out.proj_energy[s]
