@@ -135,6 +135,18 @@ module fqmc_engine
             integer(c_int32_t) :: rc
         end function fq_semi_stoch_init
 
+        ! online training of the learned proposal (excit_gen FQ_EXCIT_LEARNED):
+        ! no reference counterpart; objective 0 cross-entropy, 1 spawning variance
+        function fq_mlp_online(engine, every, steps, batch, lr, objective) result(rc) bind(c, name='fq_mlp_online')
+            import :: c_ptr, c_int64_t, c_int32_t, c_double
+            type(c_ptr), value :: engine
+            integer(c_int32_t), value :: every, steps
+            integer(c_int64_t), value :: batch
+            real(c_double), value :: lr
+            integer(c_int32_t), value :: objective
+            integer(c_int32_t) :: rc
+        end function fq_mlp_online
+
         subroutine fq_destroy(engine) bind(c, name='fq_destroy')
             import :: c_ptr
             type(c_ptr), value :: engine
